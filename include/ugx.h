@@ -1,0 +1,290 @@
+/*
+ * ugx.h - C ABI of the MI355X-native uniGasFoam particle time step.
+ *
+ * The library owns device memory; callers own every host buffer they pass.
+ * All entry points return 0 (UGX_OK) on success and a ugx_status code
+ * otherwise; ugx_last_error() returns a thread-local message.  No C++ types,
+ * exceptions or ownership transfers cross this boundary.  One case (or batch)
+ * is not thread-safe; distinct cases are.  Results are deterministic given
+ * (seed, inputs): every random draw comes from a counter-based Philox-4x32-10
+ * stream keyed by (case seed, entity, step).
+ *
+ * Reference interfaces replaced (paths relative to src/lagrangian/uniGas/ of
+ * the uniGasFoam sources):
+ *   ugx_case_create      uniGasCloud::uniGasCloud           clouds/uniGasCloud.C:396-781
+ *                        (model selection: dsmcCollisionModel::New
+ *                         dsmcCollisions/basic/dsmcCollisionModel.C:59-83,
+ *                         bgkCollisionModel::New bgkCollisions/basic/bgkCollisionModel.C:55-80,
+ *                         dsmcCollisionPartner::New dsmcCollisionPartner/basic/dsmcCollisionPartner.C:57-83,
+ *                         uniGasBoundaries ctor boundaries/basic/uniGasBoundaries/uniGasBoundaries.C:493-556)
+ *   ugx_case_fill        uniGasMeshFill::setInitialConfiguration
+ *                        uniGasInitialisation/derived/uniGasMeshFill/uniGasMeshFill.C:58-297
+ *   ugx_step             uniGasCloud::evolve                clouds/uniGasCloud.C:821-869
+ *   ugx_counters         uniGasCloud::info / noTimeCounter info lines
+ *                        clouds/uniGasCloud.C:872-920, noTimeCounter.C:318-342
+ *   ugx_samples_download uniGasVolFields raw time sums
+ *                        macroscopicProperties/derived/volumetric/uniGasVolFields/uniGasVolFields.C:723-837
+ *   ugx_fields_compute   uniGasVolFields write-time fields  uniGasVolFields.C:839-1430
+ *   ugx_particles_*      uniGasParcel state / lagrangian IO parcels/uniGasParcelIO.C:89-181
+ */
+#ifndef UGX_H
+#define UGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UGX_MAX_SPECIES    8   /* species per case */
+#define UGX_MAX_CELL_FACES 6   /* convex cells with up to six planar faces */
+
+/* Boltzmann constant used by every kernel (CODATA 2014, as OpenFOAM's
+ * physicoChemical::k). */
+#define UGX_KB 1.38064852e-23
+
+typedef enum {
+    UGX_OK = 0,
+    UGX_ERR_ARG = 1,       /* invalid argument / descriptor */
+    UGX_ERR_DEVICE = 2,    /* HIP runtime failure or no device */
+    UGX_ERR_CAPACITY = 3,  /* particle or cell capacity exceeded */
+    UGX_ERR_STATE = 4,     /* call not valid in the current state */
+    UGX_ERR_MESH = 5,      /* mesh cannot be used (non-convex, >6 faces, ...) */
+    UGX_ERR_IO = 6,        /* dictionary / case directory error */
+    UGX_ERR_UNSUPPORTED = 7
+} ugx_status;
+
+/* OpenFOAM polyPatch types (constant/polyMesh/boundary "type"). */
+typedef enum {
+    UGX_PATCH_PATCH = 0,     /* generic patch: open boundary */
+    UGX_PATCH_WALL = 1,
+    UGX_PATCH_SYMMETRY = 2,  /* symmetry and symmetryPlane: mirror */
+    UGX_PATCH_EMPTY = 3,     /* reduced-D direction, never crossed */
+    UGX_PATCH_CYCLIC = 4     /* translational periodic pair */
+} ugx_patch_type;
+
+/* Mesh in OpenFOAM polyMesh form (points, faces, owner, neighbour, boundary).
+ * Faces point from owner to neighbour; boundary faces point out of the
+ * domain.  Patches are contiguous face ranges after the internal faces. */
+typedef struct {
+    int32_t n_points;
+    int32_t n_faces;
+    int32_t n_internal_faces;
+    int32_t n_cells;
+    const double*  points;          /* [n_points*3] */
+    const int32_t* face_offsets;    /* [n_faces+1] into face_points */
+    const int32_t* face_points;     /* vertex ids, right-handed about the face normal */
+    const int32_t* owner;           /* [n_faces] */
+    const int32_t* neighbour;       /* [n_internal_faces] */
+    int32_t n_patches;
+    int32_t pad0;
+    const int32_t* patch_start;     /* [n_patches] first face id */
+    const int32_t* patch_size;      /* [n_patches] */
+    const int32_t* patch_type;      /* [n_patches] ugx_patch_type */
+    const int32_t* patch_neighbour; /* [n_patches] cyclic partner patch or -1; may be NULL */
+} ugx_mesh_desc;
+
+/* constant/uniGasProperties moleculeProperties entry
+ * (parcels/uniGasParcel.H:68-187, parcels/uniGasParcelI.H:40-141). */
+typedef struct {
+    char    name[32];
+    double  mass;
+    double  diameter;
+    double  omega;
+    double  alpha;
+    int32_t rotational_dof;
+    int32_t vibrational_modes;  /* accepted, not exchanged (out of scope) */
+    int32_t charge;
+    int32_t pad0;
+} ugx_species_desc;
+
+typedef enum { UGX_COLL_NONE = 0, UGX_COLL_DSMC = 1, UGX_COLL_BGK = 2, UGX_COLL_HYBRID = 3 } ugx_collision_model;
+typedef enum {
+    UGX_DSMC_NONE = 0,   /* noDSMCCollision */
+    UGX_DSMC_VHS = 1,    /* variableHardSphere */
+    UGX_DSMC_VSS = 2,    /* variableSoftSphere */
+    UGX_DSMC_LB_VHS = 3, /* LarsenBorgnakkeVariableHardSphere */
+    UGX_DSMC_LB_VSS = 4  /* LarsenBorgnakkeVariableSoftSphere */
+} ugx_dsmc_model;
+typedef enum {
+    UGX_BGK_NONE = 0,    /* noBGKCollision */
+    UGX_BGK_BGK = 1,     /* stochasticParticleBGK */
+    UGX_BGK_ESBGK = 2,   /* stochasticParticleESBGK */
+    UGX_BGK_USP = 3      /* unifiedStochasticParticleSBGK */
+} ugx_bgk_model;
+typedef enum { UGX_PARTNER_NTC = 0 } ugx_partner_model;
+
+/* constant/uniGasProperties + system/controlDict + hybridDecompositionDict. */
+typedef struct {
+    double  n_equivalent_particles;   /* F_N (nEquivalentParticles) */
+    double  delta_t;                  /* controlDict deltaT */
+    int32_t collision_model;          /* ugx_collision_model */
+    int32_t dsmc_model;               /* ugx_dsmc_model */
+    int32_t partner_model;            /* ugx_partner_model */
+    int32_t bgk_model;                /* ugx_bgk_model */
+    double  Tref;                     /* collisionProperties.Tref */
+    double  Zrot;                     /* rotationalRelaxationCollisionNumber (LB) */
+    double  theta;                    /* USP time-relaxation theta (default 1) */
+    int32_t macro_interpolation;      /* must be 0: cell values */
+    int32_t sample_interval;          /* fieldPropertiesDict sampleInterval (>=1) */
+    /* hybrid (localKnudsen) */
+    int32_t decomposition_interval;   /* default 100 */
+    int32_t smoothing_passes;         /* default 5 */
+    double  breakdown_max;            /* Kn_GLL threshold (default 0.05) */
+    double  kn_theta;                 /* EMA weight of Kn (default 0.2) */
+} ugx_model_desc;
+
+typedef enum {
+    UGX_BND_NONE = 0,
+    UGX_BND_SPECULAR = 1,   /* uniGasSpecularWallPatch */
+    UGX_BND_DIFFUSE = 2,    /* uniGasDiffuseWallPatch */
+    UGX_BND_DELETION = 3,   /* uniGasDeletionPatch */
+    UGX_BND_FREESTREAM = 4  /* uniGasFreeStreamInflowPatch (general boundary) */
+} ugx_boundary_kind;
+
+/* system/boundariesDict entry. A patch may carry one patch model (specular,
+ * diffuse, deletion) and, independently, one freestream inflow. */
+typedef struct {
+    int32_t patch;
+    int32_t kind;                      /* ugx_boundary_kind */
+    double  temperature;               /* diffuse wall temperature */
+    double  velocity[3];               /* diffuse wall velocity / inflow velocity */
+    double  rot_temperature;           /* inflow rotational temperature */
+    double  number_density[UGX_MAX_SPECIES]; /* inflow, per case species (0 = none) */
+} ugx_boundary_desc;
+
+/* system/uniGasInitialisationDict uniGasMeshFill entry.  T[3] are per-axis
+ * translational temperatures (equal values = the reference's isotropic
+ * fill); exact_per_cell > 0 places exactly that many simulators per cell,
+ * split over species by number density (a test initialiser). */
+typedef struct {
+    double  number_density[UGX_MAX_SPECIES];
+    double  T[3];
+    double  rot_temperature;
+    double  velocity[3];
+    int32_t exact_per_cell;
+    int32_t pad0;
+} ugx_fill_desc;
+
+typedef enum { UGX_FP64 = 0, UGX_FP32_STATE = 1 } ugx_precision;
+
+typedef struct {
+    int32_t  precision;           /* ugx_precision */
+    int32_t  device;              /* HIP device ordinal */
+    int64_t  particle_capacity;   /* 0 = automatic */
+    uint64_t seed;
+} ugx_options;
+
+/* Host-side particle arrays (always FP64 on the host). local_position, when
+ * non-NULL, holds positions relative to the cell reference point (the
+ * library's storage form); position holds global coordinates. */
+typedef struct {
+    int64_t  n;                /* in: capacity of the arrays, out: count */
+    double*  position;         /* [n*3] or NULL */
+    double*  local_position;   /* [n*3] or NULL */
+    double*  velocity;         /* [n*3] */
+    double*  erot;             /* [n] */
+    int32_t* cell;             /* [n] */
+    int32_t* species;          /* [n] case species index */
+} ugx_particle_soa;
+
+/* Raw sampler sums per cell, sum over steps of deltaT * (per-step sum over
+ * the cell's simulators); layout [UGX_S_COUNT][n_cells].  Per-species blocks
+ * follow: UGX_S_SPECIES_N + s and UGX_S_SPECIES_MCC + s. */
+typedef enum {
+    UGX_S_N = 0, UGX_S_M, UGX_S_MU, UGX_S_MV, UGX_S_MW, UGX_S_MCC,
+    UGX_S_EROT, UGX_S_ROTDOF,
+    UGX_S_MUU, UGX_S_MUV, UGX_S_MUW, UGX_S_MVV, UGX_S_MVW, UGX_S_MWW,
+    UGX_S_MCCU, UGX_S_MCCV, UGX_S_MCCW, UGX_S_EROTU, UGX_S_EROTV, UGX_S_EROTW,
+    UGX_S_BASE,
+    UGX_S_SPECIES_N = UGX_S_BASE,
+    UGX_S_SPECIES_MCC = UGX_S_BASE + UGX_MAX_SPECIES,
+    UGX_S_COUNT = UGX_S_BASE + 2 * UGX_MAX_SPECIES
+} ugx_sample_index;
+
+/* Write-time volFields (uniGasVolFields.C:839-1430); every array has n_cells
+ * entries (U_mean 3*n_cells).  Any pointer may be NULL. */
+typedef struct {
+    double* rhoN;
+    double* rhoM;
+    double* U_mean;
+    double* translational_T;
+    double* rotational_T;
+    double* overall_T;
+    double* p;
+    double* Ma;
+    double* mfp;
+    double* mcr;
+    double* density_error;
+    double* velocity_error;
+    double* temperature_error;
+    double* pressure_error;
+    double* shear_stress;      /* [6*n_cells] xx xy xz yy yz zz, Pa */
+    double* heat_flux;         /* [3*n_cells] W/m^2 */
+} ugx_fields;
+
+typedef struct {
+    int64_t n_particles;
+    int64_t steps;
+    int64_t candidates;        /* NTC collision candidates, cumulative */
+    int64_t collisions;        /* accepted DSMC collisions, cumulative */
+    int64_t bgk_collisions;    /* BGK-family relaxed simulators, cumulative */
+    int64_t inserted;
+    int64_t deleted;
+    int64_t wall_hits;
+    int64_t track_errors;      /* tracks stopped at the crossing limit */
+    int64_t n_sampled_steps;
+    double  time_sampled;      /* sum of deltaT over sampled steps */
+    int64_t n_dsmc_cells;      /* hybrid: cells using DSMC now */
+} ugx_counters;
+
+typedef struct ugx_case ugx_case;
+
+const char* ugx_version(void);
+const char* ugx_last_error(void);
+
+int ugx_case_create(const ugx_mesh_desc* mesh,
+                    const ugx_species_desc* species, int n_species,
+                    const ugx_model_desc* model,
+                    const ugx_boundary_desc* boundaries, int n_boundaries,
+                    const ugx_options* options,
+                    ugx_case** out);
+int ugx_case_destroy(ugx_case* c);
+
+int ugx_case_fill(ugx_case* c, const ugx_fill_desc* fill);
+int ugx_particles_upload(ugx_case* c, const ugx_particle_soa* p);
+int ugx_particles_download(ugx_case* c, ugx_particle_soa* p);
+int ugx_particle_count(ugx_case* c, int64_t* n);
+
+/* Enqueue n_steps calls of the evolve() equivalent on the case's stream. */
+int ugx_step(ugx_case* c, int n_steps);
+int ugx_sync(ugx_case* c);
+
+int ugx_samples_download(ugx_case* c, double* sums, int64_t len);
+int ugx_samples_reset(ugx_case* c);
+int ugx_fields_compute(ugx_case* c, ugx_fields* f);
+int ugx_counters_get(ugx_case* c, ugx_counters* out);
+int ugx_sigma_max_download(ugx_case* c, double* out, int64_t len);
+int ugx_sigma_max_upload(ugx_case* c, const double* in, int64_t len);
+int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
+int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
+int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
+
+/* Host-only geometry in kernel storage form (no device needed): per cell
+ * reference point [3n], volume [n], face count [n], outward planes
+ * [n*6*4] (nx ny nz d in the cell frame), neighbour codes [n*6] (cell id,
+ * -1-patch for boundary models, INT32_MIN for empty), frame shifts [n*6*3],
+ * and the inflow face-species expected insert counts per step.  No
+ * counterpart in the reference: it exposes what the tracking kernels use so
+ * that callers can check mesh conversion. */
+int ugx_geometry(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
+                 const ugx_model_desc* model, const ugx_boundary_desc* boundaries, int n_boundaries,
+                 double* cc, double* vol, int32_t* nfaces, double* planes, int32_t* nbr, double* delta,
+                 double* nbar, int64_t nbar_len, int64_t* n_inflow);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UGX_H */

@@ -1,0 +1,1317 @@
+/*
+ * orc.c - CPU oracle of the uniGasFoam particle time step.
+ * TEST INFRASTRUCTURE ONLY (checker for the HIP path; CPU baseline).
+ *
+ * Restates, in FP64 C with OpenMP, the functions of the reference on the
+ * path uniGasCloud::evolve (src/lagrangian/uniGas/clouds/uniGasCloud.C:821-869).
+ * Paths below are relative to src/lagrangian/uniGas/ unless stated.
+ *
+ * Conventions shared with the HIP path (they define bit-exactness):
+ *  - positions are stored relative to a per-cell reference point (the mean
+ *    of the cell's vertices); a face crossing adds the frame shift
+ *    c_from - c_to (+ the cyclic translation);
+ *  - cells are convex with <= 6 planar faces, stored as outward planes
+ *    n.x = d in the cell frame (faces in increasing face id);
+ *  - every random draw comes from a Philox stream keyed by (seed, kind,
+ *    entity, step) - see orc_prim.h;
+ *  - per-cell sums run sequentially over the cell's simulators in sorted
+ *    order (stable by cell, then previous slot);
+ *  - XnParticle sums are F_N times the raw sums (cell and radial weights are
+ *    1: weighting is out of scope);
+ *  - vibrational and electronic modes are not exchanged (out of scope; the
+ *    reference's single-level electronic branch is a no-op besides RNG use).
+ */
+#include "orc.h"
+#include "orc_prim.h"
+
+#include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define KB UGX_KB
+#define PI_D 3.14159265358979323846
+#define VSMALL 1.0e-300
+#define SMALL 1.0e-15
+#define MAXF UGX_MAX_CELL_FACES
+#define NBR_EMPTY (-2147483647 - 1)
+#define CELL_DEAD (-1)
+#define MAX_CROSSINGS 1000
+#define NS UGX_MAX_SPECIES
+#define NSAMP UGX_S_COUNT
+
+static __thread char g_err[512];
+const char* orc_last_error(void) { return g_err; }
+static int fail(int code, const char* msg) { snprintf(g_err, sizeof g_err, "%s", msg); return code; }
+
+typedef struct {
+    int kind;          /* ugx_boundary_kind for the patch (specular/diffuse/deletion) */
+    double T, U[3];
+} bmodel;
+
+typedef struct {
+    int cell, species, desc;
+    double nbar;       /* expected simulators per step (Bird 4.22) */
+    int nv;
+    double vloc[8][3]; /* face vertices in the cell frame */
+    double cum[6];     /* cumulative triangle area fractions (fan from vertex 0) */
+    double n[3], t1[3], t2[3];
+    double T, rotT, U[3];
+} inflow_fs;
+
+struct orc_world {
+    /* mesh */
+    int nc;
+    double* cc;       /* [nc*3] */
+    double* vol;      /* [nc] */
+    double* bblo;     /* [nc*3] bounding box in the cell frame */
+    double* bbhi;
+    int32_t* ncf;     /* [nc] */
+    double* plane;    /* [nc*6*4] */
+    int32_t* nbr;     /* [nc*6] */
+    double* delta;    /* [nc*6*3] */
+    double mask[3];   /* solution directions (0 for empty directions) */
+    int npatch;
+    bmodel* bm;       /* [npatch] */
+    int nin;
+    inflow_fs* inflow;
+    /* species */
+    int ns;
+    double mass[NS], diam[NS], omega[NS], alpha[NS], muref[NS], prterm[NS];
+    int rotdof[NS];
+    double pmr[NS * NS], pC[NS * NS], pe[NS * NS], pchi[NS * NS], palpha[NS * NS];
+    /* model */
+    ugx_model_desc model;
+    uint64_t seed;
+    uint32_t step;
+    /* particles */
+    int64_t n, cap;
+    double *x, *v, *erot;
+    int32_t *cell, *sp;
+    double *x2, *v2, *erot2;
+    int32_t *cell2, *sp2;
+    int64_t* off;     /* [nc+1] */
+    /* per cell */
+    double* smax;
+    double* samp;     /* [NSAMP*nc] */
+    int32_t* cmodel;  /* 1 DSMC, 0 BGK */
+    double* uspq;     /* [nc*3] previous heat flux */
+    double* usps;     /* [nc*6] previous stress */
+    double* uspmax;   /* [nc] */
+    double* knacc;    /* hybrid: time sums since the last decomposition [nc*8] */
+    double* kn;       /* [nc] */
+    double kn_time;
+    int kn_steps;
+    /* counters */
+    ugx_counters cnt;
+};
+
+/* ------------------------------------------------------------- geometry */
+
+static void cross3(const double a[3], const double b[3], double r[3])
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double dot3(const double a[3], const double b[3])
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* Face centre (vertex mean) and area vector (fan about the centre). */
+static void face_geom(const ugx_mesh_desc* m, int f, double pc[3], double A[3])
+{
+    int b = m->face_offsets[f], e = m->face_offsets[f + 1], k = e - b;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int i = b; i < e; ++i)
+        for (int d = 0; d < 3; ++d) s[d] = s[d] + m->points[3 * m->face_points[i] + d];
+    for (int d = 0; d < 3; ++d) pc[d] = s[d] / (double)k;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < k; ++i) {
+        const double* p0 = &m->points[3 * m->face_points[b + i]];
+        const double* p1 = &m->points[3 * m->face_points[b + (i + 1) % k]];
+        double u[3], w[3], c[3];
+        for (int d = 0; d < 3; ++d) { u[d] = p0[d] - pc[d]; w[d] = p1[d] - pc[d]; }
+        cross3(u, w, c);
+        for (int d = 0; d < 3; ++d) a[d] = a[d] + c[d];
+    }
+    for (int d = 0; d < 3; ++d) A[d] = 0.5 * a[d];
+}
+
+static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_desc* bnd, int nb)
+{
+    int nc = m->n_cells, nf = m->n_faces;
+    w->nc = nc;
+    w->cc = calloc((size_t)nc * 3, sizeof(double));
+    w->vol = calloc((size_t)nc, sizeof(double));
+    w->bblo = calloc((size_t)nc * 3, sizeof(double));
+    w->bbhi = calloc((size_t)nc * 3, sizeof(double));
+    w->ncf = calloc((size_t)nc, sizeof(int32_t));
+    w->plane = calloc((size_t)nc * MAXF * 4, sizeof(double));
+    w->nbr = calloc((size_t)nc * MAXF, sizeof(int32_t));
+    w->delta = calloc((size_t)nc * MAXF * 3, sizeof(double));
+    int32_t* cf = malloc(sizeof(int32_t) * (size_t)nc * MAXF);
+    double* pc = malloc(sizeof(double) * (size_t)nf * 3);
+    double* A = malloc(sizeof(double) * (size_t)nf * 3);
+    int* fpatch = malloc(sizeof(int) * (size_t)nf);
+    for (int f = 0; f < nf; ++f) { face_geom(m, f, &pc[3 * f], &A[3 * f]); fpatch[f] = -1; }
+    for (int p = 0; p < m->n_patches; ++p)
+        for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) fpatch[f] = p;
+    /* faces per cell in increasing face id */
+    for (int f = 0; f < nf; ++f) {
+        int o = m->owner[f];
+        if (w->ncf[o] >= MAXF) return fail(UGX_ERR_MESH, "cell with more than 6 faces");
+        cf[o * MAXF + w->ncf[o]++] = f;
+        if (f < m->n_internal_faces) {
+            int nbc = m->neighbour[f];
+            if (w->ncf[nbc] >= MAXF) return fail(UGX_ERR_MESH, "cell with more than 6 faces");
+            cf[nbc * MAXF + w->ncf[nbc]++] = f;
+        }
+    }
+    /* reference point: mean of unique vertices (first appearance order) */
+    for (int c = 0; c < nc; ++c) {
+        int ids[64], nid = 0;
+        for (int j = 0; j < w->ncf[c]; ++j) {
+            int f = cf[c * MAXF + j];
+            for (int i = m->face_offsets[f]; i < m->face_offsets[f + 1]; ++i) {
+                int pid = m->face_points[i], seen = 0;
+                for (int q = 0; q < nid; ++q) if (ids[q] == pid) { seen = 1; break; }
+                if (!seen && nid < 64) ids[nid++] = pid;
+            }
+        }
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < nid; ++q)
+            for (int d = 0; d < 3; ++d) s[d] = s[d] + m->points[3 * ids[q] + d];
+        for (int d = 0; d < 3; ++d) w->cc[3 * c + d] = s[d] / (double)nid;
+        for (int d = 0; d < 3; ++d) { w->bblo[3 * c + d] = DBL_MAX; w->bbhi[3 * c + d] = -DBL_MAX; }
+        for (int q = 0; q < nid; ++q)
+            for (int d = 0; d < 3; ++d) {
+                double r = m->points[3 * ids[q] + d] - w->cc[3 * c + d];
+                if (r < w->bblo[3 * c + d]) w->bblo[3 * c + d] = r;
+                if (r > w->bbhi[3 * c + d]) w->bbhi[3 * c + d] = r;
+            }
+    }
+    /* boundary models per patch */
+    w->npatch = m->n_patches;
+    w->bm = calloc((size_t)(m->n_patches > 0 ? m->n_patches : 1), sizeof(bmodel));
+    for (int p = 0; p < m->n_patches; ++p) {
+        int t = m->patch_type[p];
+        w->bm[p].kind = (t == UGX_PATCH_SYMMETRY) ? UGX_BND_SPECULAR : UGX_BND_DELETION;
+        int found = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (bnd[b].patch != p) continue;
+            int k = bnd[b].kind;
+            if (k == UGX_BND_SPECULAR || k == UGX_BND_DIFFUSE || k == UGX_BND_DELETION) {
+                w->bm[p].kind = k;
+                w->bm[p].T = bnd[b].temperature;
+                for (int d = 0; d < 3; ++d) w->bm[p].U[d] = bnd[b].velocity[d];
+                found = 1;
+            }
+        }
+        if (t == UGX_PATCH_WALL && !found) return fail(UGX_ERR_ARG, "wall patch without a boundary model");
+    }
+    /* solution directions from empty patches */
+    for (int d = 0; d < 3; ++d) w->mask[d] = 1.0;
+    for (int p = 0; p < m->n_patches; ++p) {
+        if (m->patch_type[p] != UGX_PATCH_EMPTY || m->patch_size[p] == 0) continue;
+        const double* a = &A[3 * m->patch_start[p]];
+        int ax = 0;
+        for (int d = 1; d < 3; ++d) if (fabs(a[d]) > fabs(a[ax])) ax = d;
+        w->mask[ax] = 0.0;
+    }
+    /* planes, neighbours, shifts, volumes */
+    for (int c = 0; c < nc; ++c) {
+        double vs = 0.0;
+        for (int j = 0; j < MAXF; ++j) {
+            double* pl = &w->plane[(c * MAXF + j) * 4];
+            if (j >= w->ncf[c]) { pl[0] = pl[1] = pl[2] = 0.0; pl[3] = 1.0; w->nbr[c * MAXF + j] = NBR_EMPTY; continue; }
+            int f = cf[c * MAXF + j];
+            double sg = (m->owner[f] == c) ? 1.0 : -1.0;
+            double a[3] = {A[3 * f], A[3 * f + 1], A[3 * f + 2]};
+            double r[3];
+            for (int d = 0; d < 3; ++d) r[d] = pc[3 * f + d] - w->cc[3 * c + d];
+            double dt = dot3(a, r);
+            vs = vs + sg * dt;
+            double mag = sqrt(dot3(a, a));
+            double nn[3];
+            for (int d = 0; d < 3; ++d) nn[d] = sg * (a[d] / mag);
+            pl[0] = nn[0]; pl[1] = nn[1]; pl[2] = nn[2];
+            pl[3] = dot3(nn, r);
+            double* dl = &w->delta[(c * MAXF + j) * 3];
+            if (f < m->n_internal_faces) {
+                int o = (m->owner[f] == c) ? m->neighbour[f] : m->owner[f];
+                w->nbr[c * MAXF + j] = o;
+                for (int d = 0; d < 3; ++d) dl[d] = w->cc[3 * c + d] - w->cc[3 * o + d];
+            } else {
+                int p = fpatch[f];
+                if (p < 0) return fail(UGX_ERR_MESH, "boundary face outside every patch");
+                int t = m->patch_type[p];
+                if (t == UGX_PATCH_EMPTY) {
+                    w->nbr[c * MAXF + j] = NBR_EMPTY;
+                } else if (t == UGX_PATCH_CYCLIC) {
+                    int q = m->patch_neighbour ? m->patch_neighbour[p] : -1;
+                    if (q < 0 || m->patch_size[q] != m->patch_size[p]) return fail(UGX_ERR_MESH, "cyclic patch without a matching partner");
+                    int g = m->patch_start[q] + (f - m->patch_start[p]);
+                    int o = m->owner[g];
+                    w->nbr[c * MAXF + j] = o;
+                    for (int d = 0; d < 3; ++d)
+                        dl[d] = (w->cc[3 * c + d] - w->cc[3 * o + d]) + (pc[3 * g + d] - pc[3 * f + d]);
+                } else {
+                    w->nbr[c * MAXF + j] = -1 - p;
+                }
+            }
+        }
+        w->vol[c] = vs / 3.0;
+        if (!(w->vol[c] > 0.0)) return fail(UGX_ERR_MESH, "cell with non-positive volume");
+    }
+    /* freestream inflow face-species entries (uniGasGeneralBoundary.C:115-169) */
+    int nin = 0;
+    for (int b = 0; b < nb; ++b)
+        if (bnd[b].kind == UGX_BND_FREESTREAM)
+            for (int s = 0; s < w->ns; ++s)
+                if (bnd[b].number_density[s] > 0.0) nin += m->patch_size[bnd[b].patch];
+    w->nin = nin;
+    w->inflow = calloc((size_t)(nin > 0 ? nin : 1), sizeof(inflow_fs));
+    int k = 0;
+    const double sqrtPi = sqrt(PI_D);
+    for (int b = 0; b < nb; ++b) {
+        if (bnd[b].kind != UGX_BND_FREESTREAM) continue;
+        int p = bnd[b].patch;
+        for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) {
+            for (int s = 0; s < w->ns; ++s) {
+                double nd = bnd[b].number_density[s];
+                if (!(nd > 0.0)) continue;
+                inflow_fs* e = &w->inflow[k++];
+                int c = m->owner[f];
+                e->cell = c; e->species = s; e->desc = b;
+                e->T = bnd[b].temperature; e->rotT = bnd[b].rot_temperature;
+                for (int d = 0; d < 3; ++d) e->U[d] = bnd[b].velocity[d];
+                const double* sF = &A[3 * f];
+                double fA = sqrt(dot3(sF, sF));
+                double mps = sqrt(2.0 * KB * e->T / w->mass[s]);
+                double nrm[3] = {-sF[0] / fA, -sF[1] / fA, -sF[2] / fA};
+                double sCos = dot3(e->U, nrm) / mps;
+                e->nbar = (fA * nd * w->model.delta_t * mps
+                           * (exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + erf(sCos))))
+                          / (2.0 * sqrtPi * w->model.n_equivalent_particles);
+                int fb = m->face_offsets[f], fe = m->face_offsets[f + 1];
+                e->nv = fe - fb;
+                if (e->nv > 8) return fail(UGX_ERR_MESH, "inflow face with more than 8 vertices");
+                for (int i = 0; i < e->nv; ++i)
+                    for (int d = 0; d < 3; ++d)
+                        e->vloc[i][d] = m->points[3 * m->face_points[fb + i] + d] - w->cc[3 * c + d];
+                double tot = 0.0, ar[6];
+                for (int t = 0; t < e->nv - 2; ++t) {
+                    double u[3], ww[3], cr[3];
+                    for (int d = 0; d < 3; ++d) { u[d] = e->vloc[t + 1][d] - e->vloc[0][d]; ww[d] = e->vloc[t + 2][d] - e->vloc[0][d]; }
+                    cross3(u, ww, cr);
+                    ar[t] = 0.5 * sqrt(dot3(cr, cr));
+                    tot = tot + ar[t];
+                }
+                double run = 0.0;
+                for (int t = 0; t < e->nv - 2; ++t) { run = run + ar[t]; e->cum[t] = run / tot; }
+                e->cum[e->nv - 3] = 1.0;
+                for (int d = 0; d < 3; ++d) e->n[d] = nrm[d];
+                double t1[3];
+                for (int d = 0; d < 3; ++d) t1[d] = pc[3 * f + d] - m->points[3 * m->face_points[fb] + d];
+                double tm = sqrt(dot3(t1, t1));
+                for (int d = 0; d < 3; ++d) e->t1[d] = t1[d] / tm;
+                double t2[3];
+                cross3(e->n, e->t1, t2);
+                double t2m = sqrt(dot3(t2, t2));
+                for (int d = 0; d < 3; ++d) e->t2[d] = t2[d] / t2m;
+            }
+        }
+    }
+    free(cf); free(pc); free(A); free(fpatch);
+    return UGX_OK;
+}
+
+/* ------------------------------------------------------- kinetic helpers */
+
+/* uniGasCloud::equipartitionRotationalEnergy, uniGasCloud.C:975-1017 */
+static double equip_erot(orc_rng* r, double T, int dof)
+{
+    if (dof <= 0) return 0.0;
+    if (dof == 2) return -orc_log(orc_uniform(r)) * KB * T;
+    double a = 0.5 * (double)dof - 1.0;
+    double eps = orc_uniform(r);
+    double x = 0.0, P = -1.0;
+    for (int it = 0; it < 100000; ++it) {
+        x = 10.0 * orc_uniform(r);
+        P = orc_pow(x / a, a) * orc_exp(a - x);
+        if (!(P < eps)) break;
+    }
+    return x * KB * T;
+}
+
+/* uniGasCloud::postCollisionRotationalEnergy, uniGasCloud.C:1129-1189 */
+static double post_rot_ratio(orc_rng* r, int dof, double chiB)
+{
+    if (dof == 2) return 1.0 - orc_pow(orc_uniform(r), 1.0 / chiB);
+    double chiA = 0.5 * (double)dof;
+    double am1 = chiA - 1.0, bm1 = chiB - 1.0;
+    if (am1 < SMALL && bm1 < SMALL) return orc_uniform(r);
+    double eps = orc_uniform(r);
+    double ratio = 0.0;
+    for (int it = 0; it < 100000; ++it) {
+        ratio = orc_uniform(r);
+        double P;
+        if (am1 < SMALL) P = orc_pow(1.0 - ratio, bm1);
+        else if (bm1 < SMALL) P = orc_pow(1.0 - ratio, am1);
+        else P = orc_pow((am1 + bm1) * ratio / am1, am1) * orc_pow((am1 + bm1) * (1.0 - ratio) / bm1, bm1);
+        if (!(P < eps)) break;
+    }
+    return ratio;
+}
+
+/* uniGasPatchBoundary::diffuseReflection, uniGasPatchBoundary.C:305-388 */
+static void diffuse_reflect(orc_rng* r, const double nw[3], double U[3], double* erot,
+                            double T, const double Uw[3], double mass, int rotdof)
+{
+    double udn = dot3(U, nw);
+    double Ut[3];
+    for (int d = 0; d < 3; ++d) Ut[d] = U[d] - udn * nw[d];
+    double mUt = sqrt(dot3(Ut, Ut));
+    for (int it = 0; it < 1000 && mUt < SMALL; ++it) {
+        for (int d = 0; d < 3; ++d) U[d] = U[d] * (0.8 + 0.2 * orc_uniform(r));
+        udn = dot3(U, nw);
+        for (int d = 0; d < 3; ++d) Ut[d] = U[d] - udn * nw[d];
+        mUt = sqrt(dot3(Ut, Ut));
+    }
+    double t1[3], t2[3];
+    for (int d = 0; d < 3; ++d) t1[d] = Ut[d] / mUt;
+    cross3(nw, t1, t2);
+    double sig = sqrt(KB * T / mass);
+    double g1, g2;
+    orc_gauss2(r, &g1, &g2);
+    double un = sqrt(-2.0 * orc_log(orc_uniform(r)));
+    for (int d = 0; d < 3; ++d) U[d] = sig * ((g1 * t1[d] + g2 * t2[d]) - un * nw[d]) + Uw[d];
+    *erot = equip_erot(r, T, rotdof);
+}
+
+/* ------------------------------------------------------------------ move */
+
+/* uniGasParcel::move (parcels/uniGasParcel.C:35-108) over a convex-cell face
+ * tracker, with the wall dispatch of uniGasParcel::hitWallPatch (:131-145).
+ * Returns 1 if the simulator survives. */
+static int track(orc_world* w, int32_t* cellp, double x[3], double U[3], double* erot, int sp,
+                 double tau, orc_rng* r, int64_t* hits, int64_t* errs)
+{
+    int c = *cellp;
+    double vt[3];
+    for (int d = 0; d < 3; ++d) vt[d] = U[d] * w->mask[d];
+    for (int it = 0; it < MAX_CROSSINGS; ++it) {
+        const double* pl = &w->plane[(size_t)c * MAXF * 4];
+        const int32_t* nb = &w->nbr[(size_t)c * MAXF];
+        double tmin = tau;
+        int fmin = -1;
+        for (int f = 0; f < MAXF; ++f) {
+            if (nb[f] == NBR_EMPTY) continue;
+            const double* q = &pl[4 * f];
+            double dn = q[0] * vt[0] + q[1] * vt[1] + q[2] * vt[2];
+            if (dn > 0.0) {
+                double t = (q[3] - (q[0] * x[0] + q[1] * x[1] + q[2] * x[2])) / dn;
+                if (t < 0.0) t = 0.0;
+                if (t < tmin) { tmin = t; fmin = f; }
+            }
+        }
+        if (fmin < 0) {
+            for (int d = 0; d < 3; ++d) x[d] = x[d] + vt[d] * tau;
+            *cellp = c;
+            return 1;
+        }
+        for (int d = 0; d < 3; ++d) x[d] = x[d] + vt[d] * tmin;
+        tau = tau - tmin;
+        int code = nb[fmin];
+        if (code >= 0) {
+            const double* dl = &w->delta[((size_t)c * MAXF + fmin) * 3];
+            for (int d = 0; d < 3; ++d) x[d] = x[d] + dl[d];
+            c = code;
+            continue;
+        }
+        const bmodel* bm = &w->bm[-1 - code];
+        const double* nw = &pl[4 * fmin];
+        if (bm->kind == UGX_BND_DELETION) { *cellp = CELL_DEAD; return 0; }
+        *hits += 1;
+        if (bm->kind == UGX_BND_SPECULAR) {
+            /* uniGasPatchBoundary::specularReflection, :390-403 */
+            double udn = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+            if (udn > 0.0)
+                for (int d = 0; d < 3; ++d) U[d] = U[d] - 2.0 * udn * nw[d];
+        } else {
+            diffuse_reflect(r, nw, U, erot, bm->T, bm->U, w->mass[sp], w->rotdof[sp]);
+        }
+        for (int d = 0; d < 3; ++d) vt[d] = U[d] * w->mask[d];
+    }
+    *errs += 1;
+    *cellp = c;
+    return 1;
+}
+
+/* ---------------------------------------------------------------- inflow */
+
+/* uniGasGeneralBoundary::insertParcels, uniGasGeneralBoundary.C:537-761:
+ * position on the face, Bird eq. 12.5 normal speed, tangential Maxwellian,
+ * equipartition E_rot; then the first move over (1 - R) deltaT
+ * (newParcel, uniGasParcel.C:47-53). */
+static int make_inflow(orc_world* w, const inflow_fs* e, orc_rng* r, double x[3], double U[3],
+                       double* erot, int32_t* cell, int64_t* hits, int64_t* errs)
+{
+    double us = orc_uniform(r);
+    int tri = 0;
+    for (int t = 0; t < e->nv - 2; ++t) { tri = t; if (e->cum[t] >= us) break; }
+    double s = sqrt(orc_uniform(r));
+    double tt = orc_uniform(r);
+    const double* a = e->vloc[0];
+    const double* b = e->vloc[tri + 1];
+    const double* c = e->vloc[tri + 2];
+    for (int d = 0; d < 3; ++d) x[d] = a[d] * (1.0 - s) + b[d] * (s * (1.0 - tt)) + c[d] * (s * tt);
+    double m = w->mass[e->species];
+    double mps = sqrt(2.0 * KB * e->T / m);
+    double udn = dot3(e->U, e->n);
+    double sCos = udn / mps;
+    double uNormal;
+    if (fabs(udn) > VSMALL) {
+        double cA = sCos + sqrt(sCos * sCos + 2.0);
+        double cB = 0.5 * (1.0 + sCos * (sCos - sqrt(sCos * sCos + 2.0)));
+        double scale = 3.0;
+        if (sCos < -3.0) scale = fabs(sCos) + 1.0;
+        uNormal = 0.0;
+        for (int it = 0; it < 100000; ++it) {
+            double th = scale * (2.0 * orc_uniform(r) - 1.0);
+            uNormal = th + sCos;
+            double P = (uNormal < 0.0) ? -1.0 : 2.0 * uNormal / cA * orc_exp(cB - th * th);
+            if (!(P < orc_uniform(r))) break;
+        }
+    } else {
+        uNormal = sqrt(-orc_log(orc_uniform(r)));
+    }
+    double g1, g2;
+    orc_gauss2(r, &g1, &g2);
+    double sig = sqrt(KB * e->T / m);
+    double ut1 = dot3(e->t1, e->U), ut2 = dot3(e->t2, e->U);
+    for (int d = 0; d < 3; ++d)
+        U[d] = ((sig * (g1 * e->t1[d] + g2 * e->t2[d]) + ut1 * e->t1[d]) + ut2 * e->t2[d]) + mps * uNormal * e->n[d];
+    *erot = equip_erot(r, e->rotT, w->rotdof[e->species]);
+    *cell = e->cell;
+    double frac = orc_uniform(r);
+    return track(w, cell, x, U, erot, e->species, (1.0 - frac) * w->model.delta_t, r, hits, errs);
+}
+
+/* ------------------------------------------------------------ collisions */
+
+/* sigma_T c_r for VHS/VSS/LB (variableHardSphere.C:72-115):
+ * pi d^2 (2 k Tref/(mR cR^2))^(w-1/2) / Gamma(5/2-w) cR = C_pq (cR^2)^(1-w). */
+static double sigma_cr(const orc_world* w, int p, int q, const double* Up, const double* Uq)
+{
+    double d0 = Up[0] - Uq[0], d1 = Up[1] - Uq[1], d2 = Up[2] - Uq[2];
+    double cR2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (cR2 < VSMALL) return 0.0;
+    int k = p * w->ns + q;
+    return w->pC[k] * orc_exp(w->pe[k] * orc_log(cR2));
+}
+
+/* Isotropic (VHS) or VSS scattering of the relative velocity with speed cR,
+ * then the centre-of-mass split (variableHardSphere.C:118-162,
+ * variableSoftSphere.C:116-176, LB-VSS :389-424). */
+static void scatter(orc_rng* r, int vss, double alpha, double cR, const double crc[3],
+                    double mP, double mQ, double* UP, double* UQ)
+{
+    double Ucm[3];
+    for (int d = 0; d < 3; ++d) Ucm[d] = (mP * UP[d] + mQ * UQ[d]) / (mP + mQ);
+    double cosT = vss ? 2.0 * orc_pow(orc_uniform(r), 1.0 / alpha) - 1.0 : 2.0 * orc_uniform(r) - 1.0;
+    double sinT = sqrt(1.0 - cosT * cosT);
+    double sp, cp;
+    orc_sincos2pi(orc_uniform(r), &sp, &cp);
+    double c[3];
+    double D = sqrt(crc[1] * crc[1] + crc[2] * crc[2]);
+    if (vss && D > 0.0) {
+        c[0] = cosT * crc[0] + sinT * sp * D;
+        c[1] = cosT * crc[1] + sinT * (cR * crc[2] * cp - crc[0] * crc[1] * sp) / D;
+        c[2] = cosT * crc[2] - sinT * (cR * crc[1] * cp + crc[0] * crc[2] * sp) / D;
+    } else {
+        c[0] = cR * cosT;
+        c[1] = cR * (sinT * cp);
+        c[2] = cR * (sinT * sp);
+    }
+    for (int d = 0; d < 3; ++d) {
+        UP[d] = Ucm[d] + c[d] * mQ / (mP + mQ);
+        UQ[d] = Ucm[d] - c[d] * mP / (mP + mQ);
+    }
+}
+
+/* dsmcCollisionModel::collide for VHS, VSS and the serial Larsen-Borgnakke
+ * variants (LarsenBorgnakkeVariableHardSphere.C:125-416). */
+static void binary_collide(orc_world* w, orc_rng* r, int p, int q, double* UP, double* UQ,
+                           double* eP, double* eQ)
+{
+    int model = w->model.dsmc_model;
+    int k = p * w->ns + q;
+    double mP = w->mass[p], mQ = w->mass[q];
+    double crc[3] = {UP[0] - UQ[0], UP[1] - UQ[1], UP[2] - UQ[2]};
+    double cR2 = crc[0] * crc[0] + crc[1] * crc[1] + crc[2] * crc[2];
+    double cR = sqrt(cR2);
+    int vss = (model == UGX_DSMC_VSS || model == UGX_DSMC_LB_VSS);
+    if (model == UGX_DSMC_LB_VHS || model == UGX_DSMC_LB_VSS) {
+        double mR = w->pmr[k];
+        double Et = 0.5 * mR * cR2;
+        double chiB = w->pchi[k];
+        double invZ = 1.0 / w->model.Zrot;
+        double preP = *eP, preQ = *eQ;
+        if (w->rotdof[p] > 0 && invZ > orc_uniform(r)) {
+            double Ec = Et + preP;
+            double ratio = post_rot_ratio(r, w->rotdof[p], chiB);
+            *eP = ratio * Ec;
+            Et = Ec - *eP;
+        }
+        if (w->rotdof[q] > 0 && invZ > orc_uniform(r)) {
+            double Ec = Et + preQ;
+            double ratio = post_rot_ratio(r, w->rotdof[q], chiB);
+            *eQ = ratio * Ec;
+            Et = Ec - *eQ;
+        }
+        double cRn = sqrt(2.0 * Et / mR);
+        if (vss && cR > 0.0)
+            for (int d = 0; d < 3; ++d) crc[d] = crc[d] * (cRn / cR);
+        cR = cRn;
+    }
+    scatter(r, vss, w->palpha[k], cR, crc, mP, mQ, UP, UQ);
+}
+
+/* Per-cell sums of cellMeasurements::calculateFields (cellMeasurements.C:408-513),
+ * one simulator after another in sorted order. */
+static void cell_sums(const orc_world* w, int64_t b, int64_t e, double S[NSAMP])
+{
+    for (int k = 0; k < NSAMP; ++k) S[k] = 0.0;
+    for (int64_t i = b; i < e; ++i) {
+        int s = w->sp[i];
+        double m = w->mass[s];
+        double vx = w->v[3 * i], vy = w->v[3 * i + 1], vz = w->v[3 * i + 2];
+        double er = w->erot[i];
+        double mcc = m * (vx * vx + vy * vy + vz * vz);
+        S[UGX_S_N] += 1.0;
+        S[UGX_S_M] += m;
+        S[UGX_S_MU] += m * vx;
+        S[UGX_S_MV] += m * vy;
+        S[UGX_S_MW] += m * vz;
+        S[UGX_S_MCC] += mcc;
+        S[UGX_S_EROT] += er;
+        S[UGX_S_ROTDOF] += (double)w->rotdof[s];
+        S[UGX_S_MUU] += m * (vx * vx);
+        S[UGX_S_MUV] += m * (vx * vy);
+        S[UGX_S_MUW] += m * (vx * vz);
+        S[UGX_S_MVV] += m * (vy * vy);
+        S[UGX_S_MVW] += m * (vy * vz);
+        S[UGX_S_MWW] += m * (vz * vz);
+        S[UGX_S_MCCU] += mcc * vx;
+        S[UGX_S_MCCV] += mcc * vy;
+        S[UGX_S_MCCW] += mcc * vz;
+        S[UGX_S_EROTU] += er * vx;
+        S[UGX_S_EROTV] += er * vy;
+        S[UGX_S_EROTW] += er * vz;
+        S[UGX_S_SPECIES_N + s] += 1.0;
+        S[UGX_S_SPECIES_MCC + s] += mcc;
+    }
+}
+
+/* noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell
+ * level 1 (the whole cell). */
+static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, int64_t* coll)
+{
+    if (nC < 2) return;
+    orc_rng r;
+    orc_rng_init(&r, w->seed, ORC_RNG_NTC, 0u, (uint32_t)c, w->step);
+    double smax0 = w->smax[c];
+    double smax = smax0;
+    double sel = 0.5 * (double)nC * (double)(nC - 1) * w->model.n_equivalent_particles * smax0
+                 * w->model.delta_t / w->vol[c];
+    double fl = floor(sel);
+    int64_t ncand = (int64_t)fl;
+    if (orc_uniform(&r) < sel - fl) ncand += 1;
+    *cand += ncand;
+    for (int64_t k = 0; k < ncand; ++k) {
+        uint32_t P = orc_index(&r, (uint32_t)nC);
+        uint32_t Q;
+        do { Q = orc_index(&r, (uint32_t)nC); } while (Q == P);
+        int64_t ip = b + P, iq = b + Q;
+        int sp = w->sp[ip], sq = w->sp[iq];
+        double* UP = &w->v[3 * ip];
+        double* UQ = &w->v[3 * iq];
+        double scr = sigma_cr(w, sp, sq, UP, UQ);
+        if (scr > smax) smax = scr;
+        if (scr / smax0 > orc_uniform(&r)) {
+            binary_collide(w, &r, sp, sq, UP, UQ, &w->erot[ip], &w->erot[iq]);
+            *coll += 1;
+        }
+    }
+    w->smax[c] = smax;
+}
+
+/* BGK-family relaxation for one cell: calculateProperties
+ * (stochasticParticleBGK.C:287-583, stochasticParticleESBGK.C:311-665,
+ * unifiedStochasticParticleSBGK.C:379-800), collide (:628-729, :721-834,
+ * :871-994), conserveMomentumAndEnergy (:731-780) and the post-collision
+ * samplers (:782-796, :887-906, :1047-1096).  The reference's five list
+ * shuffles followed by "take the first n" are restated as one partial
+ * Fisher-Yates draw of n distinct simulators (same distribution). */
+static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[NSAMP],
+                     uint32_t* idx, int64_t* nrel_out, int update_state)
+{
+    int model = w->model.bgk_model;
+    double FN = w->model.n_equivalent_particles, dt = w->model.delta_t, V = w->vol[c];
+    double N = S[UGX_S_N];
+    int usp = (model == UGX_BGK_USP);
+    int perform = 1;
+    double T = 0.0, p = 0.0, U[3] = {0.0, 0.0, 0.0}, Pt[6] = {0, 0, 0, 0, 0, 0};
+    double q[3], sh[6];
+    for (int d = 0; d < 3; ++d) q[d] = w->uspq[3 * c + d];
+    for (int d = 0; d < 6; ++d) sh[d] = w->usps[6 * c + d];
+    double Pr = 0.0, nu = 0.0;
+    if (N > VSMALL) {
+        double rhoN = FN * N / V;
+        for (int d = 0; d < 3; ++d) U[d] = S[UGX_S_MU + d] / S[UGX_S_M];
+        double uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+        T = (S[UGX_S_MCC] - S[UGX_S_M] * uu) / (3.0 * KB * N);
+        p = rhoN * KB * T;
+        const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        for (int k = 0; k < 6; ++k)
+            Pt[k] = rhoN * (S[UGX_S_MUU + k] / N - (S[UGX_S_M] / N) * U[ij[k][0]] * U[ij[k][1]]);
+        if (usp) {
+            double tr = (Pt[0] + Pt[3] + Pt[5]) / 3.0;
+            for (int k = 0; k < 6; ++k) sh[k] = Pt[k];
+            sh[0] = sh[0] - tr; sh[3] = sh[3] - tr; sh[5] = sh[5] - tr;
+            const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+            for (int d = 0; d < 3; ++d)
+                q[d] = rhoN * (0.5 * (S[UGX_S_MCCU + d] / N) - 0.5 * (S[UGX_S_MCC] / N) * U[d]
+                               + S[UGX_S_EROTU + d] / N - (S[UGX_S_EROT] / N) * U[d])
+                       - Pt[row[d][0]] * U[0] - Pt[row[d][1]] * U[1] - Pt[row[d][2]] * U[2];
+        }
+        if (N > (usp ? 2.0 : 1.0)) {
+            double f1 = N / (N - 1.0);
+            p = f1 * p;
+            T = f1 * T;
+            for (int k = 0; k < 6; ++k) Pt[k] = f1 * Pt[k];
+            if (usp) {
+                double f2 = N * N / (N - 1.0) / (N - 2.0);
+                for (int d = 0; d < 3; ++d) q[d] = f2 * q[d];
+                for (int k = 0; k < 6; ++k) sh[k] = f1 * sh[k];
+            }
+        } else {
+            perform = 0;
+        }
+        if (T > VSMALL) {
+            double visc = 0.0;
+            for (int s = 0; s < w->ns; ++s) {
+                double ns_ = S[UGX_S_SPECIES_N + s];
+                visc = visc + ns_ * w->muref[s] * orc_pow(T / w->model.Tref, w->omega[s]);
+                Pr = Pr + ns_ * w->prterm[s];
+            }
+            visc = visc / N;
+            Pr = Pr / N;
+            nu = (model == UGX_BGK_ESBGK) ? Pr * p / visc : p / visc;
+        } else {
+            perform = 0;
+        }
+    } else {
+        perform = 0;
+    }
+    if (usp && update_state) {
+        double th = w->model.theta;
+        for (int d = 0; d < 3; ++d)
+            q[d] = th * q[d] / (1.0 + 0.5 * Pr * nu * dt) + (1.0 - th) * w->uspq[3 * c + d];
+        for (int k = 0; k < 6; ++k)
+            sh[k] = th * sh[k] / (1.0 + 0.5 * nu * dt) + (1.0 - th) * w->usps[6 * c + k];
+        for (int d = 0; d < 3; ++d) w->uspq[3 * c + d] = q[d];
+        for (int k = 0; k < 6; ++k) w->usps[6 * c + k] = sh[k];
+    }
+    int raised = 0;
+    if (perform && w->cmodel[c] == 0) {
+        orc_rng r;
+        orc_rng_init(&r, w->seed, ORC_RNG_BGK, 0u, (uint32_t)c, w->step);
+        double pc = N * (1.0 - orc_exp(-nu * dt));
+        double fl = floor(pc);
+        int64_t nrel = (int64_t)fl;
+        if (orc_uniform(&r) < pc - fl) nrel += 1;
+        if (nrel > nC) nrel = nC;
+        *nrel_out += nrel;
+        for (int64_t i = 0; i < nC; ++i) idx[i] = (uint32_t)i;
+        double Smat[9];
+        if (model == UGX_BGK_ESBGK) {
+            double a = 0.5 * (1.0 - Pr) / Pr;
+            const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+            for (int k = 0; k < 9; ++k) {
+                double dlt = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+                Smat[k] = dlt - a * (Pt[full[k]] / p - dlt);
+            }
+        }
+        double tau = 0.5 * nu * dt;
+        double ex = 1.0 + 2.0 / (orc_exp(2.0 * tau) - 1.0);
+        double cQ = 2.0 * (1.0 - Pr * tau * ex);
+        double cS = 1.0 - tau * ex;
+        for (int64_t i = 0; i < nrel; ++i) {
+            uint32_t j = (uint32_t)i + orc_index(&r, (uint32_t)(nC - i));
+            uint32_t t = idx[i]; idx[i] = idx[j]; idx[j] = t;
+            int64_t pi = b + idx[i];
+            double m = w->mass[w->sp[pi]];
+            double u0 = sqrt(2.0 * KB * T / m);
+            double xi[3];
+            if (model == UGX_BGK_USP) {
+                for (int it = 0; it < 100000; ++it) {
+                    double g[3];
+                    orc_gauss3(&r, g);
+                    for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
+                    double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+                    double vTr = vSq / 3.0;
+                    double prob = 1.0 + cQ / (p * u0) * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq / 2.5 - 1.0)
+                                  + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
+                                              + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
+                                              + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
+                    if (prob > w->uspmax[c] && prob < 10.0) { w->uspmax[c] = prob; raised = 1; break; }
+                    if (orc_uniform(&r) < prob / w->uspmax[c]) break;
+                }
+            } else {
+                double g[3];
+                orc_gauss3(&r, g);
+                for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
+                if (model == UGX_BGK_ESBGK) {
+                    double y[3];
+                    for (int d = 0; d < 3; ++d) y[d] = Smat[3 * d] * xi[0] + Smat[3 * d + 1] * xi[1] + Smat[3 * d + 2] * xi[2];
+                    for (int d = 0; d < 3; ++d) xi[d] = y[d];
+                }
+            }
+            for (int d = 0; d < 3; ++d) w->v[3 * pi + d] = U[d] + u0 * xi[d];
+        }
+        /* conserveMomentumAndEnergy */
+        double ke = 0.0, mom[3] = {0.0, 0.0, 0.0};
+        for (int64_t i = b; i < b + nC; ++i) {
+            double m = w->mass[w->sp[i]];
+            double vx = w->v[3 * i], vy = w->v[3 * i + 1], vz = w->v[3 * i + 2];
+            ke += m * (vx * vx + vy * vy + vz * vz);
+            mom[0] += m * vx; mom[1] += m * vy; mom[2] += m * vz;
+        }
+        double pU[3];
+        for (int d = 0; d < 3; ++d) pU[d] = mom[d] / S[UGX_S_M];
+        double pT = (ke - S[UGX_S_M] * (pU[0] * pU[0] + pU[1] * pU[1] + pU[2] * pU[2])) / (3.0 * KB * (N - 1.0));
+        if (pT > VSMALL) {
+            double sc = sqrt(T / pT);
+            for (int64_t i = b; i < b + nC; ++i)
+                for (int d = 0; d < 3; ++d) w->v[3 * i + d] = U[d] + (w->v[3 * i + d] - pU[d]) * sc;
+        }
+    }
+    if (usp && update_state && !raised) w->uspmax[c] = w->uspmax[c] * 0.999;
+}
+
+/* ------------------------------------------------------------ lifecycle */
+
+static void* xcalloc(size_t n, size_t s) { void* p = calloc(n ? n : 1, s); return p; }
+
+int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
+               const ugx_model_desc* model, const ugx_boundary_desc* bnd, int n_bnd,
+               uint64_t seed, int64_t capacity, orc_world** out)
+{
+    if (!mesh || !species || !model || !out || n_species < 1 || n_species > NS)
+        return fail(UGX_ERR_ARG, "bad arguments");
+    orc_world* w = calloc(1, sizeof(orc_world));
+    w->model = *model;
+    if (w->model.sample_interval < 1) w->model.sample_interval = 1;
+    if (w->model.decomposition_interval < 1) w->model.decomposition_interval = 100;
+    w->seed = seed;
+    w->ns = n_species;
+    for (int s = 0; s < n_species; ++s) {
+        w->mass[s] = species[s].mass;
+        w->diam[s] = species[s].diameter;
+        w->omega[s] = species[s].omega;
+        w->alpha[s] = species[s].alpha;
+        w->rotdof[s] = species[s].rotational_dof;
+        double a = w->alpha[s], om = w->omega[s], d = w->diam[s], m = w->mass[s];
+        w->muref[s] = 1.25 * (1.0 + a) * (2.0 + a) * sqrt(m * KB * model->Tref)
+                      / (a * (5.0 - 2.0 * om) * (7.0 - 2.0 * om) * sqrt(PI_D) * (d * d));
+        double z = (double)w->rotdof[s];
+        w->prterm[s] = (5.0 + z) / (7.5 + z);
+    }
+    for (int p = 0; p < n_species; ++p)
+        for (int q = 0; q < n_species; ++q) {
+            int k = p * n_species + q;
+            double mP = w->mass[p], mQ = w->mass[q];
+            double mR = mP * mQ / (mP + mQ);
+            double dPQ = 0.5 * (w->diam[p] + w->diam[q]);
+            double om = 0.5 * (w->omega[p] + w->omega[q]);
+            w->pmr[k] = mR;
+            w->pC[k] = PI_D * dPQ * dPQ * pow(2.0 * KB * model->Tref / mR, om - 0.5) / exp(lgamma(2.5 - om));
+            w->pe[k] = 1.0 - om;
+            w->pchi[k] = 2.5 - om;
+            w->palpha[k] = 0.5 * (w->alpha[p] + w->alpha[q]);
+        }
+    int rc = build_mesh(w, mesh, bnd, n_bnd);
+    if (rc) { orc_destroy(w); return rc; }
+    int nc = w->nc;
+    w->cap = capacity > 0 ? capacity : 1024;
+    w->x = xcalloc((size_t)w->cap * 3, sizeof(double));
+    w->v = xcalloc((size_t)w->cap * 3, sizeof(double));
+    w->erot = xcalloc((size_t)w->cap, sizeof(double));
+    w->cell = xcalloc((size_t)w->cap, sizeof(int32_t));
+    w->sp = xcalloc((size_t)w->cap, sizeof(int32_t));
+    w->x2 = xcalloc((size_t)w->cap * 3, sizeof(double));
+    w->v2 = xcalloc((size_t)w->cap * 3, sizeof(double));
+    w->erot2 = xcalloc((size_t)w->cap, sizeof(double));
+    w->cell2 = xcalloc((size_t)w->cap, sizeof(int32_t));
+    w->sp2 = xcalloc((size_t)w->cap, sizeof(int32_t));
+    w->off = xcalloc((size_t)nc + 1, sizeof(int64_t));
+    w->smax = xcalloc((size_t)nc, sizeof(double));
+    w->samp = xcalloc((size_t)nc * NSAMP, sizeof(double));
+    w->cmodel = xcalloc((size_t)nc, sizeof(int32_t));
+    w->uspq = xcalloc((size_t)nc * 3, sizeof(double));
+    w->usps = xcalloc((size_t)nc * 6, sizeof(double));
+    w->uspmax = xcalloc((size_t)nc, sizeof(double));
+    w->knacc = xcalloc((size_t)nc * 8, sizeof(double));
+    w->kn = xcalloc((size_t)nc, sizeof(double));
+    int cm = (model->collision_model == UGX_COLL_DSMC) ? 1 : 0;  /* hybrid starts all-BGK (uniGasCloud.C:726-732) */
+    for (int c = 0; c < nc; ++c) { w->cmodel[c] = cm; w->uspmax[c] = 1.0; }
+    *out = w;
+    return UGX_OK;
+}
+
+void orc_destroy(orc_world* w)
+{
+    if (!w) return;
+    void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->bm, w->inflow,
+                    w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
+                    w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+    free(w);
+}
+
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int64_t orc_count(orc_world* w) { return w->n; }
+
+/* Stable counting sort by cell over [0, n); dead simulators dropped
+ * (the cell occupancy of CloudWithModels.C:109-138, as contiguous runs). */
+static void sort_by_cell(orc_world* w, int64_t n)
+{
+    int nc = w->nc;
+    int64_t* cnt = w->off;
+    memset(cnt, 0, sizeof(int64_t) * ((size_t)nc + 1));
+    for (int64_t i = 0; i < n; ++i) if (w->cell[i] >= 0) cnt[w->cell[i] + 1] += 1;
+    for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+    int64_t* cur = malloc(sizeof(int64_t) * (size_t)(nc > 0 ? nc : 1));
+    for (int c = 0; c < nc; ++c) cur[c] = cnt[c];
+    for (int64_t i = 0; i < n; ++i) {
+        int c = w->cell[i];
+        if (c < 0) continue;
+        int64_t j = cur[c]++;
+        for (int d = 0; d < 3; ++d) { w->x2[3 * j + d] = w->x[3 * i + d]; w->v2[3 * j + d] = w->v[3 * i + d]; }
+        w->erot2[j] = w->erot[i];
+        w->cell2[j] = c;
+        w->sp2[j] = w->sp[i];
+    }
+    free(cur);
+    double* t;
+    int32_t* ti;
+    t = w->x; w->x = w->x2; w->x2 = t;
+    t = w->v; w->v = w->v2; w->v2 = t;
+    t = w->erot; w->erot = w->erot2; w->erot2 = t;
+    ti = w->cell; w->cell = w->cell2; w->cell2 = ti;
+    ti = w->sp; w->sp = w->sp2; w->sp2 = ti;
+    w->n = cnt[nc];
+}
+
+/* uniGasMeshFill::setInitialConfiguration (uniGasMeshFill.C:58-297): per
+ * cell and species a stochastically rounded count (whole cell rather than
+ * per tet), positions uniform in the cell (rejection in its bounding box),
+ * Maxwellian velocities at per-axis temperatures, equipartition E_rot;
+ * (sigma_T c_r)max = pi d^2 c_mp(T) of the most abundant species. */
+int orc_fill(orc_world* w, const ugx_fill_desc* fl)
+{
+    int nc = w->nc, ns = w->ns;
+    double FN = w->model.n_equivalent_particles;
+    double ntot = 0.0;
+    int best = 0;
+    for (int s = 0; s < ns; ++s) { ntot += fl->number_density[s]; if (fl->number_density[s] > fl->number_density[best]) best = s; }
+    int64_t* cnt = malloc(sizeof(int64_t) * (size_t)nc * ns);
+    int64_t total = w->n;
+    for (int c = 0; c < nc; ++c)
+        for (int s = 0; s < ns; ++s) {
+            int64_t k;
+            if (fl->exact_per_cell > 0) {
+                k = (ntot > 0.0) ? (int64_t)floor((double)fl->exact_per_cell * fl->number_density[s] / ntot + 0.5) : 0;
+            } else {
+                orc_rng r;
+                orc_rng_init(&r, w->seed, ORC_RNG_FILL, (uint32_t)s, (uint32_t)c, 0u);
+                double req = fl->number_density[s] / FN * w->vol[c];
+                double f0 = floor(req);
+                k = (int64_t)f0;
+                if (orc_uniform(&r) < req - f0) k += 1;
+            }
+            cnt[c * ns + s] = k;
+            total += k;
+        }
+    if (total > w->cap) { free(cnt); return fail(UGX_ERR_CAPACITY, "particle capacity exceeded by fill"); }
+    int64_t j = w->n;
+    for (int c = 0; c < nc; ++c)
+        for (int s = 0; s < ns; ++s) {
+            orc_rng r;
+            orc_rng_init(&r, w->seed, ORC_RNG_FILL, (uint32_t)s, (uint32_t)c, 0u);
+            if (fl->exact_per_cell <= 0) (void)orc_uniform(&r);
+            double m = w->mass[s];
+            double sig[3];
+            for (int d = 0; d < 3; ++d) sig[d] = sqrt(KB * fl->T[d] / m);
+            for (int64_t i = 0; i < cnt[c * ns + s]; ++i, ++j) {
+                double x[3] = {0.0, 0.0, 0.0};
+                for (int it = 0; it < 1000; ++it) {
+                    double y[3];
+                    for (int d = 0; d < 3; ++d) {
+                        double u = orc_uniform(&r);
+                        y[d] = w->bblo[3 * c + d] + u * (w->bbhi[3 * c + d] - w->bblo[3 * c + d]);
+                    }
+                    int inside = 1;
+                    for (int f = 0; f < w->ncf[c]; ++f) {
+                        const double* q = &w->plane[((size_t)c * MAXF + f) * 4];
+                        if (q[0] * y[0] + q[1] * y[1] + q[2] * y[2] > q[3]) { inside = 0; break; }
+                    }
+                    if (inside) { x[0] = y[0]; x[1] = y[1]; x[2] = y[2]; break; }
+                }
+                double g[3];
+                orc_gauss3(&r, g);
+                for (int d = 0; d < 3; ++d) {
+                    w->x[3 * j + d] = x[d] * w->mask[d] + 0.0;
+                    w->v[3 * j + d] = sig[d] * g[d] + fl->velocity[d];
+                }
+                w->erot[j] = equip_erot(&r, fl->rot_temperature, w->rotdof[s]);
+                w->cell[j] = c;
+                w->sp[j] = s;
+            }
+        }
+    free(cnt);
+    sort_by_cell(w, j);
+    double Tm = (fl->T[0] + fl->T[1] + fl->T[2]) / 3.0;
+    double s0 = PI_D * w->diam[best] * w->diam[best] * sqrt(2.0 * KB * Tm / w->mass[best]);
+    for (int c = 0; c < nc; ++c) w->smax[c] = s0;
+    return UGX_OK;
+}
+
+int orc_upload(orc_world* w, const ugx_particle_soa* p)
+{
+    if (p->n > w->cap) return fail(UGX_ERR_CAPACITY, "particle capacity exceeded by upload");
+    for (int64_t i = 0; i < p->n; ++i) {
+        int c = p->cell[i];
+        if (c < 0 || c >= w->nc) return fail(UGX_ERR_ARG, "particle cell out of range");
+        for (int d = 0; d < 3; ++d) {
+            w->x[3 * i + d] = p->local_position ? p->local_position[3 * i + d]
+                                                : p->position[3 * i + d] - w->cc[3 * c + d];
+            w->v[3 * i + d] = p->velocity[3 * i + d];
+        }
+        w->erot[i] = p->erot ? p->erot[i] : 0.0;
+        w->cell[i] = c;
+        w->sp[i] = p->species ? p->species[i] : 0;
+    }
+    sort_by_cell(w, p->n);
+    return UGX_OK;
+}
+
+int orc_download(orc_world* w, ugx_particle_soa* p)
+{
+    if (p->n < w->n) { p->n = w->n; return fail(UGX_ERR_CAPACITY, "download arrays too small"); }
+    for (int64_t i = 0; i < w->n; ++i) {
+        int c = w->cell[i];
+        for (int d = 0; d < 3; ++d) {
+            if (p->local_position) p->local_position[3 * i + d] = w->x[3 * i + d];
+            if (p->position) p->position[3 * i + d] = w->x[3 * i + d] + w->cc[3 * c + d];
+            if (p->velocity) p->velocity[3 * i + d] = w->v[3 * i + d];
+        }
+        if (p->erot) p->erot[i] = w->erot[i];
+        if (p->cell) p->cell[i] = c;
+        if (p->species) p->species[i] = w->sp[i];
+    }
+    p->n = w->n;
+    return UGX_OK;
+}
+
+/* ------------------------------------------------------------- hybrid */
+
+/* localKnudsen::decompose (hybridDecomposition/derived/localKnudsen/localKnudsen.C:216-582),
+ * restated on the cell-face graph: time means since the last decomposition ->
+ * smoothing (face-average of neighbours) -> gradient-length Kn -> EMA ->
+ * DSMC where Kn > breakdownMax. */
+static void hybrid_switch(orc_world* w);
+
+/* ------------------------------------------------------------------ step */
+
+int orc_step_phases(orc_world* w, int mask)
+{
+    int64_t n0 = w->n;
+    int64_t nins = 0;
+    if (mask & 1) {
+        /* inflow: uniGasBoundaries::controlBeforeMove (uniGasBoundaries.C:531-538) */
+        int64_t* cnt = malloc(sizeof(int64_t) * (size_t)(w->nin > 0 ? w->nin : 1));
+        for (int k = 0; k < w->nin; ++k) {
+            orc_rng r;
+            orc_rng_init(&r, w->seed, ORC_RNG_INFLOW_N, 0u, (uint32_t)k, w->step);
+            double nb = w->inflow[k].nbar;
+            double fl = floor(nb);
+            int64_t c = (int64_t)fl;
+            if (c < 0) c = 0;
+            if (orc_uniform(&r) < nb - fl) c += 1;
+            cnt[k] = c;
+            nins += c;
+        }
+        if (n0 + nins > w->cap) { free(cnt); return fail(UGX_ERR_CAPACITY, "particle capacity exceeded by inflow"); }
+        int64_t hits = 0, errs = 0, dead = 0;
+        int64_t base = n0;
+        for (int k = 0; k < w->nin; ++k) {
+            const inflow_fs* e = &w->inflow[k];
+            for (int64_t i = 0; i < cnt[k]; ++i) {
+                int64_t j = base + i;
+                orc_rng r;
+                orc_rng_init(&r, w->seed, ORC_RNG_INFLOW_P, (uint32_t)i, (uint32_t)k, w->step);
+                w->sp[j] = e->species;
+                if (!make_inflow(w, e, &r, &w->x[3 * j], &w->v[3 * j], &w->erot[j], &w->cell[j], &hits, &errs)) dead++;
+            }
+            base += cnt[k];
+        }
+        free(cnt);
+        /* move: Cloud<uniGasParcel>::move -> uniGasParcel::move */
+#pragma omp parallel for reduction(+:hits, errs, dead) schedule(static)
+        for (int64_t i = 0; i < n0; ++i) {
+            orc_rng r;
+            orc_rng_init(&r, w->seed, ORC_RNG_MOVE, 0u, (uint32_t)i, w->step);
+            if (!track(w, &w->cell[i], &w->x[3 * i], &w->v[3 * i], &w->erot[i], w->sp[i],
+                       w->model.delta_t, &r, &hits, &errs)) dead++;
+        }
+        w->cnt.inserted += nins;
+        w->cnt.deleted += dead;
+        w->cnt.wall_hits += hits;
+        w->cnt.track_errors += errs;
+        w->n = n0 + nins;
+    }
+    if (mask & 2) sort_by_cell(w, w->n);
+    if (mask & 4) {
+        /* cellMeas_.calculateFields + fields_.calculateFields + collisions()
+         * (uniGasCloud.C:846-850, :236-248) */
+        int sample = ((w->step + 1) % (uint32_t)w->model.sample_interval) == 0;
+        double dt = w->model.delta_t;
+        int cm = w->model.collision_model;
+        int64_t cand = 0, coll = 0, nrel = 0;
+        uint32_t* idx = malloc(sizeof(uint32_t) * (size_t)(w->n > 0 ? w->n : 1));
+        int hybrid = (cm == UGX_COLL_HYBRID);
+#pragma omp parallel for reduction(+:cand, coll, nrel) schedule(dynamic, 64)
+        for (int c = 0; c < w->nc; ++c) {
+            int64_t b = w->off[c], nC = w->off[c + 1] - w->off[c];
+            double S[NSAMP];
+            cell_sums(w, b, b + nC, S);
+            if (sample)
+                for (int k = 0; k < NSAMP; ++k) w->samp[(size_t)k * w->nc + c] += dt * S[k];
+            if (hybrid)
+                for (int k = 0; k < 8; ++k) w->knacc[8 * c + k] += dt * S[k];
+            if ((cm == UGX_COLL_DSMC || hybrid) && w->cmodel[c] == 1 && w->model.dsmc_model != UGX_DSMC_NONE)
+                ntc_cell(w, c, b, nC, &cand, &coll);
+            if ((cm == UGX_COLL_BGK || hybrid) && w->model.bgk_model != UGX_BGK_NONE)
+                bgk_cell(w, c, b, nC, S, idx + b, &nrel, 1);
+        }
+        free(idx);
+        if (sample) { w->cnt.n_sampled_steps += 1; w->cnt.time_sampled += dt; }
+        if (hybrid) { w->kn_time += dt; w->kn_steps += 1; }
+        w->cnt.candidates += cand;
+        w->cnt.collisions += coll;
+        w->cnt.bgk_collisions += nrel;
+    }
+    if ((mask & 8) && w->model.collision_model == UGX_COLL_HYBRID
+        && ((w->step + 1) % (uint32_t)w->model.decomposition_interval) == 0)
+        hybrid_switch(w);
+    if (mask & 16) { w->step += 1; w->cnt.steps += 1; }
+    return UGX_OK;
+}
+
+int orc_step(orc_world* w, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        int rc = orc_step_phases(w, 31);
+        if (rc) return rc;
+    }
+    return UGX_OK;
+}
+
+/* Hybrid decomposition restated: the face-neighbour graph replaces
+ * fvc::average(fvc::interpolate()), i.e. a smoothing pass sets each cell
+ * value to the mean over its faces of the face value (internal face: mean
+ * of the two cells; boundary face: the cell value). */
+static void hybrid_switch(orc_world* w)
+{
+    int nc = w->nc;
+    if (w->kn_time <= 0.0) return;
+    double* rho = malloc(sizeof(double) * nc);
+    double* T = malloc(sizeof(double) * nc);
+    double* Um = malloc(sizeof(double) * nc);
+    double* mfp = malloc(sizeof(double) * nc);
+    double* cmp = malloc(sizeof(double) * nc);
+    double* tmp = malloc(sizeof(double) * nc);
+    double FN = w->model.n_equivalent_particles;
+    for (int c = 0; c < nc; ++c) {
+        const double* a = &w->knacc[8 * c];
+        double N = a[UGX_S_N], M = a[UGX_S_M];
+        rho[c] = M * FN / (w->kn_time * w->vol[c]);
+        if (N > VSMALL && M > 0.0) {
+            double U[3] = {a[UGX_S_MU] / M, a[UGX_S_MV] / M, a[UGX_S_MW] / M};
+            double uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+            T[c] = (a[UGX_S_MCC] - M * uu) / (3.0 * KB * N);
+            Um[c] = sqrt(uu);
+            double mbar = M / N;
+            double nd = N * FN / (w->kn_time * w->vol[c]);
+            /* Bird eq. 4.65 hard-sphere-like VHS mean free path of the mixture mean */
+            double d = w->diam[0], om = w->omega[0];
+            mfp[c] = (T[c] > VSMALL && nd > 0.0)
+                         ? 1.0 / (sqrt(2.0) * PI_D * d * d * nd * orc_pow(w->model.Tref / T[c], om - 0.5))
+                         : 1.0e300;
+            cmp[c] = (T[c] > 0.0) ? sqrt(2.0 * KB * T[c] / mbar) : 0.0;
+        } else {
+            T[c] = 0.0; Um[c] = 0.0; mfp[c] = 1.0e300; cmp[c] = 0.0;
+        }
+    }
+    double* fields[3] = {rho, T, Um};
+    for (int fi = 0; fi < 3; ++fi) {
+        double* f = fields[fi];
+        for (int pass = 0; pass < w->model.smoothing_passes; ++pass) {
+            for (int c = 0; c < nc; ++c) {
+                double s = 0.0;
+                int k = 0;
+                for (int j = 0; j < w->ncf[c]; ++j) {
+                    int o = w->nbr[c * MAXF + j];
+                    s += (o >= 0) ? 0.5 * (f[c] + f[o]) : f[c];
+                    k++;
+                }
+                tmp[c] = s / (double)k;
+            }
+            memcpy(f, tmp, sizeof(double) * nc);
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        double g = 0.0;
+        for (int j = 0; j < w->ncf[c]; ++j) {
+            int o = w->nbr[c * MAXF + j];
+            if (o < 0) continue;
+            const double* dl = &w->delta[(c * MAXF + j) * 3];
+            double dist = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+            double gr = (rho[c] > 0.0) ? fabs(rho[o] - rho[c]) / dist / rho[c] : 0.0;
+            double gt = (T[c] > 0.0) ? fabs(T[o] - T[c]) / dist / T[c] : 0.0;
+            double den = Um[c] > cmp[c] ? Um[c] : cmp[c];
+            double gu = (den > 0.0) ? fabs(Um[o] - Um[c]) / dist / den : 0.0;
+            double mx = gr > gt ? gr : gt;
+            mx = mx > gu ? mx : gu;
+            if (mx > g) g = mx;
+        }
+        double knc = (mfp[c] < 1.0e299) ? mfp[c] * g : 1.0e300;
+        w->kn[c] = w->model.kn_theta * knc + (1.0 - w->model.kn_theta) * w->kn[c];
+        w->cmodel[c] = (w->kn[c] > w->model.breakdown_max) ? 1 : 0;
+    }
+    memset(w->knacc, 0, sizeof(double) * 8 * (size_t)nc);
+    w->kn_time = 0.0;
+    w->kn_steps = 0;
+    free(rho); free(T); free(Um); free(mfp); free(cmp); free(tmp);
+}
+
+/* ------------------------------------------------------------ accessors */
+
+int orc_samples(orc_world* w, double* out, int64_t len)
+{
+    if (len < (int64_t)NSAMP * w->nc) return fail(UGX_ERR_ARG, "sample buffer too small");
+    memcpy(out, w->samp, sizeof(double) * NSAMP * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_counters(orc_world* w, ugx_counters* out)
+{
+    *out = w->cnt;
+    out->n_particles = w->n;
+    int64_t k = 0;
+    for (int c = 0; c < w->nc; ++c) k += w->cmodel[c];
+    out->n_dsmc_cells = k;
+    return UGX_OK;
+}
+
+int orc_sigma_max(orc_world* w, double* out, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(out, w->smax, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_set_sigma_max(orc_world* w, const double* in, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(w->smax, in, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_cell_model(orc_world* w, int32_t* out, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(out, w->cmodel, sizeof(int32_t) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_geometry(orc_world* w, double* cc, double* vol, int32_t* nfaces, double* planes,
+                 int32_t* nbr, double* delta)
+{
+    size_t nc = (size_t)w->nc;
+    if (cc) memcpy(cc, w->cc, sizeof(double) * 3 * nc);
+    if (vol) memcpy(vol, w->vol, sizeof(double) * nc);
+    if (nfaces) memcpy(nfaces, w->ncf, sizeof(int32_t) * nc);
+    if (planes) memcpy(planes, w->plane, sizeof(double) * 4 * MAXF * nc);
+    if (nbr) memcpy(nbr, w->nbr, sizeof(int32_t) * MAXF * nc);
+    if (delta) memcpy(delta, w->delta, sizeof(double) * 3 * MAXF * nc);
+    return UGX_OK;
+}
+
+int orc_inflow_nbar(orc_world* w, double* out, int64_t len, int64_t* n_out)
+{
+    *n_out = w->nin;
+    for (int k = 0; k < w->nin && k < len; ++k) out[k] = w->inflow[k].nbar;
+    return UGX_OK;
+}
+
+/* ------------------------------------------------------- primitive hooks */
+
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { orc_philox(ctr, key, out); }
+
+/* fn: 0 log, 1 exp, 2 sin(2 pi u), 3 cos(2 pi u), 4 pow(x, y=in2) */
+void orc_mathfn(int fn, const double* in, const double* in2, double* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double s, c;
+        switch (fn) {
+        case 0: out[i] = orc_log(in[i]); break;
+        case 1: out[i] = orc_exp(in[i]); break;
+        case 2: orc_sincos2pi(in[i], &s, &c); out[i] = s; break;
+        case 3: orc_sincos2pi(in[i], &s, &c); out[i] = c; break;
+        default: out[i] = orc_pow(in[i], in2[i]); break;
+        }
+    }
+}
+
+/* n draws of a stream: kind 0 uniform, 1 index in [0, m), 2 gauss pairs */
+void orc_stream(uint64_t seed, uint32_t kind, uint32_t sub, uint32_t id, uint32_t step, int what,
+                uint32_t m, double* out, int64_t n)
+{
+    orc_rng r;
+    orc_rng_init(&r, seed, kind, sub, id, step);
+    for (int64_t i = 0; i < n; ++i) {
+        if (what == 0) out[i] = orc_uniform(&r);
+        else if (what == 1) out[i] = (double)orc_index(&r, m);
+        else { double a, b; orc_gauss2(&r, &a, &b); out[i] = a; if (i + 1 < n) out[++i] = b; }
+    }
+}

@@ -1,0 +1,64 @@
+/*
+ * orc.h - CPU oracle of the uniGasFoam particle time step (FP64, OpenMP).
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker; never linked into the
+ * product library.
+ *
+ * The oracle takes the same POD descriptors as include/ugx.h and follows the
+ * reference functions cited in orc.c, one function per reference routine.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include "../include/ugx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_world orc_world;
+
+const char* orc_last_error(void);
+
+int  orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
+                const ugx_model_desc* model, const ugx_boundary_desc* bnd, int n_bnd,
+                uint64_t seed, int64_t capacity, orc_world** out);
+void orc_destroy(orc_world* w);
+
+int  orc_fill(orc_world* w, const ugx_fill_desc* fill);
+int  orc_upload(orc_world* w, const ugx_particle_soa* p);
+int  orc_download(orc_world* w, ugx_particle_soa* p);
+int64_t orc_count(orc_world* w);
+
+/* One evolve(): inflow, move, sort, sample+collide, hybrid switch. */
+int  orc_step(orc_world* w, int n_steps);
+/* Phases of one step, for kernel-level parity (phase_mask bits:
+ * 1 inflow+move, 2 sort, 4 sample+collide, 8 hybrid switch, 16 advance step). */
+int  orc_step_phases(orc_world* w, int phase_mask);
+
+int  orc_samples(orc_world* w, double* out, int64_t len);
+int  orc_counters(orc_world* w, ugx_counters* out);
+int  orc_sigma_max(orc_world* w, double* out, int64_t len);
+int  orc_set_sigma_max(orc_world* w, const double* in, int64_t len);
+int  orc_cell_model(orc_world* w, int32_t* out, int64_t len);
+int  orc_set_threads(int n);
+
+/* Geometry as the oracle computes it: per cell reference point [n*3],
+ * volume [n], face count [n], planes [n*6*4] (nx ny nz d, local frame),
+ * face neighbour code [n*6], frame shift [n*6*3]. */
+int  orc_geometry(orc_world* w, double* cc, double* vol, int32_t* nfaces,
+                  double* planes, int32_t* nbr, double* delta);
+/* Inflow face-species expected counts per step (Bird 4.22). */
+int  orc_inflow_nbar(orc_world* w, double* out, int64_t len, int64_t* n_out);
+
+/* Primitive hooks for the known-answer and accuracy tests. */
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_mathfn(int fn, const double* in, const double* in2, double* out, int64_t n);
+void orc_stream(uint64_t seed, uint32_t kind, uint32_t sub, uint32_t id, uint32_t step, int what,
+                uint32_t m, double* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,152 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Same interface as
+unigasfoam_amd.core.Case so that parity tests drive both identically.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from unigasfoam_amd import abi
+from unigasfoam_amd.core import FIELD_NAMES, alloc_soa, make_upload_soa
+from unigasfoam_amd.spec import CaseSpec, fill_desc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_create.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
+                                 C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
+                                 C.c_uint64, C.c_int64, C.POINTER(vp)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_fill.argtypes = [vp, C.POINTER(abi.FillDesc)]
+        L.orc_upload.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
+        L.orc_download.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
+        L.orc_count.argtypes = [vp]
+        L.orc_count.restype = C.c_int64
+        L.orc_step.argtypes = [vp, C.c_int]
+        L.orc_step_phases.argtypes = [vp, C.c_int]
+        L.orc_samples.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.orc_counters.argtypes = [vp, C.POINTER(abi.Counters)]
+        L.orc_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.orc_set_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.orc_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
+        L.orc_inflow_nbar.argtypes = [vp, abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(f"oracle error {rc}: {lib().orc_last_error().decode()}")
+
+
+def set_threads(n):
+    return lib().orc_set_threads(int(n))
+
+
+class OracleCase:
+    def __init__(self, spec: CaseSpec, capacity=1 << 20, seed=None):
+        self.L = lib()
+        self.spec = spec
+        self.nc = spec.mesh.n_cells
+        md, sp, ns, m, bd, nb = spec.descriptors()
+        h = C.c_void_p()
+        _check(self.L.orc_create(C.byref(md), sp, ns, C.byref(m), bd, nb,
+                                 spec.seed if seed is None else seed, capacity, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fill(self, number_density, T, rot_T=None, velocity=(0.0, 0.0, 0.0), exact_per_cell=0):
+        f = fill_desc(self.spec, number_density, T, rot_T, velocity, exact_per_cell)
+        _check(self.L.orc_fill(self.h, C.byref(f)))
+
+    def upload(self, velocity, cell, species=None, erot=None, position=None, local=None):
+        soa, keep = make_upload_soa(velocity, cell, species, erot, position, local)
+        _check(self.L.orc_upload(self.h, C.byref(soa)))
+
+    def step(self, n=1):
+        _check(self.L.orc_step(self.h, int(n)))
+
+    def step_phases(self, mask):
+        _check(self.L.orc_step_phases(self.h, int(mask)))
+
+    def sync(self):
+        pass
+
+    def count(self):
+        return int(self.L.orc_count(self.h))
+
+    def particles(self):
+        n = self.count()
+        soa, a = alloc_soa(max(n, 1))
+        _check(self.L.orc_download(self.h, C.byref(soa)))
+        return {k: v[: soa.n] for k, v in a.items()}
+
+    def samples(self):
+        out = np.zeros((abi.S_COUNT, self.nc))
+        _check(self.L.orc_samples(self.h, abi.dptr(out), out.size))
+        return out
+
+    def counters(self):
+        k = abi.Counters()
+        _check(self.L.orc_counters(self.h, C.byref(k)))
+        return k.as_dict()
+
+    def sigma_max(self):
+        out = np.zeros(self.nc)
+        _check(self.L.orc_sigma_max(self.h, abi.dptr(out), self.nc))
+        return out
+
+    def set_sigma_max(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.L.orc_set_sigma_max(self.h, abi.dptr(v), self.nc))
+
+    def cell_model(self):
+        out = np.zeros(self.nc, dtype=np.int32)
+        _check(self.L.orc_cell_model(self.h, abi.iptr(out), self.nc))
+        return out
+
+    def geometry(self):
+        n = self.nc
+        g = {"cc": np.zeros((n, 3)), "vol": np.zeros(n), "nfaces": np.zeros(n, dtype=np.int32),
+             "planes": np.zeros((n, 6, 4)), "nbr": np.zeros((n, 6), dtype=np.int32), "delta": np.zeros((n, 6, 3))}
+        _check(self.L.orc_geometry(self.h, abi.dptr(g["cc"]), abi.dptr(g["vol"]), abi.iptr(g["nfaces"]),
+                                   abi.dptr(g["planes"]), abi.iptr(g["nbr"]), abi.dptr(g["delta"])))
+        nb = np.zeros(1 << 16)
+        k = C.c_int64()
+        _check(self.L.orc_inflow_nbar(self.h, abi.dptr(nb), len(nb), C.byref(k)))
+        g["nbar"] = nb[: k.value]
+        return g
+
+
+__all__ = ["OracleCase", "set_threads", "lib", "FIELD_NAMES"]

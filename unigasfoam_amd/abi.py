@@ -1,0 +1,125 @@
+"""ctypes mirror of include/ugx.h (POD descriptors and enums)."""
+import ctypes as C
+
+MAX_SPECIES = 8
+MAX_CELL_FACES = 6
+KB = 1.38064852e-23
+
+# ugx_status
+OK, ERR_ARG, ERR_DEVICE, ERR_CAPACITY, ERR_STATE, ERR_MESH, ERR_IO, ERR_UNSUPPORTED = range(8)
+# ugx_patch_type
+PATCH, WALL, SYMMETRY, EMPTY, CYCLIC = range(5)
+# collision / dsmc / bgk models
+COLL_NONE, COLL_DSMC, COLL_BGK, COLL_HYBRID = range(4)
+DSMC_NONE, DSMC_VHS, DSMC_VSS, DSMC_LB_VHS, DSMC_LB_VSS = range(5)
+BGK_NONE, BGK_BGK, BGK_ESBGK, BGK_USP = range(4)
+PARTNER_NTC = 0
+# boundary kinds
+BND_NONE, BND_SPECULAR, BND_DIFFUSE, BND_DELETION, BND_FREESTREAM = range(5)
+# precision
+FP64, FP32_STATE = 0, 1
+
+# sampler layout (ugx_sample_index)
+S_NAMES = ["N", "M", "MU", "MV", "MW", "MCC", "EROT", "ROTDOF",
+           "MUU", "MUV", "MUW", "MVV", "MVW", "MWW",
+           "MCCU", "MCCV", "MCCW", "EROTU", "EROTV", "EROTW"]
+S = {n: i for i, n in enumerate(S_NAMES)}
+S_BASE = len(S_NAMES)
+S_SPECIES_N = S_BASE
+S_SPECIES_MCC = S_BASE + MAX_SPECIES
+S_COUNT = S_BASE + 2 * MAX_SPECIES
+
+DBL_P = C.POINTER(C.c_double)
+I32_P = C.POINTER(C.c_int32)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_points", C.c_int32), ("n_faces", C.c_int32),
+        ("n_internal_faces", C.c_int32), ("n_cells", C.c_int32),
+        ("points", DBL_P), ("face_offsets", I32_P), ("face_points", I32_P),
+        ("owner", I32_P), ("neighbour", I32_P),
+        ("n_patches", C.c_int32), ("pad0", C.c_int32),
+        ("patch_start", I32_P), ("patch_size", I32_P),
+        ("patch_type", I32_P), ("patch_neighbour", I32_P),
+    ]
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 32), ("mass", C.c_double), ("diameter", C.c_double),
+        ("omega", C.c_double), ("alpha", C.c_double),
+        ("rotational_dof", C.c_int32), ("vibrational_modes", C.c_int32),
+        ("charge", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("n_equivalent_particles", C.c_double), ("delta_t", C.c_double),
+        ("collision_model", C.c_int32), ("dsmc_model", C.c_int32),
+        ("partner_model", C.c_int32), ("bgk_model", C.c_int32),
+        ("Tref", C.c_double), ("Zrot", C.c_double), ("theta", C.c_double),
+        ("macro_interpolation", C.c_int32), ("sample_interval", C.c_int32),
+        ("decomposition_interval", C.c_int32), ("smoothing_passes", C.c_int32),
+        ("breakdown_max", C.c_double), ("kn_theta", C.c_double),
+    ]
+
+
+class BoundaryDesc(C.Structure):
+    _fields_ = [
+        ("patch", C.c_int32), ("kind", C.c_int32),
+        ("temperature", C.c_double), ("velocity", C.c_double * 3),
+        ("rot_temperature", C.c_double),
+        ("number_density", C.c_double * MAX_SPECIES),
+    ]
+
+
+class FillDesc(C.Structure):
+    _fields_ = [
+        ("number_density", C.c_double * MAX_SPECIES), ("T", C.c_double * 3),
+        ("rot_temperature", C.c_double), ("velocity", C.c_double * 3),
+        ("exact_per_cell", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("device", C.c_int32),
+        ("particle_capacity", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
+class ParticleSoA(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("position", DBL_P), ("local_position", DBL_P),
+        ("velocity", DBL_P), ("erot", DBL_P), ("cell", I32_P), ("species", I32_P),
+    ]
+
+
+class Fields(C.Structure):
+    _fields_ = [(n, DBL_P) for n in (
+        "rhoN", "rhoM", "U_mean", "translational_T", "rotational_T", "overall_T",
+        "p", "Ma", "mfp", "mcr", "density_error", "velocity_error",
+        "temperature_error", "pressure_error", "shear_stress", "heat_flux")]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int64), ("steps", C.c_int64), ("candidates", C.c_int64),
+        ("collisions", C.c_int64), ("bgk_collisions", C.c_int64),
+        ("inserted", C.c_int64), ("deleted", C.c_int64), ("wall_hits", C.c_int64),
+        ("track_errors", C.c_int64), ("n_sampled_steps", C.c_int64),
+        ("time_sampled", C.c_double), ("n_dsmc_cells", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def dptr(a):
+    return a.ctypes.data_as(DBL_P) if a is not None else None
+
+
+def iptr(a):
+    return a.ctypes.data_as(I32_P) if a is not None else None

@@ -1,0 +1,240 @@
+"""ctypes binding of libugx.so (the HIP library) and the plain-solver Case.
+
+The library is loaded from this package directory; importing this module on
+a machine without the built library raises, and every call that reaches the
+device fails loudly when no GPU is present - there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .spec import CaseSpec, fill_desc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libugx.so")
+_lib = None
+
+# Every symbol include/ugx.h declares.
+EXPORTS = [
+    "ugx_version", "ugx_last_error", "ugx_case_create", "ugx_case_destroy", "ugx_case_fill",
+    "ugx_particles_upload", "ugx_particles_download", "ugx_particle_count", "ugx_step", "ugx_sync",
+    "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
+    "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download",
+    "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry",
+]
+
+
+class UgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ugx error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.ugx_version.restype = C.c_char_p
+        L.ugx_last_error.restype = C.c_char_p
+        vp = C.c_void_p
+        L.ugx_case_create.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
+                                      C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
+                                      C.POINTER(abi.Options), C.POINTER(vp)]
+        L.ugx_case_destroy.argtypes = [vp]
+        L.ugx_case_fill.argtypes = [vp, C.POINTER(abi.FillDesc)]
+        L.ugx_particles_upload.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
+        L.ugx_particles_download.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
+        L.ugx_particle_count.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.ugx_step.argtypes = [vp, C.c_int]
+        L.ugx_sync.argtypes = [vp]
+        L.ugx_samples_download.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_samples_reset.argtypes = [vp]
+        L.ugx_fields_compute.argtypes = [vp, C.POINTER(abi.Fields)]
+        L.ugx_counters_get.argtypes = [vp, C.POINTER(abi.Counters)]
+        L.ugx_sigma_max_download.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_sigma_max_upload.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_cell_model_download.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.ugx_cell_volumes.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_cell_centres.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_geometry.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
+                                   C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
+                                   abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P,
+                                   abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def check(rc, L=None):
+    if rc != 0:
+        L = L or lib()
+        raise UgxError(rc, L.ugx_last_error().decode(errors="replace"))
+
+
+def alloc_soa(n):
+    a = {
+        "local": np.zeros((n, 3)), "position": np.zeros((n, 3)), "velocity": np.zeros((n, 3)),
+        "erot": np.zeros(n), "cell": np.zeros(n, dtype=np.int32), "species": np.zeros(n, dtype=np.int32),
+    }
+    soa = abi.ParticleSoA()
+    soa.n = n
+    soa.local_position = abi.dptr(a["local"])
+    soa.position = abi.dptr(a["position"])
+    soa.velocity = abi.dptr(a["velocity"])
+    soa.erot = abi.dptr(a["erot"])
+    soa.cell = abi.iptr(a["cell"])
+    soa.species = abi.iptr(a["species"])
+    return soa, a
+
+
+def make_upload_soa(velocity, cell, species=None, erot=None, position=None, local=None):
+    n = len(cell)
+    keep = {
+        "velocity": np.ascontiguousarray(velocity, dtype=np.float64),
+        "cell": np.ascontiguousarray(cell, dtype=np.int32),
+        "species": np.ascontiguousarray(species if species is not None else np.zeros(n), dtype=np.int32),
+        "erot": np.ascontiguousarray(erot if erot is not None else np.zeros(n), dtype=np.float64),
+    }
+    soa = abi.ParticleSoA()
+    soa.n = n
+    if local is not None:
+        keep["local"] = np.ascontiguousarray(local, dtype=np.float64)
+        soa.local_position = abi.dptr(keep["local"])
+    else:
+        keep["position"] = np.ascontiguousarray(position, dtype=np.float64)
+        soa.position = abi.dptr(keep["position"])
+    soa.velocity = abi.dptr(keep["velocity"])
+    soa.cell = abi.iptr(keep["cell"])
+    soa.species = abi.iptr(keep["species"])
+    soa.erot = abi.dptr(keep["erot"])
+    return soa, keep
+
+
+FIELD_NAMES = ["rhoN", "rhoM", "U_mean", "translational_T", "rotational_T", "overall_T", "p", "Ma",
+               "mfp", "mcr", "density_error", "velocity_error", "temperature_error", "pressure_error",
+               "shear_stress", "heat_flux"]
+
+
+class Case:
+    """Plain GPU solver for one case (uniGasCloud with evolve() on the GPU)."""
+
+    def __init__(self, spec: CaseSpec, precision="fp64", capacity=0, device=0, seed=None):
+        self.L = lib()
+        self.spec = spec
+        self.nc = spec.mesh.n_cells
+        md, sp, ns, m, bd, nb = spec.descriptors()
+        opt = abi.Options()
+        opt.precision = abi.FP32_STATE if precision == "fp32" else abi.FP64
+        opt.device = device
+        opt.particle_capacity = capacity
+        opt.seed = spec.seed if seed is None else seed
+        h = C.c_void_p()
+        check(self.L.ugx_case_create(C.byref(md), sp, ns, C.byref(m), bd, nb, C.byref(opt), C.byref(h)), self.L)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ugx_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fill(self, number_density, T, rot_T=None, velocity=(0.0, 0.0, 0.0), exact_per_cell=0):
+        f = fill_desc(self.spec, number_density, T, rot_T, velocity, exact_per_cell)
+        check(self.L.ugx_case_fill(self.h, C.byref(f)), self.L)
+
+    def upload(self, velocity, cell, species=None, erot=None, position=None, local=None):
+        soa, keep = make_upload_soa(velocity, cell, species, erot, position, local)
+        check(self.L.ugx_particles_upload(self.h, C.byref(soa)), self.L)
+
+    def step(self, n=1):
+        check(self.L.ugx_step(self.h, int(n)), self.L)
+
+    def sync(self):
+        check(self.L.ugx_sync(self.h), self.L)
+
+    def count(self):
+        n = C.c_int64()
+        check(self.L.ugx_particle_count(self.h, C.byref(n)), self.L)
+        return n.value
+
+    def particles(self):
+        n = self.count()
+        soa, a = alloc_soa(max(n, 1))
+        check(self.L.ugx_particles_download(self.h, C.byref(soa)), self.L)
+        return {k: v[: soa.n] for k, v in a.items()}
+
+    def samples(self):
+        out = np.zeros((abi.S_COUNT, self.nc))
+        check(self.L.ugx_samples_download(self.h, abi.dptr(out), out.size), self.L)
+        return out
+
+    def reset_samples(self):
+        check(self.L.ugx_samples_reset(self.h), self.L)
+
+    def counters(self):
+        k = abi.Counters()
+        check(self.L.ugx_counters_get(self.h, C.byref(k)), self.L)
+        return k.as_dict()
+
+    def fields(self):
+        n = self.nc
+        arrs = {name: np.zeros(n * (3 if name in ("U_mean", "heat_flux") else 6 if name == "shear_stress" else 1))
+                for name in FIELD_NAMES}
+        f = abi.Fields()
+        for name in FIELD_NAMES:
+            setattr(f, name, abi.dptr(arrs[name]))
+        check(self.L.ugx_fields_compute(self.h, C.byref(f)), self.L)
+        arrs["U_mean"] = arrs["U_mean"].reshape(n, 3)
+        arrs["heat_flux"] = arrs["heat_flux"].reshape(n, 3)
+        arrs["shear_stress"] = arrs["shear_stress"].reshape(n, 6)
+        return arrs
+
+    def sigma_max(self):
+        out = np.zeros(self.nc)
+        check(self.L.ugx_sigma_max_download(self.h, abi.dptr(out), self.nc), self.L)
+        return out
+
+    def set_sigma_max(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(self.L.ugx_sigma_max_upload(self.h, abi.dptr(v), self.nc), self.L)
+
+    def cell_model(self):
+        out = np.zeros(self.nc, dtype=np.int32)
+        check(self.L.ugx_cell_model_download(self.h, abi.iptr(out), self.nc), self.L)
+        return out
+
+    def cell_volumes(self):
+        out = np.zeros(self.nc)
+        check(self.L.ugx_cell_volumes(self.h, abi.dptr(out), self.nc), self.L)
+        return out
+
+    def cell_centres(self):
+        out = np.zeros((self.nc, 3))
+        check(self.L.ugx_cell_centres(self.h, abi.dptr(out), out.size), self.L)
+        return out
+
+
+def geometry(spec: CaseSpec):
+    """Kernel-form geometry computed by the library's host code (no GPU)."""
+    L = lib()
+    md, sp, ns, m, bd, nb = spec.descriptors()
+    n = spec.mesh.n_cells
+    g = {"cc": np.zeros((n, 3)), "vol": np.zeros(n), "nfaces": np.zeros(n, dtype=np.int32),
+         "planes": np.zeros((n, 6, 4)), "nbr": np.zeros((n, 6), dtype=np.int32), "delta": np.zeros((n, 6, 3)),
+         "nbar": np.zeros(1 << 16)}
+    nin = C.c_int64()
+    check(L.ugx_geometry(C.byref(md), sp, ns, C.byref(m), bd, nb, abi.dptr(g["cc"]), abi.dptr(g["vol"]),
+                         abi.iptr(g["nfaces"]), abi.dptr(g["planes"]), abi.iptr(g["nbr"]), abi.dptr(g["delta"]),
+                         abi.dptr(g["nbar"]), len(g["nbar"]), C.byref(nin)), L)
+    g["nbar"] = g["nbar"][: nin.value]
+    return g

@@ -1,0 +1,954 @@
+// ugx_kernels.hip - the particle time step of uniGasCloud::evolve
+// (src/lagrangian/uniGas/clouds/uniGasCloud.C:821-869) as gfx950 kernels.
+//
+// Per step, on one HIP stream, for every case of a world at once:
+//   begin      per-case slot base, insert counts (Bird 4.22, stochastic round)
+//   inflow     create + first move of new simulators     (uniGasGeneralBoundary.C:537-761)
+//   move       face tracking + wall models + histogram   (uniGasParcel.C:35-145)
+//   scan       cell offsets                              (CloudWithModels.C:109-138)
+//   scatter    counting-sort slots (wave-aggregated atomics)
+//   segsort    per-tile LDS bitonic sort of each cell's slot list by previous
+//              slot -> the stable order, independent of atomic timing
+//   gather     SoA records into the sorted buffers
+//   sample     per-cell sums -> Delta-t weighted FP64 sampler (cellMeasurements.C:408-513,
+//              uniGasVolFields.C:723-837)
+//   bgk        BGK / ES-BGK / USP relaxation per cell    (bgkCollisions/derived/*)
+//   ntc        No-Time-Counter + VHS/VSS/LB per cell     (noTimeCounter.C:66-343)
+//   end        step counters
+#include <hip/hip_runtime.h>
+
+#include "ugx_kernels.hpp"
+
+namespace ugx {
+
+constexpr int kBlock = 256;
+constexpr int kScanBlock = 1024;      // elements per scan block (256 threads x 4)
+constexpr uint32_t kTileH = 2048;     // nominal simulators per sort tile
+constexpr uint32_t kTileCap = 4096;   // LDS keys per sort tile
+
+__device__ inline uint32_t wave_agg_inc(uint32_t* ctr, uint32_t key, bool active)
+{
+    const unsigned lane = __lane_id();
+    uint32_t ret = 0;
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        int leader = __ffsll(pending) - 1;
+        uint32_t lkey = __shfl(key, leader);
+        unsigned long long grp = __ballot(active && key == lkey) & pending;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(grp));
+        base = __shfl(base, leader);
+        if ((grp >> lane) & 1ull) ret = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        pending &= ~grp;
+    }
+    return ret;
+}
+
+// Adds v to counters[ccase][idx]; one atomic per wave when the wave's
+// non-zero contributions share a case.  Called by every lane of the wave.
+__device__ inline void case_add(unsigned long long* counters, int ccase, int idx, unsigned long long v)
+{
+    unsigned long long nz = __ballot(v != 0ull);
+    if (nz == 0ull) return;
+    int leader = __ffsll(nz) - 1;
+    int lc = __shfl(ccase, leader);
+    unsigned long long other = __ballot(v != 0ull && ccase != lc);
+    if (other == 0ull) {
+        unsigned long long s = v;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((int)__lane_id() == leader) atomicAdd(&counters[(size_t)lc * kCntNum + idx], s);
+    } else if (v != 0ull) {
+        atomicAdd(&counters[(size_t)ccase * kCntNum + idx], v);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+template <class R>
+__global__ void k_begin_step(WorldView<R> w)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < w.ncases) w.cases[c].particle_begin = w.off[w.cases[c].cell_begin];
+    if (c == 0) { w.st->n_ins = 0u; }
+}
+
+// ------------------------------------------------------------- inflow
+
+// One block: insert counts per face-species (uniGasGeneralBoundary.C:602-609)
+// and their exclusive scan.
+template <class R>
+__global__ void k_inflow_count(WorldView<R> w)
+{
+    __shared__ uint32_t sh[1024];
+    const int tid = threadIdx.x;
+    uint32_t carry = 0;
+    for (int base = 0; base < w.n_inflow; base += 1024) {
+        int k = base + tid;
+        uint32_t cnt = 0;
+        if (k < w.n_inflow) {
+            const InflowFS& e = w.inflow[k];
+            const CaseDev& cs = w.cases[e.ccase];
+            if (cs.active) {
+                Rng r;
+                rng_init(r, cs.seed_lo, cs.seed_hi, kRngInflowN, 0u, (uint32_t)e.local_id, cs.step);
+                double nb = e.nbar;
+                double fl = floor(nb);
+                long long c = (long long)fl;
+                if (c < 0) c = 0;
+                if (uniform(r) < nb - fl) c += 1;
+                cnt = (uint32_t)c;
+            }
+        }
+        sh[tid] = cnt;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint32_t v = (tid >= o) ? sh[tid - o] : 0u;
+            __syncthreads();
+            sh[tid] += v;
+            __syncthreads();
+        }
+        if (k < w.n_inflow) w.ins_off[k] = carry + sh[tid] - cnt;
+        uint32_t tot = sh[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (tid == 0) {
+        w.ins_off[w.n_inflow] = carry;
+        if ((unsigned long long)w.st->n + carry > (unsigned long long)w.cap) {
+            w.st->overflow = 1u;
+            carry = 0u;
+        }
+        w.st->n_ins = carry;
+    }
+}
+
+// Bird eq. 12.5 insertion + first move over (1 - R) deltaT.
+template <class R>
+__device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, const CaseDev& cs, Rng& r,
+                                  R x[3], double U[3], double& erot, uint32_t& cell, TrackOut& to)
+{
+    double us = uniform(r);
+    int tri = 0;
+    for (int t = 0; t < e.nv - 2; ++t) { tri = t; if (e.cum[t] >= us) break; }
+    double s = sqrt(uniform(r));
+    double tt = uniform(r);
+    for (int d = 0; d < 3; ++d)
+        x[d] = (R)(e.vloc[0][d] * (1.0 - s) + e.vloc[tri + 1][d] * (s * (1.0 - tt)) + e.vloc[tri + 2][d] * (s * tt));
+    const int g = e.species;
+    double m = w.sp.mass[g];
+    double mps = sqrt(2.0 * kB * e.T / m);
+    double udn = e.U[0] * e.n[0] + e.U[1] * e.n[1] + e.U[2] * e.n[2];
+    double sCos = udn / mps;
+    double uNormal;
+    if (fabs(udn) > kVSmall) {
+        double cA = sCos + sqrt(sCos * sCos + 2.0);
+        double cB = 0.5 * (1.0 + sCos * (sCos - sqrt(sCos * sCos + 2.0)));
+        double scale = 3.0;
+        if (sCos < -3.0) scale = fabs(sCos) + 1.0;
+        uNormal = 0.0;
+        for (int it = 0; it < 100000; ++it) {
+            double th = scale * (2.0 * uniform(r) - 1.0);
+            uNormal = th + sCos;
+            double P = (uNormal < 0.0) ? -1.0 : 2.0 * uNormal / cA * dexp(cB - th * th);
+            if (!(P < uniform(r))) break;
+        }
+    } else {
+        uNormal = sqrt(-dlog(uniform(r)));
+    }
+    double g1, g2;
+    gauss2(r, g1, g2);
+    double sig = sqrt(kB * e.T / m);
+    double ut1 = e.t1[0] * e.U[0] + e.t1[1] * e.U[1] + e.t1[2] * e.U[2];
+    double ut2 = e.t2[0] * e.U[0] + e.t2[1] * e.U[1] + e.t2[2] * e.U[2];
+    for (int d = 0; d < 3; ++d)
+        U[d] = ((sig * (g1 * e.t1[d] + g2 * e.t2[d]) + ut1 * e.t1[d]) + ut2 * e.t2[d]) + mps * uNormal * e.n[d];
+    const int rd = w.sp.rotdof[g];
+    erot = equip_erot(r, e.rotT, rd);
+    cell = (uint32_t)e.cell;
+    double frac = uniform(r);
+    to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, m, rd, (1.0 - frac) * cs.dt, r);
+    return to.alive;
+}
+
+template <class R>
+__global__ void k_inflow_gen(WorldView<R> w)
+{
+    const uint32_t total = w.st->n_ins;
+    const uint32_t n0 = w.st->n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < w.max_ins; base += stride) {
+        uint32_t t = base + threadIdx.x;
+        bool active = t < total;
+        uint32_t cell = kDead;
+        int ccase = 0;
+        TrackOut to{1, 0, 0};
+        if (active) {
+            int lo = 0, hi = w.n_inflow - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (w.ins_off[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            const InflowFS& e = w.inflow[lo];
+            const CaseDev& cs = w.cases[e.ccase];
+            ccase = e.ccase;
+            uint32_t i = t - w.ins_off[lo];
+            Rng r;
+            rng_init(r, cs.seed_lo, cs.seed_hi, kRngInflowP, i, (uint32_t)e.local_id, cs.step);
+            R x[3];
+            double U[3], erot;
+            make_inflow<R>(w, e, cs, r, x, U, erot, cell, to);
+            uint32_t j = n0 + t;
+            w.a.x[j] = x[0]; w.a.y[j] = x[1]; w.a.z[j] = x[2];
+            w.a.vx[j] = (R)U[0]; w.a.vy[j] = (R)U[1]; w.a.vz[j] = (R)U[2];
+            w.a.erot[j] = (R)erot;
+            w.a.cell[j] = cell;
+            w.a.sp[j] = (uint32_t)e.species;
+        }
+        bool alive = active && cell != kDead;
+        wave_agg_inc(w.cnt, alive ? cell : 0u, alive);
+        case_add(w.counters, ccase, kCntInserted, active ? 1ull : 0ull);
+        case_add(w.counters, ccase, kCntDeleted, (active && !alive) ? 1ull : 0ull);
+        case_add(w.counters, ccase, kCntWallHits, (unsigned long long)to.hits);
+        case_add(w.counters, ccase, kCntTrackErrors, (unsigned long long)to.errs);
+    }
+}
+
+// --------------------------------------------------------------- move
+
+template <class R>
+__global__ void k_move(WorldView<R> w)
+{
+    const uint32_t n0 = w.st->n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
+        if (base >= n0) break;
+        uint32_t t = base + threadIdx.x;
+        bool active = t < n0;
+        uint32_t cell = kDead;
+        int ccase = 0;
+        TrackOut to{1, 0, 0};
+        if (active) {
+            cell = w.a.cell[t];
+            ccase = w.cell_case[cell];
+            const CaseDev& cs = w.cases[ccase];
+            const uint32_t g = w.a.sp[t];
+            R x[3] = {w.a.x[t], w.a.y[t], w.a.z[t]};
+            double U[3] = {(double)w.a.vx[t], (double)w.a.vy[t], (double)w.a.vz[t]};
+            double erot = (double)w.a.erot[t];
+            Rng r;
+            rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
+            to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w.sp.mass[g], w.sp.rotdof[g], cs.dt, r);
+            w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
+            w.a.cell[t] = cell;
+            if (to.hits) {
+                w.a.vx[t] = (R)U[0]; w.a.vy[t] = (R)U[1]; w.a.vz[t] = (R)U[2];
+                w.a.erot[t] = (R)erot;
+            }
+        }
+        bool alive = active && cell != kDead;
+        wave_agg_inc(w.cnt, alive ? cell : 0u, alive);
+        case_add(w.counters, ccase, kCntDeleted, (active && !alive) ? 1ull : 0ull);
+        case_add(w.counters, ccase, kCntWallHits, (unsigned long long)to.hits);
+        case_add(w.counters, ccase, kCntTrackErrors, (unsigned long long)to.errs);
+    }
+}
+
+// Histogram only (sort after fill / upload).
+template <class R>
+__global__ void k_hist(WorldView<R> w)
+{
+    const uint32_t n_all = w.st->n + w.st->n_ins;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
+        if (base >= n_all) break;
+        uint32_t t = base + threadIdx.x;
+        bool active = t < n_all;
+        uint32_t c = active ? w.a.cell[t] : kDead;
+        bool alive = active && c != kDead;
+        wave_agg_inc(w.cnt, alive ? c : 0u, alive);
+    }
+}
+
+// --------------------------------------------------------------- scan
+
+__global__ void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* bsum)
+{
+    __shared__ uint32_t sh[kBlock];
+    uint32_t b = blockIdx.x * kScanBlock;
+    uint32_t s = 0;
+    for (int k = 0; k < 4; ++k) {
+        uint32_t i = b + threadIdx.x * 4 + k;
+        if (i < n) s += in[i];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
+}
+
+__global__ void k_scan_bsum(uint32_t* bsum, uint32_t nb)
+{
+    __shared__ uint32_t sh[1024];
+    const int tid = threadIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        uint32_t k = base + tid;
+        uint32_t v0 = (k < nb) ? bsum[k] : 0u;
+        sh[tid] = v0;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint32_t v = (tid >= o) ? sh[tid - o] : 0u;
+            __syncthreads();
+            sh[tid] += v;
+            __syncthreads();
+        }
+        if (k < nb) bsum[k] = carry + sh[tid] - v0;
+        uint32_t tot = sh[1023];
+        __syncthreads();
+        carry += tot;
+    }
+}
+
+// Exclusive scan of in[0..n) into out; out[n-1] is the total when in[n-1]==0.
+__global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* bsum, uint32_t* out, uint32_t* total_out)
+{
+    __shared__ uint32_t sh[kBlock];
+    uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    uint32_t v[4], s = 0;
+    for (int k = 0; k < 4; ++k) { v[k] = (b + k < n) ? in[b + k] : 0u; s += v[k]; }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        uint32_t t = ((int)threadIdx.x >= o) ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = bsum[blockIdx.x] + sh[threadIdx.x] - s;
+    for (int k = 0; k < 4; ++k) {
+        if (b + k < n) {
+            out[b + k] = run;
+            if (b + k == n - 1 && total_out) *total_out = run;
+        }
+        run += v[k];
+    }
+}
+
+static void scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* bsum, uint32_t* total, hipStream_t s)
+{
+    uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, bsum);
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, s, bsum, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s, in, n, bsum, out, total);
+}
+
+// ------------------------------------------------------------ scatter
+
+template <class R>
+__global__ void k_scatter(WorldView<R> w)
+{
+    const uint32_t n_all = w.st->n + w.st->n_ins;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
+        if (base >= n_all) break;
+        uint32_t t = base + threadIdx.x;
+        bool active = t < n_all;
+        uint32_t c = active ? w.a.cell[t] : kDead;
+        bool alive = active && c != kDead;
+        uint32_t rank = wave_agg_inc(w.cursor, alive ? c : 0u, alive);
+        if (alive) w.perm[w.off[c] + rank] = t;
+    }
+}
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Sorts every cell's slot list ascending (the stable counting-sort order).
+// Tile t owns the cells whose first slot lies in [t*H, (t+1)*H).
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_segsort(WorldView<R> w)
+{
+    __shared__ unsigned long long key[kTileCap];
+    const uint32_t nn = w.st->n_new;
+    const uint32_t start = blockIdx.x * kTileH;
+    if (start >= nn) return;
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    uint32_t c0 = lower_bound_u32(w.off, ncp, start);
+    uint32_t c1 = lower_bound_u32(w.off, ncp, start + kTileH);
+    if (c0 > (uint32_t)w.nc) c0 = (uint32_t)w.nc;
+    if (c1 > (uint32_t)w.nc) c1 = (uint32_t)w.nc;
+    const uint32_t p0 = w.off[c0], p1 = w.off[c1];
+    const uint32_t size = p1 - p0;
+    if (size <= 1u) return;
+    if (size <= kTileCap) {
+        uint32_t P = 2;
+        while (P < size) P <<= 1;
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+            unsigned long long k = ~0ull;
+            if (i < size) {
+                uint32_t slot = w.perm[p0 + i];
+                k = ((unsigned long long)(w.a.cell[slot] - c0) << 32) | slot;
+            }
+            key[i] = k;
+        }
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= P; kk <<= 1) {
+            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                    uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        unsigned long long a = key[i], b = key[ixj];
+                        bool up = (i & kk) == 0;
+                        if ((a > b) == up) { key[i] = b; key[ixj] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) w.perm[p0 + i] = (uint32_t)key[i];
+    } else {
+        // A cell larger than one tile: insertion sort per cell in global memory.
+        if (threadIdx.x == 0) atomicAdd(&w.st->big_tiles, 1u);
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+            uint32_t b = w.off[c], e = w.off[c + 1];
+            for (uint32_t i = b + 1; i < e; ++i) {
+                uint32_t v = w.perm[i];
+                uint32_t j = i;
+                while (j > b && w.perm[j - 1] > v) { w.perm[j] = w.perm[j - 1]; --j; }
+                w.perm[j] = v;
+            }
+        }
+    }
+}
+
+template <class R>
+__global__ void k_gather(WorldView<R> w)
+{
+    const uint32_t nn = w.st->n_new;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nn; j += stride) {
+        uint32_t i = w.perm[j];
+        w.b.x[j] = w.a.x[i]; w.b.y[j] = w.a.y[i]; w.b.z[j] = w.a.z[i];
+        w.b.vx[j] = w.a.vx[i]; w.b.vy[j] = w.a.vy[i]; w.b.vz[j] = w.a.vz[i];
+        w.b.erot[j] = w.a.erot[i];
+        w.b.cell[j] = w.a.cell[i];
+        w.b.sp[j] = w.a.sp[i];
+    }
+}
+
+// ------------------------------------------------------------- sample
+
+// Per-cell sums of cellMeasurements::calculateFields in sorted order.
+template <class R>
+__device__ inline void cell_sums(const Parts<R>& p, const Species& sp, uint32_t b, uint32_t e, double S[kNSamp])
+{
+#pragma unroll
+    for (int k = 0; k < kNSamp; ++k) S[k] = 0.0;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t g = p.sp[i];
+        const int s = sp.local[g];
+        const double m = sp.mass[g];
+        const double vx = (double)p.vx[i], vy = (double)p.vy[i], vz = (double)p.vz[i];
+        const double er = (double)p.erot[i];
+        const double mcc = m * (vx * vx + vy * vy + vz * vz);
+        S[UGX_S_N] += 1.0;
+        S[UGX_S_M] += m;
+        S[UGX_S_MU] += m * vx;
+        S[UGX_S_MV] += m * vy;
+        S[UGX_S_MW] += m * vz;
+        S[UGX_S_MCC] += mcc;
+        S[UGX_S_EROT] += er;
+        S[UGX_S_ROTDOF] += (double)sp.rotdof[g];
+        S[UGX_S_MUU] += m * (vx * vx);
+        S[UGX_S_MUV] += m * (vx * vy);
+        S[UGX_S_MUW] += m * (vx * vz);
+        S[UGX_S_MVV] += m * (vy * vy);
+        S[UGX_S_MVW] += m * (vy * vz);
+        S[UGX_S_MWW] += m * (vz * vz);
+        S[UGX_S_MCCU] += mcc * vx;
+        S[UGX_S_MCCV] += mcc * vy;
+        S[UGX_S_MCCW] += mcc * vz;
+        S[UGX_S_EROTU] += er * vx;
+        S[UGX_S_EROTV] += er * vy;
+        S[UGX_S_EROTW] += er * vz;
+#pragma unroll
+        for (int q = 0; q < kNS; ++q)
+            if (s == q) { S[UGX_S_SPECIES_N + q] += 1.0; S[UGX_S_SPECIES_MCC + q] += mcc; }
+    }
+}
+
+template <class R>
+__global__ void k_sample(WorldView<R> w)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += stride) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!cs.active) continue;
+        const bool sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
+        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
+        if (!sample && !hybrid) continue;
+        double S[kNSamp];
+        cell_sums<R>(w.b, w.sp, w.off[c], w.off[c + 1], S);
+        const double dt = cs.dt;
+        if (sample) {
+#pragma unroll
+            for (int k = 0; k < kNSamp; ++k) w.samp[(size_t)k * w.nc + c] += dt * S[k];
+        }
+        if (hybrid) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w.knacc[(size_t)8 * c + k] += dt * S[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- NTC
+
+template <class R>
+__global__ void k_ntc(WorldView<R> w)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t cb = blockIdx.x * blockDim.x; cb < (uint32_t)w.nc; cb += stride) {
+        uint32_t c = cb + threadIdx.x;
+        unsigned long long cand = 0, coll = 0;
+        int ccase = 0;
+        if (c < (uint32_t)w.nc) {
+            ccase = w.cell_case[c];
+            const CaseDev& cs = w.cases[ccase];
+            const uint32_t b = w.off[c];
+            const uint32_t nC = w.off[c + 1] - b;
+            if (cs.active && (cs.coll_model == UGX_COLL_DSMC || cs.coll_model == UGX_COLL_HYBRID)
+                && cs.dsmc_model != UGX_DSMC_NONE && w.cmodel[c] == 1 && nC >= 2) {
+                Rng r;
+                rng_init(r, cs.seed_lo, cs.seed_hi, kRngNtc, 0u, c - (uint32_t)cs.cell_begin, cs.step);
+                const double smax0 = w.smax[c];
+                double smax = smax0;
+                const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * smax0 * cs.dt / w.vol[c];
+                const double fl = floor(sel);
+                long long ncand = (long long)fl;
+                if (uniform(r) < sel - fl) ncand += 1;
+                cand = (unsigned long long)ncand;
+                for (long long k = 0; k < ncand; ++k) {
+                    uint32_t P = uindex(r, nC);
+                    uint32_t Q;
+                    do { Q = uindex(r, nC); } while (Q == P);
+                    const uint32_t ip = b + P, iq = b + Q;
+                    const uint32_t gp = w.b.sp[ip], gq = w.b.sp[iq];
+                    const int pk = cs.pair_base + w.sp.local[gp] * cs.ns + w.sp.local[gq];
+                    double UP[3] = {(double)w.b.vx[ip], (double)w.b.vy[ip], (double)w.b.vz[ip]};
+                    double UQ[3] = {(double)w.b.vx[iq], (double)w.b.vy[iq], (double)w.b.vz[iq]};
+                    double d0 = UP[0] - UQ[0], d1 = UP[1] - UQ[1], d2 = UP[2] - UQ[2];
+                    double cR2 = d0 * d0 + d1 * d1 + d2 * d2;
+                    double scr = (cR2 < kVSmall) ? 0.0 : w.pr.C[pk] * dexp(w.pr.e[pk] * dlog(cR2));
+                    if (scr > smax) smax = scr;
+                    if (scr / smax0 > uniform(r)) {
+                        double eP = (double)w.b.erot[ip], eQ = (double)w.b.erot[iq];
+                        binary_collide(r, cs.dsmc_model, cs.inv_zrot, w.sp.rotdof[gp], w.sp.rotdof[gq],
+                                       w.sp.mass[gp], w.sp.mass[gq], w.pr.mr[pk], w.pr.chi[pk], w.pr.alpha[pk],
+                                       UP, UQ, eP, eQ);
+                        w.b.vx[ip] = (R)UP[0]; w.b.vy[ip] = (R)UP[1]; w.b.vz[ip] = (R)UP[2];
+                        w.b.vx[iq] = (R)UQ[0]; w.b.vy[iq] = (R)UQ[1]; w.b.vz[iq] = (R)UQ[2];
+                        w.b.erot[ip] = (R)eP; w.b.erot[iq] = (R)eQ;
+                        coll += 1;
+                    }
+                }
+                w.smax[c] = smax;
+            }
+        }
+        case_add(w.counters, ccase, kCntCandidates, cand);
+        case_add(w.counters, ccase, kCntCollisions, coll);
+    }
+}
+
+// ---------------------------------------------------------------- BGK
+
+template <class R>
+__global__ void k_bgk(WorldView<R> w)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t cb = blockIdx.x * blockDim.x; cb < (uint32_t)w.nc; cb += stride) {
+        uint32_t c = cb + threadIdx.x;
+        unsigned long long nrel_out = 0;
+        int ccase = 0;
+        if (c < (uint32_t)w.nc) {
+            ccase = w.cell_case[c];
+            const CaseDev& cs = w.cases[ccase];
+            if (cs.active && (cs.coll_model == UGX_COLL_BGK || cs.coll_model == UGX_COLL_HYBRID)
+                && cs.bgk_model != UGX_BGK_NONE) {
+                const int model = cs.bgk_model;
+                const uint32_t b = w.off[c];
+                const uint32_t nC = w.off[c + 1] - b;
+                double S[kNSamp];
+                cell_sums<R>(w.b, w.sp, b, b + nC, S);
+                const double FN = cs.fn, dt = cs.dt, V = w.vol[c];
+                const double N = S[UGX_S_N];
+                const bool usp = model == UGX_BGK_USP;
+                bool perform = true;
+                double T = 0.0, p = 0.0, U[3] = {0.0, 0.0, 0.0};
+                double Pt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                double q[3], sh[6];
+                for (int d = 0; d < 3; ++d) q[d] = w.uspq[3 * (size_t)c + d];
+                for (int d = 0; d < 6; ++d) sh[d] = w.usps[6 * (size_t)c + d];
+                double Pr = 0.0, nu = 0.0;
+                if (N > kVSmall) {
+                    const double rhoN = FN * N / V;
+                    for (int d = 0; d < 3; ++d) U[d] = S[UGX_S_MU + d] / S[UGX_S_M];
+                    const double uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+                    T = (S[UGX_S_MCC] - S[UGX_S_M] * uu) / (3.0 * kB * N);
+                    p = rhoN * kB * T;
+                    const int ia[6] = {0, 0, 0, 1, 1, 2};
+                    const int ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+                        Pt[k] = rhoN * (S[UGX_S_MUU + k] / N - (S[UGX_S_M] / N) * U[ia[k]] * U[ib[k]]);
+                    if (usp) {
+                        const double tr = (Pt[0] + Pt[3] + Pt[5]) / 3.0;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) sh[k] = Pt[k];
+                        sh[0] = sh[0] - tr; sh[3] = sh[3] - tr; sh[5] = sh[5] - tr;
+                        const int r0[3] = {0, 1, 2}, r1[3] = {1, 3, 4}, r2[3] = {2, 4, 5};
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                            q[d] = rhoN * (0.5 * (S[UGX_S_MCCU + d] / N) - 0.5 * (S[UGX_S_MCC] / N) * U[d]
+                                           + S[UGX_S_EROTU + d] / N - (S[UGX_S_EROT] / N) * U[d])
+                                   - Pt[r0[d]] * U[0] - Pt[r1[d]] * U[1] - Pt[r2[d]] * U[2];
+                    }
+                    if (N > (usp ? 2.0 : 1.0)) {
+                        const double f1 = N / (N - 1.0);
+                        p = f1 * p;
+                        T = f1 * T;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) Pt[k] = f1 * Pt[k];
+                        if (usp) {
+                            const double f2 = N * N / (N - 1.0) / (N - 2.0);
+                            for (int d = 0; d < 3; ++d) q[d] = f2 * q[d];
+#pragma unroll
+                            for (int k = 0; k < 6; ++k) sh[k] = f1 * sh[k];
+                        }
+                    } else {
+                        perform = false;
+                    }
+                    if (T > kVSmall) {
+                        double visc = 0.0;
+                        for (int s = 0; s < cs.ns; ++s) {
+                            const int g = cs.sp_base + s;
+                            double nsv = 0.0;
+#pragma unroll
+                            for (int qq = 0; qq < kNS; ++qq) if (qq == s) nsv = S[UGX_S_SPECIES_N + qq];
+                            visc = visc + nsv * w.sp.muref[g] * dpow(T / cs.tref, w.sp.omega[g]);
+                            Pr = Pr + nsv * w.sp.prterm[g];
+                        }
+                        visc = visc / N;
+                        Pr = Pr / N;
+                        nu = (model == UGX_BGK_ESBGK) ? Pr * p / visc : p / visc;
+                    } else {
+                        perform = false;
+                    }
+                } else {
+                    perform = false;
+                }
+                if (usp) {
+                    const double th = cs.theta;
+                    for (int d = 0; d < 3; ++d)
+                        q[d] = th * q[d] / (1.0 + 0.5 * Pr * nu * dt) + (1.0 - th) * w.uspq[3 * (size_t)c + d];
+                    for (int k = 0; k < 6; ++k)
+                        sh[k] = th * sh[k] / (1.0 + 0.5 * nu * dt) + (1.0 - th) * w.usps[6 * (size_t)c + k];
+                    for (int d = 0; d < 3; ++d) w.uspq[3 * (size_t)c + d] = q[d];
+                    for (int k = 0; k < 6; ++k) w.usps[6 * (size_t)c + k] = sh[k];
+                }
+                bool raised = false;
+                if (perform && w.cmodel[c] == 0) {
+                    Rng r;
+                    rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
+                    const double pc = N * (1.0 - dexp(-nu * dt));
+                    const double fl = floor(pc);
+                    long long nrel = (long long)fl;
+                    if (uniform(r) < pc - fl) nrel += 1;
+                    if (nrel > (long long)nC) nrel = nC;
+                    nrel_out = (unsigned long long)nrel;
+                    uint32_t* idx = w.scratch + b;
+                    for (uint32_t i = 0; i < nC; ++i) idx[i] = i;
+                    double Sm[9];
+                    if (model == UGX_BGK_ESBGK) {
+                        const double a = 0.5 * (1.0 - Pr) / Pr;
+                        const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) {
+                            const double dl = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+                            Sm[k] = dl - a * (Pt[full[k]] / p - dl);
+                        }
+                    }
+                    const double tau = 0.5 * nu * dt;
+                    const double ex = 1.0 + 2.0 / (dexp(2.0 * tau) - 1.0);
+                    const double cQ = 2.0 * (1.0 - Pr * tau * ex);
+                    const double cS = 1.0 - tau * ex;
+                    double pmax = w.uspmax[c];
+                    for (long long i = 0; i < nrel; ++i) {
+                        const uint32_t j = (uint32_t)i + uindex(r, (uint32_t)((long long)nC - i));
+                        const uint32_t tmp = idx[i]; idx[i] = idx[j]; idx[j] = tmp;
+                        const uint32_t pi = b + idx[i];
+                        const double m = w.sp.mass[w.b.sp[pi]];
+                        const double u0 = sqrt(2.0 * kB * T / m);
+                        double xi[3];
+                        if (usp) {
+                            for (int it = 0; it < 100000; ++it) {
+                                double g0, g1, g2;
+                                gauss3(r, g0, g1, g2);
+                                xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
+                                const double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+                                const double vTr = vSq / 3.0;
+                                const double prob = 1.0 + cQ / (p * u0) * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq / 2.5 - 1.0)
+                                                    + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
+                                                                + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
+                                                                + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
+                                if (prob > pmax && prob < 10.0) { pmax = prob; raised = true; break; }
+                                if (uniform(r) < prob / pmax) break;
+                            }
+                        } else {
+                            double g0, g1, g2;
+                            gauss3(r, g0, g1, g2);
+                            xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
+                            if (model == UGX_BGK_ESBGK) {
+                                double y[3];
+                                for (int d = 0; d < 3; ++d) y[d] = Sm[3 * d] * xi[0] + Sm[3 * d + 1] * xi[1] + Sm[3 * d + 2] * xi[2];
+                                xi[0] = y[0]; xi[1] = y[1]; xi[2] = y[2];
+                            }
+                        }
+                        w.b.vx[pi] = (R)(U[0] + u0 * xi[0]);
+                        w.b.vy[pi] = (R)(U[1] + u0 * xi[1]);
+                        w.b.vz[pi] = (R)(U[2] + u0 * xi[2]);
+                    }
+                    w.uspmax[c] = pmax;
+                    // conserveMomentumAndEnergy
+                    double ke = 0.0, mom0 = 0.0, mom1 = 0.0, mom2 = 0.0;
+                    for (uint32_t i = b; i < b + nC; ++i) {
+                        const double m = w.sp.mass[w.b.sp[i]];
+                        const double vx = (double)w.b.vx[i], vy = (double)w.b.vy[i], vz = (double)w.b.vz[i];
+                        ke += m * (vx * vx + vy * vy + vz * vz);
+                        mom0 += m * vx; mom1 += m * vy; mom2 += m * vz;
+                    }
+                    const double pU0 = mom0 / S[UGX_S_M], pU1 = mom1 / S[UGX_S_M], pU2 = mom2 / S[UGX_S_M];
+                    const double pT = (ke - S[UGX_S_M] * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
+                    if (pT > kVSmall) {
+                        const double sc = sqrt(T / pT);
+                        for (uint32_t i = b; i < b + nC; ++i) {
+                            w.b.vx[i] = (R)(U[0] + ((double)w.b.vx[i] - pU0) * sc);
+                            w.b.vy[i] = (R)(U[1] + ((double)w.b.vy[i] - pU1) * sc);
+                            w.b.vz[i] = (R)(U[2] + ((double)w.b.vz[i] - pU2) * sc);
+                        }
+                    }
+                }
+                if (usp && !raised) w.uspmax[c] = w.uspmax[c] * 0.999;
+            }
+        }
+        case_add(w.counters, ccase, kCntBgk, nrel_out);
+    }
+}
+
+// ------------------------------------------------------------ end step
+
+template <class R>
+__global__ void k_end_step(WorldView<R> w)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < w.ncases) {
+        CaseDev& cs = w.cases[c];
+        if (cs.active) {
+            if (((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u) w.counters[(size_t)c * kCntNum + kCntSampled] += 1ull;
+            if (cs.coll_model == UGX_COLL_HYBRID) cs.kn_time += cs.dt;
+            cs.step += 1u;
+        }
+    }
+    if (c == 0) w.st->n = w.st->n_new;
+}
+
+template <class R>
+__global__ void k_finish_sort(WorldView<R> w)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) { w.st->n = w.st->n_new; w.st->n_ins = 0u; }
+}
+
+// --------------------------------------------------------------- fill
+
+template <class R>
+__global__ void k_fill_count(WorldView<R> w, int ccase, const double* nd, int exact, uint32_t* fcnt, int ns, int ncl)
+{
+    const CaseDev& cs = w.cases[ccase];
+    double ntot = 0.0;
+    for (int s = 0; s < ns; ++s) ntot += nd[s];
+    const uint32_t total = (uint32_t)ncl * (uint32_t)ns;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t <= total; t += gridDim.x * blockDim.x) {
+        if (t == total) { fcnt[t] = 0u; continue; }
+        const uint32_t cl = t / (uint32_t)ns, s = t % (uint32_t)ns;
+        const uint32_t c = (uint32_t)cs.cell_begin + cl;
+        long long k;
+        if (exact > 0) {
+            k = (ntot > 0.0) ? (long long)floor((double)exact * nd[s] / ntot + 0.5) : 0;
+        } else {
+            Rng r;
+            rng_init(r, cs.seed_lo, cs.seed_hi, kRngFill, s, cl, 0u);
+            const double req = nd[s] / cs.fn * w.vol[c];
+            const double f0 = floor(req);
+            k = (long long)f0;
+            if (uniform(r) < req - f0) k += 1;
+        }
+        fcnt[t] = (uint32_t)k;
+    }
+}
+
+template <class R>
+__global__ void k_fill_gen(WorldView<R> w, int ccase, const double* nd, const double* Tax, double rotT,
+                           const double* Uf, int exact, const double* bblo, const double* bbhi,
+                           const uint32_t* foff, int ns, int ncl)
+{
+    const CaseDev& cs = w.cases[ccase];
+    const uint32_t total = (uint32_t)ncl * (uint32_t)ns;
+    const uint32_t nadd = foff[total];
+    const uint32_t n0 = w.st->n;
+    if ((unsigned long long)n0 + nadd > (unsigned long long)w.cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.st->overflow = 1u;
+        return;
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t cl = t / (uint32_t)ns, s = t % (uint32_t)ns;
+        const uint32_t c = (uint32_t)cs.cell_begin + cl;
+        const uint32_t g = (uint32_t)cs.sp_base + s;
+        Rng r;
+        rng_init(r, cs.seed_lo, cs.seed_hi, kRngFill, s, cl, 0u);
+        if (exact <= 0) (void)uniform(r);
+        const double m = w.sp.mass[g];
+        double sig[3];
+        for (int d = 0; d < 3; ++d) sig[d] = sqrt(kB * Tax[d] / m);
+        const uint32_t cnt = foff[t + 1] - foff[t];
+        const double* lo = bblo + 3 * (size_t)cl;
+        const double* hi = bbhi + 3 * (size_t)cl;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t j = n0 + foff[t] + i;
+            double x[3] = {0.0, 0.0, 0.0};
+            for (int it = 0; it < 1000; ++it) {
+                double y[3];
+                for (int d = 0; d < 3; ++d) {
+                    const double u = uniform(r);
+                    y[d] = lo[d] + u * (hi[d] - lo[d]);
+                }
+                bool inside = true;
+                for (int f = 0; f < kMaxFaces; ++f) {
+                    const V4<R> q = w.geom.plane[(size_t)c * kMaxFaces + f];
+                    if ((double)q.x * y[0] + (double)q.y * y[1] + (double)q.z * y[2] > (double)q.w) { inside = false; break; }
+                }
+                if (inside) { x[0] = y[0]; x[1] = y[1]; x[2] = y[2]; break; }
+            }
+            double g0, g1, g2;
+            gauss3(r, g0, g1, g2);
+            w.a.x[j] = (R)(x[0] * cs.mask[0] + 0.0);
+            w.a.y[j] = (R)(x[1] * cs.mask[1] + 0.0);
+            w.a.z[j] = (R)(x[2] * cs.mask[2] + 0.0);
+            w.a.vx[j] = (R)(sig[0] * g0 + Uf[0]);
+            w.a.vy[j] = (R)(sig[1] * g1 + Uf[1]);
+            w.a.vz[j] = (R)(sig[2] * g2 + Uf[2]);
+            w.a.erot[j] = (R)equip_erot(r, rotT, w.sp.rotdof[g]);
+            w.a.cell[j] = c;
+            w.a.sp[j] = g;
+        }
+    }
+}
+
+template <class R>
+__global__ void k_fill_finish(WorldView<R> w, const uint32_t* foff, uint32_t total)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!w.st->overflow) w.st->n = w.st->n + foff[total];
+        w.st->n_ins = 0u;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline unsigned grid_for(uint64_t n, unsigned cap_blocks = 8192)
+{
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > cap_blocks) g = cap_blocks;
+    return (unsigned)g;
+}
+
+template <class R>
+static void launch_sort_tail(WorldView<R> w, hipStream_t s)
+{
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
+    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    const unsigned tiles = (unsigned)((w.cap + kTileH - 1) / kTileH) + 1u;
+    hipLaunchKernelGGL(k_segsort<R>, dim3(tiles), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+}
+
+template <class R>
+void launch_sort(const WorldView<R>& w, hipStream_t s)
+{
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cursor, ncp);
+    hipLaunchKernelGGL(k_hist<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    launch_sort_tail(w, s);
+    hipLaunchKernelGGL(k_finish_sort<R>, dim3(1), dim3(1), 0, s, w);
+}
+
+template <class R>
+void launch_step(const WorldView<R>& w0, hipStream_t s, int)
+{
+    WorldView<R> w = w0;
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    const unsigned gc = grid_for((uint64_t)w.ncases);
+    hipLaunchKernelGGL(k_begin_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cursor, ncp);
+    if (w.n_inflow > 0) {
+        hipLaunchKernelGGL(k_inflow_count<R>, dim3(1), dim3(1024), 0, s, w);
+        hipLaunchKernelGGL(k_inflow_gen<R>, dim3(grid_for(w.max_ins)), dim3(kBlock), 0, s, w);
+    }
+    hipLaunchKernelGGL(k_move<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    launch_sort_tail(w, s);
+    // from here on the sorted records are in w.b
+    const unsigned gcell = grid_for((uint64_t)w.nc);
+    hipLaunchKernelGGL(k_sample<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_bgk<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_ntc<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_end_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
+}
+
+template <class R>
+void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase, const double* nd, const double* T, double rotT,
+                 const double* U, int exact, const double* bblo, const double* bbhi, uint32_t* fcnt,
+                 uint32_t* foff, int ns, int ncl)
+{
+    const uint32_t total = (uint32_t)ncl * (uint32_t)ns;
+    hipLaunchKernelGGL(k_fill_count<R>, dim3(grid_for(total + 1)), dim3(kBlock), 0, s, w, ccase, nd, exact, fcnt, ns, ncl);
+    scan_u32(fcnt, total + 1, foff, w.blocksum, nullptr, s);
+    hipLaunchKernelGGL(k_fill_gen<R>, dim3(grid_for(total)), dim3(kBlock), 0, s, w, ccase, nd, T, rotT, U, exact,
+                       bblo, bbhi, foff, ns, ncl);
+    hipLaunchKernelGGL(k_fill_finish<R>, dim3(1), dim3(1), 0, s, w, foff, total);
+}
+
+template void launch_step<float>(const WorldView<float>&, hipStream_t, int);
+template void launch_step<double>(const WorldView<double>&, hipStream_t, int);
+template void launch_sort<float>(const WorldView<float>&, hipStream_t);
+template void launch_sort<double>(const WorldView<double>&, hipStream_t);
+template void launch_fill<float>(const WorldView<float>&, hipStream_t, int, const double*, const double*, double,
+                                 const double*, int, const double*, const double*, uint32_t*, uint32_t*, int, int);
+template void launch_fill<double>(const WorldView<double>&, hipStream_t, int, const double*, const double*, double,
+                                  const double*, int, const double*, const double*, uint32_t*, uint32_t*, int, int);
+
+}  // namespace ugx

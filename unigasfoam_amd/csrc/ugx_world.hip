@@ -1,0 +1,815 @@
+// ugx_world.hip - host side of the C ABI (include/ugx.h): mesh geometry,
+// descriptors -> device arrays, per-step launch sequence, downloads and the
+// write-time field computation.
+//
+// A "world" holds one or more independent cases in shared device arrays:
+// global cell ids are the concatenation of the cases' cells, species and
+// boundary models are per-case blocks.  The plain solver (ugx_case_*) is a
+// world with one case; the batched and served forms put many cases in one
+// world and advance them with the same kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ugx_kernels.hpp"
+#include "ugx_world.hpp"
+
+namespace ugx {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(UGX_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------- geometry
+// Same operation order as the CPU oracle: face centre = vertex mean, area
+// vector = fan about the centre, cell reference point = mean of unique
+// vertices (first appearance over faces in id order), outward planes in the
+// cell frame, volume by the divergence theorem.
+
+static void cross3(const double a[3], const double b[3], double r[3])
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>& bnd, int ns,
+                   const double* mass, const ugx_model_desc& model, MeshGeom& g)
+{
+    const int nc = m->n_cells, nf = m->n_faces;
+    if (nc <= 0 || nf <= 0 || !m->points || !m->face_offsets || !m->face_points || !m->owner)
+        return set_err(UGX_ERR_ARG, "mesh descriptor incomplete");
+    if (m->n_internal_faces > 0 && !m->neighbour) return set_err(UGX_ERR_ARG, "mesh neighbour list missing");
+    g.nc = nc;
+    g.cc.assign((size_t)nc * 3, 0.0);
+    g.vol.assign(nc, 0.0);
+    g.bblo.assign((size_t)nc * 3, 0.0);
+    g.bbhi.assign((size_t)nc * 3, 0.0);
+    g.ncf.assign(nc, 0);
+    g.plane.assign((size_t)nc * kMaxFaces * 4, 0.0);
+    g.nbr.assign((size_t)nc * kMaxFaces, kNbrEmpty);
+    g.delta.assign((size_t)nc * kMaxFaces * 3, 0.0);
+    std::vector<int32_t> cf((size_t)nc * kMaxFaces);
+    std::vector<double> pc((size_t)nf * 3), A((size_t)nf * 3);
+    std::vector<int> fpatch(nf, -1);
+    for (int f = 0; f < nf; ++f) {
+        int b = m->face_offsets[f], e = m->face_offsets[f + 1], k = e - b;
+        if (k < 3) return set_err(UGX_ERR_MESH, "face %d has fewer than 3 vertices", f);
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int i = b; i < e; ++i)
+            for (int d = 0; d < 3; ++d) s[d] = s[d] + m->points[3 * m->face_points[i] + d];
+        for (int d = 0; d < 3; ++d) pc[3 * f + d] = s[d] / (double)k;
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < k; ++i) {
+            const double* p0 = &m->points[3 * m->face_points[b + i]];
+            const double* p1 = &m->points[3 * m->face_points[b + (i + 1) % k]];
+            double u[3], w[3], c[3];
+            for (int d = 0; d < 3; ++d) { u[d] = p0[d] - pc[3 * f + d]; w[d] = p1[d] - pc[3 * f + d]; }
+            cross3(u, w, c);
+            for (int d = 0; d < 3; ++d) a[d] = a[d] + c[d];
+        }
+        for (int d = 0; d < 3; ++d) A[3 * f + d] = 0.5 * a[d];
+    }
+    for (int p = 0; p < m->n_patches; ++p)
+        for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) {
+            if (f < 0 || f >= nf) return set_err(UGX_ERR_MESH, "patch %d face range out of bounds", p);
+            fpatch[f] = p;
+        }
+    for (int f = 0; f < nf; ++f) {
+        int o = m->owner[f];
+        if (o < 0 || o >= nc) return set_err(UGX_ERR_MESH, "face %d owner out of range", f);
+        if (g.ncf[o] >= kMaxFaces) return set_err(UGX_ERR_MESH, "cell %d has more than 6 faces", o);
+        cf[(size_t)o * kMaxFaces + g.ncf[o]++] = f;
+        if (f < m->n_internal_faces) {
+            int nb = m->neighbour[f];
+            if (nb < 0 || nb >= nc) return set_err(UGX_ERR_MESH, "face %d neighbour out of range", f);
+            if (g.ncf[nb] >= kMaxFaces) return set_err(UGX_ERR_MESH, "cell %d has more than 6 faces", nb);
+            cf[(size_t)nb * kMaxFaces + g.ncf[nb]++] = f;
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        int ids[64], nid = 0;
+        for (int j = 0; j < g.ncf[c]; ++j) {
+            int f = cf[(size_t)c * kMaxFaces + j];
+            for (int i = m->face_offsets[f]; i < m->face_offsets[f + 1]; ++i) {
+                int pid = m->face_points[i];
+                bool seen = false;
+                for (int q = 0; q < nid; ++q) if (ids[q] == pid) { seen = true; break; }
+                if (!seen && nid < 64) ids[nid++] = pid;
+            }
+        }
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < nid; ++q)
+            for (int d = 0; d < 3; ++d) s[d] = s[d] + m->points[3 * ids[q] + d];
+        for (int d = 0; d < 3; ++d) g.cc[3 * (size_t)c + d] = s[d] / (double)nid;
+        for (int d = 0; d < 3; ++d) { g.bblo[3 * (size_t)c + d] = 1.7976931348623157e308; g.bbhi[3 * (size_t)c + d] = -1.7976931348623157e308; }
+        for (int q = 0; q < nid; ++q)
+            for (int d = 0; d < 3; ++d) {
+                double r = m->points[3 * ids[q] + d] - g.cc[3 * (size_t)c + d];
+                if (r < g.bblo[3 * (size_t)c + d]) g.bblo[3 * (size_t)c + d] = r;
+                if (r > g.bbhi[3 * (size_t)c + d]) g.bbhi[3 * (size_t)c + d] = r;
+            }
+    }
+    // boundary models per patch
+    g.bm.assign(std::max(m->n_patches, 1), BModel{});
+    for (int p = 0; p < m->n_patches; ++p) {
+        int t = m->patch_type[p];
+        g.bm[p].kind = (t == UGX_PATCH_SYMMETRY) ? UGX_BND_SPECULAR : UGX_BND_DELETION;
+        bool found = false;
+        for (const auto& b : bnd) {
+            if (b.patch != p) continue;
+            if (b.kind == UGX_BND_SPECULAR || b.kind == UGX_BND_DIFFUSE || b.kind == UGX_BND_DELETION) {
+                g.bm[p].kind = b.kind;
+                g.bm[p].T = b.temperature;
+                for (int d = 0; d < 3; ++d) g.bm[p].U[d] = b.velocity[d];
+                found = true;
+            }
+        }
+        if (t == UGX_PATCH_WALL && !found) return set_err(UGX_ERR_ARG, "wall patch %d has no boundary model", p);
+    }
+    for (int d = 0; d < 3; ++d) g.mask[d] = 1.0;
+    for (int p = 0; p < m->n_patches; ++p) {
+        if (m->patch_type[p] != UGX_PATCH_EMPTY || m->patch_size[p] == 0) continue;
+        const double* a = &A[3 * (size_t)m->patch_start[p]];
+        int ax = 0;
+        for (int d = 1; d < 3; ++d) if (std::fabs(a[d]) > std::fabs(a[ax])) ax = d;
+        g.mask[ax] = 0.0;
+    }
+    for (int c = 0; c < nc; ++c) {
+        double vs = 0.0;
+        for (int j = 0; j < kMaxFaces; ++j) {
+            double* pl = &g.plane[((size_t)c * kMaxFaces + j) * 4];
+            if (j >= g.ncf[c]) { pl[0] = pl[1] = pl[2] = 0.0; pl[3] = 1.0; g.nbr[(size_t)c * kMaxFaces + j] = kNbrEmpty; continue; }
+            int f = cf[(size_t)c * kMaxFaces + j];
+            double sg = (m->owner[f] == c) ? 1.0 : -1.0;
+            double a[3] = {A[3 * (size_t)f], A[3 * (size_t)f + 1], A[3 * (size_t)f + 2]};
+            double r[3];
+            for (int d = 0; d < 3; ++d) r[d] = pc[3 * (size_t)f + d] - g.cc[3 * (size_t)c + d];
+            double dt = dot3(a, r);
+            vs = vs + sg * dt;
+            double mag = std::sqrt(dot3(a, a));
+            double nn[3];
+            for (int d = 0; d < 3; ++d) nn[d] = sg * (a[d] / mag);
+            pl[0] = nn[0]; pl[1] = nn[1]; pl[2] = nn[2];
+            pl[3] = dot3(nn, r);
+            double* dl = &g.delta[((size_t)c * kMaxFaces + j) * 3];
+            if (f < m->n_internal_faces) {
+                int o = (m->owner[f] == c) ? m->neighbour[f] : m->owner[f];
+                g.nbr[(size_t)c * kMaxFaces + j] = o;
+                for (int d = 0; d < 3; ++d) dl[d] = g.cc[3 * (size_t)c + d] - g.cc[3 * (size_t)o + d];
+            } else {
+                int p = fpatch[f];
+                if (p < 0) return set_err(UGX_ERR_MESH, "boundary face %d outside every patch", f);
+                int t = m->patch_type[p];
+                if (t == UGX_PATCH_EMPTY) {
+                    g.nbr[(size_t)c * kMaxFaces + j] = kNbrEmpty;
+                } else if (t == UGX_PATCH_CYCLIC) {
+                    int q = m->patch_neighbour ? m->patch_neighbour[p] : -1;
+                    if (q < 0 || q >= m->n_patches || m->patch_size[q] != m->patch_size[p])
+                        return set_err(UGX_ERR_MESH, "cyclic patch %d has no matching partner", p);
+                    int gf = m->patch_start[q] + (f - m->patch_start[p]);
+                    int o = m->owner[gf];
+                    g.nbr[(size_t)c * kMaxFaces + j] = o;
+                    for (int d = 0; d < 3; ++d)
+                        dl[d] = (g.cc[3 * (size_t)c + d] - g.cc[3 * (size_t)o + d]) + (pc[3 * (size_t)gf + d] - pc[3 * (size_t)f + d]);
+                } else {
+                    g.nbr[(size_t)c * kMaxFaces + j] = -1 - p;
+                }
+            }
+        }
+        g.vol[c] = vs / 3.0;
+        if (!(g.vol[c] > 0.0)) return set_err(UGX_ERR_MESH, "cell %d has non-positive volume", c);
+    }
+    // freestream inflow face-species (uniGasGeneralBoundary.C:115-169)
+    g.inflow.clear();
+    const double sqrtPi = std::sqrt(kPi);
+    for (size_t bi = 0; bi < bnd.size(); ++bi) {
+        const auto& b = bnd[bi];
+        if (b.kind != UGX_BND_FREESTREAM) continue;
+        int p = b.patch;
+        if (p < 0 || p >= m->n_patches) return set_err(UGX_ERR_ARG, "inflow patch %d out of range", p);
+        for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) {
+            for (int s = 0; s < ns; ++s) {
+                double nd = b.number_density[s];
+                if (!(nd > 0.0)) continue;
+                InflowFS e{};
+                int c = m->owner[f];
+                e.cell = c; e.species = s; e.ccase = 0; e.local_id = (int)g.inflow.size();
+                e.T = b.temperature; e.rotT = b.rot_temperature;
+                for (int d = 0; d < 3; ++d) e.U[d] = b.velocity[d];
+                const double* sF = &A[3 * (size_t)f];
+                double fA = std::sqrt(dot3(sF, sF));
+                double mps = std::sqrt(2.0 * kB * e.T / mass[s]);
+                double nrm[3] = {-sF[0] / fA, -sF[1] / fA, -sF[2] / fA};
+                double sCos = dot3(e.U, nrm) / mps;
+                e.nbar = (fA * nd * model.delta_t * mps
+                          * (std::exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + std::erf(sCos))))
+                         / (2.0 * sqrtPi * model.n_equivalent_particles);
+                int fb = m->face_offsets[f], fe = m->face_offsets[f + 1];
+                e.nv = fe - fb;
+                if (e.nv > 8) return set_err(UGX_ERR_MESH, "inflow face with more than 8 vertices");
+                for (int i = 0; i < e.nv; ++i)
+                    for (int d = 0; d < 3; ++d)
+                        e.vloc[i][d] = m->points[3 * m->face_points[fb + i] + d] - g.cc[3 * (size_t)c + d];
+                double tot = 0.0, ar[6];
+                for (int t = 0; t < e.nv - 2; ++t) {
+                    double u[3], ww[3], cr[3];
+                    for (int d = 0; d < 3; ++d) { u[d] = e.vloc[t + 1][d] - e.vloc[0][d]; ww[d] = e.vloc[t + 2][d] - e.vloc[0][d]; }
+                    cross3(u, ww, cr);
+                    ar[t] = 0.5 * std::sqrt(dot3(cr, cr));
+                    tot = tot + ar[t];
+                }
+                double run = 0.0;
+                for (int t = 0; t < e.nv - 2; ++t) { run = run + ar[t]; e.cum[t] = run / tot; }
+                e.cum[e.nv - 3] = 1.0;
+                for (int d = 0; d < 3; ++d) e.n[d] = nrm[d];
+                double t1[3];
+                for (int d = 0; d < 3; ++d) t1[d] = pc[3 * (size_t)f + d] - m->points[3 * m->face_points[fb] + d];
+                double tm = std::sqrt(dot3(t1, t1));
+                for (int d = 0; d < 3; ++d) e.t1[d] = t1[d] / tm;
+                double t2[3];
+                cross3(e.n, e.t1, t2);
+                double t2m = std::sqrt(dot3(t2, t2));
+                for (int d = 0; d < 3; ++d) e.t2[d] = t2[d] / t2m;
+                g.inflow.push_back(e);
+            }
+        }
+    }
+    return UGX_OK;
+}
+
+// --------------------------------------------------------- device buffers
+
+template <class T>
+static int dalloc(T*& p, size_t n)
+{
+    p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e != hipSuccess) return set_err(UGX_ERR_DEVICE, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return UGX_OK;
+}
+
+template <class T>
+static void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+World::~World() { release(); }
+
+void World::release()
+{
+    for (int k = 0; k < 2; ++k)
+        for (int a = 0; a < 9; ++a) dfree(pbuf[k][a]);
+    void** ptrs[] = {(void**)&d_plane, (void**)&d_delta, (void**)&d_nbr, (void**)&d_bm, (void**)&d_inflow,
+                     (void**)&d_cases, (void**)&d_cell_case, (void**)&d_vol, (void**)&d_smax, (void**)&d_cmodel,
+                     (void**)&d_samp, (void**)&d_uspq, (void**)&d_usps, (void**)&d_uspmax, (void**)&d_knacc,
+                     (void**)&d_kn, (void**)&d_cc, (void**)&d_mass, (void**)&d_rotdof, (void**)&d_omega,
+                     (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
+                     (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
+                     (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
+                     (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st};
+    for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
+    if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
+}
+
+size_t World::rsize() const { return precision == UGX_FP32_STATE ? sizeof(float) : sizeof(double); }
+
+int World::alloc_particles(uint32_t newcap)
+{
+    void* nb[2][9] = {};
+    size_t rs = rsize();
+    for (int k = 0; k < 2; ++k)
+        for (int a = 0; a < 9; ++a) {
+            size_t es = (a < 7) ? rs : sizeof(uint32_t);
+            hipError_t e = hipMalloc(&nb[k][a], (size_t)newcap * es);
+            if (e != hipSuccess) {
+                for (int kk = 0; kk < 2; ++kk) for (int aa = 0; aa < 9; ++aa) if (nb[kk][aa]) (void)hipFree(nb[kk][aa]);
+                return set_err(UGX_ERR_DEVICE, "particle allocation (%u simulators) failed: %s", newcap, hipGetErrorString(e));
+            }
+        }
+    uint32_t n = 0;
+    if (cap > 0) {
+        HIP_TRY(hipMemcpyAsync(&n, &d_st->n, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (int a = 0; a < 9; ++a) {
+            size_t es = (a < 7) ? rs : sizeof(uint32_t);
+            if (n) HIP_TRY(hipMemcpyAsync(nb[cur][a], pbuf[cur][a], (size_t)n * es, hipMemcpyDeviceToDevice, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+    for (int k = 0; k < 2; ++k)
+        for (int a = 0; a < 9; ++a) { if (pbuf[k][a]) (void)hipFree(pbuf[k][a]); pbuf[k][a] = nullptr; }
+    for (int a = 0; a < 9; ++a) { pbuf[0][a] = nb[cur][a]; pbuf[1][a] = nb[1 - cur][a]; }
+    cur = 0;
+    dfree(d_perm);
+    dfree(d_scratch);
+    if (int rc = dalloc(d_perm, newcap)) return rc;
+    if (int rc = dalloc(d_scratch, newcap)) return rc;
+    cap = newcap;
+    return UGX_OK;
+}
+
+int World::ensure_capacity(uint64_t need)
+{
+    if (need <= cap) return UGX_OK;
+    if (need > 0xF0000000ull) return set_err(UGX_ERR_CAPACITY, "more than 4e9 simulators requested");
+    uint64_t nc = std::max<uint64_t>(need + need / 4 + 1024, 4096);
+    if (nc > 0xF0000000ull) nc = 0xF0000000ull;
+    return alloc_particles((uint32_t)nc);
+}
+
+template <class R>
+WorldView<R> World::view() const
+{
+    WorldView<R> w{};
+    auto mk = [&](int k) {
+        Parts<R> p;
+        p.x = (R*)pbuf[k][0]; p.y = (R*)pbuf[k][1]; p.z = (R*)pbuf[k][2];
+        p.vx = (R*)pbuf[k][3]; p.vy = (R*)pbuf[k][4]; p.vz = (R*)pbuf[k][5];
+        p.erot = (R*)pbuf[k][6];
+        p.cell = (uint32_t*)pbuf[k][7]; p.sp = (uint32_t*)pbuf[k][8];
+        return p;
+    };
+    w.a = mk(cur);
+    w.b = mk(1 - cur);
+    w.geom.plane = (const V4<R>*)d_plane;
+    w.geom.delta = (const V4<R>*)d_delta;
+    w.geom.nbr = d_nbr;
+    w.bm = d_bm;
+    w.inflow = d_inflow;
+    w.n_inflow = n_inflow;
+    w.max_ins = max_ins;
+    w.cases = d_cases;
+    w.ncases = (int32_t)cases.size();
+    w.cell_case = d_cell_case;
+    w.vol = d_vol;
+    w.smax = d_smax;
+    w.cmodel = d_cmodel;
+    w.samp = d_samp;
+    w.uspq = d_uspq;
+    w.usps = d_usps;
+    w.uspmax = d_uspmax;
+    w.knacc = d_knacc;
+    w.kn = d_kn;
+    w.cc = d_cc;
+    w.sp = Species{d_mass, d_rotdof, d_omega, d_muref, d_prterm, d_local};
+    w.pr = Pairs{d_pmr, d_pC, d_pe, d_pchi, d_palpha};
+    w.cnt = d_cnt;
+    w.off = d_off;
+    w.cursor = d_cursor;
+    w.perm = d_perm;
+    w.ins_off = d_ins_off;
+    w.scratch = d_scratch;
+    w.blocksum = d_blocksum;
+    w.counters = d_counters;
+    w.st = d_st;
+    w.nc = nc;
+    w.cap = cap;
+    return w;
+}
+
+template WorldView<float> World::view<float>() const;
+template WorldView<double> World::view<double>() const;
+
+// Builds the device arrays for the cases in `specs` (all cells concatenated).
+int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t capacity)
+{
+    precision = prec;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreate(&stream));  // blocking: ordered with null-stream copies
+    nc = 0;
+    int nsp = 0, npair = 0, nbm = 0;
+    for (auto& s : specs) { nc += s.geom.nc; nsp += s.ns; npair += s.ns * s.ns; nbm += (int)s.geom.bm.size(); }
+    std::vector<CaseDev> hc;
+    std::vector<int32_t> cell_case(nc), nbr((size_t)nc * kMaxFaces), cmodel(nc), local(nsp), rotdof(nsp);
+    std::vector<double> vol(nc), cc((size_t)nc * 3), plane((size_t)nc * kMaxFaces * 4), delta((size_t)nc * kMaxFaces * 4, 0.0);
+    std::vector<double> mass(nsp), omega(nsp), muref(nsp), prterm(nsp), pmr(npair), pC(npair), pe(npair), pchi(npair), palpha(npair);
+    std::vector<BModel> bm;
+    std::vector<InflowFS> inflow;
+    int cbase = 0, sbase = 0, pbase = 0;
+    max_ins = 0;
+    for (size_t ci = 0; ci < specs.size(); ++ci) {
+        CaseSpec& s = specs[ci];
+        const MeshGeom& g = s.geom;
+        CaseDev d{};
+        d.dt = s.model.delta_t;
+        d.fn = s.model.n_equivalent_particles;
+        d.inv_zrot = (s.model.Zrot > 0.0) ? 1.0 / s.model.Zrot : 0.0;
+        d.theta = s.model.theta;
+        d.tref = s.model.Tref;
+        for (int k = 0; k < 3; ++k) d.mask[k] = g.mask[k];
+        d.seed_lo = (uint32_t)s.seed;
+        d.seed_hi = (uint32_t)(s.seed >> 32);
+        d.step = 0;
+        d.sample_interval = s.model.sample_interval > 0 ? s.model.sample_interval : 1;
+        d.coll_model = s.model.collision_model;
+        d.dsmc_model = s.model.dsmc_model;
+        d.bgk_model = s.model.bgk_model;
+        d.ns = s.ns;
+        d.sp_base = sbase;
+        d.pair_base = pbase;
+        d.bm_base = (int32_t)bm.size();
+        d.cell_begin = cbase;
+        d.cell_end = cbase + g.nc;
+        d.inflow_begin = (int32_t)inflow.size();
+        d.active = 1;
+        d.decomposition_interval = s.model.decomposition_interval > 0 ? s.model.decomposition_interval : 100;
+        d.smoothing_passes = s.model.smoothing_passes;
+        d.breakdown_max = s.model.breakdown_max;
+        d.kn_theta = s.model.kn_theta;
+        int cm = (s.model.collision_model == UGX_COLL_DSMC) ? 1 : 0;  // hybrid starts all-BGK (uniGasCloud.C:726-732)
+        for (int c = 0; c < g.nc; ++c) {
+            int gc = cbase + c;
+            cell_case[gc] = (int32_t)ci;
+            vol[gc] = g.vol[c];
+            cmodel[gc] = cm;
+            for (int d3 = 0; d3 < 3; ++d3) cc[3 * (size_t)gc + d3] = g.cc[3 * (size_t)c + d3];
+            for (int f = 0; f < kMaxFaces; ++f) {
+                size_t lk = (size_t)c * kMaxFaces + f, gk = (size_t)gc * kMaxFaces + f;
+                for (int q = 0; q < 4; ++q) plane[gk * 4 + q] = g.plane[lk * 4 + q];
+                for (int q = 0; q < 3; ++q) delta[gk * 4 + q] = g.delta[lk * 3 + q];
+                int32_t code = g.nbr[lk];
+                if (code >= 0) code += cbase;
+                else if (code != kNbrEmpty) code = -1 - (d.bm_base + (-1 - code));
+                nbr[gk] = code;
+            }
+        }
+        for (auto b : g.bm) bm.push_back(b);
+        for (auto e : g.inflow) {
+            e.cell += cbase;
+            e.species += sbase;
+            e.ccase = (int32_t)ci;
+            inflow.push_back(e);
+            max_ins += (uint32_t)std::floor(e.nbar) + 1u;
+        }
+        d.inflow_end = (int32_t)inflow.size();
+        for (int k = 0; k < s.ns; ++k) {
+            const auto& sd = s.species[k];
+            int gk = sbase + k;
+            mass[gk] = sd.mass;
+            omega[gk] = sd.omega;
+            rotdof[gk] = sd.rotational_dof;
+            local[gk] = k;
+            double a = sd.alpha, om = sd.omega, dd = sd.diameter, m = sd.mass;
+            muref[gk] = 1.25 * (1.0 + a) * (2.0 + a) * std::sqrt(m * kB * s.model.Tref)
+                        / (a * (5.0 - 2.0 * om) * (7.0 - 2.0 * om) * std::sqrt(kPi) * (dd * dd));
+            double z = (double)sd.rotational_dof;
+            prterm[gk] = (5.0 + z) / (7.5 + z);
+        }
+        for (int p = 0; p < s.ns; ++p)
+            for (int q = 0; q < s.ns; ++q) {
+                int k = pbase + p * s.ns + q;
+                double mP = s.species[p].mass, mQ = s.species[q].mass;
+                double mR = mP * mQ / (mP + mQ);
+                double dPQ = 0.5 * (s.species[p].diameter + s.species[q].diameter);
+                double om = 0.5 * (s.species[p].omega + s.species[q].omega);
+                pmr[k] = mR;
+                pC[k] = kPi * dPQ * dPQ * std::pow(2.0 * kB * s.model.Tref / mR, om - 0.5) / std::exp(std::lgamma(2.5 - om));
+                pe[k] = 1.0 - om;
+                pchi[k] = 2.5 - om;
+                palpha[k] = 0.5 * (s.species[p].alpha + s.species[q].alpha);
+            }
+        hc.push_back(d);
+        s.cell_begin = cbase;
+        cbase += g.nc;
+        sbase += s.ns;
+        pbase += s.ns * s.ns;
+    }
+    n_inflow = (int32_t)inflow.size();
+    cases = hc;
+    // precision-dependent geometry
+    int rc;
+    size_t rs = rsize();
+    HIP_TRY(hipMalloc(&d_plane, (size_t)nc * kMaxFaces * 4 * rs));
+    HIP_TRY(hipMalloc(&d_delta, (size_t)nc * kMaxFaces * 4 * rs));
+    if (precision == UGX_FP32_STATE) {
+        std::vector<float> pf(plane.size()), df(delta.size());
+        for (size_t i = 0; i < plane.size(); ++i) pf[i] = (float)plane[i];
+        for (size_t i = 0; i < delta.size(); ++i) df[i] = (float)delta[i];
+        HIP_TRY(hipMemcpy(d_plane, pf.data(), pf.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_delta, df.data(), df.size() * 4, hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy(d_plane, plane.data(), plane.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_delta, delta.data(), delta.size() * 8, hipMemcpyHostToDevice));
+    }
+    auto up = [&](auto*& dst, const auto& v) -> int {
+        using T = std::remove_reference_t<decltype(*dst)>;
+        if (int r = dalloc(dst, v.size())) return r;
+        if (!v.empty()) {
+            hipError_t e = hipMemcpy((void*)dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return set_err(UGX_ERR_DEVICE, "upload failed: %s", hipGetErrorString(e));
+        }
+        return UGX_OK;
+    };
+    if ((rc = up(d_nbr, nbr))) return rc;
+    if ((rc = up(d_bm, bm))) return rc;
+    if ((rc = up(d_inflow, inflow))) return rc;
+    if ((rc = up(d_cases, hc))) return rc;
+    if ((rc = up(d_cell_case, cell_case))) return rc;
+    if ((rc = up(d_vol, vol))) return rc;
+    if ((rc = up(d_cc, cc))) return rc;
+    if ((rc = up(d_cmodel, cmodel))) return rc;
+    if ((rc = up(d_mass, mass))) return rc;
+    if ((rc = up(d_rotdof, rotdof))) return rc;
+    if ((rc = up(d_omega, omega))) return rc;
+    if ((rc = up(d_muref, muref))) return rc;
+    if ((rc = up(d_prterm, prterm))) return rc;
+    if ((rc = up(d_local, local))) return rc;
+    if ((rc = up(d_pmr, pmr))) return rc;
+    if ((rc = up(d_pC, pC))) return rc;
+    if ((rc = up(d_pe, pe))) return rc;
+    if ((rc = up(d_pchi, pchi))) return rc;
+    if ((rc = up(d_palpha, palpha))) return rc;
+    std::vector<double> zeros((size_t)nc, 0.0), ones((size_t)nc, 1.0);
+    if ((rc = up(d_smax, zeros))) return rc;
+    if ((rc = up(d_uspmax, ones))) return rc;
+    if ((rc = up(d_kn, zeros))) return rc;
+    std::vector<double> z3((size_t)nc * 3, 0.0), z6((size_t)nc * 6, 0.0), z8((size_t)nc * 8, 0.0), zs((size_t)nc * kNSamp, 0.0);
+    if ((rc = up(d_uspq, z3))) return rc;
+    if ((rc = up(d_usps, z6))) return rc;
+    if ((rc = up(d_knacc, z8))) return rc;
+    if ((rc = up(d_samp, zs))) return rc;
+    std::vector<uint32_t> zc((size_t)nc + 1, 0u);
+    if ((rc = up(d_cnt, zc))) return rc;
+    if ((rc = up(d_off, zc))) return rc;
+    if ((rc = up(d_cursor, zc))) return rc;
+    std::vector<uint32_t> zi((size_t)n_inflow + 1, 0u);
+    if ((rc = up(d_ins_off, zi))) return rc;
+    size_t nbs = ((size_t)nc * kNS + 2 + 1023) / 1024 + 2;
+    if ((rc = dalloc(d_blocksum, nbs))) return rc;
+    std::vector<unsigned long long> zcnt(specs.size() * kCntNum, 0ull);
+    if ((rc = up(d_counters, zcnt))) return rc;
+    DevState st{};
+    std::vector<DevState> vst(1, st);
+    if ((rc = up(d_st, vst))) return rc;
+    specs_host = specs;
+    samp_reset_steps.assign(specs.size(), 0);
+    cap = 0;
+    uint64_t want = capacity > 0 ? capacity : 4096;
+    if ((rc = alloc_particles((uint32_t)std::min<uint64_t>(want, 0xF0000000ull)))) return rc;
+    return UGX_OK;
+}
+
+int World::check_state()
+{
+    DevState st{};
+    HIP_TRY(hipMemcpyAsync(&st, d_st, sizeof st, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (st.overflow) return set_err(UGX_ERR_CAPACITY, "particle capacity (%u) exceeded on the device", cap);
+    return UGX_OK;
+}
+
+int World::count(uint32_t& n)
+{
+    HIP_TRY(hipMemcpyAsync(&n, &d_st->n, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return UGX_OK;
+}
+
+int World::sort_now()
+{
+    if (precision == UGX_FP32_STATE) launch_sort<float>(view<float>(), stream);
+    else launch_sort<double>(view<double>(), stream);
+    HIP_TRY(hipGetLastError());
+    cur = 1 - cur;
+    return UGX_OK;
+}
+
+int World::step(int n)
+{
+    for (int i = 0; i < n; ++i) {
+        if (precision == UGX_FP32_STATE) launch_step<float>(view<float>(), stream, 0);
+        else launch_step<double>(view<double>(), stream, 0);
+        cur = 1 - cur;
+        for (auto& c : cases) c.step += 1;
+    }
+    HIP_TRY(hipGetLastError());
+    return UGX_OK;
+}
+
+// Appends host particles of case `ci` (global positions or local) and sorts.
+int World::upload(int ci, const ugx_particle_soa* p, const MeshGeom& g)
+{
+    uint32_t n0 = 0;
+    if (int rc = count(n0)) return rc;
+    if (int rc = ensure_capacity((uint64_t)n0 + (uint64_t)p->n)) return rc;
+    const size_t n = (size_t)p->n;
+    const CaseDev& cd = cases[ci];
+    std::vector<double> xyz[7];
+    for (auto& v : xyz) v.resize(n);
+    std::vector<uint32_t> cell(n), sp(n);
+    for (size_t i = 0; i < n; ++i) {
+        int c = p->cell[i];
+        if (c < 0 || c >= g.nc) return set_err(UGX_ERR_ARG, "particle %zu: cell %d out of range", i, c);
+        int s = p->species ? p->species[i] : 0;
+        if (s < 0 || s >= cd.ns) return set_err(UGX_ERR_ARG, "particle %zu: species %d out of range", i, s);
+        for (int d = 0; d < 3; ++d) {
+            xyz[d][i] = p->local_position ? p->local_position[3 * i + d] : p->position[3 * i + d] - g.cc[3 * (size_t)c + d];
+            xyz[3 + d][i] = p->velocity[3 * i + d];
+        }
+        xyz[6][i] = p->erot ? p->erot[i] : 0.0;
+        cell[i] = (uint32_t)(cd.cell_begin + c);
+        sp[i] = (uint32_t)(cd.sp_base + s);
+    }
+    size_t rs = rsize();
+    for (int a = 0; a < 7; ++a) {
+        if (precision == UGX_FP32_STATE) {
+            std::vector<float> f(n);
+            for (size_t i = 0; i < n; ++i) f[i] = (float)xyz[a][i];
+            HIP_TRY(hipMemcpy((char*)pbuf[cur][a] + (size_t)n0 * rs, f.data(), n * rs, hipMemcpyHostToDevice));
+        } else {
+            HIP_TRY(hipMemcpy((char*)pbuf[cur][a] + (size_t)n0 * rs, xyz[a].data(), n * rs, hipMemcpyHostToDevice));
+        }
+    }
+    HIP_TRY(hipMemcpy((char*)pbuf[cur][7] + (size_t)n0 * 4, cell.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((char*)pbuf[cur][8] + (size_t)n0 * 4, sp.data(), n * 4, hipMemcpyHostToDevice));
+    uint32_t ntot = n0 + (uint32_t)n;
+    HIP_TRY(hipMemcpy(&d_st->n, &ntot, sizeof(uint32_t), hipMemcpyHostToDevice));
+    uint32_t zero = 0;
+    HIP_TRY(hipMemcpy(&d_st->n_ins, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (int rc = sort_now()) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    return check_state();
+}
+
+int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g)
+{
+    const CaseDev& cd = cases[ci];
+    uint32_t n0 = 0;
+    if (int rc = count(n0)) return rc;
+    // capacity from the expected count
+    double ntot = 0.0, expected = 0.0;
+    for (int s = 0; s < cd.ns; ++s) ntot += f->number_density[s];
+    if (f->exact_per_cell > 0) expected = (double)f->exact_per_cell * g.nc + (double)g.nc * cd.ns;
+    else {
+        double V = 0.0;
+        for (double v : g.vol) V += v;
+        expected = ntot / cd.fn * V * 1.05 + (double)g.nc * cd.ns + 64.0;
+    }
+    if (int rc = ensure_capacity((uint64_t)n0 + (uint64_t)expected)) return rc;
+    int ns = cd.ns, ncl = g.nc;
+    double* d_nd = nullptr; double* d_T = nullptr; double* d_U = nullptr;
+    double* d_lo = nullptr; double* d_hi = nullptr;
+    uint32_t* d_fc = nullptr; uint32_t* d_fo = nullptr;
+    int rc = UGX_OK;
+    if ((rc = dalloc(d_nd, kNS)) || (rc = dalloc(d_T, 3)) || (rc = dalloc(d_U, 3)) || (rc = dalloc(d_lo, (size_t)ncl * 3))
+        || (rc = dalloc(d_hi, (size_t)ncl * 3)) || (rc = dalloc(d_fc, (size_t)ncl * ns + 1)) || (rc = dalloc(d_fo, (size_t)ncl * ns + 1))) {
+        dfree(d_nd); dfree(d_T); dfree(d_U); dfree(d_lo); dfree(d_hi); dfree(d_fc); dfree(d_fo);
+        return rc;
+    }
+    auto cleanup = [&]() { dfree(d_nd); dfree(d_T); dfree(d_U); dfree(d_lo); dfree(d_hi); dfree(d_fc); dfree(d_fo); };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMemcpy(d_nd, f->number_density, sizeof(double) * kNS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_T, f->T, sizeof(double) * 3, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_U, f->velocity, sizeof(double) * 3, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_lo, g.bblo.data(), sizeof(double) * 3 * ncl, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_hi, g.bbhi.data(), sizeof(double) * 3 * ncl, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { cleanup(); return set_err(UGX_ERR_DEVICE, "fill upload failed: %s", hipGetErrorString(e)); }
+    if (precision == UGX_FP32_STATE)
+        launch_fill<float>(view<float>(), stream, ci, d_nd, d_T, f->rot_temperature, d_U, f->exact_per_cell, d_lo, d_hi, d_fc, d_fo, ns, ncl);
+    else
+        launch_fill<double>(view<double>(), stream, ci, d_nd, d_T, f->rot_temperature, d_U, f->exact_per_cell, d_lo, d_hi, d_fc, d_fo, ns, ncl);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    cleanup();
+    if (e != hipSuccess) return set_err(UGX_ERR_DEVICE, "fill failed: %s", hipGetErrorString(e));
+    if (int r2 = check_state()) return r2;
+    if (int r2 = sort_now()) return r2;
+    HIP_TRY(hipStreamSynchronize(stream));
+    // (sigma_T c_r)max = pi d^2 c_mp(T) of the most abundant species (uniGasMeshFill.C:284-296)
+    int best = 0;
+    for (int s = 0; s < ns; ++s) if (f->number_density[s] > f->number_density[best]) best = s;
+    const ugx_species_desc& sb = specs_host[ci].species[best];
+    double Tm = (f->T[0] + f->T[1] + f->T[2]) / 3.0;
+    double s0 = kPi * sb.diameter * sb.diameter * std::sqrt(2.0 * kB * Tm / sb.mass);
+    std::vector<double> sm((size_t)ncl, s0);
+    HIP_TRY(hipMemcpy(d_smax + cd.cell_begin, sm.data(), sizeof(double) * ncl, hipMemcpyHostToDevice));
+    uint32_t n1 = 0;
+    if (int r2 = count(n1)) return r2;
+    return ensure_capacity((uint64_t)n1 * 2 + (uint64_t)max_ins * 8);
+}
+
+int World::download(int ci, ugx_particle_soa* p, const MeshGeom& g)
+{
+    const CaseDev& cd = cases[ci];
+    uint32_t n = 0;
+    if (int rc = count(n)) return rc;
+    std::vector<uint32_t> off((size_t)nc + 1);
+    HIP_TRY(hipMemcpy(off.data(), d_off, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
+    uint32_t b = off[cd.cell_begin], e = off[cd.cell_end];
+    size_t m = e - b;
+    if ((size_t)p->n < m) { p->n = (int64_t)m; return set_err(UGX_ERR_CAPACITY, "download arrays hold %lld, need %zu", (long long)p->n, m); }
+    size_t rs = rsize();
+    std::vector<double> arr[7];
+    for (int a = 0; a < 7; ++a) {
+        arr[a].resize(m);
+        if (precision == UGX_FP32_STATE) {
+            std::vector<float> f(m);
+            if (m) HIP_TRY(hipMemcpy(f.data(), (char*)pbuf[cur][a] + (size_t)b * rs, m * rs, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < m; ++i) arr[a][i] = (double)f[i];
+        } else if (m) {
+            HIP_TRY(hipMemcpy(arr[a].data(), (char*)pbuf[cur][a] + (size_t)b * rs, m * rs, hipMemcpyDeviceToHost));
+        }
+    }
+    std::vector<uint32_t> cell(m), sp(m);
+    if (m) {
+        HIP_TRY(hipMemcpy(cell.data(), (char*)pbuf[cur][7] + (size_t)b * 4, m * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(sp.data(), (char*)pbuf[cur][8] + (size_t)b * 4, m * 4, hipMemcpyDeviceToHost));
+    }
+    for (size_t i = 0; i < m; ++i) {
+        int c = (int)cell[i] - cd.cell_begin;
+        for (int d = 0; d < 3; ++d) {
+            if (p->local_position) p->local_position[3 * i + d] = arr[d][i];
+            if (p->position) p->position[3 * i + d] = arr[d][i] + g.cc[3 * (size_t)c + d];
+            if (p->velocity) p->velocity[3 * i + d] = arr[3 + d][i];
+        }
+        if (p->erot) p->erot[i] = arr[6][i];
+        if (p->cell) p->cell[i] = c;
+        if (p->species) p->species[i] = (int)sp[i] - cd.sp_base;
+    }
+    p->n = (int64_t)m;
+    return UGX_OK;
+}
+
+int World::case_counters(int ci, ugx_counters* out)
+{
+    std::vector<unsigned long long> cnt(kCntNum);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), d_counters + (size_t)ci * kCntNum, sizeof(unsigned long long) * kCntNum,
+                           hipMemcpyDeviceToHost, stream));
+    std::vector<uint32_t> off(2);
+    HIP_TRY(hipMemcpyAsync(&off[0], d_off + cases[ci].cell_begin, 4, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(&off[1], d_off + cases[ci].cell_end, 4, hipMemcpyDeviceToHost, stream));
+    std::vector<int32_t> cm((size_t)(cases[ci].cell_end - cases[ci].cell_begin));
+    HIP_TRY(hipMemcpyAsync(cm.data(), d_cmodel + cases[ci].cell_begin, cm.size() * 4, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::memset(out, 0, sizeof *out);
+    out->n_particles = off[1] - off[0];
+    out->steps = cases[ci].step;
+    out->candidates = (int64_t)cnt[kCntCandidates];
+    out->collisions = (int64_t)cnt[kCntCollisions];
+    out->bgk_collisions = (int64_t)cnt[kCntBgk];
+    out->inserted = (int64_t)cnt[kCntInserted];
+    out->deleted = (int64_t)cnt[kCntDeleted];
+    out->wall_hits = (int64_t)cnt[kCntWallHits];
+    out->track_errors = (int64_t)cnt[kCntTrackErrors];
+    out->n_sampled_steps = (int64_t)cnt[kCntSampled] - samp_reset_steps[ci];
+    out->time_sampled = (double)out->n_sampled_steps * cases[ci].dt;
+    int64_t k = 0;
+    for (int32_t v : cm) k += v;
+    out->n_dsmc_cells = k;
+    return UGX_OK;
+}
+
+int World::case_samples(int ci, double* out, int64_t len)
+{
+    int ncl = cases[ci].cell_end - cases[ci].cell_begin;
+    if (len < (int64_t)kNSamp * ncl) return set_err(UGX_ERR_ARG, "sample buffer holds %lld, need %d", (long long)len, kNSamp * ncl);
+    for (int k = 0; k < kNSamp; ++k)
+        HIP_TRY(hipMemcpyAsync(out + (size_t)k * ncl, d_samp + (size_t)k * nc + cases[ci].cell_begin,
+                               sizeof(double) * ncl, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return UGX_OK;
+}
+
+int World::case_samples_reset(int ci)
+{
+    HIP_TRY(hipStreamSynchronize(stream));
+    int ncl = cases[ci].cell_end - cases[ci].cell_begin;
+    std::vector<double> z((size_t)ncl, 0.0);
+    for (int k = 0; k < kNSamp; ++k)
+        HIP_TRY(hipMemcpy(d_samp + (size_t)k * nc + cases[ci].cell_begin, z.data(), sizeof(double) * ncl, hipMemcpyHostToDevice));
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpy(&v, d_counters + (size_t)ci * kCntNum + kCntSampled, sizeof v, hipMemcpyDeviceToHost));
+    samp_reset_steps[ci] = (int64_t)v;
+    return UGX_OK;
+}
+
+}  // namespace ugx

@@ -1,0 +1,116 @@
+// ugx_world.hpp - host-side world: device arrays of one or more cases.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "ugx_kernels.hpp"
+
+namespace ugx {
+
+extern thread_local std::string g_err;
+int set_err(int code, const char* fmt, ...);
+
+// Geometry of one case's mesh in the storage form of the kernels.
+struct MeshGeom {
+    int nc = 0;
+    std::vector<double> cc, vol, bblo, bbhi, plane, delta;
+    std::vector<int32_t> ncf, nbr;
+    double mask[3] = {1.0, 1.0, 1.0};
+    std::vector<BModel> bm;        // per patch
+    std::vector<InflowFS> inflow;  // face-species entries (case-local)
+};
+
+int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>& bnd, int ns,
+                   const double* mass, const ugx_model_desc& model, MeshGeom& g);
+
+struct CaseSpec {
+    MeshGeom geom;
+    std::vector<ugx_species_desc> species;
+    int ns = 0;
+    ugx_model_desc model{};
+    uint64_t seed = 0;
+    int cell_begin = 0;
+};
+
+struct World {
+    int precision = UGX_FP64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t nc = 0;
+    uint32_t cap = 0;
+    int cur = 0;
+    void* pbuf[2][9] = {};
+    void* d_plane = nullptr;
+    void* d_delta = nullptr;
+    int32_t* d_nbr = nullptr;
+    BModel* d_bm = nullptr;
+    InflowFS* d_inflow = nullptr;
+    int32_t n_inflow = 0;
+    uint32_t max_ins = 0;
+    CaseDev* d_cases = nullptr;
+    int32_t* d_cell_case = nullptr;
+    double* d_vol = nullptr;
+    double* d_smax = nullptr;
+    int32_t* d_cmodel = nullptr;
+    double* d_samp = nullptr;
+    double* d_uspq = nullptr;
+    double* d_usps = nullptr;
+    double* d_uspmax = nullptr;
+    double* d_knacc = nullptr;
+    double* d_kn = nullptr;
+    double* d_cc = nullptr;
+    double* d_mass = nullptr;
+    int32_t* d_rotdof = nullptr;
+    double* d_omega = nullptr;
+    double* d_muref = nullptr;
+    double* d_prterm = nullptr;
+    int32_t* d_local = nullptr;
+    double *d_pmr = nullptr, *d_pC = nullptr, *d_pe = nullptr, *d_pchi = nullptr, *d_palpha = nullptr;
+    uint32_t* d_cnt = nullptr;
+    uint32_t* d_off = nullptr;
+    uint32_t* d_cursor = nullptr;
+    uint32_t* d_perm = nullptr;
+    uint32_t* d_ins_off = nullptr;
+    uint32_t* d_scratch = nullptr;
+    uint32_t* d_blocksum = nullptr;
+    unsigned long long* d_counters = nullptr;
+    DevState* d_st = nullptr;
+
+    std::vector<CaseDev> cases;          // host mirror (step advanced with the device)
+    std::vector<CaseSpec> specs_host;
+    std::vector<int64_t> samp_reset_steps;
+
+    ~World();
+    void release();
+    size_t rsize() const;
+    int build(std::vector<CaseSpec>& specs, int precision, int device, uint64_t capacity);
+    int alloc_particles(uint32_t newcap);
+    int ensure_capacity(uint64_t need);
+    template <class R> WorldView<R> view() const;
+    int check_state();
+    int count(uint32_t& n);
+    int sort_now();
+    int step(int n);
+    int upload(int ci, const ugx_particle_soa* p, const MeshGeom& g);
+    int fill(int ci, const ugx_fill_desc* f, const MeshGeom& g);
+    int download(int ci, ugx_particle_soa* p, const MeshGeom& g);
+    int case_counters(int ci, ugx_counters* out);
+    int case_samples(int ci, double* out, int64_t len);
+    int case_samples_reset(int ci);
+    int hybrid_update();
+};
+
+// Write-time fields of uniGasVolFields from raw sums (ugx_fields.cpp).
+int compute_fields(const CaseSpec& spec, const double* samp, int ncl, const ugx_counters& cnt, ugx_fields* f);
+
+}  // namespace ugx
+
+// Opaque handle of the C ABI: a world plus the index of the case in it.
+struct ugx_case {
+    ugx::World* world = nullptr;
+    int index = 0;
+    bool owns_world = false;
+};
