@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the particle time step (move + collide + sample) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload nozzle]
+
+A step is one uniGasCloud::evolve() equivalent over the whole case.  The
+default workload is the 3-D nozzle expansion: 200x100x100 planar-faced hexes
+(2e6 cells), 1e8 argon simulators (50 per cell), FP32 particle state with
+FP64 sampler / (sigma_T c_r)max / NTC accumulators.  value = particle-steps/s
+= sum over timed steps of the simulator count / wall time, with the inputs
+resident in HBM.  Prints one JSON line with the per-phase HIP-event timings,
+the roofline of the dominant kernel and a CPU baseline (the FP64 OpenMP
+oracle on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s (move+collide+sample); batched vs plain; p50/p99 case latency"
+
+
+def build_workload(name, scale):
+    from unigasfoam_amd import workloads
+    if name == "nozzle":
+        nx, ny, nz = int(200 * scale), int(100 * scale), int(100 * scale)
+        spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50)
+        desc = f"3-D argon nozzle expansion, {nx}x{ny}x{nz} hex cells, {nx * ny * nz * 50:.3g} simulators (50/cell), hybrid USP/DSMC (NTC+VHS)"
+    elif name == "cylinder":
+        spec, fill = workloads.cylinder()
+        desc = "Mach-10 N2/O2 over a cylinder, 400x200x1 hex, ~5e7 simulators, LB-VHS, inflow+deletion"
+    elif name == "couette":
+        spec, fill = workloads.couette()
+        desc = "1-D N2 Couette, 500 cells, 1e5 simulators, LB-VHS, diffuse walls +/-250 m/s"
+    elif name == "heat_bath":
+        spec, fill = workloads.heat_bath()
+        desc = "0-D argon heat bath, 10^3 cells, 2e4 simulators, VHS, specular box"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return spec, fill, desc
+
+
+def cpu_baseline(name, seconds, scale):
+    """FP64 OpenMP oracle on a bounded sample of the same workload."""
+    from oracle import orc
+    from unigasfoam_amd import workloads
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = orc.set_threads(threads)
+    if name == "nozzle":
+        nx, ny, nz = 40, 20, 20
+        spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50)
+        sample = f"nozzle {nx}x{ny}x{nz} cells, {nx * ny * nz * 50} simulators, same per-cell load and models"
+    else:
+        spec, fill, _ = build_workload(name, scale)
+        sample = "same workload"
+    o = orc.OracleCase(spec, capacity=4 * (spec.mesh.n_cells * 60 + 4096))
+    o.fill(**fill)
+    o.step(1)
+    c0 = o.counters()["particle_steps"]
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds or steps < 3:
+        o.step(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    ps = o.counters()["particle_steps"] - c0
+    return {"value": ps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{sample}; {steps} steps in {dt:.1f} s (restated CPU baseline, not uniGasFoam)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="nozzle")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the nozzle mesh")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="also run the stream-copy HBM probe")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.stderr.write("bench.py: only --gpus 1 is supported (single-GPU build)\n")
+        return 2
+
+    from unigasfoam_amd import roofline
+    from unigasfoam_amd.core import Case, probe_copy
+
+    spec, fill, desc = build_workload(args.workload, args.scale)
+    print(f"[bench] workload: {desc}", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    case = Case(spec, precision=args.precision)
+    case.fill(**fill)
+    case.sync()
+    print(f"[bench] setup {time.perf_counter() - t:.1f} s, {case.count()} simulators", file=sys.stderr, flush=True)
+    case.step(args.warmup)
+    case.sync()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    c0 = case.counters()
+    case.timing(True)
+    case.sync()
+    t0 = time.perf_counter()
+    case.step(args.steps)
+    case.sync()
+    t1 = time.perf_counter()
+    c1 = case.counters()
+    phases, tsteps = case.phase_times()
+    case.timing(False)
+    elapsed = t1 - t0
+    psteps = c1["particle_steps"] - c0["particle_steps"]
+    value = psteps / elapsed
+    R = 4 if args.precision == "fp32" else 8
+    n_avg = psteps / args.steps
+    nc = spec.mesh.n_cells
+    cand = (c1["candidates"] - c0["candidates"]) / args.steps
+    coll = (c1["collisions"] - c0["collisions"]) / args.steps
+    n_bgk = 0.0 if spec.collision_model == "dsmc" else n_avg * (1.0 - c1["n_dsmc_cells"] / nc)
+    per_phase = {}
+    for p, ms in phases.items():
+        avg = ms / max(tsteps, 1)
+        b = roofline.phase_bytes(p, R, n_avg, nc, cand, coll, n_bgk)
+        per_phase[p] = {"ms": round(avg, 4), "GBs": round(b / (avg * 1e-3) / 1e9, 1) if b and avg > 0 else None}
+    byte_phases = [p for p in per_phase if roofline.phase_bytes(p, R, n_avg, nc, cand, coll, n_bgk) > 0]
+    dom = max(byte_phases, key=lambda p: per_phase[p]["ms"])
+    dom_bytes = roofline.phase_bytes(dom, R, n_avg, nc, cand, coll, n_bgk)
+    dom_s = per_phase[dom]["ms"] * 1e-3
+    achieved = dom_bytes / dom_s / 1e9
+    step_b = roofline.step_bytes(R, n_avg, nc, cand, coll, n_bgk)
+    step_gbs = step_b / (elapsed / args.steps) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "form": "plain (one case)", "precision":
+                   "FP32 particle state, FP64 samplers/sigma_max/NTC" if R == 4 else "all FP64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(dom_bytes)},
+        "step_bandwidth": {"algorithmic_bytes_per_step": int(step_b), "GBs": round(step_gbs, 1),
+                           "frac": round(step_gbs / roofline.HBM_PEAK_GBS, 4)},
+        "phases": per_phase,
+        "counters_per_step": {"simulators": n_avg, "candidates": cand, "collisions": coll,
+                              "dsmc_cells": c1["n_dsmc_cells"]},
+    }
+    if args.probe:
+        nbytes = 2 << 30
+        s = probe_copy(nbytes, reps=10)
+        out["hbm_probe"] = {"copy_bytes": nbytes, "GBs": round(2 * nbytes / s / 1e9, 1)}
+    case.close()
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, args.scale)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -237,6 +237,7 @@ typedef struct {
     int64_t n_sampled_steps;
     double  time_sampled;      /* sum of deltaT over sampled steps */
     int64_t n_dsmc_cells;      /* hybrid: cells using DSMC now */
+    int64_t particle_steps;    /* sum over steps of the simulator count after the step */
 } ugx_counters;
 
 typedef struct ugx_case ugx_case;
@@ -270,6 +271,19 @@ int ugx_sigma_max_upload(ugx_case* c, const double* in, int64_t len);
 int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
 int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
+
+/* Per-phase timing of ugx_step with HIP events on the case's stream (no
+ * counterpart in the reference, whose only timing is the ExecutionTime /
+ * ClockTime log line of uniGasFoam.C:97-101).  Phases, in launch order:
+ * begin, inflow, move, scan, scatter, segsort, gather, sample, bgk, ntc, end. */
+#define UGX_N_PHASES 11
+const char* ugx_phase_name(int i);
+int ugx_timing_enable(ugx_case* c, int on);       /* also resets the totals */
+int ugx_timing_get(ugx_case* c, double* ms_total, int n_phases, int64_t* steps);
+
+/* Device-to-device copy of `bytes` bytes timed over `reps` launches: the
+ * stream-copy probe that confirms the HBM peak (no reference counterpart). */
+int ugx_probe_copy(int device, int64_t bytes, int reps, double* seconds_per_copy);
 
 /* Host-only geometry in kernel storage form (no device needed): per cell
  * reference point [3n], volume [n], face count [n], outward planes

@@ -1133,7 +1133,7 @@ int orc_step_phases(orc_world* w, int mask)
     if ((mask & 8) && w->model.collision_model == UGX_COLL_HYBRID
         && ((w->step + 1) % (uint32_t)w->model.decomposition_interval) == 0)
         hybrid_switch(w);
-    if (mask & 16) { w->step += 1; w->cnt.steps += 1; }
+    if (mask & 16) { w->step += 1; w->cnt.steps += 1; w->cnt.particle_steps += w->n; }
     return UGX_OK;
 }
 

@@ -110,7 +110,7 @@ class Counters(C.Structure):
         ("collisions", C.c_int64), ("bgk_collisions", C.c_int64),
         ("inserted", C.c_int64), ("deleted", C.c_int64), ("wall_hits", C.c_int64),
         ("track_errors", C.c_int64), ("n_sampled_steps", C.c_int64),
-        ("time_sampled", C.c_double), ("n_dsmc_cells", C.c_int64),
+        ("time_sampled", C.c_double), ("n_dsmc_cells", C.c_int64), ("particle_steps", C.c_int64),
     ]
 
     def as_dict(self):

@@ -25,6 +25,7 @@ EXPORTS = [
     "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download",
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry",
+    "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_probe_copy",
 ]
 
 
@@ -66,8 +67,27 @@ def lib():
                                    C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
                                    abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P,
                                    abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
+        L.ugx_phase_name.restype = C.c_char_p
+        L.ugx_phase_name.argtypes = [C.c_int]
+        L.ugx_timing_enable.argtypes = [vp, C.c_int]
+        L.ugx_timing_get.argtypes = [vp, abi.DBL_P, C.c_int, C.POINTER(C.c_int64)]
+        L.ugx_probe_copy.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+N_PHASES = 11
+
+
+def phase_names():
+    return [lib().ugx_phase_name(i).decode() for i in range(N_PHASES)]
+
+
+def probe_copy(nbytes, reps=20, device=0):
+    """Seconds per device copy of nbytes (reads nbytes, writes nbytes)."""
+    s = C.c_double()
+    check(lib().ugx_probe_copy(device, int(nbytes), int(reps), C.byref(s)))
+    return s.value
 
 
 def check(rc, L=None):
@@ -212,6 +232,16 @@ class Case:
         out = np.zeros(self.nc, dtype=np.int32)
         check(self.L.ugx_cell_model_download(self.h, abi.iptr(out), self.nc), self.L)
         return out
+
+    def timing(self, on=True):
+        check(self.L.ugx_timing_enable(self.h, 1 if on else 0), self.L)
+
+    def phase_times(self):
+        """{phase: total ms} over the timed steps, and the number of steps."""
+        ms = np.zeros(N_PHASES)
+        n = C.c_int64()
+        check(self.L.ugx_timing_get(self.h, abi.dptr(ms), N_PHASES, C.byref(n)), self.L)
+        return dict(zip(phase_names(), ms.tolist())), n.value
 
     def cell_volumes(self):
         out = np.zeros(self.nc)

@@ -282,6 +282,69 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len)
     UGX_GUARD_END
 }
 
+static const char* kPhaseNames[kPhases] = {"begin", "inflow", "move", "scan", "scatter", "segsort",
+                                           "gather", "sample", "bgk", "ntc", "end"};
+
+const char* ugx_phase_name(int i) { return (i >= 0 && i < kPhases) ? kPhaseNames[i] : ""; }
+
+int ugx_timing_enable(ugx_case* c, int on)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    World* w = c->world;
+    if (int rc = w->flush_timing()) return rc;
+    w->timing = on != 0;
+    for (double& v : w->phase_ms) v = 0.0;
+    w->timed_steps = 0;
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
+int ugx_timing_get(ugx_case* c, double* ms_total, int n, int64_t* steps)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    World* w = c->world;
+    if (int rc = w->flush_timing()) return rc;
+    for (int i = 0; i < n && i < kPhases; ++i) ms_total[i] = w->phase_ms[i];
+    if (steps) *steps = w->timed_steps;
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
+int ugx_probe_copy(int device, int64_t bytes, int reps, double* seconds_per_copy)
+{
+    UGX_GUARD_BEGIN
+    if (bytes <= 0 || reps <= 0 || !seconds_per_copy) return set_err(UGX_ERR_ARG, "bad probe arguments");
+    if (hipSetDevice(device) != hipSuccess) return set_err(UGX_ERR_DEVICE, "hipSetDevice failed");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = UGX_OK;
+    bytes = bytes / 16 * 16;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess || hipStreamCreate(&s) != hipSuccess
+        || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = set_err(UGX_ERR_DEVICE, "probe allocation failed");
+    } else {
+        launch_copy_probe(a, b, (size_t)bytes, s);
+        (void)hipEventRecord(e0, s);
+        for (int i = 0; i < reps; ++i) launch_copy_probe(a, b, (size_t)bytes, s);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.0f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = set_err(UGX_ERR_DEVICE, "probe timing failed");
+        else
+            *seconds_per_copy = (double)ms * 1e-3 / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+    UGX_GUARD_END
+}
+
 // Geometry in kernel storage form, for parity with the CPU oracle
 // (no counterpart in the reference; test-facing).
 int ugx_geometry(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,

@@ -369,11 +369,41 @@ struct TrackOut {
 // Face tracking through convex planar-faced cells (uniGasParcel::move,
 // uniGasParcel.C:35-108) with the wall dispatch of hitWallPatch (:131-145).
 // Positions and planes in R; wall physics in FP64.
-template <class R>
+// Species data of a simulator, looked up only when a wall model needs it:
+// MassOf carries known values (new simulators), a world view loads
+// species id and E_rot of slot t lazily (existing simulators).
+struct MassOf {
+    double mass;
+    int rotdof;
+};
+
+__device__ inline void wall_species(const MassOf& m, uint32_t, double& erot, double& mass, int& rotdof, bool&)
+{
+    mass = m.mass;
+    rotdof = m.rotdof;
+    (void)erot;
+}
+
+template <class WV>
+__device__ inline void wall_species(const WV& w, uint32_t t, double& erot, double& mass, int& rotdof, bool& loaded)
+{
+    if (!loaded) {
+        const uint32_t g = w.a.sp[t];
+        mass = w.sp.mass[g];
+        rotdof = w.sp.rotdof[g];
+        erot = (double)w.a.erot[t];
+        loaded = true;
+    }
+}
+
+template <class R, class Src>
 __device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const double mask[3],
                                  uint32_t& cell, R x[3], double U[3], double& erot,
-                                 double mass, int rotdof, double tau_d, Rng& rng)
+                                 const Src& src, uint32_t slot, double tau_d, Rng& rng)
 {
+    double mass = 0.0;
+    int rotdof = 0;
+    bool loaded = false;
     TrackOut out{1, 0, 0};
     uint32_t c = cell;
     R vt[3];
@@ -416,6 +446,7 @@ __device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const doubl
         double nw[3] = {(double)q.x, (double)q.y, (double)q.z};
         if (b.kind == UGX_BND_DELETION) { cell = kDead; out.alive = 0; return out; }
         out.hits += 1;
+        wall_species(src, slot, erot, mass, rotdof, loaded);
         if (b.kind == UGX_BND_SPECULAR) {
             double udn = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
             if (udn > 0.0)

@@ -169,7 +169,7 @@ __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, cons
     erot = equip_erot(r, e.rotT, rd);
     cell = (uint32_t)e.cell;
     double frac = uniform(r);
-    to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, m, rd, (1.0 - frac) * cs.dt, r);
+    to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, MassOf{m, rd}, 0u, (1.0 - frac) * cs.dt, r);
     return to.alive;
 }
 
@@ -234,13 +234,12 @@ __global__ void k_move(WorldView<R> w)
             cell = w.a.cell[t];
             ccase = w.cell_case[cell];
             const CaseDev& cs = w.cases[ccase];
-            const uint32_t g = w.a.sp[t];
             R x[3] = {w.a.x[t], w.a.y[t], w.a.z[t]};
             double U[3] = {(double)w.a.vx[t], (double)w.a.vy[t], (double)w.a.vz[t]};
-            double erot = (double)w.a.erot[t];
+            double erot = 0.0;
             Rng r;
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
-            to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w.sp.mass[g], w.sp.rotdof[g], cs.dt, r);
+            to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w, t, cs.dt, r);
             w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
             w.a.cell[t] = cell;
             if (to.hits) {
@@ -767,6 +766,8 @@ __global__ void k_end_step(WorldView<R> w)
         CaseDev& cs = w.cases[c];
         if (cs.active) {
             if (((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u) w.counters[(size_t)c * kCntNum + kCntSampled] += 1ull;
+            w.counters[(size_t)c * kCntNum + kCntParticleSteps] +=
+                (unsigned long long)(w.off[cs.cell_end] - w.off[cs.cell_begin]);
             if (cs.coll_model == UGX_COLL_HYBRID) cs.kn_time += cs.dt;
             cs.step += 1u;
         }
@@ -907,26 +908,43 @@ void launch_sort(const WorldView<R>& w, hipStream_t s)
 }
 
 template <class R>
-void launch_step(const WorldView<R>& w0, hipStream_t s, int)
+void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
 {
     WorldView<R> w = w0;
     const uint32_t ncp = (uint32_t)w.nc + 1u;
     const unsigned gc = grid_for((uint64_t)w.ncases);
+    auto mark = [&](int i) { if (marks) (void)hipEventRecord(marks[i], s); };
+    mark(0);
     hipLaunchKernelGGL(k_begin_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cursor, ncp);
+    mark(1);
     if (w.n_inflow > 0) {
         hipLaunchKernelGGL(k_inflow_count<R>, dim3(1), dim3(1024), 0, s, w);
         hipLaunchKernelGGL(k_inflow_gen<R>, dim3(grid_for(w.max_ins)), dim3(kBlock), 0, s, w);
     }
+    mark(2);
     hipLaunchKernelGGL(k_move<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
-    launch_sort_tail(w, s);
+    mark(3);
+    scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
+    mark(4);
+    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    mark(5);
+    const unsigned tiles = (unsigned)((w.cap + kTileH - 1) / kTileH) + 1u;
+    hipLaunchKernelGGL(k_segsort<R>, dim3(tiles), dim3(kBlock), 0, s, w);
+    mark(6);
+    hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    mark(7);
     // from here on the sorted records are in w.b
     const unsigned gcell = grid_for((uint64_t)w.nc);
     hipLaunchKernelGGL(k_sample<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+    mark(8);
     hipLaunchKernelGGL(k_bgk<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+    mark(9);
     hipLaunchKernelGGL(k_ntc<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+    mark(10);
     hipLaunchKernelGGL(k_end_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
+    mark(11);
 }
 
 template <class R>
@@ -942,8 +960,19 @@ void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase, const double* 
     hipLaunchKernelGGL(k_fill_finish<R>, dim3(1), dim3(1), 0, s, w, foff, total);
 }
 
-template void launch_step<float>(const WorldView<float>&, hipStream_t, int);
-template void launch_step<double>(const WorldView<double>&, hipStream_t, int);
+__global__ void k_copy_probe(const float4* __restrict__ in, float4* __restrict__ out, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];
+}
+
+void launch_copy_probe(const void* in, void* out, size_t bytes, hipStream_t s)
+{
+    size_t n = bytes / sizeof(float4);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid_for(n, 65536)), dim3(kBlock), 0, s, (const float4*)in, (float4*)out, n);
+}
+
+template void launch_step<float>(const WorldView<float>&, hipStream_t, hipEvent_t*);
+template void launch_step<double>(const WorldView<double>&, hipStream_t, hipEvent_t*);
 template void launch_sort<float>(const WorldView<float>&, hipStream_t);
 template void launch_sort<double>(const WorldView<double>&, hipStream_t);
 template void launch_fill<float>(const WorldView<float>&, hipStream_t, int, const double*, const double*, double,

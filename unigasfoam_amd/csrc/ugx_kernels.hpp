@@ -19,7 +19,7 @@ struct DevState {
 // Per-case cumulative counters (uint64).
 enum CounterIdx {
     kCntCandidates = 0, kCntCollisions, kCntBgk, kCntInserted, kCntDeleted,
-    kCntWallHits, kCntTrackErrors, kCntSampled, kCntNum
+    kCntWallHits, kCntTrackErrors, kCntSampled, kCntParticleSteps, kCntNum
 };
 
 template <class R>
@@ -60,12 +60,14 @@ struct WorldView {
 };
 
 // Host launchers (ugx_kernels.hip).
-template <class R> void launch_step(const WorldView<R>& w, hipStream_t s, int sort_only);
+constexpr int kPhases = 11;  // begin, inflow, move, scan, scatter, segsort, gather, sample, bgk, ntc, end
+template <class R> void launch_step(const WorldView<R>& w, hipStream_t s, hipEvent_t* marks);
 template <class R> void launch_sort(const WorldView<R>& w, hipStream_t s);
 template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase,
                                     const double* nd, const double* T, double rotT, const double* U,
                                     int exact, const double* bblo, const double* bbhi,
                                     uint32_t* fill_cnt, uint32_t* fill_off, int ns, int ncell_case);
+void launch_copy_probe(const void* in, void* out, size_t bytes, hipStream_t s);
 template <class R> void launch_hybrid(const WorldView<R>& w, hipStream_t s, double* work);
 
 }  // namespace ugx

@@ -286,6 +286,8 @@ World::~World() { release(); }
 
 void World::release()
 {
+    for (auto& e : tev) if (e) (void)hipEventDestroy(e);
+    tev.clear();
     for (int k = 0; k < 2; ++k)
         for (int a = 0; a < 9; ++a) dfree(pbuf[k][a]);
     void** ptrs[] = {(void**)&d_plane, (void**)&d_delta, (void**)&d_nbr, (void**)&d_bm, (void**)&d_inflow,
@@ -603,11 +605,41 @@ int World::sort_now()
     return UGX_OK;
 }
 
+static constexpr int kTimeBatch = 32;
+
+int World::flush_timing()
+{
+    if (tev_used == 0) return UGX_OK;
+    HIP_TRY(hipEventSynchronize(tev[(size_t)tev_used * (kPhases + 1) - 1]));
+    for (int s = 0; s < tev_used; ++s) {
+        hipEvent_t* m = &tev[(size_t)s * (kPhases + 1)];
+        for (int p = 0; p < kPhases; ++p) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, m[p], m[p + 1]));
+            phase_ms[p] += (double)ms;
+        }
+    }
+    timed_steps += tev_used;
+    tev_used = 0;
+    return UGX_OK;
+}
+
 int World::step(int n)
 {
     for (int i = 0; i < n; ++i) {
-        if (precision == UGX_FP32_STATE) launch_step<float>(view<float>(), stream, 0);
-        else launch_step<double>(view<double>(), stream, 0);
+        hipEvent_t* marks = nullptr;
+        if (timing) {
+            if (tev.empty()) {
+                tev.resize((size_t)kTimeBatch * (kPhases + 1));
+                for (auto& e : tev) HIP_TRY(hipEventCreate(&e));
+            }
+            if (tev_used == kTimeBatch)
+                if (int rc = flush_timing()) return rc;
+            marks = &tev[(size_t)tev_used * (kPhases + 1)];
+            tev_used++;
+        }
+        if (precision == UGX_FP32_STATE) launch_step<float>(view<float>(), stream, marks);
+        else launch_step<double>(view<double>(), stream, marks);
         cur = 1 - cur;
         for (auto& c : cases) c.step += 1;
     }
@@ -781,6 +813,7 @@ int World::case_counters(int ci, ugx_counters* out)
     out->wall_hits = (int64_t)cnt[kCntWallHits];
     out->track_errors = (int64_t)cnt[kCntTrackErrors];
     out->n_sampled_steps = (int64_t)cnt[kCntSampled] - samp_reset_steps[ci];
+    out->particle_steps = (int64_t)cnt[kCntParticleSteps];
     out->time_sampled = (double)out->n_sampled_steps * cases[ci].dt;
     int64_t k = 0;
     for (int32_t v : cm) k += v;

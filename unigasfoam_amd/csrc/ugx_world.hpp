@@ -79,6 +79,14 @@ struct World {
     unsigned long long* d_counters = nullptr;
     DevState* d_st = nullptr;
 
+    // per-phase HIP-event timing (off unless enabled)
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    int tev_used = 0;
+    double phase_ms[kPhases] = {};
+    int64_t timed_steps = 0;
+    int flush_timing();
+
     std::vector<CaseDev> cases;          // host mirror (step advanced with the device)
     std::vector<CaseSpec> specs_host;
     std::vector<int64_t> samp_reset_steps;
