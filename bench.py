@@ -24,12 +24,17 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s (move+collide+sample); batched vs plain; p50/p99 case latency"
 
 
-def build_workload(name, scale):
+COLLISION_DESC = {"hybrid": "hybrid USP/DSMC (NTC+VHS)", "dsmc": "DSMC (NTC+VHS)", "bgk": "USP everywhere",
+                  "none": "collisionless"}
+
+
+def build_workload(name, scale, collision="hybrid"):
     from unigasfoam_amd import workloads
     if name == "nozzle":
         nx, ny, nz = int(200 * scale), int(100 * scale), int(100 * scale)
-        spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50)
-        desc = f"3-D argon nozzle expansion, {nx}x{ny}x{nz} hex cells, {nx * ny * nz * 50:.3g} simulators (50/cell), hybrid USP/DSMC (NTC+VHS)"
+        spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50, collision_model=collision)
+        desc = (f"3-D argon nozzle expansion, {nx}x{ny}x{nz} hex cells, {nx * ny * nz * 50:.3g} simulators "
+                f"(50/cell), {COLLISION_DESC[collision]}")
     elif name == "cylinder":
         spec, fill = workloads.cylinder()
         desc = "Mach-10 N2/O2 over a cylinder, 400x200x1 hex, ~5e7 simulators, LB-VHS, inflow+deletion"
@@ -83,6 +88,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", action="store_true", help="also run the stream-copy HBM probe")
+    ap.add_argument("--collision", default="hybrid", choices=["hybrid", "dsmc", "bgk", "none"],
+                    help="collision model of the nozzle workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -93,7 +100,7 @@ def main():
     from unigasfoam_amd import roofline
     from unigasfoam_amd.core import Case, probe_copy
 
-    spec, fill, desc = build_workload(args.workload, args.scale)
+    spec, fill, desc = build_workload(args.workload, args.scale, args.collision)
     print(f"[bench] workload: {desc}", file=sys.stderr, flush=True)
     t = time.perf_counter()
     case = Case(spec, precision=args.precision)

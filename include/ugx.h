@@ -133,6 +133,10 @@ typedef struct {
     int32_t smoothing_passes;         /* default 5 */
     double  breakdown_max;            /* Kn_GLL threshold (default 0.05) */
     double  kn_theta;                 /* EMA weight of Kn (default 0.2) */
+    int32_t sampler_full;             /* 1: also sample the second moments and heat-flux
+                                         moments (measureHeatFluxShearStress); 0: the
+                                         uniGasVolFields set (rows 0-7 and per species) */
+    int32_t pad0;
 } ugx_model_desc;
 
 typedef enum {
@@ -275,8 +279,9 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
 /* Per-phase timing of ugx_step with HIP events on the case's stream (no
  * counterpart in the reference, whose only timing is the ExecutionTime /
  * ClockTime log line of uniGasFoam.C:97-101).  Phases, in launch order:
- * begin, inflow, move, scan, scatter, segsort, gather, sample, bgk, ntc, end. */
-#define UGX_N_PHASES 11
+ * begin, inflow, move, wall (diffuse-wall continuation), scan, scatter,
+ * cells (slot order + gather + sampler + BGK + NTC), end. */
+#define UGX_N_PHASES 8
 const char* ugx_phase_name(int i);
 int ugx_timing_enable(ugx_case* c, int on);       /* also resets the totals */
 int ugx_timing_get(ugx_case* c, double* ms_total, int n_phases, int64_t* steps);

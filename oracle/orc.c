@@ -657,9 +657,16 @@ static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, 
  * (stochasticParticleBGK.C:287-583, stochasticParticleESBGK.C:311-665,
  * unifiedStochasticParticleSBGK.C:379-800), collide (:628-729, :721-834,
  * :871-994), conserveMomentumAndEnergy (:731-780) and the post-collision
- * samplers (:782-796, :887-906, :1047-1096).  The reference's five list
- * shuffles followed by "take the first n" are restated as one partial
- * Fisher-Yates draw of n distinct simulators (same distribution). */
+ * samplers (:782-796, :887-906, :1047-1096).
+ * Restated for parallel sampling, with the same distributions:
+ *  - the reference's five list shuffles followed by "take the first n" are
+ *    one partial Fisher-Yates draw of n distinct simulators (cell stream);
+ *  - each selected simulator draws its new velocity from its own stream
+ *    (ORC_RNG_BGK_V, sub = selection index);
+ *  - USP: every draw of the step is tested against the cell's maxProb at the
+ *    start of the step; a draw with maxProb < P < 10 is accepted and the
+ *    largest such P becomes the new maxProb (the value the reference's
+ *    sequential raises end on); without a raise maxProb decays by 0.999. */
 static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[NSAMP],
                      uint32_t* idx, int64_t* nrel_out, int update_state)
 {
@@ -730,7 +737,8 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
         for (int d = 0; d < 3; ++d) w->uspq[3 * c + d] = q[d];
         for (int k = 0; k < 6; ++k) w->usps[6 * c + k] = sh[k];
     }
-    int raised = 0;
+    const double m0 = w->uspmax[c];
+    double raise = 0.0;
     if (perform && w->cmodel[c] == 0) {
         orc_rng r;
         orc_rng_init(&r, w->seed, ORC_RNG_BGK, 0u, (uint32_t)c, w->step);
@@ -741,6 +749,10 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
         if (nrel > nC) nrel = nC;
         *nrel_out += nrel;
         for (int64_t i = 0; i < nC; ++i) idx[i] = (uint32_t)i;
+        for (int64_t i = 0; i < nrel; ++i) {
+            uint32_t j = (uint32_t)i + orc_index(&r, (uint32_t)(nC - i));
+            uint32_t t = idx[i]; idx[i] = idx[j]; idx[j] = t;
+        }
         double Smat[9];
         if (model == UGX_BGK_ESBGK) {
             double a = 0.5 * (1.0 - Pr) / Pr;
@@ -754,17 +766,17 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
         double ex = 1.0 + 2.0 / (orc_exp(2.0 * tau) - 1.0);
         double cQ = 2.0 * (1.0 - Pr * tau * ex);
         double cS = 1.0 - tau * ex;
+        double s2kT = sqrt(2.0 * KB * T);
         for (int64_t i = 0; i < nrel; ++i) {
-            uint32_t j = (uint32_t)i + orc_index(&r, (uint32_t)(nC - i));
-            uint32_t t = idx[i]; idx[i] = idx[j]; idx[j] = t;
             int64_t pi = b + idx[i];
-            double m = w->mass[w->sp[pi]];
-            double u0 = sqrt(2.0 * KB * T / m);
+            orc_rng rv;
+            orc_rng_init(&rv, w->seed, ORC_RNG_BGK_V, (uint32_t)i, (uint32_t)c, w->step);
+            double u0 = s2kT / sqrt(w->mass[w->sp[pi]]);
             double xi[3];
             if (model == UGX_BGK_USP) {
                 for (int it = 0; it < 100000; ++it) {
                     double g[3];
-                    orc_gauss3(&r, g);
+                    orc_gauss3(&rv, g);
                     for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
                     double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
                     double vTr = vSq / 3.0;
@@ -772,12 +784,12 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
                                   + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
                                               + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
                                               + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
-                    if (prob > w->uspmax[c] && prob < 10.0) { w->uspmax[c] = prob; raised = 1; break; }
-                    if (orc_uniform(&r) < prob / w->uspmax[c]) break;
+                    if (prob > m0 && prob < 10.0) { if (prob > raise) raise = prob; break; }
+                    if (orc_uniform(&rv) < prob / m0) break;
                 }
             } else {
                 double g[3];
-                orc_gauss3(&r, g);
+                orc_gauss3(&rv, g);
                 for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
                 if (model == UGX_BGK_ESBGK) {
                     double y[3];
@@ -804,7 +816,7 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
                 for (int d = 0; d < 3; ++d) w->v[3 * i + d] = U[d] + (w->v[3 * i + d] - pU[d]) * sc;
         }
     }
-    if (usp && update_state && !raised) w->uspmax[c] = w->uspmax[c] * 0.999;
+    if (usp && update_state) w->uspmax[c] = (raise > 0.0) ? raise : m0 * 0.999;
 }
 
 /* ------------------------------------------------------------ lifecycle */
@@ -1115,7 +1127,9 @@ int orc_step_phases(orc_world* w, int mask)
             double S[NSAMP];
             cell_sums(w, b, b + nC, S);
             if (sample)
-                for (int k = 0; k < NSAMP; ++k) w->samp[(size_t)k * w->nc + c] += dt * S[k];
+                for (int k = 0; k < NSAMP; ++k)
+                    if (w->model.sampler_full || k < UGX_S_MUU || k >= UGX_S_BASE)
+                        w->samp[(size_t)k * w->nc + c] += dt * S[k];
             if (hybrid)
                 for (int k = 0; k < 8; ++k) w->knacc[8 * c + k] += dt * S[k];
             if ((cm == UGX_COLL_DSMC || hybrid) && w->cmodel[c] == 1 && w->model.dsmc_model != UGX_DSMC_NONE)

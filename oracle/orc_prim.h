@@ -50,7 +50,8 @@ enum {
     ORC_RNG_BGK = 3,         /* id = cell, per step                     */
     ORC_RNG_INFLOW_N = 4,    /* id = inflow face-species, per step      */
     ORC_RNG_INFLOW_P = 5,    /* id = inflow face-species, sub = index   */
-    ORC_RNG_FILL = 6         /* id = cell, sub = species                */
+    ORC_RNG_FILL = 6,        /* id = cell, sub = species                */
+    ORC_RNG_BGK_V = 7        /* id = cell, sub = selection index        */
 };
 
 typedef struct {

@@ -77,6 +77,17 @@ def test_bgk_family_identical(require_gpu, bgk):
     assert_identical(g, o, bgk)
 
 
+def test_full_sampler_multispecies_identical(require_gpu):
+    """Second-moment and heat-flux sampler rows (sampler_full) with two species."""
+    spec, fill = workloads.cylinder(n_theta=20, n_r=10, target=1.5e4, dt=2.0e-7)
+    spec.sampler_full = True
+    g, o = pair(spec, fill, capacity=1 << 17)
+    g.step(10)
+    o.step(10)
+    assert_identical(g, o, "full sampler")
+    assert np.any(g.samples()[8] != 0.0)
+
+
 def test_inflow_deletion_identical(require_gpu):
     spec, fill = workloads.cylinder(n_theta=24, n_r=12, target=2.0e4, dt=2.0e-7)
     g, o = pair(spec, fill, capacity=1 << 17)

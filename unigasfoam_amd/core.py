@@ -76,7 +76,7 @@ def lib():
     return _lib
 
 
-N_PHASES = 11
+N_PHASES = 8
 
 
 def phase_names():

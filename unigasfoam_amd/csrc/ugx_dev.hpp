@@ -249,6 +249,8 @@ struct CaseDev {
     int32_t decomposition_interval, smoothing_passes;
     double breakdown_max, kn_theta;
     double kn_time;
+    int32_t sampler_full;        // sample rows 8-19 as well
+    int32_t sum_level;           // per-cell sums needed: 0 rows 0-7, 1 +8-13, 2 +14-19
 };
 
 struct BModel {
@@ -364,6 +366,9 @@ struct TrackOut {
     int alive;
     int hits;
     int errs;
+    int queued;      // stopped at a diffuse wall (deferred): face and remaining time below
+    int qface;
+    int qiter;
 };
 
 // Face tracking through convex planar-faced cells (uniGasParcel::move,
@@ -396,20 +401,23 @@ __device__ inline void wall_species(const WV& w, uint32_t t, double& erot, doubl
     }
 }
 
-template <class R, class Src>
-__device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const double mask[3],
-                                 uint32_t& cell, R x[3], double U[3], double& erot,
-                                 const Src& src, uint32_t slot, double tau_d, Rng& rng)
+// Defer = true: stop at the first diffuse wall (position on the wall, the
+// remaining time in tau_io) and report it as queued; the caller finishes
+// the track with the wall model later.  Nothing random is drawn before a
+// diffuse wall, so the continuation uses a freshly initialised stream.
+template <bool Defer, class R, class Src>
+__device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const double mask[3],
+                                   uint32_t& cell, R x[3], double U[3], double& erot,
+                                   const Src& src, uint32_t slot, R& tau, Rng& rng, int it0)
 {
     double mass = 0.0;
     int rotdof = 0;
     bool loaded = false;
-    TrackOut out{1, 0, 0};
+    TrackOut out{1, 0, 0, 0, 0, 0};
     uint32_t c = cell;
     R vt[3];
     for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * mask[d]);
-    R tau = (R)tau_d;
-    for (int it = 0; it < kMaxCrossings; ++it) {
+    for (int it = it0; it < kMaxCrossings; ++it) {
         R tmin = tau;
         int fmin = -1;
         const size_t base = (size_t)c * kMaxFaces;
@@ -445,6 +453,13 @@ __device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const doubl
         V4<R> q = g.plane[base + fmin];
         double nw[3] = {(double)q.x, (double)q.y, (double)q.z};
         if (b.kind == UGX_BND_DELETION) { cell = kDead; out.alive = 0; return out; }
+        if (Defer && b.kind == UGX_BND_DIFFUSE) {
+            out.queued = 1;
+            out.qface = fmin;
+            out.qiter = it;
+            cell = c;
+            return out;
+        }
         out.hits += 1;
         wall_species(src, slot, erot, mass, rotdof, loaded);
         if (b.kind == UGX_BND_SPECULAR) {
@@ -459,6 +474,36 @@ __device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const doubl
     out.errs = 1;
     cell = c;
     return out;
+}
+
+template <class R, class Src>
+__device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const double mask[3],
+                                 uint32_t& cell, R x[3], double U[3], double& erot,
+                                 const Src& src, uint32_t slot, double tau_d, Rng& rng)
+{
+    R tau = (R)tau_d;
+    return track_t<false, R, Src>(g, bm, mask, cell, x, U, erot, src, slot, tau, rng, 0);
+}
+
+// Diffuse reflection at face f of cell c, then the rest of the track:
+// the continuation of a deferred track_t<true>.
+template <class R, class Src>
+__device__ inline TrackOut finish_diffuse(const Geom<R>& g, const BModel* bm, const double mask[3],
+                                          uint32_t& cell, int f, int it, R x[3], double U[3], double& erot,
+                                          const Src& src, uint32_t slot, R tau, Rng& rng)
+{
+    double mass = 0.0;
+    int rotdof = 0;
+    bool loaded = false;
+    wall_species(src, slot, erot, mass, rotdof, loaded);
+    const size_t k = (size_t)cell * kMaxFaces + f;
+    const BModel b = bm[-1 - g.nbr[k]];
+    V4<R> q = g.plane[k];
+    double nw[3] = {(double)q.x, (double)q.y, (double)q.z};
+    diffuse_reflect(rng, nw, U, erot, b.T, b.U, mass, rotdof);
+    TrackOut o = track_t<false, R, Src>(g, bm, mask, cell, x, U, erot, src, slot, tau, rng, it + 1);
+    o.hits += 1;
+    return o;
 }
 
 // ------------------------------------------------------------ collisions

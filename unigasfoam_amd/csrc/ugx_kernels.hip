@@ -23,8 +23,6 @@ namespace ugx {
 
 constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;      // elements per scan block (256 threads x 4)
-constexpr uint32_t kTileH = 2048;     // nominal simulators per sort tile
-constexpr uint32_t kTileCap = 4096;   // LDS keys per sort tile
 
 __device__ inline uint32_t wave_agg_inc(uint32_t* ctr, uint32_t key, bool active)
 {
@@ -62,6 +60,59 @@ __device__ inline void case_add(unsigned long long* counters, int ccase, int idx
     }
 }
 
+
+// Per-thread counters of one case, reduced over the block at kernel end so
+// that a kernel issues one global atomic per counter per block instead of
+// one per wave.  Slots: 0 deleted, 1 wall hits, 2 track errors, 3 inserted.
+struct TCount {
+    int ccase;
+    unsigned long long v[4];
+};
+
+__device__ inline void tcount_flush_global(const TCount& t, unsigned long long* counters)
+{
+    const int idx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
+    if (t.ccase < 0) return;
+    for (int k = 0; k < 4; ++k)
+        if (t.v[k]) atomicAdd(&counters[(size_t)t.ccase * kCntNum + idx[k]], t.v[k]);
+}
+
+__device__ inline void tcount_add(TCount& t, unsigned long long* counters, int ccase, int k, unsigned long long v)
+{
+    if (!v) return;
+    if (t.ccase != ccase) {
+        tcount_flush_global(t, counters);
+        t.ccase = ccase;
+        t.v[0] = t.v[1] = t.v[2] = t.v[3] = 0ull;
+    }
+    t.v[k] += v;
+}
+
+// Called by every thread of the block.
+__device__ inline void tcount_block_flush(const TCount& t, unsigned long long* counters)
+{
+    __shared__ int s_case[kBlock];
+    __shared__ unsigned long long s_v[kBlock][4];
+    const uint32_t tid = threadIdx.x;
+    s_case[tid] = t.ccase;
+    for (int k = 0; k < 4; ++k) s_v[tid][k] = t.v[k];
+    __syncthreads();
+    if (tid == 0) {
+        TCount sum{-1, {0ull, 0ull, 0ull, 0ull}};
+        for (uint32_t i = 0; i < kBlock; ++i) {
+            if (s_case[i] < 0) continue;
+            if (sum.ccase < 0) sum.ccase = s_case[i];
+            if (s_case[i] == sum.ccase) {
+                for (int k = 0; k < 4; ++k) sum.v[k] += s_v[i][k];
+            } else {
+                TCount o{s_case[i], {s_v[i][0], s_v[i][1], s_v[i][2], s_v[i][3]}};
+                tcount_flush_global(o, counters);
+            }
+        }
+        tcount_flush_global(sum, counters);
+    }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
@@ -72,7 +123,7 @@ __global__ void k_begin_step(WorldView<R> w)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < w.ncases) w.cases[c].particle_begin = w.off[w.cases[c].cell_begin];
-    if (c == 0) { w.st->n_ins = 0u; }
+    if (c == 0) { w.st->n_ins = 0u; w.st->nwall = 0u; }
 }
 
 // ------------------------------------------------------------- inflow
@@ -176,6 +227,7 @@ __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, cons
 template <class R>
 __global__ void k_inflow_gen(WorldView<R> w)
 {
+    TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
     const uint32_t total = w.st->n_ins;
     const uint32_t n0 = w.st->n;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -209,18 +261,26 @@ __global__ void k_inflow_gen(WorldView<R> w)
         }
         bool alive = active && cell != kDead;
         wave_agg_inc(w.cnt, alive ? cell : 0u, alive);
-        case_add(w.counters, ccase, kCntInserted, active ? 1ull : 0ull);
-        case_add(w.counters, ccase, kCntDeleted, (active && !alive) ? 1ull : 0ull);
-        case_add(w.counters, ccase, kCntWallHits, (unsigned long long)to.hits);
-        case_add(w.counters, ccase, kCntTrackErrors, (unsigned long long)to.errs);
+        if (active) {
+            tcount_add(tc, w.counters, ccase, 3, 1ull);
+            tcount_add(tc, w.counters, ccase, 0, alive ? 0ull : 1ull);
+            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
+            tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
+        }
     }
+    tcount_block_flush(tc, w.counters);
 }
 
 // --------------------------------------------------------------- move
 
+// Ballistic move of the existing simulators.  Specular walls, cyclic and
+// deletion faces are handled in line; a simulator reaching a diffuse wall
+// stops there and is queued for k_wall, which keeps this kernel free of the
+// FP64 transcendental code of the wall model.
 template <class R>
-__global__ void k_move(WorldView<R> w)
+__global__ void __launch_bounds__(256) k_move(WorldView<R> w)
 {
+    TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
     const uint32_t n0 = w.st->n;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
@@ -229,7 +289,7 @@ __global__ void k_move(WorldView<R> w)
         bool active = t < n0;
         uint32_t cell = kDead;
         int ccase = 0;
-        TrackOut to{1, 0, 0};
+        TrackOut to{1, 0, 0, 0, 0, 0};
         if (active) {
             cell = w.a.cell[t];
             ccase = w.cell_case[cell];
@@ -239,20 +299,76 @@ __global__ void k_move(WorldView<R> w)
             double erot = 0.0;
             Rng r;
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
-            to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w, t, cs.dt, r);
+            R tau = (R)cs.dt;
+            to = track_t<true, R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w, t, tau, r, 0);
             w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
             w.a.cell[t] = cell;
             if (to.hits) {
                 w.a.vx[t] = (R)U[0]; w.a.vy[t] = (R)U[1]; w.a.vz[t] = (R)U[2];
                 w.a.erot[t] = (R)erot;
             }
+            if (to.queued) {
+                const uint32_t q = atomicAdd(&w.st->nwall, 1u);
+                if (q < w.wq_cap) {
+                    w.wq_slot[q] = t;
+                    w.wq_info[q] = (uint32_t)to.qface | ((uint32_t)to.qiter << 3);
+                    w.wq_tau[q] = (double)tau;
+                } else {
+                    w.st->overflow = 1u;
+                }
+            }
         }
-        bool alive = active && cell != kDead;
-        wave_agg_inc(w.cnt, alive ? cell : 0u, alive);
-        case_add(w.counters, ccase, kCntDeleted, (active && !alive) ? 1ull : 0ull);
-        case_add(w.counters, ccase, kCntWallHits, (unsigned long long)to.hits);
-        case_add(w.counters, ccase, kCntTrackErrors, (unsigned long long)to.errs);
+        bool done = active && cell != kDead && !to.queued;
+        wave_agg_inc(w.cnt, done ? cell : 0u, done);
+        if (active) {
+            tcount_add(tc, w.counters, ccase, 0, (cell == kDead) ? 1ull : 0ull);
+            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
+            tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
+        }
     }
+    tcount_block_flush(tc, w.counters);
+}
+
+// Diffuse-wall continuation of the queued tracks.
+template <class R>
+__global__ void k_wall(WorldView<R> w)
+{
+    TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
+    const uint32_t nq = min(w.st->nwall, w.wq_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += stride) {
+        uint32_t q = base + threadIdx.x;
+        bool active = q < nq;
+        uint32_t cell = kDead;
+        int ccase = 0;
+        TrackOut to{1, 0, 0, 0, 0, 0};
+        if (active) {
+            const uint32_t t = w.wq_slot[q];
+            const uint32_t info = w.wq_info[q];
+            cell = w.a.cell[t];
+            ccase = w.cell_case[cell];
+            const CaseDev& cs = w.cases[ccase];
+            R x[3] = {w.a.x[t], w.a.y[t], w.a.z[t]};
+            double U[3] = {(double)w.a.vx[t], (double)w.a.vy[t], (double)w.a.vz[t]};
+            double erot = 0.0;
+            Rng r;
+            rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
+            to = finish_diffuse<R>(w.geom, w.bm, cs.mask, cell, (int)(info & 7u), (int)(info >> 3), x, U, erot,
+                                   w, t, (R)w.wq_tau[q], r);
+            w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
+            w.a.cell[t] = cell;
+            w.a.vx[t] = (R)U[0]; w.a.vy[t] = (R)U[1]; w.a.vz[t] = (R)U[2];
+            w.a.erot[t] = (R)erot;
+        }
+        bool done = active && cell != kDead;
+        wave_agg_inc(w.cnt, done ? cell : 0u, done);
+        if (active) {
+            tcount_add(tc, w.counters, ccase, 0, (cell == kDead) ? 1ull : 0ull);
+            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
+            tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
+        }
+    }
+    tcount_block_flush(tc, w.counters);
 }
 
 // Histogram only (sort after fill / upload).
@@ -375,93 +491,79 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32
     return lo;
 }
 
-// Sorts every cell's slot list ascending (the stable counting-sort order).
-// Tile t owns the cells whose first slot lies in [t*H, (t+1)*H).
+// ------------------------------------------------------- cell accessors
+// The per-cell physics reads and writes simulators through an accessor:
+// LdsAcc for a tile staged in LDS, GlobAcc for the sorted buffers in HBM
+// (tiles too large for LDS).  Both address simulators by sorted slot.
+
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_segsort(WorldView<R> w)
-{
-    __shared__ unsigned long long key[kTileCap];
-    const uint32_t nn = w.st->n_new;
-    const uint32_t start = blockIdx.x * kTileH;
-    if (start >= nn) return;
-    const uint32_t ncp = (uint32_t)w.nc + 1u;
-    uint32_t c0 = lower_bound_u32(w.off, ncp, start);
-    uint32_t c1 = lower_bound_u32(w.off, ncp, start + kTileH);
-    if (c0 > (uint32_t)w.nc) c0 = (uint32_t)w.nc;
-    if (c1 > (uint32_t)w.nc) c1 = (uint32_t)w.nc;
-    const uint32_t p0 = w.off[c0], p1 = w.off[c1];
-    const uint32_t size = p1 - p0;
-    if (size <= 1u) return;
-    if (size <= kTileCap) {
-        uint32_t P = 2;
-        while (P < size) P <<= 1;
-        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-            unsigned long long k = ~0ull;
-            if (i < size) {
-                uint32_t slot = w.perm[p0 + i];
-                k = ((unsigned long long)(w.a.cell[slot] - c0) << 32) | slot;
-            }
-            key[i] = k;
-        }
-        __syncthreads();
-        for (uint32_t kk = 2; kk <= P; kk <<= 1) {
-            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-                    uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        unsigned long long a = key[i], b = key[ixj];
-                        bool up = (i & kk) == 0;
-                        if ((a > b) == up) { key[i] = b; key[ixj] = a; }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) w.perm[p0 + i] = (uint32_t)key[i];
-    } else {
-        // A cell larger than one tile: insertion sort per cell in global memory.
-        if (threadIdx.x == 0) atomicAdd(&w.st->big_tiles, 1u);
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-            uint32_t b = w.off[c], e = w.off[c + 1];
-            for (uint32_t i = b + 1; i < e; ++i) {
-                uint32_t v = w.perm[i];
-                uint32_t j = i;
-                while (j > b && w.perm[j - 1] > v) { w.perm[j] = w.perm[j - 1]; --j; }
-                w.perm[j] = v;
-            }
-        }
+struct LdsAcc {
+    R *vx, *vy, *vz, *er;
+    uint8_t* sl;
+    uint16_t* idx;
+    uint32_t p0;
+    __device__ void load(uint32_t i, double& a, double& b, double& c) const
+    {
+        const uint32_t k = i - p0;
+        a = (double)vx[k]; b = (double)vy[k]; c = (double)vz[k];
     }
-}
-
-template <class R>
-__global__ void k_gather(WorldView<R> w)
-{
-    const uint32_t nn = w.st->n_new;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nn; j += stride) {
-        uint32_t i = w.perm[j];
-        w.b.x[j] = w.a.x[i]; w.b.y[j] = w.a.y[i]; w.b.z[j] = w.a.z[i];
-        w.b.vx[j] = w.a.vx[i]; w.b.vy[j] = w.a.vy[i]; w.b.vz[j] = w.a.vz[i];
-        w.b.erot[j] = w.a.erot[i];
-        w.b.cell[j] = w.a.cell[i];
-        w.b.sp[j] = w.a.sp[i];
+    __device__ void store(uint32_t i, double a, double b, double c) const
+    {
+        const uint32_t k = i - p0;
+        vx[k] = (R)a; vy[k] = (R)b; vz[k] = (R)c;
     }
-}
+    __device__ double erot(uint32_t i) const { return (double)er[i - p0]; }
+    __device__ void set_erot(uint32_t i, double e) const { er[i - p0] = (R)e; }
+    __device__ int local_species(uint32_t i) const { return sl[i - p0]; }
+    __device__ uint32_t idx_get(uint32_t b, uint32_t k) const { return idx[b - p0 + k]; }
+    __device__ void idx_set(uint32_t b, uint32_t k, uint32_t v) const { idx[b - p0 + k] = (uint16_t)v; }
+};
 
-// ------------------------------------------------------------- sample
-
-// Per-cell sums of cellMeasurements::calculateFields in sorted order.
 template <class R>
-__device__ inline void cell_sums(const Parts<R>& p, const Species& sp, uint32_t b, uint32_t e, double S[kNSamp])
+struct GlobAcc {
+    Parts<R> p;
+    const int32_t* local;
+    uint32_t* idx;
+    __device__ void load(uint32_t i, double& a, double& b, double& c) const
+    {
+        a = (double)p.vx[i]; b = (double)p.vy[i]; c = (double)p.vz[i];
+    }
+    __device__ void store(uint32_t i, double a, double b, double c) const
+    {
+        p.vx[i] = (R)a; p.vy[i] = (R)b; p.vz[i] = (R)c;
+    }
+    __device__ double erot(uint32_t i) const { return (double)p.erot[i]; }
+    __device__ void set_erot(uint32_t i, double e) const { p.erot[i] = (R)e; }
+    __device__ int local_species(uint32_t i) const { return local[p.sp[i]]; }
+    __device__ uint32_t idx_get(uint32_t b, uint32_t k) const { return idx[b + k]; }
+    __device__ void idx_set(uint32_t b, uint32_t k, uint32_t v) const { idx[b + k] = v; }
+};
+
+// Per-cell sums of cellMeasurements::calculateFields (cellMeasurements.C:408-513),
+// one simulator after another in sorted order.  L selects the rows that are
+// needed: 0 rows 0-7, 1 adds the second moments 8-13, 2 adds 14-19; the
+// per-species rows follow the same sequence (a single-species case copies
+// rows N and MCC, which are the same sums).
+template <int L, class R, class Acc>
+__device__ inline void cell_sums_l(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b, uint32_t e,
+                                   double S[kNSamp])
 {
 #pragma unroll
     for (int k = 0; k < kNSamp; ++k) S[k] = 0.0;
+    const bool multi = cs.ns > 1;
+    const double m1 = w.sp.mass[cs.sp_base];
+    const double rd1 = (double)w.sp.rotdof[cs.sp_base];
     for (uint32_t i = b; i < e; ++i) {
-        const uint32_t g = p.sp[i];
-        const int s = sp.local[g];
-        const double m = sp.mass[g];
-        const double vx = (double)p.vx[i], vy = (double)p.vy[i], vz = (double)p.vz[i];
-        const double er = (double)p.erot[i];
+        int s = 0;
+        double m = m1, rd = rd1;
+        if (multi) {
+            s = A.local_species(i);
+            m = w.sp.mass[cs.sp_base + s];
+            rd = (double)w.sp.rotdof[cs.sp_base + s];
+        }
+        double vx, vy, vz;
+        A.load(i, vx, vy, vz);
+        const double er = A.erot(i);
         const double mcc = m * (vx * vx + vy * vy + vz * vz);
         S[UGX_S_N] += 1.0;
         S[UGX_S_M] += m;
@@ -470,289 +572,494 @@ __device__ inline void cell_sums(const Parts<R>& p, const Species& sp, uint32_t 
         S[UGX_S_MW] += m * vz;
         S[UGX_S_MCC] += mcc;
         S[UGX_S_EROT] += er;
-        S[UGX_S_ROTDOF] += (double)sp.rotdof[g];
-        S[UGX_S_MUU] += m * (vx * vx);
-        S[UGX_S_MUV] += m * (vx * vy);
-        S[UGX_S_MUW] += m * (vx * vz);
-        S[UGX_S_MVV] += m * (vy * vy);
-        S[UGX_S_MVW] += m * (vy * vz);
-        S[UGX_S_MWW] += m * (vz * vz);
-        S[UGX_S_MCCU] += mcc * vx;
-        S[UGX_S_MCCV] += mcc * vy;
-        S[UGX_S_MCCW] += mcc * vz;
-        S[UGX_S_EROTU] += er * vx;
-        S[UGX_S_EROTV] += er * vy;
-        S[UGX_S_EROTW] += er * vz;
+        S[UGX_S_ROTDOF] += rd;
+        if (L >= 1) {
+            S[UGX_S_MUU] += m * (vx * vx);
+            S[UGX_S_MUV] += m * (vx * vy);
+            S[UGX_S_MUW] += m * (vx * vz);
+            S[UGX_S_MVV] += m * (vy * vy);
+            S[UGX_S_MVW] += m * (vy * vz);
+            S[UGX_S_MWW] += m * (vz * vz);
+        }
+        if (L >= 2) {
+            S[UGX_S_MCCU] += mcc * vx;
+            S[UGX_S_MCCV] += mcc * vy;
+            S[UGX_S_MCCW] += mcc * vz;
+            S[UGX_S_EROTU] += er * vx;
+            S[UGX_S_EROTV] += er * vy;
+            S[UGX_S_EROTW] += er * vz;
+        }
+        if (multi) {
 #pragma unroll
-        for (int q = 0; q < kNS; ++q)
-            if (s == q) { S[UGX_S_SPECIES_N + q] += 1.0; S[UGX_S_SPECIES_MCC + q] += mcc; }
+            for (int q = 0; q < kNS; ++q)
+                if (s == q) { S[UGX_S_SPECIES_N + q] += 1.0; S[UGX_S_SPECIES_MCC + q] += mcc; }
+        }
+    }
+    if (!multi) {
+        S[UGX_S_SPECIES_N] = S[UGX_S_N];
+        S[UGX_S_SPECIES_MCC] = S[UGX_S_MCC];
     }
 }
 
-template <class R>
-__global__ void k_sample(WorldView<R> w)
+template <class R, class Acc>
+__device__ inline void cell_sums(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b, uint32_t e,
+                                 double S[kNSamp])
 {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += stride) {
+    if (cs.sum_level >= 2) cell_sums_l<2, R, Acc>(w, cs, A, b, e, S);
+    else if (cs.sum_level == 1) cell_sums_l<1, R, Acc>(w, cs, A, b, e, S);
+    else cell_sums_l<0, R, Acc>(w, cs, A, b, e, S);
+}
+
+// noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell level 1.
+template <class R, class Acc>
+__device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
+                                const Acc& A, unsigned long long& cand, unsigned long long& coll)
+{
+    Rng r;
+    rng_init(r, cs.seed_lo, cs.seed_hi, kRngNtc, 0u, c - (uint32_t)cs.cell_begin, cs.step);
+    const double smax0 = w.smax[c];
+    double smax = smax0;
+    const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * smax0 * cs.dt / w.vol[c];
+    const double fl = floor(sel);
+    long long ncand = (long long)fl;
+    if (uniform(r) < sel - fl) ncand += 1;
+    cand += (unsigned long long)ncand;
+    for (long long k = 0; k < ncand; ++k) {
+        uint32_t P = uindex(r, nC);
+        uint32_t Q;
+        do { Q = uindex(r, nC); } while (Q == P);
+        const uint32_t ip = b + P, iq = b + Q;
+        const int lp = A.local_species(ip), lq = A.local_species(iq);
+        const int gp = cs.sp_base + lp, gq = cs.sp_base + lq;
+        const int pk = cs.pair_base + lp * cs.ns + lq;
+        double UP[3], UQ[3];
+        A.load(ip, UP[0], UP[1], UP[2]);
+        A.load(iq, UQ[0], UQ[1], UQ[2]);
+        double d0 = UP[0] - UQ[0], d1 = UP[1] - UQ[1], d2 = UP[2] - UQ[2];
+        double cR2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double scr = (cR2 < kVSmall) ? 0.0 : w.pr.C[pk] * dexp(w.pr.e[pk] * dlog(cR2));
+        if (scr > smax) smax = scr;
+        if (scr / smax0 > uniform(r)) {
+            double eP = A.erot(ip), eQ = A.erot(iq);
+            binary_collide(r, cs.dsmc_model, cs.inv_zrot, w.sp.rotdof[gp], w.sp.rotdof[gq], w.sp.mass[gp],
+                           w.sp.mass[gq], w.pr.mr[pk], w.pr.chi[pk], w.pr.alpha[pk], UP, UQ, eP, eQ);
+            A.store(ip, UP[0], UP[1], UP[2]);
+            A.store(iq, UQ[0], UQ[1], UQ[2]);
+            A.set_erot(ip, eP);
+            A.set_erot(iq, eQ);
+            coll += 1;
+        }
+    }
+    w.smax[c] = smax;
+}
+
+// BGK-family relaxation (stochasticParticleBGK.C:287-796,
+// stochasticParticleESBGK.C:311-906, unifiedStochasticParticleSBGK.C:379-1096)
+// in three phases so that the velocity draws run one simulator per lane:
+//  A  (lane per cell)  moments, Pr, nu, USP q/sigma time relaxation, the
+//     number of relaxed simulators and their selection (partial Fisher-Yates
+//     on the cell stream); parameters go to a per-cell table;
+//  B  (lane per selected simulator) new velocity from its own stream
+//     (kRngBgkV, sub = selection index); USP draws test against the cell's
+//     maxProb at the start of the step and record the largest raise;
+//  C  (lane per cell)  conserveMomentumAndEnergy, maxProb update.
+constexpr uint32_t kSelNone = 0xFFFFFFFFu;
+constexpr uint32_t kRound = 64;       // cells per phase round in a tile
+constexpr uint32_t kRngBgkV = 7;
+
+struct BgkP {
+    double U[3];
+    double s2kT, p, cQ, cS, m0, T;
+    double q[3];
+    double sm[9];      // ES-BGK S matrix (row major) or USP stress (xx xy xz yy yz zz)
+    int32_t perform;
+    int32_t model;
+};
+
+template <class R, class Acc>
+__device__ inline void bgk_prepare(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
+                                   const double S[kNSamp], const Acc& A, uint32_t* sel, uint32_t sel_base,
+                                   uint32_t slot, BgkP& P, unsigned long long& nrel_out)
+{
+    const int model = cs.bgk_model;
+    const double FN = cs.fn, dt = cs.dt, V = w.vol[c];
+    const double N = S[UGX_S_N];
+    const bool usp = model == UGX_BGK_USP;
+    bool perform = true;
+    double T = 0.0, p = 0.0, U[3] = {0.0, 0.0, 0.0};
+    double Pt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double q[3], sh[6];
+    for (int d = 0; d < 3; ++d) q[d] = w.uspq[3 * (size_t)c + d];
+    for (int d = 0; d < 6; ++d) sh[d] = w.usps[6 * (size_t)c + d];
+    double Pr = 0.0, nu = 0.0;
+    if (N > kVSmall) {
+        const double rhoN = FN * N / V;
+        for (int d = 0; d < 3; ++d) U[d] = S[UGX_S_MU + d] / S[UGX_S_M];
+        const double uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+        T = (S[UGX_S_MCC] - S[UGX_S_M] * uu) / (3.0 * kB * N);
+        p = rhoN * kB * T;
+        const int ia[6] = {0, 0, 0, 1, 1, 2};
+        const int ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            Pt[k] = rhoN * (S[UGX_S_MUU + k] / N - (S[UGX_S_M] / N) * U[ia[k]] * U[ib[k]]);
+        if (usp) {
+            const double tr = (Pt[0] + Pt[3] + Pt[5]) / 3.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sh[k] = Pt[k];
+            sh[0] = sh[0] - tr; sh[3] = sh[3] - tr; sh[5] = sh[5] - tr;
+            const int r0[3] = {0, 1, 2}, r1[3] = {1, 3, 4}, r2[3] = {2, 4, 5};
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                q[d] = rhoN * (0.5 * (S[UGX_S_MCCU + d] / N) - 0.5 * (S[UGX_S_MCC] / N) * U[d]
+                               + S[UGX_S_EROTU + d] / N - (S[UGX_S_EROT] / N) * U[d])
+                       - Pt[r0[d]] * U[0] - Pt[r1[d]] * U[1] - Pt[r2[d]] * U[2];
+        }
+        if (N > (usp ? 2.0 : 1.0)) {
+            const double f1 = N / (N - 1.0);
+            p = f1 * p;
+            T = f1 * T;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Pt[k] = f1 * Pt[k];
+            if (usp) {
+                const double f2 = N * N / (N - 1.0) / (N - 2.0);
+                for (int d = 0; d < 3; ++d) q[d] = f2 * q[d];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sh[k] = f1 * sh[k];
+            }
+        } else {
+            perform = false;
+        }
+        if (T > kVSmall) {
+            double visc = 0.0;
+            for (int s = 0; s < cs.ns; ++s) {
+                const int g = cs.sp_base + s;
+                double nsv = 0.0;
+#pragma unroll
+                for (int qq = 0; qq < kNS; ++qq) if (qq == s) nsv = S[UGX_S_SPECIES_N + qq];
+                visc = visc + nsv * w.sp.muref[g] * dpow(T / cs.tref, w.sp.omega[g]);
+                Pr = Pr + nsv * w.sp.prterm[g];
+            }
+            visc = visc / N;
+            Pr = Pr / N;
+            nu = (model == UGX_BGK_ESBGK) ? Pr * p / visc : p / visc;
+        } else {
+            perform = false;
+        }
+    } else {
+        perform = false;
+    }
+    if (usp) {
+        const double th = cs.theta;
+        for (int d = 0; d < 3; ++d)
+            q[d] = th * q[d] / (1.0 + 0.5 * Pr * nu * dt) + (1.0 - th) * w.uspq[3 * (size_t)c + d];
+        for (int k = 0; k < 6; ++k)
+            sh[k] = th * sh[k] / (1.0 + 0.5 * nu * dt) + (1.0 - th) * w.usps[6 * (size_t)c + k];
+        for (int d = 0; d < 3; ++d) w.uspq[3 * (size_t)c + d] = q[d];
+        for (int k = 0; k < 6; ++k) w.usps[6 * (size_t)c + k] = sh[k];
+    }
+    P.model = model;
+    P.m0 = w.uspmax[c];
+    P.perform = 0;
+    if (perform && w.cmodel[c] == 0) {
+        P.perform = 1;
+        Rng r;
+        rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
+        const double pc = N * (1.0 - dexp(-nu * dt));
+        const double fl = floor(pc);
+        long long nrel = (long long)fl;
+        if (uniform(r) < pc - fl) nrel += 1;
+        if (nrel > (long long)nC) nrel = nC;
+        nrel_out += (unsigned long long)nrel;
+        for (uint32_t i = 0; i < nC; ++i) A.idx_set(b, i, i);
+        for (long long i = 0; i < nrel; ++i) {
+            const uint32_t j = (uint32_t)i + uindex(r, (uint32_t)((long long)nC - i));
+            const uint32_t ti = A.idx_get(b, (uint32_t)i), tj = A.idx_get(b, j);
+            A.idx_set(b, (uint32_t)i, tj);
+            A.idx_set(b, j, ti);
+            sel[b - sel_base + tj] = ((uint32_t)i << 6) | slot;
+        }
+        for (int d = 0; d < 3; ++d) { P.U[d] = U[d]; P.q[d] = q[d]; }
+        P.T = T;
+        P.p = p;
+        P.s2kT = sqrt(2.0 * kB * T);
+        const double tau = 0.5 * nu * dt;
+        const double ex = 1.0 + 2.0 / (dexp(2.0 * tau) - 1.0);
+        P.cQ = 2.0 * (1.0 - Pr * tau * ex);
+        P.cS = 1.0 - tau * ex;
+        if (model == UGX_BGK_ESBGK) {
+            const double a = 0.5 * (1.0 - Pr) / Pr;
+            const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const double dl = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+                P.sm[k] = dl - a * (Pt[full[k]] / p - dl);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) P.sm[k] = sh[k];
+        }
+    }
+}
+
+// Phase B: the new velocity of one selected simulator (global slot i).
+template <class R, class Acc>
+__device__ inline void bgk_sample(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
+                                  const BgkP& P, unsigned long long* raise, const Acc& A)
+{
+    Rng rv;
+    rng_init(rv, cs.seed_lo, cs.seed_hi, kRngBgkV, k, c - (uint32_t)cs.cell_begin, cs.step);
+    const double u0 = P.s2kT / sqrt(w.sp.mass[cs.sp_base + A.local_species(i)]);
+    double xi[3];
+    if (P.model == UGX_BGK_USP) {
+        const double m0 = P.m0, p = P.p, cQ = P.cQ, cS = P.cS;
+        for (int it = 0; it < 100000; ++it) {
+            double g0, g1, g2;
+            gauss3(rv, g0, g1, g2);
+            xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
+            const double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+            const double vTr = vSq / 3.0;
+            const double* sh = P.sm;
+            const double prob = 1.0 + cQ / (p * u0) * (P.q[0] * xi[0] + P.q[1] * xi[1] + P.q[2] * xi[2]) * (vSq / 2.5 - 1.0)
+                                + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
+                                            + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
+                                            + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
+            if (prob > m0 && prob < 10.0) {
+                atomicMax(raise, (unsigned long long)double_to_bits(prob));
+                break;
+            }
+            if (uniform(rv) < prob / m0) break;
+        }
+    } else {
+        double g0, g1, g2;
+        gauss3(rv, g0, g1, g2);
+        xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
+        if (P.model == UGX_BGK_ESBGK) {
+            double y[3];
+            for (int d = 0; d < 3; ++d) y[d] = P.sm[3 * d] * xi[0] + P.sm[3 * d + 1] * xi[1] + P.sm[3 * d + 2] * xi[2];
+            xi[0] = y[0]; xi[1] = y[1]; xi[2] = y[2];
+        }
+    }
+    A.store(i, P.U[0] + u0 * xi[0], P.U[1] + u0 * xi[1], P.U[2] + u0 * xi[2]);
+}
+
+// Phase C: conserveMomentumAndEnergy and the USP maxProb update.
+template <class R, class Acc>
+__device__ inline void bgk_finish(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
+                                  const double S[kNSamp], const BgkP& P, unsigned long long raise_bits, const Acc& A)
+{
+    if (P.perform) {
+        const double N = S[UGX_S_N];
+        double ke = 0.0, mom0 = 0.0, mom1 = 0.0, mom2 = 0.0;
+        for (uint32_t i = b; i < b + nC; ++i) {
+            const double m = w.sp.mass[cs.sp_base + A.local_species(i)];
+            double vx, vy, vz;
+            A.load(i, vx, vy, vz);
+            ke += m * (vx * vx + vy * vy + vz * vz);
+            mom0 += m * vx; mom1 += m * vy; mom2 += m * vz;
+        }
+        const double pU0 = mom0 / S[UGX_S_M], pU1 = mom1 / S[UGX_S_M], pU2 = mom2 / S[UGX_S_M];
+        const double pT = (ke - S[UGX_S_M] * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
+        if (pT > kVSmall) {
+            const double sc = sqrt(P.T / pT);
+            for (uint32_t i = b; i < b + nC; ++i) {
+                double vx, vy, vz;
+                A.load(i, vx, vy, vz);
+                A.store(i, P.U[0] + (vx - pU0) * sc, P.U[1] + (vy - pU1) * sc, P.U[2] + (vz - pU2) * sc);
+            }
+        }
+    }
+    if (P.model == UGX_BGK_USP) {
+        const double raise = bits_to_double(raise_bits);
+        w.uspmax[c] = (raise > 0.0) ? raise : P.m0 * 0.999;
+    }
+}
+
+// Round of up to kRound cells [cb, ce) of a tile: sampler, BGK family
+// (phases A, B, C) and NTC.  sel is indexed by slot - sel_base.
+template <class R, class Acc>
+__device__ inline void cell_round(const WorldView<R>& w, uint32_t cb, uint32_t ce, const Acc& A, uint32_t* sel,
+                                  uint32_t sel_base, BgkP* bp, unsigned long long* raise,
+                                  unsigned long long (*s_cnt)[3], int case0)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rb = w.off[cb], re = w.off[ce];
+    for (uint32_t i = rb + tid; i < re; i += kBlock) sel[i - sel_base] = kSelNone;
+    __syncthreads();
+    double S[kNSamp];
+    unsigned long long cnt[3] = {0ull, 0ull, 0ull};
+    // cell slot k of the round runs on wave k % 4, lane k / 4: all four SIMDs share the round
+    const uint32_t wv = tid >> 6, ln = tid & 63u;
+    const uint32_t slot = (ln < kRound / 4) ? ln * 4u + wv : kRound;
+    const uint32_t c = cb + slot;
+    const bool mine = slot < kRound && c < ce;
+    bool bgk = false;
+    uint32_t b = 0, nC = 0;
+    const CaseDev* csp = nullptr;
+    if (mine) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
-        if (!cs.active) continue;
-        const bool sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
-        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
-        if (!sample && !hybrid) continue;
-        double S[kNSamp];
-        cell_sums<R>(w.b, w.sp, w.off[c], w.off[c + 1], S);
-        const double dt = cs.dt;
-        if (sample) {
+        csp = &cs;
+        raise[slot] = 0ull;
+        bp[slot].perform = 0;
+        bp[slot].model = UGX_BGK_NONE;
+        if (cs.active) {
+            b = w.off[c];
+            nC = w.off[c + 1] - b;
+            const bool sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
+            const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
+            bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+            if (sample || hybrid || bgk) {
+                cell_sums<R>(w, cs, A, b, b + nC, S);
+                const double dt = cs.dt;
+                if (sample) {
 #pragma unroll
-            for (int k = 0; k < kNSamp; ++k) w.samp[(size_t)k * w.nc + c] += dt * S[k];
-        }
-        if (hybrid) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) w.knacc[(size_t)8 * c + k] += dt * S[k];
-        }
-    }
-}
-
-// ---------------------------------------------------------------- NTC
-
-template <class R>
-__global__ void k_ntc(WorldView<R> w)
-{
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t cb = blockIdx.x * blockDim.x; cb < (uint32_t)w.nc; cb += stride) {
-        uint32_t c = cb + threadIdx.x;
-        unsigned long long cand = 0, coll = 0;
-        int ccase = 0;
-        if (c < (uint32_t)w.nc) {
-            ccase = w.cell_case[c];
-            const CaseDev& cs = w.cases[ccase];
-            const uint32_t b = w.off[c];
-            const uint32_t nC = w.off[c + 1] - b;
-            if (cs.active && (cs.coll_model == UGX_COLL_DSMC || cs.coll_model == UGX_COLL_HYBRID)
-                && cs.dsmc_model != UGX_DSMC_NONE && w.cmodel[c] == 1 && nC >= 2) {
-                Rng r;
-                rng_init(r, cs.seed_lo, cs.seed_hi, kRngNtc, 0u, c - (uint32_t)cs.cell_begin, cs.step);
-                const double smax0 = w.smax[c];
-                double smax = smax0;
-                const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * smax0 * cs.dt / w.vol[c];
-                const double fl = floor(sel);
-                long long ncand = (long long)fl;
-                if (uniform(r) < sel - fl) ncand += 1;
-                cand = (unsigned long long)ncand;
-                for (long long k = 0; k < ncand; ++k) {
-                    uint32_t P = uindex(r, nC);
-                    uint32_t Q;
-                    do { Q = uindex(r, nC); } while (Q == P);
-                    const uint32_t ip = b + P, iq = b + Q;
-                    const uint32_t gp = w.b.sp[ip], gq = w.b.sp[iq];
-                    const int pk = cs.pair_base + w.sp.local[gp] * cs.ns + w.sp.local[gq];
-                    double UP[3] = {(double)w.b.vx[ip], (double)w.b.vy[ip], (double)w.b.vz[ip]};
-                    double UQ[3] = {(double)w.b.vx[iq], (double)w.b.vy[iq], (double)w.b.vz[iq]};
-                    double d0 = UP[0] - UQ[0], d1 = UP[1] - UQ[1], d2 = UP[2] - UQ[2];
-                    double cR2 = d0 * d0 + d1 * d1 + d2 * d2;
-                    double scr = (cR2 < kVSmall) ? 0.0 : w.pr.C[pk] * dexp(w.pr.e[pk] * dlog(cR2));
-                    if (scr > smax) smax = scr;
-                    if (scr / smax0 > uniform(r)) {
-                        double eP = (double)w.b.erot[ip], eQ = (double)w.b.erot[iq];
-                        binary_collide(r, cs.dsmc_model, cs.inv_zrot, w.sp.rotdof[gp], w.sp.rotdof[gq],
-                                       w.sp.mass[gp], w.sp.mass[gq], w.pr.mr[pk], w.pr.chi[pk], w.pr.alpha[pk],
-                                       UP, UQ, eP, eQ);
-                        w.b.vx[ip] = (R)UP[0]; w.b.vy[ip] = (R)UP[1]; w.b.vz[ip] = (R)UP[2];
-                        w.b.vx[iq] = (R)UQ[0]; w.b.vy[iq] = (R)UQ[1]; w.b.vz[iq] = (R)UQ[2];
-                        w.b.erot[ip] = (R)eP; w.b.erot[iq] = (R)eQ;
-                        coll += 1;
-                    }
+                    for (int k = 0; k < kNSamp; ++k)
+                        if (cs.sampler_full || k < UGX_S_MUU || k >= UGX_S_BASE)
+                            w.samp[(size_t)k * w.nc + c] += dt * S[k];
                 }
-                w.smax[c] = smax;
+                if (hybrid) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) w.knacc[(size_t)8 * c + k] += dt * S[k];
+                }
+                if (bgk) bgk_prepare<R>(w, cs, c, b, nC, S, A, sel, sel_base, slot, bp[slot], cnt[2]);
             }
         }
-        case_add(w.counters, ccase, kCntCandidates, cand);
-        case_add(w.counters, ccase, kCntCollisions, coll);
     }
+    __syncthreads();
+    for (uint32_t i = rb + tid; i < re; i += kBlock) {
+        const uint32_t mark = sel[i - sel_base];
+        if (mark == kSelNone) continue;
+        const uint32_t sl = mark & 63u;
+        const uint32_t cc = cb + sl;
+        bgk_sample<R>(w, w.cases[w.cell_case[cc]], cc, i, mark >> 6, bp[sl], &raise[sl], A);
+    }
+    __syncthreads();
+    if (mine && csp->active) {
+        const CaseDev& cs = *csp;
+        if (bgk) bgk_finish<R>(w, cs, c, b, nC, S, bp[slot], raise[slot], A);
+        const bool dsmc = (cs.coll_model == UGX_COLL_DSMC || cs.coll_model == UGX_COLL_HYBRID)
+                          && cs.dsmc_model != UGX_DSMC_NONE;
+        if (dsmc && w.cmodel[c] == 1 && nC >= 2) ntc_cell<R>(w, cs, c, b, nC, A, cnt[0], cnt[1]);
+        const int k = w.cell_case[c] - case0;
+        for (int q = 0; q < 3; ++q) {
+            if (!cnt[q]) continue;
+            if (k >= 0 && k < 4) atomicAdd(&s_cnt[k][q], cnt[q]);
+            else atomicAdd(&w.counters[(size_t)w.cell_case[c] * kCntNum + (q == 0 ? kCntCandidates : q == 1 ? kCntCollisions : kCntBgk)], cnt[q]);
+        }
+    }
+    __syncthreads();
 }
 
-// ---------------------------------------------------------------- BGK
+constexpr uint32_t kTileN = 512;     // nominal simulators per cell tile
+constexpr uint32_t kTileMax = 1024;  // LDS capacity of a tile
 
-template <class R>
-__global__ void k_bgk(WorldView<R> w)
+// One tile = the cells whose first sorted slot lies in [t*kTileN, (t+1)*kTileN)
+// (the last tile also owns the trailing empty cells).  Steps:
+//  1. each cell's slot list (filled by k_scatter in atomic order) is put in
+//     ascending previous-slot order: an entry's rank in its cell is the
+//     number of entries of that cell with a smaller slot;
+//  2. records are gathered into the sorted buffers (x, cell, species to HBM,
+//     velocity, E_rot and species to LDS);
+//  3. lane per cell: sampler, BGK, NTC on the LDS copy;
+//  4. velocities and E_rot are written back.
+// SortOnly skips 3.  Tiles larger than kTileMax run the same steps from HBM.
+template <class R, bool SortOnly>
+__global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
 {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t cb = blockIdx.x * blockDim.x; cb < (uint32_t)w.nc; cb += stride) {
-        uint32_t c = cb + threadIdx.x;
-        unsigned long long nrel_out = 0;
-        int ccase = 0;
-        if (c < (uint32_t)w.nc) {
-            ccase = w.cell_case[c];
-            const CaseDev& cs = w.cases[ccase];
-            if (cs.active && (cs.coll_model == UGX_COLL_BGK || cs.coll_model == UGX_COLL_HYBRID)
-                && cs.bgk_model != UGX_BGK_NONE) {
-                const int model = cs.bgk_model;
-                const uint32_t b = w.off[c];
-                const uint32_t nC = w.off[c + 1] - b;
-                double S[kNSamp];
-                cell_sums<R>(w.b, w.sp, b, b + nC, S);
-                const double FN = cs.fn, dt = cs.dt, V = w.vol[c];
-                const double N = S[UGX_S_N];
-                const bool usp = model == UGX_BGK_USP;
-                bool perform = true;
-                double T = 0.0, p = 0.0, U[3] = {0.0, 0.0, 0.0};
-                double Pt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                double q[3], sh[6];
-                for (int d = 0; d < 3; ++d) q[d] = w.uspq[3 * (size_t)c + d];
-                for (int d = 0; d < 6; ++d) sh[d] = w.usps[6 * (size_t)c + d];
-                double Pr = 0.0, nu = 0.0;
-                if (N > kVSmall) {
-                    const double rhoN = FN * N / V;
-                    for (int d = 0; d < 3; ++d) U[d] = S[UGX_S_MU + d] / S[UGX_S_M];
-                    const double uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
-                    T = (S[UGX_S_MCC] - S[UGX_S_M] * uu) / (3.0 * kB * N);
-                    p = rhoN * kB * T;
-                    const int ia[6] = {0, 0, 0, 1, 1, 2};
-                    const int ib[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-                    for (int k = 0; k < 6; ++k)
-                        Pt[k] = rhoN * (S[UGX_S_MUU + k] / N - (S[UGX_S_M] / N) * U[ia[k]] * U[ib[k]]);
-                    if (usp) {
-                        const double tr = (Pt[0] + Pt[3] + Pt[5]) / 3.0;
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) sh[k] = Pt[k];
-                        sh[0] = sh[0] - tr; sh[3] = sh[3] - tr; sh[5] = sh[5] - tr;
-                        const int r0[3] = {0, 1, 2}, r1[3] = {1, 3, 4}, r2[3] = {2, 4, 5};
-#pragma unroll
-                        for (int d = 0; d < 3; ++d)
-                            q[d] = rhoN * (0.5 * (S[UGX_S_MCCU + d] / N) - 0.5 * (S[UGX_S_MCC] / N) * U[d]
-                                           + S[UGX_S_EROTU + d] / N - (S[UGX_S_EROT] / N) * U[d])
-                                   - Pt[r0[d]] * U[0] - Pt[r1[d]] * U[1] - Pt[r2[d]] * U[2];
-                    }
-                    if (N > (usp ? 2.0 : 1.0)) {
-                        const double f1 = N / (N - 1.0);
-                        p = f1 * p;
-                        T = f1 * T;
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) Pt[k] = f1 * Pt[k];
-                        if (usp) {
-                            const double f2 = N * N / (N - 1.0) / (N - 2.0);
-                            for (int d = 0; d < 3; ++d) q[d] = f2 * q[d];
-#pragma unroll
-                            for (int k = 0; k < 6; ++k) sh[k] = f1 * sh[k];
-                        }
-                    } else {
-                        perform = false;
-                    }
-                    if (T > kVSmall) {
-                        double visc = 0.0;
-                        for (int s = 0; s < cs.ns; ++s) {
-                            const int g = cs.sp_base + s;
-                            double nsv = 0.0;
-#pragma unroll
-                            for (int qq = 0; qq < kNS; ++qq) if (qq == s) nsv = S[UGX_S_SPECIES_N + qq];
-                            visc = visc + nsv * w.sp.muref[g] * dpow(T / cs.tref, w.sp.omega[g]);
-                            Pr = Pr + nsv * w.sp.prterm[g];
-                        }
-                        visc = visc / N;
-                        Pr = Pr / N;
-                        nu = (model == UGX_BGK_ESBGK) ? Pr * p / visc : p / visc;
-                    } else {
-                        perform = false;
-                    }
-                } else {
-                    perform = false;
-                }
-                if (usp) {
-                    const double th = cs.theta;
-                    for (int d = 0; d < 3; ++d)
-                        q[d] = th * q[d] / (1.0 + 0.5 * Pr * nu * dt) + (1.0 - th) * w.uspq[3 * (size_t)c + d];
-                    for (int k = 0; k < 6; ++k)
-                        sh[k] = th * sh[k] / (1.0 + 0.5 * nu * dt) + (1.0 - th) * w.usps[6 * (size_t)c + k];
-                    for (int d = 0; d < 3; ++d) w.uspq[3 * (size_t)c + d] = q[d];
-                    for (int k = 0; k < 6; ++k) w.usps[6 * (size_t)c + k] = sh[k];
-                }
-                bool raised = false;
-                if (perform && w.cmodel[c] == 0) {
-                    Rng r;
-                    rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
-                    const double pc = N * (1.0 - dexp(-nu * dt));
-                    const double fl = floor(pc);
-                    long long nrel = (long long)fl;
-                    if (uniform(r) < pc - fl) nrel += 1;
-                    if (nrel > (long long)nC) nrel = nC;
-                    nrel_out = (unsigned long long)nrel;
-                    uint32_t* idx = w.scratch + b;
-                    for (uint32_t i = 0; i < nC; ++i) idx[i] = i;
-                    double Sm[9];
-                    if (model == UGX_BGK_ESBGK) {
-                        const double a = 0.5 * (1.0 - Pr) / Pr;
-                        const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-#pragma unroll
-                        for (int k = 0; k < 9; ++k) {
-                            const double dl = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
-                            Sm[k] = dl - a * (Pt[full[k]] / p - dl);
-                        }
-                    }
-                    const double tau = 0.5 * nu * dt;
-                    const double ex = 1.0 + 2.0 / (dexp(2.0 * tau) - 1.0);
-                    const double cQ = 2.0 * (1.0 - Pr * tau * ex);
-                    const double cS = 1.0 - tau * ex;
-                    double pmax = w.uspmax[c];
-                    for (long long i = 0; i < nrel; ++i) {
-                        const uint32_t j = (uint32_t)i + uindex(r, (uint32_t)((long long)nC - i));
-                        const uint32_t tmp = idx[i]; idx[i] = idx[j]; idx[j] = tmp;
-                        const uint32_t pi = b + idx[i];
-                        const double m = w.sp.mass[w.b.sp[pi]];
-                        const double u0 = sqrt(2.0 * kB * T / m);
-                        double xi[3];
-                        if (usp) {
-                            for (int it = 0; it < 100000; ++it) {
-                                double g0, g1, g2;
-                                gauss3(r, g0, g1, g2);
-                                xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
-                                const double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-                                const double vTr = vSq / 3.0;
-                                const double prob = 1.0 + cQ / (p * u0) * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq / 2.5 - 1.0)
-                                                    + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
-                                                                + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
-                                                                + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
-                                if (prob > pmax && prob < 10.0) { pmax = prob; raised = true; break; }
-                                if (uniform(r) < prob / pmax) break;
-                            }
-                        } else {
-                            double g0, g1, g2;
-                            gauss3(r, g0, g1, g2);
-                            xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
-                            if (model == UGX_BGK_ESBGK) {
-                                double y[3];
-                                for (int d = 0; d < 3; ++d) y[d] = Sm[3 * d] * xi[0] + Sm[3 * d + 1] * xi[1] + Sm[3 * d + 2] * xi[2];
-                                xi[0] = y[0]; xi[1] = y[1]; xi[2] = y[2];
-                            }
-                        }
-                        w.b.vx[pi] = (R)(U[0] + u0 * xi[0]);
-                        w.b.vy[pi] = (R)(U[1] + u0 * xi[1]);
-                        w.b.vz[pi] = (R)(U[2] + u0 * xi[2]);
-                    }
-                    w.uspmax[c] = pmax;
-                    // conserveMomentumAndEnergy
-                    double ke = 0.0, mom0 = 0.0, mom1 = 0.0, mom2 = 0.0;
-                    for (uint32_t i = b; i < b + nC; ++i) {
-                        const double m = w.sp.mass[w.b.sp[i]];
-                        const double vx = (double)w.b.vx[i], vy = (double)w.b.vy[i], vz = (double)w.b.vz[i];
-                        ke += m * (vx * vx + vy * vy + vz * vz);
-                        mom0 += m * vx; mom1 += m * vy; mom2 += m * vz;
-                    }
-                    const double pU0 = mom0 / S[UGX_S_M], pU1 = mom1 / S[UGX_S_M], pU2 = mom2 / S[UGX_S_M];
-                    const double pT = (ke - S[UGX_S_M] * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
-                    if (pT > kVSmall) {
-                        const double sc = sqrt(T / pT);
-                        for (uint32_t i = b; i < b + nC; ++i) {
-                            w.b.vx[i] = (R)(U[0] + ((double)w.b.vx[i] - pU0) * sc);
-                            w.b.vy[i] = (R)(U[1] + ((double)w.b.vy[i] - pU1) * sc);
-                            w.b.vz[i] = (R)(U[2] + ((double)w.b.vz[i] - pU2) * sc);
-                        }
-                    }
-                }
-                if (usp && !raised) w.uspmax[c] = w.uspmax[c] * 0.999;
+    __shared__ uint32_t s_in[kTileMax];
+    __shared__ uint32_t s_perm[kTileMax];
+    __shared__ R s_vx[SortOnly ? 1 : kTileMax], s_vy[SortOnly ? 1 : kTileMax], s_vz[SortOnly ? 1 : kTileMax];
+    __shared__ R s_er[SortOnly ? 1 : kTileMax];
+    __shared__ uint16_t s_idx[SortOnly ? 1 : kTileMax];
+    __shared__ uint8_t s_sl[SortOnly ? 1 : kTileMax];
+    __shared__ unsigned long long s_cnt[4][3];
+    __shared__ BgkP s_bp[SortOnly ? 1 : kRound];
+    __shared__ unsigned long long s_raise[SortOnly ? 1 : kRound];
+    const uint32_t nn = w.st->n_new;
+    const uint32_t ntiles = nn / kTileN + 1u;
+    if (blockIdx.x >= ntiles) return;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    const uint32_t start = blockIdx.x * kTileN;
+    uint32_t c0 = lower_bound_u32(w.off, ncp, start);
+    uint32_t c1 = (blockIdx.x == ntiles - 1u) ? (uint32_t)w.nc : lower_bound_u32(w.off, ncp, start + kTileN);
+    if (c0 > (uint32_t)w.nc) c0 = (uint32_t)w.nc;
+    if (c1 > (uint32_t)w.nc) c1 = (uint32_t)w.nc;
+    const uint32_t p0 = w.off[c0], p1 = w.off[c1];
+    const uint32_t size = p1 - p0;
+    const int case0 = (c0 < (uint32_t)w.nc) ? w.cell_case[c0] : 0;
+    if (tid < 12) s_cnt[tid / 3][tid % 3] = 0ull;
+    if (size <= kTileMax) {
+        for (uint32_t i = tid; i < size; i += kBlock) s_in[i] = w.perm[p0 + i];
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = s_in[i];
+            const uint32_t c = w.a.cell[slot];
+            const uint32_t b = w.off[c] - p0, e = w.off[c + 1] - p0;
+            uint32_t r = 0;
+            for (uint32_t j = b; j < e; ++j) r += (s_in[j] < slot) ? 1u : 0u;
+            s_perm[b + r] = slot;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = s_perm[i];
+            const uint32_t j = p0 + i;
+            w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
+            const uint32_t g = w.a.sp[slot];
+            w.b.cell[j] = w.a.cell[slot];
+            w.b.sp[j] = g;
+            if (SortOnly) {
+                w.b.vx[j] = w.a.vx[slot]; w.b.vy[j] = w.a.vy[slot]; w.b.vz[j] = w.a.vz[slot];
+                w.b.erot[j] = w.a.erot[slot];
+            } else {
+                s_vx[i] = w.a.vx[slot]; s_vy[i] = w.a.vy[slot]; s_vz[i] = w.a.vz[slot];
+                s_er[i] = w.a.erot[slot];
+                s_sl[i] = (uint8_t)w.sp.local[g];
             }
         }
-        case_add(w.counters, ccase, kCntBgk, nrel_out);
+        if (SortOnly) return;
+        __syncthreads();
+        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_idx, p0};
+        for (uint32_t cb = c0; cb < c1; cb += kRound)
+            cell_round<R>(w, cb, min(cb + kRound, c1), A, s_in, p0, s_bp, s_raise, s_cnt, case0);
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t j = p0 + i;
+            w.b.vx[j] = s_vx[i]; w.b.vy[j] = s_vy[i]; w.b.vz[j] = s_vz[i];
+            w.b.erot[j] = s_er[i];
+        }
+    } else {
+        // Tile larger than LDS: the same steps on HBM, slot lists ordered into scratch.
+        if (tid == 0) atomicAdd(&w.st->big_tiles, 1u);
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = w.perm[p0 + i];
+            const uint32_t c = w.a.cell[slot];
+            const uint32_t b = w.off[c], e = w.off[c + 1];
+            uint32_t r = 0;
+            for (uint32_t j = b; j < e; ++j) r += (w.perm[j] < slot) ? 1u : 0u;
+            w.scratch[b + r] = slot;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = w.scratch[p0 + i];
+            const uint32_t j = p0 + i;
+            w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
+            w.b.vx[j] = w.a.vx[slot]; w.b.vy[j] = w.a.vy[slot]; w.b.vz[j] = w.a.vz[slot];
+            w.b.erot[j] = w.a.erot[slot];
+            w.b.cell[j] = w.a.cell[slot];
+            w.b.sp[j] = w.a.sp[slot];
+        }
+        if (SortOnly) return;
+        __syncthreads();
+        GlobAcc<R> A{w.b, w.sp.local, w.perm};
+        for (uint32_t cb = c0; cb < c1; cb += kRound)
+            cell_round<R>(w, cb, min(cb + kRound, c1), A, w.scratch, 0u, s_bp, s_raise, s_cnt, case0);
+    }
+    __syncthreads();
+    if (tid < 12) {
+        const int k = tid / 3, q = tid % 3;
+        const unsigned long long v = s_cnt[k][q];
+        if (v && case0 + k < w.ncases)
+            atomicAdd(&w.counters[(size_t)(case0 + k) * kCntNum + (q == 0 ? kCntCandidates : q == 1 ? kCntCollisions : kCntBgk)], v);
     }
 }
 
@@ -885,16 +1192,7 @@ static inline unsigned grid_for(uint64_t n, unsigned cap_blocks = 8192)
     return (unsigned)g;
 }
 
-template <class R>
-static void launch_sort_tail(WorldView<R> w, hipStream_t s)
-{
-    const uint32_t ncp = (uint32_t)w.nc + 1u;
-    scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
-    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
-    const unsigned tiles = (unsigned)((w.cap + kTileH - 1) / kTileH) + 1u;
-    hipLaunchKernelGGL(k_segsort<R>, dim3(tiles), dim3(kBlock), 0, s, w);
-    hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
-}
+static inline unsigned tile_grid(uint32_t cap) { return (unsigned)(cap / kTileN) + 2u; }
 
 template <class R>
 void launch_sort(const WorldView<R>& w, hipStream_t s)
@@ -903,7 +1201,9 @@ void launch_sort(const WorldView<R>& w, hipStream_t s)
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cursor, ncp);
     hipLaunchKernelGGL(k_hist<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
-    launch_sort_tail(w, s);
+    scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
+    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL((k_tile<R, true>), dim3(tile_grid(w.cap)), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(k_finish_sort<R>, dim3(1), dim3(1), 0, s, w);
 }
 
@@ -926,25 +1226,16 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     mark(2);
     hipLaunchKernelGGL(k_move<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
     mark(3);
-    scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
+    hipLaunchKernelGGL(k_wall<R>, dim3(grid_for(w.wq_cap, 2048)), dim3(kBlock), 0, s, w);
     mark(4);
-    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
     mark(5);
-    const unsigned tiles = (unsigned)((w.cap + kTileH - 1) / kTileH) + 1u;
-    hipLaunchKernelGGL(k_segsort<R>, dim3(tiles), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
     mark(6);
-    hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL((k_tile<R, false>), dim3(tile_grid(w.cap)), dim3(kBlock), 0, s, w);
     mark(7);
-    // from here on the sorted records are in w.b
-    const unsigned gcell = grid_for((uint64_t)w.nc);
-    hipLaunchKernelGGL(k_sample<R>, dim3(gcell), dim3(kBlock), 0, s, w);
-    mark(8);
-    hipLaunchKernelGGL(k_bgk<R>, dim3(gcell), dim3(kBlock), 0, s, w);
-    mark(9);
-    hipLaunchKernelGGL(k_ntc<R>, dim3(gcell), dim3(kBlock), 0, s, w);
-    mark(10);
     hipLaunchKernelGGL(k_end_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
-    mark(11);
+    mark(8);
 }
 
 template <class R>

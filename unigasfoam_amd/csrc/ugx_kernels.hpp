@@ -12,8 +12,9 @@ struct DevState {
     uint32_t n_ins;         // inserted this step
     uint32_t n_new;         // after the sort
     uint32_t overflow;      // capacity exceeded
-    uint32_t big_tiles;     // sort tiles handled by the global-memory path
-    uint32_t pad[3];
+    uint32_t big_tiles;     // cell tiles handled by the global-memory path
+    uint32_t nwall;         // queued diffuse-wall tracks this step
+    uint32_t pad[2];
 };
 
 // Per-case cumulative counters (uint64).
@@ -57,10 +58,15 @@ struct WorldView {
     DevState* st;
     int32_t nc;
     uint32_t cap;
+    // diffuse-wall queue
+    uint32_t* wq_slot;
+    uint32_t* wq_info;
+    double* wq_tau;
+    uint32_t wq_cap;
 };
 
 // Host launchers (ugx_kernels.hip).
-constexpr int kPhases = 11;  // begin, inflow, move, scan, scatter, segsort, gather, sample, bgk, ntc, end
+constexpr int kPhases = 8;  // begin, inflow, move, wall, scan, scatter, cells, end
 template <class R> void launch_step(const WorldView<R>& w, hipStream_t s, hipEvent_t* marks);
 template <class R> void launch_sort(const WorldView<R>& w, hipStream_t s);
 template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase,

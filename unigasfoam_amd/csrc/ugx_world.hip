@@ -297,7 +297,8 @@ void World::release()
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
                      (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
-                     (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st};
+                     (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st,
+                     (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -333,8 +334,16 @@ int World::alloc_particles(uint32_t newcap)
     cur = 0;
     dfree(d_perm);
     dfree(d_scratch);
+    dfree(d_wq_slot);
+    dfree(d_wq_info);
+    dfree(d_wq_tau);
     if (int rc = dalloc(d_perm, newcap)) return rc;
     if (int rc = dalloc(d_scratch, newcap)) return rc;
+    // diffuse-wall queue: at most half of the simulators may reach a diffuse wall in one step
+    wq_cap = std::max<uint32_t>(newcap / 2, 1024u);
+    if (int rc = dalloc(d_wq_slot, wq_cap)) return rc;
+    if (int rc = dalloc(d_wq_info, wq_cap)) return rc;
+    if (int rc = dalloc(d_wq_tau, wq_cap)) return rc;
     cap = newcap;
     return UGX_OK;
 }
@@ -395,6 +404,10 @@ WorldView<R> World::view() const
     w.st = d_st;
     w.nc = nc;
     w.cap = cap;
+    w.wq_slot = d_wq_slot;
+    w.wq_info = d_wq_info;
+    w.wq_tau = d_wq_tau;
+    w.wq_cap = wq_cap;
     return w;
 }
 
@@ -447,6 +460,14 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         d.smoothing_passes = s.model.smoothing_passes;
         d.breakdown_max = s.model.breakdown_max;
         d.kn_theta = s.model.kn_theta;
+        d.sampler_full = s.model.sampler_full ? 1 : 0;
+        {
+            int lvl = d.sampler_full ? 2 : 0;
+            const bool bgk_on = (s.model.collision_model == UGX_COLL_BGK || s.model.collision_model == UGX_COLL_HYBRID);
+            if (bgk_on && s.model.bgk_model == UGX_BGK_ESBGK) lvl = std::max(lvl, 1);
+            if (bgk_on && s.model.bgk_model == UGX_BGK_USP) lvl = 2;
+            d.sum_level = lvl;
+        }
         int cm = (s.model.collision_model == UGX_COLL_DSMC) ? 1 : 0;  // hybrid starts all-BGK (uniGasCloud.C:726-732)
         for (int c = 0; c < g.nc; ++c) {
             int gc = cbase + c;

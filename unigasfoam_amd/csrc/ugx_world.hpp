@@ -78,6 +78,10 @@ struct World {
     uint32_t* d_blocksum = nullptr;
     unsigned long long* d_counters = nullptr;
     DevState* d_st = nullptr;
+    uint32_t* d_wq_slot = nullptr;
+    uint32_t* d_wq_info = nullptr;
+    double* d_wq_tau = nullptr;
+    uint32_t wq_cap = 0;
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;

@@ -31,9 +31,11 @@ def phase_bytes(phase, R, n, n_cells, cand=0, coll=0, n_bgk=0):
         return n_bgk * (2 * (4 * R + 4) + 2 * 3 * R)
     if phase == "scan":        # counts read, offsets written (two passes over counts)
         return n_cells * 12
+    if phase == "cells":       # slot list, gather of the record, sorted record written; per-cell sampler,
+        #                        sigma_max, offsets; collided pairs touch LDS only
+        return n * (4 + 2 * record_bytes(R)) + n_cells * (8 + 2 * 8 * N_SAMPLER_ROWS + 16 + 8)
     return 0
 
 
 def step_bytes(R, n, n_cells, cand=0, coll=0, n_bgk=0):
-    return sum(phase_bytes(p, R, n, n_cells, cand, coll, n_bgk)
-               for p in ("move", "scatter", "segsort", "gather", "sample", "ntc", "bgk", "scan"))
+    return sum(phase_bytes(p, R, n, n_cells, cand, coll, n_bgk) for p in ("move", "scatter", "cells", "scan"))

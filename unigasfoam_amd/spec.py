@@ -68,6 +68,7 @@ class CaseSpec:
     smoothing_passes: int = 5
     breakdown_max: float = 0.05
     kn_theta: float = 0.2
+    sampler_full: bool = False
     seed: int = 12345
 
     def species_index(self, name):
@@ -106,6 +107,7 @@ class CaseSpec:
         m.smoothing_passes = self.smoothing_passes
         m.breakdown_max = self.breakdown_max
         m.kn_theta = self.kn_theta
+        m.sampler_full = 1 if self.sampler_full else 0
         nb = len(self.boundaries)
         bd = (abi.BoundaryDesc * max(nb, 1))()
         for i, b in enumerate(self.boundaries):
