@@ -55,6 +55,7 @@ def cpu_baseline(name, seconds, scale):
     from unigasfoam_amd import workloads
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = orc.set_threads(threads)
+    orc.set_fast_sums(True)
     if name == "nozzle":
         nx, ny, nz = 40, 20, 20
         spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50)
@@ -73,6 +74,7 @@ def cpu_baseline(name, seconds, scale):
         steps += 1
     dt = time.perf_counter() - t0
     ps = o.counters()["particle_steps"] - c0
+    orc.set_fast_sums(False)
     return {"value": ps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"{sample}; {steps} steps in {dt:.1f} s (restated CPU baseline, not uniGasFoam)"}
 

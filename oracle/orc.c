@@ -584,40 +584,80 @@ static void binary_collide(orc_world* w, orc_rng* r, int p, int q, double* UP, d
     scatter(r, vss, w->palpha[k], cR, crc, mP, mQ, UP, UQ);
 }
 
-/* Per-cell sums of cellMeasurements::calculateFields (cellMeasurements.C:408-513),
- * one simulator after another in sorted order. */
+/* Canonical per-cell sum of n values in sorted order: chunks of 64 values
+ * (zero padded), each canonicalised (v + 0.0) and reduced by the xor
+ * butterfly with offsets 32, 16, 8, 4, 2, 1 (every slot l takes
+ * s[l] + s[l ^ off]); the chunk totals are added in order to 0.0.  This is
+ * the order of a wave-cooperative reduction on 64 lanes. */
+/* Timing mode for the CPU baseline: plain sequential sums (the butterfly
+ * order only matters for bit parity with the GPU, not for the physics). */
+static int g_fast_sums = 0;
+void orc_set_fast_sums(int on) { g_fast_sums = on; }
+
+static double canon_sum(const double* v, int64_t n)
+{
+    double acc = 0.0;
+    if (g_fast_sums) {
+        for (int64_t i = 0; i < n; ++i) acc += v[i];
+        return acc;
+    }
+    for (int64_t b = 0; b < n; b += 64) {
+        double s[64], t[64];
+        for (int l = 0; l < 64; ++l) s[l] = (b + l < n) ? v[b + l] + 0.0 : 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
+            for (int l = 0; l < 64; ++l) t[l] = s[l] + s[l ^ off];
+            for (int l = 0; l < 64; ++l) s[l] = t[l];
+        }
+        acc = acc + s[0];
+    }
+    return acc;
+}
+
+/* Per-cell sums of cellMeasurements::calculateFields (cellMeasurements.C:408-513):
+ * the per-simulator contributions of each row, reduced with canon_sum. */
 static void cell_sums(const orc_world* w, int64_t b, int64_t e, double S[NSAMP])
 {
-    for (int k = 0; k < NSAMP; ++k) S[k] = 0.0;
-    for (int64_t i = b; i < e; ++i) {
-        int s = w->sp[i];
+    int64_t n = e - b;
+    double* v = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1) * NSAMP);
+    for (int64_t i = 0; i < n; ++i) {
+        int s = w->sp[b + i];
         double m = w->mass[s];
-        double vx = w->v[3 * i], vy = w->v[3 * i + 1], vz = w->v[3 * i + 2];
-        double er = w->erot[i];
+        double vx = w->v[3 * (b + i)], vy = w->v[3 * (b + i) + 1], vz = w->v[3 * (b + i) + 2];
+        double er = w->erot[b + i];
         double mcc = m * (vx * vx + vy * vy + vz * vz);
-        S[UGX_S_N] += 1.0;
-        S[UGX_S_M] += m;
-        S[UGX_S_MU] += m * vx;
-        S[UGX_S_MV] += m * vy;
-        S[UGX_S_MW] += m * vz;
-        S[UGX_S_MCC] += mcc;
-        S[UGX_S_EROT] += er;
-        S[UGX_S_ROTDOF] += (double)w->rotdof[s];
-        S[UGX_S_MUU] += m * (vx * vx);
-        S[UGX_S_MUV] += m * (vx * vy);
-        S[UGX_S_MUW] += m * (vx * vz);
-        S[UGX_S_MVV] += m * (vy * vy);
-        S[UGX_S_MVW] += m * (vy * vz);
-        S[UGX_S_MWW] += m * (vz * vz);
-        S[UGX_S_MCCU] += mcc * vx;
-        S[UGX_S_MCCV] += mcc * vy;
-        S[UGX_S_MCCW] += mcc * vz;
-        S[UGX_S_EROTU] += er * vx;
-        S[UGX_S_EROTV] += er * vy;
-        S[UGX_S_EROTW] += er * vz;
-        S[UGX_S_SPECIES_N + s] += 1.0;
-        S[UGX_S_SPECIES_MCC + s] += mcc;
+        double* r = &v[(size_t)i * NSAMP];
+        r[UGX_S_N] = 1.0;
+        r[UGX_S_M] = m;
+        r[UGX_S_MU] = m * vx;
+        r[UGX_S_MV] = m * vy;
+        r[UGX_S_MW] = m * vz;
+        r[UGX_S_MCC] = mcc;
+        r[UGX_S_EROT] = er;
+        r[UGX_S_ROTDOF] = (double)w->rotdof[s];
+        r[UGX_S_MUU] = m * (vx * vx);
+        r[UGX_S_MUV] = m * (vx * vy);
+        r[UGX_S_MUW] = m * (vx * vz);
+        r[UGX_S_MVV] = m * (vy * vy);
+        r[UGX_S_MVW] = m * (vy * vz);
+        r[UGX_S_MWW] = m * (vz * vz);
+        r[UGX_S_MCCU] = mcc * vx;
+        r[UGX_S_MCCV] = mcc * vy;
+        r[UGX_S_MCCW] = mcc * vz;
+        r[UGX_S_EROTU] = er * vx;
+        r[UGX_S_EROTV] = er * vy;
+        r[UGX_S_EROTW] = er * vz;
+        for (int q = 0; q < NS; ++q) {
+            r[UGX_S_SPECIES_N + q] = (s == q) ? 1.0 : 0.0;
+            r[UGX_S_SPECIES_MCC + q] = (s == q) ? mcc : 0.0;
+        }
     }
+    double* col = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int k = 0; k < NSAMP; ++k) {
+        for (int64_t i = 0; i < n; ++i) col[i] = v[(size_t)i * NSAMP + k];
+        S[k] = canon_sum(col, n);
+    }
+    free(col);
+    free(v);
 }
 
 /* noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell
@@ -799,13 +839,22 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
             }
             for (int d = 0; d < 3; ++d) w->v[3 * pi + d] = U[d] + u0 * xi[d];
         }
-        /* conserveMomentumAndEnergy */
-        double ke = 0.0, mom[3] = {0.0, 0.0, 0.0};
-        for (int64_t i = b; i < b + nC; ++i) {
-            double m = w->mass[w->sp[i]];
-            double vx = w->v[3 * i], vy = w->v[3 * i + 1], vz = w->v[3 * i + 2];
-            ke += m * (vx * vx + vy * vy + vz * vz);
-            mom[0] += m * vx; mom[1] += m * vy; mom[2] += m * vz;
+        /* conserveMomentumAndEnergy (sums in canonical order) */
+        double ke, mom[3];
+        {
+            double* col = malloc(sizeof(double) * 4 * (size_t)(nC > 0 ? nC : 1));
+            for (int64_t i = 0; i < nC; ++i) {
+                int64_t j = b + i;
+                double m = w->mass[w->sp[j]];
+                double vx = w->v[3 * j], vy = w->v[3 * j + 1], vz = w->v[3 * j + 2];
+                col[i] = m * (vx * vx + vy * vy + vz * vz);
+                col[nC + i] = m * vx;
+                col[2 * nC + i] = m * vy;
+                col[3 * nC + i] = m * vz;
+            }
+            ke = canon_sum(col, nC);
+            for (int d = 0; d < 3; ++d) mom[d] = canon_sum(col + (d + 1) * nC, nC);
+            free(col);
         }
         double pU[3];
         for (int d = 0; d < 3; ++d) pU[d] = mom[d] / S[UGX_S_M];

@@ -51,6 +51,10 @@ int  orc_geometry(orc_world* w, double* cc, double* vol, int32_t* nfaces,
 /* Inflow face-species expected counts per step (Bird 4.22). */
 int  orc_inflow_nbar(orc_world* w, double* out, int64_t len, int64_t* n_out);
 
+/* Sequential per-cell sums instead of the canonical butterfly order (used
+ * only to time the CPU baseline; breaks bit parity with the GPU). */
+void orc_set_fast_sums(int on);
+
 /* Primitive hooks for the known-answer and accuracy tests. */
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_mathfn(int fn, const double* in, const double* in2, double* out, int64_t n);

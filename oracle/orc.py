@@ -49,6 +49,7 @@ def lib():
         L.orc_set_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.orc_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_fast_sums.argtypes = [C.c_int]
         L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
         L.orc_inflow_nbar.argtypes = [vp, abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
         _lib = L
@@ -62,6 +63,10 @@ def _check(rc):
 
 def set_threads(n):
     return lib().orc_set_threads(int(n))
+
+
+def set_fast_sums(on):
+    lib().orc_set_fast_sums(1 if on else 0)
 
 
 class OracleCase:

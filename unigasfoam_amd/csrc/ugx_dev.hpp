@@ -405,9 +405,9 @@ __device__ inline void wall_species(const WV& w, uint32_t t, double& erot, doubl
 // remaining time in tau_io) and report it as queued; the caller finishes
 // the track with the wall model later.  Nothing random is drawn before a
 // diffuse wall, so the continuation uses a freshly initialised stream.
-template <bool Defer, class R, class Src>
+template <bool Defer, class R, class Src, class V = double>
 __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const double mask[3],
-                                   uint32_t& cell, R x[3], double U[3], double& erot,
+                                   uint32_t& cell, R x[3], V U[3], double& erot,
                                    const Src& src, uint32_t slot, R& tau, Rng& rng, int it0)
 {
     double mass = 0.0;
@@ -416,7 +416,7 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
     TrackOut out{1, 0, 0, 0, 0, 0};
     uint32_t c = cell;
     R vt[3];
-    for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * mask[d]);
+    for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * (V)mask[d]);
     for (int it = it0; it < kMaxCrossings; ++it) {
         R tmin = tau;
         int fmin = -1;
@@ -453,7 +453,7 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
         V4<R> q = g.plane[base + fmin];
         double nw[3] = {(double)q.x, (double)q.y, (double)q.z};
         if (b.kind == UGX_BND_DELETION) { cell = kDead; out.alive = 0; return out; }
-        if (Defer && b.kind == UGX_BND_DIFFUSE) {
+        if (Defer && b.kind != UGX_BND_SPECULAR) {
             out.queued = 1;
             out.qface = fmin;
             out.qiter = it;
@@ -461,15 +461,22 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
             return out;
         }
         out.hits += 1;
-        wall_species(src, slot, erot, mass, rotdof, loaded);
         if (b.kind == UGX_BND_SPECULAR) {
-            double udn = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
-            if (udn > 0.0)
-                for (int d = 0; d < 3; ++d) U[d] = U[d] - 2.0 * udn * nw[d];
-        } else {
-            diffuse_reflect(rng, nw, U, erot, b.T, b.U, mass, rotdof);
+            // mirror; E_rot and species are not needed
+            const V n0 = (V)q.x, n1 = (V)q.y, n2 = (V)q.z;
+            V udn = U[0] * n0 + U[1] * n1 + U[2] * n2;
+            if (udn > (V)0) {
+                U[0] = U[0] - (V)2 * udn * n0;
+                U[1] = U[1] - (V)2 * udn * n1;
+                U[2] = U[2] - (V)2 * udn * n2;
+            }
+        } else if constexpr (!Defer) {
+            wall_species(src, slot, erot, mass, rotdof, loaded);
+            double Ud[3] = {(double)U[0], (double)U[1], (double)U[2]};
+            diffuse_reflect(rng, nw, Ud, erot, b.T, b.U, mass, rotdof);
+            U[0] = (V)Ud[0]; U[1] = (V)Ud[1]; U[2] = (V)Ud[2];
         }
-        for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * mask[d]);
+        for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * (V)mask[d]);
     }
     out.errs = 1;
     cell = c;

@@ -273,6 +273,19 @@ __global__ void k_inflow_gen(WorldView<R> w)
 
 // --------------------------------------------------------------- move
 
+// Case of an existing simulator from its slot: the cases own consecutive slot
+// ranges starting at particle_begin (set by k_begin_step from the offsets).
+template <class R>
+__device__ inline int case_of_slot(const WorldView<R>& w, uint32_t t)
+{
+    int lo = 0, hi = w.ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (w.cases[mid].particle_begin <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Ballistic move of the existing simulators.  Specular walls, cyclic and
 // deletion faces are handled in line; a simulator reaching a diffuse wall
 // stops there and is queued for k_wall, which keeps this kernel free of the
@@ -292,20 +305,19 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
         TrackOut to{1, 0, 0, 0, 0, 0};
         if (active) {
             cell = w.a.cell[t];
-            ccase = w.cell_case[cell];
+            ccase = case_of_slot(w, t);
             const CaseDev& cs = w.cases[ccase];
             R x[3] = {w.a.x[t], w.a.y[t], w.a.z[t]};
-            double U[3] = {(double)w.a.vx[t], (double)w.a.vy[t], (double)w.a.vz[t]};
+            R U[3] = {w.a.vx[t], w.a.vy[t], w.a.vz[t]};
             double erot = 0.0;
             Rng r;
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
             R tau = (R)cs.dt;
-            to = track_t<true, R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w, t, tau, r, 0);
+            to = track_t<true, R, WorldView<R>, R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w, t, tau, r, 0);
             w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
             w.a.cell[t] = cell;
             if (to.hits) {
-                w.a.vx[t] = (R)U[0]; w.a.vy[t] = (R)U[1]; w.a.vz[t] = (R)U[2];
-                w.a.erot[t] = (R)erot;
+                w.a.vx[t] = U[0]; w.a.vy[t] = U[1]; w.a.vz[t] = U[2];
             }
             if (to.queued) {
                 const uint32_t q = atomicAdd(&w.st->nwall, 1u);
@@ -539,60 +551,83 @@ struct GlobAcc {
     __device__ void idx_set(uint32_t b, uint32_t k, uint32_t v) const { idx[b + k] = v; }
 };
 
-// Per-cell sums of cellMeasurements::calculateFields (cellMeasurements.C:408-513),
-// one simulator after another in sorted order.  L selects the rows that are
-// needed: 0 rows 0-7, 1 adds the second moments 8-13, 2 adds 14-19; the
-// per-species rows follow the same sequence (a single-species case copies
-// rows N and MCC, which are the same sums).
+// Canonical per-cell sums, wave-cooperative: the cell's simulators are taken
+// 64 at a time (lane l holds simulator chunk + l, missing lanes hold +0.0),
+// every value is canonicalised (v + 0.0) and reduced by the xor butterfly
+// 32, 16, 8, 4, 2, 1; the chunk totals are added in order to 0.0.  Every
+// lane of the wave ends with the totals.  Called by all 64 lanes of a wave
+// with the same [b, e).
+__device__ inline double wave_bfly(double v)
+{
+    v = v + 0.0;
+    v = v + __shfl_xor(v, 32);
+    v = v + __shfl_xor(v, 16);
+    v = v + __shfl_xor(v, 8);
+    v = v + __shfl_xor(v, 4);
+    v = v + __shfl_xor(v, 2);
+    v = v + __shfl_xor(v, 1);
+    return v;
+}
+
+// Rows of cellMeasurements::calculateFields (cellMeasurements.C:408-513).
+// L selects the rows needed: 0 rows 0-7, 1 adds 8-13, 2 adds 14-19; the
+// per-species rows follow (a single-species case copies rows N and MCC,
+// which are the same sums).
 template <int L, class R, class Acc>
-__device__ inline void cell_sums_l(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b, uint32_t e,
-                                   double S[kNSamp])
+__device__ inline void wave_cell_sums_l(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b,
+                                        uint32_t e, double S[kNSamp])
 {
 #pragma unroll
     for (int k = 0; k < kNSamp; ++k) S[k] = 0.0;
+    const uint32_t lane = __lane_id();
     const bool multi = cs.ns > 1;
-    const double m1 = w.sp.mass[cs.sp_base];
-    const double rd1 = (double)w.sp.rotdof[cs.sp_base];
-    for (uint32_t i = b; i < e; ++i) {
-        int s = 0;
-        double m = m1, rd = rd1;
-        if (multi) {
+    for (uint32_t cb = b; cb < e; cb += 64u) {
+        const uint32_t i = cb + lane;
+        const bool on = i < e;
+        double m = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, er = 0.0, rd = 0.0, one = 0.0;
+        int s = -1;
+        if (on) {
             s = A.local_species(i);
-            m = w.sp.mass[cs.sp_base + s];
-            rd = (double)w.sp.rotdof[cs.sp_base + s];
+            const int g = cs.sp_base + s;
+            m = w.sp.mass[g];
+            rd = (double)w.sp.rotdof[g];
+            A.load(i, vx, vy, vz);
+            er = A.erot(i);
+            one = 1.0;
         }
-        double vx, vy, vz;
-        A.load(i, vx, vy, vz);
-        const double er = A.erot(i);
         const double mcc = m * (vx * vx + vy * vy + vz * vz);
-        S[UGX_S_N] += 1.0;
-        S[UGX_S_M] += m;
-        S[UGX_S_MU] += m * vx;
-        S[UGX_S_MV] += m * vy;
-        S[UGX_S_MW] += m * vz;
-        S[UGX_S_MCC] += mcc;
-        S[UGX_S_EROT] += er;
-        S[UGX_S_ROTDOF] += rd;
+        S[UGX_S_N] = S[UGX_S_N] + wave_bfly(one);
+        S[UGX_S_M] = S[UGX_S_M] + wave_bfly(m);
+        S[UGX_S_MU] = S[UGX_S_MU] + wave_bfly(m * vx);
+        S[UGX_S_MV] = S[UGX_S_MV] + wave_bfly(m * vy);
+        S[UGX_S_MW] = S[UGX_S_MW] + wave_bfly(m * vz);
+        S[UGX_S_MCC] = S[UGX_S_MCC] + wave_bfly(mcc);
+        S[UGX_S_EROT] = S[UGX_S_EROT] + wave_bfly(er);
+        S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + wave_bfly(rd);
         if (L >= 1) {
-            S[UGX_S_MUU] += m * (vx * vx);
-            S[UGX_S_MUV] += m * (vx * vy);
-            S[UGX_S_MUW] += m * (vx * vz);
-            S[UGX_S_MVV] += m * (vy * vy);
-            S[UGX_S_MVW] += m * (vy * vz);
-            S[UGX_S_MWW] += m * (vz * vz);
+            S[UGX_S_MUU] = S[UGX_S_MUU] + wave_bfly(m * (vx * vx));
+            S[UGX_S_MUV] = S[UGX_S_MUV] + wave_bfly(m * (vx * vy));
+            S[UGX_S_MUW] = S[UGX_S_MUW] + wave_bfly(m * (vx * vz));
+            S[UGX_S_MVV] = S[UGX_S_MVV] + wave_bfly(m * (vy * vy));
+            S[UGX_S_MVW] = S[UGX_S_MVW] + wave_bfly(m * (vy * vz));
+            S[UGX_S_MWW] = S[UGX_S_MWW] + wave_bfly(m * (vz * vz));
         }
         if (L >= 2) {
-            S[UGX_S_MCCU] += mcc * vx;
-            S[UGX_S_MCCV] += mcc * vy;
-            S[UGX_S_MCCW] += mcc * vz;
-            S[UGX_S_EROTU] += er * vx;
-            S[UGX_S_EROTV] += er * vy;
-            S[UGX_S_EROTW] += er * vz;
+            S[UGX_S_MCCU] = S[UGX_S_MCCU] + wave_bfly(mcc * vx);
+            S[UGX_S_MCCV] = S[UGX_S_MCCV] + wave_bfly(mcc * vy);
+            S[UGX_S_MCCW] = S[UGX_S_MCCW] + wave_bfly(mcc * vz);
+            S[UGX_S_EROTU] = S[UGX_S_EROTU] + wave_bfly(er * vx);
+            S[UGX_S_EROTV] = S[UGX_S_EROTV] + wave_bfly(er * vy);
+            S[UGX_S_EROTW] = S[UGX_S_EROTW] + wave_bfly(er * vz);
         }
         if (multi) {
 #pragma unroll
-            for (int q = 0; q < kNS; ++q)
-                if (s == q) { S[UGX_S_SPECIES_N + q] += 1.0; S[UGX_S_SPECIES_MCC + q] += mcc; }
+            for (int q = 0; q < kNS; ++q) {
+                if (q < cs.ns) {
+                    S[UGX_S_SPECIES_N + q] = S[UGX_S_SPECIES_N + q] + wave_bfly((s == q) ? 1.0 : 0.0);
+                    S[UGX_S_SPECIES_MCC + q] = S[UGX_S_SPECIES_MCC + q] + wave_bfly((s == q) ? mcc : 0.0);
+                }
+            }
         }
     }
     if (!multi) {
@@ -602,12 +637,12 @@ __device__ inline void cell_sums_l(const WorldView<R>& w, const CaseDev& cs, con
 }
 
 template <class R, class Acc>
-__device__ inline void cell_sums(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b, uint32_t e,
-                                 double S[kNSamp])
+__device__ inline void wave_cell_sums(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b, uint32_t e,
+                                      double S[kNSamp])
 {
-    if (cs.sum_level >= 2) cell_sums_l<2, R, Acc>(w, cs, A, b, e, S);
-    else if (cs.sum_level == 1) cell_sums_l<1, R, Acc>(w, cs, A, b, e, S);
-    else cell_sums_l<0, R, Acc>(w, cs, A, b, e, S);
+    if (cs.sum_level >= 2) wave_cell_sums_l<2, R, Acc>(w, cs, A, b, e, S);
+    else if (cs.sum_level == 1) wave_cell_sums_l<1, R, Acc>(w, cs, A, b, e, S);
+    else wave_cell_sums_l<0, R, Acc>(w, cs, A, b, e, S);
 }
 
 // noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell level 1.
@@ -664,7 +699,7 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
 //     maxProb at the start of the step and record the largest raise;
 //  C  (lane per cell)  conserveMomentumAndEnergy, maxProb update.
 constexpr uint32_t kSelNone = 0xFFFFFFFFu;
-constexpr uint32_t kRound = 64;       // cells per phase round in a tile
+constexpr uint32_t kRound = 32;       // cells per phase round in a tile
 constexpr uint32_t kRngBgkV = 7;
 
 struct BgkP {
@@ -678,7 +713,7 @@ struct BgkP {
 
 template <class R, class Acc>
 __device__ inline void bgk_prepare(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
-                                   const double S[kNSamp], const Acc& A, uint32_t* sel, uint32_t sel_base,
+                                   const double* S, const Acc& A, uint32_t* sel, uint32_t sel_base,
                                    uint32_t slot, BgkP& P, unsigned long long& nrel_out)
 {
     const int model = cs.bgk_model;
@@ -843,86 +878,100 @@ __device__ inline void bgk_sample(const WorldView<R>& w, const CaseDev& cs, uint
     A.store(i, P.U[0] + u0 * xi[0], P.U[1] + u0 * xi[1], P.U[2] + u0 * xi[2]);
 }
 
-// Phase C: conserveMomentumAndEnergy and the USP maxProb update.
+// Phase C: conserveMomentumAndEnergy (canonical wave sums) and the USP
+// maxProb update; called by all lanes of one wave for one cell.
 template <class R, class Acc>
 __device__ inline void bgk_finish(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
-                                  const double S[kNSamp], const BgkP& P, unsigned long long raise_bits, const Acc& A)
+                                  const double* S, const BgkP& P, unsigned long long raise_bits, const Acc& A)
 {
+    const uint32_t lane = __lane_id();
     if (P.perform) {
         const double N = S[UGX_S_N];
         double ke = 0.0, mom0 = 0.0, mom1 = 0.0, mom2 = 0.0;
-        for (uint32_t i = b; i < b + nC; ++i) {
-            const double m = w.sp.mass[cs.sp_base + A.local_species(i)];
-            double vx, vy, vz;
-            A.load(i, vx, vy, vz);
-            ke += m * (vx * vx + vy * vy + vz * vz);
-            mom0 += m * vx; mom1 += m * vy; mom2 += m * vz;
+        for (uint32_t cb = b; cb < b + nC; cb += 64u) {
+            const uint32_t i = cb + lane;
+            double m = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+            if (i < b + nC) {
+                m = w.sp.mass[cs.sp_base + A.local_species(i)];
+                A.load(i, vx, vy, vz);
+            }
+            ke = ke + wave_bfly(m * (vx * vx + vy * vy + vz * vz));
+            mom0 = mom0 + wave_bfly(m * vx);
+            mom1 = mom1 + wave_bfly(m * vy);
+            mom2 = mom2 + wave_bfly(m * vz);
         }
         const double pU0 = mom0 / S[UGX_S_M], pU1 = mom1 / S[UGX_S_M], pU2 = mom2 / S[UGX_S_M];
         const double pT = (ke - S[UGX_S_M] * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
         if (pT > kVSmall) {
             const double sc = sqrt(P.T / pT);
-            for (uint32_t i = b; i < b + nC; ++i) {
+            for (uint32_t i = b + lane; i < b + nC; i += 64u) {
                 double vx, vy, vz;
                 A.load(i, vx, vy, vz);
                 A.store(i, P.U[0] + (vx - pU0) * sc, P.U[1] + (vy - pU1) * sc, P.U[2] + (vz - pU2) * sc);
             }
         }
     }
-    if (P.model == UGX_BGK_USP) {
+    if (P.model == UGX_BGK_USP && lane == 0) {
         const double raise = bits_to_double(raise_bits);
         w.uspmax[c] = (raise > 0.0) ? raise : P.m0 * 0.999;
     }
 }
 
-// Round of up to kRound cells [cb, ce) of a tile: sampler, BGK family
-// (phases A, B, C) and NTC.  sel is indexed by slot - sel_base.
+// Round of up to kRound cells [cb, ce) of a tile:
+//  sums    wave per cell: canonical sums into S_tab, sampler and hybrid sums
+//  A       lane per cell: BGK moments, selection (wave 0)
+//  B       lane per selected simulator: new velocities (all waves)
+//  C       wave per cell: conservation, USP maxProb
+//  NTC     lane per cell (wave 0)
+// sel is indexed by slot - sel_base.
 template <class R, class Acc>
 __device__ inline void cell_round(const WorldView<R>& w, uint32_t cb, uint32_t ce, const Acc& A, uint32_t* sel,
-                                  uint32_t sel_base, BgkP* bp, unsigned long long* raise,
+                                  uint32_t sel_base, BgkP* bp, unsigned long long* raise, double (*S_tab)[kNSamp],
                                   unsigned long long (*s_cnt)[3], int case0)
 {
     const uint32_t tid = threadIdx.x;
+    const uint32_t wv = tid >> 6, lane = tid & 63u;
+    const uint32_t ncell = ce - cb;
     const uint32_t rb = w.off[cb], re = w.off[ce];
     for (uint32_t i = rb + tid; i < re; i += kBlock) sel[i - sel_base] = kSelNone;
-    __syncthreads();
-    double S[kNSamp];
-    unsigned long long cnt[3] = {0ull, 0ull, 0ull};
-    // cell slot k of the round runs on wave k % 4, lane k / 4: all four SIMDs share the round
-    const uint32_t wv = tid >> 6, ln = tid & 63u;
-    const uint32_t slot = (ln < kRound / 4) ? ln * 4u + wv : kRound;
-    const uint32_t c = cb + slot;
-    const bool mine = slot < kRound && c < ce;
-    bool bgk = false;
-    uint32_t b = 0, nC = 0;
-    const CaseDev* csp = nullptr;
-    if (mine) {
+    if (tid < kRound) { raise[tid] = 0ull; bp[tid].perform = 0; bp[tid].model = UGX_BGK_NONE; }
+    // sums
+    for (uint32_t k = wv; k < ncell; k += kBlock / 64u) {
+        const uint32_t c = cb + k;
         const CaseDev& cs = w.cases[w.cell_case[c]];
-        csp = &cs;
-        raise[slot] = 0ull;
-        bp[slot].perform = 0;
-        bp[slot].model = UGX_BGK_NONE;
-        if (cs.active) {
-            b = w.off[c];
-            nC = w.off[c + 1] - b;
-            const bool sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
-            const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
-            bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
-            if (sample || hybrid || bgk) {
-                cell_sums<R>(w, cs, A, b, b + nC, S);
-                const double dt = cs.dt;
-                if (sample) {
+        if (!cs.active) continue;
+        const bool sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
+        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
+        const bool bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+        if (!(sample || hybrid || bgk)) continue;
+        double S[kNSamp];
+        wave_cell_sums<R>(w, cs, A, w.off[c], w.off[c + 1], S);
+        if (lane == 0) {
+            const double dt = cs.dt;
 #pragma unroll
-                    for (int k = 0; k < kNSamp; ++k)
-                        if (cs.sampler_full || k < UGX_S_MUU || k >= UGX_S_BASE)
-                            w.samp[(size_t)k * w.nc + c] += dt * S[k];
-                }
-                if (hybrid) {
+            for (int q = 0; q < kNSamp; ++q) S_tab[k][q] = S[q];
+            if (sample) {
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) w.knacc[(size_t)8 * c + k] += dt * S[k];
-                }
-                if (bgk) bgk_prepare<R>(w, cs, c, b, nC, S, A, sel, sel_base, slot, bp[slot], cnt[2]);
+                for (int q = 0; q < kNSamp; ++q)
+                    if (cs.sampler_full || q < UGX_S_MUU || q >= UGX_S_BASE)
+                        w.samp[(size_t)q * w.nc + c] += dt * S[q];
             }
+            if (hybrid) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w.knacc[(size_t)8 * c + q] += dt * S[q];
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long cnt[3] = {0ull, 0ull, 0ull};
+    if (tid < ncell) {
+        const uint32_t c = cb + tid;
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
+        const bool bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+        if (cs.active && bgk) {
+            const uint32_t b = w.off[c];
+            bgk_prepare<R>(w, cs, c, b, w.off[c + 1] - b, S_tab[tid], A, sel, sel_base, tid, bp[tid], cnt[2]);
         }
     }
     __syncthreads();
@@ -934,12 +983,24 @@ __device__ inline void cell_round(const WorldView<R>& w, uint32_t cb, uint32_t c
         bgk_sample<R>(w, w.cases[w.cell_case[cc]], cc, i, mark >> 6, bp[sl], &raise[sl], A);
     }
     __syncthreads();
-    if (mine && csp->active) {
-        const CaseDev& cs = *csp;
-        if (bgk) bgk_finish<R>(w, cs, c, b, nC, S, bp[slot], raise[slot], A);
+    for (uint32_t k = wv; k < ncell; k += kBlock / 64u) {
+        const uint32_t c = cb + k;
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
+        const bool bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+        if (cs.active && bgk) {
+            const uint32_t b = w.off[c];
+            bgk_finish<R>(w, cs, c, b, w.off[c + 1] - b, S_tab[k], bp[k], raise[k], A);
+        }
+    }
+    __syncthreads();
+    if (tid < ncell) {
+        const uint32_t c = cb + tid;
+        const CaseDev& cs = w.cases[w.cell_case[c]];
         const bool dsmc = (cs.coll_model == UGX_COLL_DSMC || cs.coll_model == UGX_COLL_HYBRID)
                           && cs.dsmc_model != UGX_DSMC_NONE;
-        if (dsmc && w.cmodel[c] == 1 && nC >= 2) ntc_cell<R>(w, cs, c, b, nC, A, cnt[0], cnt[1]);
+        const uint32_t b = w.off[c], nC = w.off[c + 1] - b;
+        if (cs.active && dsmc && w.cmodel[c] == 1 && nC >= 2) ntc_cell<R>(w, cs, c, b, nC, A, cnt[0], cnt[1]);
         const int k = w.cell_case[c] - case0;
         for (int q = 0; q < 3; ++q) {
             if (!cnt[q]) continue;
@@ -975,6 +1036,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
     __shared__ unsigned long long s_cnt[4][3];
     __shared__ BgkP s_bp[SortOnly ? 1 : kRound];
     __shared__ unsigned long long s_raise[SortOnly ? 1 : kRound];
+    __shared__ double s_S[SortOnly ? 1 : kRound][kNSamp];
     const uint32_t nn = w.st->n_new;
     const uint32_t ntiles = nn / kTileN + 1u;
     if (blockIdx.x >= ntiles) return;
@@ -1021,7 +1083,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
         __syncthreads();
         LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_idx, p0};
         for (uint32_t cb = c0; cb < c1; cb += kRound)
-            cell_round<R>(w, cb, min(cb + kRound, c1), A, s_in, p0, s_bp, s_raise, s_cnt, case0);
+            cell_round<R>(w, cb, min(cb + kRound, c1), A, s_in, p0, s_bp, s_raise, s_S, s_cnt, case0);
         for (uint32_t i = tid; i < size; i += kBlock) {
             const uint32_t j = p0 + i;
             w.b.vx[j] = s_vx[i]; w.b.vy[j] = s_vy[i]; w.b.vz[j] = s_vz[i];
@@ -1052,7 +1114,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
         __syncthreads();
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
         for (uint32_t cb = c0; cb < c1; cb += kRound)
-            cell_round<R>(w, cb, min(cb + kRound, c1), A, w.scratch, 0u, s_bp, s_raise, s_cnt, case0);
+            cell_round<R>(w, cb, min(cb + kRound, c1), A, w.scratch, 0u, s_bp, s_raise, s_S, s_cnt, case0);
     }
     __syncthreads();
     if (tid < 12) {
