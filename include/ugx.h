@@ -280,8 +280,9 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
  * counterpart in the reference, whose only timing is the ExecutionTime /
  * ClockTime log line of uniGasFoam.C:97-101).  Phases, in launch order:
  * begin, inflow, move, wall (diffuse-wall continuation), scan, scatter,
- * cells (slot order + gather + sampler + BGK + NTC), end. */
-#define UGX_N_PHASES 8
+ * cells (slot order + gather + per-cell sums + sampler), collide (NTC and
+ * the BGK family), end. */
+#define UGX_N_PHASES 9
 const char* ugx_phase_name(int i);
 int ugx_timing_enable(ugx_case* c, int on);       /* also resets the totals */
 int ugx_timing_get(ugx_case* c, double* ms_total, int n_phases, int64_t* steps);

@@ -584,33 +584,30 @@ static void binary_collide(orc_world* w, orc_rng* r, int p, int q, double* UP, d
     scatter(r, vss, w->palpha[k], cR, crc, mP, mQ, UP, UQ);
 }
 
-/* Canonical per-cell sum of n values in sorted order: chunks of 64 values
- * (zero padded), each canonicalised (v + 0.0) and reduced by the xor
- * butterfly with offsets 32, 16, 8, 4, 2, 1 (every slot l takes
- * s[l] + s[l ^ off]); the chunk totals are added in order to 0.0.  This is
- * the order of a wave-cooperative reduction on 64 lanes. */
-/* Timing mode for the CPU baseline: plain sequential sums (the butterfly
+/* Canonical per-cell sum of n values in sorted order: eight partial sums,
+ * partial j adding v[j], v[j+8], v[j+16], ... in order to +0.0, combined by
+ * the tree ((P0+P1)+(P2+P3)) + ((P4+P5)+(P6+P7)).  This is the order of an
+ * eight-lane group reduction on the GPU.  The reference sums in list order
+ * (cellMeasurements.C:417-507); any fixed order is an equally valid sum. */
+/* Timing mode for the CPU baseline: plain sequential sums (the canonical
  * order only matters for bit parity with the GPU, not for the physics). */
 static int g_fast_sums = 0;
 void orc_set_fast_sums(int on) { g_fast_sums = on; }
 
 static double canon_sum(const double* v, int64_t n)
 {
-    double acc = 0.0;
     if (g_fast_sums) {
+        double acc = 0.0;
         for (int64_t i = 0; i < n; ++i) acc += v[i];
         return acc;
     }
-    for (int64_t b = 0; b < n; b += 64) {
-        double s[64], t[64];
-        for (int l = 0; l < 64; ++l) s[l] = (b + l < n) ? v[b + l] + 0.0 : 0.0;
-        for (int off = 32; off > 0; off >>= 1) {
-            for (int l = 0; l < 64; ++l) t[l] = s[l] + s[l ^ off];
-            for (int l = 0; l < 64; ++l) s[l] = t[l];
-        }
-        acc = acc + s[0];
+    double P[8];
+    for (int j = 0; j < 8; ++j) {
+        double s = 0.0;
+        for (int64_t i = j; i < n; i += 8) s = s + v[i];
+        P[j] = s;
     }
-    return acc;
+    return ((P[0] + P[1]) + (P[2] + P[3])) + ((P[4] + P[5]) + (P[6] + P[7]));
 }
 
 /* Per-cell sums of cellMeasurements::calculateFields (cellMeasurements.C:408-513):

@@ -1,0 +1,431 @@
+"""Pins of the CPU oracle against mathematics that does not come from the
+oracle's own formulas: closed forms, invariants, brute force.
+
+Each test targets one restated function of the reference:
+  NTC candidates / VHS rate   noTimeCounter.C:184-191, variableHardSphere.C:72-115
+  collision invariants        variableHardSphere.C:118-162, variableSoftSphere.C:116-176,
+                              LarsenBorgnakkeVariableHardSphere.C:125-416
+  LB equilibrium              uniGasCloud.C:1129-1189
+  BGK-family conservation     stochasticParticleBGK.C:731-780 (and ES-BGK / USP siblings)
+  BGK / ES-BGK limit          stochasticParticleBGK.C:782-796, stochasticParticleESBGK.C:887-906
+  diffuse wall equilibrium    uniGasPatchBoundary.C:305-388
+  inflow count                uniGasGeneralBoundary.C:115-169 (Bird eq. 4.22)
+  ballistic tracking          uniGasParcel.C:35-108, uniGasPatchBoundary.C:390-403
+  occupancy order             CloudWithModels.C:109-138
+  sampler rows                cellMeasurements.C:408-513
+  meshFill                    uniGasMeshFill.C:58-297
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from unigasfoam_amd import abi
+from unigasfoam_amd.mesh import box_mesh, graded_box_mesh
+from unigasfoam_amd.spec import Boundary, CaseSpec, Species
+from unigasfoam_amd.workloads import ARGON, N2, O2
+
+KB = abi.KB
+SIDES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+MOVE, SORT, COLLIDE, HYBRID, ADVANCE = 1, 2, 4, 8, 16
+
+
+def box_case(n=(4, 4, 4), side=0.01, species=(ARGON,), dt=2e-6, fn=1e10, kind=abi.BND_SPECULAR,
+             collision="dsmc", dsmc="variableHardSphere", bgk="noBGKCollision", T_wall=300.0,
+             patch=abi.WALL, seed=7, sampler_full=False, Zrot=5.0, grading=None, bnd=None):
+    pt = {k: patch for k in SIDES}
+    if grading is None:
+        mesh = box_mesh((0.0, 0.0, 0.0), (side, side, side), n, pt)
+    else:
+        mesh = graded_box_mesh((0.0, 0.0, 0.0), (side, side, side), n, grading, pt)
+    if bnd is None:
+        bnd = [] if patch == abi.CYCLIC else [Boundary(p, kind, temperature=T_wall) for p in SIDES]
+    return CaseSpec(mesh=mesh, species=list(species), delta_t=dt, n_equivalent_particles=fn,
+                    collision_model=collision, dsmc_model=dsmc, bgk_model=bgk, Tref=273.0, Zrot=Zrot,
+                    boundaries=bnd, seed=seed, sampler_full=sampler_full)
+
+
+def vhs_mean_sigma_cr(sp, T, Tref=273.0):
+    """<sigma_T c_r> of a simple VHS gas at equilibrium: the a8 cross-section
+    averaged over the relative-speed Maxwellian (Bird 1994 eq. 4.64)."""
+    return 4.0 * sp.diameter ** 2 * math.sqrt(math.pi * KB * Tref / sp.mass) * (T / Tref) ** (1.0 - sp.omega)
+
+
+def totals(p, masses):
+    m = masses[p["species"]]
+    v = p["velocity"]
+    mom = (m[:, None] * v).sum(0)
+    e = 0.5 * (m * (v * v).sum(1)).sum() + p["erot"].sum()
+    scale = (m * np.abs(v).sum(1)).sum()
+    return mom, e, scale
+
+
+# ------------------------------------------------------------------ NTC / VHS
+
+def test_ntc_candidates_and_vhs_rate():
+    n_density, side, ncell, per = 1.0e20, 0.01, 6, 20
+    nsim = per * ncell ** 3
+    fn = n_density * side ** 3 / nsim
+    spec = box_case(n=(ncell,) * 3, side=side, fn=fn)
+    o = orc.OracleCase(spec, capacity=4 * nsim)
+    o.fill({"Ar": n_density}, T=500.0, exact_per_cell=per)
+    vol = o.geometry()["vol"]
+    exp_cand = var_cand = exp_coll = 0.0
+    c0 = o.counters()
+    sig = vhs_mean_sigma_cr(ARGON, 500.0)
+    for _ in range(30):
+        o.step_phases(MOVE | SORT)
+        N = np.bincount(o.particles()["cell"], minlength=spec.mesh.n_cells).astype(float)
+        smax = o.sigma_max()
+        sel = 0.5 * N * (N - 1.0) * fn * smax * spec.delta_t / vol
+        frac = sel - np.floor(sel)
+        exp_cand += sel.sum()
+        var_cand += (frac * (1.0 - frac)).sum()
+        exp_coll += (0.5 * N * (N - 1.0) * fn * sig * spec.delta_t / vol).sum()
+        o.step_phases(COLLIDE | HYBRID | ADVANCE)
+    c1 = o.counters()
+    cand = c1["candidates"] - c0["candidates"]
+    coll = c1["collisions"] - c0["collisions"]
+    # the candidate count is floor + Bernoulli: its mean is the NTC selection number
+    assert abs(cand - exp_cand) <= 5.0 * math.sqrt(var_cand) + 1.0, (cand, exp_cand, var_cand)
+    # accepted collisions follow the closed-form equilibrium rate (Poisson noise)
+    assert abs(coll - exp_coll) <= 4.0 * math.sqrt(exp_coll), (coll, exp_coll)
+    assert exp_coll > 2000
+
+
+# ------------------------------------------------------- collision invariants
+
+@pytest.mark.parametrize("model,species", [
+    ("variableHardSphere", (ARGON,)),
+    ("variableSoftSphere", (Species("ArVSS", 6.63e-26, 4.11e-10, 0.81, alpha=1.4),)),
+    ("LarsenBorgnakkeVariableHardSphere", (N2, O2)),
+    ("LarsenBorgnakkeVariableSoftSphere", (Species("N2v", 4.65e-26, 4.11e-10, 0.74, alpha=1.36, rotational_dof=2),
+                                           Species("O2v", 5.31e-26, 4.01e-10, 0.77, alpha=1.4, rotational_dof=2))),
+])
+def test_collisions_conserve_momentum_and_energy(model, species):
+    spec = box_case(n=(2, 2, 2), species=species, dsmc=model, dt=2e-5, fn=1e11)
+    o = orc.OracleCase(spec, capacity=20000)
+    nd = {s.name: 1.0e21 / len(species) for s in species}
+    o.fill(nd, T=(900.0, 300.0, 450.0), rot_T=200.0, velocity=(50.0, -20.0, 10.0))
+    masses = np.array([s.mass for s in species])
+    p0 = o.particles()
+    mom0, e0, scale = totals(p0, masses)
+    c0 = o.counters()["collisions"]
+    for _ in range(10):
+        o.step_phases(COLLIDE | ADVANCE)      # collisions only: no flight, no walls
+    p1 = o.particles()
+    mom1, e1, _ = totals(p1, masses)
+    assert o.counters()["collisions"] - c0 > 500
+    assert np.all(np.abs(mom1 - mom0) <= 1e-12 * scale)
+    assert abs(e1 - e0) <= 1e-12 * e0
+    # the anisotropic start relaxes towards isotropy
+    v0, v1 = p0["velocity"], p1["velocity"]
+    an0 = v0[:, 0].var() / v0[:, 1].var()
+    an1 = v1[:, 0].var() / v1[:, 1].var()
+    assert an1 < an0
+
+
+def test_lb_rotational_equilibrium():
+    """N2 with T_tr = 600 K and no rotational energy: LB exchange (2 DoF)
+    brings both temperatures to the equipartition value 3/5 * 600 = 360 K."""
+    spec = box_case(n=(2, 2, 2), species=(N2,), dsmc="LarsenBorgnakkeVariableHardSphere", dt=2e-6, fn=1e11)
+    o = orc.OracleCase(spec, capacity=20000)
+    o.fill({"N2": 1.0e21}, T=600.0, rot_T=0.0)
+    for _ in range(150):
+        o.step_phases(COLLIDE | ADVANCE)
+    p = o.particles()
+    v = p["velocity"] - p["velocity"].mean(0)
+    n = len(v)
+    T_tr = N2.mass * (v * v).sum() / (3.0 * KB * n)
+    T_rot = p["erot"].sum() / (KB * n)
+    assert abs(T_tr - 360.0) < 0.03 * 360.0, T_tr
+    assert abs(T_rot - 360.0) < 0.04 * 360.0, T_rot
+
+
+# ------------------------------------------------------------------ BGK family
+
+@pytest.mark.parametrize("bgk", ["stochasticParticleBGK", "stochasticParticleESBGK", "unifiedStochasticParticleSBGK"])
+def test_bgk_conserves_each_cell(bgk):
+    spec = box_case(n=(3, 3, 3), species=(N2,), collision="bgk", bgk=bgk, dt=5e-6, fn=1e11)
+    o = orc.OracleCase(spec, capacity=40000)
+    o.fill({"N2": 1.0e21}, T=(800.0, 300.0, 400.0), rot_T=300.0, velocity=(30.0, 0.0, -10.0))
+    o.step_phases(COLLIDE | ADVANCE)      # first step builds the USP history
+    for _ in range(2):
+        p0 = o.particles()
+        b0 = o.counters()["bgk_collisions"]
+        o.step_phases(COLLIDE | ADVANCE)
+        p1 = o.particles()
+        assert o.counters()["bgk_collisions"] - b0 > 1000
+        assert np.array_equal(p0["cell"], p1["cell"])
+        m = N2.mass
+        nc = spec.mesh.n_cells
+        for c in range(nc):
+            s = p0["cell"] == c
+            v0, v1 = p0["velocity"][s], p1["velocity"][s]
+            assert not np.array_equal(v0, v1)
+            mom0, mom1 = m * v0.sum(0), m * v1.sum(0)
+            e0, e1 = m * (v0 * v0).sum(), m * (v1 * v1).sum()
+            assert np.all(np.abs(mom1 - mom0) <= 1e-12 * m * np.abs(v0).sum()), c
+            assert abs(e1 - e0) <= 1e-12 * e0, c
+            assert np.array_equal(p0["erot"][s], p1["erot"][s])   # E_rot is not relaxed
+
+
+@pytest.mark.parametrize("bgk", ["stochasticParticleBGK", "stochasticParticleESBGK"])
+def test_bgk_strong_relaxation_limit(bgk):
+    """nu dt >> 1: every simulator is redrawn.  BGK gives the isotropic
+    Maxwellian; ES-BGK gives covariance (kT/m) S S^T with
+    S = I - (1-Pr)/(2Pr) (Pi/p - I), then the conservation rescale.  Argon:
+    Pr = 2/3.  Start: T = (1000, 250, 250), T = 500."""
+    spec = box_case(n=(4, 4, 4), species=(ARGON,), collision="bgk", bgk=bgk, dt=1e-3, fn=1e10)
+    o = orc.OracleCase(spec, capacity=40000)
+    o.fill({"Ar": 1.0e20}, T=(1000.0, 250.0, 250.0), exact_per_cell=200)
+    o.step_phases(COLLIDE | ADVANCE)
+    p = o.particles()
+    v = p["velocity"]
+    T = np.array([ARGON.mass * ((v[:, d] - v[:, d].mean()) ** 2).mean() / KB for d in range(3)])
+    if bgk == "stochasticParticleBGK":
+        expect = np.array([500.0, 500.0, 500.0])
+    else:
+        a = 0.5 * (1.0 - 2.0 / 3.0) / (2.0 / 3.0)
+        S = 1.0 - a * (np.array([1000.0, 250.0, 250.0]) / 500.0 - 1.0)
+        Ts = 500.0 * S ** 2
+        expect = Ts * 500.0 / Ts.mean()
+    assert np.all(np.abs(T - expect) < 0.05 * expect), (T, expect)
+
+
+# ------------------------------------------------------------------ walls
+
+def test_diffuse_walls_thermalise_gas():
+    """A closed box with fully accommodating walls at T_w holds a Maxwellian
+    at T_w in its volume, translational and rotational."""
+    spec = box_case(n=(2, 2, 2), side=1e-3, species=(N2,), kind=abi.BND_DIFFUSE, T_wall=300.0,
+                    collision="none", dt=1e-7, fn=1e3)
+    o = orc.OracleCase(spec, capacity=40000)
+    o.fill({"N2": 1.0e16}, T=700.0, rot_T=50.0)
+    o.step(400)
+    p = o.particles()
+    v = p["velocity"]
+    n = len(v)
+    assert n > 5000
+    assert o.counters()["wall_hits"] > 10 * n
+    T_tr = N2.mass * (v * v).sum() / (3.0 * KB * n)
+    T_rot = p["erot"].sum() / (KB * n)
+    tol = 4.0 * math.sqrt(2.0 / (3.0 * n)) * 300.0 + 3.0
+    assert abs(T_tr - 300.0) < tol, T_tr
+    assert abs(T_rot - 300.0) < 4.0 * 300.0 / math.sqrt(n) + 3.0, T_rot
+
+
+def test_specular_box_conserves_n_and_energy():
+    spec = box_case(n=(3, 3, 3), collision="none", dt=1e-6)
+    o = orc.OracleCase(spec, capacity=20000)
+    o.fill({"Ar": 1e19}, T=400.0)
+    p0 = o.particles()
+    e0 = (p0["velocity"] ** 2).sum()
+    o.step(50)
+    p1 = o.particles()
+    assert len(p1["cell"]) == len(p0["cell"])
+    assert abs((p1["velocity"] ** 2).sum() - e0) <= 1e-13 * e0
+
+
+# ------------------------------------------------------------------ inflow
+
+def test_inflow_count_bird_4_22():
+    side, dt, fn, n_in, T_in, U_in = 0.01, 1e-6, 1e9, 1e20, 250.0, 300.0
+    pt = {k: abi.WALL for k in SIDES}
+    pt["xmin"] = abi.PATCH
+    pt["xmax"] = abi.PATCH
+    mesh = graded_box_mesh((0.0, 0.0, 0.0), (side, side, side), (3, 4, 2), (1.0, 2.0, 0.5), pt)
+    bnd = [Boundary("xmin", abi.BND_FREESTREAM, temperature=T_in, velocity=(U_in, 0.0, 0.0),
+                    number_density={"N2": 0.7 * n_in, "O2": 0.3 * n_in}),
+           Boundary("xmax", abi.BND_DELETION)] + [Boundary(p, abi.BND_SPECULAR) for p in SIDES[2:]]
+    spec = CaseSpec(mesh=mesh, species=[N2, O2], delta_t=dt, n_equivalent_particles=fn,
+                    collision_model="none", boundaries=bnd, seed=3)
+    o = orc.OracleCase(spec, capacity=200000)
+    nbar = o.geometry()["nbar"]
+    # independent Bird 4.22 from the inlet face areas
+    ys = spec.mesh.points
+    pi = spec.mesh.patch_index("xmin")
+    patch = spec.mesh.patches[pi]
+    exp = []
+    for f in range(patch.start, patch.start + patch.size):
+        v = ys[spec.mesh.face_points[spec.mesh.face_offsets[f]:spec.mesh.face_offsets[f + 1]]]
+        area = np.linalg.norm(np.cross(v[2] - v[0], v[3] - v[1])) * 0.5
+        for sp, frac in ((N2, 0.7), (O2, 0.3)):
+            cmp = math.sqrt(2.0 * KB * T_in / sp.mass)
+            s = U_in / cmp
+            flux = frac * n_in * cmp * (math.exp(-s * s) + math.sqrt(math.pi) * s * (1.0 + math.erf(s))) \
+                / (2.0 * math.sqrt(math.pi))
+            exp.append(flux * area * dt / fn)
+    exp = np.array(exp)
+    assert len(nbar) == len(exp)
+    assert np.allclose(np.sort(nbar), np.sort(exp), rtol=1e-12, atol=0.0)
+    # inserted counts are floor + Bernoulli: unbiased
+    steps = 200
+    c0 = o.counters()["inserted"]
+    o.step(steps)
+    ins = o.counters()["inserted"] - c0
+    frac = nbar - np.floor(nbar)
+    mean = steps * nbar.sum()
+    assert abs(ins - mean) <= 5.0 * math.sqrt(steps * (frac * (1 - frac)).sum()) + 1.0
+
+
+# ------------------------------------------------------------------ tracking
+
+def fold(x, L):
+    y = np.mod(x, 2.0 * L)
+    flip = y > L
+    return np.where(flip, 2.0 * L - y, y), np.floor_divide(x, L).astype(np.int64) % 2 == 1
+
+
+def brute_cells(pos, edges):
+    idx = [np.clip(np.searchsorted(edges[a], pos[:, a], side="right") - 1, 0, len(edges[a]) - 2) for a in range(3)]
+    nx, ny = len(edges[0]) - 1, len(edges[1]) - 1
+    return (idx[2] * ny + idx[1]) * nx + idx[0]
+
+
+@pytest.mark.parametrize("patch", [abi.WALL, abi.CYCLIC])
+def test_ballistic_tracking_brute_force(patch):
+    """Collisionless flight over several cells and walls against the exact
+    fold (specular box) or wrap (cyclic box) of the straight line."""
+    L = np.array([0.01, 0.007, 0.005])
+    n = (5, 4, 3)
+    grading = (1.0, 3.0, 0.5)
+    pt = {k: patch for k in SIDES}
+    mesh = graded_box_mesh((0.0, 0.0, 0.0), tuple(L), n, grading, pt)
+    bnd = [] if patch == abi.CYCLIC else [Boundary(p, abi.BND_SPECULAR) for p in SIDES]
+    dt = 2e-5
+    spec = CaseSpec(mesh=mesh, species=[ARGON], delta_t=dt, n_equivalent_particles=1e10,
+                    collision_model="none", boundaries=bnd, seed=4)
+    o = orc.OracleCase(spec, capacity=10000)
+    edges = []
+    for a in range(3):
+        m, g = n[a], grading[a]
+        w = np.ones(m) if g == 1.0 else (g ** (1.0 / (m - 1))) ** np.arange(m)
+        edges.append(np.concatenate([[0.0], np.cumsum(w)]) / w.sum() * L[a])
+    rng = np.random.default_rng(11)
+    N = 3000
+    x0 = rng.uniform(0.02, 0.98, (N, 3)) * L
+    v0 = rng.normal(0.0, 400.0, (N, 3))
+    c0 = brute_cells(x0, edges)
+    o.upload(v0, c0, position=x0)
+    # upload stores the simulators in the occupancy order: stable by cell
+    order = np.argsort(c0, kind="stable")
+    x0, v0 = x0[order], v0[order]
+    assert np.max(np.abs(o.particles()["position"] - x0)) < 1e-15
+    o.step_phases(MOVE)
+    p = o.particles()
+    assert len(p["cell"]) == N
+    t = v0 * dt
+    x = np.empty_like(x0)
+    vx = np.empty_like(v0)
+    for a in range(3):
+        if patch == abi.CYCLIC:
+            x[:, a] = np.mod(x0[:, a] + t[:, a], L[a])
+            vx[:, a] = v0[:, a]
+        else:
+            x[:, a], odd = fold(x0[:, a] + t[:, a], L[a])
+            vx[:, a] = np.where(odd, -v0[:, a], v0[:, a])
+    assert np.max(np.abs(p["position"] - x) / L) < 1e-9
+    assert np.array_equal(p["velocity"], vx)
+    # cell index: exact except within rounding of a face
+    near = np.zeros(N, dtype=bool)
+    for a in range(3):
+        near |= np.min(np.abs(x[:, a][:, None] - edges[a][None, :]), axis=1) < 1e-9 * L[a]
+    assert np.array_equal(p["cell"][~near], brute_cells(x, edges)[~near])
+    assert near.sum() < 10
+
+
+# ------------------------------------------------------------------ occupancy
+
+def test_occupancy_order_is_stable_by_cell():
+    pt = {k: abi.WALL for k in SIDES}
+    pt["xmax"] = abi.PATCH
+    mesh = box_mesh((0.0, 0.0, 0.0), (0.01, 0.01, 0.01), (4, 3, 2), pt)
+    bnd = [Boundary("xmax", abi.BND_DELETION)] + [Boundary(p, abi.BND_SPECULAR) for p in SIDES if p != "xmax"]
+    spec = CaseSpec(mesh=mesh, species=[ARGON], delta_t=4e-6, n_equivalent_particles=1e10,
+                    collision_model="none", boundaries=bnd, seed=5)
+    o = orc.OracleCase(spec, capacity=20000)
+    rng = np.random.default_rng(2)
+    N = 4000
+    x0 = rng.uniform(0.0, 0.01, (N, 3))
+    cells = brute_cells(x0, [np.linspace(0, 0.01, 5), np.linspace(0, 0.01, 4), np.linspace(0, 0.01, 3)])
+    tag = np.arange(N, dtype=float)           # monatomic gas: E_rot is carried untouched
+    o.upload(rng.normal(0.0, 500.0, (N, 3)), cells, erot=tag, position=x0)
+    assert np.array_equal(o.particles()["erot"], tag[np.argsort(cells, kind="stable")])
+    for _ in range(3):
+        o.step_phases(MOVE)
+        p1 = o.particles()
+        o.step_phases(SORT | ADVANCE)
+        p2 = o.particles()
+        alive = p1["cell"] >= 0
+        order = np.argsort(p1["cell"][alive], kind="stable")
+        assert np.array_equal(p2["erot"], p1["erot"][alive][order])
+        assert np.all(np.diff(p2["cell"]) >= 0)
+    assert o.counters()["deleted"] > 0
+
+
+# ------------------------------------------------------------------ sampler
+
+def test_sampler_rows_hand_built():
+    spec = box_case(n=(2, 1, 1), species=(N2, O2), collision="none", dt=3e-6, sampler_full=True)
+    o = orc.OracleCase(spec, capacity=100)
+    v = np.array([[100.0, -50.0, 20.0], [300.0, 10.0, -40.0], [-200.0, 70.0, 5.0],
+                  [15.0, 25.0, -35.0], [0.0, 0.0, 400.0]])
+    cell = np.array([0, 0, 0, 1, 1])
+    sp = np.array([0, 1, 0, 1, 1])
+    er = np.array([1e-21, 2e-21, 0.0, 5e-22, 3e-21])
+    centres = o.geometry()["cc"]
+    o.upload(v, cell, species=sp, erot=er, position=centres[cell])
+    o.step_phases(SORT | COLLIDE | ADVANCE)
+    S = o.samples()
+    m = np.array([N2.mass, O2.mass])[sp]
+    dt = spec.delta_t
+    for c in range(2):
+        k = cell == c
+        vv, mm, ee, ss = v[k], m[k], er[k], sp[k]
+        mcc = mm * (vv * vv).sum(1)
+        expect = np.zeros(abi.S_COUNT)
+        expect[abi.S['N']] = k.sum()
+        expect[abi.S['M']] = mm.sum()
+        expect[abi.S['MU']:abi.S['MU'] + 3] = (mm[:, None] * vv).sum(0)
+        expect[abi.S['MCC']] = mcc.sum()
+        expect[abi.S['EROT']] = ee.sum()
+        expect[abi.S['ROTDOF']] = 2.0 * k.sum()
+        pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+        for q, (a, b) in enumerate(pairs):
+            expect[abi.S['MUU'] + q] = (mm * vv[:, a] * vv[:, b]).sum()
+        expect[abi.S['MCCU']:abi.S['MCCU'] + 3] = (mcc[:, None] * vv).sum(0)
+        expect[abi.S['EROTU']:abi.S['EROTU'] + 3] = (ee[:, None] * vv).sum(0)
+        for s in range(2):
+            expect[abi.S_SPECIES_N + s] = (ss == s).sum()
+            expect[abi.S_SPECIES_MCC + s] = mcc[ss == s].sum()
+        got = S[:, c] / dt
+        assert np.allclose(got, expect, rtol=1e-14, atol=1e-300), (c, got - expect)
+
+
+# ------------------------------------------------------------------ meshFill
+
+def test_fill_moments_and_sigma_max():
+    spec = box_case(n=(4, 4, 4), species=(N2, O2), collision="dsmc", fn=1e9)
+    o = orc.OracleCase(spec, capacity=400000)
+    nd = {"N2": 0.79e20, "O2": 0.21e20}
+    U = np.array([120.0, -30.0, 0.0])
+    o.fill(nd, T=300.0, rot_T=300.0, velocity=tuple(U))
+    p = o.particles()
+    vol = o.geometry()["vol"]
+    for s, (sp, n) in enumerate(((N2, 0.79e20), (O2, 0.21e20))):
+        k = p["species"] == s
+        expect = n * vol.sum() / spec.n_equivalent_particles
+        # floor + Bernoulli per cell: variance below 1/4 per cell
+        assert abs(k.sum() - expect) <= 5.0 * math.sqrt(0.25 * spec.mesh.n_cells) + 1.0
+        v = p["velocity"][k]
+        sig = math.sqrt(KB * 300.0 / sp.mass)
+        assert np.all(np.abs(v.mean(0) - U) < 5.0 * sig / math.sqrt(k.sum()))
+        T = sp.mass * v.var(0) / KB
+        assert np.all(np.abs(T - 300.0) < 5.0 * 300.0 * math.sqrt(2.0 / k.sum()))
+        er = p["erot"][k]
+        assert abs(er.mean() - KB * 300.0) < 5.0 * KB * 300.0 / math.sqrt(k.sum())
+    # (sigma_T c_r)max = pi d^2 c_mp(T) of the most abundant species (uniGasMeshFill.C:284-296)
+    smax = o.sigma_max()
+    assert np.allclose(smax, math.pi * N2.diameter ** 2 * math.sqrt(2.0 * KB * 300.0 / N2.mass), rtol=1e-14)

@@ -76,7 +76,7 @@ def lib():
     return _lib
 
 
-N_PHASES = 8
+N_PHASES = 9
 
 
 def phase_names():

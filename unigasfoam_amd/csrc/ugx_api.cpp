@@ -282,7 +282,7 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len)
     UGX_GUARD_END
 }
 
-static const char* kPhaseNames[kPhases] = {"begin", "inflow", "move", "wall", "scan", "scatter", "cells", "end"};
+static const char* kPhaseNames[kPhases] = {"begin", "inflow", "move", "wall", "scan", "scatter", "cells", "collide", "end"};
 
 const char* ugx_phase_name(int i) { return (i >= 0 && i < kPhases) ? kPhaseNames[i] : ""; }
 

@@ -123,7 +123,7 @@ __global__ void k_begin_step(WorldView<R> w)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < w.ncases) w.cases[c].particle_begin = w.off[w.cases[c].cell_begin];
-    if (c == 0) { w.st->n_ins = 0u; w.st->nwall = 0u; }
+    if (c == 0) { w.st->n_ins = 0u; w.st->nwall = 0u; w.st->nsel = 0u; }
 }
 
 // ------------------------------------------------------------- inflow
@@ -493,42 +493,23 @@ __global__ void k_scatter(WorldView<R> w)
     }
 }
 
-__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v)
-{
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 // ------------------------------------------------------- cell accessors
-// The per-cell physics reads and writes simulators through an accessor:
-// LdsAcc for a tile staged in LDS, GlobAcc for the sorted buffers in HBM
-// (tiles too large for LDS).  Both address simulators by sorted slot.
+// The per-cell physics reads simulators through an accessor addressed by
+// sorted slot: LdsAcc for a tile staged in LDS, GlobAcc for the sorted
+// buffers in HBM.
 
 template <class R>
 struct LdsAcc {
-    R *vx, *vy, *vz, *er;
-    uint8_t* sl;
-    uint16_t* idx;
+    const R *vx, *vy, *vz, *er;
+    const uint8_t* sl;
     uint32_t p0;
     __device__ void load(uint32_t i, double& a, double& b, double& c) const
     {
         const uint32_t k = i - p0;
         a = (double)vx[k]; b = (double)vy[k]; c = (double)vz[k];
     }
-    __device__ void store(uint32_t i, double a, double b, double c) const
-    {
-        const uint32_t k = i - p0;
-        vx[k] = (R)a; vy[k] = (R)b; vz[k] = (R)c;
-    }
     __device__ double erot(uint32_t i) const { return (double)er[i - p0]; }
-    __device__ void set_erot(uint32_t i, double e) const { er[i - p0] = (R)e; }
     __device__ int local_species(uint32_t i) const { return sl[i - p0]; }
-    __device__ uint32_t idx_get(uint32_t b, uint32_t k) const { return idx[b - p0 + k]; }
-    __device__ void idx_set(uint32_t b, uint32_t k, uint32_t v) const { idx[b - p0 + k] = (uint16_t)v; }
 };
 
 template <class R>
@@ -551,99 +532,266 @@ struct GlobAcc {
     __device__ void idx_set(uint32_t b, uint32_t k, uint32_t v) const { idx[b + k] = v; }
 };
 
-// Canonical per-cell sums, wave-cooperative: the cell's simulators are taken
-// 64 at a time (lane l holds simulator chunk + l, missing lanes hold +0.0),
-// every value is canonicalised (v + 0.0) and reduced by the xor butterfly
-// 32, 16, 8, 4, 2, 1; the chunk totals are added in order to 0.0.  Every
-// lane of the wave ends with the totals.  Called by all 64 lanes of a wave
-// with the same [b, e).
-__device__ inline double wave_bfly(double v)
+// ----------------------------------------------------- canonical cell sums
+// Every per-cell sum (sampler rows, BGK moments, the conservation sums) is
+// taken in one fixed order: eight partial sums, partial j adding the
+// simulators j, j+8, j+16, ... of the cell in slot order to +0.0, then the
+// tree ((P0+P1)+(P2+P3)) + ((P4+P5)+(P6+P7)).  On the GPU a group of eight
+// lanes holds the partials and the tree is three DPP exchanges (xor 1, xor 2
+// by quad permutes, xor 4 by the half-row mirror, which pairs the two
+// uniform halves of an eight-lane row).
+
+template <int Ctrl>
+__device__ inline double dpp_f64(double v)
 {
-    v = v + 0.0;
-    v = v + __shfl_xor(v, 32);
-    v = v + __shfl_xor(v, 16);
-    v = v + __shfl_xor(v, 8);
-    v = v + __shfl_xor(v, 4);
-    v = v + __shfl_xor(v, 2);
-    v = v + __shfl_xor(v, 1);
-    return v;
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), Ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), Ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-// Rows of cellMeasurements::calculateFields (cellMeasurements.C:408-513).
-// L selects the rows needed: 0 rows 0-7, 1 adds 8-13, 2 adds 14-19; the
-// per-species rows follow (a single-species case copies rows N and MCC,
-// which are the same sums).
-template <int L, class R, class Acc>
-__device__ inline void wave_cell_sums_l(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b,
-                                        uint32_t e, double S[kNSamp])
+// Called with all 64 lanes active; returns the group total in each lane.
+__device__ inline double group8_total(double p)
 {
-#pragma unroll
-    for (int k = 0; k < kNSamp; ++k) S[k] = 0.0;
-    const uint32_t lane = __lane_id();
-    const bool multi = cs.ns > 1;
-    for (uint32_t cb = b; cb < e; cb += 64u) {
-        const uint32_t i = cb + lane;
-        const bool on = i < e;
-        double m = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, er = 0.0, rd = 0.0, one = 0.0;
-        int s = -1;
-        if (on) {
-            s = A.local_species(i);
-            const int g = cs.sp_base + s;
-            m = w.sp.mass[g];
-            rd = (double)w.sp.rotdof[g];
-            A.load(i, vx, vy, vz);
-            er = A.erot(i);
-            one = 1.0;
-        }
+    p = p + dpp_f64<0xB1>(p);    // quad_perm [1,0,3,2]
+    p = p + dpp_f64<0x4E>(p);    // quad_perm [2,3,0,1]
+    p = p + dpp_f64<0x141>(p);   // row_half_mirror
+    return p;
+}
+
+// Partial sums of cellMeasurements::calculateFields rows
+// (cellMeasurements.C:408-513) for the simulators [i0, e) step 8 of one
+// cell.  L: 0 rows 0-7, 1 adds 8-13, 2 adds 14-19; species rows for
+// q < nsw.  S must start at +0.0.
+template <int L, class R, class Acc>
+__device__ inline void cell_partials(const WorldView<R>& w, int sp_base, int nsw, const Acc& A, uint32_t i0,
+                                     uint32_t e, double S[kNSamp])
+{
+    for (uint32_t i = i0; i < e; i += 8u) {
+        const int s = A.local_species(i);
+        const int g = sp_base + s;
+        const double m = w.sp.mass[g];
+        double vx, vy, vz;
+        A.load(i, vx, vy, vz);
+        const double er = A.erot(i);
         const double mcc = m * (vx * vx + vy * vy + vz * vz);
-        S[UGX_S_N] = S[UGX_S_N] + wave_bfly(one);
-        S[UGX_S_M] = S[UGX_S_M] + wave_bfly(m);
-        S[UGX_S_MU] = S[UGX_S_MU] + wave_bfly(m * vx);
-        S[UGX_S_MV] = S[UGX_S_MV] + wave_bfly(m * vy);
-        S[UGX_S_MW] = S[UGX_S_MW] + wave_bfly(m * vz);
-        S[UGX_S_MCC] = S[UGX_S_MCC] + wave_bfly(mcc);
-        S[UGX_S_EROT] = S[UGX_S_EROT] + wave_bfly(er);
-        S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + wave_bfly(rd);
+        S[UGX_S_N] = S[UGX_S_N] + 1.0;
+        S[UGX_S_M] = S[UGX_S_M] + m;
+        S[UGX_S_MU] = S[UGX_S_MU] + m * vx;
+        S[UGX_S_MV] = S[UGX_S_MV] + m * vy;
+        S[UGX_S_MW] = S[UGX_S_MW] + m * vz;
+        S[UGX_S_MCC] = S[UGX_S_MCC] + mcc;
+        S[UGX_S_EROT] = S[UGX_S_EROT] + er;
+        S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + (double)w.sp.rotdof[g];
         if (L >= 1) {
-            S[UGX_S_MUU] = S[UGX_S_MUU] + wave_bfly(m * (vx * vx));
-            S[UGX_S_MUV] = S[UGX_S_MUV] + wave_bfly(m * (vx * vy));
-            S[UGX_S_MUW] = S[UGX_S_MUW] + wave_bfly(m * (vx * vz));
-            S[UGX_S_MVV] = S[UGX_S_MVV] + wave_bfly(m * (vy * vy));
-            S[UGX_S_MVW] = S[UGX_S_MVW] + wave_bfly(m * (vy * vz));
-            S[UGX_S_MWW] = S[UGX_S_MWW] + wave_bfly(m * (vz * vz));
+            S[UGX_S_MUU] = S[UGX_S_MUU] + m * (vx * vx);
+            S[UGX_S_MUV] = S[UGX_S_MUV] + m * (vx * vy);
+            S[UGX_S_MUW] = S[UGX_S_MUW] + m * (vx * vz);
+            S[UGX_S_MVV] = S[UGX_S_MVV] + m * (vy * vy);
+            S[UGX_S_MVW] = S[UGX_S_MVW] + m * (vy * vz);
+            S[UGX_S_MWW] = S[UGX_S_MWW] + m * (vz * vz);
         }
         if (L >= 2) {
-            S[UGX_S_MCCU] = S[UGX_S_MCCU] + wave_bfly(mcc * vx);
-            S[UGX_S_MCCV] = S[UGX_S_MCCV] + wave_bfly(mcc * vy);
-            S[UGX_S_MCCW] = S[UGX_S_MCCW] + wave_bfly(mcc * vz);
-            S[UGX_S_EROTU] = S[UGX_S_EROTU] + wave_bfly(er * vx);
-            S[UGX_S_EROTV] = S[UGX_S_EROTV] + wave_bfly(er * vy);
-            S[UGX_S_EROTW] = S[UGX_S_EROTW] + wave_bfly(er * vz);
+            S[UGX_S_MCCU] = S[UGX_S_MCCU] + mcc * vx;
+            S[UGX_S_MCCV] = S[UGX_S_MCCV] + mcc * vy;
+            S[UGX_S_MCCW] = S[UGX_S_MCCW] + mcc * vz;
+            S[UGX_S_EROTU] = S[UGX_S_EROTU] + er * vx;
+            S[UGX_S_EROTV] = S[UGX_S_EROTV] + er * vy;
+            S[UGX_S_EROTW] = S[UGX_S_EROTW] + er * vz;
         }
-        if (multi) {
+        if (nsw > 1) {
 #pragma unroll
             for (int q = 0; q < kNS; ++q) {
-                if (q < cs.ns) {
-                    S[UGX_S_SPECIES_N + q] = S[UGX_S_SPECIES_N + q] + wave_bfly((s == q) ? 1.0 : 0.0);
-                    S[UGX_S_SPECIES_MCC + q] = S[UGX_S_SPECIES_MCC + q] + wave_bfly((s == q) ? mcc : 0.0);
+                if (q < nsw) {
+                    S[UGX_S_SPECIES_N + q] = S[UGX_S_SPECIES_N + q] + ((s == q) ? 1.0 : 0.0);
+                    S[UGX_S_SPECIES_MCC + q] = S[UGX_S_SPECIES_MCC + q] + ((s == q) ? mcc : 0.0);
                 }
             }
         }
     }
-    if (!multi) {
-        S[UGX_S_SPECIES_N] = S[UGX_S_N];
-        S[UGX_S_SPECIES_MCC] = S[UGX_S_MCC];
+}
+
+// Sums of the cells of one tile, eight cells per wave (group g of lanes
+// 8g..8g+7 takes cell k), then the sampler (uniGasVolFields.C:761-797), the
+// hybrid accumulators and the BGK moments.  All 64 lanes of every wave
+// call it with the same arguments.
+template <int L, class R, class Acc>
+__device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nct, const uint32_t* s_off,
+                                 const Acc& A)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wv = tid >> 6, lane = tid & 63u, g = lane >> 3, j = lane & 7u;
+    const int nsw = w.ns_max;
+    for (uint32_t kb = 0; kb < nct; kb += (kBlock / 64) * 8) {
+        const uint32_t k = kb + wv * 8u + g;
+        uint32_t b = 0, e = 0;
+        bool sample = false, hybrid = false, bgk = false;
+        int sp_base = 0;
+        double dt = 0.0;
+        const uint32_t c = c0 + k;
+        if (k < nct) {
+            const CaseDev& cs = w.cases[w.cell_case[c]];
+            if (cs.active) {
+                sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
+                hybrid = cs.coll_model == UGX_COLL_HYBRID;
+                bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+                if (sample || hybrid || bgk) { b = s_off[k]; e = s_off[k + 1]; }
+                sp_base = cs.sp_base;
+                dt = cs.dt;
+            }
+        }
+        const bool need = sample || hybrid || bgk;
+        if (__ballot(need) == 0ull) continue;
+        double S[kNSamp];
+#pragma unroll
+        for (int q = 0; q < kNSamp; ++q) S[q] = 0.0;
+        cell_partials<L, R, Acc>(w, sp_base, nsw, A, b + j, e, S);
+#pragma unroll
+        for (int q = 0; q < kNSamp; ++q) {
+            const bool used = (q < UGX_S_MUU) || (L >= 1 && q < UGX_S_MCCU) || (L >= 2 && q < UGX_S_BASE)
+                              || (nsw > 1 && q >= UGX_S_BASE);
+            if (used) S[q] = group8_total(S[q]);
+        }
+        if (nsw <= 1) {
+            S[UGX_S_SPECIES_N] = S[UGX_S_N];
+            S[UGX_S_SPECIES_MCC] = S[UGX_S_MCC];
+        }
+        if (need && j == 0u) {
+            if (sample) {
+                const bool full = w.cases[w.cell_case[c]].sampler_full != 0;
+#pragma unroll
+                for (int q = 0; q < kNSamp; ++q)
+                    if (full || q < UGX_S_MUU || q >= UGX_S_BASE) w.samp[(size_t)q * w.nc + c] += dt * S[q];
+            }
+            if (hybrid) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w.knacc[(size_t)8 * c + q] += dt * S[q];
+            }
+            if (bgk) {
+#pragma unroll
+                for (int q = 0; q < kNMom; ++q) w.mom[(size_t)q * w.nc + c] = S[q];
+            }
+        }
     }
 }
 
-template <class R, class Acc>
-__device__ inline void wave_cell_sums(const WorldView<R>& w, const CaseDev& cs, const Acc& A, uint32_t b, uint32_t e,
-                                      double S[kNSamp])
+// ------------------------------------------------------------ tile table
+
+// tile_first[t] = first cell whose key off[c] + kCellW * c is >= t * kTileN;
+// tile_first[ntiles] = nc.  Each cell writes the tiles that start at it.
+template <class R>
+__global__ void k_tile_table(WorldView<R> w)
 {
-    if (cs.sum_level >= 2) wave_cell_sums_l<2, R, Acc>(w, cs, A, b, e, S);
-    else if (cs.sum_level == 1) wave_cell_sums_l<1, R, Acc>(w, cs, A, b, e, S);
-    else wave_cell_sums_l<0, R, Acc>(w, cs, A, b, e, S);
+    const uint32_t nc = (uint32_t)w.nc;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= nc; c += gridDim.x * blockDim.x) {
+        const uint64_t key = (uint64_t)w.off[c] + (uint64_t)kCellW * c;
+        const uint64_t lo = (c == 0u) ? 0ull : ((uint64_t)w.off[c - 1] + (uint64_t)kCellW * (c - 1u)) / kTileN + 1ull;
+        uint64_t hi = key / kTileN;
+        if (c == nc) {
+            hi += 1ull;
+            w.st->ntiles = (uint32_t)hi;
+        }
+        for (uint64_t t = lo; t <= hi; ++t) w.tile_first[t] = c;
+    }
 }
+
+// ------------------------------------------------------------ cell tiles
+
+// One block per tile (k_tile_table).  Steps:
+//  1. each cell's slot list (written by k_scatter in atomic order) is put in
+//     ascending previous-slot order: an entry's rank in its cell is the
+//     number of entries of that cell with a smaller slot;
+//  2. records are gathered into the sorted buffers B (velocity, E_rot and
+//     local species also into LDS);
+//  3. per-cell sums from LDS: sampler, hybrid accumulators, BGK moments.
+// SortOnly stops after 2.  Tiles larger than kTileMax run the same steps
+// from HBM.
+template <class R, bool SortOnly, int L>
+__global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
+{
+    constexpr uint32_t kM = SortOnly ? 1u : kTileMax;
+    __shared__ uint32_t s_off[kTileCells + 2];
+    __shared__ uint32_t s_in[kTileMax];
+    __shared__ uint32_t s_perm[kTileMax];
+    __shared__ uint8_t s_k[kTileMax];
+    __shared__ R s_vx[kM], s_vy[kM], s_vz[kM], s_er[kM];
+    __shared__ uint8_t s_sl[kM];
+    const uint32_t t = blockIdx.x;
+    if (t >= w.st->ntiles) return;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t c0 = w.tile_first[t];
+    const uint32_t nct = w.tile_first[t + 1] - c0;
+    for (uint32_t k = tid; k <= nct; k += kBlock) s_off[k] = w.off[c0 + k];
+    __syncthreads();
+    const uint32_t p0 = s_off[0];
+    const uint32_t size = s_off[nct] - p0;
+    if (size <= kTileMax) {
+        for (uint32_t i = tid; i < size; i += kBlock) s_in[i] = w.perm[p0 + i];
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = s_in[i];
+            // cell of entry i: the last k with s_off[k] <= p0 + i
+            uint32_t lo = 0, hi = nct - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (s_off[mid] <= p0 + i) lo = mid; else hi = mid - 1u;
+            }
+            const uint32_t b = s_off[lo] - p0, e = s_off[lo + 1] - p0;
+            uint32_t r = 0;
+            for (uint32_t q = b; q < e; ++q) r += (s_in[q] < slot) ? 1u : 0u;
+            s_perm[b + r] = slot;
+            s_k[b + r] = (uint8_t)lo;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = s_perm[i];
+            const uint32_t j = p0 + i;
+            const R vx = w.a.vx[slot], vy = w.a.vy[slot], vz = w.a.vz[slot], er = w.a.erot[slot];
+            const uint32_t g = w.a.sp[slot];
+            w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
+            w.b.vx[j] = vx; w.b.vy[j] = vy; w.b.vz[j] = vz;
+            w.b.erot[j] = er;
+            w.b.cell[j] = c0 + s_k[i];
+            w.b.sp[j] = g;
+            if (!SortOnly) {
+                s_vx[i] = vx; s_vy[i] = vy; s_vz[i] = vz; s_er[i] = er;
+                s_sl[i] = (uint8_t)w.sp.local[g];
+            }
+        }
+        if (SortOnly) return;
+        __syncthreads();
+        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, p0};
+        tile_sums<L, R>(w, c0, nct, s_off, A);
+    } else {
+        // Tile larger than LDS: the same steps in HBM, slot lists ordered into scratch.
+        if (tid == 0) atomicAdd(&w.st->big_tiles, 1u);
+        for (uint32_t k = 0; k < nct; ++k) {
+            const uint32_t b = s_off[k], e = s_off[k + 1];
+            for (uint32_t i = b + tid; i < e; i += kBlock) {
+                const uint32_t slot = w.perm[i];
+                uint32_t r = 0;
+                for (uint32_t q = b; q < e; ++q) r += (w.perm[q] < slot) ? 1u : 0u;
+                w.scratch[b + r] = slot;
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBlock) {
+            const uint32_t slot = w.scratch[p0 + i];
+            const uint32_t j = p0 + i;
+            w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
+            w.b.vx[j] = w.a.vx[slot]; w.b.vy[j] = w.a.vy[slot]; w.b.vz[j] = w.a.vz[slot];
+            w.b.erot[j] = w.a.erot[slot];
+            w.b.cell[j] = w.a.cell[slot];
+            w.b.sp[j] = w.a.sp[slot];
+        }
+        if (SortOnly) return;
+        __threadfence_block();
+        __syncthreads();
+        GlobAcc<R> A{w.b, w.sp.local, w.perm};
+        tile_sums<L, R>(w, c0, nct, s_off, A);
+    }
+}
+
+// ------------------------------------------------------------- collisions
 
 // noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell level 1.
 template <class R, class Acc>
@@ -690,31 +838,23 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
 
 // BGK-family relaxation (stochasticParticleBGK.C:287-796,
 // stochasticParticleESBGK.C:311-906, unifiedStochasticParticleSBGK.C:379-1096)
-// in three phases so that the velocity draws run one simulator per lane:
-//  A  (lane per cell)  moments, Pr, nu, USP q/sigma time relaxation, the
-//     number of relaxed simulators and their selection (partial Fisher-Yates
-//     on the cell stream); parameters go to a per-cell table;
-//  B  (lane per selected simulator) new velocity from its own stream
-//     (kRngBgkV, sub = selection index); USP draws test against the cell's
-//     maxProb at the start of the step and record the largest raise;
-//  C  (lane per cell)  conserveMomentumAndEnergy, maxProb update.
-constexpr uint32_t kSelNone = 0xFFFFFFFFu;
-constexpr uint32_t kRound = 32;       // cells per phase round in a tile
+// in three kernels so that the velocity draws run one simulator per lane:
+//  k_collide   (lane per cell)  moments, Pr, nu, USP q/sigma time
+//              relaxation, the number of relaxed simulators and their
+//              selection (partial Fisher-Yates on the cell stream) into a
+//              selection list; parameters into a per-cell record;
+//  k_bgk_draw  (lane per selection) new velocity from the selection's own
+//              stream (kRngBgkV, sub = selection index); USP draws test
+//              against the cell's maxProb at the start of the step and
+//              record the largest raise;
+//  k_bgk_conserve (eight lanes per cell) conserveMomentumAndEnergy and the
+//              USP maxProb update.
 constexpr uint32_t kRngBgkV = 7;
 
-struct BgkP {
-    double U[3];
-    double s2kT, p, cQ, cS, m0, T;
-    double q[3];
-    double sm[9];      // ES-BGK S matrix (row major) or USP stress (xx xy xz yy yz zz)
-    int32_t perform;
-    int32_t model;
-};
-
-template <class R, class Acc>
-__device__ inline void bgk_prepare(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
-                                   const double* S, const Acc& A, uint32_t* sel, uint32_t sel_base,
-                                   uint32_t slot, BgkP& P, unsigned long long& nrel_out)
+// Parameters and relaxed count of one cell; S holds the cell's sums.
+template <class R>
+__device__ inline void bgk_params(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t nC,
+                                  const double* S, BgkP& P, uint32_t& nrel_out)
 {
     const int model = cs.bgk_model;
     const double FN = cs.fn, dt = cs.dt, V = w.vol[c];
@@ -796,24 +936,9 @@ __device__ inline void bgk_prepare(const WorldView<R>& w, const CaseDev& cs, uin
     P.model = model;
     P.m0 = w.uspmax[c];
     P.perform = 0;
+    nrel_out = 0u;
     if (perform && w.cmodel[c] == 0) {
         P.perform = 1;
-        Rng r;
-        rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
-        const double pc = N * (1.0 - dexp(-nu * dt));
-        const double fl = floor(pc);
-        long long nrel = (long long)fl;
-        if (uniform(r) < pc - fl) nrel += 1;
-        if (nrel > (long long)nC) nrel = nC;
-        nrel_out += (unsigned long long)nrel;
-        for (uint32_t i = 0; i < nC; ++i) A.idx_set(b, i, i);
-        for (long long i = 0; i < nrel; ++i) {
-            const uint32_t j = (uint32_t)i + uindex(r, (uint32_t)((long long)nC - i));
-            const uint32_t ti = A.idx_get(b, (uint32_t)i), tj = A.idx_get(b, j);
-            A.idx_set(b, (uint32_t)i, tj);
-            A.idx_set(b, j, ti);
-            sel[b - sel_base + tj] = ((uint32_t)i << 6) | slot;
-        }
         for (int d = 0; d < 3; ++d) { P.U[d] = U[d]; P.q[d] = q[d]; }
         P.T = T;
         P.p = p;
@@ -834,13 +959,102 @@ __device__ inline void bgk_prepare(const WorldView<R>& w, const CaseDev& cs, uin
 #pragma unroll
             for (int k = 0; k < 6; ++k) P.sm[k] = sh[k];
         }
+        // relaxed count: stochastic round of N (1 - exp(-nu dt)), first draw of the cell stream
+        Rng r;
+        rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
+        const double pc = N * (1.0 - dexp(-nu * dt));
+        const double fl = floor(pc);
+        long long nrel = (long long)fl;
+        if (uniform(r) < pc - fl) nrel += 1;
+        if (nrel > (long long)nC) nrel = nC;
+        nrel_out = (uint32_t)nrel;
     }
 }
 
-// Phase B: the new velocity of one selected simulator (global slot i).
-template <class R, class Acc>
-__device__ inline void bgk_sample(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
-                                  const BgkP& P, unsigned long long* raise, const Acc& A)
+// Partial Fisher-Yates selection of nrel of the cell's nC simulators on the
+// cell stream (after the relaxed-count draw); entries go to the selection
+// list at base.
+template <class R>
+__device__ inline void bgk_select(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
+                                  uint32_t nrel, uint32_t base, const GlobAcc<R>& A)
+{
+    Rng r;
+    rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
+    (void)uniform(r);
+    for (uint32_t i = 0; i < nC; ++i) A.idx_set(b, i, i);
+    for (uint32_t i = 0; i < nrel; ++i) {
+        const uint32_t j = i + uindex(r, nC - i);
+        const uint32_t ti = A.idx_get(b, i), tj = A.idx_get(b, j);
+        A.idx_set(b, i, tj);
+        A.idx_set(b, j, ti);
+        w.sel_slot[base + i] = b + tj;
+        w.sel_cell[base + i] = c;
+        w.sel_k[base + i] = i;
+    }
+}
+
+// Exclusive wave prefix of v and one atomic reservation per wave on *ctr;
+// called by all 64 lanes.  Returns the lane's base.
+__device__ inline uint32_t wave_reserve(uint32_t* ctr, uint32_t v)
+{
+    const uint32_t lane = __lane_id();
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += u;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 63u && total) base = atomicAdd(ctr, total);
+    base = __shfl(base, 63);
+    return base + incl - v;
+}
+
+// Lane per cell: BGK-family parameters and selections, NTC collisions.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_collide(WorldView<R> w)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = c < (uint32_t)w.nc;
+    unsigned long long cand = 0ull, coll = 0ull;
+    uint32_t nrel = 0u, b = 0u, nC = 0u;
+    int ccase = 0;
+    bool bgk = false, dsmc = false;
+    if (on) {
+        ccase = w.cell_case[c];
+        const CaseDev& cs = w.cases[ccase];
+        if (cs.active) {
+            const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
+            bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+            dsmc = (cs.coll_model == UGX_COLL_DSMC || hybrid) && cs.dsmc_model != UGX_DSMC_NONE
+                   && w.cmodel[c] == 1;
+            b = w.off[c];
+            nC = w.off[c + 1] - b;
+        }
+    }
+    GlobAcc<R> A{w.b, w.sp.local, w.perm};
+    if (bgk) {
+        const CaseDev& cs = w.cases[ccase];
+        double S[kNMom];
+#pragma unroll
+        for (int q = 0; q < kNMom; ++q) S[q] = w.mom[(size_t)q * w.nc + c];
+        BgkP P;
+        bgk_params<R>(w, cs, c, nC, S, P, nrel);
+        w.bgkp[c] = P;
+        w.raise[c] = 0ull;
+    }
+    const uint32_t base = wave_reserve(&w.st->nsel, nrel);
+    if (nrel) bgk_select<R>(w, w.cases[ccase], c, b, nC, nrel, base, A);
+    if (dsmc && nC >= 2u) ntc_cell<R>(w, w.cases[ccase], c, b, nC, A, cand, coll);
+    case_add(w.counters, ccase, kCntCandidates, cand);
+    case_add(w.counters, ccase, kCntCollisions, coll);
+    case_add(w.counters, ccase, kCntBgk, (unsigned long long)nrel);
+}
+
+// New velocity of one selected simulator (slot i of cell c, selection k).
+template <class R>
+__device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
+                                const BgkP& P, unsigned long long* raise, const GlobAcc<R>& A)
 {
     Rng rv;
     rng_init(rv, cs.seed_lo, cs.seed_hi, kRngBgkV, k, c - (uint32_t)cs.cell_begin, cs.step);
@@ -878,250 +1092,79 @@ __device__ inline void bgk_sample(const WorldView<R>& w, const CaseDev& cs, uint
     A.store(i, P.U[0] + u0 * xi[0], P.U[1] + u0 * xi[1], P.U[2] + u0 * xi[2]);
 }
 
-// Phase C: conserveMomentumAndEnergy (canonical wave sums) and the USP
-// maxProb update; called by all lanes of one wave for one cell.
-template <class R, class Acc>
-__device__ inline void bgk_finish(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
-                                  const double* S, const BgkP& P, unsigned long long raise_bits, const Acc& A)
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_bgk_draw(WorldView<R> w)
 {
-    const uint32_t lane = __lane_id();
-    if (P.perform) {
-        const double N = S[UGX_S_N];
-        double ke = 0.0, mom0 = 0.0, mom1 = 0.0, mom2 = 0.0;
-        for (uint32_t cb = b; cb < b + nC; cb += 64u) {
-            const uint32_t i = cb + lane;
-            double m = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-            if (i < b + nC) {
-                m = w.sp.mass[cs.sp_base + A.local_species(i)];
-                A.load(i, vx, vy, vz);
-            }
-            ke = ke + wave_bfly(m * (vx * vx + vy * vy + vz * vz));
-            mom0 = mom0 + wave_bfly(m * vx);
-            mom1 = mom1 + wave_bfly(m * vy);
-            mom2 = mom2 + wave_bfly(m * vz);
-        }
-        const double pU0 = mom0 / S[UGX_S_M], pU1 = mom1 / S[UGX_S_M], pU2 = mom2 / S[UGX_S_M];
-        const double pT = (ke - S[UGX_S_M] * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
-        if (pT > kVSmall) {
-            const double sc = sqrt(P.T / pT);
-            for (uint32_t i = b + lane; i < b + nC; i += 64u) {
-                double vx, vy, vz;
-                A.load(i, vx, vy, vz);
-                A.store(i, P.U[0] + (vx - pU0) * sc, P.U[1] + (vy - pU1) * sc, P.U[2] + (vz - pU2) * sc);
-            }
-        }
-    }
-    if (P.model == UGX_BGK_USP && lane == 0) {
-        const double raise = bits_to_double(raise_bits);
-        w.uspmax[c] = (raise > 0.0) ? raise : P.m0 * 0.999;
+    const uint32_t ns = w.st->nsel;
+    GlobAcc<R> A{w.b, w.sp.local, w.perm};
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
+        const uint32_t c = w.sel_cell[k];
+        bgk_draw<R>(w, w.cases[w.cell_case[c]], c, w.sel_slot[k], w.sel_k[k], w.bgkp[c], &w.raise[c], A);
     }
 }
 
-// Round of up to kRound cells [cb, ce) of a tile:
-//  sums    wave per cell: canonical sums into S_tab, sampler and hybrid sums
-//  A       lane per cell: BGK moments, selection (wave 0)
-//  B       lane per selected simulator: new velocities (all waves)
-//  C       wave per cell: conservation, USP maxProb
-//  NTC     lane per cell (wave 0)
-// sel is indexed by slot - sel_base.
-template <class R, class Acc>
-__device__ inline void cell_round(const WorldView<R>& w, uint32_t cb, uint32_t ce, const Acc& A, uint32_t* sel,
-                                  uint32_t sel_base, BgkP* bp, unsigned long long* raise, double (*S_tab)[kNSamp],
-                                  unsigned long long (*s_cnt)[3], int case0)
+// conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
+// canonical sums, and the USP maxProb update; eight lanes per cell.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
 {
     const uint32_t tid = threadIdx.x;
-    const uint32_t wv = tid >> 6, lane = tid & 63u;
-    const uint32_t ncell = ce - cb;
-    const uint32_t rb = w.off[cb], re = w.off[ce];
-    for (uint32_t i = rb + tid; i < re; i += kBlock) sel[i - sel_base] = kSelNone;
-    if (tid < kRound) { raise[tid] = 0ull; bp[tid].perform = 0; bp[tid].model = UGX_BGK_NONE; }
-    // sums
-    for (uint32_t k = wv; k < ncell; k += kBlock / 64u) {
-        const uint32_t c = cb + k;
+    const uint32_t lane = tid & 63u, g = lane >> 3, j = lane & 7u;
+    const uint32_t c = (blockIdx.x * blockDim.x + tid) >> 3;
+    bool perform = false, usp = false;
+    uint32_t b = 0, e = 0;
+    double T = 0.0, U0 = 0.0, U1 = 0.0, U2 = 0.0, M = 0.0, N = 0.0;
+    int sp_base = 0;
+    (void)g;
+    if (c < (uint32_t)w.nc) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
-        if (!cs.active) continue;
-        const bool sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
         const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
-        const bool bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
-        if (!(sample || hybrid || bgk)) continue;
-        double S[kNSamp];
-        wave_cell_sums<R>(w, cs, A, w.off[c], w.off[c + 1], S);
-        if (lane == 0) {
-            const double dt = cs.dt;
-#pragma unroll
-            for (int q = 0; q < kNSamp; ++q) S_tab[k][q] = S[q];
-            if (sample) {
-#pragma unroll
-                for (int q = 0; q < kNSamp; ++q)
-                    if (cs.sampler_full || q < UGX_S_MUU || q >= UGX_S_BASE)
-                        w.samp[(size_t)q * w.nc + c] += dt * S[q];
-            }
-            if (hybrid) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) w.knacc[(size_t)8 * c + q] += dt * S[q];
+        const bool bgk = cs.active && (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
+        if (bgk) {
+            const BgkP& P = w.bgkp[c];
+            perform = P.perform != 0;
+            usp = P.model == UGX_BGK_USP;
+            if (perform) {
+                b = w.off[c];
+                e = w.off[c + 1];
+                T = P.T; U0 = P.U[0]; U1 = P.U[1]; U2 = P.U[2];
+                M = w.mom[(size_t)UGX_S_M * w.nc + c];
+                N = w.mom[(size_t)UGX_S_N * w.nc + c];
+                sp_base = cs.sp_base;
             }
         }
     }
-    __syncthreads();
-    unsigned long long cnt[3] = {0ull, 0ull, 0ull};
-    if (tid < ncell) {
-        const uint32_t c = cb + tid;
-        const CaseDev& cs = w.cases[w.cell_case[c]];
-        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
-        const bool bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
-        if (cs.active && bgk) {
-            const uint32_t b = w.off[c];
-            bgk_prepare<R>(w, cs, c, b, w.off[c + 1] - b, S_tab[tid], A, sel, sel_base, tid, bp[tid], cnt[2]);
+    if (__ballot(perform) != 0ull) {
+        double ke = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        for (uint32_t i = b + j; i < e; i += 8u) {
+            const double m = w.sp.mass[sp_base + w.sp.local[w.b.sp[i]]];
+            const double vx = (double)w.b.vx[i], vy = (double)w.b.vy[i], vz = (double)w.b.vz[i];
+            ke = ke + m * (vx * vx + vy * vy + vz * vz);
+            m0 = m0 + m * vx;
+            m1 = m1 + m * vy;
+            m2 = m2 + m * vz;
         }
-    }
-    __syncthreads();
-    for (uint32_t i = rb + tid; i < re; i += kBlock) {
-        const uint32_t mark = sel[i - sel_base];
-        if (mark == kSelNone) continue;
-        const uint32_t sl = mark & 63u;
-        const uint32_t cc = cb + sl;
-        bgk_sample<R>(w, w.cases[w.cell_case[cc]], cc, i, mark >> 6, bp[sl], &raise[sl], A);
-    }
-    __syncthreads();
-    for (uint32_t k = wv; k < ncell; k += kBlock / 64u) {
-        const uint32_t c = cb + k;
-        const CaseDev& cs = w.cases[w.cell_case[c]];
-        const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
-        const bool bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
-        if (cs.active && bgk) {
-            const uint32_t b = w.off[c];
-            bgk_finish<R>(w, cs, c, b, w.off[c + 1] - b, S_tab[k], bp[k], raise[k], A);
-        }
-    }
-    __syncthreads();
-    if (tid < ncell) {
-        const uint32_t c = cb + tid;
-        const CaseDev& cs = w.cases[w.cell_case[c]];
-        const bool dsmc = (cs.coll_model == UGX_COLL_DSMC || cs.coll_model == UGX_COLL_HYBRID)
-                          && cs.dsmc_model != UGX_DSMC_NONE;
-        const uint32_t b = w.off[c], nC = w.off[c + 1] - b;
-        if (cs.active && dsmc && w.cmodel[c] == 1 && nC >= 2) ntc_cell<R>(w, cs, c, b, nC, A, cnt[0], cnt[1]);
-        const int k = w.cell_case[c] - case0;
-        for (int q = 0; q < 3; ++q) {
-            if (!cnt[q]) continue;
-            if (k >= 0 && k < 4) atomicAdd(&s_cnt[k][q], cnt[q]);
-            else atomicAdd(&w.counters[(size_t)w.cell_case[c] * kCntNum + (q == 0 ? kCntCandidates : q == 1 ? kCntCollisions : kCntBgk)], cnt[q]);
-        }
-    }
-    __syncthreads();
-}
-
-constexpr uint32_t kTileN = 512;     // nominal simulators per cell tile
-constexpr uint32_t kTileMax = 1024;  // LDS capacity of a tile
-
-// One tile = the cells whose first sorted slot lies in [t*kTileN, (t+1)*kTileN)
-// (the last tile also owns the trailing empty cells).  Steps:
-//  1. each cell's slot list (filled by k_scatter in atomic order) is put in
-//     ascending previous-slot order: an entry's rank in its cell is the
-//     number of entries of that cell with a smaller slot;
-//  2. records are gathered into the sorted buffers (x, cell, species to HBM,
-//     velocity, E_rot and species to LDS);
-//  3. lane per cell: sampler, BGK, NTC on the LDS copy;
-//  4. velocities and E_rot are written back.
-// SortOnly skips 3.  Tiles larger than kTileMax run the same steps from HBM.
-template <class R, bool SortOnly>
-__global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
-{
-    __shared__ uint32_t s_in[kTileMax];
-    __shared__ uint32_t s_perm[kTileMax];
-    __shared__ R s_vx[SortOnly ? 1 : kTileMax], s_vy[SortOnly ? 1 : kTileMax], s_vz[SortOnly ? 1 : kTileMax];
-    __shared__ R s_er[SortOnly ? 1 : kTileMax];
-    __shared__ uint16_t s_idx[SortOnly ? 1 : kTileMax];
-    __shared__ uint8_t s_sl[SortOnly ? 1 : kTileMax];
-    __shared__ unsigned long long s_cnt[4][3];
-    __shared__ BgkP s_bp[SortOnly ? 1 : kRound];
-    __shared__ unsigned long long s_raise[SortOnly ? 1 : kRound];
-    __shared__ double s_S[SortOnly ? 1 : kRound][kNSamp];
-    const uint32_t nn = w.st->n_new;
-    const uint32_t ntiles = nn / kTileN + 1u;
-    if (blockIdx.x >= ntiles) return;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t ncp = (uint32_t)w.nc + 1u;
-    const uint32_t start = blockIdx.x * kTileN;
-    uint32_t c0 = lower_bound_u32(w.off, ncp, start);
-    uint32_t c1 = (blockIdx.x == ntiles - 1u) ? (uint32_t)w.nc : lower_bound_u32(w.off, ncp, start + kTileN);
-    if (c0 > (uint32_t)w.nc) c0 = (uint32_t)w.nc;
-    if (c1 > (uint32_t)w.nc) c1 = (uint32_t)w.nc;
-    const uint32_t p0 = w.off[c0], p1 = w.off[c1];
-    const uint32_t size = p1 - p0;
-    const int case0 = (c0 < (uint32_t)w.nc) ? w.cell_case[c0] : 0;
-    if (tid < 12) s_cnt[tid / 3][tid % 3] = 0ull;
-    if (size <= kTileMax) {
-        for (uint32_t i = tid; i < size; i += kBlock) s_in[i] = w.perm[p0 + i];
-        __syncthreads();
-        for (uint32_t i = tid; i < size; i += kBlock) {
-            const uint32_t slot = s_in[i];
-            const uint32_t c = w.a.cell[slot];
-            const uint32_t b = w.off[c] - p0, e = w.off[c + 1] - p0;
-            uint32_t r = 0;
-            for (uint32_t j = b; j < e; ++j) r += (s_in[j] < slot) ? 1u : 0u;
-            s_perm[b + r] = slot;
-        }
-        __syncthreads();
-        for (uint32_t i = tid; i < size; i += kBlock) {
-            const uint32_t slot = s_perm[i];
-            const uint32_t j = p0 + i;
-            w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
-            const uint32_t g = w.a.sp[slot];
-            w.b.cell[j] = w.a.cell[slot];
-            w.b.sp[j] = g;
-            if (SortOnly) {
-                w.b.vx[j] = w.a.vx[slot]; w.b.vy[j] = w.a.vy[slot]; w.b.vz[j] = w.a.vz[slot];
-                w.b.erot[j] = w.a.erot[slot];
-            } else {
-                s_vx[i] = w.a.vx[slot]; s_vy[i] = w.a.vy[slot]; s_vz[i] = w.a.vz[slot];
-                s_er[i] = w.a.erot[slot];
-                s_sl[i] = (uint8_t)w.sp.local[g];
+        ke = group8_total(ke);
+        m0 = group8_total(m0);
+        m1 = group8_total(m1);
+        m2 = group8_total(m2);
+        if (perform) {
+            const double pU0 = m0 / M, pU1 = m1 / M, pU2 = m2 / M;
+            const double pT = (ke - M * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
+            if (pT > kVSmall) {
+                const double sc = sqrt(T / pT);
+                for (uint32_t i = b + j; i < e; i += 8u) {
+                    const double vx = (double)w.b.vx[i], vy = (double)w.b.vy[i], vz = (double)w.b.vz[i];
+                    w.b.vx[i] = (R)(U0 + (vx - pU0) * sc);
+                    w.b.vy[i] = (R)(U1 + (vy - pU1) * sc);
+                    w.b.vz[i] = (R)(U2 + (vz - pU2) * sc);
+                }
             }
         }
-        if (SortOnly) return;
-        __syncthreads();
-        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_idx, p0};
-        for (uint32_t cb = c0; cb < c1; cb += kRound)
-            cell_round<R>(w, cb, min(cb + kRound, c1), A, s_in, p0, s_bp, s_raise, s_S, s_cnt, case0);
-        for (uint32_t i = tid; i < size; i += kBlock) {
-            const uint32_t j = p0 + i;
-            w.b.vx[j] = s_vx[i]; w.b.vy[j] = s_vy[i]; w.b.vz[j] = s_vz[i];
-            w.b.erot[j] = s_er[i];
-        }
-    } else {
-        // Tile larger than LDS: the same steps on HBM, slot lists ordered into scratch.
-        if (tid == 0) atomicAdd(&w.st->big_tiles, 1u);
-        for (uint32_t i = tid; i < size; i += kBlock) {
-            const uint32_t slot = w.perm[p0 + i];
-            const uint32_t c = w.a.cell[slot];
-            const uint32_t b = w.off[c], e = w.off[c + 1];
-            uint32_t r = 0;
-            for (uint32_t j = b; j < e; ++j) r += (w.perm[j] < slot) ? 1u : 0u;
-            w.scratch[b + r] = slot;
-        }
-        __syncthreads();
-        for (uint32_t i = tid; i < size; i += kBlock) {
-            const uint32_t slot = w.scratch[p0 + i];
-            const uint32_t j = p0 + i;
-            w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
-            w.b.vx[j] = w.a.vx[slot]; w.b.vy[j] = w.a.vy[slot]; w.b.vz[j] = w.a.vz[slot];
-            w.b.erot[j] = w.a.erot[slot];
-            w.b.cell[j] = w.a.cell[slot];
-            w.b.sp[j] = w.a.sp[slot];
-        }
-        if (SortOnly) return;
-        __syncthreads();
-        GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        for (uint32_t cb = c0; cb < c1; cb += kRound)
-            cell_round<R>(w, cb, min(cb + kRound, c1), A, w.scratch, 0u, s_bp, s_raise, s_S, s_cnt, case0);
     }
-    __syncthreads();
-    if (tid < 12) {
-        const int k = tid / 3, q = tid % 3;
-        const unsigned long long v = s_cnt[k][q];
-        if (v && case0 + k < w.ncases)
-            atomicAdd(&w.counters[(size_t)(case0 + k) * kCntNum + (q == 0 ? kCntCandidates : q == 1 ? kCntCollisions : kCntBgk)], v);
+    if (usp && j == 0u) {
+        const double raise = bits_to_double(w.raise[c]);
+        w.uspmax[c] = (raise > 0.0) ? raise : w.bgkp[c].m0 * 0.999;
     }
 }
 
@@ -1254,7 +1297,21 @@ static inline unsigned grid_for(uint64_t n, unsigned cap_blocks = 8192)
     return (unsigned)g;
 }
 
-static inline unsigned tile_grid(uint32_t cap) { return (unsigned)(cap / kTileN) + 2u; }
+template <class R>
+static void launch_tiles(const WorldView<R>& w, hipStream_t s, bool sort_only)
+{
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    const unsigned nt = (unsigned)max_tiles(w.cap, (uint64_t)w.nc);
+    hipLaunchKernelGGL(k_tile_table<R>, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w);
+    if (sort_only)
+        hipLaunchKernelGGL((k_tile<R, true, 0>), dim3(nt), dim3(kBlock), 0, s, w);
+    else if (w.sum_level >= 2)
+        hipLaunchKernelGGL((k_tile<R, false, 2>), dim3(nt), dim3(kBlock), 0, s, w);
+    else if (w.sum_level == 1)
+        hipLaunchKernelGGL((k_tile<R, false, 1>), dim3(nt), dim3(kBlock), 0, s, w);
+    else
+        hipLaunchKernelGGL((k_tile<R, false, 0>), dim3(nt), dim3(kBlock), 0, s, w);
+}
 
 template <class R>
 void launch_sort(const WorldView<R>& w, hipStream_t s)
@@ -1265,7 +1322,7 @@ void launch_sort(const WorldView<R>& w, hipStream_t s)
     hipLaunchKernelGGL(k_hist<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
     scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
     hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
-    hipLaunchKernelGGL((k_tile<R, true>), dim3(tile_grid(w.cap)), dim3(kBlock), 0, s, w);
+    launch_tiles<R>(w, s, true);
     hipLaunchKernelGGL(k_finish_sort<R>, dim3(1), dim3(1), 0, s, w);
 }
 
@@ -1294,10 +1351,20 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     mark(5);
     hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
     mark(6);
-    hipLaunchKernelGGL((k_tile<R, false>), dim3(tile_grid(w.cap)), dim3(kBlock), 0, s, w);
+    launch_tiles<R>(w, s, false);
     mark(7);
-    hipLaunchKernelGGL(k_end_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
+    if (w.has_collide) {
+        const unsigned gcell = (unsigned)(((uint64_t)w.nc + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_collide<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+        if (w.has_bgk) {
+            hipLaunchKernelGGL(k_bgk_draw<R>, dim3(grid_for(w.cap, 16384)), dim3(kBlock), 0, s, w);
+            const unsigned gcons = (unsigned)(((uint64_t)w.nc * 8u + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL(k_bgk_conserve<R>, dim3(gcons), dim3(kBlock), 0, s, w);
+        }
+    }
     mark(8);
+    hipLaunchKernelGGL(k_end_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
+    mark(9);
 }
 
 template <class R>

@@ -14,7 +14,19 @@ struct DevState {
     uint32_t overflow;      // capacity exceeded
     uint32_t big_tiles;     // cell tiles handled by the global-memory path
     uint32_t nwall;         // queued diffuse-wall tracks this step
-    uint32_t pad[2];
+    uint32_t ntiles;        // cell tiles of this step's sort
+    uint32_t nsel;          // BGK selections this step
+};
+
+// BGK-family parameters of one cell for the velocity draws and the
+// conservation step (one record per cell of a BGK-active case).
+struct BgkP {
+    double U[3];
+    double s2kT, p, cQ, cS, m0, T;
+    double q[3];
+    double sm[9];      // ES-BGK S matrix (row major) or USP stress (xx xy xz yy yz zz)
+    int32_t perform;
+    int32_t model;
 };
 
 // Per-case cumulative counters (uint64).
@@ -52,7 +64,18 @@ struct WorldView {
     uint32_t* cursor;         // [nc]
     uint32_t* perm;           // [cap]
     uint32_t* ins_off;        // [n_inflow+1]
-    uint32_t* scratch;        // [cap] per-cell scratch (BGK index lists)
+    uint32_t* scratch;        // [cap] rank scratch of tiles larger than LDS
+    uint32_t* tile_first;     // [max tiles + 2] first cell of each tile
+    double* mom;              // [kNMom * nc] per-cell sums for the BGK family
+    BgkP* bgkp;               // [nc]
+    unsigned long long* raise;// [nc] USP maxProb raise (bits of a positive double)
+    uint32_t* sel_slot;       // [cap] BGK selections: sorted slot,
+    uint32_t* sel_cell;       //       cell,
+    uint32_t* sel_k;          //       selection index in the cell
+    int32_t ns_max;           // largest species count of a case
+    int32_t sum_level;        // largest sampler level of a case (0, 1, 2)
+    int32_t has_collide;      // some case collides (DSMC or BGK family)
+    int32_t has_bgk;          // some case runs a BGK-family model
     uint32_t* blocksum;       // scan scratch
     unsigned long long* counters; // [ncases*kCntNum]
     DevState* st;
@@ -65,8 +88,22 @@ struct WorldView {
     uint32_t wq_cap;
 };
 
+// Rows of the per-cell sums kept for the BGK family: every base row and the
+// per-species counts.
+constexpr int kNMom = UGX_S_SPECIES_MCC;
+
+// Cell tiles of the sort: tile t holds the cells c whose key
+// off[c] + kCellW * c lies in [t * kTileN, (t + 1) * kTileN), so a tile has
+// at most kTileN / kCellW cells and fewer than kTileN simulators plus those
+// of its last cell.
+constexpr uint32_t kTileN = 512;
+constexpr uint32_t kCellW = 8;
+constexpr uint32_t kTileCells = kTileN / kCellW;
+constexpr uint32_t kTileMax = 1024;   // simulators of a tile staged in LDS
+inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc) / kTileN + 2; }
+
 // Host launchers (ugx_kernels.hip).
-constexpr int kPhases = 8;  // begin, inflow, move, wall, scan, scatter, cells, end
+constexpr int kPhases = 9;  // begin, inflow, move, wall, scan, scatter, cells, collide, end
 template <class R> void launch_step(const WorldView<R>& w, hipStream_t s, hipEvent_t* marks);
 template <class R> void launch_sort(const WorldView<R>& w, hipStream_t s);
 template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase,

@@ -297,6 +297,8 @@ void World::release()
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
                      (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
+                     (void**)&d_tile_first, (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
+                     (void**)&d_sel_slot, (void**)&d_sel_cell, (void**)&d_sel_k,
                      (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st,
                      (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
@@ -334,11 +336,21 @@ int World::alloc_particles(uint32_t newcap)
     cur = 0;
     dfree(d_perm);
     dfree(d_scratch);
+    dfree(d_tile_first);
+    dfree(d_sel_slot);
+    dfree(d_sel_cell);
+    dfree(d_sel_k);
     dfree(d_wq_slot);
     dfree(d_wq_info);
     dfree(d_wq_tau);
     if (int rc = dalloc(d_perm, newcap)) return rc;
     if (int rc = dalloc(d_scratch, newcap)) return rc;
+    if (int rc = dalloc(d_tile_first, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
+    if (has_bgk) {
+        if (int rc = dalloc(d_sel_slot, newcap)) return rc;
+        if (int rc = dalloc(d_sel_cell, newcap)) return rc;
+        if (int rc = dalloc(d_sel_k, newcap)) return rc;
+    }
     // diffuse-wall queue: at most half of the simulators may reach a diffuse wall in one step
     wq_cap = std::max<uint32_t>(newcap / 2, 1024u);
     if (int rc = dalloc(d_wq_slot, wq_cap)) return rc;
@@ -399,6 +411,17 @@ WorldView<R> World::view() const
     w.perm = d_perm;
     w.ins_off = d_ins_off;
     w.scratch = d_scratch;
+    w.tile_first = d_tile_first;
+    w.mom = d_mom;
+    w.bgkp = d_bgkp;
+    w.raise = d_raise;
+    w.sel_slot = d_sel_slot;
+    w.sel_cell = d_sel_cell;
+    w.sel_k = d_sel_k;
+    w.ns_max = ns_max;
+    w.sum_level = sum_level;
+    w.has_collide = has_collide;
+    w.has_bgk = has_bgk;
     w.blocksum = d_blocksum;
     w.counters = d_counters;
     w.st = d_st;
@@ -467,6 +490,12 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
             if (bgk_on && s.model.bgk_model == UGX_BGK_ESBGK) lvl = std::max(lvl, 1);
             if (bgk_on && s.model.bgk_model == UGX_BGK_USP) lvl = 2;
             d.sum_level = lvl;
+            sum_level = std::max(sum_level, lvl);
+            ns_max = std::max(ns_max, s.ns);
+            const bool dsmc_on = (s.model.collision_model == UGX_COLL_DSMC || s.model.collision_model == UGX_COLL_HYBRID)
+                                 && s.model.dsmc_model != UGX_DSMC_NONE;
+            if (bgk_on && s.model.bgk_model != UGX_BGK_NONE) { has_bgk = 1; has_collide = 1; }
+            if (dsmc_on) has_collide = 1;
         }
         int cm = (s.model.collision_model == UGX_COLL_DSMC) ? 1 : 0;  // hybrid starts all-BGK (uniGasCloud.C:726-732)
         for (int c = 0; c < g.nc; ++c) {
@@ -586,6 +615,11 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_cursor, zc))) return rc;
     std::vector<uint32_t> zi((size_t)n_inflow + 1, 0u);
     if ((rc = up(d_ins_off, zi))) return rc;
+    if (has_bgk) {
+        if ((rc = dalloc(d_mom, (size_t)nc * kNMom))) return rc;
+        if ((rc = dalloc(d_bgkp, (size_t)nc))) return rc;
+        if ((rc = dalloc(d_raise, (size_t)nc))) return rc;
+    }
     size_t nbs = ((size_t)nc * kNS + 2 + 1023) / 1024 + 2;
     if ((rc = dalloc(d_blocksum, nbs))) return rc;
     std::vector<unsigned long long> zcnt(specs.size() * kCntNum, 0ull);

@@ -75,6 +75,14 @@ struct World {
     uint32_t* d_perm = nullptr;
     uint32_t* d_ins_off = nullptr;
     uint32_t* d_scratch = nullptr;
+    uint32_t* d_tile_first = nullptr;
+    double* d_mom = nullptr;
+    BgkP* d_bgkp = nullptr;
+    unsigned long long* d_raise = nullptr;
+    uint32_t* d_sel_slot = nullptr;
+    uint32_t* d_sel_cell = nullptr;
+    uint32_t* d_sel_k = nullptr;
+    int32_t ns_max = 1, sum_level = 0, has_collide = 0, has_bgk = 0;
     uint32_t* d_blocksum = nullptr;
     unsigned long long* d_counters = nullptr;
     DevState* d_st = nullptr;
