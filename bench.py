@@ -131,18 +131,25 @@ def main():
     nc = spec.mesh.n_cells
     cand = (c1["candidates"] - c0["candidates"]) / args.steps
     coll = (c1["collisions"] - c0["collisions"]) / args.steps
-    n_bgk = 0.0 if spec.collision_model == "dsmc" else n_avg * (1.0 - c1["n_dsmc_cells"] / nc)
+    n_bgk = (c1["bgk_collisions"] - c0["bgk_collisions"]) / args.steps
+    bgk_on = spec.collision_model in ("bgk", "hybrid") and spec.bgk_model != "noBGKCollision"
+    kw = dict(rows=roofline.sampled_rows(spec.sampler_full, len(spec.species)), bgk=bgk_on,
+              hybrid=spec.collision_model == "hybrid")
+
+    def pbytes(p):
+        return roofline.phase_bytes(p, R, n_avg, nc, cand, coll, n_bgk, **kw)
+
     per_phase = {}
     for p, ms in phases.items():
         avg = ms / max(tsteps, 1)
-        b = roofline.phase_bytes(p, R, n_avg, nc, cand, coll, n_bgk)
+        b = pbytes(p)
         per_phase[p] = {"ms": round(avg, 4), "GBs": round(b / (avg * 1e-3) / 1e9, 1) if b and avg > 0 else None}
-    byte_phases = [p for p in per_phase if roofline.phase_bytes(p, R, n_avg, nc, cand, coll, n_bgk) > 0]
+    byte_phases = [p for p in per_phase if pbytes(p) > 0]
     dom = max(byte_phases, key=lambda p: per_phase[p]["ms"])
-    dom_bytes = roofline.phase_bytes(dom, R, n_avg, nc, cand, coll, n_bgk)
+    dom_bytes = pbytes(dom)
     dom_s = per_phase[dom]["ms"] * 1e-3
     achieved = dom_bytes / dom_s / 1e9
-    step_b = roofline.step_bytes(R, n_avg, nc, cand, coll, n_bgk)
+    step_b = roofline.step_bytes(R, n_avg, nc, cand, coll, n_bgk, **kw)
     step_gbs = step_b / (elapsed / args.steps) / 1e9
     out = {
         "metric": METRIC,
@@ -166,7 +173,7 @@ def main():
                            "frac": round(step_gbs / roofline.HBM_PEAK_GBS, 4)},
         "phases": per_phase,
         "counters_per_step": {"simulators": n_avg, "candidates": cand, "collisions": coll,
-                              "dsmc_cells": c1["n_dsmc_cells"]},
+                              "bgk_redrawn": n_bgk, "dsmc_cells": c1["n_dsmc_cells"]},
     }
     if args.probe:
         nbytes = 2 << 30

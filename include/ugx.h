@@ -136,7 +136,9 @@ typedef struct {
     int32_t sampler_full;             /* 1: also sample the second moments and heat-flux
                                          moments (measureHeatFluxShearStress); 0: the
                                          uniGasVolFields set (rows 0-7 and per species) */
-    int32_t pad0;
+    int32_t reset_at_decomposition;   /* hybrid: clear the Kn time sums at each
+                                         decomposition (default on) */
+    double  reset_until_time;         /* ... while time < this (<= 0: always) */
 } ugx_model_desc;
 
 typedef enum {

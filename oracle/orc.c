@@ -38,6 +38,14 @@
 #define SMALL 1.0e-15
 #define MAXF UGX_MAX_CELL_FACES
 #define NBR_EMPTY (-2147483647 - 1)
+/* cell-face kinds for the hybrid decomposition's face interpolation */
+#define KNR 16            /* hybrid time-sum rows per cell */
+#define KN_NB_CAP 256     /* neighbourhood list capacity (3 levels of <= 6 faces: <= 187) */
+#define FACE_SKIP 0       /* padded slot or empty patch: no face value */
+#define FACE_INTERNAL 1
+#define FACE_CYCLIC 2
+#define FACE_BOUNDARY 3   /* zeroGradient: face value = cell value */
+#define FACE_SYMMETRY 4   /* symmetryPlane: vectors lose their normal component */
 #define CELL_DEAD (-1)
 #define MAX_CROSSINGS 1000
 #define NS UGX_MAX_SPECIES
@@ -73,6 +81,8 @@ struct orc_world {
     double* plane;    /* [nc*6*4] */
     int32_t* nbr;     /* [nc*6] */
     double* delta;    /* [nc*6*3] */
+    double* farea;    /* [nc*6] face area (0 for padded and empty faces) */
+    int8_t* fkind;    /* [nc*6] FACE_* */
     double mask[3];   /* solution directions (0 for empty directions) */
     int npatch;
     bmodel* bm;       /* [npatch] */
@@ -101,7 +111,7 @@ struct orc_world {
     double* uspq;     /* [nc*3] previous heat flux */
     double* usps;     /* [nc*6] previous stress */
     double* uspmax;   /* [nc] */
-    double* knacc;    /* hybrid: time sums since the last decomposition [nc*8] */
+    double* knacc;    /* hybrid: time sums since the last decomposition [nc*KNR] */
     double* kn;       /* [nc] */
     double kn_time;
     int kn_steps;
@@ -155,6 +165,8 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
     w->plane = calloc((size_t)nc * MAXF * 4, sizeof(double));
     w->nbr = calloc((size_t)nc * MAXF, sizeof(int32_t));
     w->delta = calloc((size_t)nc * MAXF * 3, sizeof(double));
+    w->farea = calloc((size_t)nc * MAXF, sizeof(double));
+    w->fkind = calloc((size_t)nc * MAXF, sizeof(int8_t));
     int32_t* cf = malloc(sizeof(int32_t) * (size_t)nc * MAXF);
     double* pc = malloc(sizeof(double) * (size_t)nf * 3);
     double* A = malloc(sizeof(double) * (size_t)nf * 3);
@@ -242,10 +254,12 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
             for (int d = 0; d < 3; ++d) nn[d] = sg * (a[d] / mag);
             pl[0] = nn[0]; pl[1] = nn[1]; pl[2] = nn[2];
             pl[3] = dot3(nn, r);
+            w->farea[c * MAXF + j] = mag;
             double* dl = &w->delta[(c * MAXF + j) * 3];
             if (f < m->n_internal_faces) {
                 int o = (m->owner[f] == c) ? m->neighbour[f] : m->owner[f];
                 w->nbr[c * MAXF + j] = o;
+                w->fkind[c * MAXF + j] = FACE_INTERNAL;
                 for (int d = 0; d < 3; ++d) dl[d] = w->cc[3 * c + d] - w->cc[3 * o + d];
             } else {
                 int p = fpatch[f];
@@ -253,7 +267,9 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
                 int t = m->patch_type[p];
                 if (t == UGX_PATCH_EMPTY) {
                     w->nbr[c * MAXF + j] = NBR_EMPTY;
+                    w->farea[c * MAXF + j] = 0.0;
                 } else if (t == UGX_PATCH_CYCLIC) {
+                    w->fkind[c * MAXF + j] = FACE_CYCLIC;
                     int q = m->patch_neighbour ? m->patch_neighbour[p] : -1;
                     if (q < 0 || m->patch_size[q] != m->patch_size[p]) return fail(UGX_ERR_MESH, "cyclic patch without a matching partner");
                     int g = m->patch_start[q] + (f - m->patch_start[p]);
@@ -263,6 +279,7 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
                         dl[d] = (w->cc[3 * c + d] - w->cc[3 * o + d]) + (pc[3 * g + d] - pc[3 * f + d]);
                 } else {
                     w->nbr[c * MAXF + j] = -1 - p;
+                    w->fkind[c * MAXF + j] = (t == UGX_PATCH_SYMMETRY) ? FACE_SYMMETRY : FACE_BOUNDARY;
                 }
             }
         }
@@ -927,7 +944,7 @@ int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n
     w->uspq = xcalloc((size_t)nc * 3, sizeof(double));
     w->usps = xcalloc((size_t)nc * 6, sizeof(double));
     w->uspmax = xcalloc((size_t)nc, sizeof(double));
-    w->knacc = xcalloc((size_t)nc * 8, sizeof(double));
+    w->knacc = xcalloc((size_t)nc * KNR, sizeof(double));
     w->kn = xcalloc((size_t)nc, sizeof(double));
     int cm = (model->collision_model == UGX_COLL_DSMC) ? 1 : 0;  /* hybrid starts all-BGK (uniGasCloud.C:726-732) */
     for (int c = 0; c < nc; ++c) { w->cmodel[c] = cm; w->uspmax[c] = 1.0; }
@@ -938,7 +955,7 @@ int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n
 void orc_destroy(orc_world* w)
 {
     if (!w) return;
-    void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->bm, w->inflow,
+    void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->farea, w->fkind, w->bm, w->inflow,
                     w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
                     w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
@@ -1176,8 +1193,10 @@ int orc_step_phases(orc_world* w, int mask)
                 for (int k = 0; k < NSAMP; ++k)
                     if (w->model.sampler_full || k < UGX_S_MUU || k >= UGX_S_BASE)
                         w->samp[(size_t)k * w->nc + c] += dt * S[k];
-            if (hybrid)
-                for (int k = 0; k < 8; ++k) w->knacc[8 * c + k] += dt * S[k];
+            if (hybrid) {
+                for (int k = 0; k < 8; ++k) w->knacc[KNR * c + k] += dt * S[k];
+                for (int k = 0; k < NS; ++k) w->knacc[KNR * c + 8 + k] += dt * S[UGX_S_SPECIES_N + k];
+            }
             if ((cm == UGX_COLL_DSMC || hybrid) && w->cmodel[c] == 1 && w->model.dsmc_model != UGX_DSMC_NONE)
                 ntc_cell(w, c, b, nC, &cand, &coll);
             if ((cm == UGX_COLL_BGK || hybrid) && w->model.bgk_model != UGX_BGK_NONE)
@@ -1206,82 +1225,203 @@ int orc_step(orc_world* w, int n)
     return UGX_OK;
 }
 
-/* Hybrid decomposition restated: the face-neighbour graph replaces
- * fvc::average(fvc::interpolate()), i.e. a smoothing pass sets each cell
- * value to the mean over its faces of the face value (internal face: mean
- * of the two cells; boundary face: the cell value). */
+/* ------------------------------------------------ hybrid decomposition */
+
+/* fvc::average(fvc::interpolate(f)) of a cell field on the cell-face
+ * representation (localKnudsen.C:287-297, 407-417): face values by linear
+ * interpolation with the weight w = dN / (dP + dN) of the cell's own value
+ * (dP = distance of the face plane from the cell reference point, dN the
+ * same for the neighbour, both along the face normal); zeroGradient boundary
+ * faces carry the cell value, symmetry faces the cell value less its normal
+ * component (vectors), empty faces none; the cell value is the
+ * area-weighted mean of its face values.  ncomp = 1 (scalar) or 3. */
+static void smooth_field(const orc_world* w, const double* f, double* out, int ncomp)
+{
+    for (int c = 0; c < w->nc; ++c) {
+        double num[3] = {0.0, 0.0, 0.0}, den = 0.0;
+        for (int j = 0; j < MAXF; ++j) {
+            int k = c * MAXF + j;
+            int kind = w->fkind[k];
+            if (kind == FACE_SKIP) continue;
+            double a = w->farea[k];
+            const double* pl = &w->plane[4 * k];
+            double fv[3];
+            if (kind == FACE_INTERNAL || kind == FACE_CYCLIC) {
+                int o = w->nbr[k];
+                const double* dl = &w->delta[3 * k];
+                double dP = fabs(pl[3]);
+                double dN = fabs((pl[0] * dl[0] + pl[1] * dl[1] + pl[2] * dl[2]) + pl[3]);
+                double wt = dN / (dP + dN);
+                for (int q = 0; q < ncomp; ++q) fv[q] = wt * f[ncomp * c + q] + (1.0 - wt) * f[ncomp * o + q];
+            } else if (kind == FACE_SYMMETRY && ncomp == 3) {
+                const double* u = &f[3 * c];
+                double un = pl[0] * u[0] + pl[1] * u[1] + pl[2] * u[2];
+                for (int q = 0; q < 3; ++q) fv[q] = u[q] - un * pl[q];
+            } else {
+                for (int q = 0; q < ncomp; ++q) fv[q] = f[ncomp * c + q];
+            }
+            for (int q = 0; q < ncomp; ++q) num[q] = num[q] + a * fv[q];
+            den = den + a;
+        }
+        for (int q = 0; q < ncomp; ++q) out[ncomp * c + q] = (den > 0.0) ? num[q] / den : f[ncomp * c + q];
+    }
+}
+
+/* Cells within nLevels face-neighbour steps (internal faces only), the cell
+ * included: uniGasHybridDecomposition::fetchCellNeighborhood
+ * (uniGasHybridDecomposition.C:127-174). */
+static int neighbourhood(const orc_world* w, int c, int levels, int* list, int cap)
+{
+    int n = 0, lo = 0;
+    list[n++] = c;
+    for (int l = 0; l < levels; ++l) {
+        int hi = n;
+        for (int i = lo; i < hi; ++i)
+            for (int j = 0; j < MAXF; ++j) {
+                int k = list[i] * MAXF + j;
+                if (w->fkind[k] != FACE_INTERNAL) continue;
+                int o = w->nbr[k], seen = 0;
+                for (int q = 0; q < n; ++q) if (list[q] == o) { seen = 1; break; }
+                if (!seen && n < cap) list[n++] = o;
+            }
+        lo = hi;
+    }
+    return n;
+}
+
+/* localKnudsen::decompose at a decomposition step (localKnudsen.C:243-562).
+ * Time sums (knacc, KNR rows per cell: N, M, M u, M v, M w, M |u|^2, -, -,
+ * then the per-species counts) are Delta-t weighted raw cell sums; the
+ * XnParticle sums of the reference are F_N times them.  The refinement
+ * passes (:436-545) decide every cell of a loop from the model ids as they
+ * were at the start of that loop (the reference updates in place in cell
+ * order, which makes the result depend on the cell numbering). */
 static void hybrid_switch(orc_world* w)
 {
     int nc = w->nc;
-    if (w->kn_time <= 0.0) return;
-    double* rho = malloc(sizeof(double) * nc);
-    double* T = malloc(sizeof(double) * nc);
-    double* Um = malloc(sizeof(double) * nc);
-    double* mfp = malloc(sizeof(double) * nc);
-    double* cmp = malloc(sizeof(double) * nc);
-    double* tmp = malloc(sizeof(double) * nc);
-    double FN = w->model.n_equivalent_particles;
+    double FN = w->model.n_equivalent_particles, tav = w->kn_time;
+    double bmax = w->model.breakdown_max, th = w->model.kn_theta;
+    int passes = w->model.smoothing_passes;
+    double* rhoN = calloc((size_t)nc, sizeof(double));
+    double* sc = calloc((size_t)nc * 3, sizeof(double));      /* rhoM, p, T */
+    double* U = calloc((size_t)nc * 3, sizeof(double));
+    double* tmp = calloc((size_t)nc * 3, sizeof(double));
+    double* rhoM = calloc((size_t)nc, sizeof(double));
+    double* p = calloc((size_t)nc, sizeof(double));
+    double* T = calloc((size_t)nc, sizeof(double));
     for (int c = 0; c < nc; ++c) {
-        const double* a = &w->knacc[8 * c];
-        double N = a[UGX_S_N], M = a[UGX_S_M];
-        rho[c] = M * FN / (w->kn_time * w->vol[c]);
-        if (N > VSMALL && M > 0.0) {
-            double U[3] = {a[UGX_S_MU] / M, a[UGX_S_MV] / M, a[UGX_S_MW] / M};
-            double uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
-            T[c] = (a[UGX_S_MCC] - M * uu) / (3.0 * KB * N);
-            Um[c] = sqrt(uu);
-            double mbar = M / N;
-            double nd = N * FN / (w->kn_time * w->vol[c]);
-            /* Bird eq. 4.65 hard-sphere-like VHS mean free path of the mixture mean */
-            double d = w->diam[0], om = w->omega[0];
-            mfp[c] = (T[c] > VSMALL && nd > 0.0)
-                         ? 1.0 / (sqrt(2.0) * PI_D * d * d * nd * orc_pow(w->model.Tref / T[c], om - 0.5))
-                         : 1.0e300;
-            cmp[c] = (T[c] > 0.0) ? sqrt(2.0 * KB * T[c] / mbar) : 0.0;
-        } else {
-            T[c] = 0.0; Um[c] = 0.0; mfp[c] = 1.0e300; cmp[c] = 0.0;
+        const double* a = &w->knacc[KNR * c];
+        double V = w->vol[c];
+        if (a[0] > VSMALL) {
+            rhoN[c] = FN * a[0] / (tav * V);
+            rhoM[c] = FN * a[1] / (tav * V);
+            double rhoMMean = FN * a[1] / (V * tav);
+            for (int d = 0; d < 3; ++d) U[3 * c + d] = FN * a[2 + d] / (rhoMMean * V * tav);
+            double ke = 0.5 * (FN * a[5]) / (V * tav);
+            double rhoNMean = FN * a[0] / (V * tav);
+            double uu = U[3 * c] * U[3 * c] + U[3 * c + 1] * U[3 * c + 1] + U[3 * c + 2] * U[3 * c + 2];
+            T[c] = 2.0 / (3.0 * KB * rhoNMean) * (ke - 0.5 * rhoMMean * uu);
+            p[c] = rhoN[c] * KB * T[c];
         }
     }
-    double* fields[3] = {rho, T, Um};
-    for (int fi = 0; fi < 3; ++fi) {
-        double* f = fields[fi];
-        for (int pass = 0; pass < w->model.smoothing_passes; ++pass) {
-            for (int c = 0; c < nc; ++c) {
-                double s = 0.0;
-                int k = 0;
-                for (int j = 0; j < w->ncf[c]; ++j) {
-                    int o = w->nbr[c * MAXF + j];
-                    s += (o >= 0) ? 0.5 * (f[c] + f[o]) : f[c];
-                    k++;
-                }
-                tmp[c] = s / (double)k;
-            }
-            memcpy(f, tmp, sizeof(double) * nc);
-        }
+    for (int pass = 0; pass < passes; ++pass) {
+        smooth_field(w, rhoM, tmp, 1); memcpy(rhoM, tmp, sizeof(double) * nc);
+        smooth_field(w, p, tmp, 1); memcpy(p, tmp, sizeof(double) * nc);
+        smooth_field(w, T, tmp, 1); memcpy(T, tmp, sizeof(double) * nc);
+        smooth_field(w, U, tmp, 3); memcpy(U, tmp, sizeof(double) * nc * 3);
     }
     for (int c = 0; c < nc; ++c) {
-        double g = 0.0;
-        for (int j = 0; j < w->ncf[c]; ++j) {
-            int o = w->nbr[c * MAXF + j];
-            if (o < 0) continue;
-            const double* dl = &w->delta[(c * MAXF + j) * 3];
+        double gr = 0.0, gt = 0.0, gu = 0.0;
+        double mU = sqrt(U[3 * c] * U[3 * c] + U[3 * c + 1] * U[3 * c + 1] + U[3 * c + 2] * U[3 * c + 2]);
+        for (int j = 0; j < MAXF; ++j) {
+            int k = c * MAXF + j;
+            if (w->fkind[k] != FACE_INTERNAL) continue;
+            int o = w->nbr[k];
+            const double* dl = &w->delta[3 * k];
             double dist = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
-            double gr = (rho[c] > 0.0) ? fabs(rho[o] - rho[c]) / dist / rho[c] : 0.0;
-            double gt = (T[c] > 0.0) ? fabs(T[o] - T[c]) / dist / T[c] : 0.0;
-            double den = Um[c] > cmp[c] ? Um[c] : cmp[c];
-            double gu = (den > 0.0) ? fabs(Um[o] - Um[c]) / dist / den : 0.0;
-            double mx = gr > gt ? gr : gt;
-            mx = mx > gu ? mx : gu;
-            if (mx > g) g = mx;
+            double mUo = sqrt(U[3 * o] * U[3 * o] + U[3 * o + 1] * U[3 * o + 1] + U[3 * o + 2] * U[3 * o + 2]);
+            double v;
+            v = fabs(rhoM[o] - rhoM[c]) / dist; if (gr < v) gr = v;
+            v = fabs(T[o] - T[c]) / dist; if (gt < v) gt = v;
+            v = fabs(mUo - mU) / dist; if (gu < v) gu = v;
         }
-        double knc = (mfp[c] < 1.0e299) ? mfp[c] * g : 1.0e300;
-        w->kn[c] = w->model.kn_theta * knc + (1.0 - w->model.kn_theta) * w->kn[c];
-        w->cmodel[c] = (w->kn[c] > w->model.breakdown_max) ? 1 : 0;
+        double inst;
+        const double* a = &w->knacc[KNR * c];
+        if (a[0] > VSMALL && T[c] > VSMALL) {
+            double V = w->vol[c];
+            double smfp[NS];
+            for (int i = 0; i < w->ns; ++i) {
+                smfp[i] = 0.0;
+                for (int q = 0; q < w->ns; ++q) {
+                    double dPQ = 0.5 * (w->diam[i] + w->diam[q]);
+                    double omPQ = 0.5 * (w->omega[i] + w->omega[q]);
+                    double mr = w->mass[i] / w->mass[q];
+                    double nq = FN * a[8 + q];
+                    if (nq > VSMALL) {
+                        double nD = nq / V;
+                        /* Bird eq. 4.76 */
+                        smfp[i] += PI_D * dPQ * dPQ * nD * orc_pow(w->model.Tref / T[c], omPQ - 0.5) * sqrt(1.0 + mr);
+                    }
+                }
+            }
+            double mfp = 0.0;
+            for (int i = 0; i < w->ns; ++i) {
+                double ni = FN * a[8 + i];
+                if (ni > VSMALL) {
+                    smfp[i] = 1.0 / smfp[i];
+                    mfp += smfp[i] * ni / (rhoN[c] * V);      /* Bird eq. 4.77 */
+                }
+            }
+            double u0 = sqrt(2.0 * KB / (rhoM[c] / rhoN[c]) * T[c]);
+            double kr = mfp * gr / rhoM[c];
+            double kt = mfp * gt / T[c];
+            double ku = mfp * gu / ((mU < u0) ? u0 : mU);
+            inst = (kr < kt) ? kt : kr;
+            inst = (inst < ku) ? ku : inst;
+        } else {
+            inst = 2.0 * bmax;
+        }
+        w->kn[c] = th * inst + (1.0 - th) * w->kn[c];
     }
-    memset(w->knacc, 0, sizeof(double) * 8 * (size_t)nc);
-    w->kn_time = 0.0;
+    for (int pass = 0; pass < passes; ++pass) {
+        smooth_field(w, w->kn, tmp, 1);
+        memcpy(w->kn, tmp, sizeof(double) * nc);
+    }
+    for (int c = 0; c < nc; ++c) w->cmodel[c] = (w->kn[c] > bmax) ? 1 : 0;
+    /* refinement: 3 passes, neighbourhood of 3 levels, fraction 0.4
+     * (uniGasHybridDecomposition.C:73-75) */
+    int32_t* ids = malloc(sizeof(int32_t) * (size_t)nc);
+    int list[KN_NB_CAP];
+    for (int pass = 0; pass < 3; ++pass) {
+        for (int loop = 0; loop < 2; ++loop) {
+            int me = (loop == 0) ? 1 : 0;          /* 1: DSMC cells, 0: BGK cells */
+            memcpy(ids, w->cmodel, sizeof(int32_t) * nc);
+            for (int c = 0; c < nc; ++c) {
+                if (ids[c] != me) continue;
+                int same = 0, other = 0;
+                for (int j = 0; j < MAXF; ++j) {
+                    int k = c * MAXF + j;
+                    if (w->fkind[k] != FACE_INTERNAL) continue;
+                    if (ids[w->nbr[k]] == me) same++; else other++;
+                }
+                if (same == 0 || (same == 1 && other > 1)) { w->cmodel[c] = 1 - me; continue; }
+                int n = neighbourhood(w, c, 3, list, KN_NB_CAP);
+                int cnt = 0;
+                for (int q = 0; q < n; ++q) if (ids[list[q]] == me) cnt++;
+                if ((double)cnt < 0.4 * (double)n) w->cmodel[c] = 1 - me;
+            }
+        }
+    }
+    free(ids);
+    /* reset (localKnudsen.C:520-542): time = (step + 1) deltaT */
+    double time = (double)(w->step + 1) * w->model.delta_t;
+    double until = (w->model.reset_until_time > 0.0) ? w->model.reset_until_time : 1.0e300;
+    if (w->model.reset_at_decomposition && time < until + 0.5 * w->model.delta_t) {
+        memset(w->knacc, 0, sizeof(double) * KNR * (size_t)nc);
+        w->kn_time = 0.0;
+    }
     w->kn_steps = 0;
-    free(rho); free(T); free(Um); free(mfp); free(cmp); free(tmp);
+    free(rhoN); free(sc); free(U); free(tmp); free(rhoM); free(p); free(T);
 }
 
 /* ------------------------------------------------------------ accessors */
@@ -1317,6 +1457,13 @@ int orc_set_sigma_max(orc_world* w, const double* in, int64_t len)
     return UGX_OK;
 }
 
+int orc_kn(orc_world* w, double* out, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(out, w->kn, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
 int orc_cell_model(orc_world* w, int32_t* out, int64_t len)
 {
     if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
@@ -1334,6 +1481,12 @@ int orc_geometry(orc_world* w, double* cc, double* vol, int32_t* nfaces, double*
     if (planes) memcpy(planes, w->plane, sizeof(double) * 4 * MAXF * nc);
     if (nbr) memcpy(nbr, w->nbr, sizeof(int32_t) * MAXF * nc);
     if (delta) memcpy(delta, w->delta, sizeof(double) * 3 * MAXF * nc);
+    return UGX_OK;
+}
+
+int orc_face_data(orc_world* w, double* farea, int32_t* fkind)
+{
+    for (int k = 0; k < w->nc * MAXF; ++k) { farea[k] = w->farea[k]; fkind[k] = w->fkind[k]; }
     return UGX_OK;
 }
 

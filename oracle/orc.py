@@ -48,9 +48,11 @@ def lib():
         L.orc_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.orc_set_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.orc_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.orc_kn.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_fast_sums.argtypes = [C.c_int]
         L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
+        L.orc_face_data.argtypes = [vp, abi.DBL_P, abi.I32_P]
         L.orc_inflow_nbar.argtypes = [vp, abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
@@ -141,12 +143,19 @@ class OracleCase:
         _check(self.L.orc_cell_model(self.h, abi.iptr(out), self.nc))
         return out
 
+    def kn(self):
+        out = np.zeros(self.nc)
+        _check(self.L.orc_kn(self.h, abi.dptr(out), self.nc))
+        return out
+
     def geometry(self):
         n = self.nc
         g = {"cc": np.zeros((n, 3)), "vol": np.zeros(n), "nfaces": np.zeros(n, dtype=np.int32),
-             "planes": np.zeros((n, 6, 4)), "nbr": np.zeros((n, 6), dtype=np.int32), "delta": np.zeros((n, 6, 3))}
+             "planes": np.zeros((n, 6, 4)), "nbr": np.zeros((n, 6), dtype=np.int32), "delta": np.zeros((n, 6, 3)),
+             "farea": np.zeros((n, 6)), "fkind": np.zeros((n, 6), dtype=np.int32)}
         _check(self.L.orc_geometry(self.h, abi.dptr(g["cc"]), abi.dptr(g["vol"]), abi.iptr(g["nfaces"]),
                                    abi.dptr(g["planes"]), abi.iptr(g["nbr"]), abi.dptr(g["delta"])))
+        _check(self.L.orc_face_data(self.h, abi.dptr(g["farea"]), abi.iptr(g["fkind"])))
         nb = np.zeros(1 << 16)
         k = C.c_int64()
         _check(self.L.orc_inflow_nbar(self.h, abi.dptr(nb), len(nb), C.byref(k)))

@@ -63,7 +63,8 @@ class ModelDesc(C.Structure):
         ("macro_interpolation", C.c_int32), ("sample_interval", C.c_int32),
         ("decomposition_interval", C.c_int32), ("smoothing_passes", C.c_int32),
         ("breakdown_max", C.c_double), ("kn_theta", C.c_double),
-        ("sampler_full", C.c_int32), ("pad0", C.c_int32),
+        ("sampler_full", C.c_int32), ("reset_at_decomposition", C.c_int32),
+        ("reset_until_time", C.c_double),
     ]
 
 

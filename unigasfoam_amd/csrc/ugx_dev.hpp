@@ -24,6 +24,12 @@ constexpr double kVSmall = 1.0e-300;
 constexpr double kSmall = 1.0e-15;
 constexpr int kMaxFaces = UGX_MAX_CELL_FACES;
 constexpr int32_t kNbrEmpty = INT32_MIN;
+// Cell-face kinds for the hybrid decomposition's face interpolation.
+constexpr int8_t kFaceSkip = 0;       // padded slot or empty patch: no face value
+constexpr int8_t kFaceInternal = 1;
+constexpr int8_t kFaceCyclic = 2;
+constexpr int8_t kFaceBoundary = 3;   // zeroGradient: face value = cell value
+constexpr int8_t kFaceSymmetry = 4;   // symmetryPlane: vectors lose their normal component
 constexpr uint32_t kDead = 0xFFFFFFFFu;
 constexpr int kMaxCrossings = 1000;
 constexpr int kNS = UGX_MAX_SPECIES;

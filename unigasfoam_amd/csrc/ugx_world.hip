@@ -73,6 +73,8 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
     g.ncf.assign(nc, 0);
     g.plane.assign((size_t)nc * kMaxFaces * 4, 0.0);
     g.nbr.assign((size_t)nc * kMaxFaces, kNbrEmpty);
+    g.farea.assign((size_t)nc * kMaxFaces, 0.0);
+    g.fkind.assign((size_t)nc * kMaxFaces, kFaceSkip);
     g.delta.assign((size_t)nc * kMaxFaces * 3, 0.0);
     std::vector<int32_t> cf((size_t)nc * kMaxFaces);
     std::vector<double> pc((size_t)nf * 3), A((size_t)nf * 3);
@@ -164,7 +166,10 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
         double vs = 0.0;
         for (int j = 0; j < kMaxFaces; ++j) {
             double* pl = &g.plane[((size_t)c * kMaxFaces + j) * 4];
-            if (j >= g.ncf[c]) { pl[0] = pl[1] = pl[2] = 0.0; pl[3] = 1.0; g.nbr[(size_t)c * kMaxFaces + j] = kNbrEmpty; continue; }
+            const size_t k = (size_t)c * kMaxFaces + j;
+            g.farea[k] = 0.0;
+            g.fkind[k] = kFaceSkip;
+            if (j >= g.ncf[c]) { pl[0] = pl[1] = pl[2] = 0.0; pl[3] = 1.0; g.nbr[k] = kNbrEmpty; continue; }
             int f = cf[(size_t)c * kMaxFaces + j];
             double sg = (m->owner[f] == c) ? 1.0 : -1.0;
             double a[3] = {A[3 * (size_t)f], A[3 * (size_t)f + 1], A[3 * (size_t)f + 2]};
@@ -177,10 +182,12 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
             for (int d = 0; d < 3; ++d) nn[d] = sg * (a[d] / mag);
             pl[0] = nn[0]; pl[1] = nn[1]; pl[2] = nn[2];
             pl[3] = dot3(nn, r);
+            g.farea[k] = mag;
             double* dl = &g.delta[((size_t)c * kMaxFaces + j) * 3];
             if (f < m->n_internal_faces) {
                 int o = (m->owner[f] == c) ? m->neighbour[f] : m->owner[f];
                 g.nbr[(size_t)c * kMaxFaces + j] = o;
+                g.fkind[k] = kFaceInternal;
                 for (int d = 0; d < 3; ++d) dl[d] = g.cc[3 * (size_t)c + d] - g.cc[3 * (size_t)o + d];
             } else {
                 int p = fpatch[f];
@@ -188,7 +195,9 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
                 int t = m->patch_type[p];
                 if (t == UGX_PATCH_EMPTY) {
                     g.nbr[(size_t)c * kMaxFaces + j] = kNbrEmpty;
+                    g.farea[k] = 0.0;
                 } else if (t == UGX_PATCH_CYCLIC) {
+                    g.fkind[k] = kFaceCyclic;
                     int q = m->patch_neighbour ? m->patch_neighbour[p] : -1;
                     if (q < 0 || q >= m->n_patches || m->patch_size[q] != m->patch_size[p])
                         return set_err(UGX_ERR_MESH, "cyclic patch %d has no matching partner", p);
@@ -199,6 +208,7 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
                         dl[d] = (g.cc[3 * (size_t)c + d] - g.cc[3 * (size_t)o + d]) + (pc[3 * (size_t)gf + d] - pc[3 * (size_t)f + d]);
                 } else {
                     g.nbr[(size_t)c * kMaxFaces + j] = -1 - p;
+                    g.fkind[k] = (t == UGX_PATCH_SYMMETRY) ? kFaceSymmetry : kFaceBoundary;
                 }
             }
         }

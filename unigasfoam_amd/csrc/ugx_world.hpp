@@ -17,7 +17,9 @@ int set_err(int code, const char* fmt, ...);
 struct MeshGeom {
     int nc = 0;
     std::vector<double> cc, vol, bblo, bbhi, plane, delta;
+    std::vector<double> farea;     // per cell-face area (0 for padded and empty faces)
     std::vector<int32_t> ncf, nbr;
+    std::vector<int8_t> fkind;     // per cell-face kFace*
     double mask[3] = {1.0, 1.0, 1.0};
     std::vector<BModel> bm;        // per patch
     std::vector<InflowFS> inflow;  // face-species entries (case-local)

@@ -68,6 +68,8 @@ class CaseSpec:
     smoothing_passes: int = 5
     breakdown_max: float = 0.05
     kn_theta: float = 0.2
+    reset_at_decomposition: bool = True
+    reset_until_time: float = 0.0        # <= 0: no limit (the reference's VGREAT)
     sampler_full: bool = False
     seed: int = 12345
 
@@ -108,6 +110,8 @@ class CaseSpec:
         m.breakdown_max = self.breakdown_max
         m.kn_theta = self.kn_theta
         m.sampler_full = 1 if self.sampler_full else 0
+        m.reset_at_decomposition = 1 if self.reset_at_decomposition else 0
+        m.reset_until_time = self.reset_until_time
         nb = len(self.boundaries)
         bd = (abi.BoundaryDesc * max(nb, 1))()
         for i, b in enumerate(self.boundaries):
