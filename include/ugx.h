@@ -275,6 +275,9 @@ int ugx_counters_get(ugx_case* c, ugx_counters* out);
 int ugx_sigma_max_download(ugx_case* c, double* out, int64_t len);
 int ugx_sigma_max_upload(ugx_case* c, const double* in, int64_t len);
 int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
+/* Kn_GLL per cell of a hybrid case after the last decomposition: the
+ * KnGLL volScalarField of localKnudsen (localKnudsen.C:395-430). */
+int ugx_kn_download(ugx_case* c, double* out, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
 int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
 

@@ -1289,6 +1289,36 @@ static int neighbourhood(const orc_world* w, int c, int levels, int* list, int c
     return n;
 }
 
+/* Refinement passes of the model ids in w->cmodel: 3 passes, neighbourhood
+ * of 3 levels, fraction 0.4 (uniGasHybridDecomposition.C:73-75). */
+static void refine_models(orc_world* w)
+{
+    int nc = w->nc;
+    int32_t* ids = malloc(sizeof(int32_t) * (size_t)nc);
+    int list[KN_NB_CAP];
+    for (int pass = 0; pass < 3; ++pass) {
+        for (int loop = 0; loop < 2; ++loop) {
+            int me = (loop == 0) ? 1 : 0;          /* 1: DSMC cells, 0: BGK cells */
+            memcpy(ids, w->cmodel, sizeof(int32_t) * nc);
+            for (int c = 0; c < nc; ++c) {
+                if (ids[c] != me) continue;
+                int same = 0, other = 0;
+                for (int j = 0; j < MAXF; ++j) {
+                    int k = c * MAXF + j;
+                    if (w->fkind[k] != FACE_INTERNAL) continue;
+                    if (ids[w->nbr[k]] == me) same++; else other++;
+                }
+                if (same == 0 || (same == 1 && other > 1)) { w->cmodel[c] = 1 - me; continue; }
+                int n = neighbourhood(w, c, 3, list, KN_NB_CAP);
+                int cnt = 0;
+                for (int q = 0; q < n; ++q) if (ids[list[q]] == me) cnt++;
+                if ((double)cnt < 0.4 * (double)n) w->cmodel[c] = 1 - me;
+            }
+        }
+    }
+    free(ids);
+}
+
 /* localKnudsen::decompose at a decomposition step (localKnudsen.C:243-562).
  * Time sums (knacc, KNR rows per cell: N, M, M u, M v, M w, M |u|^2, -, -,
  * then the per-species counts) are Delta-t weighted raw cell sums; the
@@ -1388,31 +1418,7 @@ static void hybrid_switch(orc_world* w)
         memcpy(w->kn, tmp, sizeof(double) * nc);
     }
     for (int c = 0; c < nc; ++c) w->cmodel[c] = (w->kn[c] > bmax) ? 1 : 0;
-    /* refinement: 3 passes, neighbourhood of 3 levels, fraction 0.4
-     * (uniGasHybridDecomposition.C:73-75) */
-    int32_t* ids = malloc(sizeof(int32_t) * (size_t)nc);
-    int list[KN_NB_CAP];
-    for (int pass = 0; pass < 3; ++pass) {
-        for (int loop = 0; loop < 2; ++loop) {
-            int me = (loop == 0) ? 1 : 0;          /* 1: DSMC cells, 0: BGK cells */
-            memcpy(ids, w->cmodel, sizeof(int32_t) * nc);
-            for (int c = 0; c < nc; ++c) {
-                if (ids[c] != me) continue;
-                int same = 0, other = 0;
-                for (int j = 0; j < MAXF; ++j) {
-                    int k = c * MAXF + j;
-                    if (w->fkind[k] != FACE_INTERNAL) continue;
-                    if (ids[w->nbr[k]] == me) same++; else other++;
-                }
-                if (same == 0 || (same == 1 && other > 1)) { w->cmodel[c] = 1 - me; continue; }
-                int n = neighbourhood(w, c, 3, list, KN_NB_CAP);
-                int cnt = 0;
-                for (int q = 0; q < n; ++q) if (ids[list[q]] == me) cnt++;
-                if ((double)cnt < 0.4 * (double)n) w->cmodel[c] = 1 - me;
-            }
-        }
-    }
-    free(ids);
+    refine_models(w);
     /* reset (localKnudsen.C:520-542): time = (step + 1) deltaT */
     double time = (double)(w->step + 1) * w->model.delta_t;
     double until = (w->model.reset_until_time > 0.0) ? w->model.reset_until_time : 1.0e300;
@@ -1454,6 +1460,35 @@ int orc_set_sigma_max(orc_world* w, const double* in, int64_t len)
 {
     if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
     memcpy(w->smax, in, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+/* Test hooks of the hybrid decomposition (oracle only). */
+int orc_hybrid_set(orc_world* w, const double* knacc, double tav)
+{
+    memcpy(w->knacc, knacc, sizeof(double) * KNR * (size_t)w->nc);
+    w->kn_time = tav;
+    return UGX_OK;
+}
+
+int orc_hybrid_decompose(orc_world* w)
+{
+    hybrid_switch(w);
+    return UGX_OK;
+}
+
+int orc_smooth(orc_world* w, const double* in, double* out, int ncomp)
+{
+    if (ncomp != 1 && ncomp != 3) return fail(UGX_ERR_ARG, "ncomp must be 1 or 3");
+    smooth_field(w, in, out, ncomp);
+    return UGX_OK;
+}
+
+int orc_refine(orc_world* w, int32_t* ids)
+{
+    memcpy(w->cmodel, ids, sizeof(int32_t) * (size_t)w->nc);
+    refine_models(w);
+    memcpy(ids, w->cmodel, sizeof(int32_t) * (size_t)w->nc);
     return UGX_OK;
 }
 

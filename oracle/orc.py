@@ -49,6 +49,10 @@ def lib():
         L.orc_set_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.orc_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
         L.orc_kn.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.orc_hybrid_set.argtypes = [vp, abi.DBL_P, C.c_double]
+        L.orc_hybrid_decompose.argtypes = [vp]
+        L.orc_smooth.argtypes = [vp, abi.DBL_P, abi.DBL_P, C.c_int]
+        L.orc_refine.argtypes = [vp, abi.I32_P]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_fast_sums.argtypes = [C.c_int]
         L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
@@ -147,6 +151,28 @@ class OracleCase:
         out = np.zeros(self.nc)
         _check(self.L.orc_kn(self.h, abi.dptr(out), self.nc))
         return out
+
+    # hybrid decomposition hooks (tests only)
+    KN_ROWS = 16
+
+    def hybrid_set(self, knacc, tav):
+        a = np.ascontiguousarray(knacc, dtype=np.float64).reshape(self.nc, self.KN_ROWS)
+        _check(self.L.orc_hybrid_set(self.h, abi.dptr(a), float(tav)))
+
+    def hybrid_decompose(self):
+        _check(self.L.orc_hybrid_decompose(self.h))
+
+    def smooth(self, field):
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        ncomp = 1 if f.ndim == 1 else f.shape[1]
+        out = np.zeros_like(f)
+        _check(self.L.orc_smooth(self.h, abi.dptr(f), abi.dptr(out), ncomp))
+        return out
+
+    def refine(self, ids):
+        a = np.ascontiguousarray(ids, dtype=np.int32).copy()
+        _check(self.L.orc_refine(self.h, abi.iptr(a)))
+        return a
 
     def geometry(self):
         n = self.nc

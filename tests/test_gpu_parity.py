@@ -107,3 +107,59 @@ def test_fp32_state_runs(require_gpu):
     c = g.counters()
     assert c["n_particles"] == 6 ** 3 * 20
     assert c["collisions"] > 0
+
+
+def assert_hybrid_identical(g, o, what):
+    assert_identical(g, o, what)
+    kg, ko = g.kn(), o.kn()
+    assert np.array_equal(kg, ko), f"{what}: Kn differs (max rel {np.max(np.abs(kg - ko) / np.maximum(np.abs(ko), 1e-300))})"
+    assert np.array_equal(g.cell_model(), o.cell_model()), f"{what}: model ids differ"
+
+
+def test_hybrid_nozzle_identical(require_gpu):
+    """localKnudsen switching with inflow, deletion and diffuse walls: several
+    decompositions (every 3 steps, 2 smoothing passes) on a mapped mesh."""
+    spec, fill = workloads.nozzle(nx=10, ny=6, nz=6, per_cell=20, n_stag=2.4e24)
+    spec.decomposition_interval = 3
+    spec.smoothing_passes = 2
+    g, o = pair(spec, fill, capacity=1 << 16)
+    for k in range(4):
+        g.step(3)
+        o.step(3)
+        assert_hybrid_identical(g, o, f"nozzle hybrid, decomposition {k}")
+    ids = g.cell_model()
+    assert 0 < ids.sum() < ids.size, "expected both collision models"
+
+
+def test_hybrid_couette_cyclic_identical(require_gpu):
+    """Hybrid Couette: cyclic faces in the smoothing, theta averaging over
+    decompositions without a reset of the time sums."""
+    spec, fill = workloads.couette(n_cells=40, per_cell=30, collision_model="hybrid",
+                                   bgk_model="unifiedStochasticParticleSBGK")
+    spec.decomposition_interval = 4
+    spec.smoothing_passes = 3
+    spec.reset_at_decomposition = False
+    g, o = pair(spec, fill)
+    for k in range(3):
+        g.step(4)
+        o.step(4)
+        assert_hybrid_identical(g, o, f"couette hybrid, decomposition {k}")
+
+
+def test_cells_larger_than_lds_identical(require_gpu):
+    """Cells with more simulators than an LDS tile holds take the HBM path of
+    the sort and sums (chunked rank counting)."""
+    spec, fill = workloads.heat_bath(n_cells=2, per_cell=1500, dsmc_model="variableHardSphere")
+    g, o = pair(spec, fill, capacity=1 << 15)
+    g.step(6)
+    o.step(6)
+    assert_identical(g, o, "big cells DSMC")
+
+
+def test_big_cells_bgk_identical(require_gpu):
+    spec, fill = workloads.couette(n_cells=4, per_cell=1200, collision_model="bgk",
+                                   bgk_model="unifiedStochasticParticleSBGK")
+    g, o = pair(spec, fill, capacity=1 << 15)
+    g.step(6)
+    o.step(6)
+    assert_identical(g, o, "big cells USP")

@@ -429,3 +429,164 @@ def test_fill_moments_and_sigma_max():
     # (sigma_T c_r)max = pi d^2 c_mp(T) of the most abundant species (uniGasMeshFill.C:284-296)
     smax = o.sigma_max()
     assert np.allclose(smax, math.pi * N2.diameter ** 2 * math.sqrt(2.0 * KB * 300.0 / N2.mass), rtol=1e-14)
+
+
+# ------------------------------------------------- hybrid decomposition
+
+def channel_spec(ny, grading=1.0, patch_y=abi.WALL, ymin_type=None, collision="hybrid", side=1e-3, n_xz=1):
+    pt = {"xmin": abi.CYCLIC, "xmax": abi.CYCLIC, "zmin": abi.CYCLIC, "zmax": abi.CYCLIC,
+          "ymin": ymin_type if ymin_type is not None else patch_y, "ymax": patch_y}
+    mesh = graded_box_mesh((0.0, 0.0, 0.0), (side, side, side), (n_xz, ny, n_xz), (1.0, grading, 1.0), pt)
+    bnd = []
+    for p in ("ymin", "ymax"):
+        if pt[p] == abi.WALL:
+            bnd.append(Boundary(p, abi.BND_SPECULAR))
+    return CaseSpec(mesh=mesh, species=[ARGON], delta_t=1e-7, n_equivalent_particles=1e8,
+                    collision_model=collision, dsmc_model="variableHardSphere",
+                    bgk_model="unifiedStochasticParticleSBGK", boundaries=bnd, seed=9,
+                    smoothing_passes=0, kn_theta=1.0)
+
+
+@pytest.mark.parametrize("grading", [1.0, 4.0])
+def test_smoothing_is_exact_for_linear_fields(grading):
+    """fvc::average(fvc::interpolate(f)) leaves a linear field unchanged in
+    cells away from zeroGradient faces: linear weights give exact face
+    values and opposite faces of a hexahedron have equal areas."""
+    spec = channel_spec(12, grading=grading)
+    o = orc.OracleCase(spec, capacity=100)
+    y = o.geometry()["cc"][:, 1]
+    f = 3.0 + 2.5e3 * y
+    g = o.smooth(f)
+    assert np.allclose(g[1:-1], f[1:-1], rtol=1e-13, atol=0.0)
+    assert not np.isclose(g[0], f[0], rtol=1e-9)     # zeroGradient end cells differ
+    # 3-D graded box: interior cells
+    mesh = graded_box_mesh((0.0, 0.0, 0.0), (1e-3, 2e-3, 1.5e-3), (5, 4, 4), (2.0, 0.5, 3.0),
+                           {k: abi.WALL for k in SIDES})
+    spec3 = CaseSpec(mesh=mesh, species=[ARGON], delta_t=1e-7, n_equivalent_particles=1e8,
+                     collision_model="hybrid", bgk_model="stochasticParticleBGK",
+                     boundaries=[Boundary(p, abi.BND_SPECULAR) for p in SIDES])
+    o3 = orc.OracleCase(spec3, capacity=100)
+    cc = o3.geometry()["cc"]
+    f3 = 1.0 + 400.0 * cc[:, 0] - 700.0 * cc[:, 1] + 250.0 * cc[:, 2]
+    g3 = o3.smooth(f3)
+    i, j, k = np.meshgrid(np.arange(5), np.arange(4), np.arange(4), indexing="ij")
+    cid = (k * 4 + j) * 5 + i
+    inner = cid[1:-1, 1:-1, 1:-1].ravel()
+    assert np.allclose(g3[inner], f3[inner], rtol=1e-13, atol=0.0)
+
+
+def test_smoothing_symmetry_plane_reflects_vectors():
+    spec = channel_spec(3, ymin_type=abi.SYMMETRY)
+    o = orc.OracleCase(spec, capacity=100)
+    U = np.tile([1.0, 2.0, 3.0], (3, 1))
+    g = o.smooth(U)
+    # cell 0 (1 x 1/3 x 1 mm): two y faces of area Ay, four side faces of area Ax;
+    # the symmetry face carries U - (U.n)n = (1, 0, 3), the others U
+    Ay, Ax = 1e-6, 1e-6 / 3.0
+    fa = o.geometry()["farea"][0]
+    assert np.allclose(np.sort(fa), np.sort([Ax, Ax, Ax, Ax, Ay, Ay]), rtol=1e-12)
+    uy = (Ay * 0.0 + Ay * 2.0 + 4 * Ax * 2.0) / (2 * Ay + 4 * Ax)
+    assert np.allclose(g[0], [1.0, uy, 3.0], rtol=1e-12)
+    assert np.allclose(g[1:], U[1:], rtol=1e-15)
+    # scalars keep the cell value on the plane: a uniform scalar stays uniform
+    s = o.smooth(np.array([2.0, 2.0, 2.0]))
+    assert np.allclose(s, 2.0, rtol=1e-15)
+
+
+def set_linear_density(o, spec, n0, a, T0, tav, vacuum=()):
+    g = o.geometry()
+    y = g["cc"][:, 1]
+    V = g["vol"]
+    n = n0 * (1.0 + a * y)
+    A = np.zeros((spec.mesh.n_cells, 16))
+    fn = spec.n_equivalent_particles
+    m = ARGON.mass
+    A[:, 0] = n * V * tav / fn
+    A[:, 1] = A[:, 0] * m
+    A[:, 5] = 3.0 * KB * T0 * n * V * tav / fn
+    A[:, 8] = A[:, 0]
+    for c in vacuum:
+        A[c] = 0.0
+    o.hybrid_set(A, tav)
+    return n, y
+
+
+def test_kn_gll_closed_form():
+    """Linear density, uniform temperature, no flow: Kn_GLL = lambda |grad rho| / rho
+    with the hard-sphere-like VHS mean free path of Bird eq. 4.76."""
+    ny = 30
+    spec = channel_spec(ny, grading=2.0)
+    spec.breakdown_max = 0.05
+    o = orc.OracleCase(spec, capacity=100)
+    n0, a, T0 = 2.0e22, 2000.0, 300.0
+    n, y = set_linear_density(o, spec, n0, a, T0, tav=3e-6, vacuum=(ny - 1,))
+    o.hybrid_decompose()
+    kn = o.kn()
+    lam = 1.0 / (math.sqrt(2.0) * math.pi * ARGON.diameter ** 2 * n * (273.0 / T0) ** (ARGON.omega - 0.5))
+    expect = lam * n0 * a / n
+    # cells next to the vacuum cell see its zero density; the vacuum cell is 2 breakdownMax
+    assert np.allclose(kn[:ny - 2], expect[:ny - 2], rtol=1e-12)
+    assert kn[ny - 1] == 2.0 * spec.breakdown_max
+    ids = o.cell_model()
+    assert np.array_equal(ids[:ny - 2], (expect[:ny - 2] > spec.breakdown_max).astype(np.int32))
+    assert 0 < ids.sum() < ny                              # both models present
+
+
+def ball_counts(ids, shape, r=3):
+    """Jacobi refinement on a structured grid: adjacency through faces,
+    neighbourhood = Manhattan ball of radius r clipped to the box."""
+    nx, ny, nz = shape
+    grid = ids.reshape(nz, ny, nx)
+    out = grid.copy()
+    for me in (1, 0):
+        snap = out.copy()
+        for k in range(nz):
+            for j in range(ny):
+                for i in range(nx):
+                    if snap[k, j, i] != me:
+                        continue
+                    same = other = 0
+                    for dk, dj, di in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                        kk, jj, ii = k + dk, j + dj, i + di
+                        if 0 <= kk < nz and 0 <= jj < ny and 0 <= ii < nx:
+                            if snap[kk, jj, ii] == me:
+                                same += 1
+                            else:
+                                other += 1
+                    if same == 0 or (same == 1 and other > 1):
+                        out[k, j, i] = 1 - me
+                        continue
+                    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+                    ball = (np.abs(K - k) + np.abs(J - j) + np.abs(I - i)) <= r
+                    if (snap[ball] == me).sum() < 0.4 * ball.sum():
+                        out[k, j, i] = 1 - me
+    return out.ravel()
+
+
+def test_refinement_rules():
+    shape = (6, 5, 5)
+    mesh = box_mesh((0.0, 0.0, 0.0), (6e-3, 5e-3, 5e-3), shape, {k: abi.WALL for k in SIDES})
+    spec = CaseSpec(mesh=mesh, species=[ARGON], delta_t=1e-7, n_equivalent_particles=1e8,
+                    collision_model="hybrid", bgk_model="stochasticParticleBGK",
+                    boundaries=[Boundary(p, abi.BND_SPECULAR) for p in SIDES])
+    o = orc.OracleCase(spec, capacity=100)
+    nc = mesh.n_cells
+    nx, ny, nz = shape
+
+    def cid(i, j, k):
+        return (k * ny + j) * nx + i
+
+    one = np.zeros(nc, dtype=np.int32)
+    one[cid(2, 2, 2)] = 1
+    assert o.refine(one).sum() == 0                        # isolated DSMC cell joins its BGK sea
+    assert o.refine(1 - one).sum() == nc                   # and the reverse
+    pair = np.zeros(nc, dtype=np.int32)
+    pair[cid(2, 2, 2)] = pair[cid(3, 2, 2)] = 1
+    assert o.refine(pair).sum() == 0
+    rng = np.random.default_rng(4)
+    for _ in range(4):
+        ids = (rng.random(nc) < 0.5).astype(np.int32)
+        exp = ids.copy()
+        for _p in range(3):
+            exp = ball_counts(exp, shape)
+        assert np.array_equal(o.refine(ids), exp)

@@ -23,7 +23,7 @@ EXPORTS = [
     "ugx_version", "ugx_last_error", "ugx_case_create", "ugx_case_destroy", "ugx_case_fill",
     "ugx_particles_upload", "ugx_particles_download", "ugx_particle_count", "ugx_step", "ugx_sync",
     "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
-    "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download",
+    "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_probe_copy",
 ]
@@ -61,6 +61,7 @@ def lib():
         L.ugx_sigma_max_download.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_sigma_max_upload.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_model_download.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.ugx_kn_download.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_volumes.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_centres.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_geometry.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
@@ -231,6 +232,11 @@ class Case:
     def cell_model(self):
         out = np.zeros(self.nc, dtype=np.int32)
         check(self.L.ugx_cell_model_download(self.h, abi.iptr(out), self.nc), self.L)
+        return out
+
+    def kn(self):
+        out = np.zeros(self.nc)
+        check(self.L.ugx_kn_download(self.h, abi.dptr(out), self.nc), self.L)
         return out
 
     def timing(self, on=True):

@@ -260,6 +260,19 @@ int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len)
     UGX_GUARD_END
 }
 
+int ugx_kn_download(ugx_case* c, double* out, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    int b, n;
+    if (int rc = cell_range(c, len, 1, b, n)) return rc;
+    if (int rc = drain(c)) return rc;
+    if (hipMemcpy(out, c->world->d_kn + b, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_err(UGX_ERR_DEVICE, "download failed");
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len)
 {
     UGX_GUARD_BEGIN

@@ -257,6 +257,20 @@ struct CaseDev {
     double kn_time;
     int32_t sampler_full;        // sample rows 8-19 as well
     int32_t sum_level;           // per-cell sums needed: 0 rows 0-7, 1 +8-13, 2 +14-19
+    int32_t reset_at_decomposition;
+    int32_t pad1;
+    double reset_until_time;     // <= 0: no limit
+};
+
+// Hybrid decomposition: time-sum rows per cell (N, M, M u, M v, M w, M |u|^2,
+// two unused, then the per-species counts).
+constexpr int kKnRows = 16;
+
+// Per cell-face data of the decomposition's face interpolation, FP64 for
+// either particle precision: area, linear weight of the cell's own value,
+// distance between the cell reference points (internal faces), face normal.
+struct KnFace {
+    double area, w, dist, nx, ny, nz;
 };
 
 struct BModel {
@@ -294,6 +308,7 @@ struct Geom {
 
 struct Species {
     const double* mass;
+    const double* diam;
     const int32_t* rotdof;
     const double* omega;
     const double* muref;
@@ -424,19 +439,25 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
     R vt[3];
     for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * (V)mask[d]);
     for (int it = it0; it < kMaxCrossings; ++it) {
+        const size_t base = (size_t)c * kMaxFaces;
+        // every slot of a cell exists (padded ones are (0,0,0,1) / kNbrEmpty), so
+        // all face data is loaded before any decision: one memory round trip per cell
+        V4<R> q[kMaxFaces];
+        int32_t nb[kMaxFaces];
+#pragma unroll
+        for (int f = 0; f < kMaxFaces; ++f) { q[f] = g.plane[base + f]; nb[f] = g.nbr[base + f]; }
         R tmin = tau;
         int fmin = -1;
-        const size_t base = (size_t)c * kMaxFaces;
+        int32_t code = 0;
+        R qx = (R)0, qy = (R)0, qz = (R)0;
 #pragma unroll
         for (int f = 0; f < kMaxFaces; ++f) {
-            int32_t nb = g.nbr[base + f];
-            if (nb == kNbrEmpty) continue;
-            V4<R> q = g.plane[base + f];
-            R dn = q.x * vt[0] + q.y * vt[1] + q.z * vt[2];
+            if (nb[f] == kNbrEmpty) continue;
+            R dn = q[f].x * vt[0] + q[f].y * vt[1] + q[f].z * vt[2];
             if (dn > (R)0) {
-                R t = (q.w - (q.x * x[0] + q.y * x[1] + q.z * x[2])) / dn;
+                R t = (q[f].w - (q[f].x * x[0] + q[f].y * x[1] + q[f].z * x[2])) / dn;
                 if (t < (R)0) t = (R)0;
-                if (t < tmin) { tmin = t; fmin = f; }
+                if (t < tmin) { tmin = t; fmin = f; code = nb[f]; qx = q[f].x; qy = q[f].y; qz = q[f].z; }
             }
         }
         if (fmin < 0) {
@@ -446,7 +467,6 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
         }
         for (int d = 0; d < 3; ++d) x[d] = x[d] + vt[d] * tmin;
         tau = tau - tmin;
-        int32_t code = g.nbr[base + fmin];
         if (code >= 0) {
             V4<R> dl = g.delta[base + fmin];
             x[0] = x[0] + dl.x;
@@ -456,8 +476,7 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
             continue;
         }
         const BModel b = bm[-1 - code];
-        V4<R> q = g.plane[base + fmin];
-        double nw[3] = {(double)q.x, (double)q.y, (double)q.z};
+        double nw[3] = {(double)qx, (double)qy, (double)qz};
         if (b.kind == UGX_BND_DELETION) { cell = kDead; out.alive = 0; return out; }
         if (Defer && b.kind != UGX_BND_SPECULAR) {
             out.queued = 1;
@@ -469,7 +488,7 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
         out.hits += 1;
         if (b.kind == UGX_BND_SPECULAR) {
             // mirror; E_rot and species are not needed
-            const V n0 = (V)q.x, n1 = (V)q.y, n2 = (V)q.z;
+            const V n0 = (V)qx, n1 = (V)qy, n2 = (V)qz;
             V udn = U[0] * n0 + U[1] * n1 + U[2] * n2;
             if (udn > (V)0) {
                 U[0] = U[0] - (V)2 * udn * n0;

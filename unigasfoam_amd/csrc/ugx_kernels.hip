@@ -17,6 +17,8 @@
 //   end        step counters
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "ugx_kernels.hpp"
 
 namespace ugx {
@@ -26,20 +28,23 @@ constexpr int kScanBlock = 1024;      // elements per scan block (256 threads x 
 
 __device__ inline uint32_t wave_agg_inc(uint32_t* ctr, uint32_t key, bool active)
 {
+    // One atomic per distinct key of the wave; every atomic is issued before
+    // any result is used, so the wave waits for one round trip in total.
     const unsigned lane = __lane_id();
-    uint32_t ret = 0;
+    uint32_t lane_base = 0, my_leader = 0, my_rank = 0;
     unsigned long long pending = __ballot(active);
     while (pending) {
-        int leader = __ffsll(pending) - 1;
-        uint32_t lkey = __shfl(key, leader);
-        unsigned long long grp = __ballot(active && key == lkey) & pending;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(grp));
-        base = __shfl(base, leader);
-        if ((grp >> lane) & 1ull) ret = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        const int leader = __ffsll(pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const unsigned long long grp = __ballot(active && key == lkey) & pending;
+        if ((int)lane == leader) lane_base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(grp));
+        if ((grp >> lane) & 1ull) {
+            my_leader = (uint32_t)leader;
+            my_rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        }
         pending &= ~grp;
     }
-    return ret;
+    return __shfl(lane_base, (int)my_leader) + my_rank;
 }
 
 // Adds v to counters[ccase][idx]; one atomic per wave when the wave's
@@ -502,7 +507,11 @@ template <class R>
 struct LdsAcc {
     const R *vx, *vy, *vz, *er;
     const uint8_t* sl;
+    const double* m;
+    const uint8_t* rd;
     uint32_t p0;
+    __device__ double mass(uint32_t i, const Species&) const { return m[i - p0]; }
+    __device__ double rotdof(uint32_t i, const Species&) const { return (double)rd[i - p0]; }
     __device__ void load(uint32_t i, double& a, double& b, double& c) const
     {
         const uint32_t k = i - p0;
@@ -528,6 +537,8 @@ struct GlobAcc {
     __device__ double erot(uint32_t i) const { return (double)p.erot[i]; }
     __device__ void set_erot(uint32_t i, double e) const { p.erot[i] = (R)e; }
     __device__ int local_species(uint32_t i) const { return local[p.sp[i]]; }
+    __device__ double mass(uint32_t i, const Species& sp) const { return sp.mass[p.sp[i]]; }
+    __device__ double rotdof(uint32_t i, const Species& sp) const { return (double)sp.rotdof[p.sp[i]]; }
     __device__ uint32_t idx_get(uint32_t b, uint32_t k) const { return idx[b + k]; }
     __device__ void idx_set(uint32_t b, uint32_t k, uint32_t v) const { idx[b + k] = v; }
 };
@@ -568,8 +579,7 @@ __device__ inline void cell_partials(const WorldView<R>& w, int sp_base, int nsw
 {
     for (uint32_t i = i0; i < e; i += 8u) {
         const int s = A.local_species(i);
-        const int g = sp_base + s;
-        const double m = w.sp.mass[g];
+        const double m = A.mass(i, w.sp);
         double vx, vy, vz;
         A.load(i, vx, vy, vz);
         const double er = A.erot(i);
@@ -581,7 +591,7 @@ __device__ inline void cell_partials(const WorldView<R>& w, int sp_base, int nsw
         S[UGX_S_MW] = S[UGX_S_MW] + m * vz;
         S[UGX_S_MCC] = S[UGX_S_MCC] + mcc;
         S[UGX_S_EROT] = S[UGX_S_EROT] + er;
-        S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + (double)w.sp.rotdof[g];
+        S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + A.rotdof(i, w.sp);
         if (L >= 1) {
             S[UGX_S_MUU] = S[UGX_S_MUU] + m * (vx * vx);
             S[UGX_S_MUV] = S[UGX_S_MUV] + m * (vx * vy);
@@ -621,7 +631,7 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = tid >> 6, lane = tid & 63u, g = lane >> 3, j = lane & 7u;
     const int nsw = w.ns_max;
-    for (uint32_t kb = 0; kb < nct; kb += (kBlock / 64) * 8) {
+    for (uint32_t kb = 0; kb < nct; kb += (kTileBlock / 64) * 8) {
         const uint32_t k = kb + wv * 8u + g;
         uint32_t b = 0, e = 0;
         bool sample = false, hybrid = false, bgk = false;
@@ -664,7 +674,9 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
             }
             if (hybrid) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) w.knacc[(size_t)8 * c + q] += dt * S[q];
+                for (int q = 0; q < 8; ++q) w.knacc[(size_t)kKnRows * c + q] += dt * S[q];
+#pragma unroll
+                for (int q = 0; q < kNS; ++q) w.knacc[(size_t)kKnRows * c + 8 + q] += dt * S[UGX_S_SPECIES_N + q];
             }
             if (bgk) {
 #pragma unroll
@@ -706,7 +718,7 @@ __global__ void k_tile_table(WorldView<R> w)
 // SortOnly stops after 2.  Tiles larger than kTileMax run the same steps
 // from HBM.
 template <class R, bool SortOnly, int L>
-__global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
+__global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4))) k_tile(WorldView<R> w)
 {
     constexpr uint32_t kM = SortOnly ? 1u : kTileMax;
     __shared__ uint32_t s_off[kTileCells + 2];
@@ -715,19 +727,21 @@ __global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
     __shared__ uint8_t s_k[kTileMax];
     __shared__ R s_vx[kM], s_vy[kM], s_vz[kM], s_er[kM];
     __shared__ uint8_t s_sl[kM];
+    __shared__ double s_m[kM];
+    __shared__ uint8_t s_rd[kM];
     const uint32_t t = blockIdx.x;
     if (t >= w.st->ntiles) return;
     const uint32_t tid = threadIdx.x;
     const uint32_t c0 = w.tile_first[t];
     const uint32_t nct = w.tile_first[t + 1] - c0;
-    for (uint32_t k = tid; k <= nct; k += kBlock) s_off[k] = w.off[c0 + k];
+    for (uint32_t k = tid; k <= nct; k += kTileBlock) s_off[k] = w.off[c0 + k];
     __syncthreads();
     const uint32_t p0 = s_off[0];
     const uint32_t size = s_off[nct] - p0;
     if (size <= kTileMax) {
-        for (uint32_t i = tid; i < size; i += kBlock) s_in[i] = w.perm[p0 + i];
+        for (uint32_t i = tid; i < size; i += kTileBlock) s_in[i] = w.perm[p0 + i];
         __syncthreads();
-        for (uint32_t i = tid; i < size; i += kBlock) {
+        for (uint32_t i = tid; i < size; i += kTileBlock) {
             const uint32_t slot = s_in[i];
             // cell of entry i: the last k with s_off[k] <= p0 + i
             uint32_t lo = 0, hi = nct - 1u;
@@ -742,7 +756,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
             s_k[b + r] = (uint8_t)lo;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < size; i += kBlock) {
+        for (uint32_t i = tid; i < size; i += kTileBlock) {
             const uint32_t slot = s_perm[i];
             const uint32_t j = p0 + i;
             const R vx = w.a.vx[slot], vy = w.a.vy[slot], vz = w.a.vz[slot], er = w.a.erot[slot];
@@ -755,26 +769,39 @@ __global__ void __launch_bounds__(kBlock) k_tile(WorldView<R> w)
             if (!SortOnly) {
                 s_vx[i] = vx; s_vy[i] = vy; s_vz[i] = vz; s_er[i] = er;
                 s_sl[i] = (uint8_t)w.sp.local[g];
+                s_m[i] = w.sp.mass[g];
+                s_rd[i] = (uint8_t)w.sp.rotdof[g];
             }
         }
         if (SortOnly) return;
         __syncthreads();
-        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, p0};
+        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_m, s_rd, p0};
         tile_sums<L, R>(w, c0, nct, s_off, A);
     } else {
-        // Tile larger than LDS: the same steps in HBM, slot lists ordered into scratch.
+        // Tile larger than LDS: the same steps in HBM.  The rank of an entry
+        // in its cell is counted against the cell's slot list staged through
+        // LDS in chunks; the ordered slots go to scratch.
         if (tid == 0) atomicAdd(&w.st->big_tiles, 1u);
         for (uint32_t k = 0; k < nct; ++k) {
             const uint32_t b = s_off[k], e = s_off[k + 1];
-            for (uint32_t i = b + tid; i < e; i += kBlock) {
-                const uint32_t slot = w.perm[i];
+            for (uint32_t ib = b; ib < e; ib += kTileBlock) {
+                const uint32_t i = ib + tid;
+                const bool mine = i < e;
+                const uint32_t slot = mine ? w.perm[i] : 0u;
                 uint32_t r = 0;
-                for (uint32_t q = b; q < e; ++q) r += (w.perm[q] < slot) ? 1u : 0u;
-                w.scratch[b + r] = slot;
+                for (uint32_t jb = b; jb < e; jb += kTileMax) {
+                    const uint32_t nj = min(kTileMax, e - jb);
+                    __syncthreads();
+                    for (uint32_t q = tid; q < nj; q += kTileBlock) s_in[q] = w.perm[jb + q];
+                    __syncthreads();
+                    if (mine)
+                        for (uint32_t q = 0; q < nj; ++q) r += (s_in[q] < slot) ? 1u : 0u;
+                }
+                if (mine) w.scratch[b + r] = slot;
             }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < size; i += kBlock) {
+        for (uint32_t i = tid; i < size; i += kTileBlock) {
             const uint32_t slot = w.scratch[p0 + i];
             const uint32_t j = p0 + i;
             w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
@@ -1114,7 +1141,6 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     bool perform = false, usp = false;
     uint32_t b = 0, e = 0;
     double T = 0.0, U0 = 0.0, U1 = 0.0, U2 = 0.0, M = 0.0, N = 0.0;
-    int sp_base = 0;
     (void)g;
     if (c < (uint32_t)w.nc) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
@@ -1130,19 +1156,30 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
                 T = P.T; U0 = P.U[0]; U1 = P.U[1]; U2 = P.U[2];
                 M = w.mom[(size_t)UGX_S_M * w.nc + c];
                 N = w.mom[(size_t)UGX_S_N * w.nc + c];
-                sp_base = cs.sp_base;
             }
         }
     }
     if (__ballot(perform) != 0ull) {
         double ke = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
-        for (uint32_t i = b + j; i < e; i += 8u) {
-            const double m = w.sp.mass[sp_base + w.sp.local[w.b.sp[i]]];
-            const double vx = (double)w.b.vx[i], vy = (double)w.b.vy[i], vz = (double)w.b.vz[i];
-            ke = ke + m * (vx * vx + vy * vy + vz * vz);
-            m0 = m0 + m * vx;
-            m1 = m1 + m * vy;
-            m2 = m2 + m * vz;
+        // four simulators per lane per chunk, loaded before use (indices clamped
+        // into the cell), accumulated in slot order
+        for (uint32_t i0 = b + j; i0 < e; i0 += 32u) {
+            double mm[4], ux[4], uy[4], uz[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t i = min(i0 + 8u * u, e - 1u);
+                mm[u] = w.sp.mass[w.b.sp[i]];
+                ux[u] = (double)w.b.vx[i]; uy[u] = (double)w.b.vy[i]; uz[u] = (double)w.b.vz[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (i0 + 8u * u < e) {
+                    ke = ke + mm[u] * (ux[u] * ux[u] + uy[u] * uy[u] + uz[u] * uz[u]);
+                    m0 = m0 + mm[u] * ux[u];
+                    m1 = m1 + mm[u] * uy[u];
+                    m2 = m2 + mm[u] * uz[u];
+                }
+            }
         }
         ke = group8_total(ke);
         m0 = group8_total(m0);
@@ -1168,6 +1205,263 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     }
 }
 
+// --------------------------------------------------- hybrid decomposition
+// localKnudsen::decompose (localKnudsen.C:216-562) for the cells of every
+// hybrid case whose decomposition falls on this step.  Same restatement as
+// the CPU oracle: fvc::average(fvc::interpolate()) on the cell-face data,
+// Bird eq. 4.76/4.77 mean free path, theta average, smoothing, threshold,
+// and the refinement passes with each loop deciding from the model ids at
+// its start.
+
+__device__ inline bool kn_due(const CaseDev& cs)
+{
+    return cs.active && cs.coll_model == UGX_COLL_HYBRID
+           && ((cs.step + 1u) % (uint32_t)cs.decomposition_interval) == 0u;
+}
+
+template <class R>
+__global__ void k_kn_begin(WorldView<R> w)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < w.ncases) {
+        CaseDev& cs = w.cases[c];
+        if (cs.active && cs.coll_model == UGX_COLL_HYBRID) cs.kn_time += cs.dt;
+    }
+}
+
+// kw[8c + 0..6] = rhoN, rhoM, p, T, U (localKnudsen.C:243-279)
+template <class R>
+__global__ void k_kn_fields(WorldView<R> w)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs)) continue;
+        const double* a = &w.knacc[(size_t)kKnRows * c];
+        const double FN = cs.fn, tav = cs.kn_time, V = w.vol[c];
+        double f[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (a[0] > kVSmall) {
+            f[0] = FN * a[0] / (tav * V);
+            f[1] = FN * a[1] / (tav * V);
+            const double rhoMMean = FN * a[1] / (V * tav);
+            for (int d = 0; d < 3; ++d) f[4 + d] = FN * a[2 + d] / (rhoMMean * V * tav);
+            const double ke = 0.5 * (FN * a[5]) / (V * tav);
+            const double rhoNMean = FN * a[0] / (V * tav);
+            const double uu = f[4] * f[4] + f[5] * f[5] + f[6] * f[6];
+            f[3] = 2.0 / (3.0 * kB * rhoNMean) * (ke - 0.5 * rhoMMean * uu);
+            f[2] = f[0] * kB * f[3];
+        }
+        for (int q = 0; q < 7; ++q) w.kw[8 * (size_t)c + q] = f[q];
+    }
+}
+
+// One pass of fvc::average(fvc::interpolate(.)) over the fields [q0, q0+nq)
+// of a stride-8 table (nq = 1 scalar, 3 vector); fields outside are copied.
+template <class R>
+__device__ inline void kn_smooth_cell(const WorldView<R>& w, uint32_t c, const double* in, double* out, int q0,
+                                      int ncomp, int stride)
+{
+    double num[3] = {0.0, 0.0, 0.0}, den = 0.0;
+    const double* fc = &in[(size_t)stride * c + q0];
+    for (int j = 0; j < kMaxFaces; ++j) {
+        const size_t k = (size_t)c * kMaxFaces + j;
+        const int kind = w.fkind[k];
+        if (kind == kFaceSkip) continue;
+        const KnFace kf = w.knf[k];
+        double fv[3];
+        if (kind == kFaceInternal || kind == kFaceCyclic) {
+            const double* fo = &in[(size_t)stride * (uint32_t)w.geom.nbr[k] + q0];
+            for (int q = 0; q < ncomp; ++q) fv[q] = kf.w * fc[q] + (1.0 - kf.w) * fo[q];
+        } else if (kind == kFaceSymmetry && ncomp == 3) {
+            const double un = kf.nx * fc[0] + kf.ny * fc[1] + kf.nz * fc[2];
+            fv[0] = fc[0] - un * kf.nx;
+            fv[1] = fc[1] - un * kf.ny;
+            fv[2] = fc[2] - un * kf.nz;
+        } else {
+            for (int q = 0; q < ncomp; ++q) fv[q] = fc[q];
+        }
+        for (int q = 0; q < ncomp; ++q) num[q] = num[q] + kf.area * fv[q];
+        den = den + kf.area;
+    }
+    for (int q = 0; q < ncomp; ++q) out[(size_t)stride * c + q0 + q] = (den > 0.0) ? num[q] / den : fc[q];
+}
+
+// Smoothing pass of rhoM, p, T (scalars) and U (vector); rhoN is carried.
+template <class R>
+__global__ void k_kn_smooth(WorldView<R> w, const double* in, double* out, int pass)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs)) continue;
+        if (pass < cs.smoothing_passes) {
+            out[8 * (size_t)c] = in[8 * (size_t)c];
+            kn_smooth_cell<R>(w, c, in, out, 1, 1, 8);
+            kn_smooth_cell<R>(w, c, in, out, 2, 1, 8);
+            kn_smooth_cell<R>(w, c, in, out, 3, 1, 8);
+            kn_smooth_cell<R>(w, c, in, out, 4, 3, 8);
+        } else {
+            for (int q = 0; q < 7; ++q) out[8 * (size_t)c + q] = in[8 * (size_t)c + q];
+        }
+    }
+}
+
+// Smoothing pass of KnGLL (stride 1): kn -> out.
+template <class R>
+__global__ void k_kn_smooth_kn(WorldView<R> w, const double* in, double* out, int pass)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs)) continue;
+        if (pass < cs.smoothing_passes) kn_smooth_cell<R>(w, c, in, out, 0, 1, 1);
+        else out[c] = in[c];
+    }
+}
+
+// Gradients, mean free path, instantaneous Kn and its theta average
+// (localKnudsen.C:299-398); f is the smoothed field table.
+template <class R>
+__global__ void k_kn_compute(WorldView<R> w, const double* f)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs)) continue;
+        const double* fc = &f[8 * (size_t)c];
+        const double mU = sqrt(fc[4] * fc[4] + fc[5] * fc[5] + fc[6] * fc[6]);
+        double gr = 0.0, gt = 0.0, gu = 0.0;
+        for (int j = 0; j < kMaxFaces; ++j) {
+            const size_t k = (size_t)c * kMaxFaces + j;
+            if (w.fkind[k] != kFaceInternal) continue;
+            const double* fo = &f[8 * (size_t)(uint32_t)w.geom.nbr[k]];
+            const double dist = w.knf[k].dist;
+            const double mUo = sqrt(fo[4] * fo[4] + fo[5] * fo[5] + fo[6] * fo[6]);
+            double v;
+            v = fabs(fo[1] - fc[1]) / dist; if (gr < v) gr = v;
+            v = fabs(fo[3] - fc[3]) / dist; if (gt < v) gt = v;
+            v = fabs(mUo - mU) / dist; if (gu < v) gu = v;
+        }
+        const double* a = &w.knacc[(size_t)kKnRows * c];
+        const double bmax = cs.breakdown_max;
+        double inst;
+        if (a[0] > kVSmall && fc[3] > kVSmall) {
+            const double V = w.vol[c], FN = cs.fn, T = fc[3];
+            double mfp = 0.0;
+            for (int i = 0; i < cs.ns; ++i) {
+                const int gi = cs.sp_base + i;
+                double smfp = 0.0;
+                for (int q = 0; q < cs.ns; ++q) {
+                    const int gq = cs.sp_base + q;
+                    const double dPQ = 0.5 * (w.sp.diam[gi] + w.sp.diam[gq]);
+                    const double omPQ = 0.5 * (w.sp.omega[gi] + w.sp.omega[gq]);
+                    const double mr = w.sp.mass[gi] / w.sp.mass[gq];
+                    const double nq = FN * a[8 + q];
+                    if (nq > kVSmall) {
+                        const double nD = nq / V;
+                        smfp += kPi * dPQ * dPQ * nD * dpow(cs.tref / T, omPQ - 0.5) * sqrt(1.0 + mr);
+                    }
+                }
+                const double ni = FN * a[8 + i];
+                if (ni > kVSmall) {
+                    smfp = 1.0 / smfp;
+                    mfp += smfp * ni / (fc[0] * V);
+                }
+            }
+            const double u0 = sqrt(2.0 * kB / (fc[1] / fc[0]) * T);
+            const double kr = mfp * gr / fc[1];
+            const double kt = mfp * gt / T;
+            const double ku = mfp * gu / ((mU < u0) ? u0 : mU);
+            inst = (kr < kt) ? kt : kr;
+            inst = (inst < ku) ? ku : inst;
+        } else {
+            inst = 2.0 * bmax;
+        }
+        w.kn[c] = cs.kn_theta * inst + (1.0 - cs.kn_theta) * w.kn[c];
+    }
+}
+
+// Threshold, then the snapshot for the first refinement loop.
+template <class R>
+__global__ void k_kn_decide(WorldView<R> w, const double* kn)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs)) continue;
+        w.kn[c] = kn[c];
+        w.cmodel[c] = (kn[c] > cs.breakdown_max) ? 1 : 0;
+    }
+}
+
+template <class R>
+__global__ void k_kn_snapshot(WorldView<R> w)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x)
+        w.ids2[c] = w.cmodel[c];
+}
+
+constexpr int kNbCap = 192;   // 3 levels of at most 6 faces: 1 + 6 + 30 + 150 cells
+
+// One refinement loop (uniGasHybridDecomposition.C:73-75; localKnudsen.C:436-545)
+// for the cells of model `me` (1 DSMC, 0 BGK), deciding from ids2.
+template <class R>
+__global__ void __launch_bounds__(64) k_kn_refine(WorldView<R> w, int me)
+{
+    uint32_t list[kNbCap];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs) || w.ids2[c] != me) continue;
+        int same = 0, other = 0;
+        for (int j = 0; j < kMaxFaces; ++j) {
+            const size_t k = (size_t)c * kMaxFaces + j;
+            if (w.fkind[k] != kFaceInternal) continue;
+            if (w.ids2[w.geom.nbr[k]] == me) same++; else other++;
+        }
+        if (same == 0 || (same == 1 && other > 1)) { w.cmodel[c] = 1 - me; continue; }
+        int n = 0, lo = 0;
+        list[n++] = c;
+        for (int l = 0; l < 3; ++l) {
+            const int hi = n;
+            for (int i = lo; i < hi; ++i)
+                for (int j = 0; j < kMaxFaces; ++j) {
+                    const size_t k = (size_t)list[i] * kMaxFaces + j;
+                    if (w.fkind[k] != kFaceInternal) continue;
+                    const uint32_t o = (uint32_t)w.geom.nbr[k];
+                    bool seen = false;
+                    for (int q = 0; q < n; ++q) if (list[q] == o) { seen = true; break; }
+                    if (!seen && n < kNbCap) list[n++] = o;
+                }
+            lo = hi;
+        }
+        int cnt = 0;
+        for (int q = 0; q < n; ++q) if (w.ids2[list[q]] == me) cnt++;
+        if ((double)cnt < 0.4 * (double)n) w.cmodel[c] = 1 - me;
+    }
+}
+
+__device__ inline bool kn_reset_now(const CaseDev& cs)
+{
+    const double time = (double)(cs.step + 1u) * cs.dt;
+    const double until = (cs.reset_until_time > 0.0) ? cs.reset_until_time : 1.0e300;
+    return cs.reset_at_decomposition && time < until + 0.5 * cs.dt;
+}
+
+template <class R>
+__global__ void k_kn_reset(WorldView<R> w)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (!kn_due(cs) || !kn_reset_now(cs)) continue;
+        for (int q = 0; q < kKnRows; ++q) w.knacc[(size_t)kKnRows * c + q] = 0.0;
+    }
+}
+
+template <class R>
+__global__ void k_kn_end(WorldView<R> w)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < w.ncases) {
+        CaseDev& cs = w.cases[c];
+        if (kn_due(cs) && kn_reset_now(cs)) cs.kn_time = 0.0;
+    }
+}
+
 // ------------------------------------------------------------ end step
 
 template <class R>
@@ -1180,7 +1474,6 @@ __global__ void k_end_step(WorldView<R> w)
             if (((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u) w.counters[(size_t)c * kCntNum + kCntSampled] += 1ull;
             w.counters[(size_t)c * kCntNum + kCntParticleSteps] +=
                 (unsigned long long)(w.off[cs.cell_end] - w.off[cs.cell_begin]);
-            if (cs.coll_model == UGX_COLL_HYBRID) cs.kn_time += cs.dt;
             cs.step += 1u;
         }
     }
@@ -1298,19 +1591,48 @@ static inline unsigned grid_for(uint64_t n, unsigned cap_blocks = 8192)
 }
 
 template <class R>
+static void launch_decompose(const WorldView<R>& w, hipStream_t s)
+{
+    const unsigned g = grid_for((uint64_t)w.nc);
+    hipLaunchKernelGGL(k_kn_fields<R>, dim3(g), dim3(kBlock), 0, s, w);
+    double* a = w.kw;
+    double* b = w.kw2;
+    for (int p = 0; p < w.kn_passes; ++p) {
+        hipLaunchKernelGGL(k_kn_smooth<R>, dim3(g), dim3(kBlock), 0, s, w, (const double*)a, b, p);
+        std::swap(a, b);
+    }
+    hipLaunchKernelGGL(k_kn_compute<R>, dim3(g), dim3(kBlock), 0, s, w, (const double*)a);
+    // KnGLL smoothing ping-pongs between kn and the first column-free table b
+    double* k0 = w.kn;
+    double* k1 = b;
+    for (int p = 0; p < w.kn_passes; ++p) {
+        hipLaunchKernelGGL(k_kn_smooth_kn<R>, dim3(g), dim3(kBlock), 0, s, w, (const double*)k0, k1, p);
+        std::swap(k0, k1);
+    }
+    hipLaunchKernelGGL(k_kn_decide<R>, dim3(g), dim3(kBlock), 0, s, w, (const double*)k0);
+    for (int pass = 0; pass < 3; ++pass)
+        for (int me = 1; me >= 0; --me) {
+            hipLaunchKernelGGL(k_kn_snapshot<R>, dim3(g), dim3(kBlock), 0, s, w);
+            hipLaunchKernelGGL(k_kn_refine<R>, dim3(grid_for((uint64_t)w.nc * 4, 65536)), dim3(64), 0, s, w, me);
+        }
+    hipLaunchKernelGGL(k_kn_reset<R>, dim3(g), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_kn_end<R>, dim3(grid_for((uint64_t)w.ncases)), dim3(kBlock), 0, s, w);
+}
+
+template <class R>
 static void launch_tiles(const WorldView<R>& w, hipStream_t s, bool sort_only)
 {
     const uint32_t ncp = (uint32_t)w.nc + 1u;
     const unsigned nt = (unsigned)max_tiles(w.cap, (uint64_t)w.nc);
     hipLaunchKernelGGL(k_tile_table<R>, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w);
     if (sort_only)
-        hipLaunchKernelGGL((k_tile<R, true, 0>), dim3(nt), dim3(kBlock), 0, s, w);
+        hipLaunchKernelGGL((k_tile<R, true, 0>), dim3(nt), dim3(kTileBlock), 0, s, w);
     else if (w.sum_level >= 2)
-        hipLaunchKernelGGL((k_tile<R, false, 2>), dim3(nt), dim3(kBlock), 0, s, w);
+        hipLaunchKernelGGL((k_tile<R, false, 2>), dim3(nt), dim3(kTileBlock), 0, s, w);
     else if (w.sum_level == 1)
-        hipLaunchKernelGGL((k_tile<R, false, 1>), dim3(nt), dim3(kBlock), 0, s, w);
+        hipLaunchKernelGGL((k_tile<R, false, 1>), dim3(nt), dim3(kTileBlock), 0, s, w);
     else
-        hipLaunchKernelGGL((k_tile<R, false, 0>), dim3(nt), dim3(kBlock), 0, s, w);
+        hipLaunchKernelGGL((k_tile<R, false, 0>), dim3(nt), dim3(kTileBlock), 0, s, w);
 }
 
 template <class R>
@@ -1361,6 +1683,10 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
             const unsigned gcons = (unsigned)(((uint64_t)w.nc * 8u + kBlock - 1) / kBlock);
             hipLaunchKernelGGL(k_bgk_conserve<R>, dim3(gcons), dim3(kBlock), 0, s, w);
         }
+    }
+    if (w.has_hybrid) {
+        hipLaunchKernelGGL(k_kn_begin<R>, dim3(gc), dim3(kBlock), 0, s, w);
+        if (w.kn_due) launch_decompose<R>(w, s);
     }
     mark(8);
     hipLaunchKernelGGL(k_end_step<R>, dim3(gc), dim3(kBlock), 0, s, w);

@@ -54,7 +54,7 @@ struct WorldView {
     double* uspq;             // [3*nc]
     double* usps;             // [6*nc]
     double* uspmax;           // [nc]
-    double* knacc;            // [8*nc]
+    double* knacc;            // [kKnRows*nc]
     double* kn;               // [nc]
     const double* cc;         // [3*nc] reference points (hybrid distances)
     Species sp;
@@ -75,6 +75,14 @@ struct WorldView {
     int32_t ns_max;           // largest species count of a case
     int32_t sum_level;        // largest sampler level of a case (0, 1, 2)
     int32_t has_collide;      // some case collides (DSMC or BGK family)
+    int32_t has_hybrid;       // some case switches models per cell
+    int32_t kn_due;           // a hybrid case decomposes at this step
+    int32_t kn_passes;        // largest smoothing pass count of the due cases
+    const KnFace* knf;        // [nc*kMaxFaces]
+    const int8_t* fkind;      // [nc*kMaxFaces]
+    double* kw;               // [8*nc] decomposition fields (ping)
+    double* kw2;              // [8*nc] (pong)
+    int32_t* ids2;            // [nc] model ids at the start of a refinement loop
     int32_t has_bgk;          // some case runs a BGK-family model
     uint32_t* blocksum;       // scan scratch
     unsigned long long* counters; // [ncases*kCntNum]
@@ -96,10 +104,11 @@ constexpr int kNMom = UGX_S_SPECIES_MCC;
 // off[c] + kCellW * c lies in [t * kTileN, (t + 1) * kTileN), so a tile has
 // at most kTileN / kCellW cells and fewer than kTileN simulators plus those
 // of its last cell.
-constexpr uint32_t kTileN = 512;
+constexpr uint32_t kTileN = 256;
 constexpr uint32_t kCellW = 8;
 constexpr uint32_t kTileCells = kTileN / kCellW;
-constexpr uint32_t kTileMax = 1024;   // simulators of a tile staged in LDS
+constexpr uint32_t kTileMax = 512;    // simulators of a tile staged in LDS
+constexpr int kTileBlock = 128;       // threads of a tile block
 inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc) / kTileN + 2; }
 
 // Host launchers (ugx_kernels.hip).
@@ -111,6 +120,5 @@ template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int cc
                                     int exact, const double* bblo, const double* bbhi,
                                     uint32_t* fill_cnt, uint32_t* fill_off, int ns, int ncell_case);
 void launch_copy_probe(const void* in, void* out, size_t bytes, hipStream_t s);
-template <class R> void launch_hybrid(const WorldView<R>& w, hipStream_t s, double* work);
 
 }  // namespace ugx

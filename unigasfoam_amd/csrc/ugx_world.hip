@@ -304,6 +304,7 @@ void World::release()
                      (void**)&d_cases, (void**)&d_cell_case, (void**)&d_vol, (void**)&d_smax, (void**)&d_cmodel,
                      (void**)&d_samp, (void**)&d_uspq, (void**)&d_usps, (void**)&d_uspmax, (void**)&d_knacc,
                      (void**)&d_kn, (void**)&d_cc, (void**)&d_mass, (void**)&d_rotdof, (void**)&d_omega,
+                     (void**)&d_diam, (void**)&d_knf, (void**)&d_fkind, (void**)&d_kw, (void**)&d_kw2, (void**)&d_ids2,
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
                      (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
@@ -413,7 +414,15 @@ WorldView<R> World::view() const
     w.knacc = d_knacc;
     w.kn = d_kn;
     w.cc = d_cc;
-    w.sp = Species{d_mass, d_rotdof, d_omega, d_muref, d_prterm, d_local};
+    w.sp = Species{d_mass, d_diam, d_rotdof, d_omega, d_muref, d_prterm, d_local};
+    w.has_hybrid = has_hybrid;
+    w.kn_due = 0;
+    w.kn_passes = 0;
+    w.knf = d_knf;
+    w.fkind = d_fkind;
+    w.kw = d_kw;
+    w.kw2 = d_kw2;
+    w.ids2 = d_ids2;
     w.pr = Pairs{d_pmr, d_pC, d_pe, d_pchi, d_palpha};
     w.cnt = d_cnt;
     w.off = d_off;
@@ -459,6 +468,15 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     std::vector<CaseDev> hc;
     std::vector<int32_t> cell_case(nc), nbr((size_t)nc * kMaxFaces), cmodel(nc), local(nsp), rotdof(nsp);
     std::vector<double> vol(nc), cc((size_t)nc * 3), plane((size_t)nc * kMaxFaces * 4), delta((size_t)nc * kMaxFaces * 4, 0.0);
+    std::vector<double> diam(nsp);
+    std::vector<KnFace> knf;
+    std::vector<int8_t> fkind;
+    for (auto& sp_ : specs)
+        if (sp_.model.collision_model == UGX_COLL_HYBRID) has_hybrid = 1;
+    if (has_hybrid) {
+        knf.assign((size_t)nc * kMaxFaces, KnFace{0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
+        fkind.assign((size_t)nc * kMaxFaces, kFaceSkip);
+    }
     std::vector<double> mass(nsp), omega(nsp), muref(nsp), prterm(nsp), pmr(npair), pC(npair), pe(npair), pchi(npair), palpha(npair);
     std::vector<BModel> bm;
     std::vector<InflowFS> inflow;
@@ -494,6 +512,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         d.breakdown_max = s.model.breakdown_max;
         d.kn_theta = s.model.kn_theta;
         d.sampler_full = s.model.sampler_full ? 1 : 0;
+        d.reset_at_decomposition = s.model.reset_at_decomposition;
+        d.reset_until_time = s.model.reset_until_time;
         {
             int lvl = d.sampler_full ? 2 : 0;
             const bool bgk_on = (s.model.collision_model == UGX_COLL_BGK || s.model.collision_model == UGX_COLL_HYBRID);
@@ -522,6 +542,21 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
                 if (code >= 0) code += cbase;
                 else if (code != kNbrEmpty) code = -1 - (d.bm_base + (-1 - code));
                 nbr[gk] = code;
+                if (has_hybrid) {
+                    // face interpolation data, in the CPU oracle's operation order
+                    const double* pl = &g.plane[lk * 4];
+                    const double* dl = &g.delta[lk * 3];
+                    const int8_t kind = g.fkind[lk];
+                    KnFace kf{g.farea[lk], 0.0, 0.0, pl[0], pl[1], pl[2]};
+                    if (kind == kFaceInternal || kind == kFaceCyclic) {
+                        const double dP = std::fabs(pl[3]);
+                        const double dN = std::fabs((pl[0] * dl[0] + pl[1] * dl[1] + pl[2] * dl[2]) + pl[3]);
+                        kf.w = dN / (dP + dN);
+                        kf.dist = std::sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+                    }
+                    knf[gk] = kf;
+                    fkind[gk] = kind;
+                }
             }
         }
         for (auto b : g.bm) bm.push_back(b);
@@ -538,6 +573,7 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
             int gk = sbase + k;
             mass[gk] = sd.mass;
             omega[gk] = sd.omega;
+            diam[gk] = sd.diameter;
             rotdof[gk] = sd.rotational_dof;
             local[gk] = k;
             double a = sd.alpha, om = sd.omega, dd = sd.diameter, m = sd.mass;
@@ -602,6 +638,14 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_mass, mass))) return rc;
     if ((rc = up(d_rotdof, rotdof))) return rc;
     if ((rc = up(d_omega, omega))) return rc;
+    if ((rc = up(d_diam, diam))) return rc;
+    if (has_hybrid) {
+        if ((rc = up(d_knf, knf))) return rc;
+        if ((rc = up(d_fkind, fkind))) return rc;
+        if ((rc = dalloc(d_kw, (size_t)nc * 8))) return rc;
+        if ((rc = dalloc(d_kw2, (size_t)nc * 8))) return rc;
+        if ((rc = dalloc(d_ids2, (size_t)nc))) return rc;
+    }
     if ((rc = up(d_muref, muref))) return rc;
     if ((rc = up(d_prterm, prterm))) return rc;
     if ((rc = up(d_local, local))) return rc;
@@ -614,7 +658,7 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_smax, zeros))) return rc;
     if ((rc = up(d_uspmax, ones))) return rc;
     if ((rc = up(d_kn, zeros))) return rc;
-    std::vector<double> z3((size_t)nc * 3, 0.0), z6((size_t)nc * 6, 0.0), z8((size_t)nc * 8, 0.0), zs((size_t)nc * kNSamp, 0.0);
+    std::vector<double> z3((size_t)nc * 3, 0.0), z6((size_t)nc * 6, 0.0), z8((size_t)nc * kKnRows, 0.0), zs((size_t)nc * kNSamp, 0.0);
     if ((rc = up(d_uspq, z3))) return rc;
     if ((rc = up(d_usps, z6))) return rc;
     if ((rc = up(d_knacc, z8))) return rc;
@@ -703,8 +747,23 @@ int World::step(int n)
             marks = &tev[(size_t)tev_used * (kPhases + 1)];
             tev_used++;
         }
-        if (precision == UGX_FP32_STATE) launch_step<float>(view<float>(), stream, marks);
-        else launch_step<double>(view<double>(), stream, marks);
+        int kn_due = 0, kn_passes = 0;
+        for (const auto& c : cases)
+            if (c.active && c.coll_model == UGX_COLL_HYBRID && ((c.step + 1u) % (uint32_t)c.decomposition_interval) == 0u) {
+                kn_due = 1;
+                kn_passes = std::max(kn_passes, c.smoothing_passes);
+            }
+        if (precision == UGX_FP32_STATE) {
+            WorldView<float> w = view<float>();
+            w.kn_due = kn_due;
+            w.kn_passes = kn_passes;
+            launch_step<float>(w, stream, marks);
+        } else {
+            WorldView<double> w = view<double>();
+            w.kn_due = kn_due;
+            w.kn_passes = kn_passes;
+            launch_step<double>(w, stream, marks);
+        }
         cur = 1 - cur;
         for (auto& c : cases) c.step += 1;
     }

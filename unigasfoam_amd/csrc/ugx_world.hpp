@@ -67,6 +67,13 @@ struct World {
     double* d_mass = nullptr;
     int32_t* d_rotdof = nullptr;
     double* d_omega = nullptr;
+    double* d_diam = nullptr;
+    KnFace* d_knf = nullptr;
+    int8_t* d_fkind = nullptr;
+    double* d_kw = nullptr;
+    double* d_kw2 = nullptr;
+    int32_t* d_ids2 = nullptr;
+    int32_t has_hybrid = 0;
     double* d_muref = nullptr;
     double* d_prterm = nullptr;
     int32_t* d_local = nullptr;

@@ -152,7 +152,8 @@ def nozzle(nx=200, ny=100, nz=100, per_cell=50, n_stag=2.4e25, T0=300.0, n_back=
     spec = CaseSpec(mesh=mesh, species=[ARGON], delta_t=dt, n_equivalent_particles=fn,
                     collision_model=collision_model, dsmc_model="variableHardSphere",
                     bgk_model=bgk_model if collision_model in ("bgk", "hybrid") else "noBGKCollision",
-                    Tref=273.0, boundaries=bnd, seed=seed, decomposition_interval=100, breakdown_max=0.05)
+                    Tref=273.0, boundaries=bnd, seed=seed, decomposition_interval=20, smoothing_passes=5,
+                    breakdown_max=0.05, kn_theta=0.5)
     fill = dict(number_density={"Ar": n_fill}, T=T0, exact_per_cell=per_cell)
     return spec, fill
 
