@@ -180,6 +180,8 @@ typedef struct {
     int32_t  device;              /* HIP device ordinal */
     int64_t  particle_capacity;   /* 0 = automatic */
     uint64_t seed;
+    int32_t  use_graphs;          /* replay each step as a captured HIP graph */
+    int32_t  pad0;
 } ugx_options;
 
 /* Host-side particle arrays (always FP64 on the host). local_position, when
@@ -280,6 +282,47 @@ int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
 int ugx_kn_download(ugx_case* c, double* out, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
 int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
+
+/* ------------------------------------------------------------ batches
+ * Batched and served forms.  No counterpart in the reference, which runs one
+ * case per solver process (uniGasFoam.C:83); these entry points are for
+ * callers that run many independent cases (parametric sweeps, a request
+ * loop).  A batch holds many cases in one device world and advances them
+ * with the same kernels.  Admissions and retirements take effect at the next
+ * ugx_batch_step: the world is rebuilt and every remaining case keeps its
+ * particles, samplers, per-cell state and counters on the device.  Every
+ * random stream and every per-cell sum depends only on the case itself, so a
+ * case gives the same results alone, in a batch, and across rebuilds. */
+typedef struct ugx_batch ugx_batch;
+
+typedef struct {
+    int32_t precision;     /* ugx_precision */
+    int32_t device;
+    int64_t hbm_budget;    /* bytes the batch's cases may take (0: 60% of free HBM) */
+    int32_t max_cases;     /* 0: no limit */
+    int32_t use_graphs;    /* replay each step as a captured HIP graph */
+} ugx_batch_options;
+
+int ugx_batch_create(const ugx_batch_options* options, ugx_batch** out);
+int ugx_batch_destroy(ugx_batch* b);
+/* Device bytes a case takes in a batch: mesh arrays, per-cell state and
+ * particle storage for `particles` simulators. */
+int ugx_case_footprint(const ugx_mesh_desc* mesh, const ugx_model_desc* model, int64_t particles,
+                       int precision, int64_t* bytes);
+/* Queues a case: the descriptors of ugx_case_create plus its meshFill
+ * (fill may be NULL: the case starts empty).  Returns UGX_ERR_CAPACITY when
+ * its footprint does not fit the remaining budget or slot count. */
+int ugx_batch_admit(ugx_batch* b, const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
+                    const ugx_model_desc* model, const ugx_boundary_desc* boundaries, int n_boundaries,
+                    const ugx_fill_desc* fill, uint64_t seed, int64_t expected_particles, int* slot);
+int ugx_batch_retire(ugx_batch* b, int slot);
+int ugx_batch_step(ugx_batch* b, int n_steps);
+int ugx_batch_sync(ugx_batch* b);
+/* Non-owning handle of an admitted case for the per-case entry points
+ * (samples, fields, counters, particles, ...); ugx_step and ugx_case_destroy
+ * refuse it.  Valid until the slot is retired. */
+int ugx_batch_case(ugx_batch* b, int slot, ugx_case** out);
+int ugx_batch_info(ugx_batch* b, int64_t* n_cases, int64_t* bytes_used, int64_t* n_particles);
 
 /* Per-phase timing of ugx_step with HIP events on the case's stream (no
  * counterpart in the reference, whose only timing is the ExecutionTime /

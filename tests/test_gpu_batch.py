@@ -1,0 +1,89 @@
+"""Batched and served forms against the plain solver.
+
+Every random stream and every per-cell sum is case-local, so a case must
+give bit-identical samples, counters and particles alone (core.Case) and in
+a batch, including when other cases are admitted and retired around it and
+when the steps are replayed as captured HIP graphs.
+"""
+import numpy as np
+import pytest
+
+from unigasfoam_amd import workloads
+from unigasfoam_amd.batch import Batch
+from unigasfoam_amd.core import Case
+from unigasfoam_amd.serve import Server, make_request, run_alone
+
+pytestmark = pytest.mark.gpu
+
+
+def small_requests(n=5):
+    reqs = []
+    kinds = [dict(collision_model="dsmc"), dict(collision_model="bgk", bgk_model="unifiedStochasticParticleSBGK"),
+             dict(collision_model="hybrid", bgk_model="stochasticParticleESBGK"), dict(collision_model="dsmc"),
+             dict(collision_model="bgk", bgk_model="stochasticParticleBGK")]
+    for i in range(n):
+        k = kinds[i % len(kinds)]
+        spec, fill = workloads.couette(n_cells=20 + 7 * i, per_cell=20 + 5 * i, seed=100 + i, **k)
+        if spec.collision_model == "hybrid":
+            spec.decomposition_interval = 5
+            spec.smoothing_passes = 2
+        reqs.append(make_request(i, spec, fill, 20 + 5 * i, warmup=6 + 2 * i, samples_per_cell=200, arrival=0.0))
+    return reqs
+
+
+def assert_same(a, b, what):
+    assert np.array_equal(a.samples, b.samples), f"{what}: samples differ"
+    for k in ("n_particles", "candidates", "collisions", "bgk_collisions", "inserted", "deleted", "wall_hits",
+              "particle_steps", "n_sampled_steps", "n_dsmc_cells"):
+        assert a.counters[k] == b.counters[k], f"{what}: {k} {a.counters[k]} vs {b.counters[k]}"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_served_cases_match_plain_solver(require_gpu, precision):
+    reqs = small_requests()
+    alone = run_alone(reqs, precision=precision, use_graphs=False)
+    # at most 3 live cases: later requests are admitted as earlier ones retire
+    srv = Server(precision=precision, max_cases=3, chunk=4, use_graphs=True, realtime=False)
+    served = srv.run(reqs)
+    srv.close()
+    assert served.max_live == 3
+    for r in reqs:
+        assert_same(served.results[r.rid], alone.results[r.rid], f"case {r.rid} ({precision})")
+
+
+def test_batch_graph_replays_match_eager(require_gpu):
+    """Graph replays with eager work and synchronisation between them give
+    the same state as eager launches, step after step."""
+    specs = [workloads.couette(n_cells=30, per_cell=25, seed=7), workloads.heat_bath(n_cells=5, per_cell=15, seed=8)]
+    out = []
+    for graphs in (False, True):
+        b = Batch(precision="fp64", use_graphs=graphs)
+        slots = [b.admit(s, f, s.mesh.n_cells * 30) for s, f in specs]
+        snaps = []
+        for _ in range(4):
+            b.step(3)
+            b.sync()
+            snaps.append([b.case(s).samples().copy() for s in slots] + [b.case(s).particles()["velocity"].copy() for s in slots])
+        out.append(snaps)
+        b.close()
+    for k, (e, g) in enumerate(zip(*out)):
+        for x, y in zip(e, g):
+            assert np.array_equal(x, y), f"replay {k} differs"
+
+
+def test_admission_by_footprint(require_gpu):
+    spec, fill = workloads.couette(n_cells=50, per_cell=20)
+    from unigasfoam_amd.batch import footprint
+    fp = footprint(spec, 50 * 20, "fp32")
+    b = Batch(precision="fp32", hbm_budget=int(2.5 * fp))
+    assert b.admit(spec, fill, 1000) is not None
+    assert b.admit(spec, fill, 1000) is not None
+    assert b.admit(spec, fill, 1000) is None          # the third does not fit
+    s0 = min(b.cases)
+    b.step(2)
+    b.retire(s0)
+    assert b.admit(spec, fill, 1000) is not None      # room again after a retirement
+    b.step(2)
+    b.sync()
+    assert b.info()["cases"] == 2
+    b.close()

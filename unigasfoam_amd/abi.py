@@ -89,6 +89,14 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("device", C.c_int32),
         ("particle_capacity", C.c_int64), ("seed", C.c_uint64),
+        ("use_graphs", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("device", C.c_int32), ("hbm_budget", C.c_int64),
+        ("max_cases", C.c_int32), ("use_graphs", C.c_int32),
     ]
 
 

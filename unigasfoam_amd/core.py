@@ -26,6 +26,8 @@ EXPORTS = [
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_probe_copy",
+    "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
+    "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
 ]
 
 
@@ -73,6 +75,18 @@ def lib():
         L.ugx_timing_enable.argtypes = [vp, C.c_int]
         L.ugx_timing_get.argtypes = [vp, abi.DBL_P, C.c_int, C.POINTER(C.c_int64)]
         L.ugx_probe_copy.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        L.ugx_batch_create.argtypes = [C.POINTER(abi.BatchOptions), C.POINTER(vp)]
+        L.ugx_batch_destroy.argtypes = [vp]
+        L.ugx_case_footprint.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.ModelDesc), C.c_int64, C.c_int,
+                                         C.POINTER(C.c_int64)]
+        L.ugx_batch_admit.argtypes = [vp, C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
+                                      C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
+                                      C.POINTER(abi.FillDesc), C.c_uint64, C.c_int64, C.POINTER(C.c_int)]
+        L.ugx_batch_retire.argtypes = [vp, C.c_int]
+        L.ugx_batch_step.argtypes = [vp, C.c_int]
+        L.ugx_batch_sync.argtypes = [vp]
+        L.ugx_batch_case.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.ugx_batch_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -144,7 +158,7 @@ FIELD_NAMES = ["rhoN", "rhoM", "U_mean", "translational_T", "rotational_T", "ove
 class Case:
     """Plain GPU solver for one case (uniGasCloud with evolve() on the GPU)."""
 
-    def __init__(self, spec: CaseSpec, precision="fp64", capacity=0, device=0, seed=None):
+    def __init__(self, spec: CaseSpec, precision="fp64", capacity=0, device=0, seed=None, use_graphs=False):
         self.L = lib()
         self.spec = spec
         self.nc = spec.mesh.n_cells
@@ -154,6 +168,7 @@ class Case:
         opt.device = device
         opt.particle_capacity = capacity
         opt.seed = spec.seed if seed is None else seed
+        opt.use_graphs = 1 if use_graphs else 0
         h = C.c_void_p()
         check(self.L.ugx_case_create(C.byref(md), sp, ns, C.byref(m), bd, nb, C.byref(opt), C.byref(h)), self.L)
         self.h = h

@@ -32,9 +32,9 @@ static int validate_model(const ugx_model_desc* m)
     return UGX_OK;
 }
 
-static int make_spec(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
-                  const ugx_model_desc* model, const ugx_boundary_desc* bnd, int n_bnd, uint64_t seed,
-                  CaseSpec& spec)
+int ugx::make_spec(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
+                   const ugx_model_desc* model, const ugx_boundary_desc* bnd, int n_bnd, uint64_t seed,
+                   CaseSpec& spec)
 {
     if (!mesh || !species || !model) return set_err(UGX_ERR_ARG, "null descriptor");
     if (n_species < 1 || n_species > UGX_MAX_SPECIES) return set_err(UGX_ERR_ARG, "n_species must be in 1..%d", UGX_MAX_SPECIES);
@@ -84,6 +84,7 @@ int ugx_case_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, 
     auto* w = new World();
     w->device = opt.device;
     if (int rc = w->build(specs, opt.precision, opt.device, (uint64_t)opt.particle_capacity)) { delete w; return rc; }
+    w->use_graphs = opt.use_graphs != 0;
     auto* c = new ugx_case();
     c->world = w;
     c->index = 0;
@@ -96,13 +97,16 @@ int ugx_case_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, 
 int ugx_case_destroy(ugx_case* c)
 {
     if (!c) return UGX_OK;
+    if (c->batch) return set_err(UGX_ERR_STATE, "a batch case is released by ugx_batch_retire");
     if (c->owns_world) delete c->world;
     delete c;
     return UGX_OK;
 }
 
 #define CHECK_CASE(c) \
-    if (!(c) || !(c)->world) return set_err(UGX_ERR_ARG, "null case"); \
+    if (!(c)) return set_err(UGX_ERR_ARG, "null case"); \
+    if ((c)->batch) { if (int rc_ = resolve_case(c)) return rc_; } \
+    if (!(c)->world) return set_err(UGX_ERR_ARG, "null case"); \
     if (hipSetDevice((c)->world->device) != hipSuccess) return set_err(UGX_ERR_DEVICE, "hipSetDevice failed")
 
 static int drain(ugx_case* c)
@@ -115,6 +119,7 @@ int ugx_case_fill(ugx_case* c, const ugx_fill_desc* fill)
 {
     UGX_GUARD_BEGIN
     CHECK_CASE(c);
+    if (c->batch) return set_err(UGX_ERR_STATE, "a batch case is filled at ugx_batch_admit");
     if (!fill) return set_err(UGX_ERR_ARG, "null fill descriptor");
     for (int d = 0; d < 3; ++d) if (!(fill->T[d] > 0.0)) return set_err(UGX_ERR_ARG, "fill temperature must be > 0");
     return c->world->fill(c->index, fill, c->world->specs_host[c->index].geom);
@@ -125,6 +130,7 @@ int ugx_particles_upload(ugx_case* c, const ugx_particle_soa* p)
 {
     UGX_GUARD_BEGIN
     CHECK_CASE(c);
+    if (c->batch) return set_err(UGX_ERR_STATE, "particles of a batch case come from its admission fill");
     if (!p || p->n < 0) return set_err(UGX_ERR_ARG, "bad particle arrays");
     if (p->n == 0) return UGX_OK;
     if ((!p->position && !p->local_position) || !p->velocity || !p->cell) return set_err(UGX_ERR_ARG, "position, velocity and cell are required");
@@ -157,6 +163,7 @@ int ugx_step(ugx_case* c, int n_steps)
 {
     UGX_GUARD_BEGIN
     CHECK_CASE(c);
+    if (c->batch) return set_err(UGX_ERR_STATE, "a batch case advances with ugx_batch_step");
     if (n_steps < 0) return set_err(UGX_ERR_ARG, "n_steps must be >= 0");
     return c->world->step(n_steps);
     UGX_GUARD_END

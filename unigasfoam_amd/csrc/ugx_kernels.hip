@@ -1706,6 +1706,61 @@ void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase, const double* 
     hipLaunchKernelGGL(k_fill_finish<R>, dim3(1), dim3(1), 0, s, w, foff, total);
 }
 
+// ------------------------------------------------------ state carry-over
+// A case moved into a rebuilt world (batched and served forms) keeps its
+// particles, per-cell state and counters on the device: these copies shift
+// its cell and species ids to the new world's numbering.
+
+template <class R>
+__global__ void k_carry_particles(Parts<R> src, Parts<R> dst, uint32_t src_begin, uint32_t n, uint32_t dst_begin,
+                                  int32_t cell_delta, int32_t sp_delta)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t a = src_begin + i, b = dst_begin + i;
+        dst.x[b] = src.x[a]; dst.y[b] = src.y[a]; dst.z[b] = src.z[a];
+        dst.vx[b] = src.vx[a]; dst.vy[b] = src.vy[a]; dst.vz[b] = src.vz[a];
+        dst.erot[b] = src.erot[a];
+        dst.cell[b] = (uint32_t)((int32_t)src.cell[a] + cell_delta);
+        dst.sp[b] = (uint32_t)((int32_t)src.sp[a] + sp_delta);
+    }
+}
+
+// Per-cell state of ncl cells: old cells [ob, ob+ncl) -> new cells [nb, nb+ncl).
+__global__ void k_carry_cells(CellState o, CellState d, uint32_t ob, uint32_t nb, uint32_t ncl)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncl; i += gridDim.x * blockDim.x) {
+        const size_t a = ob + i, b = nb + i;
+        d.smax[b] = o.smax[a];
+        d.cmodel[b] = o.cmodel[a];
+        for (int q = 0; q < kNSamp; ++q) d.samp[(size_t)q * d.nc + b] = o.samp[(size_t)q * o.nc + a];
+        for (int q = 0; q < 3; ++q) d.uspq[3 * b + q] = o.uspq[3 * a + q];
+        for (int q = 0; q < 6; ++q) d.usps[6 * b + q] = o.usps[6 * a + q];
+        d.uspmax[b] = o.uspmax[a];
+        for (int q = 0; q < kKnRows; ++q) d.knacc[(size_t)kKnRows * b + q] = o.knacc[(size_t)kKnRows * a + q];
+        d.kn[b] = o.kn[a];
+    }
+}
+
+template <class R>
+void launch_carry_particles(const Parts<R>& src, const Parts<R>& dst, uint32_t src_begin, uint32_t n,
+                            uint32_t dst_begin, int32_t cell_delta, int32_t sp_delta, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_carry_particles<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, src_begin, n, dst_begin,
+                       cell_delta, sp_delta);
+}
+
+void launch_carry_cells(const CellState& o, const CellState& d, uint32_t ob, uint32_t nb, uint32_t ncl, hipStream_t s)
+{
+    if (ncl == 0) return;
+    hipLaunchKernelGGL(k_carry_cells, dim3(grid_for(ncl)), dim3(kBlock), 0, s, o, d, ob, nb, ncl);
+}
+
+template void launch_carry_particles<float>(const Parts<float>&, const Parts<float>&, uint32_t, uint32_t, uint32_t,
+                                            int32_t, int32_t, hipStream_t);
+template void launch_carry_particles<double>(const Parts<double>&, const Parts<double>&, uint32_t, uint32_t, uint32_t,
+                                             int32_t, int32_t, hipStream_t);
+
 __global__ void k_copy_probe(const float4* __restrict__ in, float4* __restrict__ out, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];

@@ -121,4 +121,21 @@ template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int cc
                                     uint32_t* fill_cnt, uint32_t* fill_off, int ns, int ncell_case);
 void launch_copy_probe(const void* in, void* out, size_t bytes, hipStream_t s);
 
+// Per-cell state of a world, for the carry-over of a case between worlds.
+struct CellState {
+    double* smax;
+    int32_t* cmodel;
+    double* samp;
+    double* uspq;
+    double* usps;
+    double* uspmax;
+    double* knacc;
+    double* kn;
+    int32_t nc;
+};
+template <class R>
+void launch_carry_particles(const Parts<R>& src, const Parts<R>& dst, uint32_t src_begin, uint32_t n,
+                            uint32_t dst_begin, int32_t cell_delta, int32_t sp_delta, hipStream_t s);
+void launch_carry_cells(const CellState& o, const CellState& d, uint32_t ob, uint32_t nb, uint32_t ncl, hipStream_t s);
+
 }  // namespace ugx

@@ -294,8 +294,17 @@ static void dfree(T*& p)
 
 World::~World() { release(); }
 
+void World::drop_graphs()
+{
+    for (auto& g : gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
 void World::release()
 {
+    drop_graphs();
     for (auto& e : tev) if (e) (void)hipEventDestroy(e);
     tev.clear();
     for (int k = 0; k < 2; ++k)
@@ -320,6 +329,7 @@ size_t World::rsize() const { return precision == UGX_FP32_STATE ? sizeof(float)
 
 int World::alloc_particles(uint32_t newcap)
 {
+    drop_graphs();
     void* nb[2][9] = {};
     size_t rs = rsize();
     for (int k = 0; k < 2; ++k)
@@ -753,7 +763,20 @@ int World::step(int n)
                 kn_due = 1;
                 kn_passes = std::max(kn_passes, c.smoothing_passes);
             }
-        if (precision == UGX_FP32_STATE) {
+        const bool graph = use_graphs && !marks && !kn_due;
+        if (graph && !gexec[cur]) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            if (precision == UGX_FP32_STATE) launch_step<float>(view<float>(), stream, nullptr);
+            else launch_step<double>(view<double>(), stream, nullptr);
+            HIP_TRY(hipStreamEndCapture(stream, &g));
+            hipError_t e = hipGraphInstantiate(&gexec[cur], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { gexec[cur] = nullptr; return set_err(UGX_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e)); }
+        }
+        if (graph) {
+            HIP_TRY(hipGraphLaunch(gexec[cur], stream));
+        } else if (precision == UGX_FP32_STATE) {
             WorldView<float> w = view<float>();
             w.kn_due = kn_due;
             w.kn_passes = kn_passes;
@@ -816,7 +839,7 @@ int World::upload(int ci, const ugx_particle_soa* p, const MeshGeom& g)
     return check_state();
 }
 
-int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g)
+int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort)
 {
     const CaseDev& cd = cases[ci];
     uint32_t n0 = 0;
@@ -858,8 +881,10 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g)
     cleanup();
     if (e != hipSuccess) return set_err(UGX_ERR_DEVICE, "fill failed: %s", hipGetErrorString(e));
     if (int r2 = check_state()) return r2;
-    if (int r2 = sort_now()) return r2;
-    HIP_TRY(hipStreamSynchronize(stream));
+    if (sort) {
+        if (int r2 = sort_now()) return r2;
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
     // (sigma_T c_r)max = pi d^2 c_mp(T) of the most abundant species (uniGasMeshFill.C:284-296)
     int best = 0;
     for (int s = 0; s < ns; ++s) if (f->number_density[s] > f->number_density[best]) best = s;
@@ -871,6 +896,53 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g)
     uint32_t n1 = 0;
     if (int r2 = count(n1)) return r2;
     return ensure_capacity((uint64_t)n1 * 2 + (uint64_t)max_ins * 8);
+}
+
+CellState World::cell_state()
+{
+    return CellState{d_smax, d_cmodel, d_samp, d_uspq, d_usps, d_uspmax, d_knacc, d_kn, nc};
+}
+
+int World::adopt(World& old, const std::vector<int>& old_of_new)
+{
+    HIP_TRY(hipStreamSynchronize(old.stream));
+    std::vector<uint32_t> ooff((size_t)old.nc + 1);
+    HIP_TRY(hipMemcpy(ooff.data(), old.d_off, sizeof(uint32_t) * ooff.size(), hipMemcpyDeviceToHost));
+    std::vector<CaseDev> ocd(old.cases.size());
+    HIP_TRY(hipMemcpy(ocd.data(), old.d_cases, sizeof(CaseDev) * ocd.size(), hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    for (size_t i = 0; i < old_of_new.size(); ++i) {
+        const int k = old_of_new[i];
+        if (k >= 0) total += ooff[old.cases[k].cell_end] - ooff[old.cases[k].cell_begin];
+    }
+    if (int rc = ensure_capacity(total + total / 4 + 1024)) return rc;
+    const CellState os = old.cell_state(), ns_ = cell_state();
+    uint32_t dst = 0;
+    for (size_t i = 0; i < old_of_new.size(); ++i) {
+        const int k = old_of_new[i];
+        if (k < 0) continue;
+        const CaseDev& oc = old.cases[k];
+        CaseDev& nc_ = cases[i];
+        const uint32_t ncl = (uint32_t)(oc.cell_end - oc.cell_begin);
+        launch_carry_cells(os, ns_, (uint32_t)oc.cell_begin, (uint32_t)nc_.cell_begin, ncl, stream);
+        HIP_TRY(hipMemcpyAsync(d_counters + (size_t)i * kCntNum, old.d_counters + (size_t)k * kCntNum,
+                               sizeof(unsigned long long) * kCntNum, hipMemcpyDeviceToDevice, stream));
+        const uint32_t b = ooff[oc.cell_begin], n = ooff[oc.cell_end] - b;
+        const int32_t cd = nc_.cell_begin - oc.cell_begin, sd = nc_.sp_base - oc.sp_base;
+        if (precision == UGX_FP32_STATE)
+            launch_carry_particles<float>(old.view<float>().a, view<float>().a, b, n, dst, cd, sd, stream);
+        else
+            launch_carry_particles<double>(old.view<double>().a, view<double>().a, b, n, dst, cd, sd, stream);
+        dst += n;
+        nc_.step = ocd[k].step;
+        nc_.kn_time = ocd[k].kn_time;
+        samp_reset_steps[i] = old.samp_reset_steps[k];
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(&d_st->n, &dst, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return UGX_OK;
 }
 
 int World::download(int ci, ugx_particle_soa* p, const MeshGeom& g)

@@ -102,6 +102,11 @@ struct World {
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;
+    // HIP-graph replay of whole steps: one executable graph per buffer parity
+    // (the step's kernels and arguments are otherwise fixed for a world)
+    bool use_graphs = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    void drop_graphs();
     std::vector<hipEvent_t> tev;
     int tev_used = 0;
     double phase_ms[kPhases] = {};
@@ -124,7 +129,11 @@ struct World {
     int sort_now();
     int step(int n);
     int upload(int ci, const ugx_particle_soa* p, const MeshGeom& g);
-    int fill(int ci, const ugx_fill_desc* f, const MeshGeom& g);
+    int fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort = true);
+    // Copies the state of the cases of `old` into this freshly built world:
+    // new case i takes old case old_of_new[i] (or starts empty if < 0).
+    int adopt(World& old, const std::vector<int>& old_of_new);
+    CellState cell_state();
     int download(int ci, ugx_particle_soa* p, const MeshGeom& g);
     int case_counters(int ci, ugx_counters* out);
     int case_samples(int ci, double* out, int64_t len);
@@ -142,4 +151,15 @@ struct ugx_case {
     ugx::World* world = nullptr;
     int index = 0;
     bool owns_world = false;
+    ugx_batch* batch = nullptr;   // set for the handle of a batch slot
+    int slot = -1;
 };
+
+namespace ugx {
+// Points a batch-slot handle at its current world and case index; fails
+// while the slot has not entered the world yet.
+int resolve_case(ugx_case* c);
+// CaseSpec from the C descriptors (ugx_api.cpp).
+int make_spec(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n_species,
+              const ugx_model_desc* model, const ugx_boundary_desc* bnd, int n_bnd, uint64_t seed, CaseSpec& spec);
+}  // namespace ugx
