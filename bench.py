@@ -79,6 +79,52 @@ def cpu_baseline(name, seconds, scale):
             "sample": f"{sample}; {steps} steps in {dt:.1f} s (restated CPU baseline, not uniGasFoam)"}
 
 
+def sweep_requests(n_cases, rate, seed=5):
+    """Config C5: parametric Couette/Fourier cases with Poisson arrivals."""
+    import numpy as np
+    from unigasfoam_amd import workloads
+    from unigasfoam_amd.serve import make_request
+    params = workloads.sweep_params(n_cases, seed=seed)
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n_cases)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for i, p in enumerate(params):
+        spec, fill = workloads.sweep_case(p)
+        reqs.append(make_request(i, spec, fill, p["per_cell"], warmup=50, samples_per_cell=2000,
+                                 arrival=float(arrivals[i])))
+    return reqs
+
+
+def sweep_leg(n_cases, rate, precision):
+    from unigasfoam_amd.serve import Server, run_alone
+    reqs = sweep_requests(n_cases, rate)
+    print(f"[bench] sweep: {n_cases} cases, plain one at a time ...", file=sys.stderr, flush=True)
+    alone = run_alone(reqs, precision=precision, use_graphs=True)
+    print("[bench] sweep: batched (all cases admitted at once) ...", file=sys.stderr, flush=True)
+    srv = Server(precision=precision, chunk=10, use_graphs=True, realtime=False)
+    batched = srv.run(reqs)
+    srv.close()
+    print(f"[bench] sweep: served (Poisson arrivals at {rate}/s) ...", file=sys.stderr, flush=True)
+    srv = Server(precision=precision, chunk=10, use_graphs=True, realtime=True)
+    served = srv.run(reqs)
+    srv.close()
+    a, b, c = alone.summary(), batched.summary(), served.summary()
+    return {
+        "workload": f"{n_cases} parametric 1-D Couette/Fourier cases (Kn 0.01-10, 50-2000 cells, 20-200 simulators/cell, "
+                    f"Ar or N2 LB), 50 warm-up steps then 2000 samples per cell",
+        "particle_steps": a["particle_steps"],
+        "plain_one_at_a_time": {"particle_steps_per_s": a["particle_steps_per_s"], "wall_s": round(a["wall_s"], 2)},
+        "batched": {"particle_steps_per_s": b["particle_steps_per_s"], "wall_s": round(b["wall_s"], 2),
+                    "max_concurrent_cases": b["max_concurrent_cases"],
+                    "vs_plain": b["particle_steps_per_s"] / a["particle_steps_per_s"]},
+        "served": {"arrival_rate_per_s": rate, "particle_steps_per_s": c["particle_steps_per_s"],
+                   "wall_s": round(c["wall_s"], 2), "p50_time_to_converged_sample_s": c["p50_latency_s"],
+                   "p99_time_to_converged_sample_s": c["p99_latency_s"],
+                   "max_concurrent_cases": c["max_concurrent_cases"],
+                   "vs_plain": c["particle_steps_per_s"] / a["particle_steps_per_s"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +138,8 @@ def main():
     ap.add_argument("--probe", action="store_true", help="also run the stream-copy HBM probe")
     ap.add_argument("--collision", default="hybrid", choices=["hybrid", "dsmc", "bgk", "none"],
                     help="collision model of the nozzle workload")
+    ap.add_argument("--sweep-cases", type=int, default=256, help="cases of the batched/served sweep (0: skip)")
+    ap.add_argument("--sweep-rate", type=float, default=20.0, help="arrival rate of the served sweep (cases/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -180,6 +228,8 @@ def main():
         s = probe_copy(nbytes, reps=10)
         out["hbm_probe"] = {"copy_bytes": nbytes, "GBs": round(2 * nbytes / s / 1e9, 1)}
     case.close()
+    if args.sweep_cases > 0:
+        out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, args.scale)

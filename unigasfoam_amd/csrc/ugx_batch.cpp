@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -30,9 +31,11 @@ struct ugx_batch {
     ugx_batch_options opt{};
     int64_t budget = 0;
     int64_t used_bytes = 0;
+    int64_t paused_particles = 0;   // simulators of retired cases still in the world
     std::vector<Slot> slots;
     World* world = nullptr;
     bool dirty = false;
+    int64_t rebuilds = 0;
     ~ugx_batch() { delete world; }
 };
 
@@ -83,6 +86,11 @@ static int rebuild(ugx_batch* b)
         nw->device = b->opt.device;
         if (int rc = nw->build(specs, b->opt.precision, b->opt.device, cap)) { delete nw; return rc; }
         nw->use_graphs = b->opt.use_graphs != 0;
+        if (b->world && b->world->tune_step >= 2 * kTileVariants) {
+            // keep the tuned cell-tile configuration across rebuilds
+            nw->tile_variant = b->world->tile_variant;
+            nw->tune_step = b->world->tune_step;
+        }
         if (b->world) {
             if (int rc = nw->adopt(*b->world, old_of_new)) { delete nw; return rc; }
         }
@@ -107,6 +115,8 @@ static int rebuild(ugx_batch* b)
         s.world_index = (int)i;
     }
     b->dirty = false;
+    b->paused_particles = 0;
+    b->rebuilds += 1;
     return UGX_OK;
 }
 
@@ -214,7 +224,13 @@ int ugx_batch_retire(ugx_batch* b, int slot)
     s.used = false;
     s.handle.reset();
     b->used_bytes -= s.bytes;
-    b->dirty = true;
+    if (s.live && b->world) {
+        // stop it in place; its memory is reclaimed at the next rebuild
+        ugx_counters k{};
+        if (int rc = b->world->case_counters(s.world_index, &k)) return rc;
+        b->paused_particles += k.n_particles;
+        if (int rc = b->world->pause(s.world_index)) return rc;
+    }
     return UGX_OK;
     UGX_GUARD_END
 }
@@ -225,6 +241,12 @@ int ugx_batch_step(ugx_batch* b, int n_steps)
     if (!b) return set_err(UGX_ERR_ARG, "null batch");
     if (n_steps < 0) return set_err(UGX_ERR_ARG, "n_steps must be >= 0");
     if (hipSetDevice(b->opt.device) != hipSuccess) return set_err(UGX_ERR_DEVICE, "hipSetDevice failed");
+    // rebuild for waiting admissions, or when stopped cases hold most of the simulators
+    if (!b->dirty && b->world && b->paused_particles > 0) {
+        uint32_t n = 0;
+        if (int rc = b->world->count(n)) return rc;
+        if (2 * b->paused_particles > (int64_t)n) b->dirty = true;
+    }
     if (b->dirty)
         if (int rc = rebuild(b)) return rc;
     if (!b->world || n_steps == 0) return UGX_OK;

@@ -311,6 +311,9 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
         if (active) {
             cell = w.a.cell[t];
             ccase = case_of_slot(w, t);
+        }
+        // simulators of a paused case (a retired batch case) stay where they are
+        if (active && w.cases[ccase].active) {
             const CaseDev& cs = w.cases[ccase];
             R x[3] = {w.a.x[t], w.a.y[t], w.a.z[t]};
             R U[3] = {w.a.vx[t], w.a.vy[t], w.a.vz[t]};
@@ -624,14 +627,14 @@ __device__ inline void cell_partials(const WorldView<R>& w, int sp_base, int nsw
 // 8g..8g+7 takes cell k), then the sampler (uniGasVolFields.C:761-797), the
 // hybrid accumulators and the BGK moments.  All 64 lanes of every wave
 // call it with the same arguments.
-template <int L, class R, class Acc>
+template <int L, int Block, class R, class Acc>
 __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nct, const uint32_t* s_off,
                                  const Acc& A)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = tid >> 6, lane = tid & 63u, g = lane >> 3, j = lane & 7u;
     const int nsw = w.ns_max;
-    for (uint32_t kb = 0; kb < nct; kb += (kTileBlock / 64) * 8) {
+    for (uint32_t kb = 0; kb < nct; kb += (Block / 64) * 8) {
         const uint32_t k = kb + wv * 8u + g;
         uint32_t b = 0, e = 0;
         bool sample = false, hybrid = false, bgk = false;
@@ -688,16 +691,16 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
 
 // ------------------------------------------------------------ tile table
 
-// tile_first[t] = first cell whose key off[c] + kCellW * c is >= t * kTileN;
+// tile_first[t] = first cell whose key off[c] + kCellW * c is >= t * TileN;
 // tile_first[ntiles] = nc.  Each cell writes the tiles that start at it.
-template <class R>
+template <class R, uint32_t TileN>
 __global__ void k_tile_table(WorldView<R> w)
 {
     const uint32_t nc = (uint32_t)w.nc;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= nc; c += gridDim.x * blockDim.x) {
         const uint64_t key = (uint64_t)w.off[c] + (uint64_t)kCellW * c;
-        const uint64_t lo = (c == 0u) ? 0ull : ((uint64_t)w.off[c - 1] + (uint64_t)kCellW * (c - 1u)) / kTileN + 1ull;
-        uint64_t hi = key / kTileN;
+        const uint64_t lo = (c == 0u) ? 0ull : ((uint64_t)w.off[c - 1] + (uint64_t)kCellW * (c - 1u)) / TileN + 1ull;
+        uint64_t hi = key / TileN;
         if (c == nc) {
             hi += 1ull;
             w.st->ntiles = (uint32_t)hi;
@@ -715,16 +718,16 @@ __global__ void k_tile_table(WorldView<R> w)
 //  2. records are gathered into the sorted buffers B (velocity, E_rot and
 //     local species also into LDS);
 //  3. per-cell sums from LDS: sampler, hybrid accumulators, BGK moments.
-// SortOnly stops after 2.  Tiles larger than kTileMax run the same steps
+// SortOnly stops after 2.  Tiles larger than TileMax run the same steps
 // from HBM.
-template <class R, bool SortOnly, int L>
-__global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4))) k_tile(WorldView<R> w)
+template <class R, bool SortOnly, int L, int Block, uint32_t TileN, uint32_t TileMax>
+__global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4))) k_tile(WorldView<R> w)
 {
-    constexpr uint32_t kM = SortOnly ? 1u : kTileMax;
-    __shared__ uint32_t s_off[kTileCells + 2];
-    __shared__ uint32_t s_in[kTileMax];
-    __shared__ uint32_t s_perm[kTileMax];
-    __shared__ uint8_t s_k[kTileMax];
+    constexpr uint32_t kM = SortOnly ? 1u : TileMax;
+    __shared__ uint32_t s_off[TileN / kCellW + 2];
+    __shared__ uint32_t s_in[TileMax];
+    __shared__ uint32_t s_perm[TileMax];
+    __shared__ uint8_t s_k[TileMax];
     __shared__ R s_vx[kM], s_vy[kM], s_vz[kM], s_er[kM];
     __shared__ uint8_t s_sl[kM];
     __shared__ double s_m[kM];
@@ -734,14 +737,14 @@ __global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu
     const uint32_t tid = threadIdx.x;
     const uint32_t c0 = w.tile_first[t];
     const uint32_t nct = w.tile_first[t + 1] - c0;
-    for (uint32_t k = tid; k <= nct; k += kTileBlock) s_off[k] = w.off[c0 + k];
+    for (uint32_t k = tid; k <= nct; k += Block) s_off[k] = w.off[c0 + k];
     __syncthreads();
     const uint32_t p0 = s_off[0];
     const uint32_t size = s_off[nct] - p0;
-    if (size <= kTileMax) {
-        for (uint32_t i = tid; i < size; i += kTileBlock) s_in[i] = w.perm[p0 + i];
+    if (size <= TileMax) {
+        for (uint32_t i = tid; i < size; i += Block) s_in[i] = w.perm[p0 + i];
         __syncthreads();
-        for (uint32_t i = tid; i < size; i += kTileBlock) {
+        for (uint32_t i = tid; i < size; i += Block) {
             const uint32_t slot = s_in[i];
             // cell of entry i: the last k with s_off[k] <= p0 + i
             uint32_t lo = 0, hi = nct - 1u;
@@ -756,7 +759,7 @@ __global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu
             s_k[b + r] = (uint8_t)lo;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < size; i += kTileBlock) {
+        for (uint32_t i = tid; i < size; i += Block) {
             const uint32_t slot = s_perm[i];
             const uint32_t j = p0 + i;
             const R vx = w.a.vx[slot], vy = w.a.vy[slot], vz = w.a.vz[slot], er = w.a.erot[slot];
@@ -776,7 +779,7 @@ __global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu
         if (SortOnly) return;
         __syncthreads();
         LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_m, s_rd, p0};
-        tile_sums<L, R>(w, c0, nct, s_off, A);
+        tile_sums<L, Block, R>(w, c0, nct, s_off, A);
     } else {
         // Tile larger than LDS: the same steps in HBM.  The rank of an entry
         // in its cell is counted against the cell's slot list staged through
@@ -784,15 +787,15 @@ __global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu
         if (tid == 0) atomicAdd(&w.st->big_tiles, 1u);
         for (uint32_t k = 0; k < nct; ++k) {
             const uint32_t b = s_off[k], e = s_off[k + 1];
-            for (uint32_t ib = b; ib < e; ib += kTileBlock) {
+            for (uint32_t ib = b; ib < e; ib += Block) {
                 const uint32_t i = ib + tid;
                 const bool mine = i < e;
                 const uint32_t slot = mine ? w.perm[i] : 0u;
                 uint32_t r = 0;
-                for (uint32_t jb = b; jb < e; jb += kTileMax) {
-                    const uint32_t nj = min(kTileMax, e - jb);
+                for (uint32_t jb = b; jb < e; jb += TileMax) {
+                    const uint32_t nj = min(TileMax, e - jb);
                     __syncthreads();
-                    for (uint32_t q = tid; q < nj; q += kTileBlock) s_in[q] = w.perm[jb + q];
+                    for (uint32_t q = tid; q < nj; q += Block) s_in[q] = w.perm[jb + q];
                     __syncthreads();
                     if (mine)
                         for (uint32_t q = 0; q < nj; ++q) r += (s_in[q] < slot) ? 1u : 0u;
@@ -801,7 +804,7 @@ __global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu
             }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < size; i += kTileBlock) {
+        for (uint32_t i = tid; i < size; i += Block) {
             const uint32_t slot = w.scratch[p0 + i];
             const uint32_t j = p0 + i;
             w.b.x[j] = w.a.x[slot]; w.b.y[j] = w.a.y[slot]; w.b.z[j] = w.a.z[slot];
@@ -814,7 +817,7 @@ __global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu
         __threadfence_block();
         __syncthreads();
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        tile_sums<L, R>(w, c0, nct, s_off, A);
+        tile_sums<L, Block, R>(w, c0, nct, s_off, A);
     }
 }
 
@@ -1619,20 +1622,32 @@ static void launch_decompose(const WorldView<R>& w, hipStream_t s)
     hipLaunchKernelGGL(k_kn_end<R>, dim3(grid_for((uint64_t)w.ncases)), dim3(kBlock), 0, s, w);
 }
 
+template <class R, int Block, uint32_t TileN, uint32_t TileMax>
+static void launch_tiles_cfg(const WorldView<R>& w, hipStream_t s, bool sort_only)
+{
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    const unsigned nt = (unsigned)((w.cap + (uint64_t)kCellW * (uint64_t)w.nc) / TileN + 2);
+    hipLaunchKernelGGL((k_tile_table<R, TileN>), dim3(grid_for(ncp)), dim3(kBlock), 0, s, w);
+    if (sort_only)
+        hipLaunchKernelGGL((k_tile<R, true, 0, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
+    else if (w.sum_level >= 2)
+        hipLaunchKernelGGL((k_tile<R, false, 2, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
+    else if (w.sum_level == 1)
+        hipLaunchKernelGGL((k_tile<R, false, 1, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
+    else
+        hipLaunchKernelGGL((k_tile<R, false, 0, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
+}
+
+// Cell-tile configurations (threads, key window, LDS simulators); the
+// choice changes the speed only, never a result.
 template <class R>
 static void launch_tiles(const WorldView<R>& w, hipStream_t s, bool sort_only)
 {
-    const uint32_t ncp = (uint32_t)w.nc + 1u;
-    const unsigned nt = (unsigned)max_tiles(w.cap, (uint64_t)w.nc);
-    hipLaunchKernelGGL(k_tile_table<R>, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w);
-    if (sort_only)
-        hipLaunchKernelGGL((k_tile<R, true, 0>), dim3(nt), dim3(kTileBlock), 0, s, w);
-    else if (w.sum_level >= 2)
-        hipLaunchKernelGGL((k_tile<R, false, 2>), dim3(nt), dim3(kTileBlock), 0, s, w);
-    else if (w.sum_level == 1)
-        hipLaunchKernelGGL((k_tile<R, false, 1>), dim3(nt), dim3(kTileBlock), 0, s, w);
-    else
-        hipLaunchKernelGGL((k_tile<R, false, 0>), dim3(nt), dim3(kTileBlock), 0, s, w);
+    switch (w.tile_variant) {
+    case 1: launch_tiles_cfg<R, 256, 512, 1024>(w, s, sort_only); break;
+    case 2: launch_tiles_cfg<R, 64, 128, 256>(w, s, sort_only); break;
+    default: launch_tiles_cfg<R, 128, 256, 512>(w, s, sort_only); break;
+    }
 }
 
 template <class R>

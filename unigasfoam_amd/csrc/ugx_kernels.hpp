@@ -76,6 +76,7 @@ struct WorldView {
     int32_t sum_level;        // largest sampler level of a case (0, 1, 2)
     int32_t has_collide;      // some case collides (DSMC or BGK family)
     int32_t has_hybrid;       // some case switches models per cell
+    int32_t tile_variant;     // cell-tile configuration (0..kTileVariants-1)
     int32_t kn_due;           // a hybrid case decomposes at this step
     int32_t kn_passes;        // largest smoothing pass count of the due cases
     const KnFace* knf;        // [nc*kMaxFaces]
@@ -101,15 +102,16 @@ struct WorldView {
 constexpr int kNMom = UGX_S_SPECIES_MCC;
 
 // Cell tiles of the sort: tile t holds the cells c whose key
-// off[c] + kCellW * c lies in [t * kTileN, (t + 1) * kTileN), so a tile has
-// at most kTileN / kCellW cells and fewer than kTileN simulators plus those
+// off[c] + kCellW * c lies in [t * TileN, (t + 1) * TileN), so a tile has
+// at most TileN / kCellW cells and fewer than TileN simulators plus those
 // of its last cell.
-constexpr uint32_t kTileN = 256;
+// Tile configurations: (threads, TileN window, LDS simulators) =
+// (128, 256, 512), (256, 512, 1024), (64, 128, 256); the table is sized for
+// the smallest window.
 constexpr uint32_t kCellW = 8;
-constexpr uint32_t kTileCells = kTileN / kCellW;
-constexpr uint32_t kTileMax = 512;    // simulators of a tile staged in LDS
-constexpr int kTileBlock = 128;       // threads of a tile block
-inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc) / kTileN + 2; }
+constexpr uint32_t kMinTileN = 128;
+constexpr int kTileVariants = 3;
+inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc) / kMinTileN + 2; }
 
 // Host launchers (ugx_kernels.hip).
 constexpr int kPhases = 9;  // begin, inflow, move, wall, scan, scatter, cells, collide, end

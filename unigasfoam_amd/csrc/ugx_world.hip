@@ -305,6 +305,8 @@ void World::drop_graphs()
 void World::release()
 {
     drop_graphs();
+    for (auto& e : tune_ev) if (e) (void)hipEventDestroy(e);
+    tune_ev.clear();
     for (auto& e : tev) if (e) (void)hipEventDestroy(e);
     tev.clear();
     for (int k = 0; k < 2; ++k)
@@ -426,6 +428,7 @@ WorldView<R> World::view() const
     w.cc = d_cc;
     w.sp = Species{d_mass, d_diam, d_rotdof, d_omega, d_muref, d_prterm, d_local};
     w.has_hybrid = has_hybrid;
+    w.tile_variant = tile_variant;
     w.kn_due = 0;
     w.kn_passes = 0;
     w.knf = d_knf;
@@ -763,6 +766,16 @@ int World::step(int n)
                 kn_due = 1;
                 kn_passes = std::max(kn_passes, c.smoothing_passes);
             }
+        const bool tuning = autotune && !marks && tune_step < 2 * kTileVariants;
+        int variant = tile_variant;
+        if (tuning) {
+            if (tune_ev.empty()) {
+                tune_ev.resize(kPhases + 1);
+                for (auto& e : tune_ev) HIP_TRY(hipEventCreate(&e));
+            }
+            marks = tune_ev.data();
+            variant = tune_step / 2;
+        }
         const bool graph = use_graphs && !marks && !kn_due;
         if (graph && !gexec[cur]) {
             hipGraph_t g = nullptr;
@@ -780,12 +793,26 @@ int World::step(int n)
             WorldView<float> w = view<float>();
             w.kn_due = kn_due;
             w.kn_passes = kn_passes;
+            w.tile_variant = variant;
             launch_step<float>(w, stream, marks);
         } else {
             WorldView<double> w = view<double>();
             w.kn_due = kn_due;
             w.kn_passes = kn_passes;
+            w.tile_variant = variant;
             launch_step<double>(w, stream, marks);
+        }
+        if (tuning) {
+            HIP_TRY(hipEventSynchronize(tune_ev[kPhases]));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, tune_ev[6], tune_ev[7]));
+            tune_ms[variant] += ms;
+            if (++tune_step == 2 * kTileVariants) {
+                int best = 0;
+                for (int v = 1; v < kTileVariants; ++v) if (tune_ms[v] < tune_ms[best]) best = v;
+                tile_variant = best;
+                drop_graphs();
+            }
         }
         cur = 1 - cur;
         for (auto& c : cases) c.step += 1;
@@ -896,6 +923,16 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort)
     uint32_t n1 = 0;
     if (int r2 = count(n1)) return r2;
     return ensure_capacity((uint64_t)n1 * 2 + (uint64_t)max_ins * 8);
+}
+
+int World::pause(int ci)
+{
+    static const int32_t zero = 0;
+    cases[ci].active = 0;
+    HIP_TRY(hipMemcpyAsync((char*)(d_cases + ci) + offsetof(CaseDev, active), &zero, sizeof(int32_t),
+                           hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return UGX_OK;
 }
 
 CellState World::cell_state()

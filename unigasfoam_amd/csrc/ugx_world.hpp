@@ -105,6 +105,13 @@ struct World {
     // HIP-graph replay of whole steps: one executable graph per buffer parity
     // (the step's kernels and arguments are otherwise fixed for a world)
     bool use_graphs = false;
+    // autotuner: the first 2 * kTileVariants steps time each cell-tile
+    // configuration on the live case (results do not depend on it)
+    bool autotune = true;
+    int tile_variant = 0;
+    int tune_step = 0;
+    double tune_ms[kTileVariants] = {};
+    std::vector<hipEvent_t> tune_ev;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     void drop_graphs();
     std::vector<hipEvent_t> tev;
@@ -134,6 +141,9 @@ struct World {
     // new case i takes old case old_of_new[i] (or starts empty if < 0).
     int adopt(World& old, const std::vector<int>& old_of_new);
     CellState cell_state();
+    // Stops a case in place (its simulators are kept but no longer move,
+    // collide or sample); a batch retires cases this way until its next rebuild.
+    int pause(int ci);
     int download(int ci, ugx_particle_soa* p, const MeshGeom& g);
     int case_counters(int ci, ugx_counters* out);
     int case_samples(int ci, double* out, int64_t len);
