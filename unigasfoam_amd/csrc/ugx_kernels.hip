@@ -670,10 +670,15 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         }
         if (need && j == 0u) {
             if (sample) {
-                const bool full = w.cases[w.cell_case[c]].sampler_full != 0;
+                const CaseDev& cs = w.cases[w.cell_case[c]];
+                const bool full = cs.sampler_full != 0;
+                double* sc = &w.samp[(size_t)c * kNSamp];
 #pragma unroll
-                for (int q = 0; q < kNSamp; ++q)
-                    if (full || q < UGX_S_MUU || q >= UGX_S_BASE) w.samp[(size_t)q * w.nc + c] += dt * S[q];
+                for (int q = 0; q < kNSamp; ++q) {
+                    // rows of species the case does not have stay zero
+                    const bool used = (q < UGX_S_BASE) ? (full || q < UGX_S_MUU) : (((q - UGX_S_BASE) % kNS) < cs.ns);
+                    if (used) sc[q] += dt * S[q];
+                }
             }
             if (hybrid) {
 #pragma unroll
@@ -683,7 +688,7 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
             }
             if (bgk) {
 #pragma unroll
-                for (int q = 0; q < kNMom; ++q) w.mom[(size_t)q * w.nc + c] = S[q];
+                for (int q = 0; q < kNMom; ++q) w.mom[(size_t)c * kNMom + q] = S[q];
             }
         }
     }
@@ -1067,7 +1072,7 @@ __global__ void __launch_bounds__(kBlock) k_collide(WorldView<R> w)
         const CaseDev& cs = w.cases[ccase];
         double S[kNMom];
 #pragma unroll
-        for (int q = 0; q < kNMom; ++q) S[q] = w.mom[(size_t)q * w.nc + c];
+        for (int q = 0; q < kNMom; ++q) S[q] = w.mom[(size_t)c * kNMom + q];
         BgkP P;
         bgk_params<R>(w, cs, c, nC, S, P, nrel);
         w.bgkp[c] = P;
@@ -1157,8 +1162,8 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
                 b = w.off[c];
                 e = w.off[c + 1];
                 T = P.T; U0 = P.U[0]; U1 = P.U[1]; U2 = P.U[2];
-                M = w.mom[(size_t)UGX_S_M * w.nc + c];
-                N = w.mom[(size_t)UGX_S_N * w.nc + c];
+                M = w.mom[(size_t)c * kNMom + UGX_S_M];
+                N = w.mom[(size_t)c * kNMom + UGX_S_N];
             }
         }
     }
@@ -1747,7 +1752,7 @@ __global__ void k_carry_cells(CellState o, CellState d, uint32_t ob, uint32_t nb
         const size_t a = ob + i, b = nb + i;
         d.smax[b] = o.smax[a];
         d.cmodel[b] = o.cmodel[a];
-        for (int q = 0; q < kNSamp; ++q) d.samp[(size_t)q * d.nc + b] = o.samp[(size_t)q * o.nc + a];
+        for (int q = 0; q < kNSamp; ++q) d.samp[b * kNSamp + q] = o.samp[a * kNSamp + q];
         for (int q = 0; q < 3; ++q) d.uspq[3 * b + q] = o.uspq[3 * a + q];
         for (int q = 0; q < 6; ++q) d.usps[6 * b + q] = o.usps[6 * a + q];
         d.uspmax[b] = o.uspmax[a];

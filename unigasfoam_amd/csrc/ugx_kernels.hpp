@@ -50,7 +50,7 @@ struct WorldView {
     const double* vol;        // [nc]
     double* smax;             // [nc]
     int32_t* cmodel;          // [nc] 1 DSMC, 0 BGK
-    double* samp;             // [kNSamp * nc]
+    double* samp;             // [nc][kNSamp] (cell-major; downloads transpose)
     double* uspq;             // [3*nc]
     double* usps;             // [6*nc]
     double* uspmax;           // [nc]
@@ -66,7 +66,7 @@ struct WorldView {
     uint32_t* ins_off;        // [n_inflow+1]
     uint32_t* scratch;        // [cap] rank scratch of tiles larger than LDS
     uint32_t* tile_first;     // [max tiles + 2] first cell of each tile
-    double* mom;              // [kNMom * nc] per-cell sums for the BGK family
+    double* mom;              // [nc][kNMom] per-cell sums for the BGK family
     BgkP* bgkp;               // [nc]
     unsigned long long* raise;// [nc] USP maxProb raise (bits of a positive double)
     uint32_t* sel_slot;       // [cap] BGK selections: sorted slot,

@@ -1056,22 +1056,23 @@ int World::case_counters(int ci, ugx_counters* out)
 
 int World::case_samples(int ci, double* out, int64_t len)
 {
-    int ncl = cases[ci].cell_end - cases[ci].cell_begin;
+    const int ncl = cases[ci].cell_end - cases[ci].cell_begin;
     if (len < (int64_t)kNSamp * ncl) return set_err(UGX_ERR_ARG, "sample buffer holds %lld, need %d", (long long)len, kNSamp * ncl);
-    for (int k = 0; k < kNSamp; ++k)
-        HIP_TRY(hipMemcpyAsync(out + (size_t)k * ncl, d_samp + (size_t)k * nc + cases[ci].cell_begin,
-                               sizeof(double) * ncl, hipMemcpyDeviceToHost, stream));
+    // device layout [cell][row] -> ABI layout [row][cell]
+    std::vector<double> cm((size_t)ncl * kNSamp);
+    HIP_TRY(hipMemcpyAsync(cm.data(), d_samp + (size_t)cases[ci].cell_begin * kNSamp, sizeof(double) * cm.size(),
+                           hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    for (int c = 0; c < ncl; ++c)
+        for (int k = 0; k < kNSamp; ++k) out[(size_t)k * ncl + c] = cm[(size_t)c * kNSamp + k];
     return UGX_OK;
 }
-
 int World::case_samples_reset(int ci)
 {
     HIP_TRY(hipStreamSynchronize(stream));
-    int ncl = cases[ci].cell_end - cases[ci].cell_begin;
-    std::vector<double> z((size_t)ncl, 0.0);
-    for (int k = 0; k < kNSamp; ++k)
-        HIP_TRY(hipMemcpy(d_samp + (size_t)k * nc + cases[ci].cell_begin, z.data(), sizeof(double) * ncl, hipMemcpyHostToDevice));
+    const int ncl = cases[ci].cell_end - cases[ci].cell_begin;
+    std::vector<double> z((size_t)ncl * kNSamp, 0.0);
+    HIP_TRY(hipMemcpy(d_samp + (size_t)cases[ci].cell_begin * kNSamp, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
     unsigned long long v = 0;
     HIP_TRY(hipMemcpy(&v, d_counters + (size_t)ci * kCntNum + kCntSampled, sizeof v, hipMemcpyDeviceToHost));
     samp_reset_steps[ci] = (int64_t)v;
