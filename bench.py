@@ -125,11 +125,23 @@ def sweep_leg(n_cases, rate, precision):
     }
 
 
+def precision_leg():
+    """FP32 state against all-FP64 on the GPU: Mach-10 N2/O2 cylinder (config C3
+    at a reduced mesh), fields sampled over the last 200 of 400 steps."""
+    from unigasfoam_amd import workloads
+    from unigasfoam_amd.precision import compare
+    spec, fill = workloads.cylinder(n_theta=100, n_r=50, target=2.0e6, dt=1.0e-7)
+    r = compare(spec, fill, warmup=200, sample=400, blocks=8)
+    return {"workload": "Mach-10 79/21 N2/O2 over a cylinder, 100x50x1 cells, ~2e6 simulators, 200 steps, then "
+                        "8 sample blocks of 50 steps (standard errors from the block means)",
+            "fields": r}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="nozzle")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the nozzle mesh")
@@ -140,6 +152,7 @@ def main():
                     help="collision model of the nozzle workload")
     ap.add_argument("--sweep-cases", type=int, default=256, help="cases of the batched/served sweep (0: skip)")
     ap.add_argument("--sweep-rate", type=float, default=20.0, help="arrival rate of the served sweep (cases/s)")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 field comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -230,6 +243,9 @@ def main():
     case.close()
     if args.sweep_cases > 0:
         out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision)
+    if not args.no_precision:
+        print("[bench] FP32 vs FP64 fields ...", file=sys.stderr, flush=True)
+        out["fp32_vs_fp64"] = precision_leg()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, args.scale)
