@@ -1406,12 +1406,56 @@ __global__ void k_kn_snapshot(WorldView<R> w)
 
 constexpr int kNbCap = 192;   // 3 levels of at most 6 faces: 1 + 6 + 30 + 150 cells
 
+// Cells within three face-neighbour steps of c (internal faces only), c
+// first: uniGasHybridDecomposition::fetchCellNeighborhood
+// (uniGasHybridDecomposition.C:127-174).  The lists depend on the mesh only
+// and are built once per world into a CSR table.
+template <class R>
+__device__ inline int nb_bfs(const WorldView<R>& w, uint32_t c, uint32_t* list)
+{
+    int n = 0, lo = 0;
+    list[n++] = c;
+    for (int l = 0; l < 3; ++l) {
+        const int hi = n;
+        for (int i = lo; i < hi; ++i)
+            for (int j = 0; j < kMaxFaces; ++j) {
+                const size_t k = (size_t)list[i] * kMaxFaces + j;
+                if (w.fkind[k] != kFaceInternal) continue;
+                const uint32_t o = (uint32_t)w.geom.nbr[k];
+                bool seen = false;
+                for (int q = 0; q < n; ++q) if (list[q] == o) { seen = true; break; }
+                if (!seen && n < kNbCap) list[n++] = o;
+            }
+        lo = hi;
+    }
+    return n;
+}
+
+// fill == nullptr: counts into cnt[c]; else the list of c at fill + off[c].
+template <class R>
+__global__ void __launch_bounds__(64) k_nb_build(WorldView<R> w, uint32_t* cnt, const uint32_t* off, uint32_t* fill)
+{
+    uint32_t list[kNbCap];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
+        const CaseDev& cs = w.cases[w.cell_case[c]];
+        if (cs.coll_model != UGX_COLL_HYBRID) {
+            if (!fill) cnt[c] = 0u;
+            continue;
+        }
+        const int n = nb_bfs<R>(w, c, list);
+        if (!fill) {
+            cnt[c] = (uint32_t)n;
+        } else {
+            for (int q = 0; q < n; ++q) fill[off[c] + q] = list[q];
+        }
+    }
+}
+
 // One refinement loop (uniGasHybridDecomposition.C:73-75; localKnudsen.C:436-545)
 // for the cells of model `me` (1 DSMC, 0 BGK), deciding from ids2.
 template <class R>
-__global__ void __launch_bounds__(64) k_kn_refine(WorldView<R> w, int me)
+__global__ void __launch_bounds__(kBlock) k_kn_refine(WorldView<R> w, int me)
 {
-    uint32_t list[kNbCap];
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
         if (!kn_due(cs) || w.ids2[c] != me) continue;
@@ -1422,23 +1466,9 @@ __global__ void __launch_bounds__(64) k_kn_refine(WorldView<R> w, int me)
             if (w.ids2[w.geom.nbr[k]] == me) same++; else other++;
         }
         if (same == 0 || (same == 1 && other > 1)) { w.cmodel[c] = 1 - me; continue; }
-        int n = 0, lo = 0;
-        list[n++] = c;
-        for (int l = 0; l < 3; ++l) {
-            const int hi = n;
-            for (int i = lo; i < hi; ++i)
-                for (int j = 0; j < kMaxFaces; ++j) {
-                    const size_t k = (size_t)list[i] * kMaxFaces + j;
-                    if (w.fkind[k] != kFaceInternal) continue;
-                    const uint32_t o = (uint32_t)w.geom.nbr[k];
-                    bool seen = false;
-                    for (int q = 0; q < n; ++q) if (list[q] == o) { seen = true; break; }
-                    if (!seen && n < kNbCap) list[n++] = o;
-                }
-            lo = hi;
-        }
+        const uint32_t b = w.nb_off[c], n = w.nb_off[c + 1] - b;
         int cnt = 0;
-        for (int q = 0; q < n; ++q) if (w.ids2[list[q]] == me) cnt++;
+        for (uint32_t q = 0; q < n; ++q) cnt += (w.ids2[w.nb_list[b + q]] == me) ? 1 : 0;
         if ((double)cnt < 0.4 * (double)n) w.cmodel[c] = 1 - me;
     }
 }
@@ -1621,11 +1651,35 @@ static void launch_decompose(const WorldView<R>& w, hipStream_t s)
     for (int pass = 0; pass < 3; ++pass)
         for (int me = 1; me >= 0; --me) {
             hipLaunchKernelGGL(k_kn_snapshot<R>, dim3(g), dim3(kBlock), 0, s, w);
-            hipLaunchKernelGGL(k_kn_refine<R>, dim3(grid_for((uint64_t)w.nc * 4, 65536)), dim3(64), 0, s, w, me);
+            hipLaunchKernelGGL(k_kn_refine<R>, dim3(g), dim3(kBlock), 0, s, w, me);
         }
     hipLaunchKernelGGL(k_kn_reset<R>, dim3(g), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(k_kn_end<R>, dim3(grid_for((uint64_t)w.ncases)), dim3(kBlock), 0, s, w);
 }
+
+// Neighbourhood table of the hybrid refinement: counts, exclusive scan into
+// off (off[nc] = total, also copied to *total), then the lists.
+template <class R>
+void launch_nb_count(const WorldView<R>& w, uint32_t* cnt, uint32_t* off, uint32_t* bsum, uint32_t* total, hipStream_t s)
+{
+    const uint32_t ncp = (uint32_t)w.nc + 1u;
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, cnt, ncp);
+    hipLaunchKernelGGL(k_nb_build<R>, dim3(grid_for((uint64_t)w.nc * 4, 65536)), dim3(64), 0, s, w, cnt,
+                       (const uint32_t*)nullptr, (uint32_t*)nullptr);
+    scan_u32(cnt, ncp, off, bsum, total, s);
+}
+
+template <class R>
+void launch_nb_fill(const WorldView<R>& w, const uint32_t* off, uint32_t* list, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nb_build<R>, dim3(grid_for((uint64_t)w.nc * 4, 65536)), dim3(64), 0, s, w,
+                       (uint32_t*)nullptr, off, list);
+}
+
+template void launch_nb_count<float>(const WorldView<float>&, uint32_t*, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
+template void launch_nb_count<double>(const WorldView<double>&, uint32_t*, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
+template void launch_nb_fill<float>(const WorldView<float>&, const uint32_t*, uint32_t*, hipStream_t);
+template void launch_nb_fill<double>(const WorldView<double>&, const uint32_t*, uint32_t*, hipStream_t);
 
 template <class R, int Block, uint32_t TileN, uint32_t TileMax>
 static void launch_tiles_cfg(const WorldView<R>& w, hipStream_t s, bool sort_only)

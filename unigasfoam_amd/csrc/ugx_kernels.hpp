@@ -84,6 +84,8 @@ struct WorldView {
     double* kw;               // [8*nc] decomposition fields (ping)
     double* kw2;              // [8*nc] (pong)
     int32_t* ids2;            // [nc] model ids at the start of a refinement loop
+    const uint32_t* nb_off;   // [nc+1] neighbourhood table of the refinement (CSR)
+    const uint32_t* nb_list;
     int32_t has_bgk;          // some case runs a BGK-family model
     uint32_t* blocksum;       // scan scratch
     unsigned long long* counters; // [ncases*kCntNum]
@@ -122,6 +124,10 @@ template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int cc
                                     int exact, const double* bblo, const double* bbhi,
                                     uint32_t* fill_cnt, uint32_t* fill_off, int ns, int ncell_case);
 void launch_copy_probe(const void* in, void* out, size_t bytes, hipStream_t s);
+template <class R>
+void launch_nb_count(const WorldView<R>& w, uint32_t* cnt, uint32_t* off, uint32_t* bsum, uint32_t* total, hipStream_t s);
+template <class R>
+void launch_nb_fill(const WorldView<R>& w, const uint32_t* off, uint32_t* list, hipStream_t s);
 
 // Per-cell state of a world, for the carry-over of a case between worlds.
 struct CellState {

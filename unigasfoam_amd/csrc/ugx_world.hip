@@ -316,6 +316,7 @@ void World::release()
                      (void**)&d_samp, (void**)&d_uspq, (void**)&d_usps, (void**)&d_uspmax, (void**)&d_knacc,
                      (void**)&d_kn, (void**)&d_cc, (void**)&d_mass, (void**)&d_rotdof, (void**)&d_omega,
                      (void**)&d_diam, (void**)&d_knf, (void**)&d_fkind, (void**)&d_kw, (void**)&d_kw2, (void**)&d_ids2,
+                     (void**)&d_nb_off, (void**)&d_nb_list,
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
                      (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
@@ -436,6 +437,8 @@ WorldView<R> World::view() const
     w.kw = d_kw;
     w.kw2 = d_kw2;
     w.ids2 = d_ids2;
+    w.nb_off = d_nb_off;
+    w.nb_list = d_nb_list;
     w.pr = Pairs{d_pmr, d_pC, d_pe, d_pchi, d_palpha};
     w.cnt = d_cnt;
     w.off = d_off;
@@ -699,6 +702,33 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     cap = 0;
     uint64_t want = capacity > 0 ? capacity : 4096;
     if ((rc = alloc_particles((uint32_t)std::min<uint64_t>(want, 0xF0000000ull)))) return rc;
+    if (has_hybrid)
+        if ((rc = build_neighbourhoods())) return rc;
+    return UGX_OK;
+}
+
+int World::build_neighbourhoods()
+{
+    uint32_t* cnt = nullptr;
+    uint32_t* total = nullptr;
+    int rc;
+    if ((rc = dalloc(cnt, (size_t)nc + 1)) || (rc = dalloc(d_nb_off, (size_t)nc + 1)) || (rc = dalloc(total, 1))) {
+        dfree(cnt); dfree(total);
+        return rc;
+    }
+    if (precision == UGX_FP32_STATE) launch_nb_count<float>(view<float>(), cnt, d_nb_off, d_blocksum, total, stream);
+    else launch_nb_count<double>(view<double>(), cnt, d_nb_off, d_blocksum, total, stream);
+    uint32_t n = 0;
+    hipError_t e = hipMemcpyAsync(&n, total, sizeof n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    dfree(cnt);
+    dfree(total);
+    if (e != hipSuccess) return set_err(UGX_ERR_DEVICE, "neighbourhood count failed: %s", hipGetErrorString(e));
+    if ((rc = dalloc(d_nb_list, (size_t)std::max<uint32_t>(n, 1u)))) return rc;
+    if (precision == UGX_FP32_STATE) launch_nb_fill<float>(view<float>(), d_nb_off, d_nb_list, stream);
+    else launch_nb_fill<double>(view<double>(), d_nb_off, d_nb_list, stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
     return UGX_OK;
 }
 

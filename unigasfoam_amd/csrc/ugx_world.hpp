@@ -73,6 +73,9 @@ struct World {
     double* d_kw = nullptr;
     double* d_kw2 = nullptr;
     int32_t* d_ids2 = nullptr;
+    uint32_t* d_nb_off = nullptr;    // hybrid refinement neighbourhoods (CSR)
+    uint32_t* d_nb_list = nullptr;
+    int build_neighbourhoods();
     int32_t has_hybrid = 0;
     double* d_muref = nullptr;
     double* d_prterm = nullptr;
