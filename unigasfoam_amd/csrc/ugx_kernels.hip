@@ -577,7 +577,7 @@ __device__ inline double group8_total(double p)
 // cell.  L: 0 rows 0-7, 1 adds 8-13, 2 adds 14-19; species rows for
 // q < nsw.  S must start at +0.0.
 template <int L, class R, class Acc>
-__device__ inline void cell_partials(const WorldView<R>& w, int sp_base, int nsw, const Acc& A, uint32_t i0,
+__device__ inline void cell_partials(const WorldView<R>& w, int nsw, const Acc& A, uint32_t i0,
                                      uint32_t e, double S[kNSamp])
 {
     for (uint32_t i = i0; i < e; i += 8u) {
@@ -627,9 +627,21 @@ __device__ inline void cell_partials(const WorldView<R>& w, int sp_base, int nsw
 // 8g..8g+7 takes cell k), then the sampler (uniGasVolFields.C:761-797), the
 // hybrid accumulators and the BGK moments.  All 64 lanes of every wave
 // call it with the same arguments.
+// Fields of a case that the per-cell sums need.
+struct TileCase {
+    double dt;
+    uint32_t step;
+    int32_t sample_interval, coll_model, bgk_model, active, ns, sampler_full;
+};
+
+__device__ inline TileCase tile_case_of(const CaseDev& c)
+{
+    return TileCase{c.dt, c.step, c.sample_interval, c.coll_model, c.bgk_model, c.active, c.ns, c.sampler_full};
+}
+
 template <int L, int Block, class R, class Acc>
 __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nct, const uint32_t* s_off,
-                                 const Acc& A)
+                                 const Acc& A, const int32_t* s_cc, const TileCase* s_tc, int case0)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = tid >> 6, lane = tid & 63u, g = lane >> 3, j = lane & 7u;
@@ -638,17 +650,17 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         const uint32_t k = kb + wv * 8u + g;
         uint32_t b = 0, e = 0;
         bool sample = false, hybrid = false, bgk = false;
-        int sp_base = 0;
         double dt = 0.0;
         const uint32_t c = c0 + k;
+        TileCase cs{};
         if (k < nct) {
-            const CaseDev& cs = w.cases[w.cell_case[c]];
+            const int ci = s_cc[k];
+            cs = (ci == case0) ? *s_tc : tile_case_of(w.cases[ci]);
             if (cs.active) {
                 sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
                 hybrid = cs.coll_model == UGX_COLL_HYBRID;
                 bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
                 if (sample || hybrid || bgk) { b = s_off[k]; e = s_off[k + 1]; }
-                sp_base = cs.sp_base;
                 dt = cs.dt;
             }
         }
@@ -657,7 +669,7 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         double S[kNSamp];
 #pragma unroll
         for (int q = 0; q < kNSamp; ++q) S[q] = 0.0;
-        cell_partials<L, R, Acc>(w, sp_base, nsw, A, b + j, e, S);
+        cell_partials<L, R, Acc>(w, nsw, A, b + j, e, S);
 #pragma unroll
         for (int q = 0; q < kNSamp; ++q) {
             const bool used = (q < UGX_S_MUU) || (L >= 1 && q < UGX_S_MCCU) || (L >= 2 && q < UGX_S_BASE)
@@ -670,7 +682,6 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         }
         if (need && j == 0u) {
             if (sample) {
-                const CaseDev& cs = w.cases[w.cell_case[c]];
                 const bool full = cs.sampler_full != 0;
                 double* sc = &w.samp[(size_t)c * kNSamp];
 #pragma unroll
@@ -710,7 +721,13 @@ __global__ void k_tile_table(WorldView<R> w)
             hi += 1ull;
             w.st->ntiles = (uint32_t)hi;
         }
-        for (uint64_t t = lo; t <= hi; ++t) w.tile_first[t] = c;
+        const uint32_t p = w.off[c];
+        const int32_t ci = (c < nc) ? w.cell_case[c] : 0;
+        for (uint64_t t = lo; t <= hi; ++t) {
+            w.tile_first[t] = c;
+            w.tile_p0[t] = p;
+            w.tile_case[t] = ci;
+        }
     }
 }
 
@@ -737,15 +754,34 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ uint8_t s_sl[kM];
     __shared__ double s_m[kM];
     __shared__ uint8_t s_rd[kM];
+    __shared__ int32_t s_cc[SortOnly ? 1 : TileN / kCellW + 2];
+    __shared__ TileCase s_tc;
+    __shared__ double s_spm[SortOnly ? 1 : kLdsSpecies];
+    __shared__ uint8_t s_spr[SortOnly ? 1 : kLdsSpecies], s_spl[SortOnly ? 1 : kLdsSpecies];
     const uint32_t t = blockIdx.x;
     if (t >= w.st->ntiles) return;
     const uint32_t tid = threadIdx.x;
+    // the tile table gives cells, slots and the first case in one round trip ...
     const uint32_t c0 = w.tile_first[t];
     const uint32_t nct = w.tile_first[t + 1] - c0;
-    for (uint32_t k = tid; k <= nct; k += Block) s_off[k] = w.off[c0 + k];
-    __syncthreads();
-    const uint32_t p0 = s_off[0];
-    const uint32_t size = s_off[nct] - p0;
+    const uint32_t p0 = w.tile_p0[t];
+    const uint32_t size = w.tile_p0[t + 1] - p0;
+    const int case0 = w.tile_case[t];
+    const bool lds_species = w.nsp <= (int)kLdsSpecies;
+    // ... and everything else a tile reads before its gather comes in one more
+    for (uint32_t k = tid; k <= nct; k += Block) {
+        s_off[k] = w.off[c0 + k];
+        if (!SortOnly && k < nct) s_cc[k] = w.cell_case[c0 + k];
+    }
+    if (!SortOnly) {
+        if (tid == 0) s_tc = tile_case_of(w.cases[case0]);
+        if (lds_species)
+            for (int g = (int)tid; g < w.nsp; g += Block) {
+                s_spm[g] = w.sp.mass[g];
+                s_spr[g] = (uint8_t)w.sp.rotdof[g];
+                s_spl[g] = (uint8_t)w.sp.local[g];
+            }
+    }
     if (size <= TileMax) {
         for (uint32_t i = tid; i < size; i += Block) s_in[i] = w.perm[p0 + i];
         __syncthreads();
@@ -776,16 +812,23 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4)))
             w.b.sp[j] = g;
             if (!SortOnly) {
                 s_vx[i] = vx; s_vy[i] = vy; s_vz[i] = vz; s_er[i] = er;
-                s_sl[i] = (uint8_t)w.sp.local[g];
-                s_m[i] = w.sp.mass[g];
-                s_rd[i] = (uint8_t)w.sp.rotdof[g];
+                if (lds_species) {
+                    s_sl[i] = s_spl[g];
+                    s_m[i] = s_spm[g];
+                    s_rd[i] = s_spr[g];
+                } else {
+                    s_sl[i] = (uint8_t)w.sp.local[g];
+                    s_m[i] = w.sp.mass[g];
+                    s_rd[i] = (uint8_t)w.sp.rotdof[g];
+                }
             }
         }
         if (SortOnly) return;
         __syncthreads();
         LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_m, s_rd, p0};
-        tile_sums<L, Block, R>(w, c0, nct, s_off, A);
+        tile_sums<L, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
     } else {
+        __syncthreads();
         // Tile larger than LDS: the same steps in HBM.  The rank of an entry
         // in its cell is counted against the cell's slot list staged through
         // LDS in chunks; the ordered slots go to scratch.
@@ -822,7 +865,7 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4)))
         __threadfence_block();
         __syncthreads();
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        tile_sums<L, Block, R>(w, c0, nct, s_off, A);
+        tile_sums<L, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
     }
 }
 

@@ -66,6 +66,9 @@ struct WorldView {
     uint32_t* ins_off;        // [n_inflow+1]
     uint32_t* scratch;        // [cap] rank scratch of tiles larger than LDS
     uint32_t* tile_first;     // [max tiles + 2] first cell of each tile
+    uint32_t* tile_p0;        //                 its first sorted slot
+    int32_t* tile_case;       //                 the case of its first cell
+    int32_t nsp;              // species of all cases
     double* mom;              // [nc][kNMom] per-cell sums for the BGK family
     BgkP* bgkp;               // [nc]
     unsigned long long* raise;// [nc] USP maxProb raise (bits of a positive double)
@@ -113,6 +116,7 @@ constexpr int kNMom = UGX_S_SPECIES_MCC;
 constexpr uint32_t kCellW = 8;
 constexpr uint32_t kMinTileN = 128;
 constexpr int kTileVariants = 3;
+constexpr uint32_t kLdsSpecies = 64;   // species tables up to this size are staged in LDS per tile
 inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc) / kMinTileN + 2; }
 
 // Host launchers (ugx_kernels.hip).

@@ -320,7 +320,8 @@ void World::release()
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
                      (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
-                     (void**)&d_tile_first, (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
+                     (void**)&d_tile_first, (void**)&d_tile_p0, (void**)&d_tile_case,
+                     (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
                      (void**)&d_sel_slot, (void**)&d_sel_cell, (void**)&d_sel_k,
                      (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st,
                      (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau};
@@ -361,6 +362,8 @@ int World::alloc_particles(uint32_t newcap)
     dfree(d_perm);
     dfree(d_scratch);
     dfree(d_tile_first);
+    dfree(d_tile_p0);
+    dfree(d_tile_case);
     dfree(d_sel_slot);
     dfree(d_sel_cell);
     dfree(d_sel_k);
@@ -370,6 +373,8 @@ int World::alloc_particles(uint32_t newcap)
     if (int rc = dalloc(d_perm, newcap)) return rc;
     if (int rc = dalloc(d_scratch, newcap)) return rc;
     if (int rc = dalloc(d_tile_first, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
+    if (int rc = dalloc(d_tile_p0, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
+    if (int rc = dalloc(d_tile_case, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
     if (has_bgk) {
         if (int rc = dalloc(d_sel_slot, newcap)) return rc;
         if (int rc = dalloc(d_sel_cell, newcap)) return rc;
@@ -447,6 +452,9 @@ WorldView<R> World::view() const
     w.ins_off = d_ins_off;
     w.scratch = d_scratch;
     w.tile_first = d_tile_first;
+    w.tile_p0 = d_tile_p0;
+    w.tile_case = d_tile_case;
+    w.nsp = nsp_total;
     w.mom = d_mom;
     w.bgkp = d_bgkp;
     w.raise = d_raise;
@@ -481,6 +489,7 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     nc = 0;
     int nsp = 0, npair = 0, nbm = 0;
     for (auto& s : specs) { nc += s.geom.nc; nsp += s.ns; npair += s.ns * s.ns; nbm += (int)s.geom.bm.size(); }
+    nsp_total = nsp;
     std::vector<CaseDev> hc;
     std::vector<int32_t> cell_case(nc), nbr((size_t)nc * kMaxFaces), cmodel(nc), local(nsp), rotdof(nsp);
     std::vector<double> vol(nc), cc((size_t)nc * 3), plane((size_t)nc * kMaxFaces * 4), delta((size_t)nc * kMaxFaces * 4, 0.0);

@@ -88,6 +88,9 @@ struct World {
     uint32_t* d_ins_off = nullptr;
     uint32_t* d_scratch = nullptr;
     uint32_t* d_tile_first = nullptr;
+    uint32_t* d_tile_p0 = nullptr;
+    int32_t* d_tile_case = nullptr;
+    int32_t nsp_total = 0;
     double* d_mom = nullptr;
     BgkP* d_bgkp = nullptr;
     unsigned long long* d_raise = nullptr;
