@@ -301,22 +301,36 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
     TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
     const uint32_t n0 = w.st->n;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
-        if (base >= n0) break;
-        uint32_t t = base + threadIdx.x;
-        bool active = t < n0;
-        uint32_t cell = kDead;
+    // software pipeline: the record of the next slot is loaded while the
+    // current one is tracked
+    uint32_t ncell = kDead;
+    R nx[3] = {(R)0, (R)0, (R)0}, nu[3] = {(R)0, (R)0, (R)0};
+    {
+        const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t0 < n0) {
+            ncell = w.a.cell[t0];
+            nx[0] = w.a.x[t0]; nx[1] = w.a.y[t0]; nx[2] = w.a.z[t0];
+            nu[0] = w.a.vx[t0]; nu[1] = w.a.vy[t0]; nu[2] = w.a.vz[t0];
+        }
+    }
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n0; base += stride) {
+        const uint32_t t = base + threadIdx.x;
+        const bool active = t < n0;
+        uint32_t cell = ncell;
+        R x[3] = {nx[0], nx[1], nx[2]};
+        R U[3] = {nu[0], nu[1], nu[2]};
+        const uint32_t tn = t + stride;
+        if (tn < n0) {
+            ncell = w.a.cell[tn];
+            nx[0] = w.a.x[tn]; nx[1] = w.a.y[tn]; nx[2] = w.a.z[tn];
+            nu[0] = w.a.vx[tn]; nu[1] = w.a.vy[tn]; nu[2] = w.a.vz[tn];
+        }
         int ccase = 0;
         TrackOut to{1, 0, 0, 0, 0, 0};
-        if (active) {
-            cell = w.a.cell[t];
-            ccase = case_of_slot(w, t);
-        }
+        if (active) ccase = case_of_slot(w, t);
         // simulators of a paused case (a retired batch case) stay where they are
         if (active && w.cases[ccase].active) {
             const CaseDev& cs = w.cases[ccase];
-            R x[3] = {w.a.x[t], w.a.y[t], w.a.z[t]};
-            R U[3] = {w.a.vx[t], w.a.vy[t], w.a.vz[t]};
             double erot = 0.0;
             Rng r;
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
@@ -576,12 +590,21 @@ __device__ inline double group8_total(double p)
 // (cellMeasurements.C:408-513) for the simulators [i0, e) step 8 of one
 // cell.  L: 0 rows 0-7, 1 adds 8-13, 2 adds 14-19; species rows for
 // q < nsw.  S must start at +0.0.
-template <int L, class R, class Acc>
-__device__ inline void cell_partials(const WorldView<R>& w, int nsw, const Acc& A, uint32_t i0,
-                                     uint32_t e, double S[kNSamp])
+// Accumulators of the per-cell sums: the base rows of sum level L (rows
+// 0-7, 0-13 or 0-19), then, for multi-species worlds (MS), the per-species
+// counts and M|v|^2 of every species slot.
+template <int L, bool MS>
+struct SumRows {
+    static constexpr int kBase = (L == 0) ? 8 : (L == 1) ? 14 : 20;
+    static constexpr int kN = kBase + (MS ? 2 * kNS : 0);
+};
+
+template <int L, bool MS, class R, class Acc>
+__device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32_t i0, uint32_t e,
+                                     double (&S)[SumRows<L, MS>::kN])
 {
+    constexpr int B = SumRows<L, MS>::kBase;
     for (uint32_t i = i0; i < e; i += 8u) {
-        const int s = A.local_species(i);
         const double m = A.mass(i, w.sp);
         double vx, vy, vz;
         A.load(i, vx, vy, vz);
@@ -595,7 +618,7 @@ __device__ inline void cell_partials(const WorldView<R>& w, int nsw, const Acc& 
         S[UGX_S_MCC] = S[UGX_S_MCC] + mcc;
         S[UGX_S_EROT] = S[UGX_S_EROT] + er;
         S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + A.rotdof(i, w.sp);
-        if (L >= 1) {
+        if constexpr (L >= 1) {
             S[UGX_S_MUU] = S[UGX_S_MUU] + m * (vx * vx);
             S[UGX_S_MUV] = S[UGX_S_MUV] + m * (vx * vy);
             S[UGX_S_MUW] = S[UGX_S_MUW] + m * (vx * vz);
@@ -603,7 +626,7 @@ __device__ inline void cell_partials(const WorldView<R>& w, int nsw, const Acc& 
             S[UGX_S_MVW] = S[UGX_S_MVW] + m * (vy * vz);
             S[UGX_S_MWW] = S[UGX_S_MWW] + m * (vz * vz);
         }
-        if (L >= 2) {
+        if constexpr (L >= 2) {
             S[UGX_S_MCCU] = S[UGX_S_MCCU] + mcc * vx;
             S[UGX_S_MCCV] = S[UGX_S_MCCV] + mcc * vy;
             S[UGX_S_MCCW] = S[UGX_S_MCCW] + mcc * vz;
@@ -611,22 +634,17 @@ __device__ inline void cell_partials(const WorldView<R>& w, int nsw, const Acc& 
             S[UGX_S_EROTV] = S[UGX_S_EROTV] + er * vy;
             S[UGX_S_EROTW] = S[UGX_S_EROTW] + er * vz;
         }
-        if (nsw > 1) {
+        if constexpr (MS) {
+            const int s = A.local_species(i);
 #pragma unroll
             for (int q = 0; q < kNS; ++q) {
-                if (q < nsw) {
-                    S[UGX_S_SPECIES_N + q] = S[UGX_S_SPECIES_N + q] + ((s == q) ? 1.0 : 0.0);
-                    S[UGX_S_SPECIES_MCC + q] = S[UGX_S_SPECIES_MCC + q] + ((s == q) ? mcc : 0.0);
-                }
+                S[B + q] = S[B + q] + ((s == q) ? 1.0 : 0.0);
+                S[B + kNS + q] = S[B + kNS + q] + ((s == q) ? mcc : 0.0);
             }
         }
     }
 }
 
-// Sums of the cells of one tile, eight cells per wave (group g of lanes
-// 8g..8g+7 takes cell k), then the sampler (uniGasVolFields.C:761-797), the
-// hybrid accumulators and the BGK moments.  All 64 lanes of every wave
-// call it with the same arguments.
 // Fields of a case that the per-cell sums need.
 struct TileCase {
     double dt;
@@ -639,13 +657,18 @@ __device__ inline TileCase tile_case_of(const CaseDev& c)
     return TileCase{c.dt, c.step, c.sample_interval, c.coll_model, c.bgk_model, c.active, c.ns, c.sampler_full};
 }
 
-template <int L, int Block, class R, class Acc>
+// Sums of the cells of one tile, eight cells per wave (group g of lanes
+// 8g..8g+7 takes cell k), then the sampler (uniGasVolFields.C:761-797), the
+// hybrid time sums and the BGK moments.  All 64 lanes of every wave call it
+// with the same arguments.  Single-species worlds take the species rows
+// from rows N and M|v|^2 (the same sums).
+template <int L, bool MS, int Block, class R, class Acc>
 __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nct, const uint32_t* s_off,
                                  const Acc& A, const int32_t* s_cc, const TileCase* s_tc, int case0)
 {
+    constexpr int B = SumRows<L, MS>::kBase, NR = SumRows<L, MS>::kN;
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = tid >> 6, lane = tid & 63u, g = lane >> 3, j = lane & 7u;
-    const int nsw = w.ns_max;
     for (uint32_t kb = 0; kb < nct; kb += (Block / 64) * 8) {
         const uint32_t k = kb + wv * 8u + g;
         uint32_t b = 0, e = 0;
@@ -666,40 +689,51 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         }
         const bool need = sample || hybrid || bgk;
         if (__ballot(need) == 0ull) continue;
-        double S[kNSamp];
+        double S[NR];
 #pragma unroll
-        for (int q = 0; q < kNSamp; ++q) S[q] = 0.0;
-        cell_partials<L, R, Acc>(w, nsw, A, b + j, e, S);
+        for (int q = 0; q < NR; ++q) S[q] = 0.0;
+        cell_partials<L, MS, R, Acc>(w, A, b + j, e, S);
 #pragma unroll
-        for (int q = 0; q < kNSamp; ++q) {
-            const bool used = (q < UGX_S_MUU) || (L >= 1 && q < UGX_S_MCCU) || (L >= 2 && q < UGX_S_BASE)
-                              || (nsw > 1 && q >= UGX_S_BASE);
-            if (used) S[q] = group8_total(S[q]);
-        }
-        if (nsw <= 1) {
-            S[UGX_S_SPECIES_N] = S[UGX_S_N];
-            S[UGX_S_SPECIES_MCC] = S[UGX_S_MCC];
-        }
+        for (int q = 0; q < NR; ++q) S[q] = group8_total(S[q]);
         if (need && j == 0u) {
             if (sample) {
                 const bool full = cs.sampler_full != 0;
                 double* sc = &w.samp[(size_t)c * kNSamp];
 #pragma unroll
-                for (int q = 0; q < kNSamp; ++q) {
-                    // rows of species the case does not have stay zero
-                    const bool used = (q < UGX_S_BASE) ? (full || q < UGX_S_MUU) : (((q - UGX_S_BASE) % kNS) < cs.ns);
-                    if (used) sc[q] += dt * S[q];
+                for (int q = 0; q < B; ++q)
+                    if (full || q < UGX_S_MUU) sc[q] += dt * S[q];
+                if constexpr (MS) {
+#pragma unroll
+                    for (int q = 0; q < kNS; ++q) {
+                        // rows of species the case does not have stay zero
+                        if (q < cs.ns) {
+                            sc[UGX_S_SPECIES_N + q] += dt * S[B + q];
+                            sc[UGX_S_SPECIES_MCC + q] += dt * S[B + kNS + q];
+                        }
+                    }
+                } else {
+                    sc[UGX_S_SPECIES_N] += dt * S[UGX_S_N];
+                    sc[UGX_S_SPECIES_MCC] += dt * S[UGX_S_MCC];
                 }
             }
             if (hybrid) {
+                double* ka = &w.knacc[(size_t)kKnRows * c];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) w.knacc[(size_t)kKnRows * c + q] += dt * S[q];
+                for (int q = 0; q < 8; ++q) ka[q] += dt * S[q];
+                if constexpr (MS) {
 #pragma unroll
-                for (int q = 0; q < kNS; ++q) w.knacc[(size_t)kKnRows * c + 8 + q] += dt * S[UGX_S_SPECIES_N + q];
+                    for (int q = 0; q < kNS; ++q) ka[8 + q] += dt * S[B + q];
+                } else {
+                    ka[8] += dt * S[UGX_S_N];
+                }
             }
             if (bgk) {
+                double* mo = &w.mom[(size_t)c * kNMom];
 #pragma unroll
-                for (int q = 0; q < kNMom; ++q) w.mom[(size_t)c * kNMom + q] = S[q];
+                for (int q = 0; q < UGX_S_BASE; ++q) mo[q] = (q < B) ? S[q < B ? q : 0] : 0.0;
+#pragma unroll
+                for (int q = 0; q < kNS; ++q)
+                    mo[UGX_S_SPECIES_N + q] = MS ? S[MS ? B + q : 0] : ((q == 0) ? S[UGX_S_N] : 0.0);
             }
         }
     }
@@ -742,8 +776,8 @@ __global__ void k_tile_table(WorldView<R> w)
 //  3. per-cell sums from LDS: sampler, hybrid accumulators, BGK moments.
 // SortOnly stops after 2.  Tiles larger than TileMax run the same steps
 // from HBM.
-template <class R, bool SortOnly, int L, int Block, uint32_t TileN, uint32_t TileMax>
-__global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4))) k_tile(WorldView<R> w)
+template <class R, bool SortOnly, int L, bool MS, int Block, uint32_t TileN, uint32_t TileMax>
+__global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
 {
     constexpr uint32_t kM = SortOnly ? 1u : TileMax;
     __shared__ uint32_t s_off[TileN / kCellW + 2];
@@ -826,7 +860,7 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4)))
         if (SortOnly) return;
         __syncthreads();
         LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_m, s_rd, p0};
-        tile_sums<L, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
+        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
     } else {
         __syncthreads();
         // Tile larger than LDS: the same steps in HBM.  The rank of an entry
@@ -865,7 +899,7 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(4)))
         __threadfence_block();
         __syncthreads();
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        tile_sums<L, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
+        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
     }
 }
 
@@ -1730,14 +1764,13 @@ static void launch_tiles_cfg(const WorldView<R>& w, hipStream_t s, bool sort_onl
     const uint32_t ncp = (uint32_t)w.nc + 1u;
     const unsigned nt = (unsigned)((w.cap + (uint64_t)kCellW * (uint64_t)w.nc) / TileN + 2);
     hipLaunchKernelGGL((k_tile_table<R, TileN>), dim3(grid_for(ncp)), dim3(kBlock), 0, s, w);
-    if (sort_only)
-        hipLaunchKernelGGL((k_tile<R, true, 0, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
-    else if (w.sum_level >= 2)
-        hipLaunchKernelGGL((k_tile<R, false, 2, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
-    else if (w.sum_level == 1)
-        hipLaunchKernelGGL((k_tile<R, false, 1, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
-    else
-        hipLaunchKernelGGL((k_tile<R, false, 0, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w);
+    const bool ms = w.ns_max > 1;
+#define UGX_TILE(SO, L, MS) hipLaunchKernelGGL((k_tile<R, SO, L, MS, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w)
+    if (sort_only) UGX_TILE(true, 0, false);
+    else if (w.sum_level >= 2) { if (ms) UGX_TILE(false, 2, true); else UGX_TILE(false, 2, false); }
+    else if (w.sum_level == 1) { if (ms) UGX_TILE(false, 1, true); else UGX_TILE(false, 1, false); }
+    else { if (ms) UGX_TILE(false, 0, true); else UGX_TILE(false, 0, false); }
+#undef UGX_TILE
 }
 
 // Cell-tile configurations (threads, key window, LDS simulators); the
