@@ -523,19 +523,32 @@ __global__ void k_scatter(WorldView<R> w)
 template <class R>
 struct LdsAcc {
     const R *vx, *vy, *vz, *er;
-    const uint8_t* sl;
-    const double* m;
-    const uint8_t* rd;
+    const uint16_t* g;               // species id of each entry
+    const double* spm;               // species mass, rotational DoF and case-local
+    const uint8_t *spr, *spl;        // index, staged in LDS when the table fits
+    bool lds;
     uint32_t p0;
-    __device__ double mass(uint32_t i, const Species&) const { return m[i - p0]; }
-    __device__ double rotdof(uint32_t i, const Species&) const { return (double)rd[i - p0]; }
+    __device__ double mass(uint32_t i, const Species& sp) const
+    {
+        const uint32_t s = g[i - p0];
+        return lds ? spm[s] : sp.mass[s];
+    }
+    __device__ double rotdof(uint32_t i, const Species& sp) const
+    {
+        const uint32_t s = g[i - p0];
+        return lds ? (double)spr[s] : (double)sp.rotdof[s];
+    }
     __device__ void load(uint32_t i, double& a, double& b, double& c) const
     {
         const uint32_t k = i - p0;
         a = (double)vx[k]; b = (double)vy[k]; c = (double)vz[k];
     }
     __device__ double erot(uint32_t i) const { return (double)er[i - p0]; }
-    __device__ int local_species(uint32_t i) const { return sl[i - p0]; }
+    __device__ int local_species(uint32_t i, const Species& sp) const
+    {
+        const uint32_t s = g[i - p0];
+        return lds ? (int)spl[s] : sp.local[s];
+    }
 };
 
 template <class R>
@@ -554,6 +567,7 @@ struct GlobAcc {
     __device__ double erot(uint32_t i) const { return (double)p.erot[i]; }
     __device__ void set_erot(uint32_t i, double e) const { p.erot[i] = (R)e; }
     __device__ int local_species(uint32_t i) const { return local[p.sp[i]]; }
+    __device__ int local_species(uint32_t i, const Species&) const { return local[p.sp[i]]; }
     __device__ double mass(uint32_t i, const Species& sp) const { return sp.mass[p.sp[i]]; }
     __device__ double rotdof(uint32_t i, const Species& sp) const { return (double)sp.rotdof[p.sp[i]]; }
     __device__ uint32_t idx_get(uint32_t b, uint32_t k) const { return idx[b + k]; }
@@ -635,7 +649,7 @@ __device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32
             S[UGX_S_EROTW] = S[UGX_S_EROTW] + er * vz;
         }
         if constexpr (MS) {
-            const int s = A.local_species(i);
+            const int s = A.local_species(i, w.sp);
 #pragma unroll
             for (int q = 0; q < kNS; ++q) {
                 S[B + q] = S[B + q] + ((s == q) ? 1.0 : 0.0);
@@ -784,14 +798,16 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
     __shared__ uint32_t s_in[TileMax];
     __shared__ uint32_t s_perm[TileMax];
     __shared__ uint8_t s_k[TileMax];
-    __shared__ R s_vx[kM], s_vy[kM], s_vz[kM], s_er[kM];
-    __shared__ uint8_t s_sl[kM];
-    __shared__ double s_m[kM];
-    __shared__ uint8_t s_rd[kM];
+    // FP32 x-velocities take the slot list's place once the ranks are known
+    constexpr bool kVxInSlots = sizeof(R) == sizeof(uint32_t);
+    __shared__ R s_vx_own[(SortOnly || kVxInSlots) ? 1 : kM];
+    __shared__ R s_vy[kM], s_vz[kM], s_er[kM];
+    __shared__ uint16_t s_g[kM];
     __shared__ int32_t s_cc[SortOnly ? 1 : TileN / kCellW + 2];
     __shared__ TileCase s_tc;
     __shared__ double s_spm[SortOnly ? 1 : kLdsSpecies];
     __shared__ uint8_t s_spr[SortOnly ? 1 : kLdsSpecies], s_spl[SortOnly ? 1 : kLdsSpecies];
+    R* const s_vx = kVxInSlots ? reinterpret_cast<R*>(s_in) : s_vx_own;
     const uint32_t t = blockIdx.x;
     if (t >= w.st->ntiles) return;
     const uint32_t tid = threadIdx.x;
@@ -846,20 +862,12 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
             w.b.sp[j] = g;
             if (!SortOnly) {
                 s_vx[i] = vx; s_vy[i] = vy; s_vz[i] = vz; s_er[i] = er;
-                if (lds_species) {
-                    s_sl[i] = s_spl[g];
-                    s_m[i] = s_spm[g];
-                    s_rd[i] = s_spr[g];
-                } else {
-                    s_sl[i] = (uint8_t)w.sp.local[g];
-                    s_m[i] = w.sp.mass[g];
-                    s_rd[i] = (uint8_t)w.sp.rotdof[g];
-                }
+                s_g[i] = (uint16_t)g;
             }
         }
         if (SortOnly) return;
         __syncthreads();
-        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_sl, s_m, s_rd, p0};
+        LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_g, s_spm, s_spr, s_spl, lds_species, p0};
         tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
     } else {
         __syncthreads();

@@ -489,6 +489,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     nc = 0;
     int nsp = 0, npair = 0, nbm = 0;
     for (auto& s : specs) { nc += s.geom.nc; nsp += s.ns; npair += s.ns * s.ns; nbm += (int)s.geom.bm.size(); }
+    // cell tiles keep a 16-bit species id per simulator
+    if (nsp > 65535) return UGX_ERR_CAPACITY;
     nsp_total = nsp;
     std::vector<CaseDev> hc;
     std::vector<int32_t> cell_case(nc), nbr((size_t)nc * kMaxFaces), cmodel(nc), local(nsp), rotdof(nsp);
