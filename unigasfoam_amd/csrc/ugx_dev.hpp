@@ -450,15 +450,21 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
         int fmin = -1;
         int32_t code = 0;
         R qx = (R)0, qy = (R)0, qz = (R)0;
+        // every face is evaluated and the nearest one selected without
+        // branches: a face counts when it exists and the simulator moves
+        // towards it; ties keep the first face
 #pragma unroll
         for (int f = 0; f < kMaxFaces; ++f) {
-            if (nb[f] == kNbrEmpty) continue;
-            R dn = q[f].x * vt[0] + q[f].y * vt[1] + q[f].z * vt[2];
-            if (dn > (R)0) {
-                R t = (q[f].w - (q[f].x * x[0] + q[f].y * x[1] + q[f].z * x[2])) / dn;
-                if (t < (R)0) t = (R)0;
-                if (t < tmin) { tmin = t; fmin = f; code = nb[f]; qx = q[f].x; qy = q[f].y; qz = q[f].z; }
-            }
+            const R dn = q[f].x * vt[0] + q[f].y * vt[1] + q[f].z * vt[2];
+            R t = (q[f].w - (q[f].x * x[0] + q[f].y * x[1] + q[f].z * x[2])) / dn;
+            t = (t < (R)0) ? (R)0 : t;
+            const bool take = (nb[f] != kNbrEmpty) && (dn > (R)0) && (t < tmin);
+            tmin = take ? t : tmin;
+            fmin = take ? f : fmin;
+            code = take ? nb[f] : code;
+            qx = take ? q[f].x : qx;
+            qy = take ? q[f].y : qy;
+            qz = take ? q[f].z : qz;
         }
         if (fmin < 0) {
             for (int d = 0; d < 3; ++d) x[d] = x[d] + vt[d] * tau;

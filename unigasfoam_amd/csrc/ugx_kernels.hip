@@ -26,25 +26,38 @@ namespace ugx {
 constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;      // elements per scan block (256 threads x 4)
 
-__device__ inline uint32_t wave_agg_inc(uint32_t* ctr, uint32_t key, bool active)
+// Adds one to ctr[key[u]] for every active lane and key set u and returns
+// each lane's rank among the lanes of the wave that added to the same
+// counter, offset by the counter's previous value.  One atomic per distinct
+// key of the wave and key set; every atomic of all U sets is issued before
+// any result is used, so the wave waits for one round trip in total.
+// Called by every lane of the wave.
+template <int U>
+__device__ inline void wave_agg_inc(uint32_t* ctr, const uint32_t (&key)[U], const bool (&active)[U],
+                                    uint32_t (&rank)[U])
 {
-    // One atomic per distinct key of the wave; every atomic is issued before
-    // any result is used, so the wave waits for one round trip in total.
     const unsigned lane = __lane_id();
-    uint32_t lane_base = 0, my_leader = 0, my_rank = 0;
-    unsigned long long pending = __ballot(active);
-    while (pending) {
-        const int leader = __ffsll(pending) - 1;
-        const uint32_t lkey = __shfl(key, leader);
-        const unsigned long long grp = __ballot(active && key == lkey) & pending;
-        if ((int)lane == leader) lane_base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(grp));
-        if ((grp >> lane) & 1ull) {
-            my_leader = (uint32_t)leader;
-            my_rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+    uint32_t lane_base[U], my_leader[U], my_rank[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        lane_base[u] = 0u;
+        my_leader[u] = 0u;
+        my_rank[u] = 0u;
+        unsigned long long pending = __ballot(active[u]);
+        while (pending) {
+            const int leader = __ffsll(pending) - 1;
+            const uint32_t lkey = __shfl(key[u], leader);
+            const unsigned long long grp = __ballot(active[u] && key[u] == lkey) & pending;
+            if ((int)lane == leader) lane_base[u] = atomicAdd(&ctr[lkey], (uint32_t)__popcll(grp));
+            if ((grp >> lane) & 1ull) {
+                my_leader[u] = (uint32_t)leader;
+                my_rank[u] = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+            }
+            pending &= ~grp;
         }
-        pending &= ~grp;
     }
-    return __shfl(lane_base, (int)my_leader) + my_rank;
+#pragma unroll
+    for (int u = 0; u < U; ++u) rank[u] = __shfl(lane_base[u], (int)my_leader[u]) + my_rank[u];
 }
 
 // Adds v to counters[ccase][idx]; one atomic per wave when the wave's
@@ -264,11 +277,9 @@ __global__ void k_inflow_gen(WorldView<R> w)
             w.a.cell[j] = cell;
             w.a.sp[j] = (uint32_t)e.species;
         }
-        bool alive = active && cell != kDead;
-        wave_agg_inc(w.cnt, alive ? cell : 0u, alive);
         if (active) {
             tcount_add(tc, w.counters, ccase, 3, 1ull);
-            tcount_add(tc, w.counters, ccase, 0, alive ? 0ull : 1ull);
+            tcount_add(tc, w.counters, ccase, 0, (cell != kDead) ? 0ull : 1ull);
             tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
             tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
         }
@@ -291,73 +302,244 @@ __device__ inline int case_of_slot(const WorldView<R>& w, uint32_t t)
     return lo;
 }
 
-// Ballistic move of the existing simulators.  Specular walls, cyclic and
-// deletion faces are handled in line; a simulator reaching a diffuse wall
-// stops there and is queued for k_wall, which keeps this kernel free of the
-// FP64 transcendental code of the wall model.
+// Case fields the move needs.
+struct MoveCase {
+    double dt, mask[3];
+    int32_t active;
+};
+
+template <class R>
+__device__ inline MoveCase move_case(const WorldView<R>& w, int c)
+{
+    const CaseDev& cs = w.cases[c];
+    return MoveCase{cs.dt, {cs.mask[0], cs.mask[1], cs.mask[2]}, cs.active};
+}
+
+// Slots of one refill chunk: one per lane of a wave.
+constexpr uint32_t kMoveChunk = 64;
+
+// One chunk of simulator records, lane k holding slot base + k.
+template <class R>
+struct MoveRec {
+    R x, y, z, u, v, w;
+    uint32_t cell;
+};
+
+template <class R>
+__device__ inline void load_chunk(const WorldView<R>& w, uint64_t base, uint32_t n0, uint32_t lane, MoveRec<R>& r)
+{
+    const uint64_t s = base + lane;
+    if (s < n0) {
+        r.cell = w.a.cell[s];
+        r.x = w.a.x[s]; r.y = w.a.y[s]; r.z = w.a.z[s];
+        r.u = w.a.vx[s]; r.v = w.a.vy[s]; r.w = w.a.vz[s];
+    }
+}
+
+template <class R>
+__device__ inline MoveRec<R> shfl_rec(const MoveRec<R>& r, int src)
+{
+    MoveRec<R> g;
+    g.x = __shfl(r.x, src); g.y = __shfl(r.y, src); g.z = __shfl(r.z, src);
+    g.u = __shfl(r.u, src); g.v = __shfl(r.v, src); g.w = __shfl(r.w, src);
+    g.cell = __shfl(r.cell, src);
+    return g;
+}
+
+__device__ inline uint32_t chunk_valid(uint64_t base, uint32_t n0)
+{
+    return (base < n0) ? (uint32_t)min<uint64_t>(kMoveChunk, n0 - base) : 0u;
+}
+
+// Ballistic move of the existing simulators (the face-crossing loop of
+// track_t<true> restated lane by lane).  Specular walls, cyclic and deletion
+// faces are handled in line; a simulator reaching a diffuse wall stops there
+// and is queued for k_wall, which keeps this kernel free of the FP64
+// transcendental code of the wall model.
+//
+// One loop pass advances every lane by one face crossing.  A lane whose
+// simulator has finished takes the next slot of its wave at once, so a wave
+// is not held by its slowest simulator.  A wave works through chunks of
+// kMoveChunk slots, interleaved over the waves of the grid so that regions of
+// fast flow are spread over all of them.  The records of the current chunk
+// and of the next are held in registers (lane k holds slot base + k), so a
+// refill is a cross-lane exchange rather than a memory round trip; the next
+// chunk is loaded when the current one is used up.  The arithmetic of each
+// simulator is exactly that of track_t<true>.
 template <class R>
 __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
 {
     TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
     const uint32_t n0 = w.st->n;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    // software pipeline: the record of the next slot is loaded while the
-    // current one is tracked
-    uint32_t ncell = kDead;
-    R nx[3] = {(R)0, (R)0, (R)0}, nu[3] = {(R)0, (R)0, (R)0};
-    {
-        const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
-        if (t0 < n0) {
-            ncell = w.a.cell[t0];
-            nx[0] = w.a.x[t0]; nx[1] = w.a.y[t0]; nx[2] = w.a.z[t0];
-            nu[0] = w.a.vx[t0]; nu[1] = w.a.vy[t0]; nu[2] = w.a.vz[t0];
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint64_t cstride = (uint64_t)gridDim.x * (blockDim.x >> 6) * kMoveChunk;
+    // chunk C (slots cbase.., cvalid of them, pos taken) and the next chunk N
+    uint64_t cbase = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kMoveChunk;
+    uint64_t nbase = cbase + cstride;
+    uint32_t cvalid = chunk_valid(cbase, n0), nvalid = chunk_valid(nbase, n0);
+    uint32_t pos = 0u;
+    MoveRec<R> C{}, N{};
+    load_chunk(w, cbase, n0, lane, C);
+    load_chunk(w, nbase, n0, lane, N);
+    // case cache: slots [wb, we) belong to case wc
+    int wc = 0;
+    uint64_t wb = 1u, we = 0u;
+    MoveCase wcs{};
+    // lane state
+    bool has = false;
+    uint32_t t = 0u, c = 0u, mb = 0u;
+    int ccase = 0, it = 0, hits = 0;
+    R x[3] = {(R)0, (R)0, (R)0}, U[3] = {(R)0, (R)0, (R)0}, vt[3] = {(R)0, (R)0, (R)0};
+    R tau = (R)0;
+    for (;;) {
+        // refill: lanes without a simulator take the next slots in lane order
+        unsigned long long need = __ballot(!has);
+        while (need != 0ull && pos < cvalid) {
+            const uint32_t availc = cvalid - pos;
+            const uint32_t ntake = min((uint32_t)__popcll(need), availc + nvalid);
+            const uint32_t k = (uint32_t)__popcll(need & below);
+            const bool taker = ((need >> lane) & 1ull) && k < ntake;
+            const bool fromn = k >= availc;
+            const int src = (int)(fromn ? k - availc : pos + k);
+            MoveRec<R> g = shfl_rec(C, src);
+            if (ntake > availc) {
+                const MoveRec<R> h = shfl_rec(N, src);
+                if (fromn) g = h;
+            }
+            // the slots taken lie in [first, last]; a case is a contiguous slot range
+            const uint64_t first = cbase + pos;
+            const uint64_t last = (ntake > availc) ? nbase + (ntake - availc) - 1u : cbase + pos + ntake - 1u;
+            if (!(first >= wb && last < we)) {
+                wc = case_of_slot(w, (uint32_t)first);
+                wb = w.cases[wc].particle_begin;
+                we = (wc + 1 < w.ncases) ? (uint64_t)w.cases[wc + 1].particle_begin : ~0ull;
+                wcs = move_case(w, wc);
+            }
+            if (taker) {
+                const uint32_t sl = (uint32_t)(fromn ? nbase + (k - availc) : cbase + pos + k);
+                int cc = wc;
+                MoveCase cs = wcs;
+                if (last >= we) {
+                    cc = case_of_slot(w, sl);
+                    cs = move_case(w, cc);
+                }
+                // simulators of a paused case (a retired batch case) stay where they
+                // are; their lane takes another slot
+                if (cs.active) {
+                    has = true;
+                    t = sl;
+                    ccase = cc;
+                    c = g.cell;
+                    x[0] = g.x; x[1] = g.y; x[2] = g.z;
+                    U[0] = g.u; U[1] = g.v; U[2] = g.w;
+                    tau = (R)cs.dt;
+                    it = 0;
+                    hits = 0;
+                    mb = (cs.mask[0] != 0.0 ? 1u : 0u) | (cs.mask[1] != 0.0 ? 2u : 0u) | (cs.mask[2] != 0.0 ? 4u : 0u);
+                    for (int d = 0; d < 3; ++d) vt[d] = U[d] * (((mb >> d) & 1u) ? (R)1 : (R)0);
+                }
+            }
+            pos += ntake;
+            if (pos >= cvalid) {
+                pos -= cvalid;
+                C = N;
+                cbase = nbase;
+                cvalid = nvalid;
+                nbase += cstride;
+                nvalid = chunk_valid(nbase, n0);
+                load_chunk(w, nbase, n0, lane, N);
+            }
+            need = __ballot(!has);
         }
-    }
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n0; base += stride) {
-        const uint32_t t = base + threadIdx.x;
-        const bool active = t < n0;
-        uint32_t cell = ncell;
-        R x[3] = {nx[0], nx[1], nx[2]};
-        R U[3] = {nu[0], nu[1], nu[2]};
-        const uint32_t tn = t + stride;
-        if (tn < n0) {
-            ncell = w.a.cell[tn];
-            nx[0] = w.a.x[tn]; nx[1] = w.a.y[tn]; nx[2] = w.a.z[tn];
-            nu[0] = w.a.vx[tn]; nu[1] = w.a.vy[tn]; nu[2] = w.a.vz[tn];
+        if (__ballot(has) == 0ull) break;
+        bool fin = false, queued = false, err = false;
+        int qface = 0;
+        if (has) {
+            // one crossing: all face data of the cell in one memory round trip
+            const size_t base = (size_t)c * kMaxFaces;
+            V4<R> q[kMaxFaces];
+            int32_t nb[kMaxFaces];
+#pragma unroll
+            for (int f = 0; f < kMaxFaces; ++f) { q[f] = w.geom.plane[base + f]; nb[f] = w.geom.nbr[base + f]; }
+            R tmin = tau;
+            int fmin = -1;
+            int32_t code = 0;
+            R qx = (R)0, qy = (R)0, qz = (R)0;
+#pragma unroll
+            for (int f = 0; f < kMaxFaces; ++f) {
+                const R dn = q[f].x * vt[0] + q[f].y * vt[1] + q[f].z * vt[2];
+                R tf = (q[f].w - (q[f].x * x[0] + q[f].y * x[1] + q[f].z * x[2])) / dn;
+                tf = (tf < (R)0) ? (R)0 : tf;
+                const bool take = (nb[f] != kNbrEmpty) && (dn > (R)0) && (tf < tmin);
+                tmin = take ? tf : tmin;
+                fmin = take ? f : fmin;
+                code = take ? nb[f] : code;
+                qx = take ? q[f].x : qx;
+                qy = take ? q[f].y : qy;
+                qz = take ? q[f].z : qz;
+            }
+            if (fmin < 0) {
+                for (int d = 0; d < 3; ++d) x[d] = x[d] + vt[d] * tau;
+                fin = true;
+            } else {
+                for (int d = 0; d < 3; ++d) x[d] = x[d] + vt[d] * tmin;
+                tau = tau - tmin;
+                bool next = true;
+                if (code >= 0) {
+                    const V4<R> dl = w.geom.delta[base + fmin];
+                    x[0] = x[0] + dl.x;
+                    x[1] = x[1] + dl.y;
+                    x[2] = x[2] + dl.z;
+                    c = (uint32_t)code;
+                } else {
+                    const int kind = w.bm[-1 - code].kind;
+                    if (kind == UGX_BND_DELETION) {
+                        c = kDead;
+                        fin = true;
+                        next = false;
+                    } else if (kind != UGX_BND_SPECULAR) {
+                        queued = true;
+                        qface = fmin;
+                        fin = true;
+                        next = false;
+                    } else {
+                        hits += 1;
+                        const R udn = U[0] * qx + U[1] * qy + U[2] * qz;
+                        if (udn > (R)0) {
+                            U[0] = U[0] - (R)2 * udn * qx;
+                            U[1] = U[1] - (R)2 * udn * qy;
+                            U[2] = U[2] - (R)2 * udn * qz;
+                        }
+                        for (int d = 0; d < 3; ++d) vt[d] = U[d] * (((mb >> d) & 1u) ? (R)1 : (R)0);
+                    }
+                }
+                if (next) {
+                    it += 1;
+                    if (it >= kMaxCrossings) { err = true; fin = true; }
+                }
+            }
         }
-        int ccase = 0;
-        TrackOut to{1, 0, 0, 0, 0, 0};
-        if (active) ccase = case_of_slot(w, t);
-        // simulators of a paused case (a retired batch case) stay where they are
-        if (active && w.cases[ccase].active) {
-            const CaseDev& cs = w.cases[ccase];
-            double erot = 0.0;
-            Rng r;
-            rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
-            R tau = (R)cs.dt;
-            to = track_t<true, R, WorldView<R>, R>(w.geom, w.bm, cs.mask, cell, x, U, erot, w, t, tau, r, 0);
+        if (fin) {
+            has = false;
             w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
-            w.a.cell[t] = cell;
-            if (to.hits) {
+            w.a.cell[t] = c;
+            if (hits) {
                 w.a.vx[t] = U[0]; w.a.vy[t] = U[1]; w.a.vz[t] = U[2];
             }
-            if (to.queued) {
-                const uint32_t q = atomicAdd(&w.st->nwall, 1u);
-                if (q < w.wq_cap) {
-                    w.wq_slot[q] = t;
-                    w.wq_info[q] = (uint32_t)to.qface | ((uint32_t)to.qiter << 3);
-                    w.wq_tau[q] = (double)tau;
+            if (queued) {
+                const uint32_t qi = atomicAdd(&w.st->nwall, 1u);
+                if (qi < w.wq_cap) {
+                    w.wq_slot[qi] = t;
+                    w.wq_info[qi] = (uint32_t)qface | ((uint32_t)it << 3);
+                    w.wq_tau[qi] = (double)tau;
                 } else {
                     w.st->overflow = 1u;
                 }
             }
-        }
-        bool done = active && cell != kDead && !to.queued;
-        wave_agg_inc(w.cnt, done ? cell : 0u, done);
-        if (active) {
-            tcount_add(tc, w.counters, ccase, 0, (cell == kDead) ? 1ull : 0ull);
-            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
-            tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
+            tcount_add(tc, w.counters, ccase, 0, (c == kDead) ? 1ull : 0ull);
+            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)hits);
+            tcount_add(tc, w.counters, ccase, 2, err ? 1ull : 0ull);
         }
     }
     tcount_block_flush(tc, w.counters);
@@ -394,8 +576,6 @@ __global__ void k_wall(WorldView<R> w)
             w.a.vx[t] = (R)U[0]; w.a.vy[t] = (R)U[1]; w.a.vz[t] = (R)U[2];
             w.a.erot[t] = (R)erot;
         }
-        bool done = active && cell != kDead;
-        wave_agg_inc(w.cnt, done ? cell : 0u, done);
         if (active) {
             tcount_add(tc, w.counters, ccase, 0, (cell == kDead) ? 1ull : 0ull);
             tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
@@ -405,19 +585,33 @@ __global__ void k_wall(WorldView<R> w)
     tcount_block_flush(tc, w.counters);
 }
 
-// Histogram only (sort after fill / upload).
+// Occupancy of every cell over all live slots, and the rank of each simulator
+// among those of its cell (kept in the per-slot scratch array for the
+// scatter).  The order of equal cells is arbitrary: the cell tiles order each
+// cell by slot.  A lane takes four slots per pass, so that the atomics of
+// four slots are in flight together.
+constexpr int kCountUnroll = 4;
+
 template <class R>
-__global__ void k_hist(WorldView<R> w)
+__global__ void k_count(WorldView<R> w)
 {
+    constexpr int U = kCountUnroll;
     const uint32_t n_all = w.st->n + w.st->n_ins;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
-        if (base >= n_all) break;
-        uint32_t t = base + threadIdx.x;
-        bool active = t < n_all;
-        uint32_t c = active ? w.a.cell[t] : kDead;
-        bool alive = active && c != kDead;
-        wave_agg_inc(w.cnt, alive ? c : 0u, alive);
+    const uint64_t span = (uint64_t)gridDim.x * blockDim.x * U;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U; base < n_all; base += span) {
+        uint32_t t[U], c[U], r[U];
+        bool alive[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t tu = base + (uint64_t)u * blockDim.x + threadIdx.x;
+            t[u] = (uint32_t)tu;
+            c[u] = (tu < n_all) ? w.a.cell[tu] : kDead;
+            alive[u] = c[u] != kDead;
+        }
+        wave_agg_inc<U>(w.cnt, c, alive, r);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (alive[u]) w.scratch[t[u]] = r[u];
     }
 }
 
@@ -499,19 +693,31 @@ static void scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* bs
 
 // ------------------------------------------------------------ scatter
 
+// Slot list in cell order: perm[off[c] + rank] = slot.
 template <class R>
 __global__ void k_scatter(WorldView<R> w)
 {
+    constexpr int U = kCountUnroll;
     const uint32_t n_all = w.st->n + w.st->n_ins;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < w.cap; base += stride) {
-        if (base >= n_all) break;
-        uint32_t t = base + threadIdx.x;
-        bool active = t < n_all;
-        uint32_t c = active ? w.a.cell[t] : kDead;
-        bool alive = active && c != kDead;
-        uint32_t rank = wave_agg_inc(w.cursor, alive ? c : 0u, alive);
-        if (alive) w.perm[w.off[c] + rank] = t;
+    const uint64_t span = (uint64_t)gridDim.x * blockDim.x * U;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U; base < n_all; base += span) {
+        uint32_t t[U], c[U], r[U], o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t tu = base + (uint64_t)u * blockDim.x + threadIdx.x;
+            t[u] = (uint32_t)tu;
+            c[u] = kDead;
+            r[u] = 0u;
+            if (tu < n_all) {
+                c[u] = w.a.cell[tu];
+                r[u] = w.scratch[tu];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) o[u] = (c[u] != kDead) ? w.off[c[u]] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (c[u] != kDead) w.perm[o[u] + r[u]] = t[u];
     }
 }
 
@@ -1705,12 +1911,30 @@ __global__ void k_fill_finish(WorldView<R> w, const uint32_t* foff, uint32_t tot
 
 // ------------------------------------------------------------ launchers
 
+template <class R>
+uint32_t move_blocks(int device)
+{
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_move<R>, kBlock, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (per_cu < 1) per_cu = 1;
+    return (uint32_t)(per_cu * ncu);
+}
+template uint32_t move_blocks<float>(int);
+template uint32_t move_blocks<double>(int);
+
 static inline unsigned grid_for(uint64_t n, unsigned cap_blocks = 8192)
 {
     uint64_t g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
     if (g > cap_blocks) g = cap_blocks;
     return (unsigned)g;
+}
+
+// Blocks of the counting and scatter passes (kCountUnroll slots per thread).
+static inline unsigned count_grid(uint64_t cap)
+{
+    return grid_for((cap + kCountUnroll - 1) / kCountUnroll);
 }
 
 template <class R>
@@ -1798,10 +2022,9 @@ void launch_sort(const WorldView<R>& w, hipStream_t s)
 {
     const uint32_t ncp = (uint32_t)w.nc + 1u;
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
-    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cursor, ncp);
-    hipLaunchKernelGGL(k_hist<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_count<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
     scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
-    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_scatter<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
     launch_tiles<R>(w, s, true);
     hipLaunchKernelGGL(k_finish_sort<R>, dim3(1), dim3(1), 0, s, w);
 }
@@ -1816,20 +2039,20 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     mark(0);
     hipLaunchKernelGGL(k_begin_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
-    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cursor, ncp);
     mark(1);
     if (w.n_inflow > 0) {
         hipLaunchKernelGGL(k_inflow_count<R>, dim3(1), dim3(1024), 0, s, w);
         hipLaunchKernelGGL(k_inflow_gen<R>, dim3(grid_for(w.max_ins)), dim3(kBlock), 0, s, w);
     }
     mark(2);
-    hipLaunchKernelGGL(k_move<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_move<R>, dim3(w.move_grid), dim3(kBlock), 0, s, w);
     mark(3);
     hipLaunchKernelGGL(k_wall<R>, dim3(grid_for(w.wq_cap, 2048)), dim3(kBlock), 0, s, w);
     mark(4);
+    hipLaunchKernelGGL(k_count<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
     scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
     mark(5);
-    hipLaunchKernelGGL(k_scatter<R>, dim3(grid_for(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_scatter<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
     mark(6);
     launch_tiles<R>(w, s, false);
     mark(7);

@@ -61,10 +61,10 @@ struct WorldView {
     Pairs pr;
     uint32_t* cnt;            // [nc+1] counts (histogram)
     uint32_t* off;            // [nc+1] offsets
-    uint32_t* cursor;         // [nc]
     uint32_t* perm;           // [cap]
     uint32_t* ins_off;        // [n_inflow+1]
-    uint32_t* scratch;        // [cap] rank scratch of tiles larger than LDS
+    uint32_t* scratch;        // [cap] rank of each slot within its cell (count -> scatter), then
+                              //       the slot order of tiles larger than LDS
     uint32_t* tile_first;     // [max tiles + 2] first cell of each tile
     uint32_t* tile_p0;        //                 its first sorted slot
     int32_t* tile_case;       //                 the case of its first cell
@@ -80,6 +80,7 @@ struct WorldView {
     int32_t has_collide;      // some case collides (DSMC or BGK family)
     int32_t has_hybrid;       // some case switches models per cell
     int32_t tile_variant;     // cell-tile configuration (0..kTileVariants-1)
+    uint32_t move_grid;       // blocks of the move: as many as are resident at once
     int32_t kn_due;           // a hybrid case decomposes at this step
     int32_t kn_passes;        // largest smoothing pass count of the due cases
     const KnFace* knf;        // [nc*kMaxFaces]
@@ -122,6 +123,8 @@ inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc
 // Host launchers (ugx_kernels.hip).
 constexpr int kPhases = 9;  // begin, inflow, move, wall, scan, scatter, cells, collide, end
 template <class R> void launch_step(const WorldView<R>& w, hipStream_t s, hipEvent_t* marks);
+// Blocks of the move kernel that are resident on `device` at once (0 on error).
+template <class R> uint32_t move_blocks(int device);
 template <class R> void launch_sort(const WorldView<R>& w, hipStream_t s);
 template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase,
                                     const double* nd, const double* T, double rotT, const double* U,

@@ -319,7 +319,7 @@ void World::release()
                      (void**)&d_nb_off, (void**)&d_nb_list,
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
-                     (void**)&d_cursor, (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
+                     (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
                      (void**)&d_tile_first, (void**)&d_tile_p0, (void**)&d_tile_case,
                      (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
                      (void**)&d_sel_slot, (void**)&d_sel_cell, (void**)&d_sel_k,
@@ -435,6 +435,7 @@ WorldView<R> World::view() const
     w.sp = Species{d_mass, d_diam, d_rotdof, d_omega, d_muref, d_prterm, d_local};
     w.has_hybrid = has_hybrid;
     w.tile_variant = tile_variant;
+    w.move_grid = move_grid;
     w.kn_due = 0;
     w.kn_passes = 0;
     w.knf = d_knf;
@@ -447,7 +448,6 @@ WorldView<R> World::view() const
     w.pr = Pairs{d_pmr, d_pC, d_pe, d_pchi, d_palpha};
     w.cnt = d_cnt;
     w.off = d_off;
-    w.cursor = d_cursor;
     w.perm = d_perm;
     w.ins_off = d_ins_off;
     w.scratch = d_scratch;
@@ -489,6 +489,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     nc = 0;
     int nsp = 0, npair = 0, nbm = 0;
     for (auto& s : specs) { nc += s.geom.nc; nsp += s.ns; npair += s.ns * s.ns; nbm += (int)s.geom.bm.size(); }
+    move_grid = (precision == UGX_FP32_STATE) ? move_blocks<float>(device) : move_blocks<double>(device);
+    if (move_grid == 0) return UGX_ERR_DEVICE;
     // cell tiles keep a 16-bit species id per simulator
     if (nsp > 65535) return UGX_ERR_CAPACITY;
     nsp_total = nsp;
@@ -693,7 +695,6 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     std::vector<uint32_t> zc((size_t)nc + 1, 0u);
     if ((rc = up(d_cnt, zc))) return rc;
     if ((rc = up(d_off, zc))) return rc;
-    if ((rc = up(d_cursor, zc))) return rc;
     std::vector<uint32_t> zi((size_t)n_inflow + 1, 0u);
     if ((rc = up(d_ins_off, zi))) return rc;
     if (has_bgk) {

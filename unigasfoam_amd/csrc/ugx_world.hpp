@@ -83,7 +83,6 @@ struct World {
     double *d_pmr = nullptr, *d_pC = nullptr, *d_pe = nullptr, *d_pchi = nullptr, *d_palpha = nullptr;
     uint32_t* d_cnt = nullptr;
     uint32_t* d_off = nullptr;
-    uint32_t* d_cursor = nullptr;
     uint32_t* d_perm = nullptr;
     uint32_t* d_ins_off = nullptr;
     uint32_t* d_scratch = nullptr;
@@ -115,6 +114,7 @@ struct World {
     // configuration on the live case (results do not depend on it)
     bool autotune = true;
     int tile_variant = 0;
+    uint32_t move_grid = 0;       // resident blocks of the move kernel
     int tune_step = 0;
     double tune_ms[kTileVariants] = {};
     std::vector<hipEvent_t> tune_ev;

@@ -28,10 +28,10 @@ def phase_bytes(phase, R, n, n_cells, cand=0, coll=0, n_bgk=0, rows=N_BASE_ROWS 
     bgk / hybrid: the case keeps BGK moments / hybrid accumulators."""
     if phase == "move":        # read x, v, cell; write x, cell
         return n * (3 * R + 3 * R + 4 + 3 * R + 4)
-    if phase == "scatter":     # read cell, write slot
-        return n * 8
-    if phase == "scan":        # counts read twice, offsets written
-        return n_cells * 12
+    if phase == "scatter":     # read cell and rank, write slot
+        return n * 12
+    if phase == "scan":        # count: read cell, write rank; scan: counts read twice, offsets written
+        return n * 8 + n_cells * 12
     if phase == "cells":       # slot list, gather of the record, sorted record written;
         #                        per cell: offsets, tile table, sampler r/w, hybrid r/w, BGK moments w
         per_cell = 8 + 4 + 2 * 8 * rows + (2 * 8 * 8 if hybrid else 0) + (8 * N_MOM_ROWS if bgk else 0)
