@@ -98,6 +98,19 @@ def test_inflow_deletion_identical(require_gpu):
     assert g.counters()["deleted"] > 0
 
 
+def test_capacity_growth_identical(require_gpu):
+    """Default capacity: the particle buffers grow after the fill's sort, and
+    the next step's occupancy count reads that sort's tile table."""
+    spec, fill = workloads.heat_bath(n_cells=6, per_cell=20)
+    g = Case(spec, precision="fp64")
+    o = OracleCase(spec, capacity=1 << 18)
+    g.fill(**fill)
+    o.fill(**fill)
+    g.step(12)
+    o.step(12)
+    assert_identical(g, o, "capacity growth")
+
+
 def test_fp32_state_runs(require_gpu):
     spec, fill = workloads.heat_bath(n_cells=6, per_cell=20)
     g = Case(spec, precision="fp32")

@@ -585,20 +585,36 @@ __global__ void k_wall(WorldView<R> w)
     tcount_block_flush(tc, w.counters);
 }
 
-// Occupancy of every cell over all live slots, and the rank of each simulator
-// among those of its cell (kept in the per-slot scratch array for the
-// scatter).  The order of equal cells is arbitrary: the cell tiles order each
-// cell by slot.  A lane takes four slots per pass, so that the atomics of
-// four slots are in flight together.
+// ------------------------------------------------------------ occupancy
+// The sort after the move needs each cell's new population and, for every
+// simulator, a place within its cell.  Most simulators stay in their cell or
+// cross into a face neighbour, so they are counted per old cell in LDS, over
+// the tiles of the previous sort (which partition the cells and the slots as
+// they stand at the start of the step): class 0 stays, class 1 + f left
+// through face f.  The new cell d gathers its arrivals from its distinct
+// neighbours; a mover from c is placed in d's range of the first face of d
+// that points back to c.  Simulators ending anywhere else, and the inflow,
+// are counted by device atomics (class 7).  Each simulator's word in the
+// per-slot scratch array is (class of the new cell << 28) | rank within that
+// class.  The order within a cell is arbitrary: the cell tiles order each cell
+// by slot.
+constexpr uint32_t kRankBits = 28;
+constexpr uint32_t kRankMask = (1u << kRankBits) - 1u;
+constexpr uint32_t kClassOther = 7;
 constexpr int kCountUnroll = 4;
+constexpr uint32_t kMaxTileCells = 512u / kCellW + 2u;   // cells of the largest tile variant
+constexpr int kCountTiles = 4;                           // tiles per block of k_count_tiles
 
+// Class-7 counting of the slots [begin, n + n_ins): the inflow of a step, or
+// every slot for the sort after an upload.
 template <class R>
-__global__ void k_count(WorldView<R> w)
+__global__ void k_count(WorldView<R> w, int all)
 {
     constexpr int U = kCountUnroll;
     const uint32_t n_all = w.st->n + w.st->n_ins;
+    const uint64_t begin = all ? 0u : w.st->n;
     const uint64_t span = (uint64_t)gridDim.x * blockDim.x * U;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U; base < n_all; base += span) {
+    for (uint64_t base = begin + (uint64_t)blockIdx.x * blockDim.x * U; base < n_all; base += span) {
         uint32_t t[U], c[U], r[U];
         bool alive[U];
 #pragma unroll
@@ -608,10 +624,130 @@ __global__ void k_count(WorldView<R> w)
             c[u] = (tu < n_all) ? w.a.cell[tu] : kDead;
             alive[u] = c[u] != kDead;
         }
-        wave_agg_inc<U>(w.cnt, c, alive, r);
+        wave_agg_inc<U>(w.ocnt, c, alive, r);
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (alive[u]) w.scratch[t[u]] = r[u];
+        for (int u = 0; u < U; ++u) {
+            if (!alive[u]) continue;
+            if (r[u] > kRankMask) w.st->overflow = 1u;
+            w.scratch[t[u]] = (kClassOther << kRankBits) | (r[u] & kRankMask);
+        }
+    }
+}
+
+// Classes 0-6 of the simulators that were in the cells of TPB consecutive
+// tiles of the previous sort (one block per TPB tiles); each thread takes
+// kCountUnroll simulators per pass so that the class-7 atomics of a pass are
+// in flight together.
+template <class R, int Block, int TPB>
+__global__ void __launch_bounds__(Block) k_count_tiles(WorldView<R> w)
+{
+    constexpr int U = kCountUnroll;
+    constexpr uint32_t kCells = TPB * kMaxTileCells;
+    __shared__ uint32_t s_off[kCells + 1];
+    __shared__ int32_t s_nb[kCells * kMaxFaces];
+    __shared__ int8_t s_rev[kCells * kMaxFaces];
+    __shared__ uint32_t s_cnt[kCells * 7];
+    const uint32_t ntiles = w.st->ntiles;
+    const uint32_t t0 = blockIdx.x * (uint32_t)TPB;
+    if (t0 >= ntiles) return;
+    const uint32_t t1 = min(t0 + (uint32_t)TPB, ntiles);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t c0 = w.tile_first[t0];
+    const uint32_t nct = w.tile_first[t1] - c0;
+    const uint32_t p0 = w.tile_p0[t0];
+    const uint32_t size = w.tile_p0[t1] - p0;
+    if (nct == 0u) return;
+    for (uint32_t k = tid; k <= nct; k += Block) s_off[k] = w.off[c0 + k];
+    for (uint32_t k = tid; k < nct * kMaxFaces; k += Block) {
+        s_nb[k] = w.geom.nbr[(size_t)c0 * kMaxFaces + k];
+        s_rev[k] = w.rev[(size_t)c0 * kMaxFaces + k];
+    }
+    for (uint32_t k = tid; k < nct * 7u; k += Block) s_cnt[k] = 0u;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < size; b0 += Block * U) {
+        uint32_t d[U], cls[U], lc[U], ro[U];
+        bool other[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = b0 + (uint32_t)u * Block + tid;
+            d[u] = (i < size) ? w.a.cell[p0 + i] : kDead;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = b0 + (uint32_t)u * Block + tid;
+            cls[u] = kClassOther;
+            lc[u] = 0u;
+            if (d[u] != kDead) {
+                // old cell of entry i: the last k with s_off[k] <= p0 + i
+                uint32_t lo = 0, hi = nct - 1u;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1u) >> 1;
+                    if (s_off[mid] <= p0 + i) lo = mid; else hi = mid - 1u;
+                }
+                lc[u] = lo;
+                if (d[u] == c0 + lo) {
+                    cls[u] = 0u;
+                } else {
+                    for (int f = 0; f < kMaxFaces; ++f)
+                        if (s_nb[lo * kMaxFaces + f] == (int32_t)d[u] && s_rev[lo * kMaxFaces + f] >= 0) {
+                            cls[u] = 1u + (uint32_t)f;
+                            break;
+                        }
+                }
+            }
+            other[u] = d[u] != kDead && cls[u] == kClassOther;
+        }
+        wave_agg_inc<U>(w.ocnt, d, other, ro);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (d[u] == kDead) continue;
+            const uint32_t i = b0 + (uint32_t)u * Block + tid;
+            uint32_t word;
+            if (other[u]) {
+                word = (kClassOther << kRankBits) | (ro[u] & kRankMask);
+                if (ro[u] > kRankMask) w.st->overflow = 1u;
+            } else {
+                const uint32_t r = atomicAdd(&s_cnt[lc[u] * 7u + cls[u]], 1u);
+                const uint32_t k = (cls[u] == 0u) ? 0u : 1u + (uint32_t)s_rev[lc[u] * kMaxFaces + (cls[u] - 1u)];
+                word = (k << kRankBits) | (r & kRankMask);
+                if (r > kRankMask) w.st->overflow = 1u;
+            }
+            w.scratch[p0 + i] = word;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < nct * 7u; k += Block) w.out7[(size_t)c0 * 7u + k] = s_cnt[k];
+}
+
+// Per cell: population (stayers, arrivals through each distinct neighbour,
+// class 7) and the start of each class within the cell.
+template <class R>
+__global__ void k_gather(WorldView<R> w)
+{
+    const uint32_t nc = (uint32_t)w.nc;
+    for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nc; d += gridDim.x * blockDim.x) {
+        const size_t fb = (size_t)d * kMaxFaces;
+        int32_t nb[kMaxFaces];
+#pragma unroll
+        for (int g = 0; g < kMaxFaces; ++g) nb[g] = w.geom.nbr[fb + g];
+        uint32_t in[kMaxFaces];
+#pragma unroll
+        for (int g = 0; g < kMaxFaces; ++g) {
+            bool first = nb[g] >= 0 && nb[g] != (int32_t)d;
+            for (int h = 0; h < g; ++h) first = first && nb[h] != nb[g];
+            const int f = w.rev[fb + g];
+            in[g] = (first && f >= 0) ? w.out7[(size_t)nb[g] * 7u + 1u + (uint32_t)f] : 0u;
+        }
+        uint32_t run = w.out7[(size_t)d * 7u];
+        uint32_t* pre = w.pre + (size_t)d * 8u;
+        pre[0] = 0u;
+#pragma unroll
+        for (int g = 0; g < kMaxFaces; ++g) {
+            pre[1 + g] = run;
+            run += in[g];
+        }
+        pre[7] = run;
+        w.cnt[d] = run + w.ocnt[d];
     }
 }
 
@@ -693,7 +829,7 @@ static void scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* bs
 
 // ------------------------------------------------------------ scatter
 
-// Slot list in cell order: perm[off[c] + rank] = slot.
+// Slot list in cell order: perm[off[d] + start of the class + rank] = slot.
 template <class R>
 __global__ void k_scatter(WorldView<R> w)
 {
@@ -714,10 +850,15 @@ __global__ void k_scatter(WorldView<R> w)
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) o[u] = (c[u] != kDead) ? w.off[c[u]] : 0u;
-#pragma unroll
         for (int u = 0; u < U; ++u)
-            if (c[u] != kDead) w.perm[o[u] + r[u]] = t[u];
+            o[u] = (c[u] != kDead) ? w.off[c[u]] + w.pre[(size_t)c[u] * 8u + (r[u] >> kRankBits)] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (c[u] == kDead) continue;
+            const uint32_t p = o[u] + (r[u] & kRankMask);
+            if (p < w.cap) w.perm[p] = t[u];
+            else w.st->overflow = 1u;
+        }
     }
 }
 
@@ -2021,8 +2162,12 @@ template <class R>
 void launch_sort(const WorldView<R>& w, hipStream_t s)
 {
     const uint32_t ncp = (uint32_t)w.nc + 1u;
+    const uint32_t n7 = (uint32_t)w.nc * 7u;
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
-    hipLaunchKernelGGL(k_count<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.ocnt, ncp);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n7)), dim3(kBlock), 0, s, w.out7, n7);
+    hipLaunchKernelGGL(k_count<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w, 1);
+    hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.nc)), dim3(kBlock), 0, s, w);
     scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
     hipLaunchKernelGGL(k_scatter<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
     launch_tiles<R>(w, s, true);
@@ -2039,6 +2184,7 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     mark(0);
     hipLaunchKernelGGL(k_begin_step<R>, dim3(gc), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.cnt, ncp);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.ocnt, ncp);
     mark(1);
     if (w.n_inflow > 0) {
         hipLaunchKernelGGL(k_inflow_count<R>, dim3(1), dim3(1024), 0, s, w);
@@ -2049,7 +2195,10 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     mark(3);
     hipLaunchKernelGGL(k_wall<R>, dim3(grid_for(w.wq_cap, 2048)), dim3(kBlock), 0, s, w);
     mark(4);
-    hipLaunchKernelGGL(k_count<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL((k_count_tiles<R, 256, kCountTiles>), dim3((max_tiles(w.cap, (uint64_t)w.nc) + kCountTiles - 1) / kCountTiles), dim3(256), 0, s, w);
+    if (w.n_inflow > 0)
+        hipLaunchKernelGGL(k_count<R>, dim3(grid_for(((uint64_t)w.max_ins + kCountUnroll - 1) / kCountUnroll)), dim3(kBlock), 0, s, w, 0);
+    hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.nc)), dim3(kBlock), 0, s, w);
     scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);
     mark(5);
     hipLaunchKernelGGL(k_scatter<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w);

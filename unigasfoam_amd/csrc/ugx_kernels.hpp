@@ -59,7 +59,12 @@ struct WorldView {
     const double* cc;         // [3*nc] reference points (hybrid distances)
     Species sp;
     Pairs pr;
-    uint32_t* cnt;            // [nc+1] counts (histogram)
+    uint32_t* cnt;            // [nc+1] simulators per cell after the move
+    uint32_t* ocnt;           // [nc+1] of them, those counted by atomics (class 7)
+    uint32_t* out7;           // [7*nc] per old cell: simulators staying (0) and leaving to the
+                              //        neighbour behind face f (1 + f)
+    uint32_t* pre;            // [8*nc] per cell: start of each arrival class within the cell
+    const int8_t* rev;        // [6*nc] first face of the neighbour behind face f that points back
     uint32_t* off;            // [nc+1] offsets
     uint32_t* perm;           // [cap]
     uint32_t* ins_off;        // [n_inflow+1]

@@ -318,7 +318,7 @@ void World::release()
                      (void**)&d_diam, (void**)&d_knf, (void**)&d_fkind, (void**)&d_kw, (void**)&d_kw2, (void**)&d_ids2,
                      (void**)&d_nb_off, (void**)&d_nb_list,
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
-                     (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_off,
+                     (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_ocnt, (void**)&d_out7, (void**)&d_pre, (void**)&d_rev, (void**)&d_off,
                      (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
                      (void**)&d_tile_first, (void**)&d_tile_p0, (void**)&d_tile_case,
                      (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
@@ -361,9 +361,29 @@ int World::alloc_particles(uint32_t newcap)
     cur = 0;
     dfree(d_perm);
     dfree(d_scratch);
-    dfree(d_tile_first);
-    dfree(d_tile_p0);
-    dfree(d_tile_case);
+    // the tile table of the last sort describes the particles just carried
+    // over; the occupancy count of the next step reads it
+    {
+        const size_t otiles = cap > 0 ? (size_t)max_tiles(cap, (uint64_t)nc) : 0;
+        const size_t ntiles_new = (size_t)max_tiles(newcap, (uint64_t)nc);
+        uint32_t *tf = nullptr, *tp = nullptr;
+        int32_t* tc = nullptr;
+        if (int rc = dalloc(tf, ntiles_new)) return rc;
+        if (int rc = dalloc(tp, ntiles_new)) { dfree(tf); return rc; }
+        if (int rc = dalloc(tc, ntiles_new)) { dfree(tf); dfree(tp); return rc; }
+        if (otiles > 0) {
+            HIP_TRY(hipMemcpyAsync(tf, d_tile_first, otiles * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(tp, d_tile_p0, otiles * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(tc, d_tile_case, otiles * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
+        dfree(d_tile_first);
+        dfree(d_tile_p0);
+        dfree(d_tile_case);
+        d_tile_first = tf;
+        d_tile_p0 = tp;
+        d_tile_case = tc;
+    }
     dfree(d_sel_slot);
     dfree(d_sel_cell);
     dfree(d_sel_k);
@@ -372,9 +392,6 @@ int World::alloc_particles(uint32_t newcap)
     dfree(d_wq_tau);
     if (int rc = dalloc(d_perm, newcap)) return rc;
     if (int rc = dalloc(d_scratch, newcap)) return rc;
-    if (int rc = dalloc(d_tile_first, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
-    if (int rc = dalloc(d_tile_p0, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
-    if (int rc = dalloc(d_tile_case, (size_t)max_tiles(newcap, (uint64_t)nc))) return rc;
     if (has_bgk) {
         if (int rc = dalloc(d_sel_slot, newcap)) return rc;
         if (int rc = dalloc(d_sel_cell, newcap)) return rc;
@@ -447,6 +464,10 @@ WorldView<R> World::view() const
     w.nb_list = d_nb_list;
     w.pr = Pairs{d_pmr, d_pC, d_pe, d_pchi, d_palpha};
     w.cnt = d_cnt;
+    w.ocnt = d_ocnt;
+    w.out7 = d_out7;
+    w.pre = d_pre;
+    w.rev = d_rev;
     w.off = d_off;
     w.perm = d_perm;
     w.ins_off = d_ins_off;
@@ -657,6 +678,18 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         return UGX_OK;
     };
     if ((rc = up(d_nbr, nbr))) return rc;
+    {
+        // for each face, the first face of the neighbour behind it that points back
+        std::vector<int8_t> rev((size_t)nc * kMaxFaces, (int8_t)-1);
+        for (size_t c = 0; c < (size_t)nc; ++c)
+            for (int f = 0; f < kMaxFaces; ++f) {
+                const int32_t nb = nbr[c * kMaxFaces + f];
+                if (nb < 0) continue;
+                for (int g = 0; g < kMaxFaces; ++g)
+                    if (nbr[(size_t)nb * kMaxFaces + g] == (int32_t)c) { rev[c * kMaxFaces + f] = (int8_t)g; break; }
+            }
+        if ((rc = up(d_rev, rev))) return rc;
+    }
     if ((rc = up(d_bm, bm))) return rc;
     if ((rc = up(d_inflow, inflow))) return rc;
     if ((rc = up(d_cases, hc))) return rc;
@@ -694,6 +727,12 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_samp, zs))) return rc;
     std::vector<uint32_t> zc((size_t)nc + 1, 0u);
     if ((rc = up(d_cnt, zc))) return rc;
+    if ((rc = up(d_ocnt, zc))) return rc;
+    {
+        std::vector<uint32_t> z7((size_t)nc * 7, 0u), z8((size_t)nc * 8, 0u);
+        if ((rc = up(d_out7, z7))) return rc;
+        if ((rc = up(d_pre, z8))) return rc;
+    }
     if ((rc = up(d_off, zc))) return rc;
     std::vector<uint32_t> zi((size_t)n_inflow + 1, 0u);
     if ((rc = up(d_ins_off, zi))) return rc;

@@ -82,6 +82,10 @@ struct World {
     int32_t* d_local = nullptr;
     double *d_pmr = nullptr, *d_pC = nullptr, *d_pe = nullptr, *d_pchi = nullptr, *d_palpha = nullptr;
     uint32_t* d_cnt = nullptr;
+    uint32_t* d_ocnt = nullptr;
+    uint32_t* d_out7 = nullptr;
+    uint32_t* d_pre = nullptr;
+    int8_t* d_rev = nullptr;
     uint32_t* d_off = nullptr;
     uint32_t* d_perm = nullptr;
     uint32_t* d_ins_off = nullptr;
