@@ -634,60 +634,70 @@ __global__ void k_count(WorldView<R> w, int all)
     }
 }
 
-// Classes 0-6 of the simulators that were in the cells of TPB consecutive
-// tiles of the previous sort (one block per TPB tiles); each thread takes
-// kCountUnroll simulators per pass so that the class-7 atomics of a pass are
-// in flight together.
+// Classes of the simulators that were in the cells of TPB consecutive tiles of
+// the previous sort (one block per TPB tiles), in windows of kCountWindow
+// slots.  The rank within (old cell, class) follows the order of the slots:
+// within each 64-slot chunk of the window, ballots per class give a
+// simulator's rank among the same cell's and class's earlier slots of the
+// chunk; a cell that spans chunks carries its counts from chunk to chunk, and
+// from window to window.  Class 7 takes its rank from the device atomics.
+constexpr uint32_t kCountWindow = 1024;
+
 template <class R, int Block, int TPB>
 __global__ void __launch_bounds__(Block) k_count_tiles(WorldView<R> w)
 {
-    constexpr int U = kCountUnroll;
+    constexpr int U = kCountWindow / Block;
+    constexpr uint32_t kChunks = kCountWindow / 64;
     constexpr uint32_t kCells = TPB * kMaxTileCells;
     __shared__ uint32_t s_off[kCells + 1];
     __shared__ int32_t s_nb[kCells * kMaxFaces];
     __shared__ int8_t s_rev[kCells * kMaxFaces];
     __shared__ uint32_t s_cnt[kCells * 7];
+    __shared__ uint32_t s_first[kChunks], s_last[kChunks], s_whole[kChunks];
+    __shared__ uint32_t s_lastn[kChunks * 7], s_carry[kChunks * 7];
     const uint32_t ntiles = w.st->ntiles;
     const uint32_t t0 = blockIdx.x * (uint32_t)TPB;
     if (t0 >= ntiles) return;
     const uint32_t t1 = min(t0 + (uint32_t)TPB, ntiles);
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t c0 = w.tile_first[t0];
     const uint32_t nct = w.tile_first[t1] - c0;
     const uint32_t p0 = w.tile_p0[t0];
     const uint32_t size = w.tile_p0[t1] - p0;
     if (nct == 0u) return;
-    for (uint32_t k = tid; k <= nct; k += Block) s_off[k] = w.off[c0 + k];
+    for (uint32_t k = tid; k <= nct; k += Block) s_off[k] = w.off[c0 + k] - p0;
     for (uint32_t k = tid; k < nct * kMaxFaces; k += Block) {
         s_nb[k] = w.geom.nbr[(size_t)c0 * kMaxFaces + k];
         s_rev[k] = w.rev[(size_t)c0 * kMaxFaces + k];
     }
     for (uint32_t k = tid; k < nct * 7u; k += Block) s_cnt[k] = 0u;
     __syncthreads();
-    for (uint32_t b0 = 0; b0 < size; b0 += Block * U) {
-        uint32_t d[U], cls[U], lc[U], ro[U];
+    for (uint32_t w0 = 0; w0 < size; w0 += kCountWindow) {
+        const uint32_t wn = min(kCountWindow, size - w0);
+        uint32_t d[U], cls[U], lc[U], ro[U], rin[U];
         bool other[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t i = b0 + (uint32_t)u * Block + tid;
-            d[u] = (i < size) ? w.a.cell[p0 + i] : kDead;
+            const uint32_t i = (uint32_t)u * Block + tid;
+            d[u] = (i < wn) ? w.a.cell[p0 + w0 + i] : kDead;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t i = b0 + (uint32_t)u * Block + tid;
+            const uint32_t i = (uint32_t)u * Block + tid;
             cls[u] = kClassOther;
             lc[u] = 0u;
-            if (d[u] != kDead) {
-                // old cell of entry i: the last k with s_off[k] <= p0 + i
+            if (i < wn) {
+                // old cell of entry i: the last k with s_off[k] <= w0 + i
                 uint32_t lo = 0, hi = nct - 1u;
                 while (lo < hi) {
                     const uint32_t mid = (lo + hi + 1u) >> 1;
-                    if (s_off[mid] <= p0 + i) lo = mid; else hi = mid - 1u;
+                    if (s_off[mid] <= w0 + i) lo = mid; else hi = mid - 1u;
                 }
                 lc[u] = lo;
                 if (d[u] == c0 + lo) {
                     cls[u] = 0u;
-                } else {
+                } else if (d[u] != kDead) {
                     for (int f = 0; f < kMaxFaces; ++f)
                         if (s_nb[lo * kMaxFaces + f] == (int32_t)d[u] && s_rev[lo * kMaxFaces + f] >= 0) {
                             cls[u] = 1u + (uint32_t)f;
@@ -698,24 +708,82 @@ __global__ void __launch_bounds__(Block) k_count_tiles(WorldView<R> w)
             other[u] = d[u] != kDead && cls[u] == kClassOther;
         }
         wave_agg_inc<U>(w.ocnt, d, other, ro);
+        // ranks within the chunk (entries u * Block + tid: chunk u * Block / 64 + wave)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (d[u] == kDead) continue;
-            const uint32_t i = b0 + (uint32_t)u * Block + tid;
+            const uint32_t i = (uint32_t)u * Block + tid;
+            const uint32_t chunk = i >> 6, cs0 = w0 + (chunk << 6);
+            const bool ranked = i < wn && d[u] != kDead && cls[u] < kClassOther;
+            unsigned long long m[7];
+#pragma unroll
+            for (int c = 0; c < 7; ++c) m[c] = __ballot(ranked && cls[u] == (uint32_t)c);
+            // lanes of this lane's cell in the chunk start at lane fl
+            const uint32_t fl = (i < wn && s_off[lc[u]] > cs0) ? s_off[lc[u]] - cs0 : 0u;
+            const unsigned long long seg = ~((1ull << fl) - 1ull);
+            unsigned long long mine = 0ull;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) mine = (cls[u] == (uint32_t)c) ? m[c] : mine;
+            rin[u] = (uint32_t)__popcll(mine & below & seg);
+            // the chunk's last cell: its counts per class, and whether it fills the chunk
+            const unsigned long long valid = __ballot(i < wn);
+            if (valid != 0ull) {
+                const int ll = 63 - __clzll(valid);
+                const uint32_t lcl = (uint32_t)__shfl((int)lc[u], ll);
+                const uint32_t fll = (uint32_t)__shfl((int)fl, ll);
+                const unsigned long long segl = ~((1ull << fll) - 1ull);
+                if (lane == 0u) {
+                    s_first[chunk] = lc[u];
+                    s_last[chunk] = lcl;
+                    s_whole[chunk] = (fll == 0u) ? 1u : 0u;
+#pragma unroll
+                    for (int c = 0; c < 7; ++c) s_lastn[chunk * 7u + c] = (uint32_t)__popcll(m[c] & segl);
+                }
+            } else if (lane == 0u) {
+                s_first[chunk] = 0xFFFFFFFFu;
+                s_last[chunk] = 0xFFFFFFFFu;
+                s_whole[chunk] = 0u;
+            }
+        }
+        __syncthreads();
+        // carry into chunk k's first cell (class c): the counts of the same cell in
+        // the chunks before, walking back through chunks that the cell fills
+        if (tid < kChunks * 7u) {
+            const uint32_t k = tid / 7u, c = tid % 7u;
+            uint32_t in = 0u;
+            for (uint32_t j = k; j > 0u; --j) {
+                if (s_first[j] == 0xFFFFFFFFu || s_first[j] != s_last[j - 1u]) break;
+                in += s_lastn[(j - 1u) * 7u + c];
+                if (!s_whole[j - 1u]) break;
+            }
+            s_carry[tid] = in;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = (uint32_t)u * Block + tid;
+            if (i >= wn || d[u] == kDead) continue;
             uint32_t word;
             if (other[u]) {
-                word = (kClassOther << kRankBits) | (ro[u] & kRankMask);
                 if (ro[u] > kRankMask) w.st->overflow = 1u;
+                word = (kClassOther << kRankBits) | (ro[u] & kRankMask);
             } else {
-                const uint32_t r = atomicAdd(&s_cnt[lc[u] * 7u + cls[u]], 1u);
-                const uint32_t k = (cls[u] == 0u) ? 0u : 1u + (uint32_t)s_rev[lc[u] * kMaxFaces + (cls[u] - 1u)];
-                word = (k << kRankBits) | (r & kRankMask);
+                const uint32_t chunk = i >> 6;
+                const uint32_t r = s_cnt[lc[u] * 7u + cls[u]] +
+                                   ((lc[u] == s_first[chunk]) ? s_carry[chunk * 7u + cls[u]] : 0u) + rin[u];
+                const uint32_t kc = (cls[u] == 0u) ? 0u : 1u + (uint32_t)s_rev[lc[u] * kMaxFaces + (cls[u] - 1u)];
                 if (r > kRankMask) w.st->overflow = 1u;
+                word = (kc << kRankBits) | (r & kRankMask);
             }
-            w.scratch[p0 + i] = word;
+            w.scratch[p0 + w0 + i] = word;
         }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = (uint32_t)u * Block + tid;
+            if (i < wn && d[u] != kDead && cls[u] < kClassOther) atomicAdd(&s_cnt[lc[u] * 7u + cls[u]], 1u);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (uint32_t k = tid; k < nct * 7u; k += Block) w.out7[(size_t)c0 * 7u + k] = s_cnt[k];
 }
 
@@ -738,16 +806,30 @@ __global__ void k_gather(WorldView<R> w)
             const int f = w.rev[fb + g];
             in[g] = (first && f >= 0) ? w.out7[(size_t)nb[g] * 7u + 1u + (uint32_t)f] : 0u;
         }
-        uint32_t run = w.out7[(size_t)d * 7u];
-        uint32_t* pre = w.pre + (size_t)d * 8u;
-        pre[0] = 0u;
+        // class k's source: the cell itself (k = 0) or the neighbour behind face
+        // k - 1.  A class starts after the classes of smaller source cells, so the
+        // simulators of classes 0-6 follow the order of their slots; class 7 ends
+        // the cell
+        uint32_t src[7], num[7];
+        src[0] = d;
+        num[0] = w.out7[(size_t)d * 7u];
 #pragma unroll
         for (int g = 0; g < kMaxFaces; ++g) {
-            pre[1 + g] = run;
-            run += in[g];
+            src[1 + g] = (uint32_t)nb[g];
+            num[1 + g] = in[g];
         }
-        pre[7] = run;
-        w.cnt[d] = run + w.ocnt[d];
+        uint32_t* pre = w.pre + (size_t)d * 8u;
+        uint32_t total = 0u;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            uint32_t p = 0u;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) p += (num[q] != 0u && src[q] < src[k]) ? num[q] : 0u;
+            pre[k] = p;
+            total += num[k];
+        }
+        pre[7] = total;
+        w.cnt[d] = total + w.ocnt[d];
     }
 }
 
@@ -1023,9 +1105,64 @@ __device__ inline TileCase tile_case_of(const CaseDev& c)
 // hybrid time sums and the BGK moments.  All 64 lanes of every wave call it
 // with the same arguments.  Single-species worlds take the species rows
 // from rows N and M|v|^2 (the same sums).
+// Sampler rows of one cell that lane j of the cell's 8-lane group updates
+// (slot i of five): base rows j, j + 8, j + 16 below the sum level's row
+// count (rows 8-19 only for the full sampler), then the species rows of
+// species j (several species) or, single-species, the species-N row (lane 0)
+// and the species-MCC row (lane 1).  -1: no row.
+template <int L, bool MS>
+__device__ inline void samp_lane_rows(uint32_t j, int ns, bool full, int (&rows)[5])
+{
+    constexpr int B = SumRows<L, MS>::kBase;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int q = (int)j + 8 * i;
+        rows[i] = (q < B && (full || q < UGX_S_MUU)) ? q : -1;
+    }
+    if constexpr (MS) {
+        rows[3] = ((int)j < ns) ? UGX_S_SPECIES_N + (int)j : -1;
+        rows[4] = ((int)j < ns) ? UGX_S_SPECIES_MCC + (int)j : -1;
+    } else {
+        rows[3] = (j == 0u) ? UGX_S_SPECIES_N : -1;
+        rows[4] = (j == 1u) ? UGX_S_SPECIES_MCC : -1;
+    }
+}
+
+// The totals of slots i of samp_lane_rows for lane j.
+template <int L, bool MS, int NR>
+__device__ inline void samp_lane_values(uint32_t j, const double (&S)[NR], double (&v)[5])
+{
+    constexpr int B = SumRows<L, MS>::kBase;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) v[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+        if ((uint32_t)(q & 7) == j) v[q >> 3] = S[q];
+    if constexpr (MS) {
+#pragma unroll
+        for (int q = 0; q < kNS; ++q)
+            if ((uint32_t)q == j) { v[3] = S[B + q]; v[4] = S[B + kNS + q]; }
+    } else {
+        v[3] = S[UGX_S_N];
+        v[4] = S[UGX_S_MCC];
+    }
+}
+
+// Sampler rows of the first cell pass of a tile, loaded ahead of the sums.
+struct SampPre {
+    double v[5];
+    bool valid;
+};
+
+__device__ inline bool samples_now(const TileCase& cs)
+{
+    return cs.active && ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
+}
+
 template <int L, bool MS, int Block, class R, class Acc>
 __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nct, const uint32_t* s_off,
-                                 const Acc& A, const int32_t* s_cc, const TileCase* s_tc, int case0)
+                                 const Acc& A, const int32_t* s_cc, const TileCase* s_tc, int case0,
+                                 const SampPre& pre)
 {
     constexpr int B = SumRows<L, MS>::kBase, NR = SumRows<L, MS>::kN;
     const uint32_t tid = threadIdx.x;
@@ -1056,27 +1193,19 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         cell_partials<L, MS, R, Acc>(w, A, b + j, e, S);
 #pragma unroll
         for (int q = 0; q < NR; ++q) S[q] = group8_total(S[q]);
+        if (need && sample) {
+            // every lane of the group holds the totals; lane j updates its rows
+            int rows[5];
+            double v[5];
+            samp_lane_rows<L, MS>(j, cs.ns, cs.sampler_full != 0, rows);
+            samp_lane_values<L, MS>(j, S, v);
+            double* sc = &w.samp[(size_t)c * kNSamp];
+            const bool have = pre.valid && kb == 0u;
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (rows[i] >= 0) sc[rows[i]] = (have ? pre.v[i] : sc[rows[i]]) + dt * v[i];
+        }
         if (need && j == 0u) {
-            if (sample) {
-                const bool full = cs.sampler_full != 0;
-                double* sc = &w.samp[(size_t)c * kNSamp];
-#pragma unroll
-                for (int q = 0; q < B; ++q)
-                    if (full || q < UGX_S_MUU) sc[q] += dt * S[q];
-                if constexpr (MS) {
-#pragma unroll
-                    for (int q = 0; q < kNS; ++q) {
-                        // rows of species the case does not have stay zero
-                        if (q < cs.ns) {
-                            sc[UGX_S_SPECIES_N + q] += dt * S[B + q];
-                            sc[UGX_S_SPECIES_MCC + q] += dt * S[B + kNS + q];
-                        }
-                    }
-                } else {
-                    sc[UGX_S_SPECIES_N] += dt * S[UGX_S_N];
-                    sc[UGX_S_SPECIES_MCC] += dt * S[UGX_S_MCC];
-                }
-            }
             if (hybrid) {
                 double* ka = &w.knacc[(size_t)kKnRows * c];
 #pragma unroll
@@ -1151,6 +1280,7 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
     __shared__ R s_vy[kM], s_vz[kM], s_er[kM];
     __shared__ uint16_t s_g[kM];
     __shared__ int32_t s_cc[SortOnly ? 1 : TileN / kCellW + 2];
+    __shared__ uint32_t s_oc[TileN / kCellW + 2];
     __shared__ TileCase s_tc;
     __shared__ double s_spm[SortOnly ? 1 : kLdsSpecies];
     __shared__ uint8_t s_spr[SortOnly ? 1 : kLdsSpecies], s_spl[SortOnly ? 1 : kLdsSpecies];
@@ -1168,6 +1298,7 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
     // ... and everything else a tile reads before its gather comes in one more
     for (uint32_t k = tid; k <= nct; k += Block) {
         s_off[k] = w.off[c0 + k];
+        if (k < nct) s_oc[k] = w.ocnt[c0 + k];
         if (!SortOnly && k < nct) s_cc[k] = w.cell_case[c0 + k];
     }
     if (!SortOnly) {
@@ -1182,6 +1313,23 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
     if (size <= TileMax) {
         for (uint32_t i = tid; i < size; i += Block) s_in[i] = w.perm[p0 + i];
         __syncthreads();
+        // sampler rows of the first cell pass, loaded while the tile is sorted
+        SampPre spre{{0.0, 0.0, 0.0, 0.0, 0.0}, false};
+        if constexpr (!SortOnly) {
+            const uint32_t k = (tid >> 6) * 8u + ((tid & 63u) >> 3);
+            if (k < nct) {
+                const int ci = s_cc[k];
+                const TileCase cs = (ci == case0) ? s_tc : tile_case_of(w.cases[ci]);
+                if (samples_now(cs)) {
+                    int rows[5];
+                    samp_lane_rows<L, MS>(tid & 7u, cs.ns, cs.sampler_full != 0, rows);
+                    const double* sc = &w.samp[(size_t)(c0 + k) * kNSamp];
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) spre.v[i] = (rows[i] >= 0) ? sc[rows[i]] : 0.0;
+                    spre.valid = true;
+                }
+            }
+        }
         for (uint32_t i = tid; i < size; i += Block) {
             const uint32_t slot = s_in[i];
             // cell of entry i: the last k with s_off[k] <= p0 + i
@@ -1190,9 +1338,22 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
                 const uint32_t mid = (lo + hi + 1u) >> 1;
                 if (s_off[mid] <= p0 + i) lo = mid; else hi = mid - 1u;
             }
+            // the cell's simulators up to eo are in slot order already; those from
+            // eo on (multi-hop, inflow) are merged in by slot
             const uint32_t b = s_off[lo] - p0, e = s_off[lo + 1] - p0;
-            uint32_t r = 0;
-            for (uint32_t q = b; q < e; ++q) r += (s_in[q] < slot) ? 1u : 0u;
+            const uint32_t eo = e - s_oc[lo];
+            uint32_t r;
+            if (i < eo) {
+                r = i - b;
+            } else {
+                uint32_t l2 = b, h2 = eo;
+                while (l2 < h2) {
+                    const uint32_t mid = (l2 + h2) >> 1;
+                    if (s_in[mid] < slot) l2 = mid + 1u; else h2 = mid;
+                }
+                r = l2 - b;
+            }
+            for (uint32_t q = eo; q < e; ++q) r += (s_in[q] < slot) ? 1u : 0u;
             s_perm[b + r] = slot;
             s_k[b + r] = (uint8_t)lo;
         }
@@ -1215,7 +1376,7 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
         if (SortOnly) return;
         __syncthreads();
         LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_g, s_spm, s_spr, s_spl, lds_species, p0};
-        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
+        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, spre);
     } else {
         __syncthreads();
         // Tile larger than LDS: the same steps in HBM.  The rank of an entry
@@ -1254,7 +1415,7 @@ __global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
         __threadfence_block();
         __syncthreads();
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0);
+        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, SampPre{{0.0, 0.0, 0.0, 0.0, 0.0}, false});
     }
 }
 
