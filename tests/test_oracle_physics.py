@@ -10,6 +10,7 @@ Each test targets one restated function of the reference:
   BGK / ES-BGK limit          stochasticParticleBGK.C:782-796, stochasticParticleESBGK.C:887-906
   diffuse wall equilibrium    uniGasPatchBoundary.C:305-388
   inflow count                uniGasGeneralBoundary.C:115-169 (Bird eq. 4.22)
+  inflow velocities, E_rot    uniGasGeneralBoundary.C:537-761, uniGasCloud.C:975-1017
   ballistic tracking          uniGasParcel.C:35-108, uniGasPatchBoundary.C:390-403
   occupancy order             CloudWithModels.C:109-138
   sampler rows                cellMeasurements.C:408-513
@@ -268,6 +269,107 @@ def test_inflow_count_bird_4_22():
     frac = nbar - np.floor(nbar)
     mean = steps * nbar.sum()
     assert abs(ins - mean) <= 5.0 * math.sqrt(steps * (frac * (1 - frac)).sum()) + 1.0
+
+
+def _ref_equipartition_moments(dof):
+    """Mean and variance (units of kT) of the reference's E_rot draw for DoF
+    other than 2 (uniGasCloud.C:997-1013): eps is drawn once, then x is drawn
+    uniformly on [0, 10) until g(x) = (x/a)^a exp(a - x) >= eps, a = dof/2 - 1.
+    For a given eps, x is uniform on the interval where g >= eps (g is
+    unimodal with maximum 1 at x = a), so E[x] and E[x^2] are integrals over
+    eps of that interval's moments.  (The draw is not Gamma(dof/2): eps is not
+    redrawn per trial.  The oracle keeps this.)"""
+    from scipy.integrate import quad
+    from scipy.optimize import brentq
+    a = 0.5 * dof - 1.0
+    g = lambda x: (x / a) ** a * math.exp(a - x) if x > 0.0 else 0.0
+
+    def ends(eps):
+        lo = 0.0 if g(1e-300) >= eps else brentq(lambda x: g(x) - eps, 1e-300, a)
+        hi = 10.0 if g(10.0) >= eps else brentq(lambda x: g(x) - eps, a, 10.0)
+        return lo, hi
+
+    m1 = quad(lambda e: sum(ends(e)) / 2.0, 0.0, 1.0, limit=200)[0]
+    m2 = quad(lambda e: (lambda lo, hi: (lo * lo + lo * hi + hi * hi) / 3.0)(*ends(e)), 0.0, 1.0, limit=200)[0]
+    return m1, m2 - m1 * m1
+
+
+def _flux_normal_moments(s, lo, hi):
+    """Moments of the normal speed (units of the most probable speed) of
+    simulators crossing into the domain from a Maxwellian drifting at s along
+    the normal: density proportional to u exp(-(u - s)^2) on [lo, hi]."""
+    from scipy.integrate import quad
+    f = lambda u, k: u ** k * u * math.exp(-(u - s) ** 2)
+    z = quad(f, lo, hi, args=(0,))[0]
+    m1 = quad(f, lo, hi, args=(1,))[0] / z
+    m2 = quad(f, lo, hi, args=(2,))[0] / z
+    return m1, m2 - m1 * m1
+
+
+def _check_mean_var(x, mean, var, what, k=5.0):
+    n = len(x)
+    se_m = math.sqrt(var / n)
+    assert abs(x.mean() - mean) < k * se_m, f"{what}: mean {x.mean()} vs {mean} (se {se_m})"
+    m4 = np.mean((x - mean) ** 4)
+    se_v = math.sqrt(max(m4 - var * var, 0.0) / n)
+    assert abs(x.var() - var) < k * se_v, f"{what}: var {x.var()} vs {var} (se {se_v})"
+
+
+ROT3 = Species("X3", mass=2.66e-26, diameter=4.6e-10, omega=0.8, rotational_dof=3)
+
+
+@pytest.mark.parametrize("U", [(0.0, 0.0, 0.0), (200.0, 150.0, 0.0)])
+def test_inflow_velocity_distribution(U):
+    """Inflow of one step into an empty box: normal speed from the flux
+    distribution of a drifting Maxwellian (Bird eq. 12.5, rejection window
+    of three most probable speeds about the drift), tangential components
+    N(U_t, kT/m), E_rot exponential for 2 DoF and the reference's draw
+    otherwise (_ref_equipartition_moments).  Moments from closed forms and
+    quadrature, not from the oracle's formulas.
+    uniGasGeneralBoundary.C:537-761, uniGasCloud.C:975-1017."""
+    side, dt, fn, n_in, T = 0.01, 1e-6, 1.0e7, 1.0e20, 300.0
+    pt = {k: abi.PATCH for k in SIDES}
+    mesh = box_mesh((0.0, 0.0, 0.0), (side, side, side), (2, 2, 2), pt)
+    bnd = [Boundary("xmin", abi.BND_FREESTREAM, temperature=T, rot_temperature=T, velocity=U,
+                    number_density={"N2": 0.5 * n_in, "X3": 0.5 * n_in})] + \
+          [Boundary(p, abi.BND_DELETION) for p in SIDES]
+    spec = CaseSpec(mesh=mesh, species=[N2, ROT3], delta_t=dt, n_equivalent_particles=fn,
+                    collision_model="none", boundaries=bnd, seed=11)
+    o = orc.OracleCase(spec, capacity=1 << 19)
+    o.step(1)
+    p = o.particles()
+    assert p["cell"].size > 50000
+    for sid, sp in enumerate((N2, ROT3)):
+        sel = p["species"] == sid
+        v = p["velocity"][sel]
+        cmp_ = math.sqrt(2.0 * KB * T / sp.mass)
+        s = U[0] / cmp_
+        lo, hi = (max(0.0, s - 3.0), s + 3.0) if abs(U[0]) > 0.0 else (0.0, 40.0)
+        m, var = _flux_normal_moments(s, lo, hi)
+        _check_mean_var(v[:, 0] / cmp_, m, var, f"{sp.name} normal speed")
+        sig = math.sqrt(KB * T / sp.mass)
+        for d in (1, 2):
+            _check_mean_var(v[:, d], U[d], sig * sig, f"{sp.name} tangential {d}")
+        e = p["erot"][sel] / (KB * T)
+        k = 0.5 * sp.rotational_dof
+        em, ev = (1.0, 1.0) if sp.rotational_dof == 2 else _ref_equipartition_moments(sp.rotational_dof)
+        _check_mean_var(e, em, ev, f"{sp.name} E_rot")
+
+
+@pytest.mark.parametrize("dof", [3, 4])
+def test_fill_rotational_equipartition(dof):
+    """meshFill E_rot for rotational DoF other than 0 and 2: the reference's
+    draw (uniGasCloud.C:975-1017), whose moments _ref_equipartition_moments
+    computes by quadrature."""
+    sp = Species(f"R{dof}", mass=3.0e-26, diameter=4.2e-10, omega=0.75, rotational_dof=dof)
+    spec = box_case(n=(4, 4, 4), species=(sp,), fn=1e9, collision="none")
+    o = orc.OracleCase(spec, capacity=1 << 19)
+    T_rot = 420.0
+    o.fill({sp.name: 2.0e20}, T=300.0, rot_T=T_rot)
+    e = o.particles()["erot"] / (KB * T_rot)
+    assert e.size > 100000
+    m, v = _ref_equipartition_moments(dof)
+    _check_mean_var(e, m, v, f"E_rot dof {dof}")
 
 
 # ------------------------------------------------------------------ tracking
