@@ -144,6 +144,24 @@ def test_hybrid_nozzle_identical(require_gpu):
     assert 0 < ids.sum() < ids.size, "expected both collision models"
 
 
+def test_hybrid_cylinder_symmetry_identical(require_gpu):
+    """localKnudsen switching on the half cylinder (N2/O2, USP and DSMC): the
+    decomposition's smoothing mirrors the velocity at the symmetry planes."""
+    spec, fill = workloads.cylinder(n_theta=16, n_r=8, target=1.5e4, dt=2.0e-7)
+    spec.collision_model = "hybrid"
+    spec.bgk_model = "unifiedStochasticParticleSBGK"
+    spec.decomposition_interval = 3
+    spec.smoothing_passes = 2
+    spec.breakdown_max = 0.02
+    g, o = pair(spec, fill, capacity=1 << 17)
+    for k in range(3):
+        g.step(3)
+        o.step(3)
+        assert_hybrid_identical(g, o, f"cylinder hybrid, decomposition {k}")
+    ids = g.cell_model()
+    assert 0 < ids.sum() < ids.size, "expected both collision models"
+
+
 def test_hybrid_couette_cyclic_identical(require_gpu):
     """Hybrid Couette: cyclic faces in the smoothing, theta averaging over
     decompositions without a reset of the time sums."""

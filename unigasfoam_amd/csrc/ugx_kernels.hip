@@ -1887,6 +1887,45 @@ __device__ inline void kn_smooth_cell(const WorldView<R>& w, uint32_t c, const d
 }
 
 // Smoothing pass of rhoM, p, T (scalars) and U (vector); rhoN is carried.
+// Components 1-6 of a cell's decomposition record (rhoM, p, T scalars and
+// the U vector) in one pass over its faces: per component exactly the sums of
+// kn_smooth_cell, with the symmetry-plane mirror applied to U only.
+template <class R>
+__device__ inline void kn_smooth_fields(const WorldView<R>& w, uint32_t c, const double* in, double* out)
+{
+    double num[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, den = 0.0;
+    const double* fc = &in[8 * (size_t)c + 1];
+    double own[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) own[q] = fc[q];
+    for (int j = 0; j < kMaxFaces; ++j) {
+        const size_t k = (size_t)c * kMaxFaces + j;
+        const int kind = w.fkind[k];
+        if (kind == kFaceSkip) continue;
+        const KnFace kf = w.knf[k];
+        double fv[6];
+        if (kind == kFaceInternal || kind == kFaceCyclic) {
+            const double* fo = &in[8 * (size_t)(uint32_t)w.geom.nbr[k] + 1];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) fv[q] = kf.w * own[q] + (1.0 - kf.w) * fo[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) fv[q] = own[q];
+            if (kind == kFaceSymmetry) {
+                const double un = kf.nx * own[3] + kf.ny * own[4] + kf.nz * own[5];
+                fv[3] = own[3] - un * kf.nx;
+                fv[4] = own[4] - un * kf.ny;
+                fv[5] = own[5] - un * kf.nz;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) num[q] = num[q] + kf.area * fv[q];
+        den = den + kf.area;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) out[8 * (size_t)c + 1 + q] = (den > 0.0) ? num[q] / den : own[q];
+}
+
 template <class R>
 __global__ void k_kn_smooth(WorldView<R> w, const double* in, double* out, int pass)
 {
@@ -1895,10 +1934,7 @@ __global__ void k_kn_smooth(WorldView<R> w, const double* in, double* out, int p
         if (!kn_due(cs)) continue;
         if (pass < cs.smoothing_passes) {
             out[8 * (size_t)c] = in[8 * (size_t)c];
-            kn_smooth_cell<R>(w, c, in, out, 1, 1, 8);
-            kn_smooth_cell<R>(w, c, in, out, 2, 1, 8);
-            kn_smooth_cell<R>(w, c, in, out, 3, 1, 8);
-            kn_smooth_cell<R>(w, c, in, out, 4, 3, 8);
+            kn_smooth_fields<R>(w, c, in, out);
         } else {
             for (int q = 0; q < 7; ++q) out[8 * (size_t)c + q] = in[8 * (size_t)c + q];
         }
