@@ -1466,17 +1466,17 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
 
 // BGK-family relaxation (stochasticParticleBGK.C:287-796,
 // stochasticParticleESBGK.C:311-906, unifiedStochasticParticleSBGK.C:379-1096)
-// in three kernels so that the velocity draws run one simulator per lane:
+// in two kernels:
 //  k_collide   (lane per cell)  moments, Pr, nu, USP q/sigma time
 //              relaxation, the number of relaxed simulators and their
 //              selection (partial Fisher-Yates on the cell stream) into a
 //              selection list; parameters into a per-cell record;
-//  k_bgk_draw  (lane per selection) new velocity from the selection's own
-//              stream (kRngBgkV, sub = selection index); USP draws test
-//              against the cell's maxProb at the start of the step and
-//              record the largest raise;
-//  k_bgk_conserve (eight lanes per cell) conserveMomentumAndEnergy and the
-//              USP maxProb update.
+//  k_bgk_conserve (eight lanes per cell) new velocities of the cell's
+//              selections, lane j drawing selections j, j + 8, ... from
+//              each selection's own stream (kRngBgkV, sub = selection
+//              index), USP draws testing against the cell's maxProb at the
+//              start of the step and recording the largest raise; then
+//              conserveMomentumAndEnergy and the USP maxProb update.
 constexpr uint32_t kRngBgkV = 7;
 
 // Parameters and relaxed count of one cell; S holds the cell's sums.
@@ -1672,6 +1672,10 @@ __global__ void __launch_bounds__(kBlock) k_collide(WorldView<R> w)
         w.raise[c] = 0ull;
     }
     const uint32_t base = wave_reserve(&w.st->nsel, nrel);
+    if (bgk) {
+        w.bgkp[c].sel_base = base;
+        w.bgkp[c].nrel = nrel;
+    }
     if (nrel) bgk_select<R>(w, w.cases[ccase], c, b, nC, nrel, base, A);
     if (dsmc && nC >= 2u) ntc_cell<R>(w, w.cases[ccase], c, b, nC, A, cand, coll);
     case_add(w.counters, ccase, kCntCandidates, cand);
@@ -1720,17 +1724,6 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
     A.store(i, P.U[0] + u0 * xi[0], P.U[1] + u0 * xi[1], P.U[2] + u0 * xi[2]);
 }
 
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_bgk_draw(WorldView<R> w)
-{
-    const uint32_t ns = w.st->nsel;
-    GlobAcc<R> A{w.b, w.sp.local, w.perm};
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
-        const uint32_t c = w.sel_cell[k];
-        bgk_draw<R>(w, w.cases[w.cell_case[c]], c, w.sel_slot[k], w.sel_k[k], w.bgkp[c], &w.raise[c], A);
-    }
-}
-
 // conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
 // canonical sums, and the USP maxProb update; eight lanes per cell.
 template <class R>
@@ -1740,17 +1733,21 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     const uint32_t lane = tid & 63u, g = lane >> 3, j = lane & 7u;
     const uint32_t c = (blockIdx.x * blockDim.x + tid) >> 3;
     bool perform = false, usp = false;
-    uint32_t b = 0, e = 0;
+    uint32_t b = 0, e = 0, sb = 0, nrel = 0;
+    int ccase = 0;
     double T = 0.0, U0 = 0.0, U1 = 0.0, U2 = 0.0, M = 0.0, N = 0.0;
     (void)g;
     if (c < (uint32_t)w.nc) {
-        const CaseDev& cs = w.cases[w.cell_case[c]];
+        ccase = w.cell_case[c];
+        const CaseDev& cs = w.cases[ccase];
         const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
         const bool bgk = cs.active && (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
         if (bgk) {
             const BgkP& P = w.bgkp[c];
             perform = P.perform != 0;
             usp = P.model == UGX_BGK_USP;
+            sb = P.sel_base;
+            nrel = P.nrel;
             if (perform) {
                 b = w.off[c];
                 e = w.off[c + 1];
@@ -1760,6 +1757,17 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
             }
         }
     }
+    // the cell's selections get their new velocities first, lane j taking
+    // selections j, j + 8, ... (each from its own stream)
+    if (nrel) {
+        GlobAcc<R> A{w.b, w.sp.local, w.perm};
+        const CaseDev& cs = w.cases[ccase];
+        const BgkP& P = w.bgkp[c];
+        for (uint32_t k = j; k < nrel; k += 8u)
+            bgk_draw<R>(w, cs, c, w.sel_slot[sb + k], k, P, &w.raise[c], A);
+    }
+    __threadfence_block();
+    __syncthreads();
     if (__ballot(perform) != 0ull) {
         double ke = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
         // four simulators per lane per chunk, loaded before use (indices clamped
@@ -2406,7 +2414,6 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
         const unsigned gcell = (unsigned)(((uint64_t)w.nc + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_collide<R>, dim3(gcell), dim3(kBlock), 0, s, w);
         if (w.has_bgk) {
-            hipLaunchKernelGGL(k_bgk_draw<R>, dim3(grid_for(w.cap, 16384)), dim3(kBlock), 0, s, w);
             const unsigned gcons = (unsigned)(((uint64_t)w.nc * 8u + kBlock - 1) / kBlock);
             hipLaunchKernelGGL(k_bgk_conserve<R>, dim3(gcons), dim3(kBlock), 0, s, w);
         }

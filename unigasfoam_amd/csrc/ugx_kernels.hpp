@@ -27,6 +27,8 @@ struct BgkP {
     double sm[9];      // ES-BGK S matrix (row major) or USP stress (xx xy xz yy yz zz)
     int32_t perform;
     int32_t model;
+    uint32_t sel_base;  // first entry of the cell's selections in the selection list
+    uint32_t nrel;      // number of selections
 };
 
 // Per-case cumulative counters (uint64).

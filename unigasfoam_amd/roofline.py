@@ -8,7 +8,7 @@ Per simulator record: x, y, z, vx, vy, vz, E_rot (7 R) + cell + species
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s (1 GB = 1e9 B)
 N_BASE_ROWS = 8                # sampler rows 0-7 (N, M, M v, M |v|^2, E_rot, rotDoF)
 N_MOM_ROWS = 28                # per-cell sums kept for the BGK family
-BGK_PARAMS = 176               # per-cell BGK parameter record
+BGK_PARAMS = 184               # per-cell BGK parameter record
 
 
 def record_bytes(R):
