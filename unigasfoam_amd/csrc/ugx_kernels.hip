@@ -1205,26 +1205,31 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
             for (int i = 0; i < 5; ++i)
                 if (rows[i] >= 0) sc[rows[i]] = (have ? pre.v[i] : sc[rows[i]]) + dt * v[i];
         }
-        if (need && j == 0u) {
-            if (hybrid) {
-                double* ka = &w.knacc[(size_t)kKnRows * c];
+        // lane j of the group writes rows j, j + 8, j + 16 and species j
+        double v[3] = {0.0, 0.0, 0.0}, vsp = 0.0;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) ka[q] += dt * S[q];
-                if constexpr (MS) {
+        for (int q = 0; q < B; ++q)
+            if ((uint32_t)(q & 7) == j) v[q >> 3] = S[q];
+        if constexpr (MS) {
 #pragma unroll
-                    for (int q = 0; q < kNS; ++q) ka[8 + q] += dt * S[B + q];
-                } else {
-                    ka[8] += dt * S[UGX_S_N];
-                }
+            for (int q = 0; q < kNS; ++q)
+                if ((uint32_t)q == j) vsp = S[B + q];
+        } else {
+            vsp = (j == 0u) ? S[UGX_S_N] : 0.0;
+        }
+        if (need && hybrid) {
+            double* ka = &w.knacc[(size_t)kKnRows * c];
+            ka[j] += dt * v[0];
+            if (MS || j == 0u) ka[8 + j] += dt * vsp;
+        }
+        if (need && bgk) {
+            double* mo = &w.mom[(size_t)c * kNMom];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const uint32_t q = j + 8u * (uint32_t)i;
+                if (q < (uint32_t)UGX_S_BASE) mo[q] = v[i];
             }
-            if (bgk) {
-                double* mo = &w.mom[(size_t)c * kNMom];
-#pragma unroll
-                for (int q = 0; q < UGX_S_BASE; ++q) mo[q] = (q < B) ? S[q < B ? q : 0] : 0.0;
-#pragma unroll
-                for (int q = 0; q < kNS; ++q)
-                    mo[UGX_S_SPECIES_N + q] = MS ? S[MS ? B + q : 0] : ((q == 0) ? S[UGX_S_N] : 0.0);
-            }
+            mo[UGX_S_SPECIES_N + j] = vsp;
         }
     }
 }
