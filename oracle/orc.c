@@ -707,6 +707,42 @@ static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, 
     w->smax[c] = smax;
 }
 
+/* unifiedStochasticParticleSBGK::samplePostCollisionVelocity
+ * (unifiedStochasticParticleSBGK.C:1047-1096): xi ~ N(0, I/2) accepted with
+ * P(xi) = 1 + cQ/(p u0) (q.xi)(|xi|^2/2.5 - 1) + cS/p sigma:(xi xi - |xi|^2/3 I)
+ * against the cell's maxProb m0; a draw with m0 < P < 10 is taken at once and
+ * raises maxProb (largest such P in *raise).  sh: xx xy xz yy yz zz. */
+static void usp_draw(orc_rng* rv, double cQ, double cS, double p, double u0, const double q[3], const double sh[6],
+                     double m0, double* raise, double xi[3])
+{
+    for (int it = 0; it < 100000; ++it) {
+        double g[3];
+        orc_gauss3(rv, g);
+        for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
+        double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        double vTr = vSq / 3.0;
+        double prob = 1.0 + cQ / (p * u0) * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq / 2.5 - 1.0)
+                      + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
+                                  + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
+                                  + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
+        if (prob > m0 && prob < 10.0) { if (prob > *raise) *raise = prob; break; }
+        if (orc_uniform(rv) < prob / m0) break;
+    }
+}
+
+/* Test hook: n USP draws (streams {i, 0, 0} of the BGK velocity kind). */
+int orc_usp_sample(double cQ, double cS, double p, double u0, const double* q, const double* sh, double m0,
+                   int64_t n, uint64_t seed, double* xi, double* raise)
+{
+    *raise = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        orc_rng rv;
+        orc_rng_init(&rv, seed, ORC_RNG_BGK_V, (uint32_t)i, 0u, 0u);
+        usp_draw(&rv, cQ, cS, p, u0, q, sh, m0, raise, &xi[3 * i]);
+    }
+    return UGX_OK;
+}
+
 /* BGK-family relaxation for one cell: calculateProperties
  * (stochasticParticleBGK.C:287-583, stochasticParticleESBGK.C:311-665,
  * unifiedStochasticParticleSBGK.C:379-800), collide (:628-729, :721-834,
@@ -828,19 +864,7 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
             double u0 = s2kT / sqrt(w->mass[w->sp[pi]]);
             double xi[3];
             if (model == UGX_BGK_USP) {
-                for (int it = 0; it < 100000; ++it) {
-                    double g[3];
-                    orc_gauss3(&rv, g);
-                    for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
-                    double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-                    double vTr = vSq / 3.0;
-                    double prob = 1.0 + cQ / (p * u0) * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq / 2.5 - 1.0)
-                                  + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
-                                              + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
-                                              + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
-                    if (prob > m0 && prob < 10.0) { if (prob > raise) raise = prob; break; }
-                    if (orc_uniform(&rv) < prob / m0) break;
-                }
+                usp_draw(&rv, cQ, cS, p, u0, q, sh, m0, &raise, xi);
             } else {
                 double g[3];
                 orc_gauss3(&rv, g);

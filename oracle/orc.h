@@ -61,6 +61,13 @@ void orc_mathfn(int fn, const double* in, const double* in2, double* out, int64_
 void orc_stream(uint64_t seed, uint32_t kind, uint32_t sub, uint32_t id, uint32_t step, int what,
                 uint32_t m, double* out, int64_t n);
 
+/* Test hook: n USP velocity draws (unifiedStochasticParticleSBGK.C:1047-1096)
+ * for the given cQ, cS, p, most probable speed u0, heat flux q[3], stress
+ * sh[6] (xx xy xz yy yz zz) and maxProb m0; xi[3n] are the draws in units of
+ * u0, *raise the largest accepted P above m0 (0 if none). */
+int orc_usp_sample(double cQ, double cS, double p, double u0, const double* q, const double* sh, double m0,
+                   int64_t n, uint64_t seed, double* xi, double* raise);
+
 #ifdef __cplusplus
 }
 #endif

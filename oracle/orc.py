@@ -58,6 +58,8 @@ def lib():
         L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
         L.orc_face_data.argtypes = [vp, abi.DBL_P, abi.I32_P]
         L.orc_inflow_nbar.argtypes = [vp, abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_usp_sample.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, abi.DBL_P, abi.DBL_P,
+                                     C.c_double, C.c_int64, C.c_uint64, abi.DBL_P, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -65,6 +67,17 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise OracleError(f"oracle error {rc}: {lib().orc_last_error().decode()}")
+
+
+def usp_sample(cQ, cS, p, u0, q, sigma, m0, n, seed=1):
+    """n USP velocity draws (units of u0) and the largest raise of maxProb."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    sh = np.ascontiguousarray(sigma, dtype=np.float64)
+    xi = np.zeros((int(n), 3))
+    raise_ = C.c_double(0.0)
+    _check(lib().orc_usp_sample(cQ, cS, p, u0, abi.dptr(q), abi.dptr(sh), m0, int(n), int(seed),
+                                abi.dptr(xi), C.byref(raise_)))
+    return xi, raise_.value
 
 
 def set_threads(n):

@@ -8,6 +8,7 @@ Each test targets one restated function of the reference:
   LB equilibrium              uniGasCloud.C:1129-1189
   BGK-family conservation     stochasticParticleBGK.C:731-780 (and ES-BGK / USP siblings)
   BGK / ES-BGK limit          stochasticParticleBGK.C:782-796, stochasticParticleESBGK.C:887-906
+  USP draw moments            unifiedStochasticParticleSBGK.C:1047-1096
   diffuse wall equilibrium    uniGasPatchBoundary.C:305-388
   inflow count                uniGasGeneralBoundary.C:115-169 (Bird eq. 4.22)
   inflow velocities, E_rot    uniGasGeneralBoundary.C:537-761, uniGasCloud.C:975-1017
@@ -227,6 +228,30 @@ def test_specular_box_conserves_n_and_energy():
     p1 = o.particles()
     assert len(p1["cell"]) == len(p0["cell"])
     assert abs((p1["velocity"] ** 2).sum() - e0) <= 1e-13 * e0
+
+
+def test_usp_sampling_moments():
+    """USP draw (unifiedStochasticParticleSBGK.C:1047-1096): accepted xi follow
+    pi^-3/2 exp(-|xi|^2) P(xi) with P = 1 + A (q.xi)(|xi|^2/2.5 - 1) +
+    B sigma:(xi xi - |xi|^2/3 I), A = cQ/(p u0), B = cS/p, as long as maxProb
+    bounds P.  Gaussian moments (E[x^2] = 1/2, E[x^4] = 3/4, E[x^6] = 15/8)
+    give E[xi] = 0, E[xi xi] = I/2 + B (sigma/2 - tr(sigma)/6 I) and
+    E[xi |xi|^2] = A q / 2."""
+    cQ, cS, p, u0, m0 = 0.6, 0.5, 2.0, 1.0, 4.0
+    q = np.array([0.10, -0.05, 0.08])
+    sh = np.array([0.10, 0.12, -0.10, -0.04, 0.08, -0.06])       # xx xy xz yy yz zz
+    A, B = cQ / (p * u0), cS / p
+    n = 4_000_000
+    xi, raised = orc.usp_sample(cQ, cS, p, u0, q, sh, m0, n, seed=9)
+    assert raised == 0.0                       # maxProb bounded every P
+    S = np.array([[sh[0], sh[1], sh[2]], [sh[1], sh[3], sh[4]], [sh[2], sh[4], sh[5]]])
+    se = 5.0 / math.sqrt(n)
+    assert np.all(np.abs(xi.mean(0)) < se * math.sqrt(0.5))
+    second = xi.T @ xi / n
+    want2 = 0.5 * np.eye(3) + B * (0.5 * S - np.trace(S) / 6.0 * np.eye(3))
+    assert np.all(np.abs(second - want2) < se * 1.0), (second, want2)
+    third = (xi * (xi * xi).sum(1)[:, None]).mean(0)
+    assert np.all(np.abs(third - 0.5 * A * q) < se * math.sqrt(35.0 / 8.0)), (third, 0.5 * A * q)
 
 
 # ------------------------------------------------------------------ inflow
