@@ -1606,7 +1606,14 @@ __device__ inline void bgk_params(const WorldView<R>& w, const CaseDev& cs, uint
 
 // Partial Fisher-Yates selection of nrel of the cell's nC simulators on the
 // cell stream (after the relaxed-count draw); entries go to the selection
-// list at base.
+// list at base.  Up to kSelReg selections run on a register list of the
+// displaced entries: after step i only positions above i are read again,
+// and each step displaces at most one entry (to position j > i), so the
+// list holds the (position, entry) pairs that differ from the identity and
+// a lookup falls back to the position itself.  Larger selections swap in
+// the cell's index array.
+constexpr int kSelReg = 16;
+
 template <class R>
 __device__ inline void bgk_select(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
                                   uint32_t nrel, uint32_t base, const GlobAcc<R>& A)
@@ -1614,6 +1621,33 @@ __device__ inline void bgk_select(const WorldView<R>& w, const CaseDev& cs, uint
     Rng r;
     rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
     (void)uniform(r);
+    if (nrel <= (uint32_t)kSelReg) {
+        uint32_t pos[kSelReg], val[kSelReg];
+        uint32_t nd = 0u;
+        for (uint32_t i = 0; i < nrel; ++i) {
+            const uint32_t j = i + uindex(r, nC - i);
+            uint32_t ti = i, tj = j;
+            bool have_j = false;
+#pragma unroll
+            for (int q = 0; q < kSelReg; ++q) {
+                if ((uint32_t)q < nd) {
+                    if (pos[q] == i) ti = val[q];
+                    if (pos[q] == j) { tj = val[q]; have_j = true; }
+                }
+            }
+            if (j != i) {
+                // position j now holds the entry of position i
+#pragma unroll
+                for (int q = 0; q < kSelReg; ++q) {
+                    if ((uint32_t)q < nd && pos[q] == j) val[q] = ti;
+                    if (!have_j && (uint32_t)q == nd) { pos[q] = j; val[q] = ti; }
+                }
+                if (!have_j) ++nd;
+            }
+            w.sel_slot[base + i] = b + tj;
+        }
+        return;
+    }
     for (uint32_t i = 0; i < nC; ++i) A.idx_set(b, i, i);
     for (uint32_t i = 0; i < nrel; ++i) {
         const uint32_t j = i + uindex(r, nC - i);
@@ -1621,8 +1655,6 @@ __device__ inline void bgk_select(const WorldView<R>& w, const CaseDev& cs, uint
         A.idx_set(b, i, tj);
         A.idx_set(b, j, ti);
         w.sel_slot[base + i] = b + tj;
-        w.sel_cell[base + i] = c;
-        w.sel_k[base + i] = i;
     }
 }
 
@@ -1731,6 +1763,7 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
 
 // conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
 // canonical sums, and the USP maxProb update; eight lanes per cell.
+constexpr int kConsReg = 8;   // simulators per lane kept in registers between the sums and the rescale
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
 {
@@ -1775,23 +1808,46 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     __syncthreads();
     if (__ballot(perform) != 0ull) {
         double ke = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
-        // four simulators per lane per chunk, loaded before use (indices clamped
-        // into the cell), accumulated in slot order
-        for (uint32_t i0 = b + j; i0 < e; i0 += 32u) {
-            double mm[4], ux[4], uy[4], uz[4];
+        // the first kConsReg simulators of each lane (cells up to 8 kConsReg)
+        // are loaded once and kept for the rescale; the rest of a larger cell
+        // is read again.  Accumulation in slot order.
+        R rx[kConsReg], ry[kConsReg], rz[kConsReg];
+        double mm[kConsReg];
+#pragma unroll
+        for (int u = 0; u < kConsReg; ++u) {
+            const uint32_t i = b + j + 8u * (uint32_t)u;
+            rx[u] = R(0); ry[u] = R(0); rz[u] = R(0); mm[u] = 0.0;
+            if (i < e) {
+                rx[u] = w.b.vx[i]; ry[u] = w.b.vy[i]; rz[u] = w.b.vz[i];
+                mm[u] = w.sp.mass[w.b.sp[i]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kConsReg; ++u) {
+            if (b + j + 8u * (uint32_t)u < e) {
+                const double ux = (double)rx[u], uy = (double)ry[u], uz = (double)rz[u];
+                ke = ke + mm[u] * (ux * ux + uy * uy + uz * uz);
+                m0 = m0 + mm[u] * ux;
+                m1 = m1 + mm[u] * uy;
+                m2 = m2 + mm[u] * uz;
+            }
+        }
+        const uint32_t bt = b + 8u * (uint32_t)kConsReg;
+        for (uint32_t i0 = bt + j; i0 < e; i0 += 32u) {
+            double tm[4], ux[4], uy[4], uz[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const uint32_t i = min(i0 + 8u * u, e - 1u);
-                mm[u] = w.sp.mass[w.b.sp[i]];
+                tm[u] = w.sp.mass[w.b.sp[i]];
                 ux[u] = (double)w.b.vx[i]; uy[u] = (double)w.b.vy[i]; uz[u] = (double)w.b.vz[i];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (i0 + 8u * u < e) {
-                    ke = ke + mm[u] * (ux[u] * ux[u] + uy[u] * uy[u] + uz[u] * uz[u]);
-                    m0 = m0 + mm[u] * ux[u];
-                    m1 = m1 + mm[u] * uy[u];
-                    m2 = m2 + mm[u] * uz[u];
+                    ke = ke + tm[u] * (ux[u] * ux[u] + uy[u] * uy[u] + uz[u] * uz[u]);
+                    m0 = m0 + tm[u] * ux[u];
+                    m1 = m1 + tm[u] * uy[u];
+                    m2 = m2 + tm[u] * uz[u];
                 }
             }
         }
@@ -1804,7 +1860,16 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
             const double pT = (ke - M * (pU0 * pU0 + pU1 * pU1 + pU2 * pU2)) / (3.0 * kB * (N - 1.0));
             if (pT > kVSmall) {
                 const double sc = sqrt(T / pT);
-                for (uint32_t i = b + j; i < e; i += 8u) {
+#pragma unroll
+                for (int u = 0; u < kConsReg; ++u) {
+                    const uint32_t i = b + j + 8u * (uint32_t)u;
+                    if (i < e) {
+                        w.b.vx[i] = (R)(U0 + ((double)rx[u] - pU0) * sc);
+                        w.b.vy[i] = (R)(U1 + ((double)ry[u] - pU1) * sc);
+                        w.b.vz[i] = (R)(U2 + ((double)rz[u] - pU2) * sc);
+                    }
+                }
+                for (uint32_t i = bt + j; i < e; i += 8u) {
                     const double vx = (double)w.b.vx[i], vy = (double)w.b.vy[i], vz = (double)w.b.vz[i];
                     w.b.vx[i] = (R)(U0 + (vx - pU0) * sc);
                     w.b.vy[i] = (R)(U1 + (vy - pU1) * sc);
@@ -2093,25 +2158,67 @@ __global__ void __launch_bounds__(64) k_nb_build(WorldView<R> w, uint32_t* cnt, 
     }
 }
 
+static_assert(kMaxFaces <= 8, "one lane of the group per face");
+
+// Sum of v over the eight lanes of a group (lanes 8g .. 8g+7); called with
+// all 64 lanes active.
+__device__ inline uint32_t group8_sum_u32(uint32_t v)
+{
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    return v;
+}
+
 // One refinement loop (uniGasHybridDecomposition.C:73-75; localKnudsen.C:436-545)
-// for the cells of model `me` (1 DSMC, 0 BGK), deciding from ids2.
+// for the cells of model `me` (1 DSMC, 0 BGK), deciding from ids2.  Eight
+// lanes per cell: lane j < 6 looks at face j, and lane j counts the
+// neighbourhood entries j, j + 8, ... (the eight lanes read consecutive
+// entries of the list); every lane of the wave runs every iteration.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_kn_refine(WorldView<R> w, int me)
 {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
-        const CaseDev& cs = w.cases[w.cell_case[c]];
-        if (!kn_due(cs) || w.ids2[c] != me) continue;
-        int same = 0, other = 0;
-        for (int j = 0; j < kMaxFaces; ++j) {
-            const size_t k = (size_t)c * kMaxFaces + j;
-            if (w.fkind[k] != kFaceInternal) continue;
-            if (w.ids2[w.geom.nbr[k]] == me) same++; else other++;
+    const uint32_t j = threadIdx.x & 7u;
+    const uint32_t nc = (uint32_t)w.nc;
+    const uint32_t first = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) >> 3;
+    const uint32_t stride = (gridDim.x * blockDim.x) >> 3;
+    for (uint32_t c0 = first; c0 < nc; c0 += stride) {
+        const uint32_t c = c0 + ((threadIdx.x & 63u) >> 3);
+        bool on = c < nc;
+        if (on) {
+            const CaseDev& cs = w.cases[w.cell_case[c]];
+            on = kn_due(cs) && w.ids2[c] == me;
         }
-        if (same == 0 || (same == 1 && other > 1)) { w.cmodel[c] = 1 - me; continue; }
-        const uint32_t b = w.nb_off[c], n = w.nb_off[c + 1] - b;
-        int cnt = 0;
-        for (uint32_t q = 0; q < n; ++q) cnt += (w.ids2[w.nb_list[b + q]] == me) ? 1 : 0;
-        if ((double)cnt < 0.4 * (double)n) w.cmodel[c] = 1 - me;
+        uint32_t same = 0u, other = 0u;
+        if (on && j < (uint32_t)kMaxFaces) {
+            const size_t k = (size_t)c * kMaxFaces + j;
+            if (w.fkind[k] == kFaceInternal) {
+                if (w.ids2[w.geom.nbr[k]] == me) same = 1u; else other = 1u;
+            }
+        }
+        same = group8_sum_u32(same);
+        other = group8_sum_u32(other);
+        const bool flip_faces = same == 0u || (same == 1u && other > 1u);
+        uint32_t b = 0u, n = 0u, cnt = 0u;
+        if (on && !flip_faces) {
+            b = w.nb_off[c];
+            n = w.nb_off[c + 1] - b;
+        }
+        for (uint32_t q = j; q < n; q += 32u) {
+            int32_t id[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t qq = q + 8u * (uint32_t)u;
+                id[u] = (qq < n) ? w.ids2[w.nb_list[b + qq]] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cnt += (id[u] == me) ? 1u : 0u;
+        }
+        cnt = group8_sum_u32(cnt);
+        if (on && j == 0u) {
+            if (flip_faces) w.cmodel[c] = 1 - me;
+            else if ((double)cnt < 0.4 * (double)n) w.cmodel[c] = 1 - me;
+        }
     }
 }
 
@@ -2308,10 +2415,11 @@ static void launch_decompose(const WorldView<R>& w, hipStream_t s)
         std::swap(k0, k1);
     }
     hipLaunchKernelGGL(k_kn_decide<R>, dim3(g), dim3(kBlock), 0, s, w, (const double*)k0);
+    const unsigned g8 = grid_for((uint64_t)w.nc * 8u);
     for (int pass = 0; pass < 3; ++pass)
         for (int me = 1; me >= 0; --me) {
             hipLaunchKernelGGL(k_kn_snapshot<R>, dim3(g), dim3(kBlock), 0, s, w);
-            hipLaunchKernelGGL(k_kn_refine<R>, dim3(g), dim3(kBlock), 0, s, w, me);
+            hipLaunchKernelGGL(k_kn_refine<R>, dim3(g8), dim3(kBlock), 0, s, w, me);
         }
     hipLaunchKernelGGL(k_kn_reset<R>, dim3(g), dim3(kBlock), 0, s, w);
     hipLaunchKernelGGL(k_kn_end<R>, dim3(grid_for((uint64_t)w.ncases)), dim3(kBlock), 0, s, w);

@@ -79,9 +79,7 @@ struct WorldView {
     double* mom;              // [nc][kNMom] per-cell sums for the BGK family
     BgkP* bgkp;               // [nc]
     unsigned long long* raise;// [nc] USP maxProb raise (bits of a positive double)
-    uint32_t* sel_slot;       // [cap] BGK selections: sorted slot,
-    uint32_t* sel_cell;       //       cell,
-    uint32_t* sel_k;          //       selection index in the cell
+    uint32_t* sel_slot;       // [cap] BGK selections: sorted slot of each (cell-major, in selection order)
     int32_t ns_max;           // largest species count of a case
     int32_t sum_level;        // largest sampler level of a case (0, 1, 2)
     int32_t has_collide;      // some case collides (DSMC or BGK family)

@@ -322,7 +322,7 @@ void World::release()
                      (void**)&d_perm, (void**)&d_ins_off, (void**)&d_scratch,
                      (void**)&d_tile_first, (void**)&d_tile_p0, (void**)&d_tile_case,
                      (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
-                     (void**)&d_sel_slot, (void**)&d_sel_cell, (void**)&d_sel_k,
+                     (void**)&d_sel_slot,
                      (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st,
                      (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
@@ -385,8 +385,6 @@ int World::alloc_particles(uint32_t newcap)
         d_tile_case = tc;
     }
     dfree(d_sel_slot);
-    dfree(d_sel_cell);
-    dfree(d_sel_k);
     dfree(d_wq_slot);
     dfree(d_wq_info);
     dfree(d_wq_tau);
@@ -394,8 +392,6 @@ int World::alloc_particles(uint32_t newcap)
     if (int rc = dalloc(d_scratch, newcap)) return rc;
     if (has_bgk) {
         if (int rc = dalloc(d_sel_slot, newcap)) return rc;
-        if (int rc = dalloc(d_sel_cell, newcap)) return rc;
-        if (int rc = dalloc(d_sel_k, newcap)) return rc;
     }
     // diffuse-wall queue: at most half of the simulators may reach a diffuse wall in one step
     wq_cap = std::max<uint32_t>(newcap / 2, 1024u);
@@ -480,8 +476,6 @@ WorldView<R> World::view() const
     w.bgkp = d_bgkp;
     w.raise = d_raise;
     w.sel_slot = d_sel_slot;
-    w.sel_cell = d_sel_cell;
-    w.sel_k = d_sel_k;
     w.ns_max = ns_max;
     w.sum_level = sum_level;
     w.has_collide = has_collide;
@@ -508,10 +502,10 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreate(&stream));  // blocking: ordered with null-stream copies
     nc = 0;
-    int nsp = 0, npair = 0, nbm = 0;
-    for (auto& s : specs) { nc += s.geom.nc; nsp += s.ns; npair += s.ns * s.ns; nbm += (int)s.geom.bm.size(); }
+    int nsp = 0, npair = 0;
+    for (auto& s : specs) { nc += s.geom.nc; nsp += s.ns; npair += s.ns * s.ns; }
     move_grid = (precision == UGX_FP32_STATE) ? move_blocks<float>(device) : move_blocks<double>(device);
-    if (move_grid == 0) return UGX_ERR_DEVICE;
+    if (move_grid == 0) return set_err(UGX_ERR_DEVICE, "occupancy query of the move kernel failed");
     // cell tiles keep a 16-bit species id per simulator
     if (nsp > 65535) return UGX_ERR_CAPACITY;
     nsp_total = nsp;

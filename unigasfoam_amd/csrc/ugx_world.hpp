@@ -98,8 +98,6 @@ struct World {
     BgkP* d_bgkp = nullptr;
     unsigned long long* d_raise = nullptr;
     uint32_t* d_sel_slot = nullptr;
-    uint32_t* d_sel_cell = nullptr;
-    uint32_t* d_sel_k = nullptr;
     int32_t ns_max = 1, sum_level = 0, has_collide = 0, has_bgk = 0;
     uint32_t* d_blocksum = nullptr;
     unsigned long long* d_counters = nullptr;
