@@ -82,9 +82,11 @@ __device__ inline void case_add(unsigned long long* counters, int ccase, int idx
 // Per-thread counters of one case, reduced over the block at kernel end so
 // that a kernel issues one global atomic per counter per block instead of
 // one per wave.  Slots: 0 deleted, 1 wall hits, 2 track errors, 3 inserted.
+// The per-thread counts are 32-bit and go to the global counters before
+// one of them could pass 2^31.
 struct TCount {
     int ccase;
-    unsigned long long v[4];
+    uint32_t v[4];
 };
 
 __device__ inline void tcount_flush_global(const TCount& t, unsigned long long* counters)
@@ -92,16 +94,16 @@ __device__ inline void tcount_flush_global(const TCount& t, unsigned long long* 
     const int idx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
     if (t.ccase < 0) return;
     for (int k = 0; k < 4; ++k)
-        if (t.v[k]) atomicAdd(&counters[(size_t)t.ccase * kCntNum + idx[k]], t.v[k]);
+        if (t.v[k]) atomicAdd(&counters[(size_t)t.ccase * kCntNum + idx[k]], (unsigned long long)t.v[k]);
 }
 
-__device__ inline void tcount_add(TCount& t, unsigned long long* counters, int ccase, int k, unsigned long long v)
+__device__ inline void tcount_add(TCount& t, unsigned long long* counters, int ccase, int k, uint32_t v)
 {
     if (!v) return;
-    if (t.ccase != ccase) {
+    if (t.ccase != ccase || t.v[k] >= 0x80000000u - v) {
         tcount_flush_global(t, counters);
         t.ccase = ccase;
-        t.v[0] = t.v[1] = t.v[2] = t.v[3] = 0ull;
+        t.v[0] = t.v[1] = t.v[2] = t.v[3] = 0u;
     }
     t.v[k] += v;
 }
@@ -116,18 +118,23 @@ __device__ inline void tcount_block_flush(const TCount& t, unsigned long long* c
     for (int k = 0; k < 4; ++k) s_v[tid][k] = t.v[k];
     __syncthreads();
     if (tid == 0) {
-        TCount sum{-1, {0ull, 0ull, 0ull, 0ull}};
-        for (uint32_t i = 0; i < kBlock; ++i) {
-            if (s_case[i] < 0) continue;
-            if (sum.ccase < 0) sum.ccase = s_case[i];
-            if (s_case[i] == sum.ccase) {
-                for (int k = 0; k < 4; ++k) sum.v[k] += s_v[i][k];
-            } else {
-                TCount o{s_case[i], {s_v[i][0], s_v[i][1], s_v[i][2], s_v[i][3]}};
-                tcount_flush_global(o, counters);
+        const int idx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
+        int sc = -1;
+        unsigned long long sum[4] = {0ull, 0ull, 0ull, 0ull};
+        for (uint32_t i = 0; i <= kBlock; ++i) {
+            const int ci = (i < kBlock) ? s_case[i] : -1;
+            if (i < kBlock && ci < 0) continue;
+            if (ci != sc || i == kBlock) {
+                // a run of one case ends: its sums go out
+                if (sc >= 0)
+                    for (int k = 0; k < 4; ++k)
+                        if (sum[k]) atomicAdd(&counters[(size_t)sc * kCntNum + idx[k]], sum[k]);
+                if (i == kBlock) break;
+                sc = ci;
+                for (int k = 0; k < 4; ++k) sum[k] = 0ull;
             }
+            for (int k = 0; k < 4; ++k) sum[k] += s_v[i][k];
         }
-        tcount_flush_global(sum, counters);
     }
 }
 
@@ -245,7 +252,7 @@ __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, cons
 template <class R>
 __global__ void k_inflow_gen(WorldView<R> w)
 {
-    TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
+    TCount tc{-1, {0u, 0u, 0u, 0u}};
     const uint32_t total = w.st->n_ins;
     const uint32_t n0 = w.st->n;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -278,10 +285,10 @@ __global__ void k_inflow_gen(WorldView<R> w)
             w.a.sp[j] = (uint32_t)e.species;
         }
         if (active) {
-            tcount_add(tc, w.counters, ccase, 3, 1ull);
-            tcount_add(tc, w.counters, ccase, 0, (cell != kDead) ? 0ull : 1ull);
-            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
-            tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
+            tcount_add(tc, w.counters, ccase, 3, 1u);
+            tcount_add(tc, w.counters, ccase, 0, (cell != kDead) ? 0u : 1u);
+            tcount_add(tc, w.counters, ccase, 1, (uint32_t)to.hits);
+            tcount_add(tc, w.counters, ccase, 2, (uint32_t)to.errs);
         }
     }
     tcount_block_flush(tc, w.counters);
@@ -302,17 +309,21 @@ __device__ inline int case_of_slot(const WorldView<R>& w, uint32_t t)
     return lo;
 }
 
-// Case fields the move needs.
+// Case fields the move needs: the time step in the state precision and the
+// moving directions as a bit set (bit d set when direction d moves).
+template <class R>
 struct MoveCase {
-    double dt, mask[3];
+    R dt;
+    uint32_t mb;
     int32_t active;
 };
 
 template <class R>
-__device__ inline MoveCase move_case(const WorldView<R>& w, int c)
+__device__ inline MoveCase<R> move_case(const WorldView<R>& w, int c)
 {
     const CaseDev& cs = w.cases[c];
-    return MoveCase{cs.dt, {cs.mask[0], cs.mask[1], cs.mask[2]}, cs.active};
+    const uint32_t mb = (cs.mask[0] != 0.0 ? 1u : 0u) | (cs.mask[1] != 0.0 ? 2u : 0u) | (cs.mask[2] != 0.0 ? 4u : 0u);
+    return MoveCase<R>{(R)cs.dt, mb, cs.active};
 }
 
 // Slots of one refill chunk: one per lane of a wave.
@@ -367,9 +378,9 @@ __device__ inline uint32_t chunk_valid(uint64_t base, uint32_t n0)
 // chunk is loaded when the current one is used up.  The arithmetic of each
 // simulator is exactly that of track_t<true>.
 template <class R>
-__global__ void __launch_bounds__(256) k_move(WorldView<R> w)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_move(WorldView<R> w)
 {
-    TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
+    TCount tc{-1, {0u, 0u, 0u, 0u}};
     const uint32_t n0 = w.st->n;
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -385,12 +396,12 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
     // case cache: slots [wb, we) belong to case wc
     int wc = 0;
     uint64_t wb = 1u, we = 0u;
-    MoveCase wcs{};
+    MoveCase<R> wcs{};
     // lane state
     bool has = false;
     uint32_t t = 0u, c = 0u, mb = 0u;
     int ccase = 0, it = 0, hits = 0;
-    R x[3] = {(R)0, (R)0, (R)0}, U[3] = {(R)0, (R)0, (R)0}, vt[3] = {(R)0, (R)0, (R)0};
+    R x[3] = {(R)0, (R)0, (R)0}, U[3] = {(R)0, (R)0, (R)0};
     R tau = (R)0;
     for (;;) {
         // refill: lanes without a simulator take the next slots in lane order
@@ -419,7 +430,7 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
             if (taker) {
                 const uint32_t sl = (uint32_t)(fromn ? nbase + (k - availc) : cbase + pos + k);
                 int cc = wc;
-                MoveCase cs = wcs;
+                MoveCase<R> cs = wcs;
                 if (last >= we) {
                     cc = case_of_slot(w, sl);
                     cs = move_case(w, cc);
@@ -433,11 +444,10 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
                     c = g.cell;
                     x[0] = g.x; x[1] = g.y; x[2] = g.z;
                     U[0] = g.u; U[1] = g.v; U[2] = g.w;
-                    tau = (R)cs.dt;
+                    tau = cs.dt;
                     it = 0;
                     hits = 0;
-                    mb = (cs.mask[0] != 0.0 ? 1u : 0u) | (cs.mask[1] != 0.0 ? 2u : 0u) | (cs.mask[2] != 0.0 ? 4u : 0u);
-                    for (int d = 0; d < 3; ++d) vt[d] = U[d] * (((mb >> d) & 1u) ? (R)1 : (R)0);
+                    mb = cs.mb;
                 }
             }
             pos += ntake;
@@ -456,8 +466,11 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
         bool fin = false, queued = false, err = false;
         int qface = 0;
         if (has) {
-            // one crossing: all face data of the cell in one memory round trip
+            // one crossing: all face data of the cell in one memory round trip;
+            // vt is the velocity along the moving directions
             const size_t base = (size_t)c * kMaxFaces;
+            R vt[3];
+            for (int d = 0; d < 3; ++d) vt[d] = U[d] * (((mb >> d) & 1u) ? (R)1 : (R)0);
             V4<R> q[kMaxFaces];
             int32_t nb[kMaxFaces];
 #pragma unroll
@@ -511,7 +524,6 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
                             U[1] = U[1] - (R)2 * udn * qy;
                             U[2] = U[2] - (R)2 * udn * qz;
                         }
-                        for (int d = 0; d < 3; ++d) vt[d] = U[d] * (((mb >> d) & 1u) ? (R)1 : (R)0);
                     }
                 }
                 if (next) {
@@ -537,9 +549,9 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
                     w.st->overflow = 1u;
                 }
             }
-            tcount_add(tc, w.counters, ccase, 0, (c == kDead) ? 1ull : 0ull);
-            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)hits);
-            tcount_add(tc, w.counters, ccase, 2, err ? 1ull : 0ull);
+            tcount_add(tc, w.counters, ccase, 0, (c == kDead) ? 1u : 0u);
+            tcount_add(tc, w.counters, ccase, 1, (uint32_t)hits);
+            tcount_add(tc, w.counters, ccase, 2, err ? 1u : 0u);
         }
     }
     tcount_block_flush(tc, w.counters);
@@ -549,7 +561,7 @@ __global__ void __launch_bounds__(256) k_move(WorldView<R> w)
 template <class R>
 __global__ void k_wall(WorldView<R> w)
 {
-    TCount tc{-1, {0ull, 0ull, 0ull, 0ull}};
+    TCount tc{-1, {0u, 0u, 0u, 0u}};
     const uint32_t nq = min(w.st->nwall, w.wq_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += stride) {
@@ -577,9 +589,9 @@ __global__ void k_wall(WorldView<R> w)
             w.a.erot[t] = (R)erot;
         }
         if (active) {
-            tcount_add(tc, w.counters, ccase, 0, (cell == kDead) ? 1ull : 0ull);
-            tcount_add(tc, w.counters, ccase, 1, (unsigned long long)to.hits);
-            tcount_add(tc, w.counters, ccase, 2, (unsigned long long)to.errs);
+            tcount_add(tc, w.counters, ccase, 0, (cell == kDead) ? 1u : 0u);
+            tcount_add(tc, w.counters, ccase, 1, (uint32_t)to.hits);
+            tcount_add(tc, w.counters, ccase, 2, (uint32_t)to.errs);
         }
     }
     tcount_block_flush(tc, w.counters);
@@ -1272,7 +1284,7 @@ __global__ void k_tile_table(WorldView<R> w)
 // SortOnly stops after 2.  Tiles larger than TileMax run the same steps
 // from HBM.
 template <class R, bool SortOnly, int L, bool MS, int Block, uint32_t TileN, uint32_t TileMax>
-__global__ void __launch_bounds__(Block) k_tile(WorldView<R> w)
+__global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ? 1 : 5))) k_tile(WorldView<R> w)
 {
     constexpr uint32_t kM = SortOnly ? 1u : TileMax;
     __shared__ uint32_t s_off[TileN / kCellW + 2];
@@ -1476,10 +1488,10 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
 //              relaxation, the number of relaxed simulators and their
 //              selection (partial Fisher-Yates on the cell stream) into a
 //              selection list; parameters into a per-cell record;
-//  k_bgk_conserve (eight lanes per cell) new velocities of the cell's
-//              selections, lane j drawing selections j, j + 8, ... from
-//              each selection's own stream (kRngBgkV, sub = selection
-//              index), USP draws testing against the cell's maxProb at the
+//  k_bgk_conserve (eight lanes per cell) new velocities of the
+//              selections of the wave's eight cells, spread over the 64
+//              lanes, each from the selection's own stream (kRngBgkV,
+//              sub = selection index), USP draws testing against the cell's maxProb at the
 //              start of the step and recording the largest raise; then
 //              conserveMomentumAndEnergy and the USP maxProb update.
 constexpr uint32_t kRngBgkV = 7;
@@ -1795,14 +1807,30 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
             }
         }
     }
-    // the cell's selections get their new velocities first, lane j taking
-    // selections j, j + 8, ... (each from its own stream)
-    if (nrel) {
+    // the selections of the wave's eight cells get their new velocities
+    // first, spread over all 64 lanes: draw t of the wave is selection
+    // t - pf[g] of group g (each from its own stream)
+    {
+        uint32_t pf[9], sbg[8];
+        int cg[8];
+        pf[0] = 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            pf[q + 1] = pf[q] + (uint32_t)__builtin_amdgcn_readlane((int)nrel, 8 * q);
+            sbg[q] = (uint32_t)__builtin_amdgcn_readlane((int)sb, 8 * q);
+            cg[q] = __builtin_amdgcn_readlane(ccase, 8 * q);
+        }
+        const uint32_t cw = (blockIdx.x * blockDim.x + (tid & ~63u)) >> 3;
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        const CaseDev& cs = w.cases[ccase];
-        const BgkP& P = w.bgkp[c];
-        for (uint32_t k = j; k < nrel; k += 8u)
-            bgk_draw<R>(w, cs, c, w.sel_slot[sb + k], k, P, &w.raise[c], A);
+        for (uint32_t t = lane; t < pf[8]; t += 64u) {
+            uint32_t k = 0u, slot_at = 0u, gg = 0u;
+            int ct = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (t >= pf[q] && t < pf[q + 1]) { k = t - pf[q]; slot_at = sbg[q] + k; ct = cg[q]; gg = (uint32_t)q; }
+            const uint32_t cc = cw + gg;
+            bgk_draw<R>(w, w.cases[ct], cc, w.sel_slot[slot_at], k, w.bgkp[cc], &w.raise[cc], A);
+        }
     }
     __threadfence_block();
     __syncthreads();
