@@ -82,34 +82,41 @@ __device__ inline void case_add(unsigned long long* counters, int ccase, int idx
 // Per-thread counters of one case, reduced over the block at kernel end so
 // that a kernel issues one global atomic per counter per block instead of
 // one per wave.  Slots: 0 deleted, 1 wall hits, 2 track errors, 3 inserted.
-// The per-thread counts are 32-bit and go to the global counters before
-// one of them could pass 2^31.
+// T is the per-thread count type; 32-bit counts go to the global counters
+// before one of them could pass 2^31.
+template <class T>
 struct TCount {
     int ccase;
-    uint32_t v[4];
+    T v[4];
 };
 
-__device__ inline void tcount_flush_global(const TCount& t, unsigned long long* counters)
+constexpr int kTCountIdx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
+
+__device__ inline void tcount_flush_one(int ccase, const unsigned long long (&v)[4], unsigned long long* counters)
 {
-    const int idx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
-    if (t.ccase < 0) return;
+    if (ccase < 0) return;
     for (int k = 0; k < 4; ++k)
-        if (t.v[k]) atomicAdd(&counters[(size_t)t.ccase * kCntNum + idx[k]], (unsigned long long)t.v[k]);
+        if (v[k]) atomicAdd(&counters[(size_t)ccase * kCntNum + kTCountIdx[k]], v[k]);
 }
 
-__device__ inline void tcount_add(TCount& t, unsigned long long* counters, int ccase, int k, uint32_t v)
+template <class T>
+__device__ inline void tcount_add(TCount<T>& t, unsigned long long* counters, int ccase, int k, uint32_t v)
 {
     if (!v) return;
-    if (t.ccase != ccase || t.v[k] >= 0x80000000u - v) {
-        tcount_flush_global(t, counters);
+    bool flush = t.ccase != ccase;
+    if constexpr (sizeof(T) == sizeof(uint32_t)) flush = flush || t.v[k] >= 0x80000000u - v;
+    if (flush) {
+        const unsigned long long o[4] = {t.v[0], t.v[1], t.v[2], t.v[3]};
+        tcount_flush_one(t.ccase, o, counters);
         t.ccase = ccase;
-        t.v[0] = t.v[1] = t.v[2] = t.v[3] = 0u;
+        t.v[0] = t.v[1] = t.v[2] = t.v[3] = T(0);
     }
     t.v[k] += v;
 }
 
 // Called by every thread of the block.
-__device__ inline void tcount_block_flush(const TCount& t, unsigned long long* counters)
+template <class T>
+__device__ inline void tcount_block_flush(const TCount<T>& t, unsigned long long* counters)
 {
     __shared__ int s_case[kBlock];
     __shared__ unsigned long long s_v[kBlock][4];
@@ -118,23 +125,19 @@ __device__ inline void tcount_block_flush(const TCount& t, unsigned long long* c
     for (int k = 0; k < 4; ++k) s_v[tid][k] = t.v[k];
     __syncthreads();
     if (tid == 0) {
-        const int idx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
         int sc = -1;
         unsigned long long sum[4] = {0ull, 0ull, 0ull, 0ull};
-        for (uint32_t i = 0; i <= kBlock; ++i) {
-            const int ci = (i < kBlock) ? s_case[i] : -1;
-            if (i < kBlock && ci < 0) continue;
-            if (ci != sc || i == kBlock) {
-                // a run of one case ends: its sums go out
-                if (sc >= 0)
-                    for (int k = 0; k < 4; ++k)
-                        if (sum[k]) atomicAdd(&counters[(size_t)sc * kCntNum + idx[k]], sum[k]);
-                if (i == kBlock) break;
-                sc = ci;
-                for (int k = 0; k < 4; ++k) sum[k] = 0ull;
+        for (uint32_t i = 0; i < kBlock; ++i) {
+            if (s_case[i] < 0) continue;
+            if (sc < 0) sc = s_case[i];
+            if (s_case[i] == sc) {
+                for (int k = 0; k < 4; ++k) sum[k] += s_v[i][k];
+            } else {
+                const unsigned long long o[4] = {s_v[i][0], s_v[i][1], s_v[i][2], s_v[i][3]};
+                tcount_flush_one(s_case[i], o, counters);
             }
-            for (int k = 0; k < 4; ++k) sum[k] += s_v[i][k];
         }
+        tcount_flush_one(sc, sum, counters);
     }
 }
 
@@ -250,9 +253,9 @@ __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, cons
 }
 
 template <class R>
-__global__ void k_inflow_gen(WorldView<R> w)
+__global__ void __launch_bounds__(kBlock) k_inflow_gen(WorldView<R> w)
 {
-    TCount tc{-1, {0u, 0u, 0u, 0u}};
+    TCount<unsigned long long> tc{-1, {0ull, 0ull, 0ull, 0ull}};
     const uint32_t total = w.st->n_ins;
     const uint32_t n0 = w.st->n;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -380,7 +383,7 @@ __device__ inline uint32_t chunk_valid(uint64_t base, uint32_t n0)
 template <class R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_move(WorldView<R> w)
 {
-    TCount tc{-1, {0u, 0u, 0u, 0u}};
+    TCount<uint32_t> tc{-1, {0u, 0u, 0u, 0u}};
     const uint32_t n0 = w.st->n;
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -559,9 +562,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 
 // Diffuse-wall continuation of the queued tracks.
 template <class R>
-__global__ void k_wall(WorldView<R> w)
+__global__ void __launch_bounds__(kBlock) k_wall(WorldView<R> w)
 {
-    TCount tc{-1, {0u, 0u, 0u, 0u}};
+    TCount<unsigned long long> tc{-1, {0ull, 0ull, 0ull, 0ull}};
     const uint32_t nq = min(w.st->nwall, w.wq_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += stride) {
