@@ -1964,63 +1964,81 @@ __global__ void k_kn_fields(WorldView<R> w)
     }
 }
 
-// One pass of fvc::average(fvc::interpolate(.)) over the fields [q0, q0+nq)
-// of a stride-8 table (nq = 1 scalar, 3 vector); fields outside are copied.
+// One pass of fvc::average(fvc::interpolate(.)) for one cell: the face
+// values (linear interpolation on internal and cyclic faces, the cell value
+// on the others) averaged with the face areas, faces in order.  All face
+// data and neighbour values are loaded before the sums.
+
+// Face data of cell c; nb[j] is the cell whose value face j reads (c itself
+// unless the face is linked).
 template <class R>
-__device__ inline void kn_smooth_cell(const WorldView<R>& w, uint32_t c, const double* in, double* out, int q0,
-                                      int ncomp, int stride)
+__device__ inline void kn_face_data(const WorldView<R>& w, uint32_t c, int8_t (&kd)[kMaxFaces],
+                                    uint32_t (&nb)[kMaxFaces], double2 (&aw)[kMaxFaces])
 {
-    double num[3] = {0.0, 0.0, 0.0}, den = 0.0;
-    const double* fc = &in[(size_t)stride * c + q0];
+#pragma unroll
     for (int j = 0; j < kMaxFaces; ++j) {
         const size_t k = (size_t)c * kMaxFaces + j;
-        const int kind = w.fkind[k];
-        if (kind == kFaceSkip) continue;
-        const KnFace kf = w.knf[k];
-        double fv[3];
-        if (kind == kFaceInternal || kind == kFaceCyclic) {
-            const double* fo = &in[(size_t)stride * (uint32_t)w.geom.nbr[k] + q0];
-            for (int q = 0; q < ncomp; ++q) fv[q] = kf.w * fc[q] + (1.0 - kf.w) * fo[q];
-        } else if (kind == kFaceSymmetry && ncomp == 3) {
-            const double un = kf.nx * fc[0] + kf.ny * fc[1] + kf.nz * fc[2];
-            fv[0] = fc[0] - un * kf.nx;
-            fv[1] = fc[1] - un * kf.ny;
-            fv[2] = fc[2] - un * kf.nz;
-        } else {
-            for (int q = 0; q < ncomp; ++q) fv[q] = fc[q];
-        }
-        for (int q = 0; q < ncomp; ++q) num[q] = num[q] + kf.area * fv[q];
-        den = den + kf.area;
+        kd[j] = w.fkind[k];
+        const int32_t n = w.geom.nbr[k];
+        aw[j] = w.knaw[k];
+        nb[j] = (kd[j] == kFaceInternal || kd[j] == kFaceCyclic) ? (uint32_t)n : c;
     }
-    for (int q = 0; q < ncomp; ++q) out[(size_t)stride * c + q0 + q] = (den > 0.0) ? num[q] / den : fc[q];
 }
 
-// Smoothing pass of rhoM, p, T (scalars) and U (vector); rhoN is carried.
+// KnGLL (stride 1): one scalar per cell.
+template <class R>
+__device__ inline void kn_smooth_scalar(const WorldView<R>& w, uint32_t c, const double* in, double* out)
+{
+    int8_t kd[kMaxFaces];
+    uint32_t nb[kMaxFaces];
+    double2 aw[kMaxFaces];
+    kn_face_data<R>(w, c, kd, nb, aw);
+    const double fc = in[c];
+    double fo[kMaxFaces];
+#pragma unroll
+    for (int j = 0; j < kMaxFaces; ++j) fo[j] = in[nb[j]];
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxFaces; ++j) {
+        if (kd[j] == kFaceSkip) continue;
+        const double fv = (kd[j] == kFaceInternal || kd[j] == kFaceCyclic) ? aw[j].y * fc + (1.0 - aw[j].y) * fo[j] : fc;
+        num = num + aw[j].x * fv;
+        den = den + aw[j].x;
+    }
+    out[c] = (den > 0.0) ? num / den : fc;
+}
+
 // Components 1-6 of a cell's decomposition record (rhoM, p, T scalars and
-// the U vector) in one pass over its faces: per component exactly the sums of
-// kn_smooth_cell, with the symmetry-plane mirror applied to U only.
+// the U vector) in one pass over its faces: per component the same sums as
+// the scalar pass, with the symmetry-plane mirror applied to U only.
 template <class R>
 __device__ inline void kn_smooth_fields(const WorldView<R>& w, uint32_t c, const double* in, double* out)
 {
-    double num[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, den = 0.0;
-    const double* fc = &in[8 * (size_t)c + 1];
+    int8_t kd[kMaxFaces];
+    uint32_t nb[kMaxFaces];
+    double2 aw[kMaxFaces];
+    kn_face_data<R>(w, c, kd, nb, aw);
     double own[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) own[q] = fc[q];
-    for (int j = 0; j < kMaxFaces; ++j) {
-        const size_t k = (size_t)c * kMaxFaces + j;
-        const int kind = w.fkind[k];
-        if (kind == kFaceSkip) continue;
-        const KnFace kf = w.knf[k];
-        double fv[6];
-        if (kind == kFaceInternal || kind == kFaceCyclic) {
-            const double* fo = &in[8 * (size_t)(uint32_t)w.geom.nbr[k] + 1];
+    for (int q = 0; q < 6; ++q) own[q] = in[8 * (size_t)c + 1 + q];
+    double fo[kMaxFaces][6];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) fv[q] = kf.w * own[q] + (1.0 - kf.w) * fo[q];
+    for (int j = 0; j < kMaxFaces; ++j)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) fo[j][q] = in[8 * (size_t)nb[j] + 1 + q];
+    double num[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, den = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxFaces; ++j) {
+        if (kd[j] == kFaceSkip) continue;
+        double fv[6];
+        if (kd[j] == kFaceInternal || kd[j] == kFaceCyclic) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) fv[q] = aw[j].y * own[q] + (1.0 - aw[j].y) * fo[j][q];
         } else {
 #pragma unroll
             for (int q = 0; q < 6; ++q) fv[q] = own[q];
-            if (kind == kFaceSymmetry) {
+            if (kd[j] == kFaceSymmetry) {
+                const KnFace kf = w.knf[(size_t)c * kMaxFaces + j];
                 const double un = kf.nx * own[3] + kf.ny * own[4] + kf.nz * own[5];
                 fv[3] = own[3] - un * kf.nx;
                 fv[4] = own[4] - un * kf.ny;
@@ -2028,15 +2046,15 @@ __device__ inline void kn_smooth_fields(const WorldView<R>& w, uint32_t c, const
             }
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) num[q] = num[q] + kf.area * fv[q];
-        den = den + kf.area;
+        for (int q = 0; q < 6; ++q) num[q] = num[q] + aw[j].x * fv[q];
+        den = den + aw[j].x;
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) out[8 * (size_t)c + 1 + q] = (den > 0.0) ? num[q] / den : own[q];
 }
 
 template <class R>
-__global__ void k_kn_smooth(WorldView<R> w, const double* in, double* out, int pass)
+__global__ void __launch_bounds__(kBlock) k_kn_smooth(WorldView<R> w, const double* in, double* out, int pass)
 {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
@@ -2052,12 +2070,12 @@ __global__ void k_kn_smooth(WorldView<R> w, const double* in, double* out, int p
 
 // Smoothing pass of KnGLL (stride 1): kn -> out.
 template <class R>
-__global__ void k_kn_smooth_kn(WorldView<R> w, const double* in, double* out, int pass)
+__global__ void __launch_bounds__(kBlock) k_kn_smooth_kn(WorldView<R> w, const double* in, double* out, int pass)
 {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
         if (!kn_due(cs)) continue;
-        if (pass < cs.smoothing_passes) kn_smooth_cell<R>(w, c, in, out, 0, 1, 1);
+        if (pass < cs.smoothing_passes) kn_smooth_scalar<R>(w, c, in, out);
         else out[c] = in[c];
     }
 }
@@ -2065,7 +2083,7 @@ __global__ void k_kn_smooth_kn(WorldView<R> w, const double* in, double* out, in
 // Gradients, mean free path, instantaneous Kn and its theta average
 // (localKnudsen.C:299-398); f is the smoothed field table.
 template <class R>
-__global__ void k_kn_compute(WorldView<R> w, const double* f)
+__global__ void __launch_bounds__(kBlock) k_kn_compute(WorldView<R> w, const double* f)
 {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < (uint32_t)w.nc; c += gridDim.x * blockDim.x) {
         const CaseDev& cs = w.cases[w.cell_case[c]];
@@ -2073,16 +2091,26 @@ __global__ void k_kn_compute(WorldView<R> w, const double* f)
         const double* fc = &f[8 * (size_t)c];
         const double mU = sqrt(fc[4] * fc[4] + fc[5] * fc[5] + fc[6] * fc[6]);
         double gr = 0.0, gt = 0.0, gu = 0.0;
+        // face data and neighbour values first, then the face loop in order
+        bool in_f[kMaxFaces];
+        double dist[kMaxFaces], fo1[kMaxFaces], fo3[kMaxFaces], fo4[kMaxFaces], fo5[kMaxFaces], fo6[kMaxFaces];
+#pragma unroll
         for (int j = 0; j < kMaxFaces; ++j) {
             const size_t k = (size_t)c * kMaxFaces + j;
-            if (w.fkind[k] != kFaceInternal) continue;
-            const double* fo = &f[8 * (size_t)(uint32_t)w.geom.nbr[k]];
-            const double dist = w.knf[k].dist;
-            const double mUo = sqrt(fo[4] * fo[4] + fo[5] * fo[5] + fo[6] * fo[6]);
+            in_f[j] = w.fkind[k] == kFaceInternal;
+            const int32_t n = w.geom.nbr[k];
+            dist[j] = w.knf[k].dist;
+            const double* fo = &f[8 * (size_t)(in_f[j] ? (uint32_t)n : c)];
+            fo1[j] = fo[1]; fo3[j] = fo[3]; fo4[j] = fo[4]; fo5[j] = fo[5]; fo6[j] = fo[6];
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxFaces; ++j) {
+            if (!in_f[j]) continue;
+            const double mUo = sqrt(fo4[j] * fo4[j] + fo5[j] * fo5[j] + fo6[j] * fo6[j]);
             double v;
-            v = fabs(fo[1] - fc[1]) / dist; if (gr < v) gr = v;
-            v = fabs(fo[3] - fc[3]) / dist; if (gt < v) gt = v;
-            v = fabs(mUo - mU) / dist; if (gu < v) gu = v;
+            v = fabs(fo1[j] - fc[1]) / dist[j]; if (gr < v) gr = v;
+            v = fabs(fo3[j] - fc[3]) / dist[j]; if (gt < v) gt = v;
+            v = fabs(mUo - mU) / dist[j]; if (gu < v) gu = v;
         }
         const double* a = &w.knacc[(size_t)kKnRows * c];
         const double bmax = cs.breakdown_max;

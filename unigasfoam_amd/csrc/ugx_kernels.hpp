@@ -89,6 +89,7 @@ struct WorldView {
     int32_t kn_due;           // a hybrid case decomposes at this step
     int32_t kn_passes;        // largest smoothing pass count of the due cases
     const KnFace* knf;        // [nc*kMaxFaces]
+    const double2* knaw;      // [nc*kMaxFaces] area and interpolation weight of each face (the smoothing's reads)
     const int8_t* fkind;      // [nc*kMaxFaces]
     double* kw;               // [8*nc] decomposition fields (ping)
     double* kw2;              // [8*nc] (pong)

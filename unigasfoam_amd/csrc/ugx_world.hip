@@ -315,7 +315,7 @@ void World::release()
                      (void**)&d_cases, (void**)&d_cell_case, (void**)&d_vol, (void**)&d_smax, (void**)&d_cmodel,
                      (void**)&d_samp, (void**)&d_uspq, (void**)&d_usps, (void**)&d_uspmax, (void**)&d_knacc,
                      (void**)&d_kn, (void**)&d_cc, (void**)&d_mass, (void**)&d_rotdof, (void**)&d_omega,
-                     (void**)&d_diam, (void**)&d_knf, (void**)&d_fkind, (void**)&d_kw, (void**)&d_kw2, (void**)&d_ids2,
+                     (void**)&d_diam, (void**)&d_knf, (void**)&d_knaw, (void**)&d_fkind, (void**)&d_kw, (void**)&d_kw2, (void**)&d_ids2,
                      (void**)&d_nb_off, (void**)&d_nb_list,
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
                      (void**)&d_pe, (void**)&d_pchi, (void**)&d_palpha, (void**)&d_cnt, (void**)&d_ocnt, (void**)&d_out7, (void**)&d_pre, (void**)&d_rev, (void**)&d_off,
@@ -452,6 +452,7 @@ WorldView<R> World::view() const
     w.kn_due = 0;
     w.kn_passes = 0;
     w.knf = d_knf;
+    w.knaw = d_knaw;
     w.fkind = d_fkind;
     w.kw = d_kw;
     w.kw2 = d_kw2;
@@ -697,6 +698,9 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_diam, diam))) return rc;
     if (has_hybrid) {
         if ((rc = up(d_knf, knf))) return rc;
+        std::vector<double2> knaw(knf.size());
+        for (size_t k = 0; k < knf.size(); ++k) knaw[k] = make_double2(knf[k].area, knf[k].w);
+        if ((rc = up(d_knaw, knaw))) return rc;
         if ((rc = up(d_fkind, fkind))) return rc;
         if ((rc = dalloc(d_kw, (size_t)nc * 8))) return rc;
         if ((rc = dalloc(d_kw2, (size_t)nc * 8))) return rc;

@@ -69,6 +69,7 @@ struct World {
     double* d_omega = nullptr;
     double* d_diam = nullptr;
     KnFace* d_knf = nullptr;
+    double2* d_knaw = nullptr;
     int8_t* d_fkind = nullptr;
     double* d_kw = nullptr;
     double* d_kw2 = nullptr;
