@@ -37,12 +37,13 @@ def phase_bytes(phase, R, n, n_cells, cand=0, coll=0, n_bgk=0, rows=N_BASE_ROWS 
         per_cell = 8 + 4 + 2 * 8 * rows + (2 * 8 * 8 if hybrid else 0) + (8 * N_MOM_ROWS if bgk else 0)
         return n * (4 + 2 * record_bytes(R)) + n_cells * per_cell
     if phase == "collide":     # NTC: sigma_max r/w, volume, offsets; pair reads, collided pair writes.
-        #                        BGK: moments read, parameters w + r, USP state r/w, selection list,
-        #                        redrawn velocities, conservation (read v + species twice, write v)
+        #                        BGK: moments read, parameters w + r, USP state r/w, selection list
+        #                        (one slot per redrawn simulator, w + r), redrawn velocities,
+        #                        conservation (read v + species, write v)
         b = n_cells * (16 + 8 + 8) + cand * 2 * (3 * R + 4) + coll * 2 * (4 * R)
         if bgk:
             b += n_cells * (8 * N_MOM_ROWS + 2 * BGK_PARAMS + 2 * 8 * 10 + 8)
-            b += n_bgk * (12 + 12 + 3 * R + 4) + n * (2 * (3 * R + 4) + 3 * R)
+            b += n_bgk * (4 + 4 + 3 * R + 4) + n * ((3 * R + 4) + 3 * R)
         return b
     return 0
 
