@@ -1742,7 +1742,9 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
 {
     Rng rv;
     rng_init(rv, cs.seed_lo, cs.seed_hi, kRngBgkV, k, c - (uint32_t)cs.cell_begin, cs.step);
-    const double u0 = P.s2kT / sqrt(w.sp.mass[cs.sp_base + A.local_species(i)]);
+    // a single-species case needs no species lookup
+    const int ls = (cs.ns == 1) ? 0 : A.local_species(i);
+    const double u0 = P.s2kT / sqrt(w.sp.mass[cs.sp_base + ls]);
     double xi[3];
     if (P.model == UGX_BGK_USP) {
         const double m0 = P.m0, p = P.p, cQ = P.cQ, cS = P.cS;
@@ -1789,10 +1791,12 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     uint32_t b = 0, e = 0, sb = 0, nrel = 0;
     int ccase = 0;
     double T = 0.0, U0 = 0.0, U1 = 0.0, U2 = 0.0, M = 0.0, N = 0.0;
+    int one_sp = -1;   // the global species of a single-species case, else -1
     (void)g;
     if (c < (uint32_t)w.nc) {
         ccase = w.cell_case[c];
         const CaseDev& cs = w.cases[ccase];
+        if (cs.ns == 1) one_sp = cs.sp_base;
         const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
         const bool bgk = cs.active && (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
         if (bgk) {
@@ -1850,7 +1854,7 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
             rx[u] = R(0); ry[u] = R(0); rz[u] = R(0); mm[u] = 0.0;
             if (i < e) {
                 rx[u] = w.b.vx[i]; ry[u] = w.b.vy[i]; rz[u] = w.b.vz[i];
-                mm[u] = w.sp.mass[w.b.sp[i]];
+                mm[u] = w.sp.mass[(one_sp >= 0) ? (uint32_t)one_sp : w.b.sp[i]];
             }
         }
 #pragma unroll
@@ -1869,7 +1873,7 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const uint32_t i = min(i0 + 8u * u, e - 1u);
-                tm[u] = w.sp.mass[w.b.sp[i]];
+                tm[u] = w.sp.mass[(one_sp >= 0) ? (uint32_t)one_sp : w.b.sp[i]];
                 ux[u] = (double)w.b.vx[i]; uy[u] = (double)w.b.vy[i]; uz[u] = (double)w.b.vz[i];
             }
 #pragma unroll
