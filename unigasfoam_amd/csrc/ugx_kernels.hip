@@ -1460,7 +1460,8 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
         uint32_t Q;
         do { Q = uindex(r, nC); } while (Q == P);
         const uint32_t ip = b + P, iq = b + Q;
-        const int lp = A.local_species(ip), lq = A.local_species(iq);
+        // a single-species case needs no species lookup
+        const int lp = (cs.ns == 1) ? 0 : A.local_species(ip), lq = (cs.ns == 1) ? 0 : A.local_species(iq);
         const int gp = cs.sp_base + lp, gq = cs.sp_base + lq;
         const int pk = cs.pair_base + lp * cs.ns + lq;
         double UP[3], UQ[3];
