@@ -1701,24 +1701,28 @@ __global__ void __launch_bounds__(kBlock) k_collide(WorldView<R> w)
     uint32_t nrel = 0u, b = 0u, nC = 0u;
     int ccase = 0;
     bool bgk = false, dsmc = false;
+    // the cell's own rows are loaded along with its case, not after it
+    double S[kNMom];
     if (on) {
+        const uint32_t cb = w.off[c], ce = w.off[c + 1];
+        const int cm = w.cmodel[c];
+        if (w.has_bgk) {
+#pragma unroll
+            for (int q = 0; q < kNMom; ++q) S[q] = w.mom[(size_t)c * kNMom + q];
+        }
         ccase = w.cell_case[c];
         const CaseDev& cs = w.cases[ccase];
         if (cs.active) {
             const bool hybrid = cs.coll_model == UGX_COLL_HYBRID;
             bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
-            dsmc = (cs.coll_model == UGX_COLL_DSMC || hybrid) && cs.dsmc_model != UGX_DSMC_NONE
-                   && w.cmodel[c] == 1;
-            b = w.off[c];
-            nC = w.off[c + 1] - b;
+            dsmc = (cs.coll_model == UGX_COLL_DSMC || hybrid) && cs.dsmc_model != UGX_DSMC_NONE && cm == 1;
+            b = cb;
+            nC = ce - cb;
         }
     }
     GlobAcc<R> A{w.b, w.sp.local, w.perm};
     if (bgk) {
         const CaseDev& cs = w.cases[ccase];
-        double S[kNMom];
-#pragma unroll
-        for (int q = 0; q < kNMom; ++q) S[q] = w.mom[(size_t)c * kNMom + q];
         BgkP P;
         bgk_params<R>(w, cs, c, nC, S, P, nrel);
         w.bgkp[c] = P;
@@ -1795,6 +1799,9 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     int one_sp = -1;   // the global species of a single-species case, else -1
     (void)g;
     if (c < (uint32_t)w.nc) {
+        // the cell's own rows are loaded along with its case, not after it
+        const uint32_t cb = w.off[c], ce = w.off[c + 1];
+        const double cM = w.mom[(size_t)c * kNMom + UGX_S_M], cN = w.mom[(size_t)c * kNMom + UGX_S_N];
         ccase = w.cell_case[c];
         const CaseDev& cs = w.cases[ccase];
         if (cs.ns == 1) one_sp = cs.sp_base;
@@ -1807,11 +1814,11 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
             sb = P.sel_base;
             nrel = P.nrel;
             if (perform) {
-                b = w.off[c];
-                e = w.off[c + 1];
+                b = cb;
+                e = ce;
                 T = P.T; U0 = P.U[0]; U1 = P.U[1]; U2 = P.U[2];
-                M = w.mom[(size_t)c * kNMom + UGX_S_M];
-                N = w.mom[(size_t)c * kNMom + UGX_S_N];
+                M = cM;
+                N = cN;
             }
         }
     }
