@@ -1334,8 +1334,10 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ?
         for (uint32_t i = tid; i < size; i += Block) s_in[i] = w.perm[p0 + i];
         __syncthreads();
         // sampler rows of the first cell pass, loaded while the tile is sorted
+        // (not at sum level 2, where the ten registers they hold through the
+        // sort cost more than the round trip they save)
         SampPre spre{{0.0, 0.0, 0.0, 0.0, 0.0}, false};
-        if constexpr (!SortOnly) {
+        if constexpr (!SortOnly && L < 2) {
             const uint32_t k = (tid >> 6) * 8u + ((tid & 63u) >> 3);
             if (k < nct) {
                 const int ci = s_cc[k];
