@@ -659,7 +659,7 @@ __global__ void k_count(WorldView<R> w, int all)
 constexpr uint32_t kCountWindow = 1024;
 
 template <class R, int Block, int TPB>
-__global__ void __launch_bounds__(Block) k_count_tiles(WorldView<R> w)
+__global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(7))) k_count_tiles(WorldView<R> w)
 {
     constexpr int U = kCountWindow / Block;
     constexpr uint32_t kChunks = kCountWindow / 64;
