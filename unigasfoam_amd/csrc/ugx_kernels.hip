@@ -1452,6 +1452,7 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
     rng_init(r, cs.seed_lo, cs.seed_hi, kRngNtc, 0u, c - (uint32_t)cs.cell_begin, cs.step);
     const double smax0 = w.smax[c];
     double smax = smax0;
+    const bool lb = cs.dsmc_model == UGX_DSMC_LB_VHS || cs.dsmc_model == UGX_DSMC_LB_VSS;
     const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * smax0 * cs.dt / w.vol[c];
     const double fl = floor(sel);
     long long ncand = (long long)fl;
@@ -1469,18 +1470,23 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
         double UP[3], UQ[3];
         A.load(ip, UP[0], UP[1], UP[2]);
         A.load(iq, UQ[0], UQ[1], UQ[2]);
+        // E_rot takes part only in Larsen-Borgnakke exchanges; it is read with
+        // the velocities, and the other models leave it untouched
+        double eP = 0.0, eQ = 0.0;
+        if (lb) { eP = A.erot(ip); eQ = A.erot(iq); }
         double d0 = UP[0] - UQ[0], d1 = UP[1] - UQ[1], d2 = UP[2] - UQ[2];
         double cR2 = d0 * d0 + d1 * d1 + d2 * d2;
         double scr = (cR2 < kVSmall) ? 0.0 : w.pr.C[pk] * dexp(w.pr.e[pk] * dlog(cR2));
         if (scr > smax) smax = scr;
         if (scr / smax0 > uniform(r)) {
-            double eP = A.erot(ip), eQ = A.erot(iq);
             binary_collide(r, cs.dsmc_model, cs.inv_zrot, w.sp.rotdof[gp], w.sp.rotdof[gq], w.sp.mass[gp],
                            w.sp.mass[gq], w.pr.mr[pk], w.pr.chi[pk], w.pr.alpha[pk], UP, UQ, eP, eQ);
             A.store(ip, UP[0], UP[1], UP[2]);
             A.store(iq, UQ[0], UQ[1], UQ[2]);
-            A.set_erot(ip, eP);
-            A.set_erot(iq, eQ);
+            if (lb) {
+                A.set_erot(ip, eP);
+                A.set_erot(iq, eQ);
+            }
             coll += 1;
         }
     }
