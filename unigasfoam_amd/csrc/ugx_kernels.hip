@@ -1453,6 +1453,16 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
     const double smax0 = w.smax[c];
     double smax = smax0;
     const bool lb = cs.dsmc_model == UGX_DSMC_LB_VHS || cs.dsmc_model == UGX_DSMC_LB_VSS;
+    // a single-species case has one species and one pair: their constants are
+    // read once, not per candidate
+    const bool one = cs.ns == 1;
+    double oC = 0.0, oE = 0.0, oMass = 0.0, oMr = 0.0, oChi = 0.0, oAlpha = 0.0;
+    int oRd = 0;
+    if (one) {
+        oC = w.pr.C[cs.pair_base]; oE = w.pr.e[cs.pair_base];
+        oMr = w.pr.mr[cs.pair_base]; oChi = w.pr.chi[cs.pair_base]; oAlpha = w.pr.alpha[cs.pair_base];
+        oMass = w.sp.mass[cs.sp_base]; oRd = w.sp.rotdof[cs.sp_base];
+    }
     const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * smax0 * cs.dt / w.vol[c];
     const double fl = floor(sel);
     long long ncand = (long long)fl;
@@ -1464,7 +1474,7 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
         do { Q = uindex(r, nC); } while (Q == P);
         const uint32_t ip = b + P, iq = b + Q;
         // a single-species case needs no species lookup
-        const int lp = (cs.ns == 1) ? 0 : A.local_species(ip), lq = (cs.ns == 1) ? 0 : A.local_species(iq);
+        const int lp = one ? 0 : A.local_species(ip), lq = one ? 0 : A.local_species(iq);
         const int gp = cs.sp_base + lp, gq = cs.sp_base + lq;
         const int pk = cs.pair_base + lp * cs.ns + lq;
         double UP[3], UQ[3];
@@ -1476,11 +1486,18 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
         if (lb) { eP = A.erot(ip); eQ = A.erot(iq); }
         double d0 = UP[0] - UQ[0], d1 = UP[1] - UQ[1], d2 = UP[2] - UQ[2];
         double cR2 = d0 * d0 + d1 * d1 + d2 * d2;
-        double scr = (cR2 < kVSmall) ? 0.0 : w.pr.C[pk] * dexp(w.pr.e[pk] * dlog(cR2));
+        const double pC = one ? oC : w.pr.C[pk], pE = one ? oE : w.pr.e[pk];
+        double scr = (cR2 < kVSmall) ? 0.0 : pC * dexp(pE * dlog(cR2));
         if (scr > smax) smax = scr;
         if (scr / smax0 > uniform(r)) {
-            binary_collide(r, cs.dsmc_model, cs.inv_zrot, w.sp.rotdof[gp], w.sp.rotdof[gq], w.sp.mass[gp],
-                           w.sp.mass[gq], w.pr.mr[pk], w.pr.chi[pk], w.pr.alpha[pk], UP, UQ, eP, eQ);
+            int rdP = oRd, rdQ = oRd;
+            double mP = oMass, mQ = oMass, mr = oMr, chi = oChi, alpha = oAlpha;
+            if (!one) {
+                rdP = w.sp.rotdof[gp]; rdQ = w.sp.rotdof[gq];
+                mP = w.sp.mass[gp]; mQ = w.sp.mass[gq];
+                mr = w.pr.mr[pk]; chi = w.pr.chi[pk]; alpha = w.pr.alpha[pk];
+            }
+            binary_collide(r, cs.dsmc_model, cs.inv_zrot, rdP, rdQ, mP, mQ, mr, chi, alpha, UP, UQ, eP, eQ);
             A.store(ip, UP[0], UP[1], UP[2]);
             A.store(iq, UQ[0], UQ[1], UQ[2]);
             if (lb) {
