@@ -26,6 +26,9 @@
  *                        macroscopicProperties/derived/volumetric/uniGasVolFields/uniGasVolFields.C:723-837
  *   ugx_fields_compute   uniGasVolFields write-time fields  uniGasVolFields.C:839-1430
  *   ugx_particles_*      uniGasParcel state / lagrangian IO parcels/uniGasParcelIO.C:89-181
+ *   ugx_state_*          restart of the cloud and of the resumable sampler sums
+ *                        clouds/uniGasCloud.C:577-703 (time-step and cloud state read at start-up),
+ *                        uniGasVolFields.C:550-669 (readIn / writeOut of the running sums)
  */
 #ifndef UGX_H
 #define UGX_H
@@ -282,6 +285,38 @@ int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
 int ugx_kn_download(ugx_case* c, double* out, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
 int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
+
+/* ------------------------------------------------------------ restart
+ * Everything a plain case carries from one step to the next besides its
+ * simulators: the step count (which drives every random stream and every
+ * interval), the hybrid averaging time, the cumulative counters and the
+ * per-cell arrays.  A checkpoint is this state plus the simulators from
+ * ugx_particles_download with local_position set (the storage form); a
+ * restart creates a case with the same descriptors and seed, uploads those
+ * simulators, then this state.  The restarted case continues bit for bit.
+ * Per-cell arrays are cell-major (row r of cell c at [c * rows + r]). */
+#define UGX_KN_ROWS     16   /* hybrid time sums per cell */
+#define UGX_N_COUNTERS  16   /* counter slots (library order; opaque) */
+
+typedef struct {
+    int64_t  n_cells;            /* in: the case's cell count */
+    int64_t  steps;              /* steps taken */
+    double   kn_time;            /* hybrid: time summed into kn_sums since their last reset */
+    int64_t  sampled_at_reset;   /* sampled-step counter at the last ugx_samples_reset */
+    uint64_t counters[UGX_N_COUNTERS];
+    double*  sigma_max;          /* [n_cells] (sigma_T c_r)max */
+    int32_t* cell_model;         /* [n_cells] 1 DSMC, 0 BGK family */
+    double*  samples;            /* [n_cells * UGX_S_COUNT] sampler time sums */
+    double*  usp_heat_flux;      /* [n_cells * 3] USP relaxed heat flux */
+    double*  usp_stress;         /* [n_cells * 6] USP relaxed stress (xx xy xz yy yz zz) */
+    double*  usp_max_prob;       /* [n_cells] USP maxProb */
+    double*  kn_sums;            /* [n_cells * UGX_KN_ROWS] */
+    double*  kn;                 /* [n_cells] Kn_GLL */
+} ugx_case_state;
+
+/* Copy the state out (any case) or in (a plain case; every pointer non-NULL). */
+int ugx_state_download(ugx_case* c, ugx_case_state* s);
+int ugx_state_upload(ugx_case* c, const ugx_case_state* s);
 
 /* ------------------------------------------------------------ batches
  * Batched and served forms.  No counterpart in the reference, which runs one

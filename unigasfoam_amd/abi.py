@@ -114,6 +114,26 @@ class Fields(C.Structure):
         "temperature_error", "pressure_error", "shear_stress", "heat_flux")]
 
 
+KN_ROWS = 16          # UGX_KN_ROWS
+N_COUNTERS = 16       # UGX_N_COUNTERS
+
+
+class CaseState(C.Structure):
+    """ugx_case_state: restart state of a case besides its simulators."""
+    _fields_ = [
+        ("n_cells", C.c_int64), ("steps", C.c_int64), ("kn_time", C.c_double),
+        ("sampled_at_reset", C.c_int64), ("counters", C.c_uint64 * N_COUNTERS),
+        ("sigma_max", DBL_P), ("cell_model", I32_P), ("samples", DBL_P), ("usp_heat_flux", DBL_P),
+        ("usp_stress", DBL_P), ("usp_max_prob", DBL_P), ("kn_sums", DBL_P), ("kn", DBL_P),
+    ]
+
+
+# per-cell arrays of CaseState: (name, rows per cell, dtype)
+STATE_ARRAYS = (("sigma_max", 1, "f8"), ("cell_model", 1, "i4"), ("samples", S_COUNT, "f8"),
+                ("usp_heat_flux", 3, "f8"), ("usp_stress", 6, "f8"), ("usp_max_prob", 1, "f8"),
+                ("kn_sums", KN_ROWS, "f8"), ("kn", 1, "f8"))
+
+
 class Counters(C.Structure):
     _fields_ = [
         ("n_particles", C.c_int64), ("steps", C.c_int64), ("candidates", C.c_int64),

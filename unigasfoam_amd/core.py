@@ -24,7 +24,7 @@ EXPORTS = [
     "ugx_particles_upload", "ugx_particles_download", "ugx_particle_count", "ugx_step", "ugx_sync",
     "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
-    "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry",
+    "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry", "ugx_state_download", "ugx_state_upload",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_probe_copy",
     "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
     "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
@@ -63,6 +63,8 @@ def lib():
         L.ugx_sigma_max_download.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_sigma_max_upload.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_model_download.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.ugx_state_download.argtypes = [vp, C.POINTER(abi.CaseState)]
+        L.ugx_state_upload.argtypes = [vp, C.POINTER(abi.CaseState)]
         L.ugx_kn_download.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_volumes.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_centres.argtypes = [vp, abi.DBL_P, C.c_int64]
@@ -172,6 +174,8 @@ class Case:
         h = C.c_void_p()
         check(self.L.ugx_case_create(C.byref(md), sp, ns, C.byref(m), bd, nb, C.byref(opt), C.byref(h)), self.L)
         self.h = h
+        self.precision = precision
+        self.seed = int(opt.seed)
 
     def close(self):
         if getattr(self, "h", None):
@@ -254,6 +258,58 @@ class Case:
         check(self.L.ugx_kn_download(self.h, abi.dptr(out), self.nc), self.L)
         return out
 
+    def _state_struct(self, arrs):
+        s = abi.CaseState()
+        s.n_cells = self.nc
+        for name, _, dt in abi.STATE_ARRAYS:
+            setattr(s, name, abi.iptr(arrs[name]) if dt == "i4" else abi.dptr(arrs[name]))
+        return s
+
+    def state(self):
+        """Restart state besides the simulators (ugx_state_download): step
+        count, hybrid time, counters and the per-cell arrays (cell-major)."""
+        arrs = {name: np.zeros((self.nc, rows), dtype=dt) for name, rows, dt in abi.STATE_ARRAYS}
+        s = self._state_struct(arrs)
+        check(self.L.ugx_state_download(self.h, C.byref(s)), self.L)
+        arrs.update(steps=np.int64(s.steps), kn_time=np.float64(s.kn_time),
+                    sampled_at_reset=np.int64(s.sampled_at_reset),
+                    counters=np.array(s.counters[:], dtype=np.uint64))
+        return arrs
+
+    def restore_state(self, st):
+        """Upload a state from state() (ugx_state_upload)."""
+        arrs = {name: np.ascontiguousarray(np.asarray(st[name], dtype=dt).reshape(self.nc, rows))
+                for name, rows, dt in abi.STATE_ARRAYS}
+        s = self._state_struct(arrs)
+        s.steps = int(st["steps"])
+        s.kn_time = float(st["kn_time"])
+        s.sampled_at_reset = int(st["sampled_at_reset"])
+        cnt = np.asarray(st["counters"], dtype=np.uint64)
+        for k in range(abi.N_COUNTERS):
+            s.counters[k] = int(cnt[k])
+        check(self.L.ugx_state_upload(self.h, C.byref(s)), self.L)
+
+    def checkpoint(self):
+        """The simulators in storage form (local positions, slot order) and
+        the restart state: Case.restart continues from it bit for bit."""
+        p = self.particles()
+        meta = {"precision": self.precision, "seed": self.seed, "n_cells": self.nc}
+        return {"meta": meta, "particles": {k: p[k] for k in CHECKPOINT_PARTICLE_KEYS}, "state": self.state()}
+
+    @classmethod
+    def restart(cls, spec: CaseSpec, ck, capacity=0, device=0, use_graphs=False):
+        """A new case of the same spec, precision and seed continuing from ck."""
+        meta = ck["meta"]
+        if int(meta["n_cells"]) != spec.mesh.n_cells:
+            raise ValueError("checkpoint is for a mesh of %d cells" % int(meta["n_cells"]))
+        c = cls(spec, precision=str(meta["precision"]), capacity=capacity, device=device,
+                seed=int(meta["seed"]), use_graphs=use_graphs)
+        p = ck["particles"]
+        if len(p["cell"]):
+            c.upload(velocity=p["velocity"], cell=p["cell"], species=p["species"], erot=p["erot"], local=p["local"])
+        c.restore_state(ck["state"])
+        return c
+
     def timing(self, on=True):
         check(self.L.ugx_timing_enable(self.h, 1 if on else 0), self.L)
 
@@ -273,6 +329,29 @@ class Case:
         out = np.zeros((self.nc, 3))
         check(self.L.ugx_cell_centres(self.h, abi.dptr(out), out.size), self.L)
         return out
+
+
+CHECKPOINT_PARTICLE_KEYS = ("local", "velocity", "erot", "cell", "species")
+
+
+def checkpoint_save(path, ck):
+    """Write a checkpoint from Case.checkpoint() as one .npz (no pickled objects)."""
+    flat = {"meta_precision": np.array(ck["meta"]["precision"]), "meta_seed": np.uint64(ck["meta"]["seed"]),
+            "meta_n_cells": np.int64(ck["meta"]["n_cells"])}
+    for k in CHECKPOINT_PARTICLE_KEYS:
+        flat["p_" + k] = np.asarray(ck["particles"][k])
+    for k, v in ck["state"].items():
+        flat["s_" + k] = np.asarray(v)
+    np.savez(path, **flat)
+
+
+def checkpoint_load(path):
+    """Read a checkpoint written by checkpoint_save."""
+    with np.load(path, allow_pickle=False) as z:
+        meta = {"precision": str(z["meta_precision"]), "seed": int(z["meta_seed"]), "n_cells": int(z["meta_n_cells"])}
+        particles = {k: z["p_" + k] for k in CHECKPOINT_PARTICLE_KEYS}
+        state = {k[2:]: z[k] for k in z.files if k.startswith("s_")}
+    return {"meta": meta, "particles": particles, "state": state}
 
 
 def geometry(spec: CaseSpec):

@@ -254,6 +254,25 @@ int ugx_sigma_max_upload(ugx_case* c, const double* in, int64_t len)
     UGX_GUARD_END
 }
 
+int ugx_state_download(ugx_case* c, ugx_case_state* s)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (!s) return set_err(UGX_ERR_ARG, "null state");
+    return c->world->case_state(c->index, s, false);
+    UGX_GUARD_END
+}
+
+int ugx_state_upload(ugx_case* c, const ugx_case_state* s)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (c->batch) return set_err(UGX_ERR_STATE, "the state of a batch case is carried by the batch");
+    if (!s) return set_err(UGX_ERR_ARG, "null state");
+    return c->world->case_state(c->index, const_cast<ugx_case_state*>(s), true);
+    UGX_GUARD_END
+}
+
 int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len)
 {
     UGX_GUARD_BEGIN

@@ -264,7 +264,7 @@ struct CaseDev {
 
 // Hybrid decomposition: time-sum rows per cell (N, M, M u, M v, M w, M |u|^2,
 // two unused, then the per-species counts).
-constexpr int kKnRows = 16;
+constexpr int kKnRows = UGX_KN_ROWS;
 
 // Per cell-face data of the decomposition's face interpolation, FP64 for
 // either particle precision: area, linear weight of the cell's own value,

@@ -1146,6 +1146,57 @@ int World::case_samples(int ci, double* out, int64_t len)
         for (int k = 0; k < kNSamp; ++k) out[(size_t)k * ncl + c] = cm[(size_t)c * kNSamp + k];
     return UGX_OK;
 }
+int World::case_state(int ci, ugx_case_state* s, bool upload)
+{
+    static_assert(kCntNum <= UGX_N_COUNTERS, "counter slots");
+    CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    if (s->n_cells != ncl) return set_err(UGX_ERR_ARG, "state holds %lld cells, the case has %d", (long long)s->n_cells, ncl);
+    if (!s->sigma_max || !s->cell_model || !s->samples || !s->usp_heat_flux || !s->usp_stress || !s->usp_max_prob ||
+        !s->kn_sums || !s->kn)
+        return set_err(UGX_ERR_ARG, "every state array is required");
+    HIP_TRY(hipStreamSynchronize(stream));
+    const size_t b = (size_t)cd.cell_begin, n = (size_t)ncl;
+    struct Arr { void* host; void* dev; size_t bytes; };
+    const Arr arrs[] = {
+        {s->sigma_max, d_smax + b, sizeof(double) * n},
+        {s->cell_model, d_cmodel + b, sizeof(int32_t) * n},
+        {s->samples, d_samp + b * kNSamp, sizeof(double) * n * kNSamp},
+        {s->usp_heat_flux, d_uspq + b * 3, sizeof(double) * n * 3},
+        {s->usp_stress, d_usps + b * 6, sizeof(double) * n * 6},
+        {s->usp_max_prob, d_uspmax + b, sizeof(double) * n},
+        {s->kn_sums, d_knacc + b * kKnRows, sizeof(double) * n * kKnRows},
+        {s->kn, d_kn + b, sizeof(double) * n},
+    };
+    unsigned long long cnt[kCntNum];
+    if (!upload) {
+        for (const Arr& a : arrs) HIP_TRY(hipMemcpy(a.host, a.dev, a.bytes, hipMemcpyDeviceToHost));
+        // step and hybrid time advance on the device
+        CaseDev dc;
+        HIP_TRY(hipMemcpy(&dc, d_cases + ci, sizeof(CaseDev), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(cnt, d_counters + (size_t)ci * kCntNum, sizeof cnt, hipMemcpyDeviceToHost));
+        s->steps = (int64_t)dc.step;
+        s->kn_time = dc.kn_time;
+        s->sampled_at_reset = samp_reset_steps[ci];
+        for (int k = 0; k < UGX_N_COUNTERS; ++k) s->counters[k] = (k < kCntNum) ? (uint64_t)cnt[k] : 0u;
+        return UGX_OK;
+    }
+    if (s->steps < 0 || s->steps > (int64_t)UINT32_MAX) return set_err(UGX_ERR_ARG, "step count out of range");
+    for (const Arr& a : arrs) HIP_TRY(hipMemcpy(a.dev, a.host, a.bytes, hipMemcpyHostToDevice));
+    for (int k = 0; k < kCntNum; ++k) cnt[k] = (unsigned long long)s->counters[k];
+    HIP_TRY(hipMemcpy(d_counters + (size_t)ci * kCntNum, cnt, sizeof cnt, hipMemcpyHostToDevice));
+    // the host copy decides the decomposition steps; the device copy draws
+    CaseDev dc;
+    HIP_TRY(hipMemcpy(&dc, d_cases + ci, sizeof(CaseDev), hipMemcpyDeviceToHost));
+    dc.step = (uint32_t)s->steps;
+    dc.kn_time = s->kn_time;
+    cd.step = dc.step;
+    cd.kn_time = dc.kn_time;
+    HIP_TRY(hipMemcpy(d_cases + ci, &dc, sizeof(CaseDev), hipMemcpyHostToDevice));
+    samp_reset_steps[ci] = s->sampled_at_reset;
+    return UGX_OK;
+}
+
 int World::case_samples_reset(int ci)
 {
     HIP_TRY(hipStreamSynchronize(stream));

@@ -157,6 +157,8 @@ struct World {
     int case_counters(int ci, ugx_counters* out);
     int case_samples(int ci, double* out, int64_t len);
     int case_samples_reset(int ci);
+    // restart state of case ci: out to s, or in from s
+    int case_state(int ci, ugx_case_state* s, bool upload);
     int hybrid_update();
 };
 
