@@ -1401,26 +1401,29 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ?
         tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, spre);
     } else {
         __syncthreads();
-        // Tile larger than LDS: the same steps in HBM.  The rank of an entry
-        // in its cell is counted against the cell's slot list staged through
-        // LDS in chunks; the ordered slots go to scratch.
+        // Tile larger than LDS: the same steps in HBM.  As in LDS, a cell's
+        // slot list is in slot order up to eo (the entries laid out by the
+        // occupancy count) and the few entries from eo on (multi-hop, inflow)
+        // are merged in by slot; the ordered slots go to scratch.
         if (tid == 0) atomicAdd(&w.st->big_tiles, 1u);
         for (uint32_t k = 0; k < nct; ++k) {
             const uint32_t b = s_off[k], e = s_off[k + 1];
-            for (uint32_t ib = b; ib < e; ib += Block) {
-                const uint32_t i = ib + tid;
-                const bool mine = i < e;
-                const uint32_t slot = mine ? w.perm[i] : 0u;
-                uint32_t r = 0;
-                for (uint32_t jb = b; jb < e; jb += TileMax) {
-                    const uint32_t nj = min(TileMax, e - jb);
-                    __syncthreads();
-                    for (uint32_t q = tid; q < nj; q += Block) s_in[q] = w.perm[jb + q];
-                    __syncthreads();
-                    if (mine)
-                        for (uint32_t q = 0; q < nj; ++q) r += (s_in[q] < slot) ? 1u : 0u;
+            const uint32_t eo = e - s_oc[k];
+            for (uint32_t i = b + tid; i < e; i += Block) {
+                const uint32_t slot = w.perm[i];
+                uint32_t r;
+                if (i < eo) {
+                    r = i - b;
+                } else {
+                    uint32_t l2 = b, h2 = eo;
+                    while (l2 < h2) {
+                        const uint32_t mid = (l2 + h2) >> 1;
+                        if (w.perm[mid] < slot) l2 = mid + 1u; else h2 = mid;
+                    }
+                    r = l2 - b;
                 }
-                if (mine) w.scratch[b + r] = slot;
+                for (uint32_t q = eo; q < e; ++q) r += (w.perm[q] < slot) ? 1u : 0u;
+                w.scratch[b + r] = slot;
             }
         }
         __syncthreads();
