@@ -1049,19 +1049,20 @@ __device__ inline double group8_total(double p)
 // cell.  L: 0 rows 0-7, 1 adds 8-13, 2 adds 14-19; species rows for
 // q < nsw.  S must start at +0.0.
 // Accumulators of the per-cell sums: the base rows of sum level L (rows
-// 0-7, 0-13 or 0-19), then, for multi-species worlds (MS), the per-species
-// counts and M|v|^2 of every species slot.
-template <int L, bool MS>
+// 0-7, 0-13 or 0-19), then, for worlds with NS > 1 species slots (a case
+// with up to NS species), the per-species counts and M|v|^2 of every slot.
+template <int L, int NS>
 struct SumRows {
     static constexpr int kBase = (L == 0) ? 8 : (L == 1) ? 14 : 20;
-    static constexpr int kN = kBase + (MS ? 2 * kNS : 0);
+    static constexpr int kN = kBase + (NS > 1 ? 2 * NS : 0);
 };
 
-template <int L, bool MS, class R, class Acc>
+template <int L, int NS, class R, class Acc>
 __device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32_t i0, uint32_t e,
-                                     double (&S)[SumRows<L, MS>::kN])
+                                     double (&S)[SumRows<L, NS>::kN])
 {
-    constexpr int B = SumRows<L, MS>::kBase;
+    constexpr bool MS = NS > 1;
+    constexpr int B = SumRows<L, NS>::kBase;
     for (uint32_t i = i0; i < e; i += 8u) {
         const double m = A.mass(i, w.sp);
         double vx, vy, vz;
@@ -1095,9 +1096,9 @@ __device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32
         if constexpr (MS) {
             const int s = A.local_species(i, w.sp);
 #pragma unroll
-            for (int q = 0; q < kNS; ++q) {
+            for (int q = 0; q < NS; ++q) {
                 S[B + q] = S[B + q] + ((s == q) ? 1.0 : 0.0);
-                S[B + kNS + q] = S[B + kNS + q] + ((s == q) ? mcc : 0.0);
+                S[B + NS + q] = S[B + NS + q] + ((s == q) ? mcc : 0.0);
             }
         }
     }
@@ -1125,10 +1126,11 @@ __device__ inline TileCase tile_case_of(const CaseDev& c)
 // count (rows 8-19 only for the full sampler), then the species rows of
 // species j (several species) or, single-species, the species-N row (lane 0)
 // and the species-MCC row (lane 1).  -1: no row.
-template <int L, bool MS>
+template <int L, int NS>
 __device__ inline void samp_lane_rows(uint32_t j, int ns, bool full, int (&rows)[5])
 {
-    constexpr int B = SumRows<L, MS>::kBase;
+    constexpr bool MS = NS > 1;
+    constexpr int B = SumRows<L, NS>::kBase;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int q = (int)j + 8 * i;
@@ -1144,10 +1146,11 @@ __device__ inline void samp_lane_rows(uint32_t j, int ns, bool full, int (&rows)
 }
 
 // The totals of slots i of samp_lane_rows for lane j.
-template <int L, bool MS, int NR>
+template <int L, int NS, int NR>
 __device__ inline void samp_lane_values(uint32_t j, const double (&S)[NR], double (&v)[5])
 {
-    constexpr int B = SumRows<L, MS>::kBase;
+    constexpr bool MS = NS > 1;
+    constexpr int B = SumRows<L, NS>::kBase;
 #pragma unroll
     for (int i = 0; i < 5; ++i) v[i] = 0.0;
 #pragma unroll
@@ -1155,8 +1158,8 @@ __device__ inline void samp_lane_values(uint32_t j, const double (&S)[NR], doubl
         if ((uint32_t)(q & 7) == j) v[q >> 3] = S[q];
     if constexpr (MS) {
 #pragma unroll
-        for (int q = 0; q < kNS; ++q)
-            if ((uint32_t)q == j) { v[3] = S[B + q]; v[4] = S[B + kNS + q]; }
+        for (int q = 0; q < NS; ++q)
+            if ((uint32_t)q == j) { v[3] = S[B + q]; v[4] = S[B + NS + q]; }
     } else {
         v[3] = S[UGX_S_N];
         v[4] = S[UGX_S_MCC];
@@ -1174,12 +1177,13 @@ __device__ inline bool samples_now(const TileCase& cs)
     return cs.active && ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
 }
 
-template <int L, bool MS, int Block, class R, class Acc>
+template <int L, int NS, int Block, class R, class Acc>
 __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nct, const uint32_t* s_off,
                                  const Acc& A, const int32_t* s_cc, const TileCase* s_tc, int case0,
                                  const SampPre& pre)
 {
-    constexpr int B = SumRows<L, MS>::kBase, NR = SumRows<L, MS>::kN;
+    constexpr bool MS = NS > 1;
+    constexpr int B = SumRows<L, NS>::kBase, NR = SumRows<L, NS>::kN;
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = tid >> 6, lane = tid & 63u, g = lane >> 3, j = lane & 7u;
     for (uint32_t kb = 0; kb < nct; kb += (Block / 64) * 8) {
@@ -1205,15 +1209,15 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
         double S[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) S[q] = 0.0;
-        cell_partials<L, MS, R, Acc>(w, A, b + j, e, S);
+        cell_partials<L, NS, R, Acc>(w, A, b + j, e, S);
 #pragma unroll
         for (int q = 0; q < NR; ++q) S[q] = group8_total(S[q]);
         if (need && sample) {
             // every lane of the group holds the totals; lane j updates its rows
             int rows[5];
             double v[5];
-            samp_lane_rows<L, MS>(j, cs.ns, cs.sampler_full != 0, rows);
-            samp_lane_values<L, MS>(j, S, v);
+            samp_lane_rows<L, NS>(j, cs.ns, cs.sampler_full != 0, rows);
+            samp_lane_values<L, NS>(j, S, v);
             double* sc = &w.samp[(size_t)c * kNSamp];
             const bool have = pre.valid && kb == 0u;
 #pragma unroll
@@ -1227,7 +1231,7 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
             if ((uint32_t)(q & 7) == j) v[q >> 3] = S[q];
         if constexpr (MS) {
 #pragma unroll
-            for (int q = 0; q < kNS; ++q)
+            for (int q = 0; q < NS; ++q)
                 if ((uint32_t)q == j) vsp = S[B + q];
         } else {
             vsp = (j == 0u) ? S[UGX_S_N] : 0.0;
@@ -1286,8 +1290,8 @@ __global__ void k_tile_table(WorldView<R> w)
 //  3. per-cell sums from LDS: sampler, hybrid accumulators, BGK moments.
 // SortOnly stops after 2.  Tiles larger than TileMax run the same steps
 // from HBM.
-template <class R, bool SortOnly, int L, bool MS, int Block, uint32_t TileN, uint32_t TileMax>
-__global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ? 1 : 5))) k_tile(WorldView<R> w)
+template <class R, bool SortOnly, int L, int NS, int Block, uint32_t TileN, uint32_t TileMax>
+__global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu((NS == 1 || (NS == 2 && L == 0)) ? 5 : 1))) k_tile(WorldView<R> w)
 {
     constexpr uint32_t kM = SortOnly ? 1u : TileMax;
     __shared__ uint32_t s_off[TileN / kCellW + 2];
@@ -1344,7 +1348,7 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ?
                 const TileCase cs = (ci == case0) ? s_tc : tile_case_of(w.cases[ci]);
                 if (samples_now(cs)) {
                     int rows[5];
-                    samp_lane_rows<L, MS>(tid & 7u, cs.ns, cs.sampler_full != 0, rows);
+                    samp_lane_rows<L, NS>(tid & 7u, cs.ns, cs.sampler_full != 0, rows);
                     const double* sc = &w.samp[(size_t)(c0 + k) * kNSamp];
 #pragma unroll
                     for (int i = 0; i < 5; ++i) spre.v[i] = (rows[i] >= 0) ? sc[rows[i]] : 0.0;
@@ -1398,7 +1402,7 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ?
         if (SortOnly) return;
         __syncthreads();
         LdsAcc<R> A{s_vx, s_vy, s_vz, s_er, s_g, s_spm, s_spr, s_spl, lds_species, p0};
-        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, spre);
+        tile_sums<L, NS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, spre);
     } else {
         __syncthreads();
         // Tile larger than LDS: the same steps in HBM.  As in LDS, a cell's
@@ -1440,7 +1444,7 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(MS ?
         __threadfence_block();
         __syncthreads();
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        tile_sums<L, MS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, SampPre{{0.0, 0.0, 0.0, 0.0, 0.0}, false});
+        tile_sums<L, NS, Block, R>(w, c0, nct, s_off, A, s_cc, &s_tc, case0, SampPre{{0.0, 0.0, 0.0, 0.0, 0.0}, false});
     }
 }
 
@@ -2554,12 +2558,16 @@ static void launch_tiles_cfg(const WorldView<R>& w, hipStream_t s, bool sort_onl
     const uint32_t ncp = (uint32_t)w.nc + 1u;
     const unsigned nt = (unsigned)((w.cap + (uint64_t)kCellW * (uint64_t)w.nc) / TileN + 2);
     hipLaunchKernelGGL((k_tile_table<R, TileN>), dim3(grid_for(ncp)), dim3(kBlock), 0, s, w);
-    const bool ms = w.ns_max > 1;
-#define UGX_TILE(SO, L, MS) hipLaunchKernelGGL((k_tile<R, SO, L, MS, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w)
-    if (sort_only) UGX_TILE(true, 0, false);
-    else if (w.sum_level >= 2) { if (ms) UGX_TILE(false, 2, true); else UGX_TILE(false, 2, false); }
-    else if (w.sum_level == 1) { if (ms) UGX_TILE(false, 1, true); else UGX_TILE(false, 1, false); }
-    else { if (ms) UGX_TILE(false, 0, true); else UGX_TILE(false, 0, false); }
+    // species slots of the sums: 1 (single-species cases), 2, or all kNS
+    const int nsl = (w.ns_max <= 1) ? 1 : (w.ns_max <= 2) ? 2 : kNS;
+#define UGX_TILE(SO, L, NS) hipLaunchKernelGGL((k_tile<R, SO, L, NS, Block, TileN, TileMax>), dim3(nt), dim3(Block), 0, s, w)
+#define UGX_TILE_NS(L) \
+    do { if (nsl == 1) UGX_TILE(false, L, 1); else if (nsl == 2) UGX_TILE(false, L, 2); else UGX_TILE(false, L, kNS); } while (0)
+    if (sort_only) UGX_TILE(true, 0, 1);
+    else if (w.sum_level >= 2) UGX_TILE_NS(2);
+    else if (w.sum_level == 1) UGX_TILE_NS(1);
+    else UGX_TILE_NS(0);
+#undef UGX_TILE_NS
 #undef UGX_TILE
 }
 
