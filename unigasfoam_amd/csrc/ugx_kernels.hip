@@ -966,6 +966,7 @@ __global__ void k_scatter(WorldView<R> w)
 
 template <class R>
 struct LdsAcc {
+    static constexpr bool kPairs = false;
     const R *vx, *vy, *vz, *er;
     const uint16_t* g;               // species id of each entry
     const double* spm;               // species mass, rotational DoF and case-local
@@ -997,6 +998,7 @@ struct LdsAcc {
 
 template <class R>
 struct GlobAcc {
+    static constexpr bool kPairs = true;   // cell sums keep two records in flight
     Parts<R> p;
     const int32_t* local;
     uint32_t* idx;
@@ -1057,50 +1059,75 @@ struct SumRows {
     static constexpr int kN = kBase + (NS > 1 ? 2 * NS : 0);
 };
 
+// One simulator's terms added to the partial sums S, in row order.
+template <int L, int NS, int N>
+__device__ inline void partial_add(double m, double vx, double vy, double vz, double er, double rd, int s,
+                                   double (&S)[N])
+{
+    static_assert(N == SumRows<L, NS>::kN, "sum rows");
+    constexpr int B = SumRows<L, NS>::kBase;
+    const double mcc = m * (vx * vx + vy * vy + vz * vz);
+    S[UGX_S_N] = S[UGX_S_N] + 1.0;
+    S[UGX_S_M] = S[UGX_S_M] + m;
+    S[UGX_S_MU] = S[UGX_S_MU] + m * vx;
+    S[UGX_S_MV] = S[UGX_S_MV] + m * vy;
+    S[UGX_S_MW] = S[UGX_S_MW] + m * vz;
+    S[UGX_S_MCC] = S[UGX_S_MCC] + mcc;
+    S[UGX_S_EROT] = S[UGX_S_EROT] + er;
+    S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + rd;
+    if constexpr (L >= 1) {
+        S[UGX_S_MUU] = S[UGX_S_MUU] + m * (vx * vx);
+        S[UGX_S_MUV] = S[UGX_S_MUV] + m * (vx * vy);
+        S[UGX_S_MUW] = S[UGX_S_MUW] + m * (vx * vz);
+        S[UGX_S_MVV] = S[UGX_S_MVV] + m * (vy * vy);
+        S[UGX_S_MVW] = S[UGX_S_MVW] + m * (vy * vz);
+        S[UGX_S_MWW] = S[UGX_S_MWW] + m * (vz * vz);
+    }
+    if constexpr (L >= 2) {
+        S[UGX_S_MCCU] = S[UGX_S_MCCU] + mcc * vx;
+        S[UGX_S_MCCV] = S[UGX_S_MCCV] + mcc * vy;
+        S[UGX_S_MCCW] = S[UGX_S_MCCW] + mcc * vz;
+        S[UGX_S_EROTU] = S[UGX_S_EROTU] + er * vx;
+        S[UGX_S_EROTV] = S[UGX_S_EROTV] + er * vy;
+        S[UGX_S_EROTW] = S[UGX_S_EROTW] + er * vz;
+    }
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            S[B + q] = S[B + q] + ((s == q) ? 1.0 : 0.0);
+            S[B + NS + q] = S[B + NS + q] + ((s == q) ? mcc : 0.0);
+        }
+    }
+}
+
 template <int L, int NS, class R, class Acc>
 __device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32_t i0, uint32_t e,
                                      double (&S)[SumRows<L, NS>::kN])
 {
-    constexpr bool MS = NS > 1;
-    constexpr int B = SumRows<L, NS>::kBase;
-    for (uint32_t i = i0; i < e; i += 8u) {
-        const double m = A.mass(i, w.sp);
-        double vx, vy, vz;
+    auto rec = [&](uint32_t i, double& m, double& vx, double& vy, double& vz, double& er, double& rd, int& s) {
+        m = A.mass(i, w.sp);
         A.load(i, vx, vy, vz);
-        const double er = A.erot(i);
-        const double mcc = m * (vx * vx + vy * vy + vz * vz);
-        S[UGX_S_N] = S[UGX_S_N] + 1.0;
-        S[UGX_S_M] = S[UGX_S_M] + m;
-        S[UGX_S_MU] = S[UGX_S_MU] + m * vx;
-        S[UGX_S_MV] = S[UGX_S_MV] + m * vy;
-        S[UGX_S_MW] = S[UGX_S_MW] + m * vz;
-        S[UGX_S_MCC] = S[UGX_S_MCC] + mcc;
-        S[UGX_S_EROT] = S[UGX_S_EROT] + er;
-        S[UGX_S_ROTDOF] = S[UGX_S_ROTDOF] + A.rotdof(i, w.sp);
-        if constexpr (L >= 1) {
-            S[UGX_S_MUU] = S[UGX_S_MUU] + m * (vx * vx);
-            S[UGX_S_MUV] = S[UGX_S_MUV] + m * (vx * vy);
-            S[UGX_S_MUW] = S[UGX_S_MUW] + m * (vx * vz);
-            S[UGX_S_MVV] = S[UGX_S_MVV] + m * (vy * vy);
-            S[UGX_S_MVW] = S[UGX_S_MVW] + m * (vy * vz);
-            S[UGX_S_MWW] = S[UGX_S_MWW] + m * (vz * vz);
+        er = A.erot(i);
+        rd = A.rotdof(i, w.sp);
+        s = (NS > 1) ? A.local_species(i, w.sp) : 0;
+    };
+    uint32_t i = i0;
+    if constexpr (Acc::kPairs) {
+        // records in HBM: two simulators' reads in flight, added in slot order
+        for (; i + 8u < e; i += 16u) {
+            double m0, x0, y0, z0, e0, r0, m1, x1, y1, z1, e1, r1;
+            int s0, s1;
+            rec(i, m0, x0, y0, z0, e0, r0, s0);
+            rec(i + 8u, m1, x1, y1, z1, e1, r1, s1);
+            partial_add<L, NS>(m0, x0, y0, z0, e0, r0, s0, S);
+            partial_add<L, NS>(m1, x1, y1, z1, e1, r1, s1, S);
         }
-        if constexpr (L >= 2) {
-            S[UGX_S_MCCU] = S[UGX_S_MCCU] + mcc * vx;
-            S[UGX_S_MCCV] = S[UGX_S_MCCV] + mcc * vy;
-            S[UGX_S_MCCW] = S[UGX_S_MCCW] + mcc * vz;
-            S[UGX_S_EROTU] = S[UGX_S_EROTU] + er * vx;
-            S[UGX_S_EROTV] = S[UGX_S_EROTV] + er * vy;
-            S[UGX_S_EROTW] = S[UGX_S_EROTW] + er * vz;
-        }
-        if constexpr (MS) {
-            const int s = A.local_species(i, w.sp);
-#pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                S[B + q] = S[B + q] + ((s == q) ? 1.0 : 0.0);
-                S[B + NS + q] = S[B + NS + q] + ((s == q) ? mcc : 0.0);
-            }
-        }
+    }
+    for (; i < e; i += 8u) {
+        double m, vx, vy, vz, er, rd;
+        int s;
+        rec(i, m, vx, vy, vz, er, rd, s);
+        partial_add<L, NS>(m, vx, vy, vz, er, rd, s, S);
     }
 }
 
