@@ -966,7 +966,7 @@ __global__ void k_scatter(WorldView<R> w)
 
 template <class R>
 struct LdsAcc {
-    static constexpr bool kPairs = false;
+    static constexpr int kInFlight = 1;
     const R *vx, *vy, *vz, *er;
     const uint16_t* g;               // species id of each entry
     const double* spm;               // species mass, rotational DoF and case-local
@@ -998,7 +998,7 @@ struct LdsAcc {
 
 template <class R>
 struct GlobAcc {
-    static constexpr bool kPairs = true;   // cell sums keep two records in flight
+    static constexpr int kInFlight = 2;   // cell sums keep this many records in flight per lane
     Parts<R> p;
     const int32_t* local;
     uint32_t* idx;
@@ -1112,15 +1112,16 @@ __device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32
         s = (NS > 1) ? A.local_species(i, w.sp) : 0;
     };
     uint32_t i = i0;
-    if constexpr (Acc::kPairs) {
-        // records in HBM: two simulators' reads in flight, added in slot order
-        for (; i + 8u < e; i += 16u) {
-            double m0, x0, y0, z0, e0, r0, m1, x1, y1, z1, e1, r1;
-            int s0, s1;
-            rec(i, m0, x0, y0, z0, e0, r0, s0);
-            rec(i + 8u, m1, x1, y1, z1, e1, r1, s1);
-            partial_add<L, NS>(m0, x0, y0, z0, e0, r0, s0, S);
-            partial_add<L, NS>(m1, x1, y1, z1, e1, r1, s1, S);
+    if constexpr (Acc::kInFlight > 1) {
+        // records in HBM: several simulators' reads in flight, added in slot order
+        constexpr int F = Acc::kInFlight;
+        for (; i + 8u * (F - 1) < e; i += 8u * F) {
+            double m[F], x[F], y[F], z[F], er[F], rd[F];
+            int sp[F];
+#pragma unroll
+            for (int u = 0; u < F; ++u) rec(i + 8u * u, m[u], x[u], y[u], z[u], er[u], rd[u], sp[u]);
+#pragma unroll
+            for (int u = 0; u < F; ++u) partial_add<L, NS>(m[u], x[u], y[u], z[u], er[u], rd[u], sp[u], S);
         }
     }
     for (; i < e; i += 8u) {
