@@ -83,3 +83,24 @@ def test_fields_of_a_drifting_maxwellian(require_gpu):
     nu = 4.0 * d * d * n_set * math.sqrt(math.pi * KB * Tref / N2.mass) * (T_set / Tref) ** (1.0 - om)
     assert abs(f["mcr"].mean() / nu - 1.0) < 1e-2
     c.close()
+
+
+def test_fields_through_openfoam_files(require_gpu, tmp_path):
+    """A case on a mesh read back from constant/polyMesh, and its sampled
+    fields written under the uniGasVolFields names and read back."""
+    from unigasfoam_amd import foam_io
+    mesh = box_mesh((0.0, 0.0, 0.0), (0.004, 0.004, 0.004), (4, 4, 4), {k: abi.CYCLIC for k in SIDES})
+    foam_io.write_polymesh(mesh, tmp_path)
+    spec = CaseSpec(mesh=foam_io.read_polymesh(tmp_path), species=[N2], delta_t=1.0e-7, n_equivalent_particles=1.0e9,
+                    collision_model="dsmc", dsmc_model="LarsenBorgnakkeVariableHardSphere", Tref=273.0, boundaries=[],
+                    seed=5)
+    c = Case(spec, precision="fp64", capacity=1 << 17)
+    c.fill(number_density={"N2": 1.0e20}, T=300.0, rot_T=300.0, velocity=(50.0, 0.0, 0.0))
+    c.step(10)
+    f = c.fields()
+    written = foam_io.write_fields(tmp_path, "1e-06", spec.mesh, f)
+    assert "rhoN_uniGasFields" in written and "UMean_uniGasFields" in written
+    for key, (stem, _, _) in foam_io.FIELD_FILES.items():
+        back = foam_io.read_internal_field(tmp_path / "1e-06" / f"{stem}_uniGasFields")
+        assert np.array_equal(back, np.asarray(f[key]).reshape(back.shape), equal_nan=True), key
+    c.close()
