@@ -95,7 +95,16 @@ def sweep_requests(n_cases, rate, seed=5):
     return reqs
 
 
-def sweep_leg(n_cases, rate, precision):
+def served_summary(st, rate, plain_rate):
+    c = st.summary()
+    return {"arrival_rate_per_s": rate, "particle_steps_per_s": c["particle_steps_per_s"],
+            "wall_s": round(c["wall_s"], 2), "p50_time_to_converged_sample_s": c["p50_latency_s"],
+            "p99_time_to_converged_sample_s": c["p99_latency_s"],
+            "max_concurrent_cases": c["max_concurrent_cases"],
+            "vs_plain": c["particle_steps_per_s"] / plain_rate}
+
+
+def sweep_leg(n_cases, rate, precision, sat_rate):
     from unigasfoam_amd.serve import Server, run_alone
     reqs = sweep_requests(n_cases, rate)
     print(f"[bench] sweep: {n_cases} cases, plain one at a time ...", file=sys.stderr, flush=True)
@@ -108,7 +117,13 @@ def sweep_leg(n_cases, rate, precision):
     srv = Server(precision=precision, chunk=10, use_graphs=True, realtime=True)
     served = srv.run(reqs)
     srv.close()
-    a, b, c = alone.summary(), batched.summary(), served.summary()
+    # the same cases arriving faster than they converge: the queue stays
+    # non-empty, and the latencies include the wait for admission
+    print(f"[bench] sweep: served (Poisson arrivals at {sat_rate}/s) ...", file=sys.stderr, flush=True)
+    srv = Server(precision=precision, chunk=10, use_graphs=True, realtime=True)
+    served_sat = srv.run(sweep_requests(n_cases, sat_rate))
+    srv.close()
+    a, b = alone.summary(), batched.summary()
     return {
         "workload": f"{n_cases} parametric 1-D Couette/Fourier cases (Kn 0.01-10, 50-2000 cells, 20-200 simulators/cell, "
                     f"Ar or N2 LB), 50 warm-up steps then 2000 samples per cell",
@@ -117,11 +132,8 @@ def sweep_leg(n_cases, rate, precision):
         "batched": {"particle_steps_per_s": b["particle_steps_per_s"], "wall_s": round(b["wall_s"], 2),
                     "max_concurrent_cases": b["max_concurrent_cases"],
                     "vs_plain": b["particle_steps_per_s"] / a["particle_steps_per_s"]},
-        "served": {"arrival_rate_per_s": rate, "particle_steps_per_s": c["particle_steps_per_s"],
-                   "wall_s": round(c["wall_s"], 2), "p50_time_to_converged_sample_s": c["p50_latency_s"],
-                   "p99_time_to_converged_sample_s": c["p99_latency_s"],
-                   "max_concurrent_cases": c["max_concurrent_cases"],
-                   "vs_plain": c["particle_steps_per_s"] / a["particle_steps_per_s"]},
+        "served": served_summary(served, rate, a["particle_steps_per_s"]),
+        "served_saturated": served_summary(served_sat, sat_rate, a["particle_steps_per_s"]),
     }
 
 
@@ -152,6 +164,8 @@ def main():
                     help="collision model of the nozzle workload")
     ap.add_argument("--sweep-cases", type=int, default=256, help="cases of the batched/served sweep (0: skip)")
     ap.add_argument("--sweep-rate", type=float, default=20.0, help="arrival rate of the served sweep (cases/s)")
+    ap.add_argument("--sweep-saturated-rate", type=float, default=2000.0,
+                    help="arrival rate of the second served leg, above what the batch can retire (cases/s)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 field comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -242,7 +256,7 @@ def main():
         out["hbm_probe"] = {"copy_bytes": nbytes, "GBs": round(2 * nbytes / s / 1e9, 1)}
     case.close()
     if args.sweep_cases > 0:
-        out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision)
+        out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision, args.sweep_saturated_rate)
     if not args.no_precision:
         print("[bench] FP32 vs FP64 fields ...", file=sys.stderr, flush=True)
         out["fp32_vs_fp64"] = precision_leg()
