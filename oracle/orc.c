@@ -1587,3 +1587,49 @@ void orc_stream(uint64_t seed, uint32_t kind, uint32_t sub, uint32_t id, uint32_
         else { double a, b; orc_gauss2(&r, &a, &b); out[i] = a; if (i + 1 < n) out[++i] = b; }
     }
 }
+
+/* ------------------------------------------------------- physics hooks */
+
+/* Test hook: n independent collisions of one pair (species p, q) with the
+ * same pre-collision state, each drawn from its own stream (NTC kind,
+ * entity i), through binary_collide: the dsmcCollisionModel::collide of the
+ * case's model (variableHardSphere.C:118-162, variableSoftSphere.C:116-176,
+ * LarsenBorgnakke*.C).  UP/UQ [3] and eP/eQ are the pre-collision state;
+ * outP/outQ [3n] and erP/erQ [n] receive the results. */
+int orc_pair_collide(orc_world* w, int p, int q, const double* UP, const double* UQ, double eP, double eQ,
+                     int64_t n, uint64_t seed, double* outP, double* outQ, double* erP, double* erQ)
+{
+    if (p < 0 || q < 0 || p >= w->ns || q >= w->ns) return fail(UGX_ERR_ARG, "species out of range");
+    for (int64_t i = 0; i < n; ++i) {
+        orc_rng r;
+        orc_rng_init(&r, seed, ORC_RNG_NTC, 0u, (uint32_t)i, 0u);
+        double a[3] = {UP[0], UP[1], UP[2]}, b[3] = {UQ[0], UQ[1], UQ[2]};
+        double ea = eP, eb = eQ;
+        binary_collide(w, &r, p, q, a, b, &ea, &eb);
+        for (int d = 0; d < 3; ++d) { outP[3 * i + d] = a[d]; outQ[3 * i + d] = b[d]; }
+        if (erP) erP[i] = ea;
+        if (erQ) erQ[i] = eb;
+    }
+    return UGX_OK;
+}
+
+/* Test hooks: the USP per-cell state carried between steps (relaxed heat
+ * flux q [nc*3], relaxed stress [nc*6] xx xy xz yy yz zz, maxProb [nc];
+ * unifiedStochasticParticleSBGK.C:774-783, :859-863).  NULL skips an array. */
+int orc_usp_state(orc_world* w, double* q, double* sh, double* maxp)
+{
+    size_t nc = (size_t)w->nc;
+    if (q) memcpy(q, w->uspq, sizeof(double) * 3 * nc);
+    if (sh) memcpy(sh, w->usps, sizeof(double) * 6 * nc);
+    if (maxp) memcpy(maxp, w->uspmax, sizeof(double) * nc);
+    return UGX_OK;
+}
+
+int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp)
+{
+    size_t nc = (size_t)w->nc;
+    if (q) memcpy(w->uspq, q, sizeof(double) * 3 * nc);
+    if (sh) memcpy(w->usps, sh, sizeof(double) * 6 * nc);
+    if (maxp) memcpy(w->uspmax, maxp, sizeof(double) * nc);
+    return UGX_OK;
+}

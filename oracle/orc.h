@@ -68,6 +68,17 @@ void orc_stream(uint64_t seed, uint32_t kind, uint32_t sub, uint32_t id, uint32_
 int orc_usp_sample(double cQ, double cS, double p, double u0, const double* q, const double* sh, double m0,
                    int64_t n, uint64_t seed, double* xi, double* raise);
 
+/* Test hook: n collisions of one pair (species p, q) from the same
+ * pre-collision state through the case's collision model, each on its own
+ * stream; post-collision velocities outP/outQ [3n], E_rot erP/erQ [n]. */
+int orc_pair_collide(orc_world* w, int p, int q, const double* UP, const double* UQ, double eP, double eQ,
+                     int64_t n, uint64_t seed, double* outP, double* outQ, double* erP, double* erQ);
+
+/* Test hooks: USP per-cell state (heat flux [nc*3], stress [nc*6], maxProb
+ * [nc]); NULL skips an array. */
+int orc_usp_state(orc_world* w, double* q, double* sh, double* maxp);
+int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,7 @@ unigasfoam_amd.core.Case so that parity tests drive both identically.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 
@@ -24,44 +25,69 @@ class OracleError(RuntimeError):
     pass
 
 
+def _load(path):
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run `make -C oracle`")
+    L = C.CDLL(path)
+    _bind(L)
+    return L
+
+
+@contextlib.contextmanager
+def using(path):
+    """Run the enclosed code against another build of the oracle (a library
+    compiled from an edited copy of orc.c); cases created inside bind it."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is not built; run `make -C oracle`")
-        L = C.CDLL(LIB_PATH)
-        vp = C.c_void_p
-        L.orc_last_error.restype = C.c_char_p
-        L.orc_create.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
-                                 C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
-                                 C.c_uint64, C.c_int64, C.POINTER(vp)]
-        L.orc_destroy.argtypes = [vp]
-        L.orc_fill.argtypes = [vp, C.POINTER(abi.FillDesc)]
-        L.orc_upload.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
-        L.orc_download.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
-        L.orc_count.argtypes = [vp]
-        L.orc_count.restype = C.c_int64
-        L.orc_step.argtypes = [vp, C.c_int]
-        L.orc_step_phases.argtypes = [vp, C.c_int]
-        L.orc_samples.argtypes = [vp, abi.DBL_P, C.c_int64]
-        L.orc_counters.argtypes = [vp, C.POINTER(abi.Counters)]
-        L.orc_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
-        L.orc_set_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
-        L.orc_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
-        L.orc_kn.argtypes = [vp, abi.DBL_P, C.c_int64]
-        L.orc_hybrid_set.argtypes = [vp, abi.DBL_P, C.c_double]
-        L.orc_hybrid_decompose.argtypes = [vp]
-        L.orc_smooth.argtypes = [vp, abi.DBL_P, abi.DBL_P, C.c_int]
-        L.orc_refine.argtypes = [vp, abi.I32_P]
-        L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_set_fast_sums.argtypes = [C.c_int]
-        L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
-        L.orc_face_data.argtypes = [vp, abi.DBL_P, abi.I32_P]
-        L.orc_inflow_nbar.argtypes = [vp, abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
-        L.orc_usp_sample.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, abi.DBL_P, abi.DBL_P,
-                                     C.c_double, C.c_int64, C.c_uint64, abi.DBL_P, C.POINTER(C.c_double)]
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def _bind(L):
+    vp = C.c_void_p
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_create.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
+                             C.POINTER(abi.ModelDesc), C.POINTER(abi.BoundaryDesc), C.c_int,
+                             C.c_uint64, C.c_int64, C.POINTER(vp)]
+    L.orc_destroy.argtypes = [vp]
+    L.orc_fill.argtypes = [vp, C.POINTER(abi.FillDesc)]
+    L.orc_upload.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
+    L.orc_download.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
+    L.orc_count.argtypes = [vp]
+    L.orc_count.restype = C.c_int64
+    L.orc_step.argtypes = [vp, C.c_int]
+    L.orc_step_phases.argtypes = [vp, C.c_int]
+    L.orc_samples.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_counters.argtypes = [vp, C.POINTER(abi.Counters)]
+    L.orc_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_set_sigma_max.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
+    L.orc_kn.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_hybrid_set.argtypes = [vp, abi.DBL_P, C.c_double]
+    L.orc_hybrid_decompose.argtypes = [vp]
+    L.orc_smooth.argtypes = [vp, abi.DBL_P, abi.DBL_P, C.c_int]
+    L.orc_refine.argtypes = [vp, abi.I32_P]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_fast_sums.argtypes = [C.c_int]
+    L.orc_geometry.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.I32_P, abi.DBL_P, abi.I32_P, abi.DBL_P]
+    L.orc_face_data.argtypes = [vp, abi.DBL_P, abi.I32_P]
+    L.orc_inflow_nbar.argtypes = [vp, abi.DBL_P, C.c_int64, C.POINTER(C.c_int64)]
+    L.orc_usp_sample.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, abi.DBL_P, abi.DBL_P,
+                                 C.c_double, C.c_int64, C.c_uint64, abi.DBL_P, C.POINTER(C.c_double)]
+    L.orc_pair_collide.argtypes = [vp, C.c_int, C.c_int, abi.DBL_P, abi.DBL_P, C.c_double, C.c_double,
+                                   C.c_int64, C.c_uint64, abi.DBL_P, abi.DBL_P, abi.DBL_P, abi.DBL_P]
+    L.orc_usp_state.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.DBL_P]
+    L.orc_set_usp_state.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.DBL_P]
 
 
 def _check(rc):
@@ -154,6 +180,26 @@ class OracleCase:
     def set_sigma_max(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         _check(self.L.orc_set_sigma_max(self.h, abi.dptr(v), self.nc))
+
+    def pair_collide(self, p, q, UP, UQ, eP=0.0, eQ=0.0, n=1, seed=1):
+        """n collisions of one pair from the same pre-collision state."""
+        a = np.ascontiguousarray(UP, dtype=np.float64)
+        b = np.ascontiguousarray(UQ, dtype=np.float64)
+        oP, oQ = np.zeros((int(n), 3)), np.zeros((int(n), 3))
+        eA, eB = np.zeros(int(n)), np.zeros(int(n))
+        _check(self.L.orc_pair_collide(self.h, int(p), int(q), abi.dptr(a), abi.dptr(b), float(eP), float(eQ),
+                                       int(n), int(seed), abi.dptr(oP), abi.dptr(oQ), abi.dptr(eA), abi.dptr(eB)))
+        return oP, oQ, eA, eB
+
+    def usp_state(self):
+        q, sh, mp = np.zeros((self.nc, 3)), np.zeros((self.nc, 6)), np.zeros(self.nc)
+        _check(self.L.orc_usp_state(self.h, abi.dptr(q), abi.dptr(sh), abi.dptr(mp)))
+        return q, sh, mp
+
+    def set_usp_state(self, q=None, sh=None, max_prob=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (q, sh, max_prob)]
+        ptrs = [None if a is None else abi.dptr(a) for a in arrs]
+        _check(self.L.orc_set_usp_state(self.h, *ptrs))
 
     def cell_model(self):
         out = np.zeros(self.nc, dtype=np.int32)

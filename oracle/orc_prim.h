@@ -13,7 +13,7 @@
  * truncated Taylor/atanh series using only +, -, *, / and floor, so that a
  * second implementation with the same operation order (the HIP kernels)
  * rounds identically.  Checked against libm to a few ulp in
- * tests/test_oracle_math.py.  Build with -ffp-contract=off.
+ * tests/test_oracle_prims.py.  Build with -ffp-contract=off.
  */
 #ifndef ORC_PRIM_H
 #define ORC_PRIM_H

@@ -1,0 +1,164 @@
+"""Planted mistakes: each pin of the oracle must fail when a plausible error
+is written into a copy of oracle/orc.c.
+
+For every entry the test edits one snippet of orc.c (the snippet must occur
+exactly once, so an edit of the oracle that moves it fails here loudly),
+compiles the copy with the oracle's flags into a temporary library, and runs
+the named pins against that library through ``orc.using``: every one of them
+must raise AssertionError.  The unedited oracle passes the same pins in
+tests/test_oracle_pins.py and tests/test_oracle_physics.py.
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+
+import pytest
+
+import test_oracle_physics as phys
+import test_oracle_pins as pins
+from oracle import orc
+
+ORC_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+CFLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-shared"]
+
+# name: (snippet, replacement, pins that must fail)
+MUTANTS = {
+    # VSS deflection exponent R^alpha for R^(1/alpha) (variableSoftSphere.C:147)
+    "vss_deflection_exponent": (
+        "2.0 * orc_pow(orc_uniform(r), 1.0 / alpha) - 1.0",
+        "2.0 * orc_pow(orc_uniform(r), alpha) - 1.0",
+        [lambda: pins.test_vss_deflection_law("variableSoftSphere", 1.4),
+         lambda: pins.test_vss_deflection_law("LarsenBorgnakkeVariableSoftSphere", 1.4)]),
+    # BGK reference viscosity doubled (stochasticParticleBGK.C:548-552)
+    "bgk_mu_ref_doubled": (
+        "w->muref[s] = 1.25 * (1.0 + a)",
+        "w->muref[s] = 2.5 * (1.0 + a)",
+        [lambda: pins.test_bgk_relaxed_count_chapman_enskog("stochasticParticleBGK"),
+         lambda: pins.test_bgk_relaxed_count_chapman_enskog("stochasticParticleESBGK"),
+         lambda: pins.test_bgk_anisotropy_decay("stochasticParticleBGK"),
+         lambda: pins.test_bgk_anisotropy_decay("stochasticParticleESBGK")]),
+    # ES-BGK frequency without Pr (stochasticParticleESBGK.C:641)
+    "esbgk_nu_without_pr": (
+        "nu = (model == UGX_BGK_ESBGK) ? Pr * p / visc : p / visc;",
+        "nu = p / visc;",
+        [lambda: pins.test_bgk_relaxed_count_chapman_enskog("stochasticParticleESBGK"),
+         lambda: pins.test_bgk_anisotropy_decay("stochasticParticleESBGK")]),
+    # USP heat flux without the E_rot c term (unifiedStochasticParticleSBGK.C:518-550)
+    "usp_heat_flux_without_erot": (
+        "\n                               + S[UGX_S_EROTU + d] / N - (S[UGX_S_EROT] / N) * U[d])",
+        ")",
+        [lambda: pins.test_usp_relaxed_moments(1.0)]),
+    # USP heat flux scaled N/(N-1) instead of N^2/((N-1)(N-2)) (:553-559)
+    "usp_heat_flux_factor": (
+        "double f2 = N * N / (N - 1.0) / (N - 2.0);",
+        "double f2 = N / (N - 1.0);",
+        [lambda: pins.test_usp_relaxed_moments(1.0)]),
+    # USP stress relaxed by 1/(1 + nu dt) (:780)
+    "usp_stress_relaxation_full_step": (
+        "sh[k] = th * sh[k] / (1.0 + 0.5 * nu * dt)",
+        "sh[k] = th * sh[k] / (1.0 + nu * dt)",
+        [lambda: pins.test_usp_relaxed_moments(1.0)]),
+    # USP heat flux relaxed without Pr (:779)
+    "usp_heat_flux_relaxation_without_pr": (
+        "q[d] = th * q[d] / (1.0 + 0.5 * Pr * nu * dt)",
+        "q[d] = th * q[d] / (1.0 + 0.5 * nu * dt)",
+        [lambda: pins.test_usp_relaxed_moments(1.0)]),
+    # USP theta blend with the wrong weight on the previous step (:779-780)
+    "usp_theta_blend": (
+        "+ (1.0 - th) * w->uspq[3 * c + d];",
+        "+ th * w->uspq[3 * c + d];",
+        [lambda: pins.test_usp_relaxed_moments(0.7)]),
+    # USP sampling coefficient cQ without Pr (:1063)
+    "usp_cq_without_pr": (
+        "double cQ = 2.0 * (1.0 - Pr * tau * ex);",
+        "double cQ = 2.0 * (1.0 - tau * ex);",
+        [pins.test_usp_relaxation_is_trapezoidal]),
+    # USP maxProb decay 0.99 instead of 0.999 (:58, :861)
+    "usp_max_prob_decay": (
+        "m0 * 0.999",
+        "m0 * 0.99",
+        [pins.test_usp_max_prob_decay_and_raise]),
+    # NTC accepting against the running (sigma c_r)max (noTimeCounter.C:263-268)
+    "ntc_running_max": (
+        "if (scr / smax0 > orc_uniform(&r))",
+        "if (scr / smax > orc_uniform(&r))",
+        [pins.test_ntc_accepts_against_start_of_cell_max]),
+    # NTC selection number without the factor 1/2 (noTimeCounter.C:184)
+    "ntc_half_dropped": (
+        "double sel = 0.5 * (double)nC * (double)(nC - 1)",
+        "double sel = (double)nC * (double)(nC - 1)",
+        [phys.test_ntc_candidates_and_vhs_rate]),
+    # VHS cross-section exponent (variableHardSphere.C:109-114)
+    "vhs_exponent": (
+        "w->pe[k] = 1.0 - om;",
+        "w->pe[k] = 1.5 - om;",
+        [phys.test_ntc_candidates_and_vhs_rate]),
+    # Larsen-Borgnakke energy without the rotational energy of P (LB-VHS.C:301-312)
+    "lb_erot_left_out": (
+        "double Ec = Et + preP;",
+        "double Ec = Et;",
+        [lambda: phys.test_collisions_conserve_momentum_and_energy(
+            "LarsenBorgnakkeVariableHardSphere", (phys.N2, phys.O2))]),
+    # LB 2-DoF ratio exponent (uniGasCloud.C:1137-1140)
+    "lb_ratio_exponent": (
+        "if (dof == 2) return 1.0 - orc_pow(orc_uniform(r), 1.0 / chiB);",
+        "if (dof == 2) return 1.0 - orc_pow(orc_uniform(r), chiB);",
+        [phys.test_lb_rotational_equilibrium]),
+    # BGK conservation temperature with N instead of N - 1 (stochasticParticleBGK.C:761-777)
+    "bgk_conservation_n_minus_1": (
+        "/ (3.0 * KB * (N - 1.0));",
+        "/ (3.0 * KB * N);",
+        [lambda: phys.test_bgk_conserves_each_cell("stochasticParticleBGK")]),
+    # ES-BGK S matrix sign (stochasticParticleESBGK.C:902-904)
+    "esbgk_sign": (
+        "Smat[k] = dlt - a * (Pt[full[k]] / p - dlt);",
+        "Smat[k] = dlt + a * (Pt[full[k]] / p - dlt);",
+        [lambda: phys.test_bgk_strong_relaxation_limit("stochasticParticleESBGK")]),
+    # diffuse wall normal speed sqrt(-ln R) instead of sqrt(-2 ln R) (uniGasPatchBoundary.C:356-371)
+    "diffuse_normal_speed": (
+        "double un = sqrt(-2.0 * orc_log(orc_uniform(r)));",
+        "double un = sqrt(-orc_log(orc_uniform(r)));",
+        [phys.test_diffuse_walls_thermalise_gas]),
+    # inflow count without the exp(-s^2) term (uniGasGeneralBoundary.C:140-157)
+    "inflow_exp_term": (
+        "* (exp(-(sCos * sCos)) + sqrtPi",
+        "* (0.0 + sqrtPi",
+        [phys.test_inflow_count_bird_4_22]),
+}
+
+
+@pytest.fixture(scope="module")
+def mutant_libs(tmp_path_factory):
+    gcc = shutil.which(os.environ.get("CC", "gcc"))
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = open(os.path.join(ORC_DIR, "orc.c")).read()
+    base = tmp_path_factory.mktemp("mutants")
+    root = base / "oracle"           # orc.h includes ../include/ugx.h
+    os.makedirs(root)
+    os.makedirs(base / "include")
+    for name in ("orc.h", "orc_prim.h"):
+        shutil.copy(os.path.join(ORC_DIR, name), root / name)
+    shutil.copy(os.path.join(ORC_DIR, "..", "include", "ugx.h"), base / "include" / "ugx.h")
+
+    def build(name):
+        old, new, _ = MUTANTS[name]
+        assert src.count(old) == 1, f"{name}: snippet occurs {src.count(old)} times in orc.c"
+        c = root / f"{name}.c"
+        c.write_text(src.replace(old, new))
+        so = root / f"lib{name}.so"
+        subprocess.run([gcc, *CFLAGS, "-o", str(so), str(c), "-lm"], check=True, capture_output=True)
+        return name, str(so)
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        return dict(ex.map(build, sorted(MUTANTS)))
+
+
+@pytest.mark.parametrize("name", sorted(MUTANTS))
+def test_planted_mistake_is_caught(name, mutant_libs):
+    _, _, checks = MUTANTS[name]
+    with orc.using(mutant_libs[name]):
+        for i, check in enumerate(checks):
+            with pytest.raises(AssertionError):
+                check()
