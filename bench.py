@@ -249,6 +249,7 @@ def main():
         "phases": per_phase,
         "counters_per_step": {"simulators": n_avg, "candidates": cand, "collisions": coll,
                               "bgk_redrawn": n_bgk, "dsmc_cells": c1["n_dsmc_cells"]},
+        "cell_tile": case.tile_config(),
     }
     if args.probe:
         nbytes = 2 << 30

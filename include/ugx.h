@@ -370,6 +370,14 @@ const char* ugx_phase_name(int i);
 int ugx_timing_enable(ugx_case* c, int on);       /* also resets the totals */
 int ugx_timing_get(ugx_case* c, double* ms_total, int n_phases, int64_t* steps);
 
+/* Cell-tile configuration the case runs (no counterpart in the reference):
+ * its index, threads per block, key window and LDS simulators, and whether
+ * the autotuner has finished.  The tuner times each of the three
+ * configurations on one of the case's first three steps (with or without
+ * ugx_timing_enable) and keeps the fastest; results never depend on it. */
+int ugx_tile_config(ugx_case* c, int32_t* variant, int32_t* threads, int32_t* window, int32_t* lds_simulators,
+                    int32_t* tuned);
+
 /* Device-to-device copy of `bytes` bytes timed over `reps` launches: the
  * stream-copy probe that confirms the HBM peak (no reference counterpart). */
 int ugx_probe_copy(int device, int64_t bytes, int reps, double* seconds_per_copy);

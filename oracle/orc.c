@@ -50,6 +50,8 @@
 #define MAX_CROSSINGS 1000
 #define NS UGX_MAX_SPECIES
 #define NSAMP UGX_S_COUNT
+#define ORC_SQRT_HALF 7.07106781186547524401e-01   /* 1/sqrt(2): the BGK-family draws' N(0, 1/2) */
+#define ORC_THIRD (1.0 / 3.0)
 
 static __thread char g_err[512];
 const char* orc_last_error(void) { return g_err; }
@@ -715,18 +717,22 @@ static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, 
 static void usp_draw(orc_rng* rv, double cQ, double cS, double p, double u0, const double q[3], const double sh[6],
                      double m0, double* raise, double xi[3])
 {
+    /* GaussNormal()/sqrt(2), |xi|^2/3, /2.5 and prob/maxProb of the reference
+     * as products with the reciprocals, and the constants of P divided out
+     * once: a restatement that changes the last bit of some draws only */
+    const double A = cQ / (p * u0), B = cS / p, inv_m0 = 1.0 / m0;
     for (int it = 0; it < 100000; ++it) {
         double g[3];
         orc_gauss3(rv, g);
-        for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
+        for (int d = 0; d < 3; ++d) xi[d] = g[d] * ORC_SQRT_HALF;
         double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-        double vTr = vSq / 3.0;
-        double prob = 1.0 + cQ / (p * u0) * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq / 2.5 - 1.0)
-                      + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
-                                  + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
-                                  + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
+        double vTr = vSq * ORC_THIRD;
+        double prob = 1.0 + A * (q[0] * xi[0] + q[1] * xi[1] + q[2] * xi[2]) * (vSq * 0.4 - 1.0)
+                      + B * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
+                             + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
+                             + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
         if (prob > m0 && prob < 10.0) { if (prob > *raise) *raise = prob; break; }
-        if (orc_uniform(rv) < prob / m0) break;
+        if (orc_uniform(rv) < prob * inv_m0) break;
     }
 }
 
@@ -868,7 +874,7 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
             } else {
                 double g[3];
                 orc_gauss3(&rv, g);
-                for (int d = 0; d < 3; ++d) xi[d] = g[d] / sqrt(2.0);
+                for (int d = 0; d < 3; ++d) xi[d] = g[d] * ORC_SQRT_HALF;
                 if (model == UGX_BGK_ESBGK) {
                     double y[3];
                     for (int d = 0; d < 3; ++d) y[d] = Smat[3 * d] * xi[0] + Smat[3 * d + 1] * xi[1] + Smat[3 * d + 2] * xi[2];

@@ -25,7 +25,7 @@ EXPORTS = [
     "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry", "ugx_state_download", "ugx_state_upload",
-    "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_probe_copy",
+    "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_tile_config", "ugx_probe_copy",
     "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
     "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
 ]
@@ -77,6 +77,8 @@ def lib():
         L.ugx_timing_enable.argtypes = [vp, C.c_int]
         L.ugx_timing_get.argtypes = [vp, abi.DBL_P, C.c_int, C.POINTER(C.c_int64)]
         L.ugx_probe_copy.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        i32p = C.POINTER(C.c_int32)
+        L.ugx_tile_config.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
         L.ugx_batch_create.argtypes = [C.POINTER(abi.BatchOptions), C.POINTER(vp)]
         L.ugx_batch_destroy.argtypes = [vp]
         L.ugx_case_footprint.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.ModelDesc), C.c_int64, C.c_int,
@@ -319,6 +321,13 @@ class Case:
         n = C.c_int64()
         check(self.L.ugx_timing_get(self.h, abi.dptr(ms), N_PHASES, C.byref(n)), self.L)
         return dict(zip(phase_names(), ms.tolist())), n.value
+
+    def tile_config(self):
+        """The cell-tile configuration in use and whether the tuner has finished."""
+        v = [C.c_int32() for _ in range(5)]
+        check(self.L.ugx_tile_config(self.h, *[C.byref(x) for x in v]), self.L)
+        return {"variant": v[0].value, "threads": v[1].value, "window": v[2].value, "lds_simulators": v[3].value,
+                "tuned": bool(v[4].value)}
 
     def cell_volumes(self):
         out = np.zeros(self.nc)

@@ -350,6 +350,23 @@ int ugx_timing_get(ugx_case* c, double* ms_total, int n, int64_t* steps)
     UGX_GUARD_END
 }
 
+int ugx_tile_config(ugx_case* c, int32_t* variant, int32_t* threads, int32_t* window, int32_t* lds_simulators,
+                    int32_t* tuned)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    const World* w = c->world;
+    static const int32_t cfg[kTileVariants][3] = {{128, 256, 512}, {256, 512, 1024}, {64, 128, 256}};
+    const int v = w->tile_variant;
+    if (variant) *variant = v;
+    if (threads) *threads = cfg[v][0];
+    if (window) *window = cfg[v][1];
+    if (lds_simulators) *lds_simulators = cfg[v][2];
+    if (tuned) *tuned = (!w->autotune || w->tune_step >= kTileVariants) ? 1 : 0;
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
 int ugx_probe_copy(int device, int64_t bytes, int reps, double* seconds_per_copy)
 {
     UGX_GUARD_BEGIN

@@ -109,6 +109,8 @@ constexpr double kLn2Hi = 6.93147180369123816490e-01;
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 constexpr double kInvLn2 = 1.44269504088896338700e+00;
 constexpr double kTwoPi = 6.28318530717958647692e+00;
+constexpr double kSqrtHalf = 7.07106781186547524401e-01;   // 1/sqrt(2): the BGK-family draws' N(0, 1/2)
+constexpr double kThird = 1.0 / 3.0;
 
 __host__ __device__ inline double bits_to_double(uint64_t b)
 {

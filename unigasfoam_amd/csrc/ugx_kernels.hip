@@ -17,6 +17,7 @@
 //   end        step counters
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <utility>
 
 #include "ugx_kernels.hpp"
@@ -619,6 +620,8 @@ constexpr uint32_t kClassOther = 7;
 constexpr int kCountUnroll = 4;
 constexpr uint32_t kMaxTileCells = 512u / kCellW + 2u;   // cells of the largest tile variant
 constexpr int kCountTiles = 4;                           // tiles per block of k_count_tiles
+// Largest grid of the occupancy count (blocks stride over the step's tile groups).
+constexpr uint64_t kCountGrid = 8192;
 
 // Class-7 counting of the slots [begin, n + n_ins): the inflow of a step, or
 // every slot for the sort after an upload.
@@ -675,16 +678,17 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(7)))
     __shared__ uint32_t s_first[kChunks], s_last[kChunks], s_whole[kChunks];
     __shared__ uint32_t s_lastn[kChunks * 7], s_carry[kChunks * 7];
     const uint32_t ntiles = w.st->ntiles;
-    const uint32_t t0 = blockIdx.x * (uint32_t)TPB;
-    if (t0 >= ntiles) return;
-    const uint32_t t1 = min(t0 + (uint32_t)TPB, ntiles);
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
+    // the grid is sized for the largest tile table; blocks stride over the
+    // groups of this step's tiles
+    for (uint32_t t0 = blockIdx.x * (uint32_t)TPB; t0 < ntiles; t0 += gridDim.x * (uint32_t)TPB) {
+    const uint32_t t1 = min(t0 + (uint32_t)TPB, ntiles);
     const uint32_t c0 = w.tile_first[t0];
     const uint32_t nct = w.tile_first[t1] - c0;
     const uint32_t p0 = w.tile_p0[t0];
     const uint32_t size = w.tile_p0[t1] - p0;
-    if (nct == 0u) return;
+    if (nct == 0u) continue;
     for (uint32_t k = tid; k <= nct; k += Block) s_off[k] = w.off[c0 + k] - p0;
     for (uint32_t k = tid; k < nct * kMaxFaces; k += Block) {
         s_nb[k] = w.geom.nbr[(size_t)c0 * kMaxFaces + k];
@@ -804,6 +808,8 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu(7)))
         __syncthreads();
     }
     for (uint32_t k = tid; k < nct * 7u; k += Block) w.out7[(size_t)c0 * 7u + k] = s_cnt[k];
+    __syncthreads();
+    }
 }
 
 // Per cell: population (stayers, arrivals through each distinct neighbour,
@@ -1816,28 +1822,30 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
     const double u0 = P.s2kT / sqrt(w.sp.mass[cs.sp_base + ls]);
     double xi[3];
     if (P.model == UGX_BGK_USP) {
-        const double m0 = P.m0, p = P.p, cQ = P.cQ, cS = P.cS;
+        // the draw's constants are divided out once (usp_draw of the oracle)
+        const double m0 = P.m0;
+        const double A = P.cQ / (P.p * u0), B = P.cS / P.p, inv_m0 = 1.0 / m0;
         for (int it = 0; it < 100000; ++it) {
             double g0, g1, g2;
             gauss3(rv, g0, g1, g2);
-            xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
+            xi[0] = g0 * kSqrtHalf; xi[1] = g1 * kSqrtHalf; xi[2] = g2 * kSqrtHalf;
             const double vSq = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-            const double vTr = vSq / 3.0;
+            const double vTr = vSq * kThird;
             const double* sh = P.sm;
-            const double prob = 1.0 + cQ / (p * u0) * (P.q[0] * xi[0] + P.q[1] * xi[1] + P.q[2] * xi[2]) * (vSq / 2.5 - 1.0)
-                                + cS / p * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
-                                            + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
-                                            + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
+            const double prob = 1.0 + A * (P.q[0] * xi[0] + P.q[1] * xi[1] + P.q[2] * xi[2]) * (vSq * 0.4 - 1.0)
+                                + B * (sh[0] * (xi[0] * xi[0] - vTr) + sh[3] * (xi[1] * xi[1] - vTr)
+                                       + sh[5] * (xi[2] * xi[2] - vTr) + 2.0 * sh[1] * xi[0] * xi[1]
+                                       + 2.0 * sh[2] * xi[0] * xi[2] + 2.0 * sh[4] * xi[1] * xi[2]);
             if (prob > m0 && prob < 10.0) {
                 atomicMax(raise, (unsigned long long)double_to_bits(prob));
                 break;
             }
-            if (uniform(rv) < prob / m0) break;
+            if (uniform(rv) < prob * inv_m0) break;
         }
     } else {
         double g0, g1, g2;
         gauss3(rv, g0, g1, g2);
-        xi[0] = g0 / sqrt(2.0); xi[1] = g1 / sqrt(2.0); xi[2] = g2 / sqrt(2.0);
+        xi[0] = g0 * kSqrtHalf; xi[1] = g1 * kSqrtHalf; xi[2] = g2 * kSqrtHalf;
         if (P.model == UGX_BGK_ESBGK) {
             double y[3];
             for (int d = 0; d < 3; ++d) y[d] = P.sm[3 * d] * xi[0] + P.sm[3 * d + 1] * xi[1] + P.sm[3 * d + 2] * xi[2];
@@ -2652,7 +2660,9 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     mark(3);
     hipLaunchKernelGGL(k_wall<R>, dim3(grid_for(w.wq_cap, 2048)), dim3(kBlock), 0, s, w);
     mark(4);
-    hipLaunchKernelGGL((k_count_tiles<R, 256, kCountTiles>), dim3((max_tiles(w.cap, (uint64_t)w.nc) + kCountTiles - 1) / kCountTiles), dim3(256), 0, s, w);
+    hipLaunchKernelGGL((k_count_tiles<R, 256, kCountTiles>),
+                       dim3((unsigned)std::min<uint64_t>((max_tiles(w.cap, (uint64_t)w.nc) + kCountTiles - 1) / kCountTiles, kCountGrid)),
+                       dim3(256), 0, s, w);
     if (w.n_inflow > 0)
         hipLaunchKernelGGL(k_count<R>, dim3(grid_for(((uint64_t)w.max_ins + kCountUnroll - 1) / kCountUnroll)), dim3(kBlock), 0, s, w, 0);
     hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.nc)), dim3(kBlock), 0, s, w);

@@ -845,15 +845,19 @@ int World::step(int n)
                 kn_due = 1;
                 kn_passes = std::max(kn_passes, c.smoothing_passes);
             }
-        const bool tuning = autotune && !marks && tune_step < 2 * kTileVariants;
+        // the tuner times each cell-tile configuration on one of the first
+        // kTileVariants steps, with or without step timing
+        const bool tuning = autotune && tune_step < kTileVariants;
         int variant = tile_variant;
         if (tuning) {
-            if (tune_ev.empty()) {
-                tune_ev.resize(kPhases + 1);
-                for (auto& e : tune_ev) HIP_TRY(hipEventCreate(&e));
+            if (!marks) {
+                if (tune_ev.empty()) {
+                    tune_ev.resize(kPhases + 1);
+                    for (auto& e : tune_ev) HIP_TRY(hipEventCreate(&e));
+                }
+                marks = tune_ev.data();
             }
-            marks = tune_ev.data();
-            variant = tune_step / 2;
+            variant = tune_step;
         }
         const bool graph = use_graphs && !marks && !kn_due;
         if (graph && !gexec[cur]) {
@@ -882,11 +886,11 @@ int World::step(int n)
             launch_step<double>(w, stream, marks);
         }
         if (tuning) {
-            HIP_TRY(hipEventSynchronize(tune_ev[kPhases]));
+            HIP_TRY(hipEventSynchronize(marks[kPhases]));
             float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, tune_ev[6], tune_ev[7]));
+            HIP_TRY(hipEventElapsedTime(&ms, marks[6], marks[7]));
             tune_ms[variant] += ms;
-            if (++tune_step == 2 * kTileVariants) {
+            if (++tune_step == kTileVariants) {
                 int best = 0;
                 for (int v = 1; v < kTileVariants; ++v) if (tune_ms[v] < tune_ms[best]) best = v;
                 tile_variant = best;

@@ -113,7 +113,7 @@ struct World {
     // HIP-graph replay of whole steps: one executable graph per buffer parity
     // (the step's kernels and arguments are otherwise fixed for a world)
     bool use_graphs = false;
-    // autotuner: the first 2 * kTileVariants steps time each cell-tile
+    // autotuner: each of the first kTileVariants steps times one cell-tile
     // configuration on the live case (results do not depend on it)
     bool autotune = true;
     int tile_variant = 0;

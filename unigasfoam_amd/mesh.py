@@ -149,11 +149,9 @@ def _structured(X, nx, ny, nz, patch_types, names):
 
 
 def box_mesh(lo, hi, n, patch_types=None, names=None):
-    """Axis-aligned uniform box lo..hi with n=(nx,ny,nz) cells."""
-    nx, ny, nz = n
-    xs = [np.linspace(lo[a], hi[a], n[a] + 1) for a in range(3)]
-    X = np.stack(np.meshgrid(*xs, indexing="ij"), axis=-1)
-    return _structured(X, nx, ny, nz, patch_types or {}, names or {})
+    """Axis-aligned uniform box lo..hi with n=(nx,ny,nz) cells (points at
+    lo + (hi - lo) * i / n, as the blockMeshDict reader places them)."""
+    return graded_box_mesh(lo, hi, n, (1.0, 1.0, 1.0), patch_types, names)
 
 
 def graded_box_mesh(lo, hi, n, grading=(1.0, 1.0, 1.0), patch_types=None, names=None):
