@@ -265,6 +265,17 @@ int ugx_case_create(const ugx_mesh_desc* mesh,
 int ugx_case_destroy(ugx_case* c);
 
 int ugx_case_fill(ugx_case* c, const ugx_fill_desc* fill);
+/* The fill of ugx_case_fill without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  Its scratch (the
+ * fill's parameters and the per (cell, species) counts and offsets) is a
+ * fixed 16 MiB device array that each GPU holds until the process ends, so a
+ * fill may cover at most 2,097,088 (cell, species) pairs (UGX_ERR_CAPACITY
+ * beyond).  The particle capacity is never grown: the case must have been
+ * created with room for the fill (ugx_options.particle_capacity), else
+ * UGX_ERR_CAPACITY.  Calls that overlap on one GPU race on the shared
+ * scratch.  Results equal ugx_case_fill's bit for bit. */
+int ugx_case_fill_noalloc(ugx_case* c, const ugx_fill_desc* fill);
 int ugx_particles_upload(ugx_case* c, const ugx_particle_soa* p);
 int ugx_particles_download(ugx_case* c, ugx_particle_soa* p);
 int ugx_particle_count(ugx_case* c, int64_t* n);

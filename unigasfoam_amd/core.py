@@ -20,7 +20,7 @@ _lib = None
 
 # Every symbol include/ugx.h declares.
 EXPORTS = [
-    "ugx_version", "ugx_last_error", "ugx_case_create", "ugx_case_destroy", "ugx_case_fill",
+    "ugx_version", "ugx_last_error", "ugx_case_create", "ugx_case_destroy", "ugx_case_fill", "ugx_case_fill_noalloc",
     "ugx_particles_upload", "ugx_particles_download", "ugx_particle_count", "ugx_step", "ugx_sync",
     "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
@@ -51,6 +51,7 @@ def lib():
                                       C.POINTER(abi.Options), C.POINTER(vp)]
         L.ugx_case_destroy.argtypes = [vp]
         L.ugx_case_fill.argtypes = [vp, C.POINTER(abi.FillDesc)]
+        L.ugx_case_fill_noalloc.argtypes = [vp, C.POINTER(abi.FillDesc)]
         L.ugx_particles_upload.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
         L.ugx_particles_download.argtypes = [vp, C.POINTER(abi.ParticleSoA)]
         L.ugx_particle_count.argtypes = [vp, C.POINTER(C.c_int64)]
@@ -193,6 +194,12 @@ class Case:
     def fill(self, number_density, T, rot_T=None, velocity=(0.0, 0.0, 0.0), exact_per_cell=0):
         f = fill_desc(self.spec, number_density, T, rot_T, velocity, exact_per_cell)
         check(self.L.ugx_case_fill(self.h, C.byref(f)), self.L)
+
+    def fill_noalloc(self, number_density, T, rot_T=None, velocity=(0.0, 0.0, 0.0), exact_per_cell=0):
+        """The same fill with no device allocation (fixed per-GPU scratch; the
+        case's particle capacity must already hold it)."""
+        f = fill_desc(self.spec, number_density, T, rot_T, velocity, exact_per_cell)
+        check(self.L.ugx_case_fill_noalloc(self.h, C.byref(f)), self.L)
 
     def upload(self, velocity, cell, species=None, erot=None, position=None, local=None):
         soa, keep = make_upload_soa(velocity, cell, species, erot, position, local)

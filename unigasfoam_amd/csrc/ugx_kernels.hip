@@ -2757,6 +2757,30 @@ template void launch_carry_particles<float>(const Parts<float>&, const Parts<flo
 template void launch_carry_particles<double>(const Parts<double>&, const Parts<double>&, uint32_t, uint32_t, uint32_t,
                                              int32_t, int32_t, hipStream_t);
 
+__global__ void k_fill_params(FillParams p, double* d)
+{
+    const int i = threadIdx.x;
+    if (i < kNS) d[i] = p.nd[i];
+    else if (i < kNS + 3) d[i] = p.T[i - kNS];
+    else if (i < kNS + 6) d[i] = p.U[i - kNS - 3];
+}
+
+void launch_fill_params(const FillParams& p, double* d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_params, dim3(1), dim3(64), 0, s, p, d);
+}
+
+__global__ void k_set_f64(double* d, double v, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = v;
+}
+
+void launch_set_f64(double* d, double v, uint32_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_set_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, d, v, n);
+}
+
 __global__ void k_copy_probe(const float4* __restrict__ in, float4* __restrict__ out, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];

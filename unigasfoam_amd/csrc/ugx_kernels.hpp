@@ -137,6 +137,17 @@ template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int cc
                                     int exact, const double* bblo, const double* bbhi,
                                     uint32_t* fill_cnt, uint32_t* fill_off, int ns, int ncell_case);
 void launch_copy_probe(const void* in, void* out, size_t bytes, hipStream_t s);
+// Parameters of a meshFill: number densities per species, per-axis
+// temperatures and the drift velocity, stored into device memory at d
+// (kFillParams doubles: nd[kNS], T[3], U[3]) by a kernel.
+constexpr int kFillParams = kNS + 6;
+struct FillParams {
+    double nd[kNS];
+    double T[3];
+    double U[3];
+};
+void launch_fill_params(const FillParams& p, double* d, hipStream_t s);
+void launch_set_f64(double* d, double v, uint32_t n, hipStream_t s);
 template <class R>
 void launch_nb_count(const WorldView<R>& w, uint32_t* cnt, uint32_t* off, uint32_t* bsum, uint32_t* total, hipStream_t s);
 template <class R>

@@ -314,7 +314,7 @@ void World::release()
     void** ptrs[] = {(void**)&d_plane, (void**)&d_delta, (void**)&d_nbr, (void**)&d_bm, (void**)&d_inflow,
                      (void**)&d_cases, (void**)&d_cell_case, (void**)&d_vol, (void**)&d_smax, (void**)&d_cmodel,
                      (void**)&d_samp, (void**)&d_uspq, (void**)&d_usps, (void**)&d_uspmax, (void**)&d_knacc,
-                     (void**)&d_kn, (void**)&d_cc, (void**)&d_mass, (void**)&d_rotdof, (void**)&d_omega,
+                     (void**)&d_kn, (void**)&d_cc, (void**)&d_bblo, (void**)&d_bbhi, (void**)&d_mass, (void**)&d_rotdof, (void**)&d_omega,
                      (void**)&d_diam, (void**)&d_knf, (void**)&d_knaw, (void**)&d_fkind, (void**)&d_kw, (void**)&d_kw2, (void**)&d_ids2,
                      (void**)&d_nb_off, (void**)&d_nb_list,
                      (void**)&d_muref, (void**)&d_prterm, (void**)&d_local, (void**)&d_pmr, (void**)&d_pC,
@@ -691,6 +691,17 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_cell_case, cell_case))) return rc;
     if ((rc = up(d_vol, vol))) return rc;
     if ((rc = up(d_cc, cc))) return rc;
+    {
+        std::vector<double> lo((size_t)nc * 3), hi((size_t)nc * 3);
+        size_t o = 0;
+        for (const auto& sp_ : specs) {
+            std::copy(sp_.geom.bblo.begin(), sp_.geom.bblo.end(), lo.begin() + (ptrdiff_t)o);
+            std::copy(sp_.geom.bbhi.begin(), sp_.geom.bbhi.end(), hi.begin() + (ptrdiff_t)o);
+            o += sp_.geom.bblo.size();
+        }
+        if ((rc = up(d_bblo, lo))) return rc;
+        if ((rc = up(d_bbhi, hi))) return rc;
+    }
     if ((rc = up(d_cmodel, cmodel))) return rc;
     if ((rc = up(d_mass, mass))) return rc;
     if ((rc = up(d_rotdof, rotdof))) return rc;
@@ -949,7 +960,7 @@ int World::upload(int ci, const ugx_particle_soa* p, const MeshGeom& g)
     return check_state();
 }
 
-int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort)
+int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort, const FillScratch* fixed)
 {
     const CaseDev& cd = cases[ci];
     uint32_t n0 = 0;
@@ -963,30 +974,45 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort)
         for (double v : g.vol) V += v;
         expected = ntot / cd.fn * V * 1.05 + (double)g.nc * cd.ns + 64.0;
     }
-    if (int rc = ensure_capacity((uint64_t)n0 + (uint64_t)expected)) return rc;
-    int ns = cd.ns, ncl = g.nc;
-    double* d_nd = nullptr; double* d_T = nullptr; double* d_U = nullptr;
-    double* d_lo = nullptr; double* d_hi = nullptr;
-    uint32_t* d_fc = nullptr; uint32_t* d_fo = nullptr;
-    int rc = UGX_OK;
-    if ((rc = dalloc(d_nd, kNS)) || (rc = dalloc(d_T, 3)) || (rc = dalloc(d_U, 3)) || (rc = dalloc(d_lo, (size_t)ncl * 3))
-        || (rc = dalloc(d_hi, (size_t)ncl * 3)) || (rc = dalloc(d_fc, (size_t)ncl * ns + 1)) || (rc = dalloc(d_fo, (size_t)ncl * ns + 1))) {
-        dfree(d_nd); dfree(d_T); dfree(d_U); dfree(d_lo); dfree(d_hi); dfree(d_fc); dfree(d_fo);
+    if (fixed) {
+        // never grown here: the fill must fit (checked again exactly after the count)
+        if ((double)n0 + expected > (double)cap)
+            return set_err(UGX_ERR_CAPACITY, "the fill may need %.0f simulators beyond %u; the case holds %u "
+                                             "(set ugx_options.particle_capacity)", expected, n0, cap);
+    } else if (int rc = ensure_capacity((uint64_t)n0 + (uint64_t)expected)) {
         return rc;
     }
-    auto cleanup = [&]() { dfree(d_nd); dfree(d_T); dfree(d_U); dfree(d_lo); dfree(d_hi); dfree(d_fc); dfree(d_fo); };
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = hipMemcpy(d_nd, f->number_density, sizeof(double) * kNS, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_T, f->T, sizeof(double) * 3, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_U, f->velocity, sizeof(double) * 3, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_lo, g.bblo.data(), sizeof(double) * 3 * ncl, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_hi, g.bbhi.data(), sizeof(double) * 3 * ncl, hipMemcpyHostToDevice);
-    if (e != hipSuccess) { cleanup(); return set_err(UGX_ERR_DEVICE, "fill upload failed: %s", hipGetErrorString(e)); }
+    const int ns = cd.ns, ncl = g.nc;
+    const double* d_lo = d_bblo + (size_t)cd.cell_begin * 3;
+    const double* d_hi = d_bbhi + (size_t)cd.cell_begin * 3;
+    double* d_par = nullptr;
+    uint32_t* d_fc = nullptr;
+    uint32_t* d_fo = nullptr;
+    auto cleanup = [&]() { if (!fixed) { dfree(d_par); dfree(d_fc); dfree(d_fo); } };
+    FillParams fp{};
+    for (int s = 0; s < kNS; ++s) fp.nd[s] = f->number_density[s];
+    for (int d = 0; d < 3; ++d) { fp.T[d] = f->T[d]; fp.U[d] = f->velocity[d]; }
+    if (fixed) {
+        d_par = fixed->params;
+        d_fc = fixed->fcnt;
+        d_fo = fixed->foff;
+    } else {
+        int rc = UGX_OK;
+        if ((rc = dalloc(d_par, kFillParams)) || (rc = dalloc(d_fc, (size_t)ncl * ns + 1)) || (rc = dalloc(d_fo, (size_t)ncl * ns + 1))) {
+            cleanup();
+            return rc;
+        }
+    }
+    // the parameters go to the device through a kernel argument, not a copy
+    launch_fill_params(fp, d_par, stream);
+    const double* d_nd = d_par;
+    const double* d_T = d_par + kNS;
+    const double* d_U = d_par + kNS + 3;
     if (precision == UGX_FP32_STATE)
         launch_fill<float>(view<float>(), stream, ci, d_nd, d_T, f->rot_temperature, d_U, f->exact_per_cell, d_lo, d_hi, d_fc, d_fo, ns, ncl);
     else
         launch_fill<double>(view<double>(), stream, ci, d_nd, d_T, f->rot_temperature, d_U, f->exact_per_cell, d_lo, d_hi, d_fc, d_fo, ns, ncl);
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     cleanup();
     if (e != hipSuccess) return set_err(UGX_ERR_DEVICE, "fill failed: %s", hipGetErrorString(e));
@@ -1001,8 +1027,10 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort)
     const ugx_species_desc& sb = specs_host[ci].species[best];
     double Tm = (f->T[0] + f->T[1] + f->T[2]) / 3.0;
     double s0 = kPi * sb.diameter * sb.diameter * std::sqrt(2.0 * kB * Tm / sb.mass);
-    std::vector<double> sm((size_t)ncl, s0);
-    HIP_TRY(hipMemcpy(d_smax + cd.cell_begin, sm.data(), sizeof(double) * ncl, hipMemcpyHostToDevice));
+    launch_set_f64(d_smax + cd.cell_begin, s0, (uint32_t)ncl, stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (fixed) return UGX_OK;
     uint32_t n1 = 0;
     if (int r2 = count(n1)) return r2;
     return ensure_capacity((uint64_t)n1 * 2 + (uint64_t)max_ins * 8);

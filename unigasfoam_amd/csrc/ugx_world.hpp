@@ -64,6 +64,8 @@ struct World {
     double* d_knacc = nullptr;
     double* d_kn = nullptr;
     double* d_cc = nullptr;
+    double* d_bblo = nullptr;     // [nc*3] per cell bounding box in the cell frame (fill)
+    double* d_bbhi = nullptr;
     double* d_mass = nullptr;
     int32_t* d_rotdof = nullptr;
     double* d_omega = nullptr;
@@ -145,7 +147,16 @@ struct World {
     int sort_now();
     int step(int n);
     int upload(int ci, const ugx_particle_soa* p, const MeshGeom& g);
-    int fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort = true);
+    // Scratch of a fill supplied by the caller: the fill's parameters
+    // (kFillParams doubles), the per (cell, species) counts and their
+    // offsets (ncl * ns + 1 words each).  With it the fill allocates
+    // nothing and never grows the particle capacity.
+    struct FillScratch {
+        double* params;
+        uint32_t* fcnt;
+        uint32_t* foff;
+    };
+    int fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort = true, const FillScratch* fixed = nullptr);
     // Copies the state of the cases of `old` into this freshly built world:
     // new case i takes old case old_of_new[i] (or starts empty if < 0).
     int adopt(World& old, const std::vector<int>& old_of_new);
