@@ -181,7 +181,9 @@ typedef enum { UGX_FP64 = 0, UGX_FP32_STATE = 1 } ugx_precision;
 typedef struct {
     int32_t  precision;           /* ugx_precision */
     int32_t  device;              /* HIP device ordinal */
-    int64_t  particle_capacity;   /* 0 = automatic */
+    int64_t  particle_capacity;   /* simulators the buffers hold at first (0: sized by the
+                                     fill or upload); ugx_step grows them between steps
+                                     whenever a step's inflow could exceed them */
     uint64_t seed;
     int32_t  use_graphs;          /* replay each step as a captured HIP graph */
     int32_t  pad0;

@@ -1639,3 +1639,11 @@ int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const dou
     if (maxp) memcpy(w->uspmax, maxp, sizeof(double) * nc);
     return UGX_OK;
 }
+
+/* Test hook: zero the sampler sums (the sampler reset at output of
+ * uniGasVolFields.C:1432-1502, as ugx_samples_reset). */
+int orc_samples_reset(orc_world* w)
+{
+    memset(w->samp, 0, sizeof(double) * NSAMP * (size_t)w->nc);
+    return UGX_OK;
+}

@@ -79,6 +79,9 @@ int orc_pair_collide(orc_world* w, int p, int q, const double* UP, const double*
 int orc_usp_state(orc_world* w, double* q, double* sh, double* maxp);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 
+/* Test hook: zero the sampler sums (ugx_samples_reset). */
+int orc_samples_reset(orc_world* w);
+
 #ifdef __cplusplus
 }
 #endif

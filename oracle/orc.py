@@ -88,6 +88,7 @@ def _bind(L):
                                    C.c_int64, C.c_uint64, abi.DBL_P, abi.DBL_P, abi.DBL_P, abi.DBL_P]
     L.orc_usp_state.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.DBL_P]
     L.orc_set_usp_state.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.DBL_P]
+    L.orc_samples_reset.argtypes = [vp]
 
 
 def _check(rc):
@@ -166,6 +167,9 @@ class OracleCase:
         out = np.zeros((abi.S_COUNT, self.nc))
         _check(self.L.orc_samples(self.h, abi.dptr(out), out.size))
         return out
+
+    def reset_samples(self):
+        _check(self.L.orc_samples_reset(self.h))
 
     def counters(self):
         k = abi.Counters()

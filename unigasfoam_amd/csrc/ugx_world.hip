@@ -805,6 +805,8 @@ int World::count(uint32_t& n)
 {
     HIP_TRY(hipMemcpyAsync(&n, &d_st->n, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    n_bound = n;
+    n_bound_valid = true;
     return UGX_OK;
 }
 
@@ -839,6 +841,19 @@ int World::flush_timing()
 int World::step(int n)
 {
     for (int i = 0; i < n; ++i) {
+        // The capacity grows between steps, before the device could run out:
+        // a step adds at most max_ins simulators (its inflow), so while the
+        // bound plus one step's inflow fits, no synchronisation is needed.
+        // When it may not, the count is read and the buffers are regrown
+        // with room for 64 more steps of inflow (graphs are captured again).
+        if (!n_bound_valid || (max_ins > 0 && n_bound + max_ins > cap)) {
+            if (int rc = check_state()) return rc;
+            uint32_t nn = 0;
+            if (int rc = count(nn)) return rc;
+            const uint64_t need = (uint64_t)nn + 64ull * max_ins;
+            if (max_ins > 0 && need > cap)
+                if (int rc = ensure_capacity(need + need / 4)) return rc;
+        }
         hipEvent_t* marks = nullptr;
         if (timing) {
             if (tev.empty()) {
@@ -910,6 +925,7 @@ int World::step(int n)
         }
         cur = 1 - cur;
         for (auto& c : cases) c.step += 1;
+        n_bound += max_ins;
     }
     HIP_TRY(hipGetLastError());
     return UGX_OK;
@@ -955,6 +971,7 @@ int World::upload(int ci, const ugx_particle_soa* p, const MeshGeom& g)
     HIP_TRY(hipMemcpy(&d_st->n, &ntot, sizeof(uint32_t), hipMemcpyHostToDevice));
     uint32_t zero = 0;
     HIP_TRY(hipMemcpy(&d_st->n_ins, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+    n_bound_valid = false;
     if (int rc = sort_now()) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     return check_state();
@@ -1030,10 +1047,12 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort, co
     launch_set_f64(d_smax + cd.cell_begin, s0, (uint32_t)ncl, stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(stream));
-    if (fixed) return UGX_OK;
     uint32_t n1 = 0;
     if (int r2 = count(n1)) return r2;
-    return ensure_capacity((uint64_t)n1 * 2 + (uint64_t)max_ins * 8);
+    if (fixed) return UGX_OK;
+    // room for the filled simulators and 16 steps of inflow; ugx_step grows
+    // the buffers further when the inflow needs it
+    return ensure_capacity((uint64_t)n1 + (uint64_t)max_ins * 16);
 }
 
 int World::pause(int ci)
@@ -1089,6 +1108,7 @@ int World::adopt(World& old, const std::vector<int>& old_of_new)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipMemcpyAsync(&d_st->n, &dst, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    n_bound_valid = false;
     HIP_TRY(hipStreamSynchronize(stream));
     return UGX_OK;
 }

@@ -43,6 +43,11 @@ struct World {
     hipStream_t stream = nullptr;
     int32_t nc = 0;
     uint32_t cap = 0;
+    // upper bound of the simulator count at the start of the next step: the
+    // count read at the last synchronisation plus the most inflow since
+    // (valid only when n_bound_valid)
+    uint64_t n_bound = 0;
+    bool n_bound_valid = false;
     int cur = 0;
     void* pbuf[2][9] = {};
     void* d_plane = nullptr;
