@@ -237,6 +237,41 @@ typedef struct {
     double* heat_flux;         /* [3*n_cells] W/m^2 */
 } ugx_fields;
 
+/* Wall measurements: the boundaryMeasurements of every wall patch with a
+ * specular or diffuse model (boundaries/basic/uniGasPatchBoundary/
+ * uniGasPatchBoundary.C:130-302), time-summed as uniGasVolFields does
+ * (uniGasVolFields.C:800-830).  Wall faces are numbered in patch then face
+ * order.  Per face and case species, each row is the sum over sampled steps
+ * of deltaT times the step's sum over the wall hits of that species: a hit
+ * adds its before- and after-interaction terms weighted by
+ * 1/max(|U.n| A, VSMALL), and to Q and F the energy and momentum it leaves
+ * on the wall per unit area and time (F_N (before - after)/(deltaT A)).
+ * Only the hits of sampling steps are recorded.  The step's hits of a face
+ * are summed per species in the order (slot, crossing).
+ * Layout [face][UGX_MAX_SPECIES][UGX_W_COUNT]. */
+typedef enum {
+    UGX_W_RHON = 0, UGX_W_RHON_INT, UGX_W_RHOM, UGX_W_LINEAR_KE, UGX_W_MCC,
+    UGX_W_MU, UGX_W_MV, UGX_W_MW, UGX_W_EROT, UGX_W_ROTDOF,
+    UGX_W_Q, UGX_W_FX, UGX_W_FY, UGX_W_FZ,
+    UGX_W_COUNT
+} ugx_wall_index;
+
+/* Write-time wall values of the uniGasVolFields (uniGasVolFields.C:1262-1350),
+ * one entry per wall face (U_mean and force_density 3 per face).  Any
+ * pointer may be NULL. */
+typedef struct {
+    double* rhoN;
+    double* rhoM;
+    double* U_mean;
+    double* translational_T;
+    double* rotational_T;
+    double* overall_T;
+    double* heat_flux;         /* q: energy left on the wall, W/m^2 */
+    double* force_density;     /* fD: momentum left on the wall, Pa */
+    double* pressure;          /* fD . n */
+    double* shear;             /* |fD . t1, fD . t2| */
+} ugx_wall_fields;
+
 typedef struct {
     int64_t n_particles;
     int64_t steps;
@@ -297,6 +332,13 @@ int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
  * KnGLL volScalarField of localKnudsen (localKnudsen.C:395-430). */
 int ugx_kn_download(ugx_case* c, double* out, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
+/* Wall faces of the case: their count, and per face its patch and its face
+ * index within the patch. */
+int ugx_wall_face_count(ugx_case* c, int64_t* n);
+int ugx_wall_faces(ugx_case* c, int32_t* patch, int32_t* face, int64_t len);
+/* Raw wall sums, layout [face][UGX_MAX_SPECIES][UGX_W_COUNT]. */
+int ugx_wall_samples_download(ugx_case* c, double* sums, int64_t len);
+int ugx_wall_fields_compute(ugx_case* c, ugx_wall_fields* f);
 int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
 
 /* ------------------------------------------------------------ restart
@@ -325,6 +367,9 @@ typedef struct {
     double*  usp_max_prob;       /* [n_cells] USP maxProb */
     double*  kn_sums;            /* [n_cells * UGX_KN_ROWS] */
     double*  kn;                 /* [n_cells] Kn_GLL */
+    int64_t  n_wall_faces;       /* in: the case's wall face count (ugx_wall_face_count) */
+    double*  wall_samples;       /* [n_wall_faces * UGX_MAX_SPECIES * UGX_W_COUNT] wall sums;
+                                    NULL on upload: the case's wall sums are left as they are */
 } ugx_case_state;
 
 /* Copy the state out (any case) or in (a plain case; every pointer non-NULL). */

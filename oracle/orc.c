@@ -72,6 +72,17 @@ typedef struct {
     double T, rotT, U[3];
 } inflow_fs;
 
+/* One interaction with a measured wall face: velocity and E_rot before and
+ * after, the wall face and the sort key (species << 42 | slot << 10 |
+ * crossing). */
+typedef struct {
+    uint64_t key;
+    int32_t wf;
+    double u0[3], u1[3], e0, e1;
+} wall_ev;
+
+#define WROWS UGX_W_COUNT
+
 struct orc_world {
     /* mesh */
     int nc;
@@ -117,6 +128,13 @@ struct orc_world {
     double* kn;       /* [nc] */
     double kn_time;
     int kn_steps;
+    /* measured wall faces (uniGasPatchBoundary measurePropertiesAtWall_) */
+    int nwf;
+    int32_t* wface;   /* [nc*6] wall face id of a cell face, or -1 */
+    double* wgeo;     /* [nwf*4] unit outward normal, area */
+    double* wsamp;    /* [nwf*NS*WROWS] wall time sums */
+    wall_ev* ev;      /* this step's interactions */
+    int64_t nev, evcap;
     /* counters */
     ugx_counters cnt;
 };
@@ -229,6 +247,25 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
         }
         if (t == UGX_PATCH_WALL && !found) return fail(UGX_ERR_ARG, "wall patch without a boundary model");
     }
+    /* measured wall faces, numbered in patch then face order */
+    int* wid = malloc(sizeof(int) * (size_t)(nf > 0 ? nf : 1));
+    for (int f = 0; f < nf; ++f) wid[f] = -1;
+    w->nwf = 0;
+    for (int p = 0; p < m->n_patches; ++p)
+        if (m->patch_type[p] == UGX_PATCH_WALL && (w->bm[p].kind == UGX_BND_SPECULAR || w->bm[p].kind == UGX_BND_DIFFUSE))
+            for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) wid[f] = w->nwf++;
+    w->wgeo = calloc((size_t)(w->nwf > 0 ? w->nwf : 1) * 4, sizeof(double));
+    w->wsamp = calloc((size_t)(w->nwf > 0 ? w->nwf : 1) * NS * WROWS, sizeof(double));
+    w->wface = malloc(sizeof(int32_t) * (size_t)nc * MAXF);
+    for (int k = 0; k < nc * MAXF; ++k) w->wface[k] = -1;
+    for (int f = 0; f < nf; ++f) {
+        if (wid[f] < 0) continue;
+        const double* a = &A[3 * f];
+        double mag = sqrt(dot3(a, a));
+        double* gq = &w->wgeo[4 * (size_t)wid[f]];
+        for (int d = 0; d < 3; ++d) gq[d] = a[d] / mag;
+        gq[3] = mag;
+    }
     /* solution directions from empty patches */
     for (int d = 0; d < 3; ++d) w->mask[d] = 1.0;
     for (int p = 0; p < m->n_patches; ++p) {
@@ -282,6 +319,7 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
                 } else {
                     w->nbr[c * MAXF + j] = -1 - p;
                     w->fkind[c * MAXF + j] = (t == UGX_PATCH_SYMMETRY) ? FACE_SYMMETRY : FACE_BOUNDARY;
+                    w->wface[c * MAXF + j] = wid[f];
                 }
             }
         }
@@ -347,7 +385,7 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
             }
         }
     }
-    free(cf); free(pc); free(A); free(fpatch);
+    free(cf); free(pc); free(A); free(fpatch); free(wid);
     return UGX_OK;
 }
 
@@ -416,12 +454,101 @@ static void diffuse_reflect(orc_rng* r, const double nw[3], double U[3], double*
 
 /* ------------------------------------------------------------------ move */
 
+static int sampling_step(const orc_world* w)
+{
+    return ((w->step + 1) % (uint32_t)w->model.sample_interval) == 0;
+}
+
+/* Records one wall interaction of a sampling step (measurePropertiesBefore/
+ * AfterControl are called around the wall model, uniGasDiffuseWallPatch.C
+ * and uniGasSpecularWallPatch.C controlParticle). */
+static void wall_event(orc_world* w, int wf, int sp, int64_t slot, int it, const double u0[3], const double u1[3],
+                       double e0, double e1)
+{
+    wall_ev e;
+    e.key = ((uint64_t)sp << 42) | ((uint64_t)slot << 10) | (uint64_t)it;
+    e.wf = wf;
+    for (int d = 0; d < 3; ++d) { e.u0[d] = u0[d]; e.u1[d] = u1[d]; }
+    e.e0 = e0;
+    e.e1 = e1;
+#pragma omp critical(orc_wall_events)
+    {
+        if (w->nev == w->evcap) {
+            w->evcap = w->evcap ? 2 * w->evcap : 4096;
+            w->ev = realloc(w->ev, sizeof(wall_ev) * (size_t)w->evcap);
+        }
+        w->ev[w->nev++] = e;
+    }
+}
+
+static int cmp_ev(const void* a, const void* b)
+{
+    const wall_ev* p = a;
+    const wall_ev* q = b;
+    if (p->wf != q->wf) return p->wf < q->wf ? -1 : 1;
+    return (p->key < q->key) ? -1 : (p->key > q->key);
+}
+
+/* before/after terms of one velocity and E_rot, uniGasPatchBoundary.C:147-184, 223-254 */
+static void wall_terms(const double U[3], double e, const double n[3], double fA, double m, int rotdof,
+                       double a[WROWS])
+{
+    double Un = U[0] * n[0] + U[1] * n[1] + U[2] * n[2];
+    double inv = 1.0 / fmax(fabs(Un) * fA, VSMALL);
+    double UU = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+    a[UGX_W_RHON] += inv;
+    if (rotdof > 0) a[UGX_W_RHON_INT] += inv;
+    a[UGX_W_RHOM] += m * inv;
+    a[UGX_W_LINEAR_KE] += 0.5 * m * UU * inv;
+    a[UGX_W_MCC] += m * UU * inv;
+    a[UGX_W_MU] += m * U[0] * inv;
+    a[UGX_W_MV] += m * U[1] * inv;
+    a[UGX_W_MW] += m * U[2] * inv;
+    a[UGX_W_EROT] += e * inv;
+    a[UGX_W_ROTDOF] += (double)rotdof * inv;
+}
+
+/* The step's wall sums (boundaryMeasurements, uniGasPatchBoundary.C:130-302)
+ * added into the time sums at a sampling step (uniGasVolFields.C:800-830);
+ * each face's interactions are summed per species in key order. */
+static void wall_reduce(orc_world* w)
+{
+    if (w->nev == 0) return;
+    qsort(w->ev, (size_t)w->nev, sizeof(wall_ev), cmp_ev);
+    const double FN = w->model.n_equivalent_particles, dt = w->model.delta_t;
+    int64_t i = 0;
+    while (i < w->nev) {
+        const int wf = w->ev[i].wf;
+        const int sp = (int)(w->ev[i].key >> 42);
+        const double* gq = &w->wgeo[4 * (size_t)wf];
+        const double n[3] = {gq[0], gq[1], gq[2]};
+        const double fA = gq[3];
+        const double m = w->mass[sp];
+        const int rotdof = w->rotdof[sp];
+        double a[WROWS];
+        for (int r = 0; r < WROWS; ++r) a[r] = 0.0;
+        for (; i < w->nev && w->ev[i].wf == wf && (int)(w->ev[i].key >> 42) == sp; ++i) {
+            const wall_ev* e = &w->ev[i];
+            wall_terms(e->u0, e->e0, n, fA, m, rotdof, a);
+            double preIE = 0.5 * m * (e->u0[0] * e->u0[0] + e->u0[1] * e->u0[1] + e->u0[2] * e->u0[2]) + e->e0;
+            wall_terms(e->u1, e->e1, n, fA, m, rotdof, a);
+            double postIE = 0.5 * m * (e->u1[0] * e->u1[0] + e->u1[1] * e->u1[1] + e->u1[2] * e->u1[2]) + e->e1;
+            a[UGX_W_Q] += FN * (preIE - postIE) / (dt * fA);
+            for (int d = 0; d < 3; ++d) a[UGX_W_FX + d] += FN * (m * e->u0[d] - m * e->u1[d]) / (dt * fA);
+        }
+        double* dst = &w->wsamp[((size_t)wf * NS + sp) * WROWS];
+        for (int r = 0; r < WROWS; ++r) dst[r] += dt * a[r];
+    }
+    w->nev = 0;
+}
+
 /* uniGasParcel::move (parcels/uniGasParcel.C:35-108) over a convex-cell face
  * tracker, with the wall dispatch of uniGasParcel::hitWallPatch (:131-145).
  * Returns 1 if the simulator survives. */
-static int track(orc_world* w, int32_t* cellp, double x[3], double U[3], double* erot, int sp,
+static int track(orc_world* w, int32_t* cellp, double x[3], double U[3], double* erot, int sp, int64_t slot,
                  double tau, orc_rng* r, int64_t* hits, int64_t* errs)
 {
+    const int meas = w->nwf > 0 && sampling_step(w);
     int c = *cellp;
     double vt[3];
     for (int d = 0; d < 3; ++d) vt[d] = U[d] * w->mask[d];
@@ -458,6 +585,9 @@ static int track(orc_world* w, int32_t* cellp, double x[3], double U[3], double*
         const double* nw = &pl[4 * fmin];
         if (bm->kind == UGX_BND_DELETION) { *cellp = CELL_DEAD; return 0; }
         *hits += 1;
+        const int wf = meas ? w->wface[(size_t)c * MAXF + fmin] : -1;
+        const double pre[3] = {U[0], U[1], U[2]};
+        const double e0 = *erot;
         if (bm->kind == UGX_BND_SPECULAR) {
             /* uniGasPatchBoundary::specularReflection, :390-403 */
             double udn = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
@@ -466,6 +596,7 @@ static int track(orc_world* w, int32_t* cellp, double x[3], double U[3], double*
         } else {
             diffuse_reflect(r, nw, U, erot, bm->T, bm->U, w->mass[sp], w->rotdof[sp]);
         }
+        if (wf >= 0) wall_event(w, wf, sp, slot, it, pre, U, e0, *erot);
         for (int d = 0; d < 3; ++d) vt[d] = U[d] * w->mask[d];
     }
     *errs += 1;
@@ -480,7 +611,7 @@ static int track(orc_world* w, int32_t* cellp, double x[3], double U[3], double*
  * equipartition E_rot; then the first move over (1 - R) deltaT
  * (newParcel, uniGasParcel.C:47-53). */
 static int make_inflow(orc_world* w, const inflow_fs* e, orc_rng* r, double x[3], double U[3],
-                       double* erot, int32_t* cell, int64_t* hits, int64_t* errs)
+                       double* erot, int32_t* cell, int64_t slot, int64_t* hits, int64_t* errs)
 {
     double us = orc_uniform(r);
     int tri = 0;
@@ -520,7 +651,7 @@ static int make_inflow(orc_world* w, const inflow_fs* e, orc_rng* r, double x[3]
     *erot = equip_erot(r, e->rotT, w->rotdof[e->species]);
     *cell = e->cell;
     double frac = orc_uniform(r);
-    return track(w, cell, x, U, erot, e->species, (1.0 - frac) * w->model.delta_t, r, hits, errs);
+    return track(w, cell, x, U, erot, e->species, slot, (1.0 - frac) * w->model.delta_t, r, hits, errs);
 }
 
 /* ------------------------------------------------------------ collisions */
@@ -987,7 +1118,8 @@ void orc_destroy(orc_world* w)
     if (!w) return;
     void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->farea, w->fkind, w->bm, w->inflow,
                     w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
-                    w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn};
+                    w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn,
+                    w->wface, w->wgeo, w->wsamp, w->ev};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(w);
 }
@@ -1185,7 +1317,7 @@ int orc_step_phases(orc_world* w, int mask)
                 orc_rng r;
                 orc_rng_init(&r, w->seed, ORC_RNG_INFLOW_P, (uint32_t)i, (uint32_t)k, w->step);
                 w->sp[j] = e->species;
-                if (!make_inflow(w, e, &r, &w->x[3 * j], &w->v[3 * j], &w->erot[j], &w->cell[j], &hits, &errs)) dead++;
+                if (!make_inflow(w, e, &r, &w->x[3 * j], &w->v[3 * j], &w->erot[j], &w->cell[j], j, &hits, &errs)) dead++;
             }
             base += cnt[k];
         }
@@ -1195,9 +1327,10 @@ int orc_step_phases(orc_world* w, int mask)
         for (int64_t i = 0; i < n0; ++i) {
             orc_rng r;
             orc_rng_init(&r, w->seed, ORC_RNG_MOVE, 0u, (uint32_t)i, w->step);
-            if (!track(w, &w->cell[i], &w->x[3 * i], &w->v[3 * i], &w->erot[i], w->sp[i],
+            if (!track(w, &w->cell[i], &w->x[3 * i], &w->v[3 * i], &w->erot[i], w->sp[i], i,
                        w->model.delta_t, &r, &hits, &errs)) dead++;
         }
+        wall_reduce(w);
         w->cnt.inserted += nins;
         w->cnt.deleted += dead;
         w->cnt.wall_hits += hits;
@@ -1645,5 +1778,15 @@ int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const dou
 int orc_samples_reset(orc_world* w)
 {
     memset(w->samp, 0, sizeof(double) * NSAMP * (size_t)w->nc);
+    memset(w->wsamp, 0, sizeof(double) * NS * WROWS * (size_t)w->nwf);
+    return UGX_OK;
+}
+
+int64_t orc_wall_face_count(orc_world* w) { return w->nwf; }
+
+int orc_wall_samples(orc_world* w, double* out, int64_t len)
+{
+    if (len < (int64_t)w->nwf * NS * WROWS) return fail(UGX_ERR_ARG, "wall sum buffer too small");
+    memcpy(out, w->wsamp, sizeof(double) * NS * WROWS * (size_t)w->nwf);
     return UGX_OK;
 }

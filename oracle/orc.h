@@ -77,6 +77,11 @@ int orc_pair_collide(orc_world* w, int p, int q, const double* UP, const double*
 /* Test hooks: USP per-cell state (heat flux [nc*3], stress [nc*6], maxProb
  * [nc]); NULL skips an array. */
 int orc_usp_state(orc_world* w, double* q, double* sh, double* maxp);
+
+/* Wall measurements: the count of measured wall faces and their time sums,
+ * layout [face][UGX_MAX_SPECIES][UGX_W_COUNT] as ugx_wall_samples_download. */
+int64_t orc_wall_face_count(orc_world* w);
+int orc_wall_samples(orc_world* w, double* out, int64_t len);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 
 /* Test hook: zero the sampler sums (ugx_samples_reset). */

@@ -89,6 +89,9 @@ def _bind(L):
     L.orc_usp_state.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.DBL_P]
     L.orc_set_usp_state.argtypes = [vp, abi.DBL_P, abi.DBL_P, abi.DBL_P]
     L.orc_samples_reset.argtypes = [vp]
+    L.orc_wall_face_count.argtypes = [vp]
+    L.orc_wall_face_count.restype = C.c_int64
+    L.orc_wall_samples.argtypes = [vp, abi.DBL_P, C.c_int64]
 
 
 def _check(rc):
@@ -162,6 +165,15 @@ class OracleCase:
         soa, a = alloc_soa(max(n, 1))
         _check(self.L.orc_download(self.h, C.byref(soa)))
         return {k: v[: soa.n] for k, v in a.items()}
+
+    def wall_face_count(self):
+        return int(self.L.orc_wall_face_count(self.h))
+
+    def wall_samples(self):
+        """Wall time sums, [face, species, row] as Case.wall_samples."""
+        out = np.zeros((self.wall_face_count(), abi.MAX_SPECIES, abi.W_COUNT))
+        _check(self.L.orc_wall_samples(self.h, abi.dptr(out), out.size))
+        return out
 
     def samples(self):
         out = np.zeros((abi.S_COUNT, self.nc))

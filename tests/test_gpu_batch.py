@@ -33,6 +33,7 @@ def small_requests(n=5):
 
 def assert_same(a, b, what):
     assert np.array_equal(a.samples, b.samples), f"{what}: samples differ"
+    assert a.wall_samples.shape[0] == 2 and np.array_equal(a.wall_samples, b.wall_samples), f"{what}: wall sums differ"
     for k in ("n_particles", "candidates", "collisions", "bgk_collisions", "inserted", "deleted", "wall_hits",
               "particle_steps", "n_sampled_steps", "n_dsmc_cells"):
         assert a.counters[k] == b.counters[k], f"{what}: {k} {a.counters[k]} vs {b.counters[k]}"

@@ -40,6 +40,9 @@ def assert_identical(g, o, what=""):
     assert np.array_equal(g.sigma_max(), o.sigma_max()), f"{what}: sigma_max differs"
     sg, so = g.samples(), o.samples()
     assert np.array_equal(sg, so), f"{what}: samples differ (max abs {np.max(np.abs(sg - so))})"
+    wg, wo = g.wall_samples(), o.wall_samples()
+    assert wg.shape == wo.shape, f"{what}: wall faces {wg.shape[0]} vs {wo.shape[0]}"
+    assert np.array_equal(wg, wo), f"{what}: wall sums differ (max abs {np.max(np.abs(wg - wo))})"
 
 
 def test_fill_identical(require_gpu):

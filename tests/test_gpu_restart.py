@@ -29,7 +29,7 @@ def results(c):
         out["c_" + k] = cnt[k]
     st = c.state()
     out.update(kn_sums=st["kn_sums"], usp_heat_flux=st["usp_heat_flux"], usp_stress=st["usp_stress"],
-               usp_max_prob=st["usp_max_prob"])
+               usp_max_prob=st["usp_max_prob"], wall_samples=c.wall_samples())
     return out
 
 
@@ -76,7 +76,7 @@ def test_restart_couette_lb(require_gpu):
     spec, fill = workloads.couette(n_cells=40, per_cell=30)
     ra, rb = run_restart(spec, fill, before=10, after=15)
     assert_same(ra, rb, "couette LB")
-    assert ra["c_collisions"] > 0 and ra["c_wall_hits"] > 0
+    assert ra["c_collisions"] > 0 and ra["c_wall_hits"] > 0 and ra["wall_samples"].any()
 
 
 @pytest.mark.parametrize("bgk", ["stochasticParticleESBGK", "unifiedStochasticParticleSBGK"])

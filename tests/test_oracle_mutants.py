@@ -17,6 +17,7 @@ import pytest
 
 import test_oracle_physics as phys
 import test_oracle_pins as pins
+import test_oracle_wall as wall
 from oracle import orc
 
 ORC_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
@@ -125,6 +126,43 @@ MUTANTS = {
         "* (exp(-(sCos * sCos)) + sqrtPi",
         "* (0.0 + sqrtPi",
         [phys.test_inflow_count_bird_4_22]),
+    # wall estimator without the face area (uniGasPatchBoundary.C:155)
+    "wall_weight_without_area": (
+        "double inv = 1.0 / fmax(fabs(Un) * fA, VSMALL);",
+        "double inv = 1.0 / fmax(fabs(Un), VSMALL);",
+        [lambda: wall.test_diffuse_wall_sums_match_kinetic_theory(wall.AR),
+         wall.test_specular_wall_is_exact_and_gives_the_pressure]),
+    # wall heat flux after minus before (uniGasPatchBoundary.C:295)
+    "wall_heat_flux_sign": (
+        "a[UGX_W_Q] += FN * (preIE - postIE) / (dt * fA);",
+        "a[UGX_W_Q] += FN * (postIE - preIE) / (dt * fA);",
+        [lambda: wall.test_diffuse_wall_sums_match_kinetic_theory(wall.AR)]),
+    # wall energy without E_rot (uniGasPatchBoundary.C:187-190, 257-260)
+    "wall_energy_without_erot": (
+        "e->u0[2] * e->u0[2]) + e->e0;",
+        "e->u0[2] * e->u0[2]);",
+        [lambda: wall.test_diffuse_wall_sums_match_kinetic_theory(wall.N2)]),
+    # wall force from the momentum after minus before (uniGasPatchBoundary.C:296)
+    "wall_force_sign": (
+        "a[UGX_W_FX + d] += FN * (m * e->u0[d] - m * e->u1[d]) / (dt * fA);",
+        "a[UGX_W_FX + d] += FN * (m * e->u1[d] - m * e->u0[d]) / (dt * fA);",
+        [wall.test_moving_wall_tangential_force, wall.test_specular_wall_is_exact_and_gives_the_pressure]),
+    # only the before-interaction terms measured (uniGasPatchBoundary.C:217-254)
+    "wall_after_terms_dropped": (
+        "        wall_terms(e->u1, e->e1, n, fA, m, rotdof, a);\n",
+        "",
+        [lambda: wall.test_diffuse_wall_sums_match_kinetic_theory(wall.N2),
+         wall.test_moving_wall_tangential_force]),
+    # E_rot before and after swapped (uniGasPatchBoundary.C:177, 247)
+    "wall_erot_swapped": (
+        "wall_event(w, wf, sp, slot, it, pre, U, e0, *erot);",
+        "wall_event(w, wf, sp, slot, it, pre, U, *erot, e0);",
+        [lambda: wall.test_diffuse_wall_sums_match_kinetic_theory(wall.N2)]),
+    # every step summed, not only the sampling steps (uniGasVolFields.C:728-830)
+    "wall_every_step": (
+        "const int meas = w->nwf > 0 && sampling_step(w);",
+        "const int meas = w->nwf > 0;",
+        [wall.test_only_sampling_steps_are_summed]),
 }
 
 

@@ -125,6 +125,7 @@ class CaseState(C.Structure):
         ("sampled_at_reset", C.c_int64), ("counters", C.c_uint64 * N_COUNTERS),
         ("sigma_max", DBL_P), ("cell_model", I32_P), ("samples", DBL_P), ("usp_heat_flux", DBL_P),
         ("usp_stress", DBL_P), ("usp_max_prob", DBL_P), ("kn_sums", DBL_P), ("kn", DBL_P),
+        ("n_wall_faces", C.c_int64), ("wall_samples", DBL_P),
     ]
 
 
@@ -132,6 +133,18 @@ class CaseState(C.Structure):
 STATE_ARRAYS = (("sigma_max", 1, "f8"), ("cell_model", 1, "i4"), ("samples", S_COUNT, "f8"),
                 ("usp_heat_flux", 3, "f8"), ("usp_stress", 6, "f8"), ("usp_max_prob", 1, "f8"),
                 ("kn_sums", KN_ROWS, "f8"), ("kn", 1, "f8"))
+
+
+# ugx_wall_index: rows of the wall sums per (face, species)
+(W_RHON, W_RHON_INT, W_RHOM, W_LINEAR_KE, W_MCC, W_MU, W_MV, W_MW, W_EROT, W_ROTDOF,
+ W_Q, W_FX, W_FY, W_FZ, W_COUNT) = range(15)
+
+WALL_FIELD_NAMES = ("rhoN", "rhoM", "U_mean", "translational_T", "rotational_T", "overall_T",
+                    "heat_flux", "force_density", "pressure", "shear")
+
+
+class WallFields(C.Structure):
+    _fields_ = [(n, DBL_P) for n in WALL_FIELD_NAMES]
 
 
 class Counters(C.Structure):

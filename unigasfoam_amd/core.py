@@ -25,6 +25,7 @@ EXPORTS = [
     "ugx_samples_download", "ugx_samples_reset", "ugx_fields_compute", "ugx_counters_get",
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry", "ugx_state_download", "ugx_state_upload",
+    "ugx_wall_face_count", "ugx_wall_faces", "ugx_wall_samples_download", "ugx_wall_fields_compute",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_tile_config", "ugx_probe_copy",
     "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
     "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
@@ -67,6 +68,10 @@ def lib():
         L.ugx_state_download.argtypes = [vp, C.POINTER(abi.CaseState)]
         L.ugx_state_upload.argtypes = [vp, C.POINTER(abi.CaseState)]
         L.ugx_kn_download.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_wall_face_count.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.ugx_wall_faces.argtypes = [vp, abi.I32_P, abi.I32_P, C.c_int64]
+        L.ugx_wall_samples_download.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_wall_fields_compute.argtypes = [vp, C.POINTER(abi.WallFields)]
         L.ugx_cell_volumes.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_centres.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_geometry.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpeciesDesc), C.c_int,
@@ -248,6 +253,36 @@ class Case:
         arrs["shear_stress"] = arrs["shear_stress"].reshape(n, 6)
         return arrs
 
+    def wall_face_count(self):
+        n = C.c_int64()
+        check(self.L.ugx_wall_face_count(self.h, C.byref(n)), self.L)
+        return n.value
+
+    def wall_faces(self):
+        """(patch, face within the patch) of every measured wall face."""
+        n = self.wall_face_count()
+        patch = np.zeros(n, dtype=np.int32)
+        face = np.zeros(n, dtype=np.int32)
+        check(self.L.ugx_wall_faces(self.h, abi.iptr(patch), abi.iptr(face), n), self.L)
+        return patch, face
+
+    def wall_samples(self):
+        """Raw wall sums, [face, species, row] (rows: abi.W_*)."""
+        n = self.wall_face_count()
+        out = np.zeros((n, abi.MAX_SPECIES, abi.W_COUNT))
+        check(self.L.ugx_wall_samples_download(self.h, abi.dptr(out), out.size), self.L)
+        return out
+
+    def wall_fields(self):
+        """Write-time wall values per measured wall face."""
+        n = self.wall_face_count()
+        arrs = {k: np.zeros((n, 3) if k in ("U_mean", "force_density") else n) for k in abi.WALL_FIELD_NAMES}
+        f = abi.WallFields()
+        for k in abi.WALL_FIELD_NAMES:
+            setattr(f, k, abi.dptr(arrs[k]))
+        check(self.L.ugx_wall_fields_compute(self.h, C.byref(f)), self.L)
+        return arrs
+
     def sigma_max(self):
         out = np.zeros(self.nc)
         check(self.L.ugx_sigma_max_download(self.h, abi.dptr(out), self.nc), self.L)
@@ -272,12 +307,18 @@ class Case:
         s.n_cells = self.nc
         for name, _, dt in abi.STATE_ARRAYS:
             setattr(s, name, abi.iptr(arrs[name]) if dt == "i4" else abi.dptr(arrs[name]))
+        w = arrs.get("wall_samples")
+        if w is not None:
+            s.n_wall_faces = w.shape[0]
+            s.wall_samples = abi.dptr(w)
         return s
 
     def state(self):
         """Restart state besides the simulators (ugx_state_download): step
-        count, hybrid time, counters and the per-cell arrays (cell-major)."""
+        count, hybrid time, counters, the per-cell arrays (cell-major) and
+        the wall sums."""
         arrs = {name: np.zeros((self.nc, rows), dtype=dt) for name, rows, dt in abi.STATE_ARRAYS}
+        arrs["wall_samples"] = np.zeros((self.wall_face_count(), abi.MAX_SPECIES, abi.W_COUNT))
         s = self._state_struct(arrs)
         check(self.L.ugx_state_download(self.h, C.byref(s)), self.L)
         arrs.update(steps=np.int64(s.steps), kn_time=np.float64(s.kn_time),
@@ -289,6 +330,9 @@ class Case:
         """Upload a state from state() (ugx_state_upload)."""
         arrs = {name: np.ascontiguousarray(np.asarray(st[name], dtype=dt).reshape(self.nc, rows))
                 for name, rows, dt in abi.STATE_ARRAYS}
+        if st.get("wall_samples") is not None:
+            arrs["wall_samples"] = np.ascontiguousarray(np.asarray(st["wall_samples"], dtype=np.float64).reshape(
+                -1, abi.MAX_SPECIES, abi.W_COUNT))
         s = self._state_struct(arrs)
         s.steps = int(st["steps"])
         s.kn_time = float(st["kn_time"])

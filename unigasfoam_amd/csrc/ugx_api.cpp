@@ -321,6 +321,53 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len)
     UGX_GUARD_END
 }
 
+int ugx_wall_face_count(ugx_case* c, int64_t* n)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (!n) return set_err(UGX_ERR_ARG, "null output");
+    *n = c->world->specs_host[c->index].geom.nwf;
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
+int ugx_wall_faces(ugx_case* c, int32_t* patch, int32_t* face, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    const MeshGeom& g = c->world->specs_host[c->index].geom;
+    if (len < (int64_t)g.nwf) return set_err(UGX_ERR_ARG, "buffer holds %lld, need %d", (long long)len, g.nwf);
+    if (patch && g.nwf) std::memcpy(patch, g.wpatch.data(), sizeof(int32_t) * g.nwf);
+    if (face && g.nwf) std::memcpy(face, g.wlocal.data(), sizeof(int32_t) * g.nwf);
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
+int ugx_wall_samples_download(ugx_case* c, double* sums, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (!sums) return set_err(UGX_ERR_ARG, "null output");
+    if (int rc = c->world->check_state()) return rc;
+    return c->world->case_wall_samples(c->index, sums, len, false);
+    UGX_GUARD_END
+}
+
+int ugx_wall_fields_compute(ugx_case* c, ugx_wall_fields* f)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (!f) return set_err(UGX_ERR_ARG, "null output");
+    const CaseSpec& spec = c->world->specs_host[c->index];
+    std::vector<double> s((size_t)spec.geom.nwf * kNS * kWRows);
+    if (int rc = c->world->check_state()) return rc;
+    if (int rc = c->world->case_wall_samples(c->index, s.data(), (int64_t)s.size(), false)) return rc;
+    ugx_counters k;
+    if (int rc = c->world->case_counters(c->index, &k)) return rc;
+    return compute_wall_fields(spec, s.data(), k, f);
+    UGX_GUARD_END
+}
+
 static const char* kPhaseNames[kPhases] = {"begin", "inflow", "move", "wall", "scan", "scatter", "cells", "collide", "end"};
 
 const char* ugx_phase_name(int i) { return (i >= 0 && i < kPhases) ? kPhaseNames[i] : ""; }

@@ -34,6 +34,8 @@ constexpr uint32_t kDead = 0xFFFFFFFFu;
 constexpr int kMaxCrossings = 1000;
 constexpr int kNS = UGX_MAX_SPECIES;
 constexpr int kNSamp = UGX_S_COUNT;
+// wall-face geometry record: unit outward normal (3), area, tangent t1 (3), pad
+constexpr int kWGeo = 8;
 
 enum RngKind : uint32_t {
     kRngMove = 1, kRngNtc = 2, kRngBgk = 3, kRngInflowN = 4, kRngInflowP = 5, kRngFill = 6
@@ -403,38 +405,54 @@ struct TrackOut {
 struct MassOf {
     double mass;
     int rotdof;
+    uint32_t sp;      // global species id
 };
 
-__device__ inline void wall_species(const MassOf& m, uint32_t, double& erot, double& mass, int& rotdof, bool&)
+__device__ inline void wall_species(const MassOf& m, uint32_t, double& erot, double& mass, int& rotdof, uint32_t& sp,
+                                    bool&)
 {
     mass = m.mass;
     rotdof = m.rotdof;
+    sp = m.sp;
     (void)erot;
 }
 
 template <class WV>
-__device__ inline void wall_species(const WV& w, uint32_t t, double& erot, double& mass, int& rotdof, bool& loaded)
+__device__ inline void wall_species(const WV& w, uint32_t t, double& erot, double& mass, int& rotdof, uint32_t& sp,
+                                    bool& loaded)
 {
     if (!loaded) {
         const uint32_t g = w.a.sp[t];
         mass = w.sp.mass[g];
         rotdof = w.sp.rotdof[g];
         erot = (double)w.a.erot[t];
+        sp = g;
         loaded = true;
     }
 }
+
+// A wall-hit recorder for track_t: measured(c, f) tells whether face f of
+// cell c is a measured wall face, hit(...) records one interaction (cell,
+// face, slot, crossing number, velocity and E_rot before and after, global
+// species).  NoSink records nothing.
+struct NoSink {
+    __device__ bool measured(uint32_t, int) const { return false; }
+    __device__ void hit(uint32_t, int, uint32_t, int, const double*, const double*, double, double, uint32_t) const {}
+};
 
 // Defer = true: stop at the first diffuse wall (position on the wall, the
 // remaining time in tau_io) and report it as queued; the caller finishes
 // the track with the wall model later.  Nothing random is drawn before a
 // diffuse wall, so the continuation uses a freshly initialised stream.
-template <bool Defer, class R, class Src, class V = double>
+template <bool Defer, class R, class Src, class V = double, class Sink = NoSink>
 __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const double mask[3],
                                    uint32_t& cell, R x[3], V U[3], double& erot,
-                                   const Src& src, uint32_t slot, R& tau, Rng& rng, int it0)
+                                   const Src& src, uint32_t slot, R& tau, Rng& rng, int it0,
+                                   const Sink& sink = Sink{})
 {
     double mass = 0.0;
     int rotdof = 0;
+    uint32_t spg = 0u;
     bool loaded = false;
     TrackOut out{1, 0, 0, 0, 0, 0};
     uint32_t c = cell;
@@ -494,8 +512,10 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
             return out;
         }
         out.hits += 1;
+        const bool meas = sink.measured(c, fmin);
+        const double pre[3] = {(double)U[0], (double)U[1], (double)U[2]};
         if (b.kind == UGX_BND_SPECULAR) {
-            // mirror; E_rot and species are not needed
+            // mirror; E_rot and species are needed only to measure the hit
             const V n0 = (V)qx, n1 = (V)qy, n2 = (V)qz;
             V udn = U[0] * n0 + U[1] * n1 + U[2] * n2;
             if (udn > (V)0) {
@@ -503,11 +523,21 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
                 U[1] = U[1] - (V)2 * udn * n1;
                 U[2] = U[2] - (V)2 * udn * n2;
             }
+            if (meas) {
+                wall_species(src, slot, erot, mass, rotdof, spg, loaded);
+                const double post[3] = {(double)U[0], (double)U[1], (double)U[2]};
+                sink.hit(c, fmin, slot, it, pre, post, erot, erot, spg);
+            }
         } else if constexpr (!Defer) {
-            wall_species(src, slot, erot, mass, rotdof, loaded);
+            wall_species(src, slot, erot, mass, rotdof, spg, loaded);
+            const double e0 = erot;
             double Ud[3] = {(double)U[0], (double)U[1], (double)U[2]};
             diffuse_reflect(rng, nw, Ud, erot, b.T, b.U, mass, rotdof);
             U[0] = (V)Ud[0]; U[1] = (V)Ud[1]; U[2] = (V)Ud[2];
+            if (meas) {
+                const double post[3] = {(double)U[0], (double)U[1], (double)U[2]};
+                sink.hit(c, fmin, slot, it, pre, post, e0, erot, spg);
+            }
         }
         for (int d = 0; d < 3; ++d) vt[d] = (R)(U[d] * (V)mask[d]);
     }
@@ -516,32 +546,38 @@ __device__ inline TrackOut track_t(const Geom<R>& g, const BModel* bm, const dou
     return out;
 }
 
-template <class R, class Src>
+template <class R, class Src, class Sink = NoSink>
 __device__ inline TrackOut track(const Geom<R>& g, const BModel* bm, const double mask[3],
                                  uint32_t& cell, R x[3], double U[3], double& erot,
-                                 const Src& src, uint32_t slot, double tau_d, Rng& rng)
+                                 const Src& src, uint32_t slot, double tau_d, Rng& rng, const Sink& sink = Sink{})
 {
     R tau = (R)tau_d;
-    return track_t<false, R, Src>(g, bm, mask, cell, x, U, erot, src, slot, tau, rng, 0);
+    return track_t<false, R, Src, double, Sink>(g, bm, mask, cell, x, U, erot, src, slot, tau, rng, 0, sink);
 }
 
 // Diffuse reflection at face f of cell c, then the rest of the track:
 // the continuation of a deferred track_t<true>.
-template <class R, class Src>
+template <class R, class Src, class Sink = NoSink>
 __device__ inline TrackOut finish_diffuse(const Geom<R>& g, const BModel* bm, const double mask[3],
                                           uint32_t& cell, int f, int it, R x[3], double U[3], double& erot,
-                                          const Src& src, uint32_t slot, R tau, Rng& rng)
+                                          const Src& src, uint32_t slot, R tau, Rng& rng, const Sink& sink = Sink{})
 {
     double mass = 0.0;
     int rotdof = 0;
+    uint32_t spg = 0u;
     bool loaded = false;
-    wall_species(src, slot, erot, mass, rotdof, loaded);
+    wall_species(src, slot, erot, mass, rotdof, spg, loaded);
     const size_t k = (size_t)cell * kMaxFaces + f;
     const BModel b = bm[-1 - g.nbr[k]];
     V4<R> q = g.plane[k];
     double nw[3] = {(double)q.x, (double)q.y, (double)q.z};
+    const double pre[3] = {U[0], U[1], U[2]};
+    const double e0 = erot;
     diffuse_reflect(rng, nw, U, erot, b.T, b.U, mass, rotdof);
-    TrackOut o = track_t<false, R, Src>(g, bm, mask, cell, x, U, erot, src, slot, tau, rng, it + 1);
+    if (sink.measured(cell, f)) sink.hit(cell, f, slot, it, pre, U, e0, erot, spg);
+    // the rest of the track carries this E_rot (not the stored one)
+    TrackOut o = track_t<false, R, MassOf, double, Sink>(g, bm, mask, cell, x, U, erot, MassOf{mass, rotdof, spg}, slot,
+                                                         tau, rng, it + 1, sink);
     o.hits += 1;
     return o;
 }

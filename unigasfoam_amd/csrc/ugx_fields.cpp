@@ -125,4 +125,55 @@ int compute_fields(const CaseSpec& spec, const double* A, int ncl, const ugx_cou
     return UGX_OK;
 }
 
+// Wall values (uniGasVolFields.C:1262-1350) per measured wall face, the
+// species sums added together; faces never hit give zeros, as there.
+int compute_wall_fields(const CaseSpec& spec, const double* W, const ugx_counters& cnt, ugx_wall_fields* f)
+{
+    const MeshGeom& g = spec.geom;
+    const double tAv = cnt.time_sampled;
+    const double FN = spec.model.n_equivalent_particles;
+    for (int k = 0; k < g.nwf; ++k) {
+        double a[kWRows] = {};
+        for (int s = 0; s < spec.ns; ++s)
+            for (int r = 0; r < kWRows; ++r) a[r] += W[((size_t)k * kNS + s) * kWRows + r];
+        double rhoN = 0.0, rhoM = 0.0, U[3] = {0.0, 0.0, 0.0}, T = 0.0;
+        if (a[UGX_W_RHON] > kVSmall && tAv > 0.0) {
+            rhoN = a[UGX_W_RHON] * FN / tAv;
+            rhoM = a[UGX_W_RHOM] * FN / tAv;
+            for (int d = 0; d < 3; ++d) U[d] = a[UGX_W_MU + d] * FN / (rhoM * tAv);
+            const double rhoMMean = a[UGX_W_RHOM] * FN / tAv;
+            const double keMean = a[UGX_W_LINEAR_KE] * FN / tAv;
+            const double rhoNMean = a[UGX_W_RHON] * FN / tAv;
+            T = 2.0 / (3.0 * kB * rhoNMean) * (keMean - 0.5 * rhoMMean * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]));
+        }
+        const double Trot = (a[UGX_W_ROTDOF] > kVSmall) ? (2.0 / kB) * (a[UGX_W_EROT] / a[UGX_W_ROTDOF]) : 0.0;
+        const double nRotDof = (a[UGX_W_RHON] > kVSmall) ? a[UGX_W_ROTDOF] / a[UGX_W_RHON] : 0.0;
+        const double Tov = (3.0 * T + nRotDof * Trot) / (3.0 + nRotDof);
+        double q = 0.0, fD[3] = {0.0, 0.0, 0.0};
+        if (tAv > 0.0) {
+            q = a[UGX_W_Q] / tAv;
+            for (int d = 0; d < 3; ++d) fD[d] = a[UGX_W_FX + d] / tAv;
+        }
+        const double* gq = &g.wgeo[(size_t)k * kWGeo];
+        const double n[3] = {gq[0], gq[1], gq[2]}, t1[3] = {gq[4], gq[5], gq[6]};
+        double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+        const double m2 = std::sqrt(t2[0] * t2[0] + t2[1] * t2[1] + t2[2] * t2[2]);
+        if (m2 > kVSmall) for (int d = 0; d < 3; ++d) t2[d] /= m2;
+        const double fn = fD[0] * n[0] + fD[1] * n[1] + fD[2] * n[2];
+        const double f1 = fD[0] * t1[0] + fD[1] * t1[1] + fD[2] * t1[2];
+        const double f2 = fD[0] * t2[0] + fD[1] * t2[1] + fD[2] * t2[2];
+        if (f->rhoN) f->rhoN[k] = rhoN;
+        if (f->rhoM) f->rhoM[k] = rhoM;
+        if (f->U_mean) for (int d = 0; d < 3; ++d) f->U_mean[3 * (size_t)k + d] = U[d];
+        if (f->translational_T) f->translational_T[k] = T;
+        if (f->rotational_T) f->rotational_T[k] = Trot;
+        if (f->overall_T) f->overall_T[k] = Tov;
+        if (f->heat_flux) f->heat_flux[k] = q;
+        if (f->force_density) for (int d = 0; d < 3; ++d) f->force_density[3 * (size_t)k + d] = fD[d];
+        if (f->pressure) f->pressure[k] = fn;
+        if (f->shear) f->shear[k] = std::sqrt(f1 * f1 + f2 * f2);
+    }
+    return UGX_OK;
+}
+
 }  // namespace ugx

@@ -152,7 +152,11 @@ __global__ void k_begin_step(WorldView<R> w)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < w.ncases) w.cases[c].particle_begin = w.off[w.cases[c].cell_begin];
-    if (c == 0) { w.st->n_ins = 0u; w.st->nwall = 0u; w.st->nsel = 0u; }
+    if (c == 0) {
+        w.st->n_ins = 0u; w.st->nwall = 0u; w.st->nsel = 0u;
+        w.st->nev_peak = max(w.st->nev_peak, w.st->nev);
+        w.st->nev = 0u;
+    }
 }
 
 // ------------------------------------------------------------- inflow
@@ -208,7 +212,7 @@ __global__ void k_inflow_count(WorldView<R> w)
 // Bird eq. 12.5 insertion + first move over (1 - R) deltaT.
 template <class R>
 __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, const CaseDev& cs, Rng& r,
-                                  R x[3], double U[3], double& erot, uint32_t& cell, TrackOut& to)
+                                  R x[3], double U[3], double& erot, uint32_t& cell, TrackOut& to, uint32_t slot)
 {
     double us = uniform(r);
     int tri = 0;
@@ -249,7 +253,8 @@ __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, cons
     erot = equip_erot(r, e.rotT, rd);
     cell = (uint32_t)e.cell;
     double frac = uniform(r);
-    to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, MassOf{m, rd}, 0u, (1.0 - frac) * cs.dt, r);
+    to = track<R>(w.geom, w.bm, cs.mask, cell, x, U, erot, MassOf{m, rd, (uint32_t)g}, slot, (1.0 - frac) * cs.dt, r,
+                  wall_sink(w));
     return to.alive;
 }
 
@@ -280,8 +285,8 @@ __global__ void __launch_bounds__(kBlock) k_inflow_gen(WorldView<R> w)
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngInflowP, i, (uint32_t)e.local_id, cs.step);
             R x[3];
             double U[3], erot;
-            make_inflow<R>(w, e, cs, r, x, U, erot, cell, to);
             uint32_t j = n0 + t;
+            make_inflow<R>(w, e, cs, r, x, U, erot, cell, to, j);
             w.a.x[j] = x[0]; w.a.y[j] = x[1]; w.a.z[j] = x[2];
             w.a.vx[j] = (R)U[0]; w.a.vy[j] = (R)U[1]; w.a.vz[j] = (R)U[2];
             w.a.erot[j] = (R)erot;
@@ -522,11 +527,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                         next = false;
                     } else {
                         hits += 1;
+                        const double pre[3] = {(double)U[0], (double)U[1], (double)U[2]};
                         const R udn = U[0] * qx + U[1] * qy + U[2] * qz;
                         if (udn > (R)0) {
                             U[0] = U[0] - (R)2 * udn * qx;
                             U[1] = U[1] - (R)2 * udn * qy;
                             U[2] = U[2] - (R)2 * udn * qz;
+                        }
+                        const WallSink sk = wall_sink(w);
+                        if (sk.measured(c, fmin)) {
+                            const double post[3] = {(double)U[0], (double)U[1], (double)U[2]};
+                            const double er = (double)w.a.erot[t];
+                            sk.hit(c, fmin, t, it, pre, post, er, er, w.a.sp[t]);
                         }
                     }
                 }
@@ -586,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(WorldView<R> w)
             Rng r;
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngMove, 0u, t - cs.particle_begin, cs.step);
             to = finish_diffuse<R>(w.geom, w.bm, cs.mask, cell, (int)(info & 7u), (int)(info >> 3), x, U, erot,
-                                   w, t, (R)w.wq_tau[q], r);
+                                   w, t, (R)w.wq_tau[q], r, wall_sink(w));
             w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
             w.a.cell[t] = cell;
             w.a.vx[t] = (R)U[0]; w.a.vy[t] = (R)U[1]; w.a.vz[t] = (R)U[2];
@@ -599,6 +611,112 @@ __global__ void __launch_bounds__(kBlock) k_wall(WorldView<R> w)
         }
     }
     tcount_block_flush(tc, w.counters);
+}
+
+// ------------------------------------------------------ wall measurements
+// The step's wall hits (events) go to per-face buckets; each face sorts its
+// bucket by (species, slot, crossing) and sums it in that order, so the sums
+// do not depend on the order in which the hits were recorded
+// (uniGasPatchBoundary.C:130-302 measures in the order of the cloud's list).
+
+template <class R>
+__global__ void k_wall_count(WorldView<R> w)
+{
+    const uint32_t n = min(w.st->nev, w.ev_cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&w.wcnt[w.ev[i].wf], 1u);
+}
+
+template <class R>
+__global__ void k_wall_bucket(WorldView<R> w)
+{
+    const uint32_t n = min(w.st->nev, w.ev_cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t wf = w.ev[i].wf;
+        const uint32_t pos = w.woff[wf] + atomicAdd(&w.wcnt[wf], 1u);
+        w.bkey[pos] = w.ev_key[i];
+        w.bidx[pos] = i;
+    }
+}
+
+// In-place heap sort of n keys (ascending), carrying their values.
+__device__ inline void heap_sort(unsigned long long* k, uint32_t* v, uint32_t n)
+{
+    auto sift = [&](uint32_t root, uint32_t end) {
+        for (;;) {
+            uint32_t ch = 2u * root + 1u;
+            if (ch >= end) break;
+            if (ch + 1u < end && k[ch] < k[ch + 1u]) ch += 1u;
+            if (!(k[root] < k[ch])) break;
+            const unsigned long long tk = k[root]; k[root] = k[ch]; k[ch] = tk;
+            const uint32_t tv = v[root]; v[root] = v[ch]; v[ch] = tv;
+            root = ch;
+        }
+    };
+    for (uint32_t s = n / 2u; s-- > 0u;) sift(s, n);
+    for (uint32_t e = n; e-- > 1u;) {
+        const unsigned long long tk = k[0]; k[0] = k[e]; k[e] = tk;
+        const uint32_t tv = v[0]; v[0] = v[e]; v[e] = tv;
+        sift(0u, e);
+    }
+}
+
+// measurePropertiesBeforeControl / AfterControl terms of one velocity and
+// E_rot (uniGasPatchBoundary.C:147-184, 223-254) added to a[0..9].
+__device__ inline void wall_terms(const double U[3], double e, const double n[3], double fA, double m, int rotdof,
+                                  double a[kWRows])
+{
+    const double Un = U[0] * n[0] + U[1] * n[1] + U[2] * n[2];
+    const double inv = 1.0 / fmax(fabs(Un) * fA, kVSmall);
+    const double UU = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+    a[UGX_W_RHON] += inv;
+    if (rotdof > 0) a[UGX_W_RHON_INT] += inv;
+    a[UGX_W_RHOM] += m * inv;
+    a[UGX_W_LINEAR_KE] += 0.5 * m * UU * inv;
+    a[UGX_W_MCC] += m * UU * inv;
+    a[UGX_W_MU] += m * U[0] * inv;
+    a[UGX_W_MV] += m * U[1] * inv;
+    a[UGX_W_MW] += m * U[2] * inv;
+    a[UGX_W_EROT] += e * inv;
+    a[UGX_W_ROTDOF] += (double)rotdof * inv;
+}
+
+// One thread per wall face: sort its bucket, sum each species' hits in key
+// order, and add deltaT times the sums at sample steps.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_wall_reduce(WorldView<R> w)
+{
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= (uint32_t)w.nwf) return;
+    const uint32_t b = w.woff[f], e = w.woff[f + 1];
+    if (e == b) return;
+    const CaseDev& cs = w.cases[w.cell_case[w.wcell[f]]];
+    if (((cs.step + 1u) % (uint32_t)cs.sample_interval) != 0u) return;
+    heap_sort(w.bkey + b, w.bidx + b, e - b);
+    const double* gq = &w.wgeo[(size_t)f * kWGeo];
+    const double n[3] = {gq[0], gq[1], gq[2]};
+    const double fA = gq[3];
+    const double FN = cs.fn, dt = cs.dt;
+    uint32_t i = b;
+    while (i < e) {
+        const uint32_t ls = (uint32_t)(w.bkey[i] >> 42);
+        const uint32_t g = (uint32_t)cs.sp_base + ls;
+        const double m = w.sp.mass[g];
+        const int rotdof = w.sp.rotdof[g];
+        double a[kWRows];
+        for (int r = 0; r < kWRows; ++r) a[r] = 0.0;
+        for (; i < e && (uint32_t)(w.bkey[i] >> 42) == ls; ++i) {
+            const WallEv ev = w.ev[w.bidx[i]];
+            wall_terms(ev.u0, ev.e0, n, fA, m, rotdof, a);
+            const double preIE = 0.5 * m * (ev.u0[0] * ev.u0[0] + ev.u0[1] * ev.u0[1] + ev.u0[2] * ev.u0[2]) + ev.e0;
+            wall_terms(ev.u1, ev.e1, n, fA, m, rotdof, a);
+            const double postIE = 0.5 * m * (ev.u1[0] * ev.u1[0] + ev.u1[1] * ev.u1[1] + ev.u1[2] * ev.u1[2]) + ev.e1;
+            a[UGX_W_Q] += FN * (preIE - postIE) / (dt * fA);
+            for (int d = 0; d < 3; ++d) a[UGX_W_FX + d] += FN * (m * ev.u0[d] - m * ev.u1[d]) / (dt * fA);
+        }
+        double* dst = &w.wsamp[((size_t)f * kNS + ls) * kWRows];
+        for (int r = 0; r < kWRows; ++r) dst[r] += dt * a[r];
+    }
 }
 
 // ------------------------------------------------------------ occupancy
@@ -2659,6 +2777,15 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     hipLaunchKernelGGL(k_move<R>, dim3(w.move_grid), dim3(kBlock), 0, s, w);
     mark(3);
     hipLaunchKernelGGL(k_wall<R>, dim3(grid_for(w.wq_cap, 2048)), dim3(kBlock), 0, s, w);
+    if (w.nwf > 0) {
+        const uint32_t nw1 = (uint32_t)w.nwf + 1u;
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(nw1)), dim3(kBlock), 0, s, w.wcnt, nw1);
+        hipLaunchKernelGGL(k_wall_count<R>, dim3(grid_for(w.ev_cap, 1024)), dim3(kBlock), 0, s, w);
+        scan_u32(w.wcnt, nw1, w.woff, w.blocksum, nullptr, s);
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(nw1)), dim3(kBlock), 0, s, w.wcnt, nw1);
+        hipLaunchKernelGGL(k_wall_bucket<R>, dim3(grid_for(w.ev_cap, 1024)), dim3(kBlock), 0, s, w);
+        hipLaunchKernelGGL(k_wall_reduce<R>, dim3(grid_for((uint64_t)w.nwf)), dim3(kBlock), 0, s, w);
+    }
     mark(4);
     hipLaunchKernelGGL((k_count_tiles<R, 256, kCountTiles>),
                        dim3((unsigned)std::min<uint64_t>((max_tiles(w.cap, (uint64_t)w.nc) + kCountTiles - 1) / kCountTiles, kCountGrid)),

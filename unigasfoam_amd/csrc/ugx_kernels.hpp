@@ -16,7 +16,22 @@ struct DevState {
     uint32_t nwall;         // queued diffuse-wall tracks this step
     uint32_t ntiles;        // cell tiles of this step's sort
     uint32_t nsel;          // BGK selections this step
+    uint32_t nev;           // wall-hit events this step
+    uint32_t nev_peak;      // largest nev of a finished step
+    uint32_t evover;        // a step had more wall hits than the event buffer holds
+    uint32_t pad0;
 };
+
+// One interaction of a simulator with a measured wall face: velocity and
+// E_rot before and after it, the world wall face and the global species.
+struct WallEv {
+    double u0[3], u1[3];
+    double e0, e1;
+    uint32_t wf;
+    uint32_t sp;
+};
+// Rows of the wall sums per (face, species): ugx_wall_index.
+constexpr int kWRows = UGX_W_COUNT;
 
 // BGK-family parameters of one cell for the velocity draws and the
 // conservation step (one record per cell of a BGK-active case).
@@ -107,7 +122,66 @@ struct WorldView {
     uint32_t* wq_info;
     double* wq_tau;
     uint32_t wq_cap;
+    // wall measurements (boundaryMeasurements): per cell-face the world wall
+    // face or -1; per wall face its geometry (kWGeo doubles) and cell; the
+    // step's events, their per-face buckets, and the time sums
+    int32_t nwf;
+    const int32_t* wface;     // [nc*kMaxFaces]
+    const double* wgeo;       // [nwf*kWGeo]
+    const int32_t* wcell;     // [nwf]
+    WallEv* ev;               // [ev_cap]
+    unsigned long long* ev_key;   // [ev_cap] local species << 42 | slot << 10 | crossing
+    uint32_t ev_cap;
+    uint32_t* wcnt;           // [nwf+1]
+    uint32_t* woff;           // [nwf+1]
+    unsigned long long* bkey; // [ev_cap] keys in face buckets
+    uint32_t* bidx;           // [ev_cap]
+    double* wsamp;            // [nwf][kNS][kWRows]
 };
+
+// Records wall hits of measured faces into the step's event list.  Only
+// the hits of a sampling step of the cell's case are recorded: the
+// reference's per-step boundary sums enter its time sums only at those
+// steps and are cleared after every step (uniGasVolFields.C:800-830,
+// uniGasCloud.C:865).
+struct WallSink {
+    const int32_t* wface;
+    WallEv* ev;
+    unsigned long long* key;
+    uint32_t cap;
+    DevState* st;
+    const int32_t* local;
+    const int32_t* cell_case;
+    const CaseDev* cases;
+    __device__ bool measured(uint32_t c, int f) const
+    {
+        if (wface == nullptr || wface[(size_t)c * kMaxFaces + f] < 0) return false;
+        const CaseDev& cs = cases[cell_case[c]];
+        return ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
+    }
+    __device__ void hit(uint32_t c, int f, uint32_t slot, int ord, const double* up, const double* uq, double e0,
+                        double e1, uint32_t sp) const
+    {
+        const int32_t wf = wface[(size_t)c * kMaxFaces + f];
+        const uint32_t i = atomicAdd(&st->nev, 1u);
+        if (i >= cap) { st->evover = 1u; return; }
+        key[i] = ((unsigned long long)(uint32_t)local[sp] << 42) | ((unsigned long long)slot << 10) |
+                 (unsigned long long)(uint32_t)ord;
+        WallEv e;
+        for (int d = 0; d < 3; ++d) { e.u0[d] = up[d]; e.u1[d] = uq[d]; }
+        e.e0 = e0;
+        e.e1 = e1;
+        e.wf = (uint32_t)wf;
+        e.sp = sp;
+        ev[i] = e;
+    }
+};
+
+template <class R>
+__device__ inline WallSink wall_sink(const WorldView<R>& w)
+{
+    return WallSink{w.nwf > 0 ? w.wface : nullptr, w.ev, w.ev_key, w.ev_cap, w.st, w.sp.local, w.cell_case, w.cases};
+}
 
 // Rows of the per-cell sums kept for the BGK family: every base row and the
 // per-species counts.

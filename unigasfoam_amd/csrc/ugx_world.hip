@@ -154,6 +154,32 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
         }
         if (t == UGX_PATCH_WALL && !found) return set_err(UGX_ERR_ARG, "wall patch %d has no boundary model", p);
     }
+    // measured wall faces, numbered in patch then face order
+    std::vector<int32_t> wid(nf, -1);
+    g.nwf = 0;
+    g.wgeo.clear(); g.wpatch.clear(); g.wlocal.clear(); g.wcell.clear();
+    for (int p = 0; p < m->n_patches; ++p) {
+        if (m->patch_type[p] != UGX_PATCH_WALL) continue;
+        if (g.bm[p].kind != UGX_BND_SPECULAR && g.bm[p].kind != UGX_BND_DIFFUSE) continue;
+        for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) {
+            wid[f] = g.nwf++;
+            const double* a = &A[3 * (size_t)f];
+            const double mag = std::sqrt(dot3(a, a));
+            double n[3], t1[3];
+            for (int d = 0; d < 3; ++d) n[d] = a[d] / mag;
+            // tangent: face centre minus the face's first vertex (uniGasVolFields.C:687-691)
+            const double* v0 = &m->points[3 * (size_t)m->face_points[m->face_offsets[f]]];
+            for (int d = 0; d < 3; ++d) t1[d] = pc[3 * (size_t)f + d] - v0[d];
+            const double tm = std::sqrt(dot3(t1, t1));
+            for (int d = 0; d < 3; ++d) t1[d] = t1[d] / tm;
+            const double w[kWGeo] = {n[0], n[1], n[2], mag, t1[0], t1[1], t1[2], 0.0};
+            g.wgeo.insert(g.wgeo.end(), w, w + kWGeo);
+            g.wpatch.push_back(p);
+            g.wlocal.push_back(f - m->patch_start[p]);
+            g.wcell.push_back(m->owner[f]);
+        }
+    }
+    g.wface.assign((size_t)nc * kMaxFaces, -1);
     for (int d = 0; d < 3; ++d) g.mask[d] = 1.0;
     for (int p = 0; p < m->n_patches; ++p) {
         if (m->patch_type[p] != UGX_PATCH_EMPTY || m->patch_size[p] == 0) continue;
@@ -209,6 +235,7 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
                 } else {
                     g.nbr[(size_t)c * kMaxFaces + j] = -1 - p;
                     g.fkind[k] = (t == UGX_PATCH_SYMMETRY) ? kFaceSymmetry : kFaceBoundary;
+                    g.wface[k] = wid[f];
                 }
             }
         }
@@ -324,7 +351,9 @@ void World::release()
                      (void**)&d_mom, (void**)&d_bgkp, (void**)&d_raise,
                      (void**)&d_sel_slot,
                      (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st,
-                     (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau};
+                     (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau,
+                     (void**)&d_wface, (void**)&d_wgeo, (void**)&d_wcell, (void**)&d_ev, (void**)&d_ev_key,
+                     (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -399,6 +428,26 @@ int World::alloc_particles(uint32_t newcap)
     if (int rc = dalloc(d_wq_info, wq_cap)) return rc;
     if (int rc = dalloc(d_wq_tau, wq_cap)) return rc;
     cap = newcap;
+    if (nwf > 0 && ev_cap < std::max<uint32_t>(newcap / 8, 65536u))
+        if (int rc = alloc_events(std::max<uint32_t>(newcap / 8, 65536u))) return rc;
+    return UGX_OK;
+}
+
+// Event buffers of the wall measurements for n hits per step (the event
+// list, its keys, and the per-face bucket copy).
+int World::alloc_events(uint32_t n)
+{
+    drop_graphs();
+    dfree(d_ev);
+    dfree(d_ev_key);
+    dfree(d_bkey);
+    dfree(d_bidx);
+    ev_cap = 0;
+    if (int rc = dalloc(d_ev, n)) return rc;
+    if (int rc = dalloc(d_ev_key, n)) return rc;
+    if (int rc = dalloc(d_bkey, n)) return rc;
+    if (int rc = dalloc(d_bidx, n)) return rc;
+    ev_cap = n;
     return UGX_OK;
 }
 
@@ -490,6 +539,18 @@ WorldView<R> World::view() const
     w.wq_info = d_wq_info;
     w.wq_tau = d_wq_tau;
     w.wq_cap = wq_cap;
+    w.nwf = nwf;
+    w.wface = d_wface;
+    w.wgeo = d_wgeo;
+    w.wcell = d_wcell;
+    w.ev = d_ev;
+    w.ev_key = d_ev_key;
+    w.ev_cap = ev_cap;
+    w.wcnt = d_wcnt;
+    w.woff = d_woff;
+    w.bkey = d_bkey;
+    w.bidx = d_bidx;
+    w.wsamp = d_wsamp;
     return w;
 }
 
@@ -525,6 +586,9 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     std::vector<double> mass(nsp), omega(nsp), muref(nsp), prterm(nsp), pmr(npair), pC(npair), pe(npair), pchi(npair), palpha(npair);
     std::vector<BModel> bm;
     std::vector<InflowFS> inflow;
+    std::vector<int32_t> wface_all((size_t)nc * kMaxFaces, -1), wcell_all;
+    std::vector<double> wgeo_all;
+    wf_begin.assign(1, 0);
     int cbase = 0, sbase = 0, pbase = 0;
     max_ins = 0;
     for (size_t ci = 0; ci < specs.size(); ++ci) {
@@ -603,6 +667,14 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
                     fkind[gk] = kind;
                 }
             }
+        }
+        {
+            const int32_t wb = wf_begin.back();
+            for (size_t k = 0; k < g.wface.size(); ++k)
+                if (g.wface[k] >= 0) wface_all[(size_t)cbase * kMaxFaces + k] = wb + g.wface[k];
+            for (int k = 0; k < g.nwf; ++k) wcell_all.push_back(cbase + g.wcell[k]);
+            wgeo_all.insert(wgeo_all.end(), g.wgeo.begin(), g.wgeo.end());
+            wf_begin.push_back(wb + g.nwf);
         }
         for (auto b : g.bm) bm.push_back(b);
         for (auto e : g.inflow) {
@@ -750,7 +822,17 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         if ((rc = dalloc(d_bgkp, (size_t)nc))) return rc;
         if ((rc = dalloc(d_raise, (size_t)nc))) return rc;
     }
-    size_t nbs = ((size_t)nc * kNS + 2 + 1023) / 1024 + 2;
+    nwf = wf_begin.back();
+    if (nwf > 0) {
+        if ((rc = up(d_wface, wface_all))) return rc;
+        if ((rc = up(d_wgeo, wgeo_all))) return rc;
+        if ((rc = up(d_wcell, wcell_all))) return rc;
+        if ((rc = dalloc(d_wcnt, (size_t)nwf + 1))) return rc;
+        if ((rc = dalloc(d_woff, (size_t)nwf + 1))) return rc;
+        std::vector<double> zw((size_t)nwf * kNS * kWRows, 0.0);
+        if ((rc = up(d_wsamp, zw))) return rc;
+    }
+    size_t nbs = (std::max<size_t>((size_t)nc * kNS + 2, (size_t)nwf + 1) + 1023) / 1024 + 2;
     if ((rc = dalloc(d_blocksum, nbs))) return rc;
     std::vector<unsigned long long> zcnt(specs.size() * kCntNum, 0ull);
     if ((rc = up(d_counters, zcnt))) return rc;
@@ -798,6 +880,14 @@ int World::check_state()
     HIP_TRY(hipMemcpyAsync(&st, d_st, sizeof st, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     if (st.overflow) return set_err(UGX_ERR_CAPACITY, "particle capacity (%u) exceeded on the device", cap);
+    if (st.evover)
+        return set_err(UGX_ERR_CAPACITY, "a step had more wall hits than the %u-event buffer holds", ev_cap);
+    // room for twice the busiest step's wall hits seen so far
+    const uint32_t peak = std::max(st.nev_peak, st.nev);
+    if (nwf > 0 && (uint64_t)peak * 2u > ev_cap) {
+        const uint64_t want = std::min<uint64_t>((uint64_t)peak * 4u, 0xF0000000ull);
+        if (int rc = alloc_events((uint32_t)want)) return rc;
+    }
     return UGX_OK;
 }
 
@@ -1101,6 +1191,11 @@ int World::adopt(World& old, const std::vector<int>& old_of_new)
         else
             launch_carry_particles<double>(old.view<double>().a, view<double>().a, b, n, dst, cd, sd, stream);
         dst += n;
+        const int32_t nwk = old.wf_begin[k + 1] - old.wf_begin[k];
+        if (nwk > 0)
+            HIP_TRY(hipMemcpyAsync(d_wsamp + (size_t)wf_begin[i] * kNS * kWRows,
+                                   old.d_wsamp + (size_t)old.wf_begin[k] * kNS * kWRows,
+                                   sizeof(double) * (size_t)nwk * kNS * kWRows, hipMemcpyDeviceToDevice, stream));
         nc_.step = ocd[k].step;
         nc_.kn_time = ocd[k].kn_time;
         samp_reset_steps[i] = old.samp_reset_steps[k];
@@ -1220,6 +1315,12 @@ int World::case_state(int ci, ugx_case_state* s, bool upload)
         {s->kn_sums, d_knacc + b * kKnRows, sizeof(double) * n * kKnRows},
         {s->kn, d_kn + b, sizeof(double) * n},
     };
+    const int64_t nwk = wf_begin[ci + 1] - wf_begin[ci];
+    if (s->wall_samples && s->n_wall_faces != nwk)
+        return set_err(UGX_ERR_ARG, "state holds %lld wall faces, the case has %lld", (long long)s->n_wall_faces,
+                       (long long)nwk);
+    if (s->wall_samples)
+        if (int rc = case_wall_samples(ci, s->wall_samples, nwk * kNS * kWRows, upload)) return rc;
     unsigned long long cnt[kCntNum];
     if (!upload) {
         for (const Arr& a : arrs) HIP_TRY(hipMemcpy(a.host, a.dev, a.bytes, hipMemcpyDeviceToHost));
@@ -1255,9 +1356,27 @@ int World::case_samples_reset(int ci)
     const int ncl = cases[ci].cell_end - cases[ci].cell_begin;
     std::vector<double> z((size_t)ncl * kNSamp, 0.0);
     HIP_TRY(hipMemcpy(d_samp + (size_t)cases[ci].cell_begin * kNSamp, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
+    const size_t nw = (size_t)(wf_begin[ci + 1] - wf_begin[ci]) * kNS * kWRows;
+    if (nw > 0) {
+        std::vector<double> zw(nw, 0.0);
+        HIP_TRY(hipMemcpy(d_wsamp + (size_t)wf_begin[ci] * kNS * kWRows, zw.data(), sizeof(double) * nw,
+                          hipMemcpyHostToDevice));
+    }
     unsigned long long v = 0;
     HIP_TRY(hipMemcpy(&v, d_counters + (size_t)ci * kCntNum + kCntSampled, sizeof v, hipMemcpyDeviceToHost));
     samp_reset_steps[ci] = (int64_t)v;
+    return UGX_OK;
+}
+
+int World::case_wall_samples(int ci, double* buf, int64_t len, bool upload)
+{
+    const size_t nw = (size_t)(wf_begin[ci + 1] - wf_begin[ci]) * kNS * kWRows;
+    if (len < (int64_t)nw) return set_err(UGX_ERR_ARG, "wall sum buffer holds %lld, need %zu", (long long)len, nw);
+    if (nw == 0) return UGX_OK;
+    HIP_TRY(hipStreamSynchronize(stream));
+    double* dev = d_wsamp + (size_t)wf_begin[ci] * kNS * kWRows;
+    if (upload) HIP_TRY(hipMemcpy(dev, buf, sizeof(double) * nw, hipMemcpyHostToDevice));
+    else HIP_TRY(hipMemcpy(buf, dev, sizeof(double) * nw, hipMemcpyDeviceToHost));
     return UGX_OK;
 }
 

@@ -23,6 +23,13 @@ struct MeshGeom {
     double mask[3] = {1.0, 1.0, 1.0};
     std::vector<BModel> bm;        // per patch
     std::vector<InflowFS> inflow;  // face-species entries (case-local)
+    // measured wall faces: the faces of wall patches with a specular or
+    // diffuse model (uniGasPatchBoundary measurePropertiesAtWall_), in
+    // patch then face order
+    int nwf = 0;
+    std::vector<int32_t> wface;    // per cell-face: wall face index, or -1
+    std::vector<double> wgeo;      // per wall face [kWGeo]: unit outward normal, area, tangent t1
+    std::vector<int32_t> wpatch, wlocal, wcell;   // per wall face: patch, face within the patch, cell
 };
 
 int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>& bnd, int ns,
@@ -114,6 +121,22 @@ struct World {
     uint32_t* d_wq_info = nullptr;
     double* d_wq_tau = nullptr;
     uint32_t wq_cap = 0;
+    // wall measurements: world wall faces (each case's in a contiguous
+    // range), the event buffers of a step and the per-face time sums
+    int32_t nwf = 0;
+    std::vector<int32_t> wf_begin;   // [ncases + 1]
+    int32_t* d_wface = nullptr;
+    double* d_wgeo = nullptr;
+    int32_t* d_wcell = nullptr;
+    WallEv* d_ev = nullptr;
+    unsigned long long* d_ev_key = nullptr;
+    uint32_t ev_cap = 0;
+    uint32_t* d_wcnt = nullptr;
+    uint32_t* d_woff = nullptr;
+    unsigned long long* d_bkey = nullptr;
+    uint32_t* d_bidx = nullptr;
+    double* d_wsamp = nullptr;
+    int alloc_events(uint32_t n);
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;
@@ -173,6 +196,8 @@ struct World {
     int case_counters(int ci, ugx_counters* out);
     int case_samples(int ci, double* out, int64_t len);
     int case_samples_reset(int ci);
+    // wall sums of case ci, [face][kNS][kWRows]: out to `buf`, or in from it
+    int case_wall_samples(int ci, double* buf, int64_t len, bool upload);
     // restart state of case ci: out to s, or in from s
     int case_state(int ci, ugx_case_state* s, bool upload);
     int hybrid_update();
@@ -180,6 +205,8 @@ struct World {
 
 // Write-time fields of uniGasVolFields from raw sums (ugx_fields.cpp).
 int compute_fields(const CaseSpec& spec, const double* samp, int ncl, const ugx_counters& cnt, ugx_fields* f);
+// Write-time wall values of uniGasVolFields from the raw wall sums.
+int compute_wall_fields(const CaseSpec& spec, const double* wsamp, const ugx_counters& cnt, ugx_wall_fields* f);
 
 }  // namespace ugx
 

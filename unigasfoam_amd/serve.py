@@ -50,6 +50,7 @@ class Result:
     arrival: float
     admitted: float
     done: float
+    wall_samples: np.ndarray = None
 
     @property
     def latency(self):
@@ -154,7 +155,7 @@ class Server:
                 r, done, t_adm = live.pop(slot)
                 c = self.batch.case(slot)
                 cnt = c.counters()
-                st.results[r.rid] = Result(r.rid, c.samples(), cnt, r.arrival, t_adm, now())
+                st.results[r.rid] = Result(r.rid, c.samples(), cnt, r.arrival, t_adm, now(), c.wall_samples())
                 st.particle_steps += cnt["particle_steps"]
                 self.batch.retire(slot)
             st.busy += time.perf_counter() - tb
@@ -177,7 +178,7 @@ def run_alone(requests, precision="fp32", use_graphs=True):
         c.step(r.sample_steps)
         c.sync()
         cnt = c.counters()
-        st.results[r.rid] = Result(r.rid, c.samples(), cnt, 0.0, tb - t0, time.perf_counter() - t0)
+        st.results[r.rid] = Result(r.rid, c.samples(), cnt, 0.0, tb - t0, time.perf_counter() - t0, c.wall_samples())
         st.particle_steps += cnt["particle_steps"]
         c.close()
         st.busy += time.perf_counter() - tb
