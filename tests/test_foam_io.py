@@ -147,7 +147,9 @@ def test_field_files_use_the_volfields_names(tmp_path):
               "U_mean": rng.normal(size=(n, 3)), "shear_stress": rng.normal(size=(n, 6)), "mfp": rng.random(n)}
     written = foam_io.write_fields(tmp_path, "0.001", m, fields, field_name="uniGasFields")
     assert sorted(written) == sorted(["rhoN_uniGasFields", "translationalT_uniGasFields", "UMean_uniGasFields",
-                                      "pressureTensor_uniGasFields", "variableHardSphereMeanFreePath_uniGasFields"])
+                                      "pressureTensor_uniGasFields", "variableHardSphereMeanFreePath_uniGasFields",
+                                      "surfaceHeatTransfer_uniGasFields", "surfaceShearStress_uniGasFields",
+                                      "fD_uniGasFields"])
     for key, arr in fields.items():
         stem = foam_io.FIELD_FILES[key][0]
         back = foam_io.read_internal_field(os.path.join(tmp_path, "0.001", f"{stem}_uniGasFields"))
@@ -156,3 +158,33 @@ def test_field_files_use_the_volfields_names(tmp_path):
     for p in m.patches:
         assert p.name in text
     assert "type            empty;" in text and "type            calculated;" in text
+
+
+def test_boundary_values_follow_volfields(tmp_path):
+    """Wall patches carry the wall measurements for T, p and UMean and the
+    owner cell's value for rhoN; other patches the owner cell's value; the
+    wall-only fields are zero inside (uniGasVolFields.C:1262-1392)."""
+    m = channel_mesh(4, 1.0e-3, 1.0e-4)       # cyclic x/z, walls at y
+    n = m.n_cells
+    rng = np.random.default_rng(2)
+    fields = {"rhoN": rng.random(n) * 1e20, "translational_T": 300.0 + rng.random(n), "p": rng.random(n),
+              "U_mean": rng.normal(size=(n, 3))}
+    lo, up = m.patch_index("lowerWall"), m.patch_index("upperWall")
+    wf = {"translational_T": np.array([501.0, 502.0]), "rotational_T": np.array([0.0, 0.0]),
+          "overall_T": np.array([503.0, 504.0]), "pressure": np.array([1.5, 2.5]),
+          "U_mean": np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]]), "heat_flux": np.array([-7.0, 8.0]),
+          "shear": np.array([0.25, 0.5]), "force_density": np.array([[0.1, -1.5, 0.0], [0.2, 2.5, 0.0]])}
+    wall = {"patch": np.array([lo, up]), "face": np.array([0, 0]), "fields": wf}
+    foam_io.write_fields(tmp_path, "1", m, fields, wall=wall)
+    d = os.path.join(tmp_path, "1")
+    T = lambda name, patch: foam_io.read_patch_field(os.path.join(d, f"{name}_uniGasFields"), patch)
+    assert T("translationalT", "lowerWall").tolist() == [501.0]
+    assert T("translationalT", "upperWall").tolist() == [502.0]
+    assert T("p", "upperWall").tolist() == [2.5]
+    assert T("UMean", "lowerWall").tolist() == [[1.0, 2.0, 3.0]]
+    assert T("surfaceHeatTransfer", "lowerWall").tolist() == [-7.0]
+    assert T("surfaceShearStress", "upperWall").tolist() == [0.5]
+    assert T("fD", "upperWall").tolist() == [[0.2, 2.5, 0.0]]
+    owner_lo = m.owner[m.patches[lo].start]
+    assert T("rhoN", "lowerWall").tolist() == [fields["rhoN"][owner_lo]]
+    assert np.all(foam_io.read_internal_field(os.path.join(d, "surfaceHeatTransfer_uniGasFields")) == 0.0)

@@ -53,10 +53,19 @@ def test_run_driver_writes_fields_and_matches_oracle(require_gpu, tmp_path):
     assert_identical(g, o, "couette from dictionaries")
     # the written vol fields: the sampler at the end time, under uniGasVolFields' names
     fields = g.fields()
-    foam_io.write_fields(str(case), "2e-07", spec.mesh, fields, run.field_name)
+    patch, face = g.wall_faces()
+    wf = g.wall_fields()
+    foam_io.write_fields(str(case), "2e-07", spec.mesh, fields, run.field_name,
+                         wall={"patch": patch, "face": face, "fields": wf})
     rho = foam_io.read_internal_field(str(case / "2e-07" / "rhoN_N2"))
     assert np.allclose(rho, fields["rhoN"], rtol=1e-15)
     assert abs(rho.mean() / 1.3e22 - 1.0) < 0.01
+    # the walls' measured values: 273 K walls under a gas filled at 273 K
+    for k, name in enumerate(spec.mesh.patches[i].name for i in patch):
+        q = foam_io.read_patch_field(str(case / "2e-07" / "surfaceHeatTransfer_N2"), name)
+        assert np.array_equal(q, wf["heat_flux"][k:k + 1])
+        p = foam_io.read_patch_field(str(case / "2e-07" / "p_N2"), name)
+        assert p[0] == wf["pressure"][k] and abs(p[0] / (1.3e22 * 1.380649e-23 * 273.0) - 1.0) < 0.1
     g.close()
 
 

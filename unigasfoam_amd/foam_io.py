@@ -189,20 +189,61 @@ def _values(a, cls):
     return "".join("(" + " ".join(f"{v:.17g}" for v in row) + ")\n" for row in a)
 
 
-def write_fields(case_dir, time_name, mesh: PolyMesh, fields, field_name="uniGasFields"):
+# wall-only vol fields of uniGasVolFields (zero inside, measured values on
+# wall faces): key of Case.wall_fields() -> (file stem, class, dimensions)
+WALL_FIELD_FILES = {
+    "heat_flux": ("surfaceHeatTransfer", "volScalarField", "[1 0 -3 0 0 0 0]"),
+    "shear": ("surfaceShearStress", "volScalarField", "[1 -1 -2 0 0 0 0]"),
+    "force_density": ("fD", "volVectorField", "[1 -1 -2 0 0 0 0]"),
+}
+# cell fields whose wall-face values come from the wall measurements
+# (uniGasVolFields.C:1262-1350); on other patches every field except these
+# wall-only ones takes the owner cell's value (:1354-1392)
+_WALL_MEASURED = {"translational_T": "translational_T", "rotational_T": "rotational_T", "overall_T": "overall_T",
+                  "p": "pressure", "U_mean": "U_mean"}
+_WIDTH = {"volScalarField": 1, "volVectorField": 3, "volSymmTensorField": 6}
+
+
+def _patch_values(mesh, p, key, a, wall):
+    """Boundary values of field `key` (cell values `a`) on patch p."""
+    owners = mesh.owner[p.start:p.start + p.size]
+    if key in WALL_FIELD_FILES:
+        vals = np.zeros((p.size,) + a.shape[1:])
+    elif p.type == abi.WALL and key in _WALL_MEASURED:
+        vals = np.zeros((p.size,) + a.shape[1:])
+    else:
+        return a[owners]
+    if p.type == abi.WALL and wall is not None:
+        src = wall["fields"][key if key in WALL_FIELD_FILES else _WALL_MEASURED[key]]
+        for k in np.nonzero(wall["patch"] == mesh.patch_index(p.name))[0]:
+            vals[wall["face"][k]] = src[k]
+    return vals
+
+
+def write_fields(case_dir, time_name, mesh: PolyMesh, fields, field_name="uniGasFields", wall=None):
     """Write the sampled fields (Case.fields()) to <case>/<time>/<stem>_<field_name>
-    as OpenFOAM vol fields; every patch gets a calculated (or empty) entry."""
+    as OpenFOAM vol fields, with the boundary values uniGasVolFields writes
+    (uniGasVolFields.C:1262-1392): on wall patches temperatures, pressure
+    (fD.n) and UMean from the wall measurements, other fields the owner
+    cell's value; and the wall-only fields surfaceHeatTransfer,
+    surfaceShearStress and fD.  `wall` is {"patch", "face", "fields"} from
+    Case.wall_faces() and Case.wall_fields(); without it wall-measured
+    values are written as zero."""
     d = os.path.join(case_dir, str(time_name))
     os.makedirs(d, exist_ok=True)
     written = []
-    for key, (stem, cls, dims) in FIELD_FILES.items():
-        if key not in fields:
+    files = dict(FIELD_FILES)
+    cells = dict(fields)
+    for key, spec in WALL_FIELD_FILES.items():
+        files[key] = spec
+        cells[key] = np.zeros((mesh.n_cells, _WIDTH[spec[1]]) if _WIDTH[spec[1]] > 1 else mesh.n_cells)
+    for key, (stem, cls, dims) in files.items():
+        if key not in cells:
             continue
-        a = np.asarray(fields[key], dtype=np.float64)
-        width = {"volScalarField": 1, "volVectorField": 3, "volSymmTensorField": 6}[cls]
+        a = np.asarray(cells[key], dtype=np.float64)
+        width = _WIDTH[cls]
         a = a.reshape(mesh.n_cells, width) if width > 1 else a.reshape(mesh.n_cells)
         obj = f"{stem}_{field_name}"
-        zero = "0" if width == 1 else "(" + " ".join(["0"] * width) + ")"
         kind = cls[3:].replace("Field", "").lower()     # scalar, vector, symmtensor
         kind = {"symmtensor": "symmTensor"}.get(kind, kind)
         with open(os.path.join(d, obj), "w") as f:
@@ -216,11 +257,27 @@ def write_fields(case_dir, time_name, mesh: PolyMesh, fields, field_name="uniGas
                 elif p.type == abi.CYCLIC:
                     f.write(f"    {p.name}\n    {{\n        type            cyclic;\n    }}\n")
                 else:
+                    v = _patch_values(mesh, p, key, a, wall)
                     f.write(f"    {p.name}\n    {{\n        type            calculated;\n"
-                            f"        value           uniform {zero};\n    }}\n")
+                            f"        value           nonuniform List<{kind}>\n{p.size}\n(\n{_values(v, cls)})\n;\n    }}\n")
             f.write("}\n")
         written.append(obj)
     return written
+
+
+def read_patch_field(path, patch):
+    """Values of one patch's entry in an ASCII vol field file."""
+    with open(path) as f:
+        body = _body(f.read(), path)
+    i = body.index("boundaryField")
+    m = re.search(r"\b" + re.escape(patch) + r"\s*\{[^{}]*?value\s+nonuniform\s+List<(\w+)>\s*(\d+)\s*\(", body[i:])
+    if not m:
+        raise ValueError(f"{path}: no nonuniform value for patch {patch}")
+    n = int(m.group(2))
+    start = i + m.end()
+    end = body.index(";", start)
+    a = _numbers(body[start:body.rfind(")", start, end)], np.float64)
+    return a.reshape(n, -1) if m.group(1) != "scalar" else a
 
 
 def read_internal_field(path):

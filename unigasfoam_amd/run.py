@@ -7,7 +7,8 @@ the library's evolve().
 Reads the case (unigasfoam_amd.case_dir), applies its uniGasMeshFill
 configurations, then advances deltaT steps to endTime.  At every
 writeInterval the uniGasVolFields of the sampler are written to
-CASE/<time>/ (foam_io.write_fields) and, while resetAtOutput holds, the
+CASE/<time>/ (foam_io.write_fields, with the wall measurements on wall
+patches) and, while resetAtOutput holds, the
 sampler is reset (uniGasVolFields.C:1432-1502).  Every nTerminalOutputs steps
 it prints the reference's log lines: the time, the number of particles
 (uniGasCloud.C:898) and ExecutionTime / ClockTime (uniGasFoam.C:97-101).
@@ -58,7 +59,9 @@ def run_case(path, precision="fp64", end_time=None, seed=12345, out=sys.stdout, 
                   f"{time.perf_counter() - t0_wall:.0f} s", file=out, flush=True)
         if write and w_steps and step % w_steps == 0:
             fields = case.fields()
-            foam_io.write_fields(path, time_name(t), spec.mesh, fields, run.field_name)
+            patch, face = case.wall_faces()
+            wall = {"patch": patch, "face": face, "fields": case.wall_fields()}
+            foam_io.write_fields(path, time_name(t), spec.mesh, fields, run.field_name, wall=wall)
             written.append(time_name(t))
             if run.reset_at_output and t < run.reset_at_output_until_time:
                 case.reset_samples()

@@ -12,7 +12,8 @@ Convergence: after `warmup` steps the samplers are cleared, then the case
 samples until every cell holds at least `samples_per_cell` simulator
 samples on average, i.e. ceil(samples_per_cell / simulators per cell)
 steps.  The reported latency of a request is completion time minus
-arrival time.
+arrival time; with the simulated clock (realtime=False), where requests are
+released early whenever the loop is idle, it counts from the release.
 """
 from __future__ import annotations
 
@@ -112,13 +113,17 @@ class Server:
         def now():
             return time.perf_counter() - t0
 
+        released = {}      # rid -> time the request entered the queue (its latency starts there)
         while pending or queue or live:
             t = now()
             if not self.realtime:
-                # simulated clock: arrivals are released as soon as the loop is idle
+                # simulated clock: arrivals are released as soon as the loop is
+                # idle, and their latency counts from that release
                 while pending and (not live or pending[0].arrival <= t):
+                    released[pending[0].rid] = t
                     queue.append(pending.pop(0))
             while pending and pending[0].arrival <= t:
+                released[pending[0].rid] = pending[0].arrival
                 queue.append(pending.pop(0))
             while queue:
                 r = queue[0]
@@ -155,7 +160,7 @@ class Server:
                 r, done, t_adm = live.pop(slot)
                 c = self.batch.case(slot)
                 cnt = c.counters()
-                st.results[r.rid] = Result(r.rid, c.samples(), cnt, r.arrival, t_adm, now(), c.wall_samples())
+                st.results[r.rid] = Result(r.rid, c.samples(), cnt, released[r.rid], t_adm, now(), c.wall_samples())
                 st.particle_steps += cnt["particle_steps"]
                 self.batch.retire(slot)
             st.busy += time.perf_counter() - tb
