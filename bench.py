@@ -8,7 +8,9 @@ default workload is the 3-D nozzle expansion: 200x100x100 planar-faced hexes
 (2e6 cells), 1e8 argon simulators (50 per cell), FP32 particle state with
 FP64 sampler / (sigma_T c_r)max / NTC accumulators.  value = particle-steps/s
 = sum over timed steps of the simulator count / wall time, with the inputs
-resident in HBM.  Prints one JSON line with the per-phase HIP-event timings,
+resident in HBM, timed after the flow has developed from the fill (DSMC
+cells switched on, collisions running).  Prints one JSON line with the
+per-phase HIP-event timings,
 the roofline of the dominant kernel and a CPU baseline (the FP64 OpenMP
 oracle on a bounded sample of the same workload).
 """
@@ -149,11 +151,36 @@ def precision_leg():
             "fields": r}
 
 
+def fp64_line(args, spec, fill, steps=20):
+    """The same workload and developed state with all-FP64 particle state
+    (the precision whose results equal the oracle bit for bit)."""
+    from unigasfoam_amd.core import Case
+    print("[bench] FP64 line ...", file=sys.stderr, flush=True)
+    case = Case(spec, precision="fp64")
+    case.fill(**fill)
+    case.step(args.develop + args.warmup)
+    case.sync()
+    c0 = case.counters()
+    t0 = time.perf_counter()
+    case.step(steps)
+    case.sync()
+    el = time.perf_counter() - t0
+    c1 = case.counters()
+    case.close()
+    ps = c1["particle_steps"] - c0["particle_steps"]
+    return {"value": ps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3, "steps": steps,
+            "dtype": "f64", "untimed_steps": args.develop + args.warmup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--develop", type=int, default=300,
+                    help="steps run from the fill before the warm-up, so the timed steps see a developed flow "
+                         "(DSMC cells, collisions) rather than the initial state")
+    ap.add_argument("--no-fp64-line", action="store_true", help="skip the all-FP64 timing of the same workload")
     ap.add_argument("--workload", default="nozzle")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the nozzle mesh")
@@ -184,6 +211,11 @@ def main():
     case.fill(**fill)
     case.sync()
     print(f"[bench] setup {time.perf_counter() - t:.1f} s, {case.count()} simulators", file=sys.stderr, flush=True)
+    if args.develop > 0:
+        t = time.perf_counter()
+        case.step(args.develop)
+        case.sync()
+        print(f"[bench] developed {args.develop} steps in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     case.step(args.warmup)
     case.sync()
     print("[bench] warmup done", file=sys.stderr, flush=True)
@@ -240,7 +272,9 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": desc, "form": "plain (one case)", "precision":
-                   "FP32 particle state, FP64 samplers/sigma_max/NTC" if R == 4 else "all FP64"},
+                   "FP32 particle state, FP64 samplers/sigma_max/NTC" if R == 4 else "all FP64",
+                   "state": f"developed: timed after {args.develop + args.warmup} steps from the fill",
+                   "decomposition_interval": spec.decomposition_interval},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(dom_bytes)},
@@ -256,6 +290,8 @@ def main():
         s = probe_copy(nbytes, reps=10)
         out["hbm_probe"] = {"copy_bytes": nbytes, "GBs": round(2 * nbytes / s / 1e9, 1)}
     case.close()
+    if args.precision == "fp32" and not args.no_fp64_line:
+        out["fp64_line"] = fp64_line(args, spec, fill)
     if args.sweep_cases > 0:
         out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision, args.sweep_saturated_rate)
     if not args.no_precision:

@@ -236,6 +236,34 @@ __host__ __device__ inline void gauss3(Rng& r, double& a, double& b, double& c)
     gauss2(r, c, d);
 }
 
+// Single-precision draws for the FP32-state path, where the drawn
+// velocities are stored in FP32: hardware log2 and sin/cos (argument in
+// revolutions), 24-bit uniforms.  The FP64 path uses the deterministic
+// functions above, which the oracle restates bit for bit.
+// Uniform on (0, 1] from the top 24 bits of a word, and on [0, 1) from 24 bits.
+__device__ inline float uniform24_open0(uint32_t w) { return (float)((w >> 8) + 1u) * 0x1.0p-24f; }
+__device__ inline float uniform24(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }
+
+__device__ inline void gauss2f(Rng& r, float& g0, float& g1)
+{
+    const uint64_t v = rng_u64(r);
+    const float u1 = uniform24_open0((uint32_t)(v >> 32));
+    const float u2 = uniform24((uint32_t)v);
+    // -2 ln u1 = -2 ln2 log2 u1
+    const float rad = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    g0 = rad * __builtin_amdgcn_cosf(u2);
+    g1 = rad * __builtin_amdgcn_sinf(u2);
+}
+
+__device__ inline void gauss3f(Rng& r, float& a, float& b, float& c)
+{
+    float d;
+    gauss2f(r, a, b);
+    gauss2f(r, c, d);
+}
+
+__device__ inline float uniformf(Rng& r) { return uniform24_open0((uint32_t)(rng_u64(r) >> 32)); }
+
 // ------------------------------------------------------------ world view
 
 template <class R>

@@ -1938,6 +1938,45 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
     // a single-species case needs no species lookup
     const int ls = (cs.ns == 1) ? 0 : A.local_species(i);
     const double u0 = P.s2kT / sqrt(w.sp.mass[cs.sp_base + ls]);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        // FP32 state: the draw in single precision (the stored velocity is FP32)
+        float xf[3];
+        if (P.model == UGX_BGK_USP) {
+            const float m0 = (float)P.m0;
+            const float Af = (float)(P.cQ / (P.p * u0)), Bf = (float)(P.cS / P.p), inv_m0 = 1.0f / m0;
+            const float q0 = (float)P.q[0], q1 = (float)P.q[1], q2 = (float)P.q[2];
+            const float s0 = (float)P.sm[0], s1 = (float)P.sm[1], s2 = (float)P.sm[2], s3 = (float)P.sm[3],
+                        s4 = (float)P.sm[4], s5 = (float)P.sm[5];
+            for (int it = 0; it < 100000; ++it) {
+                float g0, g1, g2;
+                gauss3f(rv, g0, g1, g2);
+                xf[0] = g0 * (float)kSqrtHalf; xf[1] = g1 * (float)kSqrtHalf; xf[2] = g2 * (float)kSqrtHalf;
+                const float vSq = xf[0] * xf[0] + xf[1] * xf[1] + xf[2] * xf[2];
+                const float vTr = vSq * (float)kThird;
+                const float prob = 1.0f + Af * (q0 * xf[0] + q1 * xf[1] + q2 * xf[2]) * (vSq * 0.4f - 1.0f)
+                                   + Bf * (s0 * (xf[0] * xf[0] - vTr) + s3 * (xf[1] * xf[1] - vTr)
+                                           + s5 * (xf[2] * xf[2] - vTr) + 2.0f * s1 * xf[0] * xf[1]
+                                           + 2.0f * s2 * xf[0] * xf[2] + 2.0f * s4 * xf[1] * xf[2]);
+                if (prob > m0 && prob < 10.0f) {
+                    atomicMax(raise, (unsigned long long)double_to_bits((double)prob));
+                    break;
+                }
+                if (uniformf(rv) < prob * inv_m0) break;
+            }
+        } else {
+            float g0, g1, g2;
+            gauss3f(rv, g0, g1, g2);
+            xf[0] = g0 * (float)kSqrtHalf; xf[1] = g1 * (float)kSqrtHalf; xf[2] = g2 * (float)kSqrtHalf;
+            if (P.model == UGX_BGK_ESBGK) {
+                float y[3];
+                for (int d = 0; d < 3; ++d)
+                    y[d] = (float)P.sm[3 * d] * xf[0] + (float)P.sm[3 * d + 1] * xf[1] + (float)P.sm[3 * d + 2] * xf[2];
+                xf[0] = y[0]; xf[1] = y[1]; xf[2] = y[2];
+            }
+        }
+        A.store(i, P.U[0] + u0 * (double)xf[0], P.U[1] + u0 * (double)xf[1], P.U[2] + u0 * (double)xf[2]);
+        return;
+    }
     double xi[3];
     if (P.model == UGX_BGK_USP) {
         // the draw's constants are divided out once (usp_draw of the oracle)
