@@ -140,15 +140,20 @@ def sweep_leg(n_cases, rate, precision, sat_rate):
 
 
 def precision_leg():
-    """FP32 state against all-FP64 on the GPU: Mach-10 N2/O2 cylinder (config C3
-    at a reduced mesh), fields sampled over the last 200 of 400 steps."""
+    """FP32 state against all-FP64 on the GPU as paired ensembles of
+    independent replicas (precision.compare_replicas): 1-D Couette flows of
+    N2 with DSMC (LB) and with USP, 48 replicas of 1e5 simulators each."""
     from unigasfoam_amd import workloads
-    from unigasfoam_amd.precision import compare
-    spec, fill = workloads.cylinder(n_theta=100, n_r=50, target=2.0e6, dt=1.0e-7)
-    r = compare(spec, fill, warmup=200, sample=400, blocks=8)
-    return {"workload": "Mach-10 79/21 N2/O2 over a cylinder, 100x50x1 cells, ~2e6 simulators, 200 steps, then "
-                        "8 sample blocks of 50 steps (standard errors from the block means)",
-            "fields": r}
+    from unigasfoam_amd.precision import compare_replicas
+    out = {}
+    for name, kw in (("couette_dsmc_lb", {}),
+                     ("couette_usp", dict(collision_model="bgk", bgk_model="unifiedStochasticParticleSBGK"))):
+        spec, fill = workloads.couette(n_cells=100, per_cell=1000, U_wall=250.0, **kw)
+        spec.sampler_full = True
+        out[name] = compare_replicas(spec, fill, warmup=1000, sample=2000, replicas=48)
+    return {"workload": "1-D N2 Couette, walls +/-250 m/s, 100 cells x 1000 simulators, 48 paired replicas per "
+                        "precision, 1000 steps then 2000 sampled steps; standard errors from the replicas",
+            "fields": out}
 
 
 def fp64_line(args, spec, fill, steps=20):
