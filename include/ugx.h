@@ -142,6 +142,15 @@ typedef struct {
     int32_t reset_at_decomposition;   /* hybrid: clear the Kn time sums at each
                                          decomposition (default on) */
     double  reset_until_time;         /* ... while time < this (<= 0: always) */
+    /* cell weighting (cellWeightedSimulation; uniGasCloud.C:1353-1425 and
+       uniGasMeshFill.C:111-168): the fill sets a weight W_c per cell from its
+       number density, a simulator of cell c stands for F_N W_c molecules, and
+       a simulator entering a cell of another weight is cloned or deleted */
+    int32_t cell_weighted;            /* 0 off */
+    int32_t particles_per_subcell;    /* cellWeightedProperties.particlesPerSubCell (> 0 when weighted) */
+    int32_t min_particles_per_subcell;/* ... minParticlesPerSubCell (0: particles_per_subcell) */
+    int32_t max_smoothing_passes;     /* weight smoothing passes at most (0: 500) */
+    double  max_cell_weight_ratio;    /* smoothing stops once neighbour ratios are below 1 + this (0: 0.05) */
 } ugx_model_desc;
 
 typedef enum {
@@ -286,6 +295,8 @@ typedef struct {
     double  time_sampled;      /* sum of deltaT over sampled steps */
     int64_t n_dsmc_cells;      /* hybrid: cells using DSMC now */
     int64_t particle_steps;    /* sum over steps of the simulator count after the step */
+    int64_t cloned;            /* cell weighting: clones made, cumulative */
+    int64_t weight_deleted;    /* cell weighting: simulators removed, cumulative */
 } ugx_counters;
 
 typedef struct ugx_case ugx_case;
@@ -331,6 +342,12 @@ int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
 /* Kn_GLL per cell of a hybrid case after the last decomposition: the
  * KnGLL volScalarField of localKnudsen (localKnudsen.C:395-430). */
 int ugx_kn_download(ugx_case* c, double* out, int64_t len);
+/* Cell weights W_c (1 in a case without cell weighting): set by the fill of
+ * a cell-weighted case (uniGasMeshFill.C:111-168) or uploaded (a restart
+ * reads them back, as the reference reads its cellWeightFactor field,
+ * uniGasCloud.C:612-623); an upload also rescales the inflow counts. */
+int ugx_cell_weights_download(ugx_case* c, double* out, int64_t len);
+int ugx_cell_weights_upload(ugx_case* c, const double* in, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
 /* Wall faces of the case: their count, and per face its patch and its face
  * index within the patch. */
@@ -370,6 +387,7 @@ typedef struct {
     int64_t  n_wall_faces;       /* in: the case's wall face count (ugx_wall_face_count) */
     double*  wall_samples;       /* [n_wall_faces * UGX_MAX_SPECIES * UGX_W_COUNT] wall sums;
                                     NULL on upload: the case's wall sums are left as they are */
+    double*  cell_weight;        /* [n_cells] cell weights; NULL: not carried */
 } ugx_case_state;
 
 /* Copy the state out (any case) or in (a plain case; every pointer non-NULL). */

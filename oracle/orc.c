@@ -64,7 +64,8 @@ typedef struct {
 
 typedef struct {
     int cell, species, desc;
-    double nbar;       /* expected simulators per step (Bird 4.22) */
+    double nbar;       /* expected simulators per step (Bird 4.22) at cell weight 1 */
+    double nbar_num, nbar_den;   /* its numerator and F_N part of the denominator */
     int nv;
     double vloc[8][3]; /* face vertices in the cell frame */
     double cum[6];     /* cumulative triangle area fractions (fan from vertex 0) */
@@ -135,9 +136,16 @@ struct orc_world {
     double* wsamp;    /* [nwf*NS*WROWS] wall time sums */
     wall_ev* ev;      /* this step's interactions */
     int64_t nev, evcap;
+    int32_t* wcell;   /* [nwf] cell of each measured wall face */
+    /* cell weighting */
+    double* cw;       /* [nc] cell weights (1 without weighting) */
+    int32_t* cstart;  /* [cap] cell each simulator started the step in */
     /* counters */
     ugx_counters cnt;
 };
+
+static void smooth_field(const orc_world* w, const double* f, double* out, int ncomp);
+static int fill_weights(orc_world* w, double ntot);
 
 /* ------------------------------------------------------------- geometry */
 
@@ -255,6 +263,7 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
         if (m->patch_type[p] == UGX_PATCH_WALL && (w->bm[p].kind == UGX_BND_SPECULAR || w->bm[p].kind == UGX_BND_DIFFUSE))
             for (int f = m->patch_start[p]; f < m->patch_start[p] + m->patch_size[p]; ++f) wid[f] = w->nwf++;
     w->wgeo = calloc((size_t)(w->nwf > 0 ? w->nwf : 1) * 4, sizeof(double));
+    w->wcell = calloc((size_t)(w->nwf > 0 ? w->nwf : 1), sizeof(int32_t));
     w->wsamp = calloc((size_t)(w->nwf > 0 ? w->nwf : 1) * NS * WROWS, sizeof(double));
     w->wface = malloc(sizeof(int32_t) * (size_t)nc * MAXF);
     for (int k = 0; k < nc * MAXF; ++k) w->wface[k] = -1;
@@ -265,6 +274,7 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
         double* gq = &w->wgeo[4 * (size_t)wid[f]];
         for (int d = 0; d < 3; ++d) gq[d] = a[d] / mag;
         gq[3] = mag;
+        w->wcell[wid[f]] = m->owner[f];
     }
     /* solution directions from empty patches */
     for (int d = 0; d < 3; ++d) w->mask[d] = 1.0;
@@ -353,9 +363,10 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
                 double mps = sqrt(2.0 * KB * e->T / w->mass[s]);
                 double nrm[3] = {-sF[0] / fA, -sF[1] / fA, -sF[2] / fA};
                 double sCos = dot3(e->U, nrm) / mps;
-                e->nbar = (fA * nd * w->model.delta_t * mps
-                           * (exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + erf(sCos))))
-                          / (2.0 * sqrtPi * w->model.n_equivalent_particles);
+                e->nbar_num = fA * nd * w->model.delta_t * mps
+                              * (exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + erf(sCos)));
+                e->nbar_den = 2.0 * sqrtPi * w->model.n_equivalent_particles;
+                e->nbar = e->nbar_num / e->nbar_den;
                 int fb = m->face_offsets[f], fe = m->face_offsets[f + 1];
                 e->nv = fe - fb;
                 if (e->nv > 8) return fail(UGX_ERR_MESH, "inflow face with more than 8 vertices");
@@ -515,7 +526,7 @@ static void wall_reduce(orc_world* w)
 {
     if (w->nev == 0) return;
     qsort(w->ev, (size_t)w->nev, sizeof(wall_ev), cmp_ev);
-    const double FN = w->model.n_equivalent_particles, dt = w->model.delta_t;
+    const double dt = w->model.delta_t;
     int64_t i = 0;
     while (i < w->nev) {
         const int wf = w->ev[i].wf;
@@ -525,6 +536,8 @@ static void wall_reduce(orc_world* w)
         const double fA = gq[3];
         const double m = w->mass[sp];
         const int rotdof = w->rotdof[sp];
+        /* nParticle F_N W_c of the face's cell (uniGasPatchBoundary.C:292-294) */
+        const double FN = w->model.n_equivalent_particles * w->cw[w->wcell[wf]];
         double a[WROWS];
         for (int r = 0; r < WROWS; ++r) a[r] = 0.0;
         for (; i < w->nev && w->ev[i].wf == wf && (int)(w->ev[i].key >> 42) == sp; ++i) {
@@ -816,7 +829,7 @@ static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, 
     orc_rng_init(&r, w->seed, ORC_RNG_NTC, 0u, (uint32_t)c, w->step);
     double smax0 = w->smax[c];
     double smax = smax0;
-    double sel = 0.5 * (double)nC * (double)(nC - 1) * w->model.n_equivalent_particles * smax0
+    double sel = 0.5 * (double)nC * (double)(nC - 1) * w->model.n_equivalent_particles * w->cw[c] * smax0
                  * w->model.delta_t / w->vol[c];
     double fl = floor(sel);
     int64_t ncand = (int64_t)fl;
@@ -898,7 +911,7 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
                      uint32_t* idx, int64_t* nrel_out, int update_state)
 {
     int model = w->model.bgk_model;
-    double FN = w->model.n_equivalent_particles, dt = w->model.delta_t, V = w->vol[c];
+    double FN = w->model.n_equivalent_particles * w->cw[c], dt = w->model.delta_t, V = w->vol[c];
     double N = S[UGX_S_N];
     int usp = (model == UGX_BGK_USP);
     int perform = 1;
@@ -1107,6 +1120,9 @@ int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n
     w->uspmax = xcalloc((size_t)nc, sizeof(double));
     w->knacc = xcalloc((size_t)nc * KNR, sizeof(double));
     w->kn = xcalloc((size_t)nc, sizeof(double));
+    w->cw = xcalloc((size_t)nc, sizeof(double));
+    for (int c = 0; c < nc; ++c) w->cw[c] = 1.0;
+    w->cstart = xcalloc((size_t)w->cap, sizeof(int32_t));
     int cm = (model->collision_model == UGX_COLL_DSMC) ? 1 : 0;  /* hybrid starts all-BGK (uniGasCloud.C:726-732) */
     for (int c = 0; c < nc; ++c) { w->cmodel[c] = cm; w->uspmax[c] = 1.0; }
     *out = w;
@@ -1119,7 +1135,7 @@ void orc_destroy(orc_world* w)
     void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->farea, w->fkind, w->bm, w->inflow,
                     w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
                     w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn,
-                    w->wface, w->wgeo, w->wsamp, w->ev};
+                    w->wface, w->wgeo, w->wsamp, w->ev, w->wcell, w->cw, w->cstart};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(w);
 }
@@ -1180,6 +1196,11 @@ int orc_fill(orc_world* w, const ugx_fill_desc* fl)
     double ntot = 0.0;
     int best = 0;
     for (int s = 0; s < ns; ++s) { ntot += fl->number_density[s]; if (fl->number_density[s] > fl->number_density[best]) best = s; }
+    if (w->model.cell_weighted) {
+        if (fl->exact_per_cell > 0) return fail(UGX_ERR_ARG, "a cell-weighted case is filled by number density, not per cell");
+        int rc = fill_weights(w, ntot);
+        if (rc) return rc;
+    }
     int64_t* cnt = malloc(sizeof(int64_t) * (size_t)nc * ns);
     int64_t total = w->n;
     for (int c = 0; c < nc; ++c)
@@ -1190,7 +1211,7 @@ int orc_fill(orc_world* w, const ugx_fill_desc* fl)
             } else {
                 orc_rng r;
                 orc_rng_init(&r, w->seed, ORC_RNG_FILL, (uint32_t)s, (uint32_t)c, 0u);
-                double req = fl->number_density[s] / FN * w->vol[c];
+                double req = fl->number_density[s] / FN * w->vol[c] / w->cw[c];
                 double f0 = floor(req);
                 k = (int64_t)f0;
                 if (orc_uniform(&r) < req - f0) k += 1;
@@ -1287,6 +1308,118 @@ int orc_download(orc_world* w, ugx_particle_soa* p)
  * DSMC where Kn > breakdownMax. */
 static void hybrid_switch(orc_world* w);
 
+/* -------------------------------------------------------- cell weighting */
+
+/* Cell weights of a weighted fill (uniGasMeshFill.C:111-168), one subcell
+ * per cell and no radial weight: W_c = n V_c / (N_pps F_N), then passes of
+ * fvc::average(fvc::interpolate(W)) (zeroGradient boundaries) each followed
+ * by W_c = max(min(n V_c / (N_min F_N), W_c), SMALL), until no internal face
+ * has max(W_o/W_n, W_n/W_o) > 1 + maxCellWeightRatio or maxSmoothingPasses
+ * passes were made. */
+static int fill_weights(orc_world* w, double ntot)
+{
+    const ugx_model_desc* m = &w->model;
+    double FN = m->n_equivalent_particles;
+    int npps = m->particles_per_subcell;
+    int nmin = m->min_particles_per_subcell > 0 ? m->min_particles_per_subcell : npps;
+    int max_passes = m->max_smoothing_passes > 0 ? m->max_smoothing_passes : 500;
+    double max_ratio = m->max_cell_weight_ratio > 0.0 ? m->max_cell_weight_ratio : 0.05;
+    if (npps <= 0) return fail(UGX_ERR_ARG, "a cell-weighted case needs particles_per_subcell > 0");
+    if (!(ntot > 0.0)) return fail(UGX_ERR_ARG, "a cell-weighted fill needs a number density > 0");
+    int nc = w->nc;
+    double* tmp = malloc(sizeof(double) * (size_t)nc);
+    for (int c = 0; c < nc; ++c) w->cw[c] = (ntot * w->vol[c]) / ((double)npps * 1.0 * FN * 1.0);
+    int passes = 0;
+    double max_r;
+    do {
+        passes++;
+        smooth_field(w, w->cw, tmp, 1);
+        memcpy(w->cw, tmp, sizeof(double) * (size_t)nc);
+        for (int c = 0; c < nc; ++c) {
+            double lim = (ntot * w->vol[c]) / ((double)nmin * 1.0 * FN * 1.0);
+            double v = w->cw[c] < lim ? w->cw[c] : lim;
+            w->cw[c] = v > SMALL ? v : SMALL;
+        }
+        max_r = 1.0e-300;
+        for (int c = 0; c < nc; ++c)
+            for (int j = 0; j < MAXF; ++j) {
+                int k = c * MAXF + j;
+                if (w->fkind[k] != FACE_INTERNAL) continue;
+                int o = w->nbr[k];
+                double a = w->cw[c] / w->cw[o], b = w->cw[o] / w->cw[c];
+                double r = a > b ? a : b;
+                if (r > max_r) max_r = r;
+            }
+    } while (max_r > 1.0 + max_ratio && passes < max_passes);
+    free(tmp);
+    return UGX_OK;
+}
+
+/* uniGasCloud::cellWeighting (uniGasCloud.C:1353-1425) over the slots
+ * [0, nall) after the move: a simulator whose weight fell from W_o (its
+ * start cell's; a new one's insertion cell's) to W_n gets a clone per whole
+ * unit of W_o/W_n - 1 and one more with the probability of the rest; one
+ * whose weight rose is removed with probability 1 - W_o/W_n.  Draws from a
+ * stream per simulator (slot n0 + i of the inflow keyed as 2^31 | i); the
+ * clones are copies appended in the order of their parents. */
+static int cell_weighting(orc_world* w, int64_t n0, int64_t nall, int64_t* nclone)
+{
+    int32_t* kc = malloc(sizeof(int32_t) * (size_t)(nall > 0 ? nall : 1));
+    int64_t tot = 0, wdel = 0;
+    for (int64_t i = 0; i < nall; ++i) {
+        int32_t k = 0;
+        int c = w->cell[i];
+        if (c != CELL_DEAD) {
+            double wo = w->cw[w->cstart[i]], wn = w->cw[c];
+            if (wo != wn) {
+                uint32_t id = (i < n0) ? (uint32_t)i : (0x80000000u | (uint32_t)(i - n0));
+                orc_rng r;
+                orc_rng_init(&r, w->seed, ORC_RNG_WEIGHT, 0u, id, w->step);
+                if (wo > wn) {
+                    double prob = wo / wn - 1.0;
+                    while (prob > 1.0) { k += 1; prob = prob - 1.0; }
+                    if (prob > orc_uniform(&r)) k += 1;
+                } else if (wo / wn < orc_uniform(&r)) {
+                    w->cell[i] = CELL_DEAD;
+                    wdel += 1;
+                }
+            }
+        }
+        kc[i] = k;
+        tot += k;
+    }
+    if (nall + tot > w->cap) { free(kc); return fail(UGX_ERR_CAPACITY, "particle capacity exceeded by clones"); }
+    int64_t j = nall;
+    for (int64_t i = 0; i < nall; ++i)
+        for (int32_t q = 0; q < kc[i]; ++q, ++j) {
+            for (int d = 0; d < 3; ++d) { w->x[3 * j + d] = w->x[3 * i + d]; w->v[3 * j + d] = w->v[3 * i + d]; }
+            w->erot[j] = w->erot[i];
+            w->cell[j] = w->cell[i];
+            w->sp[j] = w->sp[i];
+        }
+    free(kc);
+    w->cnt.cloned += tot;
+    w->cnt.weight_deleted += wdel;
+    *nclone = tot;
+    return UGX_OK;
+}
+
+int orc_cell_weights(orc_world* w, double* out, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(out, w->cw, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_set_cell_weights(orc_world* w, const double* in, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    for (int c = 0; c < w->nc; ++c)
+        if (!(in[c] > 0.0)) return fail(UGX_ERR_ARG, "cell weight not > 0");
+    memcpy(w->cw, in, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
 /* ------------------------------------------------------------------ step */
 
 int orc_step_phases(orc_world* w, int mask)
@@ -1299,7 +1432,8 @@ int orc_step_phases(orc_world* w, int mask)
         for (int k = 0; k < w->nin; ++k) {
             orc_rng r;
             orc_rng_init(&r, w->seed, ORC_RNG_INFLOW_N, 0u, (uint32_t)k, w->step);
-            double nb = w->inflow[k].nbar;
+            /* divided by F_N W_c of the face's cell (uniGasGeneralBoundary.C:154-165) */
+            double nb = w->inflow[k].nbar_num / (w->inflow[k].nbar_den * w->cw[w->inflow[k].cell]);
             double fl = floor(nb);
             int64_t c = (int64_t)fl;
             if (c < 0) c = 0;
@@ -1317,6 +1451,7 @@ int orc_step_phases(orc_world* w, int mask)
                 orc_rng r;
                 orc_rng_init(&r, w->seed, ORC_RNG_INFLOW_P, (uint32_t)i, (uint32_t)k, w->step);
                 w->sp[j] = e->species;
+                w->cstart[j] = e->cell;   /* a new simulator carries its insertion cell's weight */
                 if (!make_inflow(w, e, &r, &w->x[3 * j], &w->v[3 * j], &w->erot[j], &w->cell[j], j, &hits, &errs)) dead++;
             }
             base += cnt[k];
@@ -1327,6 +1462,7 @@ int orc_step_phases(orc_world* w, int mask)
         for (int64_t i = 0; i < n0; ++i) {
             orc_rng r;
             orc_rng_init(&r, w->seed, ORC_RNG_MOVE, 0u, (uint32_t)i, w->step);
+            w->cstart[i] = w->cell[i];
             if (!track(w, &w->cell[i], &w->x[3 * i], &w->v[3 * i], &w->erot[i], w->sp[i], i,
                        w->model.delta_t, &r, &hits, &errs)) dead++;
         }
@@ -1335,6 +1471,12 @@ int orc_step_phases(orc_world* w, int mask)
         w->cnt.deleted += dead;
         w->cnt.wall_hits += hits;
         w->cnt.track_errors += errs;
+        if (w->model.cell_weighted) {
+            int64_t ncl = 0;
+            int rc = cell_weighting(w, n0, n0 + nins, &ncl);
+            if (rc) return rc;
+            nins += ncl;
+        }
         w->n = n0 + nins;
     }
     if (mask & 2) sort_by_cell(w, w->n);
@@ -1492,7 +1634,7 @@ static void refine_models(orc_world* w)
 static void hybrid_switch(orc_world* w)
 {
     int nc = w->nc;
-    double FN = w->model.n_equivalent_particles, tav = w->kn_time;
+    double FN0 = w->model.n_equivalent_particles, tav = w->kn_time;
     double bmax = w->model.breakdown_max, th = w->model.kn_theta;
     int passes = w->model.smoothing_passes;
     double* rhoN = calloc((size_t)nc, sizeof(double));
@@ -1505,6 +1647,7 @@ static void hybrid_switch(orc_world* w)
     for (int c = 0; c < nc; ++c) {
         const double* a = &w->knacc[KNR * c];
         double V = w->vol[c];
+        double FN = FN0 * w->cw[c];
         if (a[0] > VSMALL) {
             rhoN[c] = FN * a[0] / (tav * V);
             rhoM[c] = FN * a[1] / (tav * V);
@@ -1542,6 +1685,7 @@ static void hybrid_switch(orc_world* w)
         const double* a = &w->knacc[KNR * c];
         if (a[0] > VSMALL && T[c] > VSMALL) {
             double V = w->vol[c];
+            double FN = FN0 * w->cw[c];
             double smfp[NS];
             for (int i = 0; i < w->ns; ++i) {
                 smfp[i] = 0.0;

@@ -81,6 +81,9 @@ int orc_usp_state(orc_world* w, double* q, double* sh, double* maxp);
 /* Wall measurements: the count of measured wall faces and their time sums,
  * layout [face][UGX_MAX_SPECIES][UGX_W_COUNT] as ugx_wall_samples_download. */
 int64_t orc_wall_face_count(orc_world* w);
+/* Cell weights [n_cells] (1 unless a cell-weighted fill or a set changed them). */
+int orc_cell_weights(orc_world* w, double* out, int64_t len);
+int orc_set_cell_weights(orc_world* w, const double* in, int64_t len);
 int orc_wall_samples(orc_world* w, double* out, int64_t len);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 

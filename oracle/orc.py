@@ -92,6 +92,8 @@ def _bind(L):
     L.orc_wall_face_count.argtypes = [vp]
     L.orc_wall_face_count.restype = C.c_int64
     L.orc_wall_samples.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_cell_weights.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_set_cell_weights.argtypes = [vp, abi.DBL_P, C.c_int64]
 
 
 def _check(rc):
@@ -165,6 +167,15 @@ class OracleCase:
         soa, a = alloc_soa(max(n, 1))
         _check(self.L.orc_download(self.h, C.byref(soa)))
         return {k: v[: soa.n] for k, v in a.items()}
+
+    def cell_weights(self):
+        out = np.zeros(self.nc)
+        _check(self.L.orc_cell_weights(self.h, abi.dptr(out), self.nc))
+        return out
+
+    def set_cell_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        _check(self.L.orc_set_cell_weights(self.h, abi.dptr(w), self.nc))
 
     def wall_face_count(self):
         return int(self.L.orc_wall_face_count(self.h))

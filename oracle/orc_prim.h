@@ -51,7 +51,8 @@ enum {
     ORC_RNG_INFLOW_N = 4,    /* id = inflow face-species, per step      */
     ORC_RNG_INFLOW_P = 5,    /* id = inflow face-species, sub = index   */
     ORC_RNG_FILL = 6,        /* id = cell, sub = species                */
-    ORC_RNG_BGK_V = 7        /* id = cell, sub = selection index        */
+    ORC_RNG_BGK_V = 7,       /* id = cell, sub = selection index        */
+    ORC_RNG_WEIGHT = 8       /* id = slot (inflow: 2^31 | index), per step */
 };
 
 typedef struct {

@@ -141,7 +141,7 @@ def test_unsupported_switches_are_named(tmp_path):
     shutil.copytree(os.path.join(CASES, "heatBath"), case)
     p = case / "constant" / "uniGasProperties"
     text = p.read_text()
-    text = text.replace("cellWeightedSimulation          false;", "cellWeightedSimulation          true;")
+    text = text.replace("axisymmetricSimulation          false;", "axisymmetricSimulation          true;")
     text = text.replace("adaptiveSimulation              false;", "adaptiveSimulation              on;")
     text = text.replace("dsmcCollisionPartnerModel       noTimeCounter;",
                         "dsmcCollisionPartnerModel       noTimeCounterSubCycled;")
@@ -150,9 +150,21 @@ def test_unsupported_switches_are_named(tmp_path):
         case_dir.read_case(str(case))
     assert e.value.code == abi.ERR_UNSUPPORTED == 7
     words = " ".join(w for _, w in e.value.items)
-    assert "cellWeightedSimulation" in words and "adaptiveSimulation" in words
+    assert "axisymmetricSimulation" in words and "adaptiveSimulation" in words
     assert "noTimeCounterSubCycled" in words
     assert len(e.value.items) == 3
+
+
+def test_cell_weighting_switch_is_read(tmp_path):
+    case = tmp_path / "c"
+    shutil.copytree(os.path.join(CASES, "heatBath"), case)
+    p = case / "constant" / "uniGasProperties"
+    text = p.read_text().replace("cellWeightedSimulation          false;",
+                                 "cellWeightedSimulation          true;\ncellWeightedProperties\n{\n"
+                                 "    minParticlesPerSubCell 10;\n    particlesPerSubCell 20;\n}")
+    p.write_text(text)
+    spec, fills, run = case_dir.read_case(str(case))
+    assert spec.cell_weighted and spec.particles_per_subcell == 20 and spec.min_particles_per_subcell == 10
 
 
 def test_missing_files_are_io_errors(tmp_path):
@@ -163,8 +175,7 @@ def test_missing_files_are_io_errors(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir(REF_TUT), reason="reference tutorials not present")
 def test_reference_tutorials_report_exactly_their_unsupported_switches():
-    common = {"uniGasProperties": ["collisionProperties.macroInterpolation", "cellWeightedSimulation",
-                                   "adaptiveSimulation"]}
+    common = {"uniGasProperties": ["collisionProperties.macroInterpolation", "adaptiveSimulation"]}
     expect = {
         "expansionInVacuum": {"uniGasProperties": common["uniGasProperties"] + ["axisymmetricSimulation"],
                               "fieldPropertiesDict": ["fieldModel uniGasMassFluxSurface"],

@@ -144,10 +144,12 @@ def test_field_files_use_the_volfields_names(tmp_path):
     rng = np.random.default_rng(1)
     n = m.n_cells
     fields = {"rhoN": rng.random(n) * 1e20, "translational_T": rng.random(n) * 300.0,
-              "U_mean": rng.normal(size=(n, 3)), "shear_stress": rng.normal(size=(n, 6)), "mfp": rng.random(n)}
+              "U_mean": rng.normal(size=(n, 3)), "shear_stress": rng.normal(size=(n, 6)), "mfp": rng.random(n),
+              "heat_flux": rng.normal(size=(n, 3))}
     written = foam_io.write_fields(tmp_path, "0.001", m, fields, field_name="uniGasFields")
     assert sorted(written) == sorted(["rhoN_uniGasFields", "translationalT_uniGasFields", "UMean_uniGasFields",
                                       "pressureTensor_uniGasFields", "variableHardSphereMeanFreePath_uniGasFields",
+                                      "heatFlux_uniGasFields",
                                       "surfaceHeatTransfer_uniGasFields", "surfaceShearStress_uniGasFields",
                                       "fD_uniGasFields"])
     for key, arr in fields.items():

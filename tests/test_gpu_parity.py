@@ -35,7 +35,7 @@ def assert_identical(g, o, what=""):
             raise AssertionError(f"{what}: {k} differs at {len(bad)} entries; first slot {i}: gpu={a[i]} oracle={b[i]}")
     cg, co = g.counters(), o.counters()
     for k in ("n_particles", "candidates", "collisions", "bgk_collisions", "inserted", "deleted", "wall_hits",
-              "track_errors", "n_sampled_steps", "particle_steps"):
+              "track_errors", "n_sampled_steps", "particle_steps", "cloned", "weight_deleted"):
         assert cg[k] == co[k], f"{what}: counter {k}: gpu {cg[k]} oracle {co[k]}"
     assert np.array_equal(g.sigma_max(), o.sigma_max()), f"{what}: sigma_max differs"
     sg, so = g.samples(), o.samples()

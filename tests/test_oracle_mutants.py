@@ -18,6 +18,7 @@ import pytest
 import test_oracle_physics as phys
 import test_oracle_pins as pins
 import test_oracle_wall as wall
+import test_oracle_weighting as weighting
 from oracle import orc
 
 ORC_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
@@ -163,6 +164,36 @@ MUTANTS = {
         "const int meas = w->nwf > 0 && sampling_step(w);",
         "const int meas = w->nwf > 0;",
         [wall.test_only_sampling_steps_are_summed]),
+    # clone probability W_o/W_n instead of W_o/W_n - 1 (uniGasCloud.C:1367)
+    "weight_clone_probability": (
+        "double prob = wo / wn - 1.0;",
+        "double prob = wo / wn;",
+        [weighting.test_clones_and_removals_keep_the_physical_density]),
+    # removal test inverted (uniGasCloud.C:1409-1418)
+    "weight_removal_inverted": (
+        "} else if (wo / wn < orc_uniform(&r)) {",
+        "} else if (wn / wo < orc_uniform(&r)) {",
+        [weighting.test_clones_and_removals_keep_the_physical_density]),
+    # NTC selection number without the cell weight (noTimeCounter.C:184)
+    "weight_ntc": (
+        "w->model.n_equivalent_particles * w->cw[c] * smax0",
+        "w->model.n_equivalent_particles * smax0",
+        [weighting.test_collision_rate_per_volume_independent_of_weight]),
+    # inflow count without the cell weight (uniGasGeneralBoundary.C:165)
+    "weight_inflow": (
+        "(w->inflow[k].nbar_den * w->cw[w->inflow[k].cell])",
+        "(w->inflow[k].nbar_den)",
+        [weighting.test_inflow_count_divided_by_the_cell_weight]),
+    # fill count without the cell weight (uniGasMeshFill.C:203-205)
+    "weight_fill": (
+        "w->vol[c] / w->cw[c];",
+        "w->vol[c];",
+        [weighting.test_fill_weights_uniform_mesh_exact, weighting.test_fill_weights_graded_mesh_smoothed_and_capped]),
+    # smoothing capped at N_pps instead of minParticlesPerSubCell (uniGasMeshFill.C:141-142)
+    "weight_cap": (
+        "double lim = (ntot * w->vol[c]) / ((double)nmin * 1.0 * FN * 1.0);",
+        "double lim = (ntot * w->vol[c]) / ((double)npps * 1.0 * FN * 1.0);",
+        [weighting.test_fill_weights_graded_mesh_smoothed_and_capped]),
 }
 
 

@@ -65,6 +65,9 @@ class ModelDesc(C.Structure):
         ("breakdown_max", C.c_double), ("kn_theta", C.c_double),
         ("sampler_full", C.c_int32), ("reset_at_decomposition", C.c_int32),
         ("reset_until_time", C.c_double),
+        ("cell_weighted", C.c_int32), ("particles_per_subcell", C.c_int32),
+        ("min_particles_per_subcell", C.c_int32), ("max_smoothing_passes", C.c_int32),
+        ("max_cell_weight_ratio", C.c_double),
     ]
 
 
@@ -125,7 +128,7 @@ class CaseState(C.Structure):
         ("sampled_at_reset", C.c_int64), ("counters", C.c_uint64 * N_COUNTERS),
         ("sigma_max", DBL_P), ("cell_model", I32_P), ("samples", DBL_P), ("usp_heat_flux", DBL_P),
         ("usp_stress", DBL_P), ("usp_max_prob", DBL_P), ("kn_sums", DBL_P), ("kn", DBL_P),
-        ("n_wall_faces", C.c_int64), ("wall_samples", DBL_P),
+        ("n_wall_faces", C.c_int64), ("wall_samples", DBL_P), ("cell_weight", DBL_P),
     ]
 
 
@@ -154,6 +157,7 @@ class Counters(C.Structure):
         ("inserted", C.c_int64), ("deleted", C.c_int64), ("wall_hits", C.c_int64),
         ("track_errors", C.c_int64), ("n_sampled_steps", C.c_int64),
         ("time_sampled", C.c_double), ("n_dsmc_cells", C.c_int64), ("particle_steps", C.c_int64),
+        ("cloned", C.c_int64), ("weight_deleted", C.c_int64),
     ]
 
     def as_dict(self):

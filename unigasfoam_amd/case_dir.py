@@ -86,7 +86,6 @@ PATCH_MODELS = {"uniGasSpecularWallPatch": abi.BND_SPECULAR, "uniGasDiffuseWallP
 GENERAL_MODELS = {"uniGasFreeStreamInflowPatch": abi.BND_FREESTREAM}
 COLLISION_WORDS = ("dsmc", "bgk", "hybrid")
 _UNSUPPORTED_SWITCHES = (
-    ("cellWeightedSimulation", "cell weighting (uniGasCloud.C:1353-1425)"),
     ("adaptiveSimulation", "dynamic adaptation: subcells, time step, cell weights (uniGasDynamicAdapter.C)"),
     ("axisymmetricSimulation", "axisymmetric radial weighting"),
     ("chemicalReactions", "chemistry"),
@@ -162,6 +161,12 @@ def _model(props, spec_kw, bad):
         spec_kw["bgk_model"] = "noBGKCollision"
     if spec_kw["dsmc_model"] != "noDSMCCollision" or spec_kw["bgk_model"] != "noBGKCollision":
         spec_kw["Tref"] = cp.scalar("Tref")
+    # cell weighting (uniGasCloud.C:555-560): the weights come from the fill
+    if "cellWeightedSimulation" in props and as_switch(props["cellWeightedSimulation"], "cellWeightedSimulation"):
+        cw = props.sub("cellWeightedProperties")
+        spec_kw["cell_weighted"] = True
+        spec_kw["particles_per_subcell"] = cw.label("particlesPerSubCell")
+        spec_kw["min_particles_per_subcell"] = cw.label("minParticlesPerSubCell", spec_kw["particles_per_subcell"])
     for key, what in _UNSUPPORTED_SWITCHES:
         if key in props and as_switch(props[key], key):
             bad.append(("uniGasProperties", f"{key} ({what})"))

@@ -26,6 +26,7 @@ EXPORTS = [
     "ugx_sigma_max_download", "ugx_sigma_max_upload", "ugx_cell_model_download", "ugx_kn_download",
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry", "ugx_state_download", "ugx_state_upload",
     "ugx_wall_face_count", "ugx_wall_faces", "ugx_wall_samples_download", "ugx_wall_fields_compute",
+    "ugx_cell_weights_download", "ugx_cell_weights_upload",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_tile_config", "ugx_probe_copy",
     "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
     "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
@@ -68,6 +69,8 @@ def lib():
         L.ugx_state_download.argtypes = [vp, C.POINTER(abi.CaseState)]
         L.ugx_state_upload.argtypes = [vp, C.POINTER(abi.CaseState)]
         L.ugx_kn_download.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_cell_weights_download.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_cell_weights_upload.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_wall_face_count.argtypes = [vp, C.POINTER(C.c_int64)]
         L.ugx_wall_faces.argtypes = [vp, abi.I32_P, abi.I32_P, C.c_int64]
         L.ugx_wall_samples_download.argtypes = [vp, abi.DBL_P, C.c_int64]
@@ -253,6 +256,16 @@ class Case:
         arrs["shear_stress"] = arrs["shear_stress"].reshape(n, 6)
         return arrs
 
+    def cell_weights(self):
+        """Cell weights W_c (a simulator of cell c stands for F_N W_c molecules)."""
+        out = np.zeros(self.nc)
+        check(self.L.ugx_cell_weights_download(self.h, abi.dptr(out), self.nc), self.L)
+        return out
+
+    def set_cell_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        check(self.L.ugx_cell_weights_upload(self.h, abi.dptr(w), self.nc), self.L)
+
     def wall_face_count(self):
         n = C.c_int64()
         check(self.L.ugx_wall_face_count(self.h, C.byref(n)), self.L)
@@ -311,6 +324,8 @@ class Case:
         if w is not None:
             s.n_wall_faces = w.shape[0]
             s.wall_samples = abi.dptr(w)
+        if arrs.get("cell_weight") is not None:
+            s.cell_weight = abi.dptr(arrs["cell_weight"])
         return s
 
     def state(self):
@@ -319,6 +334,7 @@ class Case:
         the wall sums."""
         arrs = {name: np.zeros((self.nc, rows), dtype=dt) for name, rows, dt in abi.STATE_ARRAYS}
         arrs["wall_samples"] = np.zeros((self.wall_face_count(), abi.MAX_SPECIES, abi.W_COUNT))
+        arrs["cell_weight"] = np.zeros(self.nc)
         s = self._state_struct(arrs)
         check(self.L.ugx_state_download(self.h, C.byref(s)), self.L)
         arrs.update(steps=np.int64(s.steps), kn_time=np.float64(s.kn_time),
@@ -330,6 +346,8 @@ class Case:
         """Upload a state from state() (ugx_state_upload)."""
         arrs = {name: np.ascontiguousarray(np.asarray(st[name], dtype=dt).reshape(self.nc, rows))
                 for name, rows, dt in abi.STATE_ARRAYS}
+        if st.get("cell_weight") is not None:
+            arrs["cell_weight"] = np.ascontiguousarray(np.asarray(st["cell_weight"], dtype=np.float64).reshape(self.nc))
         if st.get("wall_samples") is not None:
             arrs["wall_samples"] = np.ascontiguousarray(np.asarray(st["wall_samples"], dtype=np.float64).reshape(
                 -1, abi.MAX_SPECIES, abi.W_COUNT))

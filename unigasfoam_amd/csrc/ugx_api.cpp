@@ -29,6 +29,8 @@ static int validate_model(const ugx_model_desc* m)
     if ((m->dsmc_model != UGX_DSMC_NONE) && !(m->Tref > 0.0)) return set_err(UGX_ERR_ARG, "collisionProperties.Tref must be > 0");
     if ((m->dsmc_model == UGX_DSMC_LB_VHS || m->dsmc_model == UGX_DSMC_LB_VSS) && !(m->Zrot > 0.0))
         return set_err(UGX_ERR_ARG, "rotationalRelaxationCollisionNumber must be > 0");
+    if (m->cell_weighted && m->particles_per_subcell <= 0)
+        return set_err(UGX_ERR_ARG, "cellWeightedProperties.particlesPerSubCell must be > 0");
     return UGX_OK;
 }
 
@@ -215,7 +217,7 @@ int ugx_fields_compute(ugx_case* c, ugx_fields* f)
     if (int rc = c->world->case_samples(c->index, s.data(), (int64_t)s.size())) return rc;
     ugx_counters k;
     if (int rc = c->world->case_counters(c->index, &k)) return rc;
-    return compute_fields(spec, s.data(), ncl, k, f);
+    return compute_fields(spec, s.data(), ncl, k, c->world->cw_host.data() + c->world->cases[c->index].cell_begin, f);
     UGX_GUARD_END
 }
 
@@ -321,6 +323,30 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len)
     UGX_GUARD_END
 }
 
+int ugx_cell_weights_download(ugx_case* c, double* out, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (!out) return set_err(UGX_ERR_ARG, "null output");
+    int b, n;
+    if (int rc = cell_range(c, len, 1, b, n)) return rc;
+    std::memcpy(out, c->world->cw_host.data() + b, sizeof(double) * n);
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
+int ugx_cell_weights_upload(ugx_case* c, const double* in, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (c->batch) return set_err(UGX_ERR_STATE, "the weights of a batch case are carried by the batch");
+    if (!in) return set_err(UGX_ERR_ARG, "null input");
+    int b, n;
+    if (int rc = cell_range(c, len, 1, b, n)) return rc;
+    return c->world->set_cell_weights(c->index, in);
+    UGX_GUARD_END
+}
+
 int ugx_wall_face_count(ugx_case* c, int64_t* n)
 {
     UGX_GUARD_BEGIN
@@ -364,7 +390,7 @@ int ugx_wall_fields_compute(ugx_case* c, ugx_wall_fields* f)
     if (int rc = c->world->case_wall_samples(c->index, s.data(), (int64_t)s.size(), false)) return rc;
     ugx_counters k;
     if (int rc = c->world->case_counters(c->index, &k)) return rc;
-    return compute_wall_fields(spec, s.data(), k, f);
+    return compute_wall_fields(spec, s.data(), k, c->world->cw_host.data() + c->world->cases[c->index].cell_begin, f);
     UGX_GUARD_END
 }
 

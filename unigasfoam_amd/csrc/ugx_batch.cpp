@@ -169,6 +169,8 @@ int ugx_case_footprint(const ugx_mesh_desc* mesh, const ugx_model_desc* model, i
     const bool hybrid = model->collision_model == UGX_COLL_HYBRID;
     const bool bgk = (model->collision_model == UGX_COLL_BGK || hybrid) && model->bgk_model != UGX_BGK_NONE;
     *bytes = footprint(mesh->n_cells, precision, particles, bgk, hybrid);
+    // cell weighting: start cell, clone count and offset per simulator slot, weight per cell
+    if (model->cell_weighted) *bytes += (int64_t)(12.0 * 1.5 * (double)particles) + 8 * (int64_t)mesh->n_cells;
     return UGX_OK;
 }
 

@@ -38,7 +38,8 @@ constexpr int kNSamp = UGX_S_COUNT;
 constexpr int kWGeo = 8;
 
 enum RngKind : uint32_t {
-    kRngMove = 1, kRngNtc = 2, kRngBgk = 3, kRngInflowN = 4, kRngInflowP = 5, kRngFill = 6
+    kRngMove = 1, kRngNtc = 2, kRngBgk = 3, kRngInflowN = 4, kRngInflowP = 5, kRngFill = 6,
+    kRngWeight = 8
 };
 
 // ------------------------------------------------------------ Philox RNG
@@ -290,7 +291,7 @@ struct CaseDev {
     int32_t sampler_full;        // sample rows 8-19 as well
     int32_t sum_level;           // per-cell sums needed: 0 rows 0-7, 1 +8-13, 2 +14-19
     int32_t reset_at_decomposition;
-    int32_t pad1;
+    int32_t weighted;            // cell-weighted case (clones and deletions on weight changes)
     double reset_until_time;     // <= 0: no limit
 };
 
@@ -318,7 +319,9 @@ struct InflowFS {
     int32_t local_id;    // face-species index within the case
     int32_t nv;
     int32_t pad;
-    double nbar;
+    double nbar;         // expected inserts per step at cell weight 1: nbar_num / nbar_den
+    double nbar_num;     // (uniGasGeneralBoundary.C:154-165) numerator and F_N part of the
+    double nbar_den;     // denominator; the count is nbar_num / (nbar_den W_c)
     double vloc[8][3];
     double cum[6];
     double n[3], t1[3], t2[3];

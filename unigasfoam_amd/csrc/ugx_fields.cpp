@@ -17,15 +17,18 @@ namespace ugx {
 
 static constexpr double kNA = 6.02214076e23;
 
-int compute_fields(const CaseSpec& spec, const double* A, int ncl, const ugx_counters& cnt, ugx_fields* f)
+int compute_fields(const CaseSpec& spec, const double* A, int ncl, const ugx_counters& cnt, const double* cw,
+                   ugx_fields* f)
 {
     const double tAv = cnt.time_sampled;
     const double nAv = (double)cnt.n_sampled_steps;
-    const double FN = spec.model.n_equivalent_particles;
+    const double FN0 = spec.model.n_equivalent_particles;
     const double Tref = spec.model.Tref;
     const int ns = spec.ns;
     auto S = [&](int k, int c) { return A[(size_t)k * ncl + c]; };
     for (int c = 0; c < ncl; ++c) {
+        // every simulator of cell c stands for F_N W_c molecules when sampled
+        const double FN = FN0 * (cw ? cw[c] : 1.0);
         const double V = spec.geom.vol[c];
         const double aN = S(UGX_S_N, c);
         double rhoN = 0.0, rhoM = 0.0, U[3] = {0.0, 0.0, 0.0}, T = 0.0, p = 0.0, uMean = 0.001;
@@ -127,12 +130,14 @@ int compute_fields(const CaseSpec& spec, const double* A, int ncl, const ugx_cou
 
 // Wall values (uniGasVolFields.C:1262-1350) per measured wall face, the
 // species sums added together; faces never hit give zeros, as there.
-int compute_wall_fields(const CaseSpec& spec, const double* W, const ugx_counters& cnt, ugx_wall_fields* f)
+int compute_wall_fields(const CaseSpec& spec, const double* W, const ugx_counters& cnt, const double* cw,
+                        ugx_wall_fields* f)
 {
     const MeshGeom& g = spec.geom;
     const double tAv = cnt.time_sampled;
-    const double FN = spec.model.n_equivalent_particles;
     for (int k = 0; k < g.nwf; ++k) {
+        // nParticle F_N W_c of the face's cell (uniGasVolFields.C:1276-1278)
+        const double FN = spec.model.n_equivalent_particles * (cw ? cw[g.wcell[k]] : 1.0);
         double a[kWRows] = {};
         for (int s = 0; s < spec.ns; ++s)
             for (int r = 0; r < kWRows; ++r) a[r] += W[((size_t)k * kNS + s) * kWRows + r];

@@ -85,30 +85,39 @@ __device__ inline void case_add(unsigned long long* counters, int ccase, int idx
 // one per wave.  Slots: 0 deleted, 1 wall hits, 2 track errors, 3 inserted.
 // T is the per-thread count type; 32-bit counts go to the global counters
 // before one of them could pass 2^31.
-template <class T>
+// Slot maps: M = 0 the tracking counters above, M = 1 the cell weighting's
+// (0 clones, 1 removals).
+template <class T, int M = 0>
 struct TCount {
     int ccase;
     T v[4];
 };
 
-constexpr int kTCountIdx[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
+template <int M>
+__device__ inline int tcount_idx(int k)
+{
+    constexpr int kMap0[4] = {kCntDeleted, kCntWallHits, kCntTrackErrors, kCntInserted};
+    constexpr int kMap1[4] = {kCntCloned, kCntWeightDeleted, kCntCloned, kCntCloned};
+    return M == 0 ? kMap0[k] : kMap1[k];
+}
 
+template <int M>
 __device__ inline void tcount_flush_one(int ccase, const unsigned long long (&v)[4], unsigned long long* counters)
 {
     if (ccase < 0) return;
     for (int k = 0; k < 4; ++k)
-        if (v[k]) atomicAdd(&counters[(size_t)ccase * kCntNum + kTCountIdx[k]], v[k]);
+        if (v[k]) atomicAdd(&counters[(size_t)ccase * kCntNum + tcount_idx<M>(k)], v[k]);
 }
 
-template <class T>
-__device__ inline void tcount_add(TCount<T>& t, unsigned long long* counters, int ccase, int k, uint32_t v)
+template <class T, int M>
+__device__ inline void tcount_add(TCount<T, M>& t, unsigned long long* counters, int ccase, int k, uint32_t v)
 {
     if (!v) return;
     bool flush = t.ccase != ccase;
     if constexpr (sizeof(T) == sizeof(uint32_t)) flush = flush || t.v[k] >= 0x80000000u - v;
     if (flush) {
         const unsigned long long o[4] = {t.v[0], t.v[1], t.v[2], t.v[3]};
-        tcount_flush_one(t.ccase, o, counters);
+        tcount_flush_one<M>(t.ccase, o, counters);
         t.ccase = ccase;
         t.v[0] = t.v[1] = t.v[2] = t.v[3] = T(0);
     }
@@ -116,8 +125,8 @@ __device__ inline void tcount_add(TCount<T>& t, unsigned long long* counters, in
 }
 
 // Called by every thread of the block.
-template <class T>
-__device__ inline void tcount_block_flush(const TCount<T>& t, unsigned long long* counters)
+template <class T, int M>
+__device__ inline void tcount_block_flush(const TCount<T, M>& t, unsigned long long* counters)
 {
     __shared__ int s_case[kBlock];
     __shared__ unsigned long long s_v[kBlock][4];
@@ -135,10 +144,10 @@ __device__ inline void tcount_block_flush(const TCount<T>& t, unsigned long long
                 for (int k = 0; k < 4; ++k) sum[k] += s_v[i][k];
             } else {
                 const unsigned long long o[4] = {s_v[i][0], s_v[i][1], s_v[i][2], s_v[i][3]};
-                tcount_flush_one(s_case[i], o, counters);
+                tcount_flush_one<M>(s_case[i], o, counters);
             }
         }
-        tcount_flush_one(sc, sum, counters);
+        tcount_flush_one<M>(sc, sum, counters);
     }
 }
 
@@ -178,7 +187,8 @@ __global__ void k_inflow_count(WorldView<R> w)
             if (cs.active) {
                 Rng r;
                 rng_init(r, cs.seed_lo, cs.seed_hi, kRngInflowN, 0u, (uint32_t)e.local_id, cs.step);
-                double nb = e.nbar;
+                // divided by F_N W_c of the face's cell (W_c = 1 unweighted: the same as nbar)
+                double nb = e.nbar_num / (e.nbar_den * w.cw[e.cell]);
                 double fl = floor(nb);
                 long long c = (long long)fl;
                 if (c < 0) c = 0;
@@ -292,6 +302,8 @@ __global__ void __launch_bounds__(kBlock) k_inflow_gen(WorldView<R> w)
             w.a.erot[j] = (R)erot;
             w.a.cell[j] = cell;
             w.a.sp[j] = (uint32_t)e.species;
+            // a new simulator carries the weight of its insertion cell (uniGasGeneralBoundary.C:738-748)
+            if (w.has_weighting) w.cstart[j] = (uint32_t)e.cell;
         }
         if (active) {
             tcount_add(tc, w.counters, ccase, 3, 1u);
@@ -451,6 +463,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                     t = sl;
                     ccase = cc;
                     c = g.cell;
+                    if (w.has_weighting) w.cstart[sl] = g.cell;
                     x[0] = g.x; x[1] = g.y; x[2] = g.z;
                     U[0] = g.u; U[1] = g.v; U[2] = g.w;
                     tau = cs.dt;
@@ -613,6 +626,98 @@ __global__ void __launch_bounds__(kBlock) k_wall(WorldView<R> w)
     tcount_block_flush(tc, w.counters);
 }
 
+// ---------------------------------------------------------- cell weighting
+// uniGasCloud::cellWeighting (uniGasCloud.C:1353-1425) after the move: a
+// simulator whose cell weight went down from W_o to W_n (W_o > W_n) gets
+// floor-like clones for W_o/W_n - 1 (one per whole unit, one more with the
+// probability of the rest), one whose weight went up is removed with
+// probability 1 - W_o/W_n.  W_o is the weight of the cell the simulator
+// started the step in (a new inflow simulator: its insertion cell).  Each
+// simulator draws from its own stream: existing ones keyed by their slot in
+// the case, new ones by their index in the case's inflow of the step.  The
+// clones are copies appended after the step's inflow in the order of their
+// parents' slots.
+template <class R>
+__global__ void k_weight(WorldView<R> w)
+{
+    TCount<uint32_t, 1> tc{-1, {0u, 0u, 0u, 0u}};
+    const uint32_t n0 = w.st->n;
+    const uint32_t n_all = n0 + w.st->n_ins;
+    // every entry up to cap is written (zero past the simulators): the scan
+    // runs over a host-known length
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t <= w.cap; t += gridDim.x * blockDim.x) {
+        uint32_t k = 0u;
+        bool del = false;
+        int ccase = 0;
+        if (t < n_all) {
+            const uint32_t c = w.a.cell[t];
+            if (c != kDead) {
+                ccase = w.cell_case[c];
+                const CaseDev& cs = w.cases[ccase];
+                if (cs.active && cs.weighted) {
+                    const double wo = w.cw[w.cstart[t]], wn = w.cw[c];
+                    if (wo != wn) {
+                        const uint32_t id = (t < n0) ? t - cs.particle_begin
+                                                     : (0x80000000u | (t - n0 - w.ins_off[cs.inflow_begin]));
+                        Rng r;
+                        rng_init(r, cs.seed_lo, cs.seed_hi, kRngWeight, 0u, id, cs.step);
+                        if (wo > wn) {
+                            double prob = wo / wn - 1.0;
+                            while (prob > 1.0) { k += 1u; prob = prob - 1.0; }
+                            if (prob > uniform(r)) k += 1u;
+                        } else if (wo / wn < uniform(r)) {
+                            del = true;
+                            w.a.cell[t] = kDead;
+                        }
+                    }
+                }
+            }
+            tcount_add(tc, w.counters, ccase, 0, k);
+            tcount_add(tc, w.counters, ccase, 1, del ? 1u : 0u);
+        }
+        w.wclone[t] = k;
+    }
+    tcount_block_flush(tc, w.counters);
+}
+
+// The clones: copies of slot t at n_all + wcoff[t] + i.  Nothing is written
+// when they would exceed the capacity (the overflow is reported).
+template <class R>
+__global__ void k_clone(WorldView<R> w)
+{
+    const uint32_t n_all = w.st->n + w.st->n_ins;
+    const uint32_t total = w.st->nclone;
+    if ((unsigned long long)n_all + total > (unsigned long long)w.cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.st->overflow = 1u;
+        return;
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_all; t += gridDim.x * blockDim.x) {
+        const uint32_t k = w.wclone[t];
+        if (!k) continue;
+        const uint32_t d0 = n_all + w.wcoff[t];
+        const R x = w.a.x[t], y = w.a.y[t], z = w.a.z[t], vx = w.a.vx[t], vy = w.a.vy[t], vz = w.a.vz[t];
+        const R er = w.a.erot[t];
+        const uint32_t c = w.a.cell[t], g = w.a.sp[t];
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint32_t j = d0 + i;
+            w.a.x[j] = x; w.a.y[j] = y; w.a.z[j] = z;
+            w.a.vx[j] = vx; w.a.vy[j] = vy; w.a.vz[j] = vz;
+            w.a.erot[j] = er;
+            w.a.cell[j] = c;
+            w.a.sp[j] = g;
+        }
+    }
+}
+
+template <class R>
+__global__ void k_clone_finish(WorldView<R> w)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t n_all = w.st->n + w.st->n_ins;
+        if ((unsigned long long)n_all + w.st->nclone <= (unsigned long long)w.cap) w.st->n_ins += w.st->nclone;
+    }
+}
+
 // ------------------------------------------------------ wall measurements
 // The step's wall hits (events) go to per-face buckets; each face sorts its
 // bucket by (species, slot, crossing) and sums it in that order, so the sums
@@ -696,7 +801,8 @@ __global__ void __launch_bounds__(kBlock) k_wall_reduce(WorldView<R> w)
     const double* gq = &w.wgeo[(size_t)f * kWGeo];
     const double n[3] = {gq[0], gq[1], gq[2]};
     const double fA = gq[3];
-    const double FN = cs.fn, dt = cs.dt;
+    // nParticle F_N W_c of the face's cell (uniGasPatchBoundary.C:292-294)
+    const double FN = cs.fn * w.cw[w.wcell[f]], dt = cs.dt;
     uint32_t i = b;
     while (i < e) {
         const uint32_t ls = (uint32_t)(w.bkey[i] >> 42);
@@ -1626,7 +1732,7 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
         oMr = w.pr.mr[cs.pair_base]; oChi = w.pr.chi[cs.pair_base]; oAlpha = w.pr.alpha[cs.pair_base];
         oMass = w.sp.mass[cs.sp_base]; oRd = w.sp.rotdof[cs.sp_base];
     }
-    const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * smax0 * cs.dt / w.vol[c];
+    const double sel = 0.5 * (double)nC * (double)(nC - 1) * cs.fn * w.cw[c] * smax0 * cs.dt / w.vol[c];
     const double fl = floor(sel);
     long long ncand = (long long)fl;
     if (uniform(r) < sel - fl) ncand += 1;
@@ -1694,7 +1800,7 @@ __device__ inline void bgk_params(const WorldView<R>& w, const CaseDev& cs, uint
                                   const double* S, BgkP& P, uint32_t& nrel_out)
 {
     const int model = cs.bgk_model;
-    const double FN = cs.fn, dt = cs.dt, V = w.vol[c];
+    const double FN = cs.fn * w.cw[c], dt = cs.dt, V = w.vol[c];
     const double N = S[UGX_S_N];
     const bool usp = model == UGX_BGK_USP;
     bool perform = true;
@@ -2188,7 +2294,7 @@ __global__ void k_kn_fields(WorldView<R> w)
         const CaseDev& cs = w.cases[w.cell_case[c]];
         if (!kn_due(cs)) continue;
         const double* a = &w.knacc[(size_t)kKnRows * c];
-        const double FN = cs.fn, tav = cs.kn_time, V = w.vol[c];
+        const double FN = cs.fn * w.cw[c], tav = cs.kn_time, V = w.vol[c];
         double f[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (a[0] > kVSmall) {
             f[0] = FN * a[0] / (tav * V);
@@ -2357,7 +2463,7 @@ __global__ void __launch_bounds__(kBlock) k_kn_compute(WorldView<R> w, const dou
         const double bmax = cs.breakdown_max;
         double inst;
         if (a[0] > kVSmall && fc[3] > kVSmall) {
-            const double V = w.vol[c], FN = cs.fn, T = fc[3];
+            const double V = w.vol[c], FN = cs.fn * w.cw[c], T = fc[3];
             double mfp = 0.0;
             for (int i = 0; i < cs.ns; ++i) {
                 const int gi = cs.sp_base + i;
@@ -2592,7 +2698,7 @@ __global__ void k_fill_count(WorldView<R> w, int ccase, const double* nd, int ex
         } else {
             Rng r;
             rng_init(r, cs.seed_lo, cs.seed_hi, kRngFill, s, cl, 0u);
-            const double req = nd[s] / cs.fn * w.vol[c];
+            const double req = nd[s] / cs.fn * w.vol[c] / w.cw[c];
             const double f0 = floor(req);
             k = (long long)f0;
             if (uniform(r) < req - f0) k += 1;
@@ -2825,11 +2931,20 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
         hipLaunchKernelGGL(k_wall_bucket<R>, dim3(grid_for(w.ev_cap, 1024)), dim3(kBlock), 0, s, w);
         hipLaunchKernelGGL(k_wall_reduce<R>, dim3(grid_for((uint64_t)w.nwf)), dim3(kBlock), 0, s, w);
     }
+    if (w.has_weighting) {
+        const unsigned gw = count_grid(w.cap);
+        hipLaunchKernelGGL(k_weight<R>, dim3(gw), dim3(kBlock), 0, s, w);
+        scan_u32(w.wclone, w.cap + 1u, w.wcoff, w.wbsum, &w.st->nclone, s);
+        hipLaunchKernelGGL(k_clone<R>, dim3(gw), dim3(kBlock), 0, s, w);
+        hipLaunchKernelGGL(k_clone_finish<R>, dim3(1), dim3(64), 0, s, w);
+    }
     mark(4);
     hipLaunchKernelGGL((k_count_tiles<R, 256, kCountTiles>),
                        dim3((unsigned)std::min<uint64_t>((max_tiles(w.cap, (uint64_t)w.nc) + kCountTiles - 1) / kCountTiles, kCountGrid)),
                        dim3(256), 0, s, w);
-    if (w.n_inflow > 0)
+    if (w.has_weighting)
+        hipLaunchKernelGGL(k_count<R>, dim3(count_grid(w.cap)), dim3(kBlock), 0, s, w, 0);
+    else if (w.n_inflow > 0)
         hipLaunchKernelGGL(k_count<R>, dim3(grid_for(((uint64_t)w.max_ins + kCountUnroll - 1) / kCountUnroll)), dim3(kBlock), 0, s, w, 0);
     hipLaunchKernelGGL(k_gather<R>, dim3(grid_for(w.nc)), dim3(kBlock), 0, s, w);
     scan_u32(w.cnt, ncp, w.off, w.blocksum, &w.st->n_new, s);

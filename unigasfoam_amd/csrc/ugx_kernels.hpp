@@ -19,7 +19,7 @@ struct DevState {
     uint32_t nev;           // wall-hit events this step
     uint32_t nev_peak;      // largest nev of a finished step
     uint32_t evover;        // a step had more wall hits than the event buffer holds
-    uint32_t pad0;
+    uint32_t nclone;        // clones of this step's cell weighting
 };
 
 // One interaction of a simulator with a measured wall face: velocity and
@@ -49,7 +49,7 @@ struct BgkP {
 // Per-case cumulative counters (uint64).
 enum CounterIdx {
     kCntCandidates = 0, kCntCollisions, kCntBgk, kCntInserted, kCntDeleted,
-    kCntWallHits, kCntTrackErrors, kCntSampled, kCntParticleSteps, kCntNum
+    kCntWallHits, kCntTrackErrors, kCntSampled, kCntParticleSteps, kCntCloned, kCntWeightDeleted, kCntNum
 };
 
 template <class R>
@@ -137,6 +137,15 @@ struct WorldView {
     unsigned long long* bkey; // [ev_cap] keys in face buckets
     uint32_t* bidx;           // [ev_cap]
     double* wsamp;            // [nwf][kNS][kWRows]
+    // cell weighting: per cell its weight W_c (1 in unweighted cases); with
+    // weighting on, per slot the cell a simulator started the step in, its
+    // clone count and their offsets
+    const double* cw;         // [nc]
+    int32_t has_weighting;
+    uint32_t* cstart;         // [cap]
+    uint32_t* wclone;         // [cap + 1]
+    uint32_t* wcoff;          // [cap + 1]
+    uint32_t* wbsum;          // scan scratch for cap entries
 };
 
 // Records wall hits of measured faces into the step's event list.  Only

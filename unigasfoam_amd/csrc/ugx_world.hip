@@ -264,9 +264,10 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
                 double mps = std::sqrt(2.0 * kB * e.T / mass[s]);
                 double nrm[3] = {-sF[0] / fA, -sF[1] / fA, -sF[2] / fA};
                 double sCos = dot3(e.U, nrm) / mps;
-                e.nbar = (fA * nd * model.delta_t * mps
-                          * (std::exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + std::erf(sCos))))
-                         / (2.0 * sqrtPi * model.n_equivalent_particles);
+                e.nbar_num = fA * nd * model.delta_t * mps
+                             * (std::exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + std::erf(sCos)));
+                e.nbar_den = 2.0 * sqrtPi * model.n_equivalent_particles;
+                e.nbar = e.nbar_num / e.nbar_den;
                 int fb = m->face_offsets[f], fe = m->face_offsets[f + 1];
                 e.nv = fe - fb;
                 if (e.nv > 8) return set_err(UGX_ERR_MESH, "inflow face with more than 8 vertices");
@@ -353,7 +354,8 @@ void World::release()
                      (void**)&d_blocksum, (void**)&d_counters, (void**)&d_st,
                      (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau,
                      (void**)&d_wface, (void**)&d_wgeo, (void**)&d_wcell, (void**)&d_ev, (void**)&d_ev_key,
-                     (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp};
+                     (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp,
+                     (void**)&d_cw, (void**)&d_cstart, (void**)&d_wclone, (void**)&d_wcoff, (void**)&d_wbsum};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -427,6 +429,16 @@ int World::alloc_particles(uint32_t newcap)
     if (int rc = dalloc(d_wq_slot, wq_cap)) return rc;
     if (int rc = dalloc(d_wq_info, wq_cap)) return rc;
     if (int rc = dalloc(d_wq_tau, wq_cap)) return rc;
+    if (has_weighting) {
+        dfree(d_cstart);
+        dfree(d_wclone);
+        dfree(d_wcoff);
+        dfree(d_wbsum);
+        if (int rc = dalloc(d_cstart, newcap)) return rc;
+        if (int rc = dalloc(d_wclone, (size_t)newcap + 1)) return rc;
+        if (int rc = dalloc(d_wcoff, (size_t)newcap + 1)) return rc;
+        if (int rc = dalloc(d_wbsum, ((size_t)newcap + 1 + 1023) / 1024 + 2)) return rc;
+    }
     cap = newcap;
     if (nwf > 0 && ev_cap < std::max<uint32_t>(newcap / 8, 65536u))
         if (int rc = alloc_events(std::max<uint32_t>(newcap / 8, 65536u))) return rc;
@@ -551,6 +563,12 @@ WorldView<R> World::view() const
     w.bkey = d_bkey;
     w.bidx = d_bidx;
     w.wsamp = d_wsamp;
+    w.cw = d_cw;
+    w.has_weighting = has_weighting;
+    w.cstart = d_cstart;
+    w.wclone = d_wclone;
+    w.wcoff = d_wcoff;
+    w.wbsum = d_wbsum;
     return w;
 }
 
@@ -622,6 +640,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         d.kn_theta = s.model.kn_theta;
         d.sampler_full = s.model.sampler_full ? 1 : 0;
         d.reset_at_decomposition = s.model.reset_at_decomposition;
+        d.weighted = s.model.cell_weighted ? 1 : 0;
+        if (d.weighted) has_weighting = 1;
         d.reset_until_time = s.model.reset_until_time;
         {
             int lvl = d.sampler_full ? 2 : 0;
@@ -822,6 +842,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         if ((rc = dalloc(d_bgkp, (size_t)nc))) return rc;
         if ((rc = dalloc(d_raise, (size_t)nc))) return rc;
     }
+    cw_host.assign((size_t)nc, 1.0);
+    if ((rc = up(d_cw, cw_host))) return rc;
     nwf = wf_begin.back();
     if (nwf > 0) {
         if ((rc = up(d_wface, wface_all))) return rc;
@@ -936,12 +958,16 @@ int World::step(int n)
         // bound plus one step's inflow fits, no synchronisation is needed.
         // When it may not, the count is read and the buffers are regrown
         // with room for 64 more steps of inflow (graphs are captured again).
-        if (!n_bound_valid || (max_ins > 0 && n_bound + max_ins > cap)) {
+        // With cell weighting a step may also add clones: the bound assumes
+        // at most n/16 + 1024 per step (more is reported as an overflow).
+        const uint64_t step_add = (uint64_t)max_ins + (has_weighting ? n_bound / 16 + 1024 : 0);
+        if (!n_bound_valid || (step_add > 0 && n_bound + step_add > cap)) {
             if (int rc = check_state()) return rc;
             uint32_t nn = 0;
             if (int rc = count(nn)) return rc;
-            const uint64_t need = (uint64_t)nn + 64ull * max_ins;
-            if (max_ins > 0 && need > cap)
+            const uint64_t add = (uint64_t)max_ins + (has_weighting ? (uint64_t)nn / 16 + 1024 : 0);
+            const uint64_t need = (uint64_t)nn + 64ull * max_ins + (has_weighting ? 4 * add : 0);
+            if (add > 0 && need > cap)
                 if (int rc = ensure_capacity(need + need / 4)) return rc;
         }
         hipEvent_t* marks = nullptr;
@@ -1015,7 +1041,7 @@ int World::step(int n)
         }
         cur = 1 - cur;
         for (auto& c : cases) c.step += 1;
-        n_bound += max_ins;
+        n_bound += max_ins + (has_weighting ? n_bound / 16 + 1024 : 0);
     }
     HIP_TRY(hipGetLastError());
     return UGX_OK;
@@ -1072,14 +1098,21 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort, co
     const CaseDev& cd = cases[ci];
     uint32_t n0 = 0;
     if (int rc = count(n0)) return rc;
-    // capacity from the expected count
     double ntot = 0.0, expected = 0.0;
     for (int s = 0; s < cd.ns; ++s) ntot += f->number_density[s];
+    // a cell-weighted case takes its cell weights from this fill (uniGasMeshFill.C:111-168)
+    if (cd.weighted) {
+        if (f->exact_per_cell > 0) return set_err(UGX_ERR_ARG, "a cell-weighted case is filled by number density, not per cell");
+        std::vector<double> wts;
+        if (int rc = fill_cell_weights(specs_host[ci], ntot, wts)) return rc;
+        if (int rc = set_cell_weights(ci, wts.data())) return rc;
+    }
+    // capacity from the expected count
     if (f->exact_per_cell > 0) expected = (double)f->exact_per_cell * g.nc + (double)g.nc * cd.ns;
     else {
-        double V = 0.0;
-        for (double v : g.vol) V += v;
-        expected = ntot / cd.fn * V * 1.05 + (double)g.nc * cd.ns + 64.0;
+        double VW = 0.0;
+        for (int c = 0; c < g.nc; ++c) VW += g.vol[c] / cw_host[(size_t)cd.cell_begin + c];
+        expected = ntot / cd.fn * VW * 1.05 + (double)g.nc * cd.ns + 64.0;
     }
     if (fixed) {
         // never grown here: the fill must fit (checked again exactly after the count)
@@ -1145,6 +1178,32 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort, co
     return ensure_capacity((uint64_t)n1 + (uint64_t)max_ins * 16);
 }
 
+int World::set_cell_weights(int ci, const double* wts)
+{
+    const CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    for (int c = 0; c < ncl; ++c)
+        if (!(wts[c] > 0.0) || !std::isfinite(wts[c])) return set_err(UGX_ERR_ARG, "cell weight %d is not > 0", c);
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::copy(wts, wts + ncl, cw_host.begin() + cd.cell_begin);
+    HIP_TRY(hipMemcpy(d_cw + cd.cell_begin, wts, sizeof(double) * ncl, hipMemcpyHostToDevice));
+    update_max_ins();
+    return UGX_OK;
+}
+
+// Bound of one step's inflow: floor + 1 of every face-species' expected count
+// at its cell's weight.
+void World::update_max_ins()
+{
+    max_ins = 0;
+    for (size_t ci = 0; ci < specs_host.size(); ++ci)
+        for (const auto& e : specs_host[ci].geom.inflow) {
+            const double nb = e.nbar_num / (e.nbar_den * cw_host[(size_t)cases[ci].cell_begin + e.cell]);
+            max_ins += (uint32_t)std::floor(nb) + 1u;
+        }
+    n_bound_valid = false;
+}
+
 int World::pause(int ci)
 {
     static const int32_t zero = 0;
@@ -1191,6 +1250,10 @@ int World::adopt(World& old, const std::vector<int>& old_of_new)
         else
             launch_carry_particles<double>(old.view<double>().a, view<double>().a, b, n, dst, cd, sd, stream);
         dst += n;
+        std::copy(old.cw_host.begin() + oc.cell_begin, old.cw_host.begin() + oc.cell_end,
+                  cw_host.begin() + nc_.cell_begin);
+        HIP_TRY(hipMemcpyAsync(d_cw + nc_.cell_begin, old.d_cw + oc.cell_begin, sizeof(double) * ncl,
+                               hipMemcpyDeviceToDevice, stream));
         const int32_t nwk = old.wf_begin[k + 1] - old.wf_begin[k];
         if (nwk > 0)
             HIP_TRY(hipMemcpyAsync(d_wsamp + (size_t)wf_begin[i] * kNS * kWRows,
@@ -1203,7 +1266,7 @@ int World::adopt(World& old, const std::vector<int>& old_of_new)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipMemcpyAsync(&d_st->n, &dst, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-    n_bound_valid = false;
+    update_max_ins();
     HIP_TRY(hipStreamSynchronize(stream));
     return UGX_OK;
 }
@@ -1273,6 +1336,8 @@ int World::case_counters(int ci, ugx_counters* out)
     out->track_errors = (int64_t)cnt[kCntTrackErrors];
     out->n_sampled_steps = (int64_t)cnt[kCntSampled] - samp_reset_steps[ci];
     out->particle_steps = (int64_t)cnt[kCntParticleSteps];
+    out->cloned = (int64_t)cnt[kCntCloned];
+    out->weight_deleted = (int64_t)cnt[kCntWeightDeleted];
     out->time_sampled = (double)out->n_sampled_steps * cases[ci].dt;
     int64_t k = 0;
     for (int32_t v : cm) k += v;
@@ -1321,6 +1386,13 @@ int World::case_state(int ci, ugx_case_state* s, bool upload)
                        (long long)nwk);
     if (s->wall_samples)
         if (int rc = case_wall_samples(ci, s->wall_samples, nwk * kNS * kWRows, upload)) return rc;
+    if (s->cell_weight) {
+        if (upload) {
+            if (int rc = set_cell_weights(ci, s->cell_weight)) return rc;
+        } else {
+            std::copy(cw_host.begin() + cd.cell_begin, cw_host.begin() + cd.cell_end, s->cell_weight);
+        }
+    }
     unsigned long long cnt[kCntNum];
     if (!upload) {
         for (const Arr& a : arrs) HIP_TRY(hipMemcpy(a.host, a.dev, a.bytes, hipMemcpyDeviceToHost));

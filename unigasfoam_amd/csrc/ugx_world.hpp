@@ -137,6 +137,19 @@ struct World {
     uint32_t* d_bidx = nullptr;
     double* d_wsamp = nullptr;
     int alloc_events(uint32_t n);
+    // cell weighting: per-cell weights (device and host copies; 1 unless a
+    // weighted case's fill or an upload set them), and with any weighted
+    // case the per-slot start cells, clone counts and offsets
+    int32_t has_weighting = 0;
+    double* d_cw = nullptr;
+    std::vector<double> cw_host;
+    uint32_t* d_cstart = nullptr;
+    uint32_t* d_wclone = nullptr;
+    uint32_t* d_wcoff = nullptr;
+    uint32_t* d_wbsum = nullptr;
+    // sets the weights of case ci's cells (ncl values) and the bound of a step's inflow
+    int set_cell_weights(int ci, const double* w);
+    void update_max_ins();
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;
@@ -204,9 +217,17 @@ struct World {
 };
 
 // Write-time fields of uniGasVolFields from raw sums (ugx_fields.cpp).
-int compute_fields(const CaseSpec& spec, const double* samp, int ncl, const ugx_counters& cnt, ugx_fields* f);
+// cw: the case's cell weights (F_N W_c molecules per simulator).
+int compute_fields(const CaseSpec& spec, const double* samp, int ncl, const ugx_counters& cnt, const double* cw,
+                   ugx_fields* f);
 // Write-time wall values of uniGasVolFields from the raw wall sums.
-int compute_wall_fields(const CaseSpec& spec, const double* wsamp, const ugx_counters& cnt, ugx_wall_fields* f);
+int compute_wall_fields(const CaseSpec& spec, const double* wsamp, const ugx_counters& cnt, const double* cw,
+                        ugx_wall_fields* f);
+// Cell weights of a weighted case's fill (uniGasMeshFill.C:111-168): the
+// total number density ntot spread at particles_per_subcell simulators per
+// cell, smoothed until neighbour ratios stay below 1 + max_cell_weight_ratio,
+// never below minParticlesPerSubCell's weight (ugx_weights.cpp).
+int fill_cell_weights(const CaseSpec& spec, double ntot, std::vector<double>& w);
 
 }  // namespace ugx
 

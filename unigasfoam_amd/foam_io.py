@@ -190,11 +190,11 @@ def _values(a, cls):
 
 
 # wall-only vol fields of uniGasVolFields (zero inside, measured values on
-# wall faces): key of Case.wall_fields() -> (file stem, class, dimensions)
+# wall faces): file stem -> (class, dimensions, key of Case.wall_fields())
 WALL_FIELD_FILES = {
-    "heat_flux": ("surfaceHeatTransfer", "volScalarField", "[1 0 -3 0 0 0 0]"),
-    "shear": ("surfaceShearStress", "volScalarField", "[1 -1 -2 0 0 0 0]"),
-    "force_density": ("fD", "volVectorField", "[1 -1 -2 0 0 0 0]"),
+    "surfaceHeatTransfer": ("volScalarField", "[1 0 -3 0 0 0 0]", "heat_flux"),
+    "surfaceShearStress": ("volScalarField", "[1 -1 -2 0 0 0 0]", "shear"),
+    "fD": ("volVectorField", "[1 -1 -2 0 0 0 0]", "force_density"),
 }
 # cell fields whose wall-face values come from the wall measurements
 # (uniGasVolFields.C:1262-1350); on other patches every field except these
@@ -204,17 +204,18 @@ _WALL_MEASURED = {"translational_T": "translational_T", "rotational_T": "rotatio
 _WIDTH = {"volScalarField": 1, "volVectorField": 3, "volSymmTensorField": 6}
 
 
-def _patch_values(mesh, p, key, a, wall):
-    """Boundary values of field `key` (cell values `a`) on patch p."""
+def _patch_values(mesh, p, key, a, wall, wall_only):
+    """Boundary values of field `key` (cell values `a`) on patch p; for a
+    wall-only field `key` names the wall value."""
     owners = mesh.owner[p.start:p.start + p.size]
-    if key in WALL_FIELD_FILES:
+    if wall_only:
         vals = np.zeros((p.size,) + a.shape[1:])
     elif p.type == abi.WALL and key in _WALL_MEASURED:
         vals = np.zeros((p.size,) + a.shape[1:])
     else:
         return a[owners]
     if p.type == abi.WALL and wall is not None:
-        src = wall["fields"][key if key in WALL_FIELD_FILES else _WALL_MEASURED[key]]
+        src = wall["fields"][key if wall_only else _WALL_MEASURED[key]]
         for k in np.nonzero(wall["patch"] == mesh.patch_index(p.name))[0]:
             vals[wall["face"][k]] = src[k]
     return vals
@@ -232,15 +233,15 @@ def write_fields(case_dir, time_name, mesh: PolyMesh, fields, field_name="uniGas
     d = os.path.join(case_dir, str(time_name))
     os.makedirs(d, exist_ok=True)
     written = []
-    files = dict(FIELD_FILES)
-    cells = dict(fields)
-    for key, spec in WALL_FIELD_FILES.items():
-        files[key] = spec
-        cells[key] = np.zeros((mesh.n_cells, _WIDTH[spec[1]]) if _WIDTH[spec[1]] > 1 else mesh.n_cells)
-    for key, (stem, cls, dims) in files.items():
-        if key not in cells:
+    # (key, file stem, class, dimensions, cell values, wall-only)
+    files = [(key, stem, cls, dims, fields.get(key), False) for key, (stem, cls, dims) in FIELD_FILES.items()]
+    for stem, (cls, dims, wkey) in WALL_FIELD_FILES.items():
+        width = _WIDTH[cls]
+        files.append((wkey, stem, cls, dims, np.zeros((mesh.n_cells, width) if width > 1 else mesh.n_cells), True))
+    for key, stem, cls, dims, cell_values, wall_only in files:
+        if cell_values is None:
             continue
-        a = np.asarray(cells[key], dtype=np.float64)
+        a = np.asarray(cell_values, dtype=np.float64)
         width = _WIDTH[cls]
         a = a.reshape(mesh.n_cells, width) if width > 1 else a.reshape(mesh.n_cells)
         obj = f"{stem}_{field_name}"
@@ -257,7 +258,7 @@ def write_fields(case_dir, time_name, mesh: PolyMesh, fields, field_name="uniGas
                 elif p.type == abi.CYCLIC:
                     f.write(f"    {p.name}\n    {{\n        type            cyclic;\n    }}\n")
                 else:
-                    v = _patch_values(mesh, p, key, a, wall)
+                    v = _patch_values(mesh, p, key, a, wall, wall_only)
                     f.write(f"    {p.name}\n    {{\n        type            calculated;\n"
                             f"        value           nonuniform List<{kind}>\n{p.size}\n(\n{_values(v, cls)})\n;\n    }}\n")
             f.write("}\n")

@@ -72,6 +72,12 @@ class CaseSpec:
     reset_until_time: float = 0.0        # <= 0: no limit (the reference's VGREAT)
     sampler_full: bool = False
     seed: int = 12345
+    # cell weighting (cellWeightedSimulation / cellWeightedProperties)
+    cell_weighted: bool = False
+    particles_per_subcell: int = 0
+    min_particles_per_subcell: int = 0
+    max_smoothing_passes: int = 500
+    max_cell_weight_ratio: float = 0.05
 
     def species_index(self, name):
         for i, s in enumerate(self.species):
@@ -112,6 +118,11 @@ class CaseSpec:
         m.sampler_full = 1 if self.sampler_full else 0
         m.reset_at_decomposition = 1 if self.reset_at_decomposition else 0
         m.reset_until_time = self.reset_until_time
+        m.cell_weighted = 1 if self.cell_weighted else 0
+        m.particles_per_subcell = self.particles_per_subcell
+        m.min_particles_per_subcell = self.min_particles_per_subcell
+        m.max_smoothing_passes = self.max_smoothing_passes
+        m.max_cell_weight_ratio = self.max_cell_weight_ratio
         nb = len(self.boundaries)
         bd = (abi.BoundaryDesc * max(nb, 1))()
         for i, b in enumerate(self.boundaries):
