@@ -366,7 +366,11 @@ int ugx_cell_centres(ugx_case* c, double* out, int64_t len);
  * ugx_particles_download with local_position set (the storage form); a
  * restart creates a case with the same descriptors and seed, uploads those
  * simulators, then this state.  The restarted case continues bit for bit.
- * Per-cell arrays are cell-major (row r of cell c at [c * rows + r]). */
+ * Per-cell arrays are cell-major (row r of cell c at [c * rows + r]).
+ * This is the library's own restart form: it is not the reference's
+ * <time>/lagrangian/uniGas parcel files (uniGasParcelIO.C:89-181), which
+ * store positions as tet-barycentric coordinates, so a state written by
+ * uniGasFoam cannot be read here nor the other way round. */
 #define UGX_KN_ROWS     16   /* hybrid time sums per cell */
 #define UGX_N_COUNTERS  16   /* counter slots (library order; opaque) */
 
