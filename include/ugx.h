@@ -348,6 +348,13 @@ int ugx_kn_download(ugx_case* c, double* out, int64_t len);
  * uniGasCloud.C:612-623); an upload also rescales the inflow counts. */
 int ugx_cell_weights_download(ugx_case* c, double* out, int64_t len);
 int ugx_cell_weights_upload(ugx_case* c, const double* in, int64_t len);
+/* NTC subcell levels per cell and axis, [n_cells][3] (the SubCellLevels
+ * field of uniGasCloud.C:461-474 read by noTimeCounter.C:97-161): the
+ * second candidate of a pair comes from the first's virtual Cartesian
+ * subcell when that holds two or more simulators.  Levels are 1..10 and 1
+ * in empty directions; all 1 (the default) is the plain NTC. */
+int ugx_subcell_levels_download(ugx_case* c, int32_t* out, int64_t len);
+int ugx_subcell_levels_upload(ugx_case* c, const int32_t* in, int64_t len);
 int ugx_cell_volumes(ugx_case* c, double* out, int64_t len);
 /* Wall faces of the case: their count, and per face its patch and its face
  * index within the patch. */

@@ -140,9 +140,14 @@ struct orc_world {
     /* cell weighting */
     double* cw;       /* [nc] cell weights (1 without weighting) */
     int32_t* cstart;  /* [cap] cell each simulator started the step in */
+    /* subcells of the NTC partner selection: levels per cell and axis */
+    int32_t* sclev;   /* [nc*3] (1 in every direction by default) */
     /* counters */
     ugx_counters cnt;
 };
+
+/* largest subcell level along an axis (uniGasDynamicAdapter.C:49) */
+#define ORC_MAX_SUBCELL_LEVELS 10
 
 static void smooth_field(const orc_world* w, const double* f, double* out, int ncomp);
 static int fill_weights(orc_world* w, double ntot);
@@ -820,11 +825,63 @@ static void cell_sums(const orc_world* w, int64_t b, int64_t e, double S[NSAMP])
     free(v);
 }
 
-/* noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell
- * level 1 (the whole cell). */
+/* Subcell of a simulator along one axis (noTimeCounter.C:141-155): the
+ * virtual Cartesian subcells split the cell's bounding box into lv equal
+ * slabs; x and lo are in the cell frame.  The index is truncated like the
+ * reference's label() and kept inside [0, lv - 1] (a simulator on the box's
+ * upper face would otherwise index one slab past the last). */
+static int subcell_axis(double x, double lo, double len, int lv)
+{
+    double t = (double)lv * (x - lo) / len;
+    int k = (int)t;
+    if (k < 0) k = 0;
+    if (k > lv - 1) k = lv - 1;
+    return k;
+}
+
+/* Subcell lists of a cell (noTimeCounter.C:133-161): sid[i] the subcell of
+ * the cell's i-th simulator, start[s]..start[s+1] the range of list[] that
+ * holds subcell s's simulators in ascending order (the reference appends
+ * them to its per-subcell lists in cell order).  The subcell index weights
+ * the axes x, y, z by 1, lx, lx*ly; levels of empty directions are 1. */
+static void subcell_lists(const orc_world* w, int c, int64_t b, int64_t nC, const int lv[3], int nsub,
+                          int32_t* sid, int32_t* start, int32_t* list)
+{
+    double lo[3], len[3];
+    for (int d = 0; d < 3; ++d) { lo[d] = w->bblo[3 * c + d]; len[d] = w->bbhi[3 * c + d] - lo[d]; }
+    for (int s = 0; s <= nsub; ++s) start[s] = 0;
+    for (int64_t i = 0; i < nC; ++i) {
+        const double* x = &w->x[3 * (b + i)];
+        int k0 = subcell_axis(x[0], lo[0], len[0], lv[0]);
+        int k1 = subcell_axis(x[1], lo[1], len[1], lv[1]);
+        int k2 = subcell_axis(x[2], lo[2], len[2], lv[2]);
+        int s = k0 + lv[0] * (k1 + lv[1] * k2);
+        sid[i] = s;
+        start[s + 1] += 1;
+    }
+    for (int s = 1; s <= nsub; ++s) start[s] += start[s - 1];
+    int32_t* cur = malloc(sizeof(int32_t) * (size_t)nsub);
+    for (int s = 0; s < nsub; ++s) cur[s] = start[s];
+    for (int64_t i = 0; i < nC; ++i) list[cur[sid[i]]++] = (int32_t)i;
+    free(cur);
+}
+
+/* noTimeCounter::collide for one cell (noTimeCounter.C:83-312).  With more
+ * than one subcell the second candidate comes from the first's subcell
+ * when that holds two or more simulators, else from the whole cell
+ * (:193-225). */
 static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, int64_t* coll)
 {
     if (nC < 2) return;
+    const int lv[3] = {w->sclev[3 * c], w->sclev[3 * c + 1], w->sclev[3 * c + 2]};
+    const int nsub = lv[0] * lv[1] * lv[2];
+    int32_t *sid = NULL, *start = NULL, *list = NULL;
+    if (nsub > 1) {
+        sid = malloc(sizeof(int32_t) * (size_t)nC);
+        list = malloc(sizeof(int32_t) * (size_t)nC);
+        start = malloc(sizeof(int32_t) * (size_t)(nsub + 1));
+        subcell_lists(w, c, b, nC, lv, nsub, sid, start, list);
+    }
     orc_rng r;
     orc_rng_init(&r, w->seed, ORC_RNG_NTC, 0u, (uint32_t)c, w->step);
     double smax0 = w->smax[c];
@@ -838,7 +895,13 @@ static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, 
     for (int64_t k = 0; k < ncand; ++k) {
         uint32_t P = orc_index(&r, (uint32_t)nC);
         uint32_t Q;
-        do { Q = orc_index(&r, (uint32_t)nC); } while (Q == P);
+        uint32_t nSC = 0, s0 = 0;
+        if (nsub > 1) { s0 = (uint32_t)start[sid[P]]; nSC = (uint32_t)start[sid[P] + 1] - s0; }
+        if (nSC > 1) {
+            do { Q = (uint32_t)list[s0 + orc_index(&r, nSC)]; } while (Q == P);
+        } else {
+            do { Q = orc_index(&r, (uint32_t)nC); } while (Q == P);
+        }
         int64_t ip = b + P, iq = b + Q;
         int sp = w->sp[ip], sq = w->sp[iq];
         double* UP = &w->v[3 * ip];
@@ -851,6 +914,9 @@ static void ntc_cell(orc_world* w, int c, int64_t b, int64_t nC, int64_t* cand, 
         }
     }
     w->smax[c] = smax;
+    free(sid);
+    free(start);
+    free(list);
 }
 
 /* unifiedStochasticParticleSBGK::samplePostCollisionVelocity
@@ -1123,6 +1189,8 @@ int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n
     w->cw = xcalloc((size_t)nc, sizeof(double));
     for (int c = 0; c < nc; ++c) w->cw[c] = 1.0;
     w->cstart = xcalloc((size_t)w->cap, sizeof(int32_t));
+    w->sclev = xcalloc((size_t)nc * 3, sizeof(int32_t));
+    for (int c = 0; c < 3 * nc; ++c) w->sclev[c] = 1;
     int cm = (model->collision_model == UGX_COLL_DSMC) ? 1 : 0;  /* hybrid starts all-BGK (uniGasCloud.C:726-732) */
     for (int c = 0; c < nc; ++c) { w->cmodel[c] = cm; w->uspmax[c] = 1.0; }
     *out = w;
@@ -1135,7 +1203,7 @@ void orc_destroy(orc_world* w)
     void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->farea, w->fkind, w->bm, w->inflow,
                     w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
                     w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn,
-                    w->wface, w->wgeo, w->wsamp, w->ev, w->wcell, w->cw, w->cstart};
+                    w->wface, w->wgeo, w->wsamp, w->ev, w->wcell, w->cw, w->cstart, w->sclev};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(w);
 }
@@ -1417,6 +1485,24 @@ int orc_set_cell_weights(orc_world* w, const double* in, int64_t len)
     for (int c = 0; c < w->nc; ++c)
         if (!(in[c] > 0.0)) return fail(UGX_ERR_ARG, "cell weight not > 0");
     memcpy(w->cw, in, sizeof(double) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_subcell_levels(orc_world* w, int32_t* out, int64_t len)
+{
+    if (len < 3 * (int64_t)w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(out, w->sclev, sizeof(int32_t) * 3 * (size_t)w->nc);
+    return UGX_OK;
+}
+
+int orc_set_subcell_levels(orc_world* w, const int32_t* in, int64_t len)
+{
+    if (len < 3 * (int64_t)w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    for (int64_t i = 0; i < 3 * (int64_t)w->nc; ++i) {
+        if (in[i] < 1 || in[i] > ORC_MAX_SUBCELL_LEVELS) return fail(UGX_ERR_ARG, "subcell level out of range");
+        if (w->mask[i % 3] == 0.0 && in[i] != 1) return fail(UGX_ERR_ARG, "subcell level > 1 in an empty direction");
+    }
+    memcpy(w->sclev, in, sizeof(int32_t) * 3 * (size_t)w->nc);
     return UGX_OK;
 }
 

@@ -84,6 +84,9 @@ int64_t orc_wall_face_count(orc_world* w);
 /* Cell weights [n_cells] (1 unless a cell-weighted fill or a set changed them). */
 int orc_cell_weights(orc_world* w, double* out, int64_t len);
 int orc_set_cell_weights(orc_world* w, const double* in, int64_t len);
+/* NTC subcell levels per cell and axis, [nc*3] (noTimeCounter.C:97-161) */
+int orc_subcell_levels(orc_world* w, int32_t* out, int64_t len);
+int orc_set_subcell_levels(orc_world* w, const int32_t* in, int64_t len);
 int orc_wall_samples(orc_world* w, double* out, int64_t len);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 

@@ -94,6 +94,8 @@ def _bind(L):
     L.orc_wall_samples.argtypes = [vp, abi.DBL_P, C.c_int64]
     L.orc_cell_weights.argtypes = [vp, abi.DBL_P, C.c_int64]
     L.orc_set_cell_weights.argtypes = [vp, abi.DBL_P, C.c_int64]
+    L.orc_subcell_levels.argtypes = [vp, abi.I32_P, C.c_int64]
+    L.orc_set_subcell_levels.argtypes = [vp, abi.I32_P, C.c_int64]
 
 
 def _check(rc):
@@ -176,6 +178,15 @@ class OracleCase:
     def set_cell_weights(self, w):
         w = np.ascontiguousarray(w, dtype=np.float64)
         _check(self.L.orc_set_cell_weights(self.h, abi.dptr(w), self.nc))
+
+    def subcell_levels(self):
+        out = np.zeros((self.nc, 3), dtype=np.int32)
+        _check(self.L.orc_subcell_levels(self.h, abi.iptr(out), out.size))
+        return out
+
+    def set_subcell_levels(self, lv):
+        lv = np.ascontiguousarray(np.broadcast_to(np.asarray(lv, dtype=np.int32), (self.nc, 3)))
+        _check(self.L.orc_set_subcell_levels(self.h, abi.iptr(lv), lv.size))
 
     def wall_face_count(self):
         return int(self.L.orc_wall_face_count(self.h))

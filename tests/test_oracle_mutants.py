@@ -17,6 +17,7 @@ import pytest
 
 import test_oracle_physics as phys
 import test_oracle_pins as pins
+import test_oracle_subcells as subcells
 import test_oracle_wall as wall
 import test_oracle_weighting as weighting
 from oracle import orc
@@ -26,6 +27,18 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fas
 
 # name: (snippet, replacement, pins that must fail)
 MUTANTS = {
+    # NTC second candidate always from the whole cell (noTimeCounter.C:193-225)
+    "ntc_subcells_ignored": (
+        "        if (nSC > 1) {\n            do { Q = (uint32_t)list[",
+        "        if (0) {\n            do { Q = (uint32_t)list[",
+        [lambda: subcells.test_subcell_momentum_is_conserved((2, 2, 1)),
+         lambda: subcells.test_subcell_momentum_is_conserved((1, 1, 4))]),
+    # subcell index with the y axis unweighted: subcells (1,0) and (0,1) share a list (:151-155)
+    "subcell_index_weights": (
+        "int s = k0 + lv[0] * (k1 + lv[1] * k2);",
+        "int s = k0 + k1 + lv[1] * k2;",
+        [lambda: subcells.test_subcell_momentum_is_conserved((2, 2, 1)),
+         lambda: subcells.test_subcell_momentum_is_conserved((2, 3, 1))]),
     # VSS deflection exponent R^alpha for R^(1/alpha) (variableSoftSphere.C:147)
     "vss_deflection_exponent": (
         "2.0 * orc_pow(orc_uniform(r), 1.0 / alpha) - 1.0",

@@ -27,6 +27,7 @@ EXPORTS = [
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry", "ugx_state_download", "ugx_state_upload",
     "ugx_wall_face_count", "ugx_wall_faces", "ugx_wall_samples_download", "ugx_wall_fields_compute",
     "ugx_cell_weights_download", "ugx_cell_weights_upload",
+    "ugx_subcell_levels_download", "ugx_subcell_levels_upload",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_tile_config", "ugx_probe_copy",
     "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
     "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
@@ -71,6 +72,8 @@ def lib():
         L.ugx_kn_download.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_weights_download.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_cell_weights_upload.argtypes = [vp, abi.DBL_P, C.c_int64]
+        L.ugx_subcell_levels_download.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.ugx_subcell_levels_upload.argtypes = [vp, abi.I32_P, C.c_int64]
         L.ugx_wall_face_count.argtypes = [vp, C.POINTER(C.c_int64)]
         L.ugx_wall_faces.argtypes = [vp, abi.I32_P, abi.I32_P, C.c_int64]
         L.ugx_wall_samples_download.argtypes = [vp, abi.DBL_P, C.c_int64]
@@ -265,6 +268,16 @@ class Case:
     def set_cell_weights(self, w):
         w = np.ascontiguousarray(w, dtype=np.float64)
         check(self.L.ugx_cell_weights_upload(self.h, abi.dptr(w), self.nc), self.L)
+
+    def subcell_levels(self):
+        """NTC subcell levels per cell and axis, [n_cells, 3]."""
+        out = np.zeros((self.nc, 3), dtype=np.int32)
+        check(self.L.ugx_subcell_levels_download(self.h, abi.iptr(out), out.size), self.L)
+        return out
+
+    def set_subcell_levels(self, lv):
+        lv = np.ascontiguousarray(np.broadcast_to(np.asarray(lv, dtype=np.int32), (self.nc, 3)))
+        check(self.L.ugx_subcell_levels_upload(self.h, abi.iptr(lv), lv.size), self.L)
 
     def wall_face_count(self):
         n = C.c_int64()

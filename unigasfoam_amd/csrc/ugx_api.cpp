@@ -347,6 +347,35 @@ int ugx_cell_weights_upload(ugx_case* c, const double* in, int64_t len)
     UGX_GUARD_END
 }
 
+int ugx_subcell_levels_download(ugx_case* c, int32_t* out, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (!out) return set_err(UGX_ERR_ARG, "null output");
+    int b, n;
+    if (int rc = cell_range(c, len, 3, b, n)) return rc;
+    for (int i = 0; i < n; ++i) {
+        const uint32_t p = c->world->sclev_host[(size_t)b + i];
+        out[3 * (size_t)i] = (int32_t)(p & 0xffu);
+        out[3 * (size_t)i + 1] = (int32_t)((p >> 8) & 0xffu);
+        out[3 * (size_t)i + 2] = (int32_t)((p >> 16) & 0xffu);
+    }
+    return UGX_OK;
+    UGX_GUARD_END
+}
+
+int ugx_subcell_levels_upload(ugx_case* c, const int32_t* in, int64_t len)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    if (c->batch) return set_err(UGX_ERR_STATE, "the subcell levels of a batch case are carried by the batch");
+    if (!in) return set_err(UGX_ERR_ARG, "null input");
+    int b, n;
+    if (int rc = cell_range(c, len, 3, b, n)) return rc;
+    return c->world->set_subcell_levels(c->index, in);
+    UGX_GUARD_END
+}
+
 int ugx_wall_face_count(ugx_case* c, int64_t* n)
 {
     UGX_GUARD_BEGIN

@@ -1712,11 +1712,61 @@ __global__ void __launch_bounds__(Block) __attribute__((amdgpu_waves_per_eu((NS 
 
 // ------------------------------------------------------------- collisions
 
-// noTimeCounter::collide for one cell (noTimeCounter.C:83-312), subcell level 1.
+// Subcell of a simulator along one axis (noTimeCounter.C:141-155): the
+// cell's bounding box in lv equal slabs, the index truncated and kept in
+// [0, lv - 1] (a simulator on the upper face would index one past the last).
+__device__ inline uint32_t subcell_axis(double x, double lo, double len, uint32_t lv)
+{
+    const double t = (double)lv * (x - lo) / len;
+    int k = (int)t;
+    k = k < 0 ? 0 : k;
+    return (uint32_t)min(k, (int)lv - 1);
+}
+
+// Subcell lists of cell c (noTimeCounter.C:133-161), sequential in the
+// cell's lane: scid[b + i] the subcell of its i-th simulator (axes weighted
+// 1, lx, lx ly), tab[s]..tab[s+1] the range of sclist[b + .] holding subcell
+// s's cell-local indices in ascending order; tab[nsub + 1 + s] are cursors.
+template <class R>
+__device__ inline void subcell_lists(const WorldView<R>& w, uint32_t c, uint32_t b, uint32_t nC, const uint32_t lv[3],
+                                     uint32_t nsub, uint32_t* tab)
+{
+    double lo[3], len[3];
+    for (int d = 0; d < 3; ++d) { lo[d] = w.bblo[3 * (size_t)c + d]; len[d] = w.bbhi[3 * (size_t)c + d] - lo[d]; }
+    for (uint32_t s = 0; s <= nsub; ++s) tab[s] = 0u;
+    for (uint32_t i = 0; i < nC; ++i) {
+        const uint32_t j = b + i;
+        const uint32_t k0 = subcell_axis((double)w.b.x[j], lo[0], len[0], lv[0]);
+        const uint32_t k1 = subcell_axis((double)w.b.y[j], lo[1], len[1], lv[1]);
+        const uint32_t k2 = subcell_axis((double)w.b.z[j], lo[2], len[2], lv[2]);
+        const uint32_t s = k0 + lv[0] * (k1 + lv[1] * k2);
+        w.scid[j] = s;
+        tab[s + 1u] += 1u;
+    }
+    for (uint32_t s = 1; s <= nsub; ++s) tab[s] += tab[s - 1u];
+    uint32_t* cur = tab + nsub + 1u;
+    for (uint32_t s = 0; s < nsub; ++s) cur[s] = tab[s];
+    for (uint32_t i = 0; i < nC; ++i) w.sclist[b + cur[w.scid[b + i]]++] = i;
+}
+
+// noTimeCounter::collide for one cell (noTimeCounter.C:83-312).  With more
+// than one subcell the second candidate comes from the first's subcell when
+// that holds two or more simulators, else from the whole cell (:193-225).
 template <class R, class Acc>
 __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
                                 const Acc& A, unsigned long long& cand, unsigned long long& coll)
 {
+    uint32_t lv[3] = {1u, 1u, 1u}, nsub = 1u;
+    uint32_t* tab = nullptr;
+    if (w.has_subcells) {
+        const uint32_t p = w.sclev[c];
+        lv[0] = p & 0xffu; lv[1] = (p >> 8) & 0xffu; lv[2] = (p >> 16) & 0xffu;
+        nsub = lv[0] * lv[1] * lv[2];
+        if (nsub > 1u) {
+            tab = w.sctab + w.scoff[c];
+            subcell_lists<R>(w, c, b, nC, lv, nsub, tab);
+        }
+    }
     Rng r;
     rng_init(r, cs.seed_lo, cs.seed_hi, kRngNtc, 0u, c - (uint32_t)cs.cell_begin, cs.step);
     const double smax0 = w.smax[c];
@@ -1739,8 +1789,17 @@ __device__ inline void ntc_cell(const WorldView<R>& w, const CaseDev& cs, uint32
     cand += (unsigned long long)ncand;
     for (long long k = 0; k < ncand; ++k) {
         uint32_t P = uindex(r, nC);
-        uint32_t Q;
-        do { Q = uindex(r, nC); } while (Q == P);
+        uint32_t Q, nSC = 0u, s0 = 0u;
+        if (nsub > 1u) {
+            const uint32_t s = w.scid[b + P];
+            s0 = tab[s];
+            nSC = tab[s + 1u] - s0;
+        }
+        if (nSC > 1u) {
+            do { Q = w.sclist[b + s0 + uindex(r, nSC)]; } while (Q == P);
+        } else {
+            do { Q = uindex(r, nC); } while (Q == P);
+        }
         const uint32_t ip = b + P, iq = b + Q;
         // a single-species case needs no species lookup
         const int lp = one ? 0 : A.local_species(ip), lq = one ? 0 : A.local_species(iq);

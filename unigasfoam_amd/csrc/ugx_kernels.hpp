@@ -146,7 +146,22 @@ struct WorldView {
     uint32_t* wclone;         // [cap + 1]
     uint32_t* wcoff;          // [cap + 1]
     uint32_t* wbsum;          // scan scratch for cap entries
+    // NTC subcells (noTimeCounter.C:97-161): with any cell split, per cell
+    // its packed levels (x | y << 8 | z << 16) and the offset of its subcell
+    // table (2 nsub + 1 words: subcell starts, then cursors), per slot of
+    // the sorted order its subcell and the cells' subcell lists
+    int32_t has_subcells;
+    const uint32_t* sclev;    // [nc]
+    const uint32_t* scoff;    // [nc]
+    uint32_t* sctab;          // [sum over cells of 2 nsub + 1]
+    uint32_t* scid;           // [cap]
+    uint32_t* sclist;         // [cap]
+    const double* bblo;       // [nc*3] cell bounding box in the cell frame
+    const double* bbhi;       // [nc*3]
 };
+
+// Largest subcell level along an axis (uniGasDynamicAdapter.C:49).
+constexpr int kMaxSubcellLevels = 10;
 
 // Records wall hits of measured faces into the step's event list.  Only
 // the hits of a sampling step of the cell's case are recorded: the

@@ -355,7 +355,8 @@ void World::release()
                      (void**)&d_wq_slot, (void**)&d_wq_info, (void**)&d_wq_tau,
                      (void**)&d_wface, (void**)&d_wgeo, (void**)&d_wcell, (void**)&d_ev, (void**)&d_ev_key,
                      (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp,
-                     (void**)&d_cw, (void**)&d_cstart, (void**)&d_wclone, (void**)&d_wcoff, (void**)&d_wbsum};
+                     (void**)&d_cw, (void**)&d_cstart, (void**)&d_wclone, (void**)&d_wcoff, (void**)&d_wbsum,
+                     (void**)&d_sclev, (void**)&d_scoff, (void**)&d_sctab, (void**)&d_scid, (void**)&d_sclist};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -438,6 +439,12 @@ int World::alloc_particles(uint32_t newcap)
         if (int rc = dalloc(d_wclone, (size_t)newcap + 1)) return rc;
         if (int rc = dalloc(d_wcoff, (size_t)newcap + 1)) return rc;
         if (int rc = dalloc(d_wbsum, ((size_t)newcap + 1 + 1023) / 1024 + 2)) return rc;
+    }
+    if (has_subcells) {
+        dfree(d_scid);
+        dfree(d_sclist);
+        if (int rc = dalloc(d_scid, newcap)) return rc;
+        if (int rc = dalloc(d_sclist, newcap)) return rc;
     }
     cap = newcap;
     if (nwf > 0 && ev_cap < std::max<uint32_t>(newcap / 8, 65536u))
@@ -569,6 +576,14 @@ WorldView<R> World::view() const
     w.wclone = d_wclone;
     w.wcoff = d_wcoff;
     w.wbsum = d_wbsum;
+    w.has_subcells = has_subcells;
+    w.sclev = d_sclev;
+    w.scoff = d_scoff;
+    w.sctab = d_sctab;
+    w.scid = d_scid;
+    w.sclist = d_sclist;
+    w.bblo = d_bblo;
+    w.bbhi = d_bbhi;
     return w;
 }
 
@@ -844,6 +859,7 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     }
     cw_host.assign((size_t)nc, 1.0);
     if ((rc = up(d_cw, cw_host))) return rc;
+    sclev_host.assign((size_t)nc, 0x010101u);
     nwf = wf_begin.back();
     if (nwf > 0) {
         if ((rc = up(d_wface, wface_all))) return rc;
@@ -1191,6 +1207,62 @@ int World::set_cell_weights(int ci, const double* wts)
     return UGX_OK;
 }
 
+int World::set_subcell_levels(int ci, const int32_t* lv)
+{
+    const CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    const double* mask = specs_host[ci].geom.mask;
+    for (int c = 0; c < ncl; ++c)
+        for (int d = 0; d < 3; ++d) {
+            const int32_t l = lv[3 * (size_t)c + d];
+            if (l < 1 || l > kMaxSubcellLevels)
+                return set_err(UGX_ERR_ARG, "subcell level %d of cell %d is outside 1..%d", l, c, kMaxSubcellLevels);
+            if (mask[d] == 0.0 && l != 1) return set_err(UGX_ERR_ARG, "subcell level > 1 in the empty direction %d", d);
+        }
+    for (int c = 0; c < ncl; ++c)
+        sclev_host[(size_t)cd.cell_begin + c] = (uint32_t)lv[3 * (size_t)c] | ((uint32_t)lv[3 * (size_t)c + 1] << 8) |
+                                                ((uint32_t)lv[3 * (size_t)c + 2] << 16);
+    return upload_subcells();
+}
+
+int World::upload_subcells()
+{
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    std::vector<uint32_t> off((size_t)nc);
+    uint64_t tot = 0;
+    int any = 0;
+    for (int c = 0; c < nc; ++c) {
+        const uint32_t p = sclev_host[(size_t)c];
+        const uint32_t nsub = (p & 0xffu) * ((p >> 8) & 0xffu) * ((p >> 16) & 0xffu);
+        off[(size_t)c] = (uint32_t)tot;
+        if (nsub > 1u) { any = 1; tot += 2ull * nsub + 1ull; }
+    }
+    if (tot > 0xF0000000ull) return set_err(UGX_ERR_CAPACITY, "subcell tables too large");
+    dfree(d_sctab);
+    if (!any) {
+        // no cell is split: the tables are released and the NTC takes the whole cell
+        dfree(d_sclev); dfree(d_scoff); dfree(d_scid); dfree(d_sclist);
+        has_subcells = 0;
+        return UGX_OK;
+    }
+    if (!d_sclev) {
+        if (int rc = dalloc(d_sclev, (size_t)nc)) return rc;
+        if (int rc = dalloc(d_scoff, (size_t)nc)) return rc;
+    }
+    if (int rc = dalloc(d_sctab, (size_t)tot)) return rc;
+    HIP_TRY(hipMemcpy(d_sclev, sclev_host.data(), sizeof(uint32_t) * (size_t)nc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_scoff, off.data(), sizeof(uint32_t) * (size_t)nc, hipMemcpyHostToDevice));
+    if (!has_subcells) {
+        has_subcells = 1;
+        if (cap > 0) {
+            if (int rc = dalloc(d_scid, cap)) return rc;
+            if (int rc = dalloc(d_sclist, cap)) return rc;
+        }
+    }
+    return UGX_OK;
+}
+
 // Bound of one step's inflow: floor + 1 of every face-species' expected count
 // at its cell's weight.
 void World::update_max_ins()
@@ -1268,6 +1340,17 @@ int World::adopt(World& old, const std::vector<int>& old_of_new)
     HIP_TRY(hipMemcpyAsync(&d_st->n, &dst, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     update_max_ins();
     HIP_TRY(hipStreamSynchronize(stream));
+    bool split = false;
+    for (size_t i = 0; i < old_of_new.size(); ++i) {
+        const int k = old_of_new[i];
+        if (k < 0) continue;
+        const CaseDev &oc = old.cases[k], &nc_ = cases[i];
+        std::copy(old.sclev_host.begin() + oc.cell_begin, old.sclev_host.begin() + oc.cell_end,
+                  sclev_host.begin() + nc_.cell_begin);
+        split = split || old.has_subcells;
+    }
+    if (split)
+        if (int rc = upload_subcells()) return rc;
     return UGX_OK;
 }
 

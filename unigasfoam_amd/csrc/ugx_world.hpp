@@ -150,6 +150,20 @@ struct World {
     // sets the weights of case ci's cells (ncl values) and the bound of a step's inflow
     int set_cell_weights(int ci, const double* w);
     void update_max_ins();
+    // NTC subcells: packed levels per cell (x | y << 8 | z << 16; host and
+    // device), and once any cell is split the subcell tables and the per-slot
+    // subcell ids and lists
+    int32_t has_subcells = 0;
+    std::vector<uint32_t> sclev_host;
+    uint32_t* d_sclev = nullptr;
+    uint32_t* d_scoff = nullptr;
+    uint32_t* d_sctab = nullptr;
+    uint32_t* d_scid = nullptr;
+    uint32_t* d_sclist = nullptr;
+    // sets the levels of case ci's cells ([ncl*3], each in 1..10, 1 in empty directions)
+    int set_subcell_levels(int ci, const int32_t* lv);
+    // the subcell tables and per-slot buffers for sclev_host
+    int upload_subcells();
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;
