@@ -151,6 +151,21 @@ typedef struct {
     int32_t min_particles_per_subcell;/* ... minParticlesPerSubCell (0: particles_per_subcell) */
     int32_t max_smoothing_passes;     /* weight smoothing passes at most (0: 500) */
     double  max_cell_weight_ratio;    /* smoothing stops once neighbour ratios are below 1 + this (0: 0.05) */
+    /* adaptive simulation (adaptiveSimulation and adaptiveProperties;
+       uniGasDynamicAdapter.C:187-217, 526-700): the fill sets the time step
+       and NTC subcell levels from the filled gas, then every
+       adaptation_interval steps the time-averaged density, velocity and
+       temperature set them again (and the cell weights of a cell-weighted
+       case) */
+    int32_t adaptive;                 /* 0 off */
+    int32_t time_step_adaptation;     /* timeStepAdaptation */
+    int32_t subcell_adaptation;       /* subCellAdaptation */
+    int32_t cell_weight_adaptation;   /* cellWeightAdaptation (a cell-weighted case only) */
+    int32_t adaptation_interval;      /* adaptationInterval (0: 20) */
+    int32_t adaptation_smoothing_passes; /* smoothingPasses (0: 25; < 0: none) */
+    double  max_time_step_mct_ratio;  /* maxTimeStepMCTRatio (0: 0.2) */
+    double  max_courant_number;       /* maxCourantNumber (0: 0.5) */
+    double  max_subcell_size_mfp_ratio; /* maxSubCellSizeMFPRatio (0: 0.5) */
 } ugx_model_desc;
 
 typedef enum {
@@ -336,6 +351,9 @@ int ugx_samples_download(ugx_case* c, double* sums, int64_t len);
 int ugx_samples_reset(ugx_case* c);
 int ugx_fields_compute(ugx_case* c, ugx_fields* f);
 int ugx_counters_get(ugx_case* c, ugx_counters* out);
+/* The case's current time step (the controlDict deltaT, or the adapter's,
+ * uniGasDynamicAdapter.C:316-372) and the simulated time. */
+int ugx_time_get(ugx_case* c, double* delta_t, double* time);
 int ugx_sigma_max_download(ugx_case* c, double* out, int64_t len);
 int ugx_sigma_max_upload(ugx_case* c, const double* in, int64_t len);
 int ugx_cell_model_download(ugx_case* c, int32_t* out, int64_t len);
@@ -399,6 +417,16 @@ typedef struct {
     double*  wall_samples;       /* [n_wall_faces * UGX_MAX_SPECIES * UGX_W_COUNT] wall sums;
                                     NULL on upload: the case's wall sums are left as they are */
     double*  cell_weight;        /* [n_cells] cell weights; NULL: not carried */
+    /* time and adaptation */
+    double   delta_t;            /* the current time step (changed by time-step adaptation) */
+    double   time;               /* simulated time */
+    double   time_sampled;       /* time steps of the sampled steps summed since the last sampler reset
+                                    (NaN on upload: the sampled steps times the time step) */
+    double   adapt_time;         /* time summed into adapt_sums since the last adaptation */
+    int32_t* subcell_levels;     /* [n_cells * 3]; NULL: not carried */
+    double*  adapt_sums;         /* [n_cells * UGX_KN_ROWS] adapter time sums; NULL: not carried */
+    double*  adapt_ratio;        /* [n_cells * 3] cell size to mean free path ratio of the last
+                                    adaptation; NULL: not carried */
 } ugx_case_state;
 
 /* Copy the state out (any case) or in (a plain case; every pointer non-NULL). */

@@ -66,6 +66,7 @@ typedef struct {
     int cell, species, desc;
     double nbar;       /* expected simulators per step (Bird 4.22) at cell weight 1 */
     double nbar_num, nbar_den;   /* its numerator and F_N part of the denominator */
+    double flux_an, flux_mps, flux_br;   /* face area x number density, most probable speed, flux bracket */
     int nv;
     double vloc[8][3]; /* face vertices in the cell frame */
     double cum[6];     /* cumulative triangle area fractions (fan from vertex 0) */
@@ -142,6 +143,11 @@ struct orc_world {
     int32_t* cstart;  /* [cap] cell each simulator started the step in */
     /* subcells of the NTC partner selection: levels per cell and axis */
     int32_t* sclev;   /* [nc*3] (1 in every direction by default) */
+    /* time and the dynamic adapter */
+    double time;      /* simulated time at the start of the step */
+    double* adacc;    /* [nc*KNR] adapter time sums since the last adaptation */
+    double* adratio;  /* [nc*3] cell size to mean free path ratio of the last adaptation */
+    double adapt_time;
     /* counters */
     ugx_counters cnt;
 };
@@ -151,6 +157,8 @@ struct orc_world {
 
 static void smooth_field(const orc_world* w, const double* f, double* out, int ncomp);
 static int fill_weights(orc_world* w, double ntot);
+static int adapt(orc_world* w);
+static int adapt_fill(orc_world* w, const ugx_fill_desc* fl);
 
 /* ------------------------------------------------------------- geometry */
 
@@ -368,8 +376,10 @@ static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_d
                 double mps = sqrt(2.0 * KB * e->T / w->mass[s]);
                 double nrm[3] = {-sF[0] / fA, -sF[1] / fA, -sF[2] / fA};
                 double sCos = dot3(e->U, nrm) / mps;
-                e->nbar_num = fA * nd * w->model.delta_t * mps
-                              * (exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + erf(sCos)));
+                e->flux_an = fA * nd;
+                e->flux_mps = mps;
+                e->flux_br = exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + erf(sCos));
+                e->nbar_num = e->flux_an * w->model.delta_t * e->flux_mps * e->flux_br;
                 e->nbar_den = 2.0 * sqrtPi * w->model.n_equivalent_particles;
                 e->nbar = e->nbar_num / e->nbar_den;
                 int fb = m->face_offsets[f], fe = m->face_offsets[f + 1];
@@ -1191,6 +1201,18 @@ int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n
     w->cstart = xcalloc((size_t)w->cap, sizeof(int32_t));
     w->sclev = xcalloc((size_t)nc * 3, sizeof(int32_t));
     for (int c = 0; c < 3 * nc; ++c) w->sclev[c] = 1;
+    w->adacc = xcalloc((size_t)nc * KNR, sizeof(double));
+    w->adratio = xcalloc((size_t)nc * 3, sizeof(double));
+    if (w->model.adaptive) {
+        /* adaptiveProperties defaults (uniGasDynamicAdapter.C:198-216) */
+        ugx_model_desc* a = &w->model;
+        if (a->adaptation_interval == 0) a->adaptation_interval = 20;
+        if (a->adaptation_smoothing_passes == 0) a->adaptation_smoothing_passes = 25;
+        if (a->adaptation_smoothing_passes < 0) a->adaptation_smoothing_passes = 0;
+        if (a->max_time_step_mct_ratio == 0.0) a->max_time_step_mct_ratio = 0.2;
+        if (a->max_courant_number == 0.0) a->max_courant_number = 0.5;
+        if (a->max_subcell_size_mfp_ratio == 0.0) a->max_subcell_size_mfp_ratio = 0.5;
+    }
     int cm = (model->collision_model == UGX_COLL_DSMC) ? 1 : 0;  /* hybrid starts all-BGK (uniGasCloud.C:726-732) */
     for (int c = 0; c < nc; ++c) { w->cmodel[c] = cm; w->uspmax[c] = 1.0; }
     *out = w;
@@ -1203,7 +1225,8 @@ void orc_destroy(orc_world* w)
     void* ptrs[] = {w->cc, w->vol, w->bblo, w->bbhi, w->ncf, w->plane, w->nbr, w->delta, w->farea, w->fkind, w->bm, w->inflow,
                     w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
                     w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn,
-                    w->wface, w->wgeo, w->wsamp, w->ev, w->wcell, w->cw, w->cstart, w->sclev};
+                    w->wface, w->wgeo, w->wsamp, w->ev, w->wcell, w->cw, w->cstart, w->sclev,
+                    w->adacc, w->adratio};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(w);
 }
@@ -1252,6 +1275,200 @@ static void sort_by_cell(orc_world* w, int64_t n)
     w->n = cnt[nc];
 }
 
+/* The time step from now on, and the freestream counts per step that
+ * follow it (uniGasGeneralBoundary.C:154-165). */
+static void set_delta_t(orc_world* w, double dt)
+{
+    w->model.delta_t = dt;
+    for (int k = 0; k < w->nin; ++k) {
+        inflow_fs* e = &w->inflow[k];
+        e->nbar_num = e->flux_an * dt * e->flux_mps * e->flux_br;
+        e->nbar = e->nbar_num / e->nbar_den;
+    }
+}
+
+/* ---------------------------------------------------------- the adapter
+ * uniGasDynamicAdapter (uniGasDynamicAdapter.C) of an adaptive case. */
+
+/* calculateAdaptationQuantities (uniGasDynamicAdapter.C:224-314): per cell
+ * from the number densities nsp[], temperature T and velocity U, the time
+ * step over the mean collision time (Bird eq. 1.38 with 4.74), the Courant
+ * numbers max(c_mp, U_d) dt / L_d of the cell's box, and the box size over
+ * the mean free path (Bird eq. 4.76-4.77). */
+static void adapt_quantities(const orc_world* w, int c, double rhoN, double T, const double U[3], const double* nsp,
+                             double dt, const double* prev, double* mct, double co[3], double ratio[3])
+{
+    int ns = w->ns;
+    double Tref = w->model.Tref;
+    if (!(T > SMALL)) {
+        *mct = 0.0;
+        for (int d = 0; d < 3; ++d) { co[d] = 0.0; ratio[d] = prev[d]; }
+        return;
+    }
+    double u0 = 0.0, mcr[NS] = {0.0}, mfp[NS] = {0.0};
+    for (int i = 0; i < ns; ++i) {
+        for (int q = 0; q < ns; ++q) {
+            double dPQ = 0.5 * (w->diam[i] + w->diam[q]);
+            double omPQ = 0.5 * (w->omega[i] + w->omega[q]);
+            double massRatio = w->mass[i] / w->mass[q];
+            if (nsp[q] > VSMALL && T > VSMALL) {
+                double mr = w->mass[i] * w->mass[q] / (w->mass[i] + w->mass[q]);
+                mcr[i] = mcr[i] + 2.0 * sqrt(PI_D) * dPQ * dPQ * nsp[q] * pow(T / Tref, 1.0 - omPQ) * sqrt(2.0 * KB * Tref / mr);
+                mfp[i] = mfp[i] + PI_D * dPQ * dPQ * nsp[q] * pow(Tref / T, omPQ - 0.5) * sqrt(1.0 + massRatio);
+            }
+        }
+        if (nsp[i] > VSMALL) {
+            double c0 = sqrt(2.0 * KB / w->mass[i] * T);
+            if (c0 > u0) u0 = c0;
+        }
+    }
+    double MCR = 0.0, MFP = 0.0;
+    for (int i = 0; i < ns; ++i) {
+        if (!(nsp[i] > VSMALL)) continue;
+        double lam = 1.0 / mfp[i];
+        MCR = MCR + mcr[i] * nsp[i] / rhoN;
+        MFP = MFP + lam * nsp[i] / rhoN;
+    }
+    *mct = dt * MCR;
+    for (int d = 0; d < 3; ++d) {
+        double L = w->bbhi[3 * c + d] - w->bblo[3 * c + d];
+        double v = U[d] > u0 ? U[d] : u0;
+        co[d] = v * dt / L;
+        ratio[d] = L / MFP;
+    }
+}
+
+/* calculateTimeStep (uniGasDynamicAdapter.C:316-372): the largest ratio
+ * over the DSMC cells (none under the USP model) and the largest Courant
+ * number of the solution directions each scale the step to their limit. */
+static double adapt_time_step(const orc_world* w, const double* mct, const double* co, double dt)
+{
+    const ugx_model_desc* m = &w->model;
+    int usp = m->collision_model != UGX_COLL_DSMC && m->bgk_model == UGX_BGK_USP;
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < w->nc; ++c)
+        if (!usp && w->cmodel[c] != 0 && a < mct[c]) a = mct[c];
+    for (int c = 0; c < w->nc; ++c)
+        for (int d = 0; d < 3; ++d)
+            if (w->mask[d] != 0.0 && b < co[3 * c + d]) b = co[3 * c + d];
+    if (a > VSMALL && b < VSMALL) return dt * (m->max_time_step_mct_ratio / a);
+    if (a < VSMALL && b > VSMALL) return dt * (m->max_courant_number / b);
+    if (a > VSMALL && b > VSMALL) {
+        double x = m->max_time_step_mct_ratio / a, y = m->max_courant_number / b;
+        return dt * (y < x ? y : x);
+    }
+    return dt;
+}
+
+/* calculateSubCellLevels (uniGasDynamicAdapter.C:374-414): a DSMC cell
+ * takes ceil(ratio / maxSubCellSizeMFPRatio) clamped to 1..10 per solution
+ * direction, any other cell 1. */
+static void adapt_levels(orc_world* w, const double* ratio)
+{
+    double lim = w->model.max_subcell_size_mfp_ratio;
+    for (int c = 0; c < w->nc; ++c)
+        for (int d = 0; d < 3; ++d) {
+            int l = 1;
+            if (w->mask[d] != 0.0 && w->cmodel[c] == 1) {
+                double x = ceil(ratio[3 * c + d] / lim);
+                if (x < 1.0) x = 1.0;
+                if (x > (double)ORC_MAX_SUBCELL_LEVELS) x = (double)ORC_MAX_SUBCELL_LEVELS;
+                l = (int)x;
+            }
+            w->sclev[3 * c + d] = l;
+        }
+}
+
+/* One smoothing pass of an ncomp-component field in place. */
+static void smooth_inplace(const orc_world* w, double* f, int ncomp)
+{
+    double* t = malloc(sizeof(double) * (size_t)w->nc * (size_t)ncomp);
+    smooth_field(w, f, t, ncomp);
+    memcpy(f, t, sizeof(double) * (size_t)w->nc * (size_t)ncomp);
+    free(t);
+}
+
+/* setInitialConfiguration with the filled gas (uniGasDynamicAdapter.C:700-770):
+ * all three fields smoothed, then the time step and the subcell levels. */
+static int adapt_fill(orc_world* w, const ugx_fill_desc* fl)
+{
+    int nc = w->nc;
+    double rhoN = 0.0;
+    for (int s = 0; s < w->ns; ++s) rhoN = rhoN + fl->number_density[s];
+    double T = (fl->T[0] + fl->T[1] + fl->T[2]) / 3.0;
+    double* mct = calloc((size_t)nc, sizeof(double));
+    double* co = calloc((size_t)nc * 3, sizeof(double));
+    double zero[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nc; ++c)
+        adapt_quantities(w, c, rhoN, T, fl->velocity, fl->number_density, w->model.delta_t, zero, &mct[c], &co[3 * c],
+                         &w->adratio[3 * c]);
+    for (int pass = 0; pass < w->model.adaptation_smoothing_passes; ++pass) {
+        smooth_inplace(w, mct, 1);
+        smooth_inplace(w, co, 3);
+        smooth_inplace(w, w->adratio, 3);
+    }
+    if (w->model.time_step_adaptation) set_delta_t(w, adapt_time_step(w, mct, co, w->model.delta_t));
+    if (w->model.subcell_adaptation) adapt_levels(w, w->adratio);
+    free(mct);
+    free(co);
+    return UGX_OK;
+}
+
+/* adapt (uniGasDynamicAdapter.C:526-700) at the end of every
+ * adaptationInterval-th step: the fields averaged over the time since the
+ * last adaptation (an empty cell takes the smallest density, split evenly
+ * over the species, a cold one the lowest temperature), the size ratio
+ * smoothed, then the time step and subcell levels; the sums start again. */
+static int adapt(orc_world* w)
+{
+    int nc = w->nc, ns = w->ns;
+    double FN = w->model.n_equivalent_particles, tav = w->adapt_time, dt = w->model.delta_t;
+    if (!(tav > 0.0)) return fail(UGX_ERR_STATE, "adaptation without sampled time");
+    double* rhoN = calloc((size_t)nc, sizeof(double));
+    double* T = calloc((size_t)nc, sizeof(double));
+    double* U = calloc((size_t)nc * 3, sizeof(double));
+    double* mct = calloc((size_t)nc, sizeof(double));
+    double* co = calloc((size_t)nc * 3, sizeof(double));
+    double* ratio = calloc((size_t)nc * 3, sizeof(double));
+    double minRhoN = 1.0e300, minT = 1.0e300;
+    for (int c = 0; c < nc; ++c) {
+        const double* a = &w->adacc[KNR * c];
+        if (!(a[0] > VSMALL)) continue;
+        double FW = FN * w->cw[c], V = w->vol[c];
+        rhoN[c] = FW * a[0] / (V * tav);
+        double rhoM = FW * a[1] / (V * tav);
+        for (int d = 0; d < 3; ++d) U[3 * c + d] = FW * a[2 + d] / (rhoM * V * tav);
+        double ke = 0.5 * (FW * a[5]) / (V * tav);
+        double rn = FW * a[0] / (V * tav);
+        double uu = U[3 * c] * U[3 * c] + U[3 * c + 1] * U[3 * c + 1] + U[3 * c + 2] * U[3 * c + 2];
+        T[c] = 2.0 / (3.0 * KB * rn) * (ke - 0.5 * rhoM * uu);
+        if (T[c] > VSMALL) {
+            if (minRhoN > rhoN[c]) minRhoN = rhoN[c];
+            if (minT > T[c]) minT = T[c];
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        const double* a = &w->adacc[KNR * c];
+        double FW = FN * w->cw[c], V = w->vol[c];
+        double nsp[NS] = {0.0};
+        for (int s = 0; s < ns; ++s) nsp[s] = FW * a[8 + s] / (V * tav);
+        if (rhoN[c] == 0.0) {
+            rhoN[c] = minRhoN;
+            for (int s = 0; s < ns; ++s) nsp[s] = rhoN[c] / (double)ns;
+        }
+        if (T[c] == 0.0) T[c] = minT;
+        adapt_quantities(w, c, rhoN[c], T[c], &U[3 * c], nsp, dt, &w->adratio[3 * c], &mct[c], &co[3 * c], &ratio[3 * c]);
+    }
+    for (int pass = 0; pass < w->model.adaptation_smoothing_passes; ++pass) smooth_inplace(w, ratio, 3);
+    if (w->model.time_step_adaptation) set_delta_t(w, adapt_time_step(w, mct, co, dt));
+    if (w->model.subcell_adaptation) adapt_levels(w, ratio);
+    memcpy(w->adratio, ratio, sizeof(double) * (size_t)nc * 3);
+    memset(w->adacc, 0, sizeof(double) * (size_t)nc * KNR);
+    w->adapt_time = 0.0;
+    free(rhoN); free(T); free(U); free(mct); free(co); free(ratio);
+    return UGX_OK;
+}
+
 /* uniGasMeshFill::setInitialConfiguration (uniGasMeshFill.C:58-297): per
  * cell and species a stochastically rounded count (whole cell rather than
  * per tet), positions uniform in the cell (rejection in its bounding box),
@@ -1264,6 +1481,11 @@ int orc_fill(orc_world* w, const ugx_fill_desc* fl)
     double ntot = 0.0;
     int best = 0;
     for (int s = 0; s < ns; ++s) { ntot += fl->number_density[s]; if (fl->number_density[s] > fl->number_density[best]) best = s; }
+    if (w->model.adaptive) {
+        /* the time step and subcell levels from the filled gas (uniGasMeshFill.C:100-109) */
+        int rc = adapt_fill(w, fl);
+        if (rc) return rc;
+    }
     if (w->model.cell_weighted) {
         if (fl->exact_per_cell > 0) return fail(UGX_ERR_ARG, "a cell-weighted case is filled by number density, not per cell");
         int rc = fill_weights(w, ntot);
@@ -1396,7 +1618,11 @@ static int fill_weights(orc_world* w, double ntot)
     if (!(ntot > 0.0)) return fail(UGX_ERR_ARG, "a cell-weighted fill needs a number density > 0");
     int nc = w->nc;
     double* tmp = malloc(sizeof(double) * (size_t)nc);
-    for (int c = 0; c < nc; ++c) w->cw[c] = (ntot * w->vol[c]) / ((double)npps * 1.0 * FN * 1.0);
+    /* nSubCells of each cell (uniGasMeshFill.C:118-120) */
+    double* nsub = malloc(sizeof(double) * (size_t)nc);
+    for (int c = 0; c < nc; ++c)
+        nsub[c] = (double)w->sclev[3 * c] * (double)w->sclev[3 * c + 1] * (double)w->sclev[3 * c + 2];
+    for (int c = 0; c < nc; ++c) w->cw[c] = (ntot * w->vol[c]) / ((double)npps * nsub[c] * FN * 1.0);
     int passes = 0;
     double max_r;
     do {
@@ -1404,7 +1630,7 @@ static int fill_weights(orc_world* w, double ntot)
         smooth_field(w, w->cw, tmp, 1);
         memcpy(w->cw, tmp, sizeof(double) * (size_t)nc);
         for (int c = 0; c < nc; ++c) {
-            double lim = (ntot * w->vol[c]) / ((double)nmin * 1.0 * FN * 1.0);
+            double lim = (ntot * w->vol[c]) / ((double)nmin * nsub[c] * FN * 1.0);
             double v = w->cw[c] < lim ? w->cw[c] : lim;
             w->cw[c] = v > SMALL ? v : SMALL;
         }
@@ -1420,6 +1646,7 @@ static int fill_weights(orc_world* w, double ntot)
             }
     } while (max_r > 1.0 + max_ratio && passes < max_passes);
     free(tmp);
+    free(nsub);
     return UGX_OK;
 }
 
@@ -1487,6 +1714,26 @@ int orc_set_cell_weights(orc_world* w, const double* in, int64_t len)
     memcpy(w->cw, in, sizeof(double) * (size_t)w->nc);
     return UGX_OK;
 }
+
+/* Test hook: the adapter's time sums and summed time set directly, then
+ * one adaptation. */
+int orc_adapt_inject(orc_world* w, const double* acc, double tav)
+{
+    memcpy(w->adacc, acc, sizeof(double) * (size_t)w->nc * KNR);
+    w->adapt_time = tav;
+    return adapt(w);
+}
+
+int orc_set_cell_model(orc_world* w, const int32_t* in, int64_t len)
+{
+    if (len < w->nc) return fail(UGX_ERR_ARG, "buffer too small");
+    memcpy(w->cmodel, in, sizeof(int32_t) * (size_t)w->nc);
+    return UGX_OK;
+}
+
+double orc_delta_t(orc_world* w) { return w->model.delta_t; }
+
+double orc_time(orc_world* w) { return w->time; }
 
 int orc_subcell_levels(orc_world* w, int32_t* out, int64_t len)
 {
@@ -1588,6 +1835,11 @@ int orc_step_phases(orc_world* w, int mask)
                 for (int k = 0; k < 8; ++k) w->knacc[KNR * c + k] += dt * S[k];
                 for (int k = 0; k < NS; ++k) w->knacc[KNR * c + 8 + k] += dt * S[UGX_S_SPECIES_N + k];
             }
+            if (w->model.adaptive) {
+                /* the adapter's cell measurements (uniGasDynamicAdapter.C:526-547) */
+                for (int k = 0; k < 8; ++k) w->adacc[KNR * c + k] += dt * S[k];
+                for (int k = 0; k < NS; ++k) w->adacc[KNR * c + 8 + k] += dt * S[UGX_S_SPECIES_N + k];
+            }
             if ((cm == UGX_COLL_DSMC || hybrid) && w->cmodel[c] == 1 && w->model.dsmc_model != UGX_DSMC_NONE)
                 ntc_cell(w, c, b, nC, &cand, &coll);
             if ((cm == UGX_COLL_BGK || hybrid) && w->model.bgk_model != UGX_BGK_NONE)
@@ -1603,7 +1855,15 @@ int orc_step_phases(orc_world* w, int mask)
     if ((mask & 8) && w->model.collision_model == UGX_COLL_HYBRID
         && ((w->step + 1) % (uint32_t)w->model.decomposition_interval) == 0)
         hybrid_switch(w);
-    if (mask & 16) { w->step += 1; w->cnt.steps += 1; w->cnt.particle_steps += w->n; }
+    if (mask & 16) {
+        w->step += 1; w->cnt.steps += 1; w->cnt.particle_steps += w->n;
+        if (w->model.adaptive) w->adapt_time = w->adapt_time + w->model.delta_t;
+        w->time = w->time + w->model.delta_t;
+        if (w->model.adaptive && (w->step % (uint32_t)w->model.adaptation_interval) == 0u) {
+            int rc = adapt(w);
+            if (rc) return rc;
+        }
+    }
     return UGX_OK;
 }
 
@@ -1813,7 +2073,7 @@ static void hybrid_switch(orc_world* w)
     for (int c = 0; c < nc; ++c) w->cmodel[c] = (w->kn[c] > bmax) ? 1 : 0;
     refine_models(w);
     /* reset (localKnudsen.C:520-542): time = (step + 1) deltaT */
-    double time = (double)(w->step + 1) * w->model.delta_t;
+    double time = w->time + w->model.delta_t;
     double until = (w->model.reset_until_time > 0.0) ? w->model.reset_until_time : 1.0e300;
     if (w->model.reset_at_decomposition && time < until + 0.5 * w->model.delta_t) {
         memset(w->knacc, 0, sizeof(double) * KNR * (size_t)nc);

@@ -87,6 +87,13 @@ int orc_set_cell_weights(orc_world* w, const double* in, int64_t len);
 /* NTC subcell levels per cell and axis, [nc*3] (noTimeCounter.C:97-161) */
 int orc_subcell_levels(orc_world* w, int32_t* out, int64_t len);
 int orc_set_subcell_levels(orc_world* w, const int32_t* in, int64_t len);
+/* the current time step (adaptive cases change it) and the simulated time */
+double orc_delta_t(orc_world* w);
+double orc_time(orc_world* w);
+/* test hooks: adapter sums [nc*UGX_KN_ROWS] and time, then one adaptation;
+ * cell model ids */
+int orc_adapt_inject(orc_world* w, const double* acc, double tav);
+int orc_set_cell_model(orc_world* w, const int32_t* in, int64_t len);
 int orc_wall_samples(orc_world* w, double* out, int64_t len);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 

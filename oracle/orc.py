@@ -96,6 +96,12 @@ def _bind(L):
     L.orc_set_cell_weights.argtypes = [vp, abi.DBL_P, C.c_int64]
     L.orc_subcell_levels.argtypes = [vp, abi.I32_P, C.c_int64]
     L.orc_set_subcell_levels.argtypes = [vp, abi.I32_P, C.c_int64]
+    L.orc_delta_t.argtypes = [vp]
+    L.orc_delta_t.restype = C.c_double
+    L.orc_time.argtypes = [vp]
+    L.orc_time.restype = C.c_double
+    L.orc_adapt_inject.argtypes = [vp, abi.DBL_P, C.c_double]
+    L.orc_set_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
 
 
 def _check(rc):
@@ -178,6 +184,21 @@ class OracleCase:
     def set_cell_weights(self, w):
         w = np.ascontiguousarray(w, dtype=np.float64)
         _check(self.L.orc_set_cell_weights(self.h, abi.dptr(w), self.nc))
+
+    def delta_t(self):
+        return float(self.L.orc_delta_t(self.h))
+
+    def time(self):
+        return float(self.L.orc_time(self.h))
+
+    def adapt_inject(self, acc, tav):
+        """Sets the adapter's time sums [n_cells, 16] and their time, then adapts once."""
+        acc = np.ascontiguousarray(acc, dtype=np.float64).reshape(self.nc, abi.KN_ROWS)
+        _check(self.L.orc_adapt_inject(self.h, abi.dptr(acc), float(tav)))
+
+    def set_cell_model(self, ids):
+        ids = np.ascontiguousarray(np.broadcast_to(np.asarray(ids, dtype=np.int32), (self.nc,)))
+        _check(self.L.orc_set_cell_model(self.h, abi.iptr(ids), self.nc))
 
     def subcell_levels(self):
         out = np.zeros((self.nc, 3), dtype=np.int32)

@@ -16,6 +16,7 @@ import subprocess
 import pytest
 
 import test_oracle_physics as phys
+import test_oracle_adapt as adapt
 import test_oracle_pins as pins
 import test_oracle_subcells as subcells
 import test_oracle_wall as wall
@@ -27,6 +28,37 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fas
 
 # name: (snippet, replacement, pins that must fail)
 MUTANTS = {
+    # collision rate with the exponent omega for 1 - omega (uniGasDynamicAdapter.C:262-263, Bird eq. 4.74)
+    "adapt_collision_rate_exponent": (
+        "pow(T / Tref, 1.0 - omPQ)",
+        "pow(T / Tref, omPQ)",
+        [adapt.test_fill_mct_limited_step_and_levels, adapt.test_adaptation_from_time_sums]),
+    # mean free path without the mass-ratio factor (:266, Bird eq. 4.76)
+    "adapt_mfp_mass_ratio": (
+        "sqrt(1.0 + massRatio)",
+        "sqrt(massRatio)",
+        [adapt.test_fill_mct_limited_step_and_levels, adapt.test_empty_direction_counts_for_nothing]),
+    # Courant number from the most probable speed alone (:297-299)
+    "adapt_courant_ignores_drift": (
+        "double v = U[d] > u0 ? U[d] : u0;",
+        "double v = u0;",
+        [adapt.test_fill_courant_limited_step_with_drift]),
+    # empty directions counted in the largest Courant number (:344-350)
+    "adapt_courant_empty_direction": (
+        "if (w->mask[d] != 0.0 && b < co[3 * c + d]) b = co[3 * c + d];",
+        "if (b < co[3 * c + d]) b = co[3 * c + d];",
+        [adapt.test_empty_direction_counts_for_nothing]),
+    # the collision-time limit taken over BGK-family cells too (:326-334)
+    "adapt_mct_all_cells": (
+        "if (!usp && w->cmodel[c] != 0 && a < mct[c]) a = mct[c];",
+        "if (a < mct[c]) a = mct[c];",
+        [lambda: adapt.test_bgk_cells_count_only_for_courant("stochasticParticleBGK"),
+         adapt.test_hybrid_dsmc_cells_set_step_and_levels_usp_cells_do_not]),
+    # an empty cell keeps zero density instead of the smallest (:597-604)
+    "adapt_empty_cell_density": (
+        "            rhoN[c] = minRhoN;\n",
+        "            rhoN[c] = rhoN[c];\n",
+        [adapt.test_adaptation_from_time_sums]),
     # NTC second candidate always from the whole cell (noTimeCounter.C:193-225)
     "ntc_subcells_ignored": (
         "        if (nSC > 1) {\n            do { Q = (uint32_t)list[",
@@ -137,8 +169,8 @@ MUTANTS = {
         [phys.test_diffuse_walls_thermalise_gas]),
     # inflow count without the exp(-s^2) term (uniGasGeneralBoundary.C:140-157)
     "inflow_exp_term": (
-        "* (exp(-(sCos * sCos)) + sqrtPi",
-        "* (0.0 + sqrtPi",
+        "e->flux_br = exp(-(sCos * sCos)) + sqrtPi",
+        "e->flux_br = 0.0 + sqrtPi",
         [phys.test_inflow_count_bird_4_22]),
     # wall estimator without the face area (uniGasPatchBoundary.C:155)
     "wall_weight_without_area": (
@@ -204,8 +236,8 @@ MUTANTS = {
         [weighting.test_fill_weights_uniform_mesh_exact, weighting.test_fill_weights_graded_mesh_smoothed_and_capped]),
     # smoothing capped at N_pps instead of minParticlesPerSubCell (uniGasMeshFill.C:141-142)
     "weight_cap": (
-        "double lim = (ntot * w->vol[c]) / ((double)nmin * 1.0 * FN * 1.0);",
-        "double lim = (ntot * w->vol[c]) / ((double)npps * 1.0 * FN * 1.0);",
+        "double lim = (ntot * w->vol[c]) / ((double)nmin * nsub[c] * FN * 1.0);",
+        "double lim = (ntot * w->vol[c]) / ((double)npps * nsub[c] * FN * 1.0);",
         [weighting.test_fill_weights_graded_mesh_smoothed_and_capped]),
 }
 

@@ -68,6 +68,11 @@ class ModelDesc(C.Structure):
         ("cell_weighted", C.c_int32), ("particles_per_subcell", C.c_int32),
         ("min_particles_per_subcell", C.c_int32), ("max_smoothing_passes", C.c_int32),
         ("max_cell_weight_ratio", C.c_double),
+        ("adaptive", C.c_int32), ("time_step_adaptation", C.c_int32),
+        ("subcell_adaptation", C.c_int32), ("cell_weight_adaptation", C.c_int32),
+        ("adaptation_interval", C.c_int32), ("adaptation_smoothing_passes", C.c_int32),
+        ("max_time_step_mct_ratio", C.c_double), ("max_courant_number", C.c_double),
+        ("max_subcell_size_mfp_ratio", C.c_double),
     ]
 
 
@@ -129,6 +134,9 @@ class CaseState(C.Structure):
         ("sigma_max", DBL_P), ("cell_model", I32_P), ("samples", DBL_P), ("usp_heat_flux", DBL_P),
         ("usp_stress", DBL_P), ("usp_max_prob", DBL_P), ("kn_sums", DBL_P), ("kn", DBL_P),
         ("n_wall_faces", C.c_int64), ("wall_samples", DBL_P), ("cell_weight", DBL_P),
+        ("delta_t", C.c_double), ("time", C.c_double), ("time_sampled", C.c_double),
+        ("adapt_time", C.c_double), ("subcell_levels", I32_P), ("adapt_sums", DBL_P),
+        ("adapt_ratio", DBL_P),
     ]
 
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "ugx_cell_volumes", "ugx_cell_centres", "ugx_geometry", "ugx_state_download", "ugx_state_upload",
     "ugx_wall_face_count", "ugx_wall_faces", "ugx_wall_samples_download", "ugx_wall_fields_compute",
     "ugx_cell_weights_download", "ugx_cell_weights_upload",
-    "ugx_subcell_levels_download", "ugx_subcell_levels_upload",
+    "ugx_subcell_levels_download", "ugx_subcell_levels_upload", "ugx_time_get",
     "ugx_phase_name", "ugx_timing_enable", "ugx_timing_get", "ugx_tile_config", "ugx_probe_copy",
     "ugx_batch_create", "ugx_batch_destroy", "ugx_case_footprint", "ugx_batch_admit", "ugx_batch_retire",
     "ugx_batch_step", "ugx_batch_sync", "ugx_batch_case", "ugx_batch_info",
@@ -74,6 +74,7 @@ def lib():
         L.ugx_cell_weights_upload.argtypes = [vp, abi.DBL_P, C.c_int64]
         L.ugx_subcell_levels_download.argtypes = [vp, abi.I32_P, C.c_int64]
         L.ugx_subcell_levels_upload.argtypes = [vp, abi.I32_P, C.c_int64]
+        L.ugx_time_get.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ugx_wall_face_count.argtypes = [vp, C.POINTER(C.c_int64)]
         L.ugx_wall_faces.argtypes = [vp, abi.I32_P, abi.I32_P, C.c_int64]
         L.ugx_wall_samples_download.argtypes = [vp, abi.DBL_P, C.c_int64]
@@ -269,6 +270,17 @@ class Case:
         w = np.ascontiguousarray(w, dtype=np.float64)
         check(self.L.ugx_cell_weights_upload(self.h, abi.dptr(w), self.nc), self.L)
 
+    def delta_t(self):
+        """The current time step (an adaptive case changes it)."""
+        dt, t = C.c_double(), C.c_double()
+        check(self.L.ugx_time_get(self.h, C.byref(dt), C.byref(t)), self.L)
+        return dt.value
+
+    def time(self):
+        dt, t = C.c_double(), C.c_double()
+        check(self.L.ugx_time_get(self.h, C.byref(dt), C.byref(t)), self.L)
+        return t.value
+
     def subcell_levels(self):
         """NTC subcell levels per cell and axis, [n_cells, 3]."""
         out = np.zeros((self.nc, 3), dtype=np.int32)
@@ -339,6 +351,12 @@ class Case:
             s.wall_samples = abi.dptr(w)
         if arrs.get("cell_weight") is not None:
             s.cell_weight = abi.dptr(arrs["cell_weight"])
+        if arrs.get("subcell_levels") is not None:
+            s.subcell_levels = abi.iptr(arrs["subcell_levels"])
+        if arrs.get("adapt_sums") is not None:
+            s.adapt_sums = abi.dptr(arrs["adapt_sums"])
+        if arrs.get("adapt_ratio") is not None:
+            s.adapt_ratio = abi.dptr(arrs["adapt_ratio"])
         return s
 
     def state(self):
@@ -348,11 +366,16 @@ class Case:
         arrs = {name: np.zeros((self.nc, rows), dtype=dt) for name, rows, dt in abi.STATE_ARRAYS}
         arrs["wall_samples"] = np.zeros((self.wall_face_count(), abi.MAX_SPECIES, abi.W_COUNT))
         arrs["cell_weight"] = np.zeros(self.nc)
+        arrs["subcell_levels"] = np.zeros((self.nc, 3), dtype=np.int32)
+        arrs["adapt_sums"] = np.zeros((self.nc, abi.KN_ROWS))
+        arrs["adapt_ratio"] = np.zeros((self.nc, 3))
         s = self._state_struct(arrs)
         check(self.L.ugx_state_download(self.h, C.byref(s)), self.L)
         arrs.update(steps=np.int64(s.steps), kn_time=np.float64(s.kn_time),
                     sampled_at_reset=np.int64(s.sampled_at_reset),
-                    counters=np.array(s.counters[:], dtype=np.uint64))
+                    counters=np.array(s.counters[:], dtype=np.uint64),
+                    delta_t=np.float64(s.delta_t), time=np.float64(s.time),
+                    time_sampled=np.float64(s.time_sampled), adapt_time=np.float64(s.adapt_time))
         return arrs
 
     def restore_state(self, st):
@@ -364,10 +387,21 @@ class Case:
         if st.get("wall_samples") is not None:
             arrs["wall_samples"] = np.ascontiguousarray(np.asarray(st["wall_samples"], dtype=np.float64).reshape(
                 -1, abi.MAX_SPECIES, abi.W_COUNT))
+        for name, rows, dt in (("subcell_levels", 3, np.int32), ("adapt_sums", abi.KN_ROWS, np.float64),
+                               ("adapt_ratio", 3, np.float64)):
+            if st.get(name) is not None:
+                arrs[name] = np.ascontiguousarray(np.asarray(st[name], dtype=dt).reshape(self.nc, rows))
         s = self._state_struct(arrs)
         s.steps = int(st["steps"])
         s.kn_time = float(st["kn_time"])
         s.sampled_at_reset = int(st["sampled_at_reset"])
+        # a state without the time fields (an older checkpoint) keeps the
+        # case's time step, and its time follows from the step count
+        s.delta_t = float(st.get("delta_t", 0.0))
+        dt = s.delta_t if s.delta_t > 0.0 else self.spec.delta_t
+        s.time = float(st.get("time", int(st["steps"]) * dt))
+        s.time_sampled = float(st["time_sampled"]) if "time_sampled" in st else float("nan")
+        s.adapt_time = float(st.get("adapt_time", 0.0))
         cnt = np.asarray(st["counters"], dtype=np.uint64)
         for k in range(abi.N_COUNTERS):
             s.counters[k] = int(cnt[k])

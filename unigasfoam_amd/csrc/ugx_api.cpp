@@ -31,6 +31,15 @@ static int validate_model(const ugx_model_desc* m)
         return set_err(UGX_ERR_ARG, "rotationalRelaxationCollisionNumber must be > 0");
     if (m->cell_weighted && m->particles_per_subcell <= 0)
         return set_err(UGX_ERR_ARG, "cellWeightedProperties.particlesPerSubCell must be > 0");
+    if (m->adaptive) {
+        if (m->adaptation_interval < 0) return set_err(UGX_ERR_ARG, "adaptiveProperties.adaptationInterval must be > 0");
+        if (m->max_time_step_mct_ratio < 0.0 || m->max_courant_number < 0.0 || m->max_subcell_size_mfp_ratio < 0.0)
+            return set_err(UGX_ERR_ARG, "adaptiveProperties ratios must be > 0");
+        if (m->cell_weight_adaptation && !m->cell_weighted)
+            return set_err(UGX_ERR_ARG, "cellWeightAdaptation needs cellWeightedSimulation");
+        if (m->cell_weight_adaptation)
+            return set_err(UGX_ERR_UNSUPPORTED, "adaptiveProperties.cellWeightAdaptation is not supported");
+    }
     return UGX_OK;
 }
 
@@ -52,6 +61,16 @@ int ugx::make_spec(const ugx_mesh_desc* mesh, const ugx_species_desc* species, i
     spec.model = *model;
     if (spec.model.sample_interval < 1) spec.model.sample_interval = 1;
     if (spec.model.theta == 0.0 && spec.model.bgk_model == UGX_BGK_USP) spec.model.theta = 1.0;
+    if (spec.model.adaptive) {
+        // adaptiveProperties defaults (uniGasDynamicAdapter.C:198-216)
+        ugx_model_desc& a = spec.model;
+        if (a.adaptation_interval == 0) a.adaptation_interval = 20;
+        if (a.adaptation_smoothing_passes == 0) a.adaptation_smoothing_passes = 25;
+        if (a.adaptation_smoothing_passes < 0) a.adaptation_smoothing_passes = 0;
+        if (a.max_time_step_mct_ratio == 0.0) a.max_time_step_mct_ratio = 0.2;
+        if (a.max_courant_number == 0.0) a.max_courant_number = 0.5;
+        if (a.max_subcell_size_mfp_ratio == 0.0) a.max_subcell_size_mfp_ratio = 0.5;
+    }
     spec.seed = seed;
     std::vector<ugx_boundary_desc> b(bnd, bnd + n_bnd);
     for (const auto& x : b)
@@ -344,6 +363,17 @@ int ugx_cell_weights_upload(ugx_case* c, const double* in, int64_t len)
     int b, n;
     if (int rc = cell_range(c, len, 1, b, n)) return rc;
     return c->world->set_cell_weights(c->index, in);
+    UGX_GUARD_END
+}
+
+int ugx_time_get(ugx_case* c, double* delta_t, double* time)
+{
+    UGX_GUARD_BEGIN
+    CHECK_CASE(c);
+    const CaseDev& cd = c->world->cases[c->index];
+    if (delta_t) *delta_t = cd.dt;
+    if (time) *time = cd.time;
+    return UGX_OK;
     UGX_GUARD_END
 }
 

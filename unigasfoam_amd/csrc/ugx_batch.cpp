@@ -171,6 +171,8 @@ int ugx_case_footprint(const ugx_mesh_desc* mesh, const ugx_model_desc* model, i
     *bytes = footprint(mesh->n_cells, precision, particles, bgk, hybrid);
     // cell weighting: start cell, clone count and offset per simulator slot, weight per cell
     if (model->cell_weighted) *bytes += (int64_t)(12.0 * 1.5 * (double)particles) + 8 * (int64_t)mesh->n_cells;
+    // adaptive simulation: the adapter's time sums per cell; subcell ids and lists per simulator slot
+    if (model->adaptive) *bytes += 8 * UGX_KN_ROWS * (int64_t)mesh->n_cells + (int64_t)(8.0 * 1.5 * (double)particles);
     return UGX_OK;
 }
 

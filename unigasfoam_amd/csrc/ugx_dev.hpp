@@ -293,6 +293,9 @@ struct CaseDev {
     int32_t reset_at_decomposition;
     int32_t weighted;            // cell-weighted case (clones and deletions on weight changes)
     double reset_until_time;     // <= 0: no limit
+    double time;                 // simulated time at the start of the step
+    int32_t adaptive;            // adaptive simulation: the adapter's time sums are kept every step
+    int32_t pad_adapt;
 };
 
 // Hybrid decomposition: time-sum rows per cell (N, M, M u, M v, M w, M |u|^2,

@@ -1370,12 +1370,13 @@ __device__ inline void cell_partials(const WorldView<R>& w, const Acc& A, uint32
 struct TileCase {
     double dt;
     uint32_t step;
-    int32_t sample_interval, coll_model, bgk_model, active, ns, sampler_full;
+    int32_t sample_interval, coll_model, bgk_model, active, ns, sampler_full, adaptive;
 };
 
 __device__ inline TileCase tile_case_of(const CaseDev& c)
 {
-    return TileCase{c.dt, c.step, c.sample_interval, c.coll_model, c.bgk_model, c.active, c.ns, c.sampler_full};
+    return TileCase{c.dt, c.step, c.sample_interval, c.coll_model, c.bgk_model, c.active, c.ns, c.sampler_full,
+                    c.adaptive};
 }
 
 // Sums of the cells of one tile, eight cells per wave (group g of lanes
@@ -1451,7 +1452,7 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
     for (uint32_t kb = 0; kb < nct; kb += (Block / 64) * 8) {
         const uint32_t k = kb + wv * 8u + g;
         uint32_t b = 0, e = 0;
-        bool sample = false, hybrid = false, bgk = false;
+        bool sample = false, hybrid = false, bgk = false, adapt = false;
         double dt = 0.0;
         const uint32_t c = c0 + k;
         TileCase cs{};
@@ -1462,11 +1463,12 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
                 sample = ((cs.step + 1u) % (uint32_t)cs.sample_interval) == 0u;
                 hybrid = cs.coll_model == UGX_COLL_HYBRID;
                 bgk = (cs.coll_model == UGX_COLL_BGK || hybrid) && cs.bgk_model != UGX_BGK_NONE;
-                if (sample || hybrid || bgk) { b = s_off[k]; e = s_off[k + 1]; }
+                adapt = cs.adaptive != 0;
+                if (sample || hybrid || bgk || adapt) { b = s_off[k]; e = s_off[k + 1]; }
                 dt = cs.dt;
             }
         }
-        const bool need = sample || hybrid || bgk;
+        const bool need = sample || hybrid || bgk || adapt;
         if (__ballot(need) == 0ull) continue;
         double S[NR];
 #pragma unroll
@@ -1502,6 +1504,13 @@ __device__ inline void tile_sums(const WorldView<R>& w, uint32_t c0, uint32_t nc
             double* ka = &w.knacc[(size_t)kKnRows * c];
             ka[j] += dt * v[0];
             if (MS || j == 0u) ka[8 + j] += dt * vsp;
+        }
+        if (need && adapt) {
+            // uniGasDynamicAdapter::adapt (uniGasDynamicAdapter.C:526-547): the
+            // cell measurements of every step, weighted by the time step
+            double* aa = &w.adacc[(size_t)kKnRows * c];
+            aa[j] += dt * v[0];
+            if (MS || j == 0u) aa[8 + j] += dt * vsp;
         }
         if (need && bgk) {
             double* mo = &w.mom[(size_t)c * kNMom];
@@ -2689,7 +2698,7 @@ __global__ void __launch_bounds__(kBlock) k_kn_refine(WorldView<R> w, int me)
 
 __device__ inline bool kn_reset_now(const CaseDev& cs)
 {
-    const double time = (double)(cs.step + 1u) * cs.dt;
+    const double time = cs.time + cs.dt;
     const double until = (cs.reset_until_time > 0.0) ? cs.reset_until_time : 1.0e300;
     return cs.reset_at_decomposition && time < until + 0.5 * cs.dt;
 }
@@ -2727,6 +2736,7 @@ __global__ void k_end_step(WorldView<R> w)
             w.counters[(size_t)c * kCntNum + kCntParticleSteps] +=
                 (unsigned long long)(w.off[cs.cell_end] - w.off[cs.cell_begin]);
             cs.step += 1u;
+            cs.time = cs.time + cs.dt;
         }
     }
     if (c == 0) w.st->n = w.st->n_new;

@@ -158,6 +158,9 @@ struct WorldView {
     uint32_t* sclist;         // [cap]
     const double* bblo;       // [nc*3] cell bounding box in the cell frame
     const double* bbhi;       // [nc*3]
+    // adaptive simulation: per cell the adapter's time sums since its last
+    // adaptation (rows as the hybrid time sums)
+    double* adacc;            // [nc*kKnRows]
 };
 
 // Largest subcell level along an axis (uniGasDynamicAdapter.C:49).

@@ -1,10 +1,9 @@
 // ugx_weights.cpp - cell weights of a cell-weighted case's fill
 // (src/lagrangian/uniGas/uniGasInitialisation/derived/uniGasMeshFill/
-// uniGasMeshFill.C:111-168), without subcells or radial weighting (one subcell
-// per cell, RWF = 1):
-//   W_c = n V_c / (N_pps F_N);
+// uniGasMeshFill.C:111-168), without radial weighting (RWF = 1):
+//   W_c = n V_c / (N_pps N_sub,c F_N)  (N_sub,c the cell's subcell count);
 //   repeat: W = fvc::average(fvc::interpolate(W)) (zeroGradient boundaries),
-//           W_c = max(min(n V_c / (N_min F_N), W_c), SMALL),
+//           W_c = max(min(n V_c / (N_min N_sub,c F_N), W_c), SMALL),
 //   while some internal face has max(W_o/W_n, W_n/W_o) > 1 + maxCellWeightRatio
 //   and fewer than maxSmoothingPasses passes were made.
 // The smoothing pass is the face interpolation and area average of the
@@ -45,7 +44,7 @@ static void smooth_pass(const MeshGeom& g, const std::vector<double>& f, std::ve
     }
 }
 
-int fill_cell_weights(const CaseSpec& spec, double ntot, std::vector<double>& w)
+int fill_cell_weights(const CaseSpec& spec, double ntot, const uint32_t* lv, std::vector<double>& w)
 {
     const MeshGeom& g = spec.geom;
     const ugx_model_desc& m = spec.model;
@@ -57,7 +56,11 @@ int fill_cell_weights(const CaseSpec& spec, double ntot, std::vector<double>& w)
     if (npps <= 0) return set_err(UGX_ERR_ARG, "a cell-weighted case needs particles_per_subcell > 0");
     if (!(ntot > 0.0)) return set_err(UGX_ERR_ARG, "a cell-weighted fill needs a number density > 0");
     w.assign((size_t)g.nc, 1.0);
-    for (int c = 0; c < g.nc; ++c) w[c] = (ntot * g.vol[c]) / ((double)npps * 1.0 * FN * 1.0);
+    std::vector<double> nsub((size_t)g.nc, 1.0);
+    if (lv)
+        for (int c = 0; c < g.nc; ++c)
+            nsub[c] = (double)(lv[c] & 0xffu) * (double)((lv[c] >> 8) & 0xffu) * (double)((lv[c] >> 16) & 0xffu);
+    for (int c = 0; c < g.nc; ++c) w[c] = (ntot * g.vol[c]) / ((double)npps * nsub[c] * FN * 1.0);
     std::vector<double> tmp((size_t)g.nc);
     int passes = 0;
     double max_r;
@@ -66,7 +69,7 @@ int fill_cell_weights(const CaseSpec& spec, double ntot, std::vector<double>& w)
         smooth_pass(g, w, tmp);
         w.swap(tmp);
         for (int c = 0; c < g.nc; ++c)
-            w[c] = std::max(std::min((ntot * g.vol[c]) / ((double)nmin * 1.0 * FN * 1.0), w[c]), kSmall);
+            w[c] = std::max(std::min((ntot * g.vol[c]) / ((double)nmin * nsub[c] * FN * 1.0), w[c]), kSmall);
         max_r = 1.0e-300;
         for (int c = 0; c < g.nc; ++c)
             for (int j = 0; j < kMaxFaces; ++j) {

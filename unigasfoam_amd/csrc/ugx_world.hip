@@ -244,6 +244,7 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
     }
     // freestream inflow face-species (uniGasGeneralBoundary.C:115-169)
     g.inflow.clear();
+    g.inflow_parts.clear();
     const double sqrtPi = std::sqrt(kPi);
     for (size_t bi = 0; bi < bnd.size(); ++bi) {
         const auto& b = bnd[bi];
@@ -264,9 +265,13 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
                 double mps = std::sqrt(2.0 * kB * e.T / mass[s]);
                 double nrm[3] = {-sF[0] / fA, -sF[1] / fA, -sF[2] / fA};
                 double sCos = dot3(e.U, nrm) / mps;
-                e.nbar_num = fA * nd * model.delta_t * mps
-                             * (std::exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + std::erf(sCos)));
+                const double fAn = fA * nd;
+                const double br = std::exp(-(sCos * sCos)) + sqrtPi * sCos * (1.0 + std::erf(sCos));
+                e.nbar_num = fAn * model.delta_t * mps * br;
                 e.nbar_den = 2.0 * sqrtPi * model.n_equivalent_particles;
+                g.inflow_parts.push_back(fAn);
+                g.inflow_parts.push_back(mps);
+                g.inflow_parts.push_back(br);
                 e.nbar = e.nbar_num / e.nbar_den;
                 int fb = m->face_offsets[f], fe = m->face_offsets[f + 1];
                 e.nv = fe - fb;
@@ -356,7 +361,8 @@ void World::release()
                      (void**)&d_wface, (void**)&d_wgeo, (void**)&d_wcell, (void**)&d_ev, (void**)&d_ev_key,
                      (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp,
                      (void**)&d_cw, (void**)&d_cstart, (void**)&d_wclone, (void**)&d_wcoff, (void**)&d_wbsum,
-                     (void**)&d_sclev, (void**)&d_scoff, (void**)&d_sctab, (void**)&d_scid, (void**)&d_sclist};
+                     (void**)&d_sclev, (void**)&d_scoff, (void**)&d_sctab, (void**)&d_scid, (void**)&d_sclist,
+                     (void**)&d_adacc};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -584,6 +590,7 @@ WorldView<R> World::view() const
     w.sclist = d_sclist;
     w.bblo = d_bblo;
     w.bbhi = d_bbhi;
+    w.adacc = d_adacc;
     return w;
 }
 
@@ -658,6 +665,9 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         d.weighted = s.model.cell_weighted ? 1 : 0;
         if (d.weighted) has_weighting = 1;
         d.reset_until_time = s.model.reset_until_time;
+        d.time = 0.0;
+        d.adaptive = s.model.adaptive ? 1 : 0;
+        if (d.adaptive) has_adaptive = 1;
         {
             int lvl = d.sampler_full ? 2 : 0;
             const bool bgk_on = (s.model.collision_model == UGX_COLL_BGK || s.model.collision_model == UGX_COLL_HYBRID);
@@ -840,6 +850,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_uspq, z3))) return rc;
     if ((rc = up(d_usps, z6))) return rc;
     if ((rc = up(d_knacc, z8))) return rc;
+    if (has_adaptive && (rc = up(d_adacc, z8))) return rc;
+    adapt_ratio.assign((size_t)nc * 3, 0.0);
     if ((rc = up(d_samp, zs))) return rc;
     std::vector<uint32_t> zc((size_t)nc + 1, 0u);
     if ((rc = up(d_cnt, zc))) return rc;
@@ -879,6 +891,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
     if ((rc = up(d_st, vst))) return rc;
     specs_host = specs;
     samp_reset_steps.assign(specs.size(), 0);
+    adapt_time.assign(specs.size(), 0.0);
+    tsamp.assign(specs.size(), 0.0);
     cap = 0;
     uint64_t want = capacity > 0 ? capacity : 4096;
     if ((rc = alloc_particles((uint32_t)std::min<uint64_t>(want, 0xF0000000ull)))) return rc;
@@ -1056,8 +1070,23 @@ int World::step(int n)
             }
         }
         cur = 1 - cur;
-        for (auto& c : cases) c.step += 1;
+        for (size_t ci = 0; ci < cases.size(); ++ci) {
+            CaseDev& c = cases[ci];
+            if (c.active) {
+                if (((c.step + 1u) % (uint32_t)c.sample_interval) == 0u) tsamp[ci] = tsamp[ci] + c.dt;
+                if (c.adaptive) adapt_time[ci] = adapt_time[ci] + c.dt;
+                c.time = c.time + c.dt;
+            }
+            c.step += 1;
+        }
         n_bound += max_ins + (has_weighting ? n_bound / 16 + 1024 : 0);
+        // the dynamic adapter at the end of every adaptationInterval-th step
+        // (uniGasCloud.C:862, uniGasDynamicAdapter.C:549)
+        for (size_t ci = 0; ci < cases.size(); ++ci) {
+            const CaseDev& c = cases[ci];
+            if (c.active && c.adaptive && (c.step % (uint32_t)specs_host[ci].model.adaptation_interval) == 0u)
+                if (int rc = adapt_case((int)ci)) return rc;
+        }
     }
     HIP_TRY(hipGetLastError());
     return UGX_OK;
@@ -1116,11 +1145,17 @@ int World::fill(int ci, const ugx_fill_desc* f, const MeshGeom& g, bool sort, co
     if (int rc = count(n0)) return rc;
     double ntot = 0.0, expected = 0.0;
     for (int s = 0; s < cd.ns; ++s) ntot += f->number_density[s];
+    // an adaptive case starts its time step and subcell levels from the
+    // filled gas (uniGasMeshFill.C:100-109)
+    if (cd.adaptive) {
+        if (fixed) return set_err(UGX_ERR_ARG, "an adaptive case is filled with allocation (ugx_fill)");
+        if (int rc = adapt_fill(ci, f)) return rc;
+    }
     // a cell-weighted case takes its cell weights from this fill (uniGasMeshFill.C:111-168)
     if (cd.weighted) {
         if (f->exact_per_cell > 0) return set_err(UGX_ERR_ARG, "a cell-weighted case is filled by number density, not per cell");
         std::vector<double> wts;
-        if (int rc = fill_cell_weights(specs_host[ci], ntot, wts)) return rc;
+        if (int rc = fill_cell_weights(specs_host[ci], ntot, sclev_host.data() + cd.cell_begin, wts)) return rc;
         if (int rc = set_cell_weights(ci, wts.data())) return rc;
     }
     // capacity from the expected count
@@ -1204,6 +1239,86 @@ int World::set_cell_weights(int ci, const double* wts)
     std::copy(wts, wts + ncl, cw_host.begin() + cd.cell_begin);
     HIP_TRY(hipMemcpy(d_cw + cd.cell_begin, wts, sizeof(double) * ncl, hipMemcpyHostToDevice));
     update_max_ins();
+    return UGX_OK;
+}
+
+int World::set_time_step(int ci, double dt)
+{
+    if (!(dt > 0.0) || !std::isfinite(dt)) return set_err(UGX_ERR_STATE, "adapted time step %g is not > 0", dt);
+    HIP_TRY(hipStreamSynchronize(stream));
+    CaseDev& cd = cases[ci];
+    cd.dt = dt;
+    HIP_TRY(hipMemcpy((char*)(d_cases + ci) + offsetof(CaseDev, dt), &dt, sizeof(double), hipMemcpyHostToDevice));
+    // the freestream counts are per time step (uniGasGeneralBoundary.C:154-165)
+    MeshGeom& g = specs_host[ci].geom;
+    for (size_t k = 0; k < g.inflow.size(); ++k) {
+        InflowFS& e = g.inflow[k];
+        const double* pt = &g.inflow_parts[3 * k];
+        e.nbar_num = pt[0] * dt * pt[1] * pt[2];
+        e.nbar = e.nbar_num / e.nbar_den;
+        InflowFS d = e;
+        d.cell += cd.cell_begin;
+        d.species += cd.sp_base;
+        d.ccase = ci;
+        HIP_TRY(hipMemcpy(d_inflow + cd.inflow_begin + (int)k, &d, sizeof(InflowFS), hipMemcpyHostToDevice));
+    }
+    update_max_ins();
+    return UGX_OK;
+}
+
+int World::apply_adaptation(int ci, const AdaptOut& a)
+{
+    const CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    std::copy(a.ratio.begin(), a.ratio.end(), adapt_ratio.begin() + 3 * (size_t)cd.cell_begin);
+    if (a.dt != cd.dt)
+        if (int rc = set_time_step(ci, a.dt)) return rc;
+    if (!a.levels.empty()) {
+        bool changed = false;
+        for (int c = 0; c < ncl; ++c)
+            if (sclev_host[(size_t)cd.cell_begin + c] != a.levels[c]) {
+                sclev_host[(size_t)cd.cell_begin + c] = a.levels[c];
+                changed = true;
+            }
+        if (changed)
+            if (int rc = upload_subcells()) return rc;
+    }
+    return UGX_OK;
+}
+
+int World::adapt_fill(int ci, const ugx_fill_desc* f)
+{
+    const CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    // the filled gas; the adapter takes one translational temperature
+    const double T = (f->T[0] + f->T[1] + f->T[2]) / 3.0;
+    std::vector<int32_t> cm((size_t)ncl);
+    HIP_TRY(hipMemcpy(cm.data(), d_cmodel + cd.cell_begin, sizeof(int32_t) * ncl, hipMemcpyDeviceToHost));
+    AdaptOut a;
+    if (int rc = adapt_initial(specs_host[ci], f->number_density, T, f->velocity, cm.data(), a)) return rc;
+    return apply_adaptation(ci, a);
+}
+
+int World::adapt_case(int ci)
+{
+    const CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::vector<double> acc((size_t)ncl * kKnRows);
+    std::vector<int32_t> cm((size_t)ncl);
+    HIP_TRY(hipMemcpy(acc.data(), d_adacc + (size_t)cd.cell_begin * kKnRows, sizeof(double) * acc.size(),
+                      hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cm.data(), d_cmodel + cd.cell_begin, sizeof(int32_t) * ncl, hipMemcpyDeviceToHost));
+    AdaptOut a;
+    if (int rc = adapt_step(specs_host[ci], acc.data(), adapt_time[ci], cw_host.data() + cd.cell_begin, cm.data(),
+                            cd.dt, adapt_ratio.data() + 3 * (size_t)cd.cell_begin, a))
+        return rc;
+    if (int rc = apply_adaptation(ci, a)) return rc;
+    // the sums start again (uniGasDynamicAdapter.C:676-694)
+    std::fill(acc.begin(), acc.end(), 0.0);
+    HIP_TRY(hipMemcpy(d_adacc + (size_t)cd.cell_begin * kKnRows, acc.data(), sizeof(double) * acc.size(),
+                      hipMemcpyHostToDevice));
+    adapt_time[ci] = 0.0;
     return UGX_OK;
 }
 
@@ -1333,7 +1448,29 @@ int World::adopt(World& old, const std::vector<int>& old_of_new)
                                    sizeof(double) * (size_t)nwk * kNS * kWRows, hipMemcpyDeviceToDevice, stream));
         nc_.step = ocd[k].step;
         nc_.kn_time = ocd[k].kn_time;
+        nc_.time = ocd[k].time;
+        nc_.dt = ocd[k].dt;
         samp_reset_steps[i] = old.samp_reset_steps[k];
+        tsamp[i] = old.tsamp[k];
+        adapt_time[i] = old.adapt_time[k];
+        std::copy(old.adapt_ratio.begin() + 3 * (size_t)oc.cell_begin, old.adapt_ratio.begin() + 3 * (size_t)oc.cell_end,
+                  adapt_ratio.begin() + 3 * (size_t)nc_.cell_begin);
+        if (d_adacc && old.d_adacc)
+            HIP_TRY(hipMemcpyAsync(d_adacc + (size_t)nc_.cell_begin * kKnRows, old.d_adacc + (size_t)oc.cell_begin * kKnRows,
+                                   sizeof(double) * (size_t)ncl * kKnRows, hipMemcpyDeviceToDevice, stream));
+        if (ocd[k].dt != specs_host[i].model.delta_t) {
+            // an adapted time step: the inflow counts follow it
+            MeshGeom& gi = specs_host[i].geom;
+            gi.inflow = old.specs_host[k].geom.inflow;
+            for (size_t q = 0; q < gi.inflow.size(); ++q) {
+                InflowFS d = gi.inflow[q];
+                d.cell += nc_.cell_begin;
+                d.species += nc_.sp_base;
+                d.ccase = (int32_t)i;
+                HIP_TRY(hipMemcpyAsync(d_inflow + nc_.inflow_begin + (int)q, &d, sizeof(InflowFS), hipMemcpyHostToDevice, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+            }
+        }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, stream));
@@ -1421,7 +1558,7 @@ int World::case_counters(int ci, ugx_counters* out)
     out->particle_steps = (int64_t)cnt[kCntParticleSteps];
     out->cloned = (int64_t)cnt[kCntCloned];
     out->weight_deleted = (int64_t)cnt[kCntWeightDeleted];
-    out->time_sampled = (double)out->n_sampled_steps * cases[ci].dt;
+    out->time_sampled = tsamp[ci];
     int64_t k = 0;
     for (int32_t v : cm) k += v;
     out->n_dsmc_cells = k;
@@ -1487,9 +1624,36 @@ int World::case_state(int ci, ugx_case_state* s, bool upload)
         s->kn_time = dc.kn_time;
         s->sampled_at_reset = samp_reset_steps[ci];
         for (int k = 0; k < UGX_N_COUNTERS; ++k) s->counters[k] = (k < kCntNum) ? (uint64_t)cnt[k] : 0u;
+        s->delta_t = dc.dt;
+        s->time = dc.time;
+        s->time_sampled = tsamp[ci];
+        s->adapt_time = adapt_time[ci];
+        if (s->subcell_levels)
+            for (size_t c = 0; c < n; ++c) {
+                const uint32_t p = sclev_host[b + c];
+                for (int d = 0; d < 3; ++d) s->subcell_levels[3 * c + d] = (int32_t)((p >> (8 * d)) & 0xffu);
+            }
+        if (s->adapt_sums) {
+            if (d_adacc) HIP_TRY(hipMemcpy(s->adapt_sums, d_adacc + b * kKnRows, sizeof(double) * n * kKnRows,
+                                           hipMemcpyDeviceToHost));
+            else std::fill(s->adapt_sums, s->adapt_sums + n * kKnRows, 0.0);
+        }
+        if (s->adapt_ratio) std::copy(adapt_ratio.begin() + 3 * b, adapt_ratio.begin() + 3 * (b + n), s->adapt_ratio);
         return UGX_OK;
     }
     if (s->steps < 0 || s->steps > (int64_t)UINT32_MAX) return set_err(UGX_ERR_ARG, "step count out of range");
+    if (s->delta_t > 0.0 && s->delta_t != cd.dt)
+        if (int rc = set_time_step(ci, s->delta_t)) return rc;
+    if (s->subcell_levels)
+        if (int rc = set_subcell_levels(ci, s->subcell_levels)) return rc;
+    if (s->adapt_sums && d_adacc)
+        HIP_TRY(hipMemcpy(d_adacc + b * kKnRows, s->adapt_sums, sizeof(double) * n * kKnRows, hipMemcpyHostToDevice));
+    if (s->adapt_ratio) std::copy(s->adapt_ratio, s->adapt_ratio + 3 * n, adapt_ratio.begin() + 3 * b);
+    // a state without the sampled time (NaN) takes the sampled steps' count times the step
+    tsamp[ci] = std::isfinite(s->time_sampled)
+                    ? s->time_sampled
+                    : (double)((int64_t)s->counters[kCntSampled] - s->sampled_at_reset) * cases[ci].dt;
+    adapt_time[ci] = s->adapt_time;
     for (const Arr& a : arrs) HIP_TRY(hipMemcpy(a.dev, a.host, a.bytes, hipMemcpyHostToDevice));
     for (int k = 0; k < kCntNum; ++k) cnt[k] = (unsigned long long)s->counters[k];
     HIP_TRY(hipMemcpy(d_counters + (size_t)ci * kCntNum, cnt, sizeof cnt, hipMemcpyHostToDevice));
@@ -1498,8 +1662,10 @@ int World::case_state(int ci, ugx_case_state* s, bool upload)
     HIP_TRY(hipMemcpy(&dc, d_cases + ci, sizeof(CaseDev), hipMemcpyDeviceToHost));
     dc.step = (uint32_t)s->steps;
     dc.kn_time = s->kn_time;
+    dc.time = s->time;
     cd.step = dc.step;
     cd.kn_time = dc.kn_time;
+    cd.time = dc.time;
     HIP_TRY(hipMemcpy(d_cases + ci, &dc, sizeof(CaseDev), hipMemcpyHostToDevice));
     samp_reset_steps[ci] = s->sampled_at_reset;
     return UGX_OK;
@@ -1520,6 +1686,7 @@ int World::case_samples_reset(int ci)
     unsigned long long v = 0;
     HIP_TRY(hipMemcpy(&v, d_counters + (size_t)ci * kCntNum + kCntSampled, sizeof v, hipMemcpyDeviceToHost));
     samp_reset_steps[ci] = (int64_t)v;
+    tsamp[ci] = 0.0;
     return UGX_OK;
 }
 

@@ -23,6 +23,10 @@ struct MeshGeom {
     double mask[3] = {1.0, 1.0, 1.0};
     std::vector<BModel> bm;        // per patch
     std::vector<InflowFS> inflow;  // face-species entries (case-local)
+    // per entry the parts of its count that do not depend on the time step:
+    // face area times number density, most probable speed, and the flux
+    // bracket; nbar_num = ((fA n) dt) c_mp bracket
+    std::vector<double> inflow_parts;   // [3 * inflow.size()]
     // measured wall faces: the faces of wall patches with a specular or
     // diffuse model (uniGasPatchBoundary measurePropertiesAtWall_), in
     // patch then face order
@@ -164,6 +168,23 @@ struct World {
     int set_subcell_levels(int ci, const int32_t* lv);
     // the subcell tables and per-slot buffers for sclev_host
     int upload_subcells();
+    // adaptive simulation: the adapter's per-cell time sums (device), per
+    // cell the size ratio of the last adaptation, per case the time summed
+    // into the sums since then; per case the time of the sampled steps
+    // since the last sampler reset (every case)
+    int32_t has_adaptive = 0;
+    double* d_adacc = nullptr;
+    std::vector<double> adapt_ratio;    // [nc*3]
+    std::vector<double> adapt_time;     // [ncases]
+    std::vector<double> tsamp;          // [ncases]
+    // the adaptation of case ci at the end of a step
+    int adapt_case(int ci);
+    // the adapter's start from case ci's fill (before the fill's weights)
+    int adapt_fill(int ci, const ugx_fill_desc* f);
+    // sets case ci's time step and the inflow counts that depend on it
+    int set_time_step(int ci, double dt);
+    // applies an adapter's result to case ci
+    int apply_adaptation(int ci, const struct AdaptOut& a);
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;
@@ -241,7 +262,24 @@ int compute_wall_fields(const CaseSpec& spec, const double* wsamp, const ugx_cou
 // total number density ntot spread at particles_per_subcell simulators per
 // cell, smoothed until neighbour ratios stay below 1 + max_cell_weight_ratio,
 // never below minParticlesPerSubCell's weight (ugx_weights.cpp).
-int fill_cell_weights(const CaseSpec& spec, double ntot, std::vector<double>& w);
+// lv: packed subcell levels of the case's cells (nullptr: one subcell each).
+int fill_cell_weights(const CaseSpec& spec, double ntot, const uint32_t* lv, std::vector<double>& w);
+
+// Result of the dynamic adapter for one case (ugx_adapt.cpp).
+struct AdaptOut {
+    double dt = 0.0;                  // the time step from now on
+    std::vector<double> ratio;        // [nc*3] smoothed cell size to mean free path ratio
+    std::vector<uint32_t> levels;     // packed subcell levels; empty: unchanged
+};
+// At the fill: the filled gas (number densities nd[ns], temperature T,
+// velocity U) with the case's initial model ids.
+int adapt_initial(const CaseSpec& spec, const double* nd, double T, const double U[3], const int32_t* cmodel,
+                  AdaptOut& out);
+// Every adaptation interval: acc the adapter's time sums [nc*kKnRows] over
+// the time tav, cw the cell weights, cmodel the model ids, dt the current
+// time step, prev [nc*3] the size ratio of the last adaptation.
+int adapt_step(const CaseSpec& spec, const double* acc, double tav, const double* cw, const int32_t* cmodel,
+               double dt, const double* prev, AdaptOut& out);
 
 }  // namespace ugx
 

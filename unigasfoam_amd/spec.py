@@ -78,6 +78,16 @@ class CaseSpec:
     min_particles_per_subcell: int = 0
     max_smoothing_passes: int = 500
     max_cell_weight_ratio: float = 0.05
+    # adaptive simulation (adaptiveSimulation / adaptiveProperties)
+    adaptive: bool = False
+    time_step_adaptation: bool = False
+    subcell_adaptation: bool = False
+    cell_weight_adaptation: bool = False
+    adaptation_interval: int = 20
+    adaptation_smoothing_passes: int = 25
+    max_time_step_mct_ratio: float = 0.2
+    max_courant_number: float = 0.5
+    max_subcell_size_mfp_ratio: float = 0.5
 
     def species_index(self, name):
         for i, s in enumerate(self.species):
@@ -123,6 +133,15 @@ class CaseSpec:
         m.min_particles_per_subcell = self.min_particles_per_subcell
         m.max_smoothing_passes = self.max_smoothing_passes
         m.max_cell_weight_ratio = self.max_cell_weight_ratio
+        m.adaptive = 1 if self.adaptive else 0
+        m.time_step_adaptation = 1 if self.time_step_adaptation else 0
+        m.subcell_adaptation = 1 if self.subcell_adaptation else 0
+        m.cell_weight_adaptation = 1 if self.cell_weight_adaptation else 0
+        m.adaptation_interval = self.adaptation_interval
+        m.adaptation_smoothing_passes = self.adaptation_smoothing_passes if self.adaptation_smoothing_passes > 0 else -1
+        m.max_time_step_mct_ratio = self.max_time_step_mct_ratio
+        m.max_courant_number = self.max_courant_number
+        m.max_subcell_size_mfp_ratio = self.max_subcell_size_mfp_ratio
         nb = len(self.boundaries)
         bd = (abi.BoundaryDesc * max(nb, 1))()
         for i, b in enumerate(self.boundaries):
