@@ -158,6 +158,8 @@ struct orc_world {
 static void smooth_field(const orc_world* w, const double* f, double* out, int ncomp);
 static int fill_weights(orc_world* w, double ntot);
 static int adapt(orc_world* w);
+static void smooth_max_ratio(const orc_world* w, double* f, double max_ratio);
+static int cell_weighting(orc_world* w, int64_t n0, int64_t nall, int64_t* nclone, const double* cw_old);
 static int adapt_fill(orc_world* w, const ugx_fill_desc* fl);
 
 /* ------------------------------------------------------------- geometry */
@@ -1414,6 +1416,87 @@ static int adapt_fill(orc_world* w, const ugx_fill_desc* fl)
     return UGX_OK;
 }
 
+/* fvc::smooth(W, 1.3) (OpenFOAM fvcSmooth.C, called at
+ * uniGasDynamicAdapter.C:456): the smallest field at least W in which no
+ * cell is below a face neighbour's value over 2.3, i.e. each cell the
+ * largest W_j / 2.3^d over cells j at d faces from it.  OpenFOAM reaches it
+ * with a FaceCellWave whose 1% propagation tolerance may leave a cell up to
+ * 1% below this fixed point; that tolerance is not restated. */
+static void smooth_max_ratio(const orc_world* w, double* f, double max_ratio)
+{
+    int changed = 1;
+    while (changed) {
+        changed = 0;
+        for (int c = 0; c < w->nc; ++c)
+            for (int j = 0; j < MAXF; ++j) {
+                int k = c * MAXF + j;
+                if (w->fkind[k] != FACE_INTERNAL && w->fkind[k] != FACE_CYCLIC) continue;
+                double v = f[w->nbr[k]] / max_ratio;
+                if (v > f[c]) { f[c] = v; changed = 1; }
+            }
+    }
+}
+
+/* Cell weight adaptation (uniGasDynamicAdapter.C:416-503, 650-672): new
+ * weights n V / (N_pps N_sub F_N) from the averaged densities, the max-ratio
+ * smoothing, then area-average passes capped at n V / (N_min N_sub F_N)
+ * until neighbour ratios are below 1 + maxCellWeightRatio, blended into the
+ * old weights with theta = 0.2; every simulator then goes from its cell's
+ * old weight to the new one (uniGasCloud.C:1353-1425) and the cloud is
+ * sorted again. */
+static int adapt_weights(orc_world* w, const double* rhoN)
+{
+    const ugx_model_desc* m = &w->model;
+    int nc = w->nc;
+    double FN = m->n_equivalent_particles;
+    int npps = m->particles_per_subcell;
+    int nmin = m->min_particles_per_subcell > 0 ? m->min_particles_per_subcell : npps;
+    int max_passes = m->max_smoothing_passes > 0 ? m->max_smoothing_passes : 500;
+    double max_ratio = m->max_cell_weight_ratio > 0.0 ? m->max_cell_weight_ratio : 0.05;
+    const double theta = 0.2;
+    double* wn = malloc(sizeof(double) * (size_t)nc);
+    double* tmp = malloc(sizeof(double) * (size_t)nc);
+    double* nsub = malloc(sizeof(double) * (size_t)nc);
+    for (int c = 0; c < nc; ++c)
+        nsub[c] = (double)w->sclev[3 * c] * (double)w->sclev[3 * c + 1] * (double)w->sclev[3 * c + 2];
+    for (int c = 0; c < nc; ++c) wn[c] = (rhoN[c] * w->vol[c]) / ((double)npps * nsub[c] * FN * 1.0);
+    smooth_max_ratio(w, wn, 1.0 + 1.3);
+    int passes = 0;
+    double max_r;
+    do {
+        passes++;
+        smooth_field(w, wn, tmp, 1);
+        memcpy(wn, tmp, sizeof(double) * (size_t)nc);
+        for (int c = 0; c < nc; ++c) {
+            double lim = (rhoN[c] * w->vol[c]) / ((double)nmin * nsub[c] * FN * 1.0);
+            double v = wn[c] < lim ? wn[c] : lim;
+            wn[c] = v > SMALL ? v : SMALL;
+        }
+        max_r = 1.0e-300;
+        for (int c = 0; c < nc; ++c)
+            for (int j = 0; j < MAXF; ++j) {
+                int k = c * MAXF + j;
+                if (w->fkind[k] != FACE_INTERNAL) continue;
+                int o = w->nbr[k];
+                double a = wn[c] / wn[o], b = wn[o] / wn[c];
+                double r = a > b ? a : b;
+                if (r > max_r) max_r = r;
+            }
+    } while (max_r > 1.0 + max_ratio && passes < max_passes);
+    /* the blend; the simulators go from the old weights to it */
+    for (int c = 0; c < nc; ++c) {
+        tmp[c] = w->cw[c];
+        w->cw[c] = theta * wn[c] + (1.0 - theta) * w->cw[c];
+    }
+    int64_t ncl = 0;
+    int rc = cell_weighting(w, w->n, w->n, &ncl, tmp);
+    if (rc == UGX_OK) sort_by_cell(w, w->n + ncl);
+    free(wn);
+    free(tmp);
+    free(nsub);
+    return rc;
+}
+
 /* adapt (uniGasDynamicAdapter.C:526-700) at the end of every
  * adaptationInterval-th step: the fields averaged over the time since the
  * last adaptation (an empty cell takes the smallest density, split evenly
@@ -1462,11 +1545,13 @@ static int adapt(orc_world* w)
     for (int pass = 0; pass < w->model.adaptation_smoothing_passes; ++pass) smooth_inplace(w, ratio, 3);
     if (w->model.time_step_adaptation) set_delta_t(w, adapt_time_step(w, mct, co, dt));
     if (w->model.subcell_adaptation) adapt_levels(w, ratio);
+    int rc = UGX_OK;
+    if (w->model.cell_weight_adaptation) rc = adapt_weights(w, rhoN);
     memcpy(w->adratio, ratio, sizeof(double) * (size_t)nc * 3);
     memset(w->adacc, 0, sizeof(double) * (size_t)nc * KNR);
     w->adapt_time = 0.0;
     free(rhoN); free(T); free(U); free(mct); free(co); free(ratio);
-    return UGX_OK;
+    return rc;
 }
 
 /* uniGasMeshFill::setInitialConfiguration (uniGasMeshFill.C:58-297): per
@@ -1657,7 +1742,7 @@ static int fill_weights(orc_world* w, double ntot)
  * whose weight rose is removed with probability 1 - W_o/W_n.  Draws from a
  * stream per simulator (slot n0 + i of the inflow keyed as 2^31 | i); the
  * clones are copies appended in the order of their parents. */
-static int cell_weighting(orc_world* w, int64_t n0, int64_t nall, int64_t* nclone)
+static int cell_weighting(orc_world* w, int64_t n0, int64_t nall, int64_t* nclone, const double* cw_old)
 {
     int32_t* kc = malloc(sizeof(int32_t) * (size_t)(nall > 0 ? nall : 1));
     int64_t tot = 0, wdel = 0;
@@ -1665,11 +1750,13 @@ static int cell_weighting(orc_world* w, int64_t n0, int64_t nall, int64_t* nclon
         int32_t k = 0;
         int c = w->cell[i];
         if (c != CELL_DEAD) {
-            double wo = w->cw[w->cstart[i]], wn = w->cw[c];
+            /* after the move: from the start cell's weight; after an
+             * adaptation: from the cell's previous weight (own stream) */
+            double wo = cw_old ? cw_old[c] : w->cw[w->cstart[i]], wn = w->cw[c];
             if (wo != wn) {
                 uint32_t id = (i < n0) ? (uint32_t)i : (0x80000000u | (uint32_t)(i - n0));
                 orc_rng r;
-                orc_rng_init(&r, w->seed, ORC_RNG_WEIGHT, 0u, id, w->step);
+                orc_rng_init(&r, w->seed, ORC_RNG_WEIGHT, cw_old ? 1u : 0u, id, w->step);
                 if (wo > wn) {
                     double prob = wo / wn - 1.0;
                     while (prob > 1.0) { k += 1; prob = prob - 1.0; }
@@ -1722,6 +1809,12 @@ int orc_adapt_inject(orc_world* w, const double* acc, double tav)
     memcpy(w->adacc, acc, sizeof(double) * (size_t)w->nc * KNR);
     w->adapt_time = tav;
     return adapt(w);
+}
+
+int orc_smooth_max_ratio(orc_world* w, double* f, double max_ratio)
+{
+    smooth_max_ratio(w, f, max_ratio);
+    return UGX_OK;
 }
 
 int orc_set_cell_model(orc_world* w, const int32_t* in, int64_t len)
@@ -1806,7 +1899,7 @@ int orc_step_phases(orc_world* w, int mask)
         w->cnt.track_errors += errs;
         if (w->model.cell_weighted) {
             int64_t ncl = 0;
-            int rc = cell_weighting(w, n0, n0 + nins, &ncl);
+            int rc = cell_weighting(w, n0, n0 + nins, &ncl, NULL);
             if (rc) return rc;
             nins += ncl;
         }

@@ -94,6 +94,8 @@ double orc_time(orc_world* w);
  * cell model ids */
 int orc_adapt_inject(orc_world* w, const double* acc, double tav);
 int orc_set_cell_model(orc_world* w, const int32_t* in, int64_t len);
+/* test hook: fvc::smooth(f, max_ratio - 1) in place, [nc] */
+int orc_smooth_max_ratio(orc_world* w, double* f, double max_ratio);
 int orc_wall_samples(orc_world* w, double* out, int64_t len);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 

@@ -102,6 +102,7 @@ def _bind(L):
     L.orc_time.restype = C.c_double
     L.orc_adapt_inject.argtypes = [vp, abi.DBL_P, C.c_double]
     L.orc_set_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
+    L.orc_smooth_max_ratio.argtypes = [vp, abi.DBL_P, C.c_double]
 
 
 def _check(rc):
@@ -195,6 +196,11 @@ class OracleCase:
         """Sets the adapter's time sums [n_cells, 16] and their time, then adapts once."""
         acc = np.ascontiguousarray(acc, dtype=np.float64).reshape(self.nc, abi.KN_ROWS)
         _check(self.L.orc_adapt_inject(self.h, abi.dptr(acc), float(tav)))
+
+    def smooth_max_ratio(self, f, max_ratio):
+        f = np.array(f, dtype=np.float64)
+        _check(self.L.orc_smooth_max_ratio(self.h, abi.dptr(f), float(max_ratio)))
+        return f
 
     def set_cell_model(self, ids):
         ids = np.ascontiguousarray(np.broadcast_to(np.asarray(ids, dtype=np.int32), (self.nc,)))

@@ -3,7 +3,8 @@
 adaptation interval are those of the oracle, and the run stays
 bit-identical to the oracle across the changes (collisions, a freestream
 inflow whose counts follow the time step, cell weights of a weighted fill
-with subcells, the hybrid switch), the same alone and in a batch, and
+with subcells and their adaptation with the re-weighting of every
+simulator, the hybrid switch), the same alone and in a batch, and
 through a restart between two adaptations."""
 import numpy as np
 import pytest
@@ -38,6 +39,7 @@ def inflow_case(weighted=False):
         spec.cell_weighted = True
         spec.particles_per_subcell = 4
         spec.max_cell_weight_ratio = 0.5
+        spec.cell_weight_adaptation = True
     return spec, {"number_density": {"Ar": nd}, "T": 300.0}
 
 

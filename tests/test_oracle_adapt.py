@@ -39,6 +39,7 @@ AR = Species("Ar", mass=6.63e-26, diameter=4.17e-10, omega=0.81)
 AR2 = Species("Ar2", mass=6.63e-26, diameter=4.17e-10, omega=0.81)
 SIDES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
 TREF = 273.0
+FACE_INTERNAL, FACE_CYCLIC = 1, 2     # face kinds of the oracle's geometry
 
 
 def nu(sp, n, T):
@@ -140,10 +141,10 @@ def test_two_identical_species_act_as_one():
 
 def state_sums(spec, o, n, T, U, fn, tav, sp=AR):
     """Adapter time sums of a gas at (n, T, U) in every cell: per step and
-    unit time the cell holds N = n V / F_N simulators."""
+    unit time the cell holds N = n V / (F_N W_c) simulators."""
     vol = o.geometry()["vol"]
     acc = np.zeros((spec.mesh.n_cells, abi.KN_ROWS))
-    N = n * vol / fn * tav
+    N = n * vol / (fn * o.cell_weights()) * tav
     u2 = float(np.dot(U, U))
     acc[:, 0] = N
     acc[:, 1] = sp.mass * N
@@ -240,4 +241,127 @@ def test_stepping_adapts_every_interval():
     co = max(max(cmp(AR, b), u) * dt1 / L for b, U3 in zip(Tc, Uc) for u in U3)
     want = dt1 * min(0.2 / mct, 0.5 / co)
     assert o.delta_t() == pytest.approx(want, rel=1e-9)
+    o.close()
+
+
+# ------------------------------------------------------------ cell weights
+
+def face_graph(o, nc):
+    """Neighbour lists over internal and cyclic faces from the oracle's
+    geometry."""
+    g = o.geometry()
+    nbrs = [set() for _ in range(nc)]
+    for c in range(nc):
+        for j in range(6):
+            if g["fkind"][c, j] in (FACE_INTERNAL, FACE_CYCLIC):
+                nbrs[c].add(int(g["nbr"][c, j]))
+    return nbrs
+
+
+def test_max_ratio_smoothing_is_the_decayed_maximum():
+    """fvc::smooth(W, 1.3): every cell ends at the largest W_j / 2.3^d(j, c)
+    over all cells j, d the number of faces crossed (breadth-first search
+    over internal and cyclic faces; the division repeated d times)."""
+    pt = {k: abi.WALL for k in SIDES}
+    pt["xmin"] = pt["xmax"] = abi.CYCLIC
+    mesh = box_mesh((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (7, 5, 2), pt)
+    spec = CaseSpec(mesh=mesh, species=[AR], delta_t=1e-6, n_equivalent_particles=1.0, collision_model="dsmc",
+                    dsmc_model="variableHardSphere", Tref=TREF, seed=1,
+                    boundaries=[Boundary(p, abi.BND_SPECULAR) for p in SIDES[2:]])
+    o = orc.OracleCase(spec, capacity=16)
+    nc = mesh.n_cells
+    rng = np.random.default_rng(2)
+    w0 = 10.0 ** rng.uniform(-3.0, 2.0, nc)
+    got = o.smooth_max_ratio(w0, 2.3)
+    nbrs = face_graph(o, nc)
+    want = np.array(w0)
+    for j in range(nc):
+        dist = {j: 0}
+        frontier = [j]
+        while frontier:
+            nxt = []
+            for a in frontier:
+                for b in nbrs[a]:
+                    if b not in dist:
+                        dist[b] = dist[a] + 1
+                        nxt.append(b)
+            frontier = nxt
+        for c, d in dist.items():
+            v = w0[j]
+            for _ in range(d):
+                v = v / 2.3
+            want[c] = max(want[c], v)
+    assert np.array_equal(got, want)
+    assert (got >= w0).all() and (got > w0).any()
+    o.close()
+
+
+def weighted_adaptive(passes=0, npps=20, **kw):
+    spec = adaptive_box(L=1.5e-4, fn=2e8, passes=passes, **kw)
+    spec.cell_weighted = True
+    spec.particles_per_subcell = npps
+    spec.cell_weight_adaptation = True
+    spec.time_step_adaptation = False
+    spec.subcell_adaptation = False
+    return spec
+
+
+def test_weight_adaptation_blends_and_reweights():
+    """A gas twice as dense as the fill: the new weights are 2 W, blended
+    with theta = 0.2 into 1.2 W; each simulator then survives with
+    probability 1 / 1.2.  The same density as the fill changes nothing."""
+    n, T, tav = 1.2e22, 300.0, 2.0e-6
+    for factor in (2.0, 1.0):
+        spec = weighted_adaptive()
+        o = orc.OracleCase(spec, capacity=1 << 18)
+        o.fill({"Ar": n}, T)
+        w0 = o.cell_weights()
+        n0 = o.count()
+        c0 = o.counters()
+        o.adapt_inject(state_sums(spec, o, factor * n, T, (0.0, 0.0, 0.0), spec.n_equivalent_particles, tav), tav)
+        want = 0.2 * ((factor * n * o.geometry()["vol"]) / (20.0 * spec.n_equivalent_particles)) + 0.8 * w0
+        assert np.allclose(o.cell_weights(), want, rtol=1e-12, atol=0.0)
+        n1 = o.count()
+        c1 = o.counters()
+        if factor == 1.0:
+            assert n1 == n0 and c1["weight_deleted"] == c0["weight_deleted"]
+        else:
+            p = 1.0 / 1.2
+            assert abs(n1 - p * n0) < 5.0 * np.sqrt(n0 * p * (1.0 - p))
+            assert c1["weight_deleted"] - c0["weight_deleted"] == n0 - n1
+            # the survivors are sorted by cell again
+            cells = o.particles()["cell"]
+            assert (np.diff(cells) >= 0).all()
+        o.close()
+
+
+def test_weight_adaptation_keeps_the_molecule_count():
+    """Going from W_o to W_n every simulator leaves W_o / W_n copies of
+    itself on average, so F_N sum_c W_c N_c is kept in expectation: a
+    density step across the box (weights rising by up to 2.3 per cell,
+    clones in the cells whose weights fall)."""
+    n, T, tav = 1.2e22, 300.0, 2.0e-6
+    spec = weighted_adaptive(npps=200)
+    spec.max_cell_weight_ratio = 3.0          # one smoothing pass: the step stays sharp
+    o = orc.OracleCase(spec, capacity=1 << 19)
+    o.fill({"Ar": n}, T)
+    vol = o.geometry()["vol"]
+    acc = state_sums(spec, o, n, T, (0.0, 0.0, 0.0), spec.n_equivalent_particles, tav)
+    cc = o.geometry()["cc"]
+    hi = cc[:, 0] > cc[:, 0].mean()
+    acc[hi] *= 8.0
+    acc[~hi] *= 0.25
+
+    def molecules():
+        p = o.particles()
+        return float((o.cell_weights()[p["cell"]]).sum()) * spec.n_equivalent_particles, len(p["cell"])
+
+    m0, n0 = molecules()
+    o.adapt_inject(acc, tav)
+    m1, n1 = molecules()
+    c = o.counters()
+    assert c["cloned"] > 100 and c["weight_deleted"] > 100
+    assert abs(m1 - m0) < 5.0 * m0 / np.sqrt(n0)
+    w = o.cell_weights()
+    assert w[hi].mean() > w[~hi].mean()
     o.close()

@@ -28,6 +28,21 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fas
 
 # name: (snippet, replacement, pins that must fail)
 MUTANTS = {
+    # weights blended with the weights swapped (uniGasDynamicAdapter.C:665)
+    "adapt_weight_blend": (
+        "w->cw[c] = theta * wn[c] + (1.0 - theta) * w->cw[c];",
+        "w->cw[c] = (1.0 - theta) * wn[c] + theta * w->cw[c];",
+        [adapt.test_weight_adaptation_blends_and_reweights]),
+    # fvc::smooth's ratio taken as the coefficient itself, 1.3 for 1 + 1.3 (:456)
+    "adapt_max_ratio_coefficient": (
+        "    if (w->fkind[k] != FACE_INTERNAL && w->fkind[k] != FACE_CYCLIC) continue;\n                double v = f[w->nbr[k]] / max_ratio;",
+        "    if (w->fkind[k] != FACE_INTERNAL && w->fkind[k] != FACE_CYCLIC) continue;\n                double v = f[w->nbr[k]] / (max_ratio - 1.0);",
+        [adapt.test_max_ratio_smoothing_is_the_decayed_maximum]),
+    # the re-weighting after an adaptation from the start cells instead of the old weights (uniGasCloud.C:228-232)
+    "adapt_reweight_old_weights": (
+        "double wo = cw_old ? cw_old[c] : w->cw[w->cstart[i]], wn = w->cw[c];",
+        "double wo = w->cw[w->cstart[i]], wn = w->cw[c];",
+        [adapt.test_weight_adaptation_blends_and_reweights, adapt.test_weight_adaptation_keeps_the_molecule_count]),
     # collision rate with the exponent omega for 1 - omega (uniGasDynamicAdapter.C:262-263, Bird eq. 4.74)
     "adapt_collision_rate_exponent": (
         "pow(T / Tref, 1.0 - omPQ)",

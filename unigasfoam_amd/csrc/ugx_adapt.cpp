@@ -10,7 +10,9 @@
 //    the largest Courant number of the solution directions meet their
 //    limits (calculateTimeStep, :316-372);
 //  * the subcell levels of the DSMC cells: ceil(size ratio / limit) in
-//    1..10 per solution direction (calculateSubCellLevels, :374-414).
+//    1..10 per solution direction (calculateSubCellLevels, :374-414);
+//  * the cell weights of a weighted case (calculateCellWeightFactor and
+//    smoothCellWeightFactor, :416-503, blended at :661-668).
 // At the fill the gas is the filled one and all three fields are smoothed
 // (setInitialConfiguration, :700-770); afterwards it is the time average of
 // the cell measurements since the last adaptation and only the size ratio
@@ -180,8 +182,66 @@ int adapt_initial(const CaseSpec& spec, const double* nd, double T, const double
     return UGX_OK;
 }
 
+void smooth_max_ratio(const MeshGeom& g, std::vector<double>& f, double max_ratio)
+{
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int c = 0; c < g.nc; ++c)
+            for (int j = 0; j < kMaxFaces; ++j) {
+                const size_t k = (size_t)c * kMaxFaces + j;
+                if (g.fkind[k] != kFaceInternal && g.fkind[k] != kFaceCyclic) continue;
+                const double v = f[g.nbr[k]] / max_ratio;
+                if (v > f[c]) { f[c] = v; changed = true; }
+            }
+    }
+}
+
+// Cell weight adaptation (uniGasDynamicAdapter.C:416-503, 650-672): the
+// weights of the averaged densities at particlesPerSubCell per subcell,
+// fvc::smooth(W, 1.3), area-average passes capped at minParticlesPerSubCell
+// per subcell until neighbour ratios are below 1 + maxCellWeightRatio,
+// then blended into the old weights with theta = 0.2.
+static void adapt_weights(const CaseSpec& spec, const std::vector<double>& rhoN, const uint32_t* lv,
+                          const double* cw_old, std::vector<double>& cw)
+{
+    const MeshGeom& g = spec.geom;
+    const ugx_model_desc& m = spec.model;
+    const int nc = g.nc;
+    const double FN = m.n_equivalent_particles;
+    const int npps = m.particles_per_subcell;
+    const int nmin = m.min_particles_per_subcell > 0 ? m.min_particles_per_subcell : npps;
+    const int max_passes = m.max_smoothing_passes > 0 ? m.max_smoothing_passes : 500;
+    const double max_ratio = m.max_cell_weight_ratio > 0.0 ? m.max_cell_weight_ratio : 0.05;
+    const double theta = 0.2;
+    std::vector<double> nsub((size_t)nc), wn((size_t)nc);
+    for (int c = 0; c < nc; ++c)
+        nsub[c] = (double)(lv[c] & 0xffu) * (double)((lv[c] >> 8) & 0xffu) * (double)((lv[c] >> 16) & 0xffu);
+    for (int c = 0; c < nc; ++c) wn[c] = (rhoN[c] * g.vol[c]) / ((double)npps * nsub[c] * FN * 1.0);
+    smooth_max_ratio(g, wn, 1.0 + 1.3);
+    int passes = 0;
+    double max_r;
+    do {
+        passes++;
+        smooth_field(g, wn, 1);
+        for (int c = 0; c < nc; ++c)
+            wn[c] = std::max(std::min((rhoN[c] * g.vol[c]) / ((double)nmin * nsub[c] * FN * 1.0), wn[c]), kSmall);
+        max_r = 1.0e-300;
+        for (int c = 0; c < nc; ++c)
+            for (int j = 0; j < kMaxFaces; ++j) {
+                const size_t k = (size_t)c * kMaxFaces + j;
+                if (g.fkind[k] != kFaceInternal) continue;
+                const int o = g.nbr[k];
+                const double r = std::max(wn[c] / wn[o], wn[o] / wn[c]);
+                if (r > max_r) max_r = r;
+            }
+    } while (max_r > 1.0 + max_ratio && passes < max_passes);
+    cw.resize((size_t)nc);
+    for (int c = 0; c < nc; ++c) cw[c] = theta * wn[c] + (1.0 - theta) * cw_old[c];
+}
+
 int adapt_step(const CaseSpec& spec, const double* acc, double tav, const double* cw, const int32_t* cmodel,
-               double dt, const double* prev, AdaptOut& out)
+               double dt, const double* prev, const uint32_t* levels, AdaptOut& out)
 {
     const MeshGeom& g = spec.geom;
     const ugx_model_desc& m = spec.model;
@@ -236,6 +296,9 @@ int adapt_step(const CaseSpec& spec, const double* acc, double tav, const double
         out.levels.resize((size_t)nc);
         subcell_levels(spec, out.ratio, cmodel, out.levels.data());
     }
+    out.cw.clear();
+    if (m.cell_weight_adaptation)
+        adapt_weights(spec, rhoN, out.levels.empty() ? levels : out.levels.data(), cw, out.cw);
     return UGX_OK;
 }
 

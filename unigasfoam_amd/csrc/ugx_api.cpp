@@ -37,8 +37,6 @@ static int validate_model(const ugx_model_desc* m)
             return set_err(UGX_ERR_ARG, "adaptiveProperties ratios must be > 0");
         if (m->cell_weight_adaptation && !m->cell_weighted)
             return set_err(UGX_ERR_ARG, "cellWeightAdaptation needs cellWeightedSimulation");
-        if (m->cell_weight_adaptation)
-            return set_err(UGX_ERR_UNSUPPORTED, "adaptiveProperties.cellWeightAdaptation is not supported");
     }
     return UGX_OK;
 }

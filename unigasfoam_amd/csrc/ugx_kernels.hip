@@ -709,6 +709,53 @@ __global__ void k_clone(WorldView<R> w)
     }
 }
 
+// The re-weighting after a cell weight adaptation (uniGasDynamicAdapter.C:
+// 661-672, then uniGasCloud.C:1353-1425): every simulator of case ccase
+// goes from its cell's previous weight cw_old to the current one, by the
+// rule of k_weight, on its own stream (sub 1, keyed by its slot in the case
+// and the step count after the step).  Run between steps on the sorted
+// simulators; the clones follow them and a sort compacts.
+template <class R>
+__global__ void k_reweight_begin(WorldView<R> w)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.st->n_ins = 0u;
+}
+
+template <class R>
+__global__ void k_reweight(WorldView<R> w, const double* cw_old, int ccase)
+{
+    TCount<uint32_t, 1> tc{-1, {0u, 0u, 0u, 0u}};
+    const uint32_t n = w.st->n;
+    const CaseDev& cs = w.cases[ccase];
+    const uint32_t first = w.off[cs.cell_begin];   // the case's first slot in the sorted order
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t <= w.cap; t += gridDim.x * blockDim.x) {
+        uint32_t k = 0u;
+        bool del = false;
+        if (t < n) {
+            const uint32_t c = w.a.cell[t];
+            if (c != kDead && w.cell_case[c] == ccase) {
+                const double wo = cw_old[c], wn = w.cw[c];
+                if (wo != wn) {
+                    Rng r;
+                    rng_init(r, cs.seed_lo, cs.seed_hi, kRngWeight, 1u, t - first, cs.step);
+                    if (wo > wn) {
+                        double prob = wo / wn - 1.0;
+                        while (prob > 1.0) { k += 1u; prob = prob - 1.0; }
+                        if (prob > uniform(r)) k += 1u;
+                    } else if (wo / wn < uniform(r)) {
+                        del = true;
+                        w.a.cell[t] = kDead;
+                    }
+                }
+            }
+            tcount_add(tc, w.counters, ccase, 0, k);
+            tcount_add(tc, w.counters, ccase, 1, del ? 1u : 0u);
+        }
+        w.wclone[t] = k;
+    }
+    tcount_block_flush(tc, w.counters);
+}
+
 template <class R>
 __global__ void k_clone_finish(WorldView<R> w)
 {
@@ -2972,6 +3019,17 @@ void launch_sort(const WorldView<R>& w, hipStream_t s)
 }
 
 template <class R>
+void launch_reweight(const WorldView<R>& w, const double* cw_old, int ccase, hipStream_t s)
+{
+    const unsigned gw = count_grid(w.cap);
+    hipLaunchKernelGGL(k_reweight_begin<R>, dim3(1), dim3(1), 0, s, w);
+    hipLaunchKernelGGL(k_reweight<R>, dim3(gw), dim3(kBlock), 0, s, w, cw_old, ccase);
+    scan_u32(w.wclone, w.cap + 1u, w.wcoff, w.wbsum, &w.st->nclone, s);
+    hipLaunchKernelGGL(k_clone<R>, dim3(gw), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_clone_finish<R>, dim3(1), dim3(64), 0, s, w);
+}
+
+template <class R>
 void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
 {
     WorldView<R> w = w0;
@@ -3146,6 +3204,8 @@ template void launch_step<float>(const WorldView<float>&, hipStream_t, hipEvent_
 template void launch_step<double>(const WorldView<double>&, hipStream_t, hipEvent_t*);
 template void launch_sort<float>(const WorldView<float>&, hipStream_t);
 template void launch_sort<double>(const WorldView<double>&, hipStream_t);
+template void launch_reweight<float>(const WorldView<float>&, const double*, int, hipStream_t);
+template void launch_reweight<double>(const WorldView<double>&, const double*, int, hipStream_t);
 template void launch_fill<float>(const WorldView<float>&, hipStream_t, int, const double*, const double*, double,
                                  const double*, int, const double*, const double*, uint32_t*, uint32_t*, int, int);
 template void launch_fill<double>(const WorldView<double>&, hipStream_t, int, const double*, const double*, double,

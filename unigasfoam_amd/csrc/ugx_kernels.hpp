@@ -233,6 +233,9 @@ template <class R> void launch_step(const WorldView<R>& w, hipStream_t s, hipEve
 // Blocks of the move kernel that are resident on `device` at once (0 on error).
 template <class R> uint32_t move_blocks(int device);
 template <class R> void launch_sort(const WorldView<R>& w, hipStream_t s);
+// Every simulator of case ccase from its cell's weight in cw_old to the
+// current weight (clones appended, removals marked); a sort follows.
+template <class R> void launch_reweight(const WorldView<R>& w, const double* cw_old, int ccase, hipStream_t s);
 template <class R> void launch_fill(const WorldView<R>& w, hipStream_t s, int ccase,
                                     const double* nd, const double* T, double rotT, const double* U,
                                     int exact, const double* bblo, const double* bbhi,

@@ -362,7 +362,7 @@ void World::release()
                      (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp,
                      (void**)&d_cw, (void**)&d_cstart, (void**)&d_wclone, (void**)&d_wcoff, (void**)&d_wbsum,
                      (void**)&d_sclev, (void**)&d_scoff, (void**)&d_sctab, (void**)&d_scid, (void**)&d_sclist,
-                     (void**)&d_adacc};
+                     (void**)&d_adacc, (void**)&d_cw_old};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -1286,6 +1286,36 @@ int World::apply_adaptation(int ci, const AdaptOut& a)
     return UGX_OK;
 }
 
+int World::reweight(int ci, const std::vector<double>& old_w)
+{
+    const CaseDev& cd = cases[ci];
+    const int ncl = cd.cell_end - cd.cell_begin;
+    // room for the clones: each simulator of cell c gets at most ceil(W_o / W_n) - 1
+    std::vector<uint32_t> off((size_t)nc + 1);
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipMemcpy(off.data(), d_off, sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
+    uint32_t n = 0;
+    if (int rc = count(n)) return rc;
+    double extra = 0.0;
+    for (int c = 0; c < ncl; ++c) {
+        const double r = old_w[c] / cw_host[(size_t)cd.cell_begin + c];
+        if (r > 1.0) extra += std::ceil(r - 1.0) * (double)(off[(size_t)cd.cell_begin + c + 1] - off[(size_t)cd.cell_begin + c]);
+    }
+    if (int rc = ensure_capacity((uint64_t)n + (uint64_t)extra + 1024u)) return rc;
+    if (!d_cw_old)
+        if (int rc = dalloc(d_cw_old, (size_t)nc)) return rc;
+    std::vector<double> full(cw_host);
+    std::copy(old_w.begin(), old_w.end(), full.begin() + cd.cell_begin);
+    HIP_TRY(hipMemcpy(d_cw_old, full.data(), sizeof(double) * (size_t)nc, hipMemcpyHostToDevice));
+    if (precision == UGX_FP32_STATE) launch_reweight<float>(view<float>(), d_cw_old, ci, stream);
+    else launch_reweight<double>(view<double>(), d_cw_old, ci, stream);
+    HIP_TRY(hipGetLastError());
+    if (int rc = sort_now()) return rc;
+    if (int rc = check_state()) return rc;
+    n_bound_valid = false;
+    return UGX_OK;
+}
+
 int World::adapt_fill(int ci, const ugx_fill_desc* f)
 {
     const CaseDev& cd = cases[ci];
@@ -1311,9 +1341,15 @@ int World::adapt_case(int ci)
     HIP_TRY(hipMemcpy(cm.data(), d_cmodel + cd.cell_begin, sizeof(int32_t) * ncl, hipMemcpyDeviceToHost));
     AdaptOut a;
     if (int rc = adapt_step(specs_host[ci], acc.data(), adapt_time[ci], cw_host.data() + cd.cell_begin, cm.data(),
-                            cd.dt, adapt_ratio.data() + 3 * (size_t)cd.cell_begin, a))
+                            cd.dt, adapt_ratio.data() + 3 * (size_t)cd.cell_begin, sclev_host.data() + cd.cell_begin, a))
         return rc;
     if (int rc = apply_adaptation(ci, a)) return rc;
+    if (!a.cw.empty()) {
+        // the new weights, then every simulator from its cell's old weight (uniGasCloud.C:228-232)
+        std::vector<double> old_w(cw_host.begin() + cd.cell_begin, cw_host.begin() + cd.cell_end);
+        if (int rc = set_cell_weights(ci, a.cw.data())) return rc;
+        if (int rc = reweight(ci, old_w)) return rc;
+    }
     // the sums start again (uniGasDynamicAdapter.C:676-694)
     std::fill(acc.begin(), acc.end(), 0.0);
     HIP_TRY(hipMemcpy(d_adacc + (size_t)cd.cell_begin * kKnRows, acc.data(), sizeof(double) * acc.size(),

@@ -185,6 +185,10 @@ struct World {
     int set_time_step(int ci, double dt);
     // applies an adapter's result to case ci
     int apply_adaptation(int ci, const struct AdaptOut& a);
+    // every simulator of case ci from its cell's weight in old_w to the
+    // current one, then sorted again
+    int reweight(int ci, const std::vector<double>& old_w);
+    double* d_cw_old = nullptr;         // [nc] the weights before an adaptation
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;
@@ -270,6 +274,7 @@ struct AdaptOut {
     double dt = 0.0;                  // the time step from now on
     std::vector<double> ratio;        // [nc*3] smoothed cell size to mean free path ratio
     std::vector<uint32_t> levels;     // packed subcell levels; empty: unchanged
+    std::vector<double> cw;           // new cell weights (cell weight adaptation); empty: unchanged
 };
 // At the fill: the filled gas (number densities nd[ns], temperature T,
 // velocity U) with the case's initial model ids.
@@ -279,7 +284,10 @@ int adapt_initial(const CaseSpec& spec, const double* nd, double T, const double
 // the time tav, cw the cell weights, cmodel the model ids, dt the current
 // time step, prev [nc*3] the size ratio of the last adaptation.
 int adapt_step(const CaseSpec& spec, const double* acc, double tav, const double* cw, const int32_t* cmodel,
-               double dt, const double* prev, AdaptOut& out);
+               double dt, const double* prev, const uint32_t* levels, AdaptOut& out);
+// fvc::smooth(f, coeff): the smallest field >= f with no cell below a face
+// neighbour's value over 1 + coeff (internal and cyclic faces).
+void smooth_max_ratio(const MeshGeom& g, std::vector<double>& f, double max_ratio);
 
 }  // namespace ugx
 
