@@ -142,7 +142,8 @@ def test_unsupported_switches_are_named(tmp_path):
     p = case / "constant" / "uniGasProperties"
     text = p.read_text()
     text = text.replace("axisymmetricSimulation          false;", "axisymmetricSimulation          true;")
-    text = text.replace("adaptiveSimulation              false;", "adaptiveSimulation              on;")
+    text = text.replace("adaptiveSimulation              false;",
+                        "adaptiveSimulation              on;\nadaptiveProperties { cellWeightAdaptation true; }")
     text = text.replace("dsmcCollisionPartnerModel       noTimeCounter;",
                         "dsmcCollisionPartnerModel       noTimeCounterSubCycled;")
     p.write_text(text)
@@ -150,9 +151,25 @@ def test_unsupported_switches_are_named(tmp_path):
         case_dir.read_case(str(case))
     assert e.value.code == abi.ERR_UNSUPPORTED == 7
     words = " ".join(w for _, w in e.value.items)
-    assert "axisymmetricSimulation" in words and "adaptiveSimulation" in words
+    assert "axisymmetricSimulation" in words and "cellWeightAdaptation without cellWeightedSimulation" in words
     assert "noTimeCounterSubCycled" in words
     assert len(e.value.items) == 3
+
+
+def test_adaptive_properties_are_read(tmp_path):
+    case = tmp_path / "c"
+    shutil.copytree(os.path.join(CASES, "heatBath"), case)
+    p = case / "constant" / "uniGasProperties"
+    text = p.read_text().replace("adaptiveSimulation              false;",
+                                 "adaptiveSimulation              true;\nadaptiveProperties\n{\n"
+                                 "    timeStepAdaptation true;\n    subCellAdaptation yes;\n    adaptationInterval 50;\n"
+                                 "    maxCourantNumber 0.4;\n}")
+    p.write_text(text)
+    spec, fills, run = case_dir.read_case(str(case))
+    assert spec.adaptive and spec.time_step_adaptation and spec.subcell_adaptation
+    assert not spec.cell_weight_adaptation
+    assert spec.adaptation_interval == 50 and spec.max_courant_number == 0.4
+    assert spec.max_time_step_mct_ratio == 0.2 and spec.adaptation_smoothing_passes == 25
 
 
 def test_cell_weighting_switch_is_read(tmp_path):
@@ -175,7 +192,7 @@ def test_missing_files_are_io_errors(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir(REF_TUT), reason="reference tutorials not present")
 def test_reference_tutorials_report_exactly_their_unsupported_switches():
-    common = {"uniGasProperties": ["collisionProperties.macroInterpolation", "adaptiveSimulation"]}
+    common = {"uniGasProperties": ["collisionProperties.macroInterpolation"]}
     expect = {
         "expansionInVacuum": {"uniGasProperties": common["uniGasProperties"] + ["axisymmetricSimulation"],
                               "fieldPropertiesDict": ["fieldModel uniGasMassFluxSurface"],
@@ -203,6 +220,10 @@ def test_reference_tutorials_report_exactly_their_unsupported_switches():
     inp = case_dir.scan_case(os.path.join(REF_TUT, "supersonicPlate"))
     assert inp.spec.mesh.n_cells == 45000
     assert inp.spec.collision_model == "hybrid" and inp.spec.bgk_model == "unifiedStochasticParticleSBGK"
+    # its adaptive properties are read (uniGasProperties:57-66 of the tutorial)
+    sp = inp.spec
+    assert sp.adaptive and sp.time_step_adaptation and sp.subcell_adaptation and sp.cell_weight_adaptation
+    assert sp.cell_weighted and sp.adaptation_interval == 10 and sp.max_subcell_size_mfp_ratio == 0.5
     assert np.all(core.geometry(inp.spec)["vol"] > 0)
 
 

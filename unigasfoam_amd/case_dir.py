@@ -13,7 +13,9 @@ word onto the ABI's enums:
                                      dsmcCollisionPartnerModel, collisionProperties
                                      (LarsenBorgnakkeVariableHardSphere.C:59-66,
                                      stochasticParticleBGK.C:53-54,
-                                     unifiedStochasticParticleSBGK.C:53-58)
+                                     unifiedStochasticParticleSBGK.C:53-58),
+                                     cellWeightedProperties, adaptiveProperties
+                                     (dynamicAdaptation/uniGasDynamicAdapter.C:187-217)
   system/controlDict                 deltaT, endTime, writeInterval, nTerminalOutputs
   system/boundariesDict              uniGasBoundaries (boundaries/basic/uniGasBoundaries/
                                      uniGasBoundaries.C:33-556): patch, general and cyclic models
@@ -23,9 +25,9 @@ word onto the ABI's enums:
   constant/polyMesh or system/blockMeshDict   the mesh (foam_io.read_polymesh, blockmesh.build)
 
 Defaults are the reference's (``getOrDefault`` values of the cited lines).
-A switch or model this build does not run (cell weighting, adaptivity,
-axisymmetry, chemistry, charged species, vibrational or electronic
-exchange, macroInterpolation, other model names) is collected, and
+A switch or model this build does not run (axisymmetry, chemistry,
+charged species, vibrational or electronic exchange, macroInterpolation,
+other model names) is collected, and
 ``read_case`` raises ``Unsupported`` (code ``abi.ERR_UNSUPPORTED``, the
 C ABI's UGX_ERR_UNSUPPORTED) listing every one of them; ``scan_case``
 returns the list without raising.
@@ -86,7 +88,6 @@ PATCH_MODELS = {"uniGasSpecularWallPatch": abi.BND_SPECULAR, "uniGasDiffuseWallP
 GENERAL_MODELS = {"uniGasFreeStreamInflowPatch": abi.BND_FREESTREAM}
 COLLISION_WORDS = ("dsmc", "bgk", "hybrid")
 _UNSUPPORTED_SWITCHES = (
-    ("adaptiveSimulation", "dynamic adaptation: subcells, time step, cell weights (uniGasDynamicAdapter.C)"),
     ("axisymmetricSimulation", "axisymmetric radial weighting"),
     ("chemicalReactions", "chemistry"),
     ("chargedParticles", "charged species"),
@@ -167,6 +168,20 @@ def _model(props, spec_kw, bad):
         spec_kw["cell_weighted"] = True
         spec_kw["particles_per_subcell"] = cw.label("particlesPerSubCell")
         spec_kw["min_particles_per_subcell"] = cw.label("minParticlesPerSubCell", spec_kw["particles_per_subcell"])
+    # dynamic adaptation (uniGasCloud.C:416, uniGasDynamicAdapter.C:187-217)
+    if "adaptiveSimulation" in props and as_switch(props["adaptiveSimulation"], "adaptiveSimulation"):
+        ad = props.sub_or_empty("adaptiveProperties")
+        spec_kw["adaptive"] = True
+        for key, kw in (("timeStepAdaptation", "time_step_adaptation"), ("subCellAdaptation", "subcell_adaptation"),
+                        ("cellWeightAdaptation", "cell_weight_adaptation")):
+            spec_kw[kw] = as_switch(ad.get(key, False), key)
+        spec_kw["adaptation_interval"] = ad.label("adaptationInterval", 20)
+        spec_kw["adaptation_smoothing_passes"] = ad.label("smoothingPasses", 25)
+        spec_kw["max_time_step_mct_ratio"] = ad.scalar("maxTimeStepMCTRatio", 0.2)
+        spec_kw["max_courant_number"] = ad.scalar("maxCourantNumber", 0.5)
+        spec_kw["max_subcell_size_mfp_ratio"] = ad.scalar("maxSubCellSizeMFPRatio", 0.5)
+        if spec_kw["cell_weight_adaptation"] and not spec_kw.get("cell_weighted"):
+            bad.append(("uniGasProperties", "adaptiveProperties.cellWeightAdaptation without cellWeightedSimulation"))
     for key, what in _UNSUPPORTED_SWITCHES:
         if key in props and as_switch(props[key], key):
             bad.append(("uniGasProperties", f"{key} ({what})"))
