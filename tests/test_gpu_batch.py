@@ -52,6 +52,31 @@ def test_served_cases_match_plain_solver(require_gpu, precision):
         assert_same(served.results[r.rid], alone.results[r.rid], f"case {r.rid} ({precision})")
 
 
+def test_served_to_a_statistical_error(require_gpu):
+    """Requests that sample until the density error estimate
+    1/sqrt(N n_steps) (uniGasVolFields.C:1242-1252) is at most 4% in every
+    cell: each stops at the first check where that holds, the same step as
+    the plain solver with the same criterion, with identical results."""
+    reqs = []
+    for i in range(4):
+        spec, fill = workloads.couette(n_cells=20 + 5 * i, per_cell=15 + 10 * i, seed=200 + i)
+        reqs.append(make_request(i, spec, fill, 15 + 10 * i, warmup=5, error_target=0.04, check_every=3,
+                                 max_sample_steps=300))
+    alone = run_alone(reqs, precision="fp64", use_graphs=False)
+    srv = Server(precision="fp64", max_cases=2, chunk=8, use_graphs=False, realtime=False)
+    served = srv.run(reqs)
+    srv.close()
+    for r in reqs:
+        a, b = served.results[r.rid], alone.results[r.rid]
+        assert_same(a, b, f"case {r.rid}")
+        n = a.counters["n_sampled_steps"]
+        assert 0 < n < 300 and n % 3 == 0
+        # the criterion holds at the stop and did not at the check before it
+        N = a.samples[0] / a.counters["time_sampled"]       # mean simulators per cell
+        assert (1.0 / np.sqrt(N * n)).max() <= 0.04
+        assert (1.0 / np.sqrt(N * (n - 3))).max() > 0.04 * 0.995 or n == 3
+
+
 def test_batch_graph_replays_match_eager(require_gpu):
     """Graph replays with eager work and synchronisation between them give
     the same state as eager launches, step after step."""

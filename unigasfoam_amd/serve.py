@@ -9,9 +9,14 @@ beside it (every random stream and per-cell sum is case-local), which
 tests/test_gpu_batch.py checks against the plain solver.
 
 Convergence: after `warmup` steps the samplers are cleared, then the case
-samples until every cell holds at least `samples_per_cell` simulator
-samples on average, i.e. ceil(samples_per_cell / simulators per cell)
-steps.  The reported latency of a request is completion time minus
+samples either a fixed number of steps (enough for `samples_per_cell`
+simulator samples per cell on average) or, with an `error_target`, until the
+statistical error estimate of the sampled fields (uniGasVolFields.C:
+1242-1252, after Hadjiconstantinou et al. 2003; `error_field` picks density,
+velocity, temperature or pressure) is at most the target in every cell that
+holds gas.  The error is checked every `check_every` sampled steps, from
+the write-time fields of the case, and sampling stops at `max_sample_steps`
+in any case.  The reported latency of a request is completion time minus
 arrival time; with the simulated clock (realtime=False), where requests are
 released early whenever the loop is idle, it counts from the release.
 """
@@ -35,12 +40,21 @@ class Request:
     fill: dict
     expected: int            # expected simulators
     warmup: int
-    sample_steps: int
+    sample_steps: int        # sampled steps; with an error target, the most taken
     arrival: float = 0.0     # seconds after the start of the run
+    error_target: float = 0.0    # > 0: sample until the error estimate is at most this
+    error_field: str = "density_error"
+    check_every: int = 10
 
     @property
     def total_steps(self):
         return self.warmup + self.sample_steps
+
+    def converged(self, fields):
+        """The error criterion on the write-time fields of the case."""
+        err = np.asarray(fields[self.error_field])
+        live = np.asarray(fields["rhoN"]) > 0.0
+        return bool(live.any()) and float(err[live].max()) <= self.error_target
 
 
 @dataclass
@@ -58,9 +72,16 @@ class Result:
         return self.done - self.arrival
 
 
-def make_request(rid, spec, fill, per_cell, warmup=50, samples_per_cell=2000, arrival=0.0):
+def make_request(rid, spec, fill, per_cell, warmup=50, samples_per_cell=2000, arrival=0.0, error_target=0.0,
+                 error_field="density_error", check_every=10, max_sample_steps=None):
+    """A request; with error_target > 0 it samples until that statistical
+    error is reached (at most max_sample_steps, by default four times the
+    fixed-count estimate)."""
     n = spec.mesh.n_cells * per_cell
-    return Request(rid, spec, fill, n, warmup, int(math.ceil(samples_per_cell / per_cell)), arrival)
+    steps = int(math.ceil(samples_per_cell / per_cell))
+    if error_target > 0.0:
+        steps = int(max_sample_steps) if max_sample_steps else 4 * steps
+    return Request(rid, spec, fill, n, warmup, steps, arrival, float(error_target), error_field, int(check_every))
 
 
 @dataclass
@@ -139,11 +160,11 @@ class Server:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
                 continue
             st.max_live = max(st.max_live, len(live))
-            # step to the next milestone of any live case (end of warmup or of sampling)
+            # step to the next milestone of any live case (end of warmup,
+            # an error check, or the end of sampling)
             k = self.chunk
             for r, done, _ in live.values():
-                m = r.warmup if done < r.warmup else r.total_steps
-                k = min(k, m - done)
+                k = min(k, next_milestone(r, done) - done)
             tb = time.perf_counter()
             self.batch.step(k)
             for v in live.values():
@@ -154,6 +175,9 @@ class Server:
                     self.batch.case(slot).reset_samples()
                 if done >= r.total_steps:
                     finished.append(slot)
+                elif r.error_target > 0.0 and done > r.warmup and (done - r.warmup) % r.check_every == 0:
+                    if r.converged(self.batch.case(slot).fields()):
+                        finished.append(slot)
             if finished:
                 self.batch.sync()
             for slot in finished:
@@ -169,6 +193,15 @@ class Server:
         return st
 
 
+def next_milestone(r: Request, done: int) -> int:
+    """The next step count at which the loop must look at request r."""
+    if done < r.warmup:
+        return r.warmup
+    if r.error_target > 0.0:
+        return min(r.total_steps, done + r.check_every - (done - r.warmup) % r.check_every)
+    return r.total_steps
+
+
 def run_alone(requests, precision="fp32", use_graphs=True):
     """The same requests one at a time through the plain solver."""
     st = ServeStats()
@@ -180,7 +213,13 @@ def run_alone(requests, precision="fp32", use_graphs=True):
         c.step(r.warmup)
         if r.sample_steps > 0:
             c.reset_samples()
-        c.step(r.sample_steps)
+        done = r.warmup
+        while done < r.total_steps:
+            nxt = next_milestone(r, done)
+            c.step(nxt - done)
+            done = nxt
+            if r.error_target > 0.0 and done < r.total_steps and r.converged(c.fields()):
+                break
         c.sync()
         cnt = c.counters()
         st.results[r.rid] = Result(r.rid, c.samples(), cnt, 0.0, tb - t0, time.perf_counter() - t0, c.wall_samples())

@@ -132,8 +132,8 @@ def nozzle_mesh(nx, ny, nz, L=0.02, h_in=4.0e-3, h_throat=1.0e-3, h_out=6.0e-3):
 def nozzle(nx=200, ny=100, nz=100, per_cell=50, n_stag=2.4e25, T0=300.0, n_back=2.4e20, dt=1.0e-8,
            collision_model="hybrid", bgk_model="unifiedStochasticParticleSBGK", seed=4):
     """Argon nozzle expansion with per-cell DSMC/USP switching.  Uniform
-    simulator weight (cell weighting is out of scope), so the fill uses one
-    stagnation-side density everywhere and the inlet keeps feeding it."""
+    simulator weight (the case does not turn on cell weighting), so the fill
+    uses one stagnation-side density everywhere and the inlet keeps feeding it."""
     mesh = nozzle_mesh(nx, ny, nz)
     n_cells = nx * ny * nz
     # volume of the channel
