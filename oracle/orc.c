@@ -34,6 +34,12 @@
 
 #define KB UGX_KB
 #define PI_D 3.14159265358979323846
+/* macroInterpolation: field rows (T, U, Pr, collision frequency, p, q, then
+ * the pressure tensor (ES-BGK) or stress (USP) as xx xy xz yy yz zz), points
+ * kept per face, and the tolerance of the tet search */
+#define MIR 16
+#define MI_FV 8
+#define MI_TOL 1.0e-12
 #define VSMALL 1.0e-300
 #define SMALL 1.0e-15
 #define MAXF UGX_MAX_CELL_FACES
@@ -148,6 +154,17 @@ struct orc_world {
     double* adacc;    /* [nc*KNR] adapter time sums since the last adaptation */
     double* adratio;  /* [nc*3] cell size to mean free path ratio of the last adaptation */
     double adapt_time;
+    /* macroInterpolation */
+    int macro;
+    int npts;
+    double* mi_cent;   /* [nc*3] OpenFOAM cell centroid in the cell frame */
+    int32_t* mi_fp;    /* [nc*6*8] point ids of the cell's faces (ascending face order), -1 padded */
+    double* mi_pts;    /* [npts*3] */
+    uint32_t* mi_poff; /* [npts+1] */
+    int32_t* mi_pcell;
+    double* mi_pw;
+    double* mfield;    /* [nc*16] */
+    double* pfield;    /* [npts*16] */
     /* counters */
     ugx_counters cnt;
 };
@@ -194,6 +211,243 @@ static void face_geom(const ugx_mesh_desc* m, int f, double pc[3], double A[3])
         for (int d = 0; d < 3; ++d) a[d] = a[d] + c[d];
     }
     for (int d = 0; d < 3; ++d) A[d] = 0.5 * a[d];
+}
+
+/* The geometry of the cellPoint interpolation (OpenFOAM interpolationCellPoint
+ * with volPointInterpolation): face centroids and areas of the fan about the
+ * vertex mean, cell centroids of the face pyramids about the mean of the face
+ * centroids (owner faces, then neighbour faces), per cell its faces' points in
+ * ascending face order, and per point the inverse-distance weights of the cells
+ * around it, or for a point on a wall, patch or symmetry face of those faces'
+ * owner values; points joined by cyclic patches share one normalised set. */
+static int find_root(int* up, int a)
+{
+    while (up[a] != a) { up[a] = up[up[a]]; a = up[a]; }
+    return a;
+}
+
+static double pdist(const double* a, const double* b)
+{
+    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+static int cmp_int(const void* a, const void* b)
+{
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+static int build_interp(orc_world* w, const ugx_mesh_desc* m)
+{
+    int nc = m->n_cells, nf = m->n_faces, np = m->n_points, ni = m->n_internal_faces;
+    const double* P = m->points;
+    double* fc = calloc((size_t)nf * 3, sizeof(double));
+    double* fa = calloc((size_t)nf * 3, sizeof(double));
+    for (int f = 0; f < nf; ++f) {
+        int b = m->face_offsets[f], n = m->face_offsets[f + 1] - b;
+        const int32_t* fp = &m->face_points[b];
+        double* C = &fc[3 * f];
+        double* A = &fa[3 * f];
+        if (n == 3) {
+            double u[3], v[3];
+            for (int d = 0; d < 3; ++d) C[d] = (1.0 / 3.0) * (P[3 * fp[0] + d] + P[3 * fp[1] + d] + P[3 * fp[2] + d]);
+            for (int d = 0; d < 3; ++d) { u[d] = P[3 * fp[1] + d] - P[3 * fp[0] + d]; v[d] = P[3 * fp[2] + d] - P[3 * fp[0] + d]; }
+            cross3(u, v, A);
+            for (int d = 0; d < 3; ++d) A[d] = 0.5 * A[d];
+            continue;
+        }
+        double mean[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) mean[d] = mean[d] + P[3 * fp[i] + d];
+        for (int d = 0; d < 3; ++d) mean[d] = mean[d] / (double)n;
+        double sumN[3] = {0.0, 0.0, 0.0}, sumA = 0.0, sumAc[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < n; ++i) {
+            const double* p0 = &P[3 * fp[i]];
+            const double* p1 = &P[3 * fp[(i + 1) % n]];
+            double c[3], u[3], v[3], nn[3];
+            for (int d = 0; d < 3; ++d) { c[d] = p0[d] + p1[d] + mean[d]; u[d] = p1[d] - p0[d]; v[d] = mean[d] - p0[d]; }
+            cross3(u, v, nn);
+            double a = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            for (int d = 0; d < 3; ++d) { sumN[d] = sumN[d] + nn[d]; sumAc[d] = sumAc[d] + a * c[d]; }
+            sumA = sumA + a;
+        }
+        if (sumA < 1.0e-150) {
+            for (int d = 0; d < 3; ++d) { C[d] = mean[d]; A[d] = 0.0; }
+        } else {
+            for (int d = 0; d < 3; ++d) { C[d] = (1.0 / 3.0) * sumAc[d] / sumA; A[d] = 0.5 * sumN[d]; }
+        }
+    }
+    double* cest = calloc((size_t)nc * 3, sizeof(double));
+    double* cctr = calloc((size_t)nc * 3, sizeof(double));
+    double* cvol = calloc((size_t)nc, sizeof(double));
+    int* nfc = calloc((size_t)nc, sizeof(int));
+    for (int side = 0; side < 2; ++side)
+        for (int f = 0; f < (side == 0 ? nf : ni); ++f) {
+            int c = side == 0 ? m->owner[f] : m->neighbour[f];
+            for (int d = 0; d < 3; ++d) cest[3 * c + d] = cest[3 * c + d] + fc[3 * f + d];
+            nfc[c] += 1;
+        }
+    for (int c = 0; c < nc; ++c)
+        for (int d = 0; d < 3; ++d) cest[3 * c + d] = cest[3 * c + d] / (double)nfc[c];
+    for (int side = 0; side < 2; ++side)
+        for (int f = 0; f < (side == 0 ? nf : ni); ++f) {
+            int c = side == 0 ? m->owner[f] : m->neighbour[f];
+            const double* C = &fc[3 * f];
+            const double* A = &fa[3 * f];
+            const double* E = &cest[3 * c];
+            double pyr = side == 0 ? A[0] * (C[0] - E[0]) + A[1] * (C[1] - E[1]) + A[2] * (C[2] - E[2])
+                                   : A[0] * (E[0] - C[0]) + A[1] * (E[1] - C[1]) + A[2] * (E[2] - C[2]);
+            if (pyr < 1.0e-300) pyr = 1.0e-300;
+            for (int d = 0; d < 3; ++d) cctr[3 * c + d] = cctr[3 * c + d] + pyr * (0.75 * C[d] + 0.25 * E[d]);
+            cvol[c] = cvol[c] + pyr;
+        }
+    w->mi_cent = calloc((size_t)nc * 3, sizeof(double));
+    for (int c = 0; c < nc; ++c)
+        for (int d = 0; d < 3; ++d) {
+            double x = fabs(cvol[c]) > 1.0e-300 ? cctr[3 * c + d] / cvol[c] : cest[3 * c + d];
+            cctr[3 * c + d] = x;
+            w->mi_cent[3 * c + d] = x - w->cc[3 * c + d];
+        }
+    /* faces of each cell, and per point its faces and cells */
+    w->mi_fp = malloc(sizeof(int32_t) * (size_t)nc * MAXF * MI_FV);
+    for (size_t k = 0; k < (size_t)nc * MAXF * MI_FV; ++k) w->mi_fp[k] = -1;
+    int* nface = calloc((size_t)nc, sizeof(int));
+    int* pfn = calloc((size_t)np + 1, sizeof(int));
+    int* pcn = calloc((size_t)np + 1, sizeof(int));
+    for (int f = 0; f < nf; ++f)
+        for (int i = m->face_offsets[f]; i < m->face_offsets[f + 1]; ++i) {
+            pfn[m->face_points[i] + 1] += 1;
+            pcn[m->face_points[i] + 1] += (f < ni) ? 2 : 1;
+        }
+    for (int p = 0; p < np; ++p) { pfn[p + 1] += pfn[p]; pcn[p + 1] += pcn[p]; }
+    int* pf = malloc(sizeof(int) * (size_t)(pfn[np] > 0 ? pfn[np] : 1));
+    int* pcl = malloc(sizeof(int) * (size_t)(pcn[np] > 0 ? pcn[np] : 1));
+    int* pfi = calloc((size_t)np, sizeof(int));
+    int* pci = calloc((size_t)np, sizeof(int));
+    for (int f = 0; f < nf; ++f) {
+        int b = m->face_offsets[f], n = m->face_offsets[f + 1] - b;
+        if (n > MI_FV) return fail(UGX_ERR_MESH, "macroInterpolation: face with more than 8 points");
+        for (int side = 0; side < (f < ni ? 2 : 1); ++side) {
+            int c = side == 0 ? m->owner[f] : m->neighbour[f];
+            int j = nface[c]++;
+            for (int i = 0; i < n; ++i) w->mi_fp[((size_t)c * MAXF + j) * MI_FV + i] = m->face_points[b + i];
+        }
+        for (int i = 0; i < n; ++i) {
+            int p = m->face_points[b + i];
+            pf[pfn[p] + pfi[p]++] = f;
+            for (int side = 0; side < (f < ni ? 2 : 1); ++side)
+                pcl[pcn[p] + pci[p]++] = side == 0 ? m->owner[f] : m->neighbour[f];
+        }
+    }
+    /* each point's cells ascending, without repeats */
+    int* pcu = calloc((size_t)np, sizeof(int));
+    for (int p = 0; p < np; ++p) {
+        int* v = &pcl[pcn[p]];
+        int n = pcn[p + 1] - pcn[p], k = 0;
+        qsort(v, (size_t)n, sizeof(int), cmp_int);
+        for (int i = 0; i < n; ++i) if (i == 0 || v[i] != v[i - 1]) v[k++] = v[i];
+        pcu[p] = k;
+    }
+    int8_t* vface = calloc((size_t)nf, 1);
+    int8_t* ppoint = calloc((size_t)np, 1);
+    int* up = malloc(sizeof(int) * (size_t)np);
+    for (int p = 0; p < np; ++p) up[p] = p;
+    for (int pa = 0; pa < m->n_patches; ++pa) {
+        int t = m->patch_type[pa];
+        if (t == UGX_PATCH_EMPTY) continue;
+        if (t == UGX_PATCH_CYCLIC) {
+            int pb = m->patch_neighbour ? m->patch_neighbour[pa] : -1;
+            if (pb < pa) continue;
+            for (int i = 0; i < m->patch_size[pa]; ++i) {
+                int f0 = m->patch_start[pa] + i, f1 = m->patch_start[pb] + i;
+                double T[3];
+                for (int d = 0; d < 3; ++d) T[d] = fc[3 * f1 + d] - fc[3 * f0 + d];
+                double sz = sqrt(fa[3 * f0] * fa[3 * f0] + fa[3 * f0 + 1] * fa[3 * f0 + 1] + fa[3 * f0 + 2] * fa[3 * f0 + 2]);
+                for (int a = m->face_offsets[f0]; a < m->face_offsets[f0 + 1]; ++a) {
+                    int p0 = m->face_points[a], best = -1;
+                    double bd = 1.0e300;
+                    for (int q = m->face_offsets[f1]; q < m->face_offsets[f1 + 1]; ++q) {
+                        int p1 = m->face_points[q];
+                        double x[3];
+                        for (int d = 0; d < 3; ++d) x[d] = P[3 * p0 + d] + T[d];
+                        double dd = pdist(x, &P[3 * p1]);
+                        if (dd < bd) { bd = dd; best = p1; }
+                    }
+                    if (best < 0 || bd > 1.0e-6 * sqrt(sz)) return fail(UGX_ERR_MESH, "cyclic faces do not match point for point");
+                    int r0 = find_root(up, p0), r1 = find_root(up, best);
+                    if (r0 != r1) { if (r0 < r1) up[r1] = r0; else up[r0] = r1; }
+                }
+            }
+            continue;
+        }
+        for (int f = m->patch_start[pa]; f < m->patch_start[pa] + m->patch_size[pa]; ++f) {
+            vface[f] = 1;
+            for (int a = m->face_offsets[f]; a < m->face_offsets[f + 1]; ++a) ppoint[m->face_points[a]] = 1;
+        }
+    }
+    /* classes of joined points (members ascending), their weights */
+    int* root = malloc(sizeof(int) * (size_t)np);
+    int* mcount = calloc((size_t)np + 1, sizeof(int));
+    for (int p = 0; p < np; ++p) { root[p] = find_root(up, p); mcount[root[p] + 1] += 1; }
+    for (int p = 0; p < np; ++p) mcount[p + 1] += mcount[p];
+    int* mem = malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+    int* mfill = calloc((size_t)np, sizeof(int));
+    for (int p = 0; p < np; ++p) mem[mcount[root[p]] + mfill[root[p]]++] = p;
+    /* entries per class */
+    int* cls_n = calloc((size_t)np, sizeof(int));
+    int total = 0;
+    for (int r = 0; r < np; ++r) {
+        int n0 = mcount[r], n1 = mcount[r + 1];
+        if (n1 == n0) continue;
+        int patch = 0;
+        for (int k = n0; k < n1; ++k) patch = patch || ppoint[mem[k]];
+        int cnt = 0;
+        for (int k = n0; k < n1; ++k) {
+            int p = mem[k];
+            if (patch) { for (int i = pfn[p]; i < pfn[p + 1]; ++i) if (vface[pf[i]]) cnt++; }
+            else cnt += pcu[p];
+        }
+        cls_n[r] = cnt;
+    }
+    w->mi_poff = calloc((size_t)np + 1, sizeof(uint32_t));
+    for (int p = 0; p < np; ++p) w->mi_poff[p + 1] = w->mi_poff[p] + (uint32_t)cls_n[root[p]];
+    total = (int)w->mi_poff[np];
+    w->mi_pcell = malloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1));
+    w->mi_pw = malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+    int32_t* ec = malloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1));
+    double* ew = malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+    for (int r = 0; r < np; ++r) {
+        int n0 = mcount[r], n1 = mcount[r + 1];
+        if (n1 == n0) continue;
+        int patch = 0, e = 0;
+        for (int k = n0; k < n1; ++k) patch = patch || ppoint[mem[k]];
+        for (int k = n0; k < n1; ++k) {
+            int p = mem[k];
+            if (patch) {
+                for (int i = pfn[p]; i < pfn[p + 1]; ++i)
+                    if (vface[pf[i]]) { ec[e] = m->owner[pf[i]]; ew[e] = 1.0 / pdist(&P[3 * p], &fc[3 * pf[i]]); e++; }
+            } else {
+                for (int i = 0; i < pcu[p]; ++i) { int c = pcl[pcn[p] + i]; ec[e] = c; ew[e] = 1.0 / pdist(&P[3 * p], &cctr[3 * c]); e++; }
+            }
+        }
+        double sum = 0.0;
+        for (int k = 0; k < e; ++k) sum = sum + ew[k];
+        for (int k = 0; k < e; ++k) ew[k] = ew[k] / sum;
+        for (int k = n0; k < n1; ++k) {
+            int p = mem[k];
+            for (int q = 0; q < e; ++q) { w->mi_pcell[w->mi_poff[p] + q] = ec[q]; w->mi_pw[w->mi_poff[p] + q] = ew[q]; }
+        }
+    }
+    w->npts = np;
+    w->mi_pts = malloc(sizeof(double) * (size_t)np * 3);
+    memcpy(w->mi_pts, P, sizeof(double) * (size_t)np * 3);
+    w->mfield = calloc((size_t)nc * MIR, sizeof(double));
+    w->pfield = calloc((size_t)np * MIR, sizeof(double));
+    void* tmp[] = {fc, fa, cest, cctr, cvol, nfc, nface, pfn, pcn, pf, pcl, pfi, pci, pcu, vface, ppoint, up, root,
+                   mcount, mem, mfill, cls_n, ec, ew};
+    for (size_t i = 0; i < sizeof tmp / sizeof tmp[0]; ++i) free(tmp[i]);
+    return UGX_OK;
 }
 
 static int build_mesh(orc_world* w, const ugx_mesh_desc* m, const ugx_boundary_desc* bnd, int nb)
@@ -985,8 +1239,77 @@ int orc_usp_sample(double cQ, double cS, double p, double u0, const double* q, c
  *    start of the step; a draw with maxProb < P < 10 is accepted and the
  *    largest such P becomes the new maxProb (the value the reference's
  *    sequential raises end on); without a raise maxProb decays by 0.999. */
+/* ---------------------------------------------------- macroInterpolation
+ * interpolationCellPoint (OpenFOAM) of the BGK-family fields at a
+ * simulator's position: linear on the tets (cell centroid, then per face in
+ * ascending order the fan (p0, p_i, p_i+1)), the cell's value at the
+ * centroid and point values at the points. */
+
+static void mi_interp(const orc_world* w, int c, const double x[3], int nr, double* out)
+{
+    const double* ref = &w->cc[3 * c];
+    double a[3], r[3];
+    for (int d = 0; d < 3; ++d) { a[d] = w->mi_cent[3 * c + d]; r[d] = x[d] - a[d]; }
+    double best = -1.0e300, bl[4] = {1.0, 0.0, 0.0, 0.0};
+    int32_t bid[3] = {-1, -1, -1};
+    int found = 0;
+    for (int j = 0; j < MAXF && !found; ++j) {
+        const int32_t* fp = &w->mi_fp[((size_t)c * MAXF + j) * MI_FV];
+        if (fp[0] < 0) break;
+        int nv = 3;
+        while (nv < MI_FV && fp[nv] >= 0) ++nv;
+        double e1[3];
+        for (int d = 0; d < 3; ++d) e1[d] = (w->mi_pts[3 * (size_t)fp[0] + d] - ref[d]) - a[d];
+        for (int t = 1; t + 1 < nv; ++t) {
+            double e2[3], e3[3];
+            for (int d = 0; d < 3; ++d) {
+                e2[d] = (w->mi_pts[3 * (size_t)fp[t] + d] - ref[d]) - a[d];
+                e3[d] = (w->mi_pts[3 * (size_t)fp[t + 1] + d] - ref[d]) - a[d];
+            }
+            double n23[3] = {e2[1] * e3[2] - e2[2] * e3[1], e2[2] * e3[0] - e2[0] * e3[2], e2[0] * e3[1] - e2[1] * e3[0]};
+            double det = e1[0] * n23[0] + e1[1] * n23[1] + e1[2] * n23[2];
+            if (det == 0.0) continue;
+            double rx3[3] = {r[1] * e3[2] - r[2] * e3[1], r[2] * e3[0] - r[0] * e3[2], r[0] * e3[1] - r[1] * e3[0]};
+            double e2r[3] = {e2[1] * r[2] - e2[2] * r[1], e2[2] * r[0] - e2[0] * r[2], e2[0] * r[1] - e2[1] * r[0]};
+            double l1 = (r[0] * n23[0] + r[1] * n23[1] + r[2] * n23[2]) / det;
+            double l2 = (e1[0] * rx3[0] + e1[1] * rx3[1] + e1[2] * rx3[2]) / det;
+            double l3 = (e1[0] * e2r[0] + e1[1] * e2r[1] + e1[2] * e2r[2]) / det;
+            double l0 = ((1.0 - l1) - l2) - l3;
+            double mn = fmin(fmin(l0, l1), fmin(l2, l3));
+            if (mn > best) {
+                best = mn;
+                bl[0] = l0; bl[1] = l1; bl[2] = l2; bl[3] = l3;
+                bid[0] = fp[0]; bid[1] = fp[t]; bid[2] = fp[t + 1];
+            }
+            if (mn >= -MI_TOL) { found = 1; break; }
+        }
+    }
+    const double* fc = &w->mfield[(size_t)MIR * c];
+    for (int q = 0; q < nr; ++q) {
+        double v = bl[0] * fc[q];
+        for (int k = 0; k < 3; ++k)
+            if (bid[k] >= 0) v = v + bl[k + 1] * w->pfield[(size_t)MIR * bid[k] + q];
+        out[q] = v;
+    }
+}
+
+/* volPointInterpolation of the step's cell fields to the points. */
+static void mi_point_fields(orc_world* w)
+{
+    for (int p = 0; p < w->npts; ++p) {
+        double v[MIR];
+        for (int q = 0; q < MIR; ++q) v[q] = 0.0;
+        for (uint32_t k = w->mi_poff[p]; k < w->mi_poff[p + 1]; ++k) {
+            double wt = w->mi_pw[k];
+            const double* f = &w->mfield[(size_t)MIR * w->mi_pcell[k]];
+            for (int q = 0; q < MIR; ++q) v[q] = v[q] + wt * f[q];
+        }
+        for (int q = 0; q < MIR; ++q) w->pfield[(size_t)MIR * p + q] = v[q];
+    }
+}
+
 static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[NSAMP],
-                     uint32_t* idx, int64_t* nrel_out, int update_state)
+                     uint32_t* idx, int64_t* nrel_out, int update_state, int fields_only)
 {
     int model = w->model.bgk_model;
     double FN = w->model.n_equivalent_particles * w->cw[c], dt = w->model.delta_t, V = w->vol[c];
@@ -1046,14 +1369,28 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
     } else {
         perform = 0;
     }
-    if (usp && update_state) {
+    if (usp && (update_state || fields_only)) {
         double th = w->model.theta;
         for (int d = 0; d < 3; ++d)
             q[d] = th * q[d] / (1.0 + 0.5 * Pr * nu * dt) + (1.0 - th) * w->uspq[3 * c + d];
         for (int k = 0; k < 6; ++k)
             sh[k] = th * sh[k] / (1.0 + 0.5 * nu * dt) + (1.0 - th) * w->usps[6 * c + k];
-        for (int d = 0; d < 3; ++d) w->uspq[3 * c + d] = q[d];
-        for (int k = 0; k < 6; ++k) w->usps[6 * c + k] = sh[k];
+        if (!fields_only) {
+            for (int d = 0; d < 3; ++d) w->uspq[3 * c + d] = q[d];
+            for (int k = 0; k < 6; ++k) w->usps[6 * c + k] = sh[k];
+        }
+    }
+    if (fields_only) {
+        /* the cell's fields for the cellPoint interpolation (every cell) */
+        double* mf = &w->mfield[(size_t)MIR * c];
+        mf[0] = T;
+        for (int d = 0; d < 3; ++d) mf[1 + d] = U[d];
+        mf[4] = Pr;
+        mf[5] = nu;
+        mf[6] = p;
+        for (int d = 0; d < 3; ++d) mf[7 + d] = usp ? q[d] : 0.0;
+        for (int k = 0; k < 6; ++k) mf[10 + k] = usp ? sh[k] : Pt[k];
+        return;
     }
     const double m0 = w->uspmax[c];
     double raise = 0.0;
@@ -1071,7 +1408,7 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
             uint32_t j = (uint32_t)i + orc_index(&r, (uint32_t)(nC - i));
             uint32_t t = idx[i]; idx[i] = idx[j]; idx[j] = t;
         }
-        double Smat[9];
+        double Smat[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (model == UGX_BGK_ESBGK) {
             double a = 0.5 * (1.0 - Pr) / Pr;
             const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
@@ -1089,21 +1426,53 @@ static void bgk_cell(orc_world* w, int c, int64_t b, int64_t nC, const double S[
             int64_t pi = b + idx[i];
             orc_rng rv;
             orc_rng_init(&rv, w->seed, ORC_RNG_BGK_V, (uint32_t)i, (uint32_t)c, w->step);
-            double u0 = s2kT / sqrt(w->mass[w->sp[pi]]);
+            /* the draw's parameters: the cell's, or interpolated at the
+             * simulator (stochasticParticleBGK.C:680-687,
+             * stochasticParticleESBGK.C:779-789, unifiedStochasticParticleSBGK.C:933-946) */
+            double dU[3] = {U[0], U[1], U[2]}, dS2kT = s2kT, dp = p, dcQ = cQ, dcS = cS, dq[3] = {q[0], q[1], q[2]};
+            double dsh[6], dSm[9];
+            for (int k = 0; k < 6; ++k) dsh[k] = sh[k];
+            for (int k = 0; k < 9; ++k) dSm[k] = Smat[k];
+            if (w->macro) {
+                double f[MIR];
+                mi_interp(w, c, &w->x[3 * pi], model == UGX_BGK_BGK ? 4 : MIR, f);
+                for (int d = 0; d < 3; ++d) dU[d] = f[1 + d];
+                dS2kT = sqrt(2.0 * KB * f[0]);
+                if (model == UGX_BGK_ESBGK) {
+                    double iPr = f[4];
+                    dp = f[6];
+                    double a = 0.5 * (1.0 - iPr) / iPr;
+                    const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+                    for (int k = 0; k < 9; ++k) {
+                        double dlt = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+                        dSm[k] = dlt - a * (f[10 + full[k]] / dp - dlt);
+                    }
+                } else if (model == UGX_BGK_USP) {
+                    double iPr = f[4], inu = f[5];
+                    dp = f[6];
+                    for (int d = 0; d < 3; ++d) dq[d] = f[7 + d];
+                    for (int k = 0; k < 6; ++k) dsh[k] = f[10 + k];
+                    double itau = 0.5 * inu * dt;
+                    double iex = 1.0 + 2.0 / (orc_exp(2.0 * itau) - 1.0);
+                    dcQ = 2.0 * (1.0 - iPr * itau * iex);
+                    dcS = 1.0 - itau * iex;
+                }
+            }
+            double u0 = dS2kT / sqrt(w->mass[w->sp[pi]]);
             double xi[3];
             if (model == UGX_BGK_USP) {
-                usp_draw(&rv, cQ, cS, p, u0, q, sh, m0, &raise, xi);
+                usp_draw(&rv, dcQ, dcS, dp, u0, dq, dsh, m0, &raise, xi);
             } else {
                 double g[3];
                 orc_gauss3(&rv, g);
                 for (int d = 0; d < 3; ++d) xi[d] = g[d] * ORC_SQRT_HALF;
                 if (model == UGX_BGK_ESBGK) {
                     double y[3];
-                    for (int d = 0; d < 3; ++d) y[d] = Smat[3 * d] * xi[0] + Smat[3 * d + 1] * xi[1] + Smat[3 * d + 2] * xi[2];
+                    for (int d = 0; d < 3; ++d) y[d] = dSm[3 * d] * xi[0] + dSm[3 * d + 1] * xi[1] + dSm[3 * d + 2] * xi[2];
                     for (int d = 0; d < 3; ++d) xi[d] = y[d];
                 }
             }
-            for (int d = 0; d < 3; ++d) w->v[3 * pi + d] = U[d] + u0 * xi[d];
+            for (int d = 0; d < 3; ++d) w->v[3 * pi + d] = dU[d] + u0 * xi[d];
         }
         /* conserveMomentumAndEnergy (sums in canonical order) */
         double ke, mom[3];
@@ -1177,6 +1546,13 @@ int orc_create(const ugx_mesh_desc* mesh, const ugx_species_desc* species, int n
         }
     int rc = build_mesh(w, mesh, bnd, n_bnd);
     if (rc) { orc_destroy(w); return rc; }
+    /* macroInterpolation reaches the BGK-family draws only */
+    w->macro = model->macro_interpolation && model->bgk_model != UGX_BGK_NONE
+               && (model->collision_model == UGX_COLL_BGK || model->collision_model == UGX_COLL_HYBRID);
+    if (w->macro) {
+        rc = build_interp(w, mesh);
+        if (rc) { orc_destroy(w); return rc; }
+    }
     int nc = w->nc;
     w->cap = capacity > 0 ? capacity : 1024;
     w->x = xcalloc((size_t)w->cap * 3, sizeof(double));
@@ -1228,7 +1604,8 @@ void orc_destroy(orc_world* w)
                     w->x, w->v, w->erot, w->cell, w->sp, w->x2, w->v2, w->erot2, w->cell2, w->sp2,
                     w->off, w->smax, w->samp, w->cmodel, w->uspq, w->usps, w->uspmax, w->knacc, w->kn,
                     w->wface, w->wgeo, w->wsamp, w->ev, w->wcell, w->cw, w->cstart, w->sclev,
-                    w->adacc, w->adratio};
+                    w->adacc, w->adratio, w->mi_cent, w->mi_fp, w->mi_pts, w->mi_poff, w->mi_pcell, w->mi_pw,
+                    w->mfield, w->pfield};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(w);
 }
@@ -1811,6 +2188,23 @@ int orc_adapt_inject(orc_world* w, const double* acc, double tav)
     return adapt(w);
 }
 
+/* Test hook: cell fields [nc*16] set, point values computed, then the
+ * cellPoint interpolation at n positions (global coordinates) in the given
+ * cells into out [n*16]. */
+int orc_mi_probe(orc_world* w, const double* fields, int64_t n, const double* x, const int32_t* cells, double* out)
+{
+    if (!w->macro) return fail(UGX_ERR_STATE, "macroInterpolation is off");
+    memcpy(w->mfield, fields, sizeof(double) * (size_t)w->nc * MIR);
+    mi_point_fields(w);
+    for (int64_t i = 0; i < n; ++i) {
+        int c = cells[i];
+        double xl[3];
+        for (int d = 0; d < 3; ++d) xl[d] = x[3 * i + d] - w->cc[3 * c + d];
+        mi_interp(w, c, xl, MIR, &out[MIR * i]);
+    }
+    return UGX_OK;
+}
+
 int orc_smooth_max_ratio(orc_world* w, double* f, double max_ratio)
 {
     smooth_max_ratio(w, f, max_ratio);
@@ -1915,6 +2309,19 @@ int orc_step_phases(orc_world* w, int mask)
         int64_t cand = 0, coll = 0, nrel = 0;
         uint32_t* idx = malloc(sizeof(uint32_t) * (size_t)(w->n > 0 ? w->n : 1));
         int hybrid = (cm == UGX_COLL_HYBRID);
+        if (w->macro) {
+            /* every cell's fields, then the point values, before any draw
+             * (stochasticParticleBGK.C:638-648) */
+#pragma omp parallel for schedule(dynamic, 64)
+            for (int c = 0; c < w->nc; ++c) {
+                int64_t b = w->off[c], nC = w->off[c + 1] - w->off[c];
+                double S[NSAMP];
+                cell_sums(w, b, b + nC, S);
+                int64_t nr0 = 0;
+                bgk_cell(w, c, b, nC, S, idx + b, &nr0, 0, 1);
+            }
+            mi_point_fields(w);
+        }
 #pragma omp parallel for reduction(+:cand, coll, nrel) schedule(dynamic, 64)
         for (int c = 0; c < w->nc; ++c) {
             int64_t b = w->off[c], nC = w->off[c + 1] - w->off[c];
@@ -1936,7 +2343,7 @@ int orc_step_phases(orc_world* w, int mask)
             if ((cm == UGX_COLL_DSMC || hybrid) && w->cmodel[c] == 1 && w->model.dsmc_model != UGX_DSMC_NONE)
                 ntc_cell(w, c, b, nC, &cand, &coll);
             if ((cm == UGX_COLL_BGK || hybrid) && w->model.bgk_model != UGX_BGK_NONE)
-                bgk_cell(w, c, b, nC, S, idx + b, &nrel, 1);
+                bgk_cell(w, c, b, nC, S, idx + b, &nrel, 1, 0);
         }
         free(idx);
         if (sample) { w->cnt.n_sampled_steps += 1; w->cnt.time_sampled += dt; }

@@ -96,6 +96,8 @@ int orc_adapt_inject(orc_world* w, const double* acc, double tav);
 int orc_set_cell_model(orc_world* w, const int32_t* in, int64_t len);
 /* test hook: fvc::smooth(f, max_ratio - 1) in place, [nc] */
 int orc_smooth_max_ratio(orc_world* w, double* f, double max_ratio);
+/* test hook: cellPoint interpolation of cell fields [nc*16] at n global positions in cells[] */
+int orc_mi_probe(orc_world* w, const double* fields, int64_t n, const double* x, const int32_t* cells, double* out);
 int orc_wall_samples(orc_world* w, double* out, int64_t len);
 int orc_set_usp_state(orc_world* w, const double* q, const double* sh, const double* maxp);
 

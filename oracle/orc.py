@@ -103,6 +103,7 @@ def _bind(L):
     L.orc_adapt_inject.argtypes = [vp, abi.DBL_P, C.c_double]
     L.orc_set_cell_model.argtypes = [vp, abi.I32_P, C.c_int64]
     L.orc_smooth_max_ratio.argtypes = [vp, abi.DBL_P, C.c_double]
+    L.orc_mi_probe.argtypes = [vp, abi.DBL_P, C.c_int64, abi.DBL_P, abi.I32_P, abi.DBL_P]
 
 
 def _check(rc):
@@ -196,6 +197,16 @@ class OracleCase:
         """Sets the adapter's time sums [n_cells, 16] and their time, then adapts once."""
         acc = np.ascontiguousarray(acc, dtype=np.float64).reshape(self.nc, abi.KN_ROWS)
         _check(self.L.orc_adapt_inject(self.h, abi.dptr(acc), float(tav)))
+
+    def mi_probe(self, fields, x, cells):
+        """cellPoint interpolation of per-cell fields [n_cells, 16] at global
+        positions x [n, 3] lying in cells [n]; returns [n, 16]."""
+        fields = np.ascontiguousarray(fields, dtype=np.float64).reshape(self.nc, 16)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        out = np.zeros((len(x), 16))
+        _check(self.L.orc_mi_probe(self.h, abi.dptr(fields), len(x), abi.dptr(x), abi.iptr(cells), abi.dptr(out)))
+        return out
 
     def smooth_max_ratio(self, f, max_ratio):
         f = np.array(f, dtype=np.float64)

@@ -17,6 +17,7 @@ import pytest
 
 import test_oracle_physics as phys
 import test_oracle_adapt as adapt
+import test_oracle_macro as macro
 import test_oracle_pins as pins
 import test_oracle_subcells as subcells
 import test_oracle_wall as wall
@@ -28,6 +29,16 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fas
 
 # name: (snippet, replacement, pins that must fail)
 MUTANTS = {
+    # a tet weight dropped from the centroid's barycentric coordinate (interpolationCellPoint)
+    "macro_barycentric": (
+        "            double l0 = ((1.0 - l1) - l2) - l3;\n            double mn",
+        "            double l0 = (1.0 - l1) - l2;\n            double mn",
+        [lambda: macro.test_constant_field_is_reproduced(False), macro.test_linear_field_is_exact_in_interior_cells]),
+    # the draws with the cell's fields although macroInterpolation is on (stochasticParticleBGK.C:680-687)
+    "macro_draw_cell_values": (
+        "            if (w->macro) {\n                double f[MIR];",
+        "            if (0) {\n                double f[MIR];",
+        [lambda: macro.test_relaxed_draws_keep_the_local_shear(True)]),
     # weights blended with the weights swapped (uniGasDynamicAdapter.C:665)
     "adapt_weight_blend": (
         "w->cw[c] = theta * wn[c] + (1.0 - theta) * w->cw[c];",

@@ -26,8 +26,8 @@ word onto the ABI's enums:
 
 Defaults are the reference's (``getOrDefault`` values of the cited lines).
 A switch or model this build does not run (axisymmetry, chemistry,
-charged species, vibrational or electronic exchange, macroInterpolation,
-other model names) is collected, and
+charged species, vibrational or electronic exchange, other model names) is
+collected, and
 ``read_case`` raises ``Unsupported`` (code ``abi.ERR_UNSUPPORTED``, the
 C ABI's UGX_ERR_UNSUPPORTED) listing every one of them; ``scan_case``
 returns the list without raising.
@@ -155,9 +155,9 @@ def _model(props, spec_kw, bad):
         spec_kw["bgk_model"] = bm
         if bm == "unifiedStochasticParticleSBGK":
             spec_kw["theta"] = cp.scalar("theta", 1.0)
-        if bm != "noBGKCollision" and as_switch(cp.get("macroInterpolation", False), "macroInterpolation"):
-            bad.append(("uniGasProperties", "collisionProperties.macroInterpolation (cell-point interpolation, "
-                                            "stochasticParticleBGK.C:645-649)"))
+        # cellPoint interpolation of the fields at each relaxed simulator (stochasticParticleBGK.C:54, 645-688)
+        spec_kw["macro_interpolation"] = bm != "noBGKCollision" and as_switch(cp.get("macroInterpolation", False),
+                                                                             "macroInterpolation")
     else:
         spec_kw["bgk_model"] = "noBGKCollision"
     if spec_kw["dsmc_model"] != "noDSMCCollision" or spec_kw["bgk_model"] != "noBGKCollision":

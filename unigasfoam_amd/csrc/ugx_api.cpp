@@ -25,7 +25,6 @@ static int validate_model(const ugx_model_desc* m)
     if (m->dsmc_model < UGX_DSMC_NONE || m->dsmc_model > UGX_DSMC_LB_VSS) return set_err(UGX_ERR_ARG, "unknown dsmcCollisionModel");
     if (m->bgk_model < UGX_BGK_NONE || m->bgk_model > UGX_BGK_USP) return set_err(UGX_ERR_ARG, "unknown bgkCollisionModel");
     if (m->partner_model != UGX_PARTNER_NTC) return set_err(UGX_ERR_ARG, "unknown dsmcCollisionPartnerModel");
-    if (m->macro_interpolation) return set_err(UGX_ERR_UNSUPPORTED, "macroInterpolation is not supported (cell values only)");
     if ((m->dsmc_model != UGX_DSMC_NONE) && !(m->Tref > 0.0)) return set_err(UGX_ERR_ARG, "collisionProperties.Tref must be > 0");
     if ((m->dsmc_model == UGX_DSMC_LB_VHS || m->dsmc_model == UGX_DSMC_LB_VSS) && !(m->Zrot > 0.0))
         return set_err(UGX_ERR_ARG, "rotationalRelaxationCollisionNumber must be > 0");

@@ -295,7 +295,7 @@ struct CaseDev {
     double reset_until_time;     // <= 0: no limit
     double time;                 // simulated time at the start of the step
     int32_t adaptive;            // adaptive simulation: the adapter's time sums are kept every step
-    int32_t pad_adapt;
+    int32_t macro;               // macroInterpolation: BGK-family draws take cellPoint-interpolated fields
 };
 
 // Hybrid decomposition: time-sum rows per cell (N, M, M u, M v, M w, M |u|^2,

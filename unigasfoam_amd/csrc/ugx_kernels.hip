@@ -1991,6 +1991,17 @@ __device__ inline void bgk_params(const WorldView<R>& w, const CaseDev& cs, uint
         for (int d = 0; d < 3; ++d) w.uspq[3 * (size_t)c + d] = q[d];
         for (int k = 0; k < 6; ++k) w.usps[6 * (size_t)c + k] = sh[k];
     }
+    if (w.has_macro && cs.macro) {
+        // the cell's fields for the cellPoint interpolation, every cell of the case
+        double* mf = &w.mfield[(size_t)kMiRows * c];
+        mf[0] = T;
+        for (int d = 0; d < 3; ++d) mf[1 + d] = U[d];
+        mf[4] = Pr;
+        mf[5] = nu;
+        mf[6] = p;
+        for (int d = 0; d < 3; ++d) mf[7 + d] = usp ? q[d] : 0.0;
+        for (int k = 0; k < 6; ++k) mf[10 + k] = usp ? sh[k] : Pt[k];
+    }
     P.model = model;
     P.m0 = w.uspmax[c];
     P.perform = 0;
@@ -2149,10 +2160,11 @@ __global__ void __launch_bounds__(kBlock) k_collide(WorldView<R> w)
     case_add(w.counters, ccase, kCntBgk, (unsigned long long)nrel);
 }
 
-// New velocity of one selected simulator (slot i of cell c, selection k).
+// New velocity of one selected simulator (slot i of cell c, selection k)
+// from the parameters P.
 template <class R>
-__device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
-                                const BgkP& P, unsigned long long* raise, const GlobAcc<R>& A)
+__device__ inline void bgk_draw_core(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
+                                     const BgkP& P, unsigned long long* raise, const GlobAcc<R>& A)
 {
     Rng rv;
     rng_init(rv, cs.seed_lo, cs.seed_hi, kRngBgkV, k, c - (uint32_t)cs.cell_begin, cs.step);
@@ -2231,6 +2243,133 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
         }
     }
     A.store(i, P.U[0] + u0 * xi[0], P.U[1] + u0 * xi[1], P.U[2] + u0 * xi[2]);
+}
+
+// interpolationCellPoint at position x (cell frame) of cell c: the tet of
+// the cell's decomposition (centroid, then per face the fan (p0, p_i,
+// p_i+1), faces in ascending order) that holds x, or the one x is least
+// outside of, and its barycentric weights; rows 0..nr-1 of the cell's and
+// the tet points' fields combined with them.
+constexpr double kMiTol = 1.0e-12;
+
+template <class R>
+__device__ inline void mi_interp(const WorldView<R>& w, uint32_t c, const double x[3], int nr, double* out)
+{
+    const double* ref = &w.cc[3 * (size_t)c];
+    double a[3], r[3];
+    for (int d = 0; d < 3; ++d) { a[d] = w.mi_cent[3 * (size_t)c + d]; r[d] = x[d] - a[d]; }
+    double best = -1.0e300, bl[4] = {1.0, 0.0, 0.0, 0.0};
+    int32_t bid[3] = {-1, -1, -1};
+    bool found = false;
+    for (int j = 0; j < kMaxFaces && !found; ++j) {
+        const int32_t* fp = &w.mi_fp[((size_t)c * kMaxFaces + j) * kMiFaceVerts];
+        if (fp[0] < 0) break;
+        int nv = 3;
+        while (nv < kMiFaceVerts && fp[nv] >= 0) ++nv;
+        double e1[3];
+        for (int d = 0; d < 3; ++d) e1[d] = (w.mi_pts[3 * (size_t)fp[0] + d] - ref[d]) - a[d];
+        for (int t = 1; t + 1 < nv; ++t) {
+            double e2[3], e3[3];
+            for (int d = 0; d < 3; ++d) {
+                e2[d] = (w.mi_pts[3 * (size_t)fp[t] + d] - ref[d]) - a[d];
+                e3[d] = (w.mi_pts[3 * (size_t)fp[t + 1] + d] - ref[d]) - a[d];
+            }
+            const double n23[3] = {e2[1] * e3[2] - e2[2] * e3[1], e2[2] * e3[0] - e2[0] * e3[2], e2[0] * e3[1] - e2[1] * e3[0]};
+            const double det = e1[0] * n23[0] + e1[1] * n23[1] + e1[2] * n23[2];
+            if (det == 0.0) continue;
+            const double rx3[3] = {r[1] * e3[2] - r[2] * e3[1], r[2] * e3[0] - r[0] * e3[2], r[0] * e3[1] - r[1] * e3[0]};
+            const double e2r[3] = {e2[1] * r[2] - e2[2] * r[1], e2[2] * r[0] - e2[0] * r[2], e2[0] * r[1] - e2[1] * r[0]};
+            const double l1 = (r[0] * n23[0] + r[1] * n23[1] + r[2] * n23[2]) / det;
+            const double l2 = (e1[0] * rx3[0] + e1[1] * rx3[1] + e1[2] * rx3[2]) / det;
+            const double l3 = (e1[0] * e2r[0] + e1[1] * e2r[1] + e1[2] * e2r[2]) / det;
+            const double l0 = ((1.0 - l1) - l2) - l3;
+            const double mn = fmin(fmin(l0, l1), fmin(l2, l3));
+            if (mn > best) {
+                best = mn;
+                bl[0] = l0; bl[1] = l1; bl[2] = l2; bl[3] = l3;
+                bid[0] = fp[0]; bid[1] = fp[t]; bid[2] = fp[t + 1];
+            }
+            if (mn >= -kMiTol) { found = true; break; }
+        }
+    }
+    const double* fc = &w.mfield[(size_t)kMiRows * c];
+    for (int q = 0; q < nr; ++q) {
+        double v = bl[0] * fc[q];
+        for (int k = 0; k < 3; ++k)
+            if (bid[k] >= 0) v = v + bl[k + 1] * w.pfield[(size_t)kMiRows * bid[k] + q];
+        out[q] = v;
+    }
+}
+
+// The draw's parameters at the simulator's position (stochasticParticleBGK.C:
+// 680-687, stochasticParticleESBGK.C:779-789, unifiedStochasticParticleSBGK.C:
+// 933-946 with the samplers at :887-906 and :1047-1096): T and U for the
+// BGK model; with Pr, p and the pressure tensor the ES-BGK matrix
+// I - (1 - Pr)/(2 Pr) (P/p - I); with Pr, the collision frequency, p, q and
+// the stress the USP coefficients cQ, cS.  maxProb stays the cell's.
+template <class R>
+__device__ inline void macro_params(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, BgkP& Q)
+{
+    const double x[3] = {(double)w.b.x[i], (double)w.b.y[i], (double)w.b.z[i]};
+    double f[kMiRows];
+    mi_interp<R>(w, c, x, Q.model == UGX_BGK_BGK ? 4 : kMiRows, f);
+    const double T = f[0];
+    for (int d = 0; d < 3; ++d) Q.U[d] = f[1 + d];
+    Q.T = T;
+    Q.s2kT = sqrt(2.0 * kB * T);
+    if (Q.model == UGX_BGK_ESBGK) {
+        const double Pr = f[4], p = f[6];
+        const double a = 0.5 * (1.0 - Pr) / Pr;
+        const int full[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const double dl = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+            Q.sm[k] = dl - a * (f[10 + full[k]] / p - dl);
+        }
+        Q.p = p;
+    } else if (Q.model == UGX_BGK_USP) {
+        const double Pr = f[4], nu = f[5];
+        Q.p = f[6];
+        for (int d = 0; d < 3; ++d) Q.q[d] = f[7 + d];
+        for (int k = 0; k < 6; ++k) Q.sm[k] = f[10 + k];
+        const double tau = 0.5 * nu * cs.dt;
+        const double ex = 1.0 + 2.0 / (dexp(2.0 * tau) - 1.0);
+        Q.cQ = 2.0 * (1.0 - Pr * tau * ex);
+        Q.cS = 1.0 - tau * ex;
+    }
+}
+
+template <class R>
+__device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
+                                const BgkP& P, unsigned long long* raise, const GlobAcc<R>& A)
+{
+    if (w.has_macro && cs.macro) {
+        BgkP Q = P;
+        macro_params<R>(w, cs, c, i, Q);
+        bgk_draw_core<R>(w, cs, c, i, k, Q, raise, A);
+    } else {
+        bgk_draw_core<R>(w, cs, c, i, k, P, raise, A);
+    }
+}
+
+// The point values of the step's fields (volPointInterpolation): per point
+// the weighted sum of its cells' rows, in the order of its entries.
+template <class R>
+__global__ void k_point_fields(WorldView<R> w)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < w.npts; p += gridDim.x * blockDim.x) {
+        double v[kMiRows];
+#pragma unroll
+        for (int q = 0; q < kMiRows; ++q) v[q] = 0.0;
+        for (uint32_t k = w.mi_poff[p]; k < w.mi_poff[p + 1]; ++k) {
+            const double wt = w.mi_pw[k];
+            const double* f = &w.mfield[(size_t)kMiRows * w.mi_pcell[k]];
+#pragma unroll
+            for (int q = 0; q < kMiRows; ++q) v[q] = v[q] + wt * f[q];
+        }
+#pragma unroll
+        for (int q = 0; q < kMiRows; ++q) w.pfield[(size_t)kMiRows * p + q] = v[q];
+    }
 }
 
 // conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
@@ -3083,6 +3222,8 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     if (w.has_collide) {
         const unsigned gcell = (unsigned)(((uint64_t)w.nc + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_collide<R>, dim3(gcell), dim3(kBlock), 0, s, w);
+        if (w.has_macro)
+            hipLaunchKernelGGL(k_point_fields<R>, dim3(grid_for(w.npts)), dim3(kBlock), 0, s, w);
         if (w.has_bgk) {
             const unsigned gcons = (unsigned)(((uint64_t)w.nc * 8u + kBlock - 1) / kBlock);
             hipLaunchKernelGGL(k_bgk_conserve<R>, dim3(gcons), dim3(kBlock), 0, s, w);

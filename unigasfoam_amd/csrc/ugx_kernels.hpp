@@ -161,10 +161,29 @@ struct WorldView {
     // adaptive simulation: per cell the adapter's time sums since its last
     // adaptation (rows as the hybrid time sums)
     double* adacc;            // [nc*kKnRows]
+    // macroInterpolation (cells of cases with it on): per cell the centroid
+    // in the cell frame and the point ids of its faces, the mesh points,
+    // per point the cells and weights of its value; per cell and per point
+    // the kMiRows field rows of the step
+    int32_t has_macro;
+    uint32_t npts;
+    const double* mi_cent;    // [nc*3]
+    const int32_t* mi_fp;     // [nc*kMaxFaces*kMiFaceVerts] world point ids, -1 padded
+    const double* mi_pts;     // [npts*3]
+    const uint32_t* mi_poff;  // [npts+1]
+    const int32_t* mi_pcell;  // world cells
+    const double* mi_pw;
+    double* mfield;           // [nc*kMiRows]
+    double* pfield;           // [npts*kMiRows]
 };
 
 // Largest subcell level along an axis (uniGasDynamicAdapter.C:49).
 constexpr int kMaxSubcellLevels = 10;
+// macroInterpolation: points per face kept for the tets, and the per-cell
+// and per-point field rows (T, U, Pr, collision frequency, p, q, then the
+// pressure tensor (ES-BGK) or the stress (USP) as xx xy xz yy yz zz).
+constexpr int kMiFaceVerts = 8;
+constexpr int kMiRows = 16;
 
 // Records wall hits of measured faces into the step's event list.  Only
 // the hits of a sampling step of the cell's case are recorded: the

@@ -303,6 +303,8 @@ int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>&
             }
         }
     }
+    if (model.macro_interpolation)
+        if (int rc = build_interp(m, g)) return rc;
     return UGX_OK;
 }
 
@@ -362,7 +364,8 @@ void World::release()
                      (void**)&d_wcnt, (void**)&d_woff, (void**)&d_bkey, (void**)&d_bidx, (void**)&d_wsamp,
                      (void**)&d_cw, (void**)&d_cstart, (void**)&d_wclone, (void**)&d_wcoff, (void**)&d_wbsum,
                      (void**)&d_sclev, (void**)&d_scoff, (void**)&d_sctab, (void**)&d_scid, (void**)&d_sclist,
-                     (void**)&d_adacc, (void**)&d_cw_old};
+                     (void**)&d_adacc, (void**)&d_cw_old, (void**)&d_mi_cent, (void**)&d_mi_fp, (void**)&d_mi_pts,
+                     (void**)&d_mi_poff, (void**)&d_mi_pcell, (void**)&d_mi_pw, (void**)&d_mfield, (void**)&d_pfield};
     for (auto pp : ptrs) { if (*pp) (void)hipFree(*pp); *pp = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -591,6 +594,16 @@ WorldView<R> World::view() const
     w.bblo = d_bblo;
     w.bbhi = d_bbhi;
     w.adacc = d_adacc;
+    w.has_macro = has_macro;
+    w.npts = npts;
+    w.mi_cent = d_mi_cent;
+    w.mi_fp = d_mi_fp;
+    w.mi_pts = d_mi_pts;
+    w.mi_poff = d_mi_poff;
+    w.mi_pcell = d_mi_pcell;
+    w.mi_pw = d_mi_pw;
+    w.mfield = d_mfield;
+    w.pfield = d_pfield;
     return w;
 }
 
@@ -667,6 +680,10 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         d.reset_until_time = s.model.reset_until_time;
         d.time = 0.0;
         d.adaptive = s.model.adaptive ? 1 : 0;
+        // macroInterpolation reaches the BGK-family draws only
+        d.macro = (s.model.macro_interpolation && s.model.bgk_model != UGX_BGK_NONE &&
+                   (s.model.collision_model == UGX_COLL_BGK || s.model.collision_model == UGX_COLL_HYBRID)) ? 1 : 0;
+        if (d.macro) has_macro = 1;
         if (d.adaptive) has_adaptive = 1;
         {
             int lvl = d.sampler_full ? 2 : 0;
@@ -818,6 +835,37 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         }
         if ((rc = up(d_bblo, lo))) return rc;
         if ((rc = up(d_bbhi, hi))) return rc;
+    }
+    if (has_macro) {
+        // the cases' interpolation geometry, point and cell ids made world-wide
+        std::vector<double> cent((size_t)nc * 3, 0.0), pts, pw;
+        std::vector<int32_t> fp((size_t)nc * kMaxFaces * kMiFaceVerts, -1), pcell;
+        std::vector<uint32_t> poff(1, 0u);
+        int32_t pb = 0;
+        for (size_t ci = 0; ci < specs.size(); ++ci) {
+            const MeshGeom& gi = specs[ci].geom;
+            const int cb = hc[ci].cell_begin;
+            if (!hc[ci].macro) continue;
+            std::copy(gi.mi_cent.begin(), gi.mi_cent.end(), cent.begin() + 3 * (size_t)cb);
+            for (size_t k = 0; k < gi.mi_fp.size(); ++k)
+                fp[(size_t)cb * kMaxFaces * kMiFaceVerts + k] = gi.mi_fp[k] < 0 ? -1 : gi.mi_fp[k] + pb;
+            pts.insert(pts.end(), gi.mi_pts.begin(), gi.mi_pts.end());
+            const uint32_t eb = poff.back();
+            for (size_t p = 0; p + 1 < gi.mi_poff.size(); ++p) poff.push_back(eb + gi.mi_poff[p + 1]);
+            for (int32_t c : gi.mi_pcell) pcell.push_back(c + cb);
+            pw.insert(pw.end(), gi.mi_pw.begin(), gi.mi_pw.end());
+            pb += (int32_t)(gi.mi_pts.size() / 3);
+        }
+        npts = (uint32_t)pb;
+        if ((rc = up(d_mi_cent, cent))) return rc;
+        if ((rc = up(d_mi_fp, fp))) return rc;
+        if ((rc = up(d_mi_pts, pts))) return rc;
+        if ((rc = up(d_mi_poff, poff))) return rc;
+        if ((rc = up(d_mi_pcell, pcell))) return rc;
+        if ((rc = up(d_mi_pw, pw))) return rc;
+        std::vector<double> zf((size_t)nc * kMiRows, 0.0), zp((size_t)npts * kMiRows, 0.0);
+        if ((rc = up(d_mfield, zf))) return rc;
+        if ((rc = up(d_pfield, zp))) return rc;
     }
     if ((rc = up(d_cmodel, cmodel))) return rc;
     if ((rc = up(d_mass, mass))) return rc;

@@ -34,7 +34,20 @@ struct MeshGeom {
     std::vector<int32_t> wface;    // per cell-face: wall face index, or -1
     std::vector<double> wgeo;      // per wall face [kWGeo]: unit outward normal, area, tangent t1
     std::vector<int32_t> wpatch, wlocal, wcell;   // per wall face: patch, face within the patch, cell
+    // macroInterpolation (ugx_interp.cpp): per cell its centroid in the
+    // cell frame and per cell-face up to kMiFaceVerts point ids (-1 padded;
+    // the face's tets are (centroid, p0, p_i, p_i+1)); the mesh points; per
+    // point the cells and weights of its value
+    std::vector<double> mi_cent;     // [nc*3]
+    std::vector<int32_t> mi_fp;      // [nc*kMaxFaces*kMiFaceVerts]
+    std::vector<double> mi_pts;      // [npts*3]
+    std::vector<uint32_t> mi_poff;   // [npts+1]
+    std::vector<int32_t> mi_pcell;
+    std::vector<double> mi_pw;
 };
+
+// The geometry of macroInterpolation for mesh m into g (g.cc set).
+int build_interp(const ugx_mesh_desc* m, MeshGeom& g);
 
 int build_geometry(const ugx_mesh_desc* m, const std::vector<ugx_boundary_desc>& bnd, int ns,
                    const double* mass, const ugx_model_desc& model, MeshGeom& g);
@@ -189,6 +202,18 @@ struct World {
     // current one, then sorted again
     int reweight(int ci, const std::vector<double>& old_w);
     double* d_cw_old = nullptr;         // [nc] the weights before an adaptation
+    // macroInterpolation: the cases' interpolation geometry (world-wide ids)
+    // and the step's cell and point fields
+    int32_t has_macro = 0;
+    uint32_t npts = 0;
+    double* d_mi_cent = nullptr;
+    int32_t* d_mi_fp = nullptr;
+    double* d_mi_pts = nullptr;
+    uint32_t* d_mi_poff = nullptr;
+    int32_t* d_mi_pcell = nullptr;
+    double* d_mi_pw = nullptr;
+    double* d_mfield = nullptr;
+    double* d_pfield = nullptr;
 
     // per-phase HIP-event timing (off unless enabled)
     bool timing = false;

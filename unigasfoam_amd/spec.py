@@ -62,6 +62,7 @@ class CaseSpec:
     Tref: float = 273.0
     Zrot: float = 5.0
     theta: float = 1.0
+    macro_interpolation: bool = False    # collisionProperties.macroInterpolation (BGK family)
     sample_interval: int = 1
     boundaries: list = field(default_factory=list)
     decomposition_interval: int = 100
@@ -119,7 +120,7 @@ class CaseSpec:
         m.Tref = self.Tref
         m.Zrot = self.Zrot
         m.theta = self.theta
-        m.macro_interpolation = 0
+        m.macro_interpolation = 1 if self.macro_interpolation else 0
         m.sample_interval = self.sample_interval
         m.decomposition_interval = self.decomposition_interval
         m.smoothing_passes = self.smoothing_passes
