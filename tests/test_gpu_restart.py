@@ -101,3 +101,33 @@ def test_restart_after_sampler_reset(require_gpu):
     a.step(4)
     b.step(4)
     assert_same(results(a), results(b), "sampler reset")
+
+
+def test_start_from_the_reference_cloud_layout(require_gpu, tmp_path):
+    """A case written in uniGasFoam's restart layout (lagrangian/uniGas,
+    the restart cell fields, uniform/time) and started again from it holds
+    the same simulators (positions to rounding of the cell frame), the same
+    (sigma_T c_r)max and model ids, and steps on."""
+    from unigasfoam_amd import foam_io
+    spec, fill = workloads.nozzle(nx=10, ny=6, nz=6, per_cell=20, n_stag=2.4e24)
+    spec.decomposition_interval = 3
+    spec.smoothing_passes = 2
+    a = Case(spec, precision="fp64", capacity=1 << 17)
+    a.fill(**fill)
+    a.step(7)
+    foam_io.write_cloud(str(tmp_path), "7e-08", a)
+    b = foam_io.case_from_cloud(spec, str(tmp_path), "7e-08", precision="fp64", capacity=1 << 17)
+    pa, pb = a.particles(), b.particles()
+    order_a = np.lexsort((pa["velocity"][:, 0], pa["cell"]))
+    order_b = np.lexsort((pb["velocity"][:, 0], pb["cell"]))
+    for k in ("cell", "species", "velocity", "erot"):
+        assert np.array_equal(pa[k][order_a], pb[k][order_b]), k
+    assert np.allclose(pa["position"][order_a], pb["position"][order_b], rtol=0.0, atol=1e-15)
+    assert np.array_equal(a.sigma_max(), b.sigma_max())
+    assert np.array_equal(a.cell_model(), b.cell_model())
+    assert 0 < b.cell_model().sum() < b.cell_model().size
+    b.step(5)
+    b.sync()
+    assert np.isfinite(b.particles()["velocity"]).all() and b.counters()["collisions"] > 0
+    a.close()
+    b.close()

@@ -26,8 +26,8 @@ import numpy as np
 import pytest
 
 from oracle import orc
-from unigasfoam_amd import abi
-from unigasfoam_amd.mesh import box_mesh
+from unigasfoam_amd import abi, foam_io
+from unigasfoam_amd.mesh import box_mesh, mapped_box_mesh
 from unigasfoam_amd.spec import Boundary, CaseSpec, Species
 
 KB = abi.KB
@@ -153,4 +153,30 @@ def test_relaxed_draws_keep_the_local_shear(macro):
         assert slope == pytest.approx(s, rel=0.06)
     else:
         assert abs(slope) < 0.06 * s
+    o.close()
+
+
+def test_tets_meet_at_openfoams_centroid():
+    """On cells whose centroid is not the vertex mean (trapezoidal prisms)
+    the interpolation gives the cell's own value at OpenFOAM's volume
+    centroid (there the centroid's barycentric weight is 1); the centroid is
+    computed independently in numpy (foam_io.foam_centroids, whose box and
+    trapezoid centroids are pinned to closed forms)."""
+    def fn(S):
+        x, t, u = S[..., 0], S[..., 1], S[..., 2]
+        return np.stack([1e-3 * x, 1e-3 * t * (1.0 + x), 1e-3 * u], axis=-1)
+
+    pt = {k: abi.WALL for k in SIDES}
+    mesh = mapped_box_mesh((3, 2, 2), fn, pt)
+    spec = CaseSpec(mesh=mesh, species=[AR], delta_t=1e-6, n_equivalent_particles=1.0, collision_model="bgk",
+                    bgk_model="stochasticParticleBGK", Tref=273.0, seed=3, macro_interpolation=True,
+                    boundaries=[Boundary(p, abi.BND_DIFFUSE, temperature=300.0) for p in SIDES])
+    o = orc.OracleCase(spec, capacity=16)
+    _, cc = foam_io.foam_centroids(mesh)
+    vm = o.geometry()["cc"]
+    assert np.abs(cc - vm).max() > 1e-6          # not the vertex means
+    rng = np.random.default_rng(4)
+    fields = rng.uniform(1.0, 5.0, (mesh.n_cells, 16))
+    out = o.mi_probe(fields, cc, np.arange(mesh.n_cells, dtype=np.int32))
+    assert np.allclose(out, fields, rtol=1e-10, atol=0.0)
     o.close()

@@ -7,6 +7,8 @@ A mesh read here is the same PolyMesh the builders in mesh.py make, so a case
 can start from a blockMesh (or any other) polyMesh of convex cells with up to
 six planar faces.  Points are written with 17 significant digits, so a write
 and a read give back the same doubles.  Binary polyMesh files are refused.
+The cloud's restart layout (<time>/lagrangian/uniGas and the restart cell
+fields) is read and written as well (write_lagrangian, read_lagrangian).
 """
 from __future__ import annotations
 
@@ -296,3 +298,273 @@ def read_internal_field(path):
     s = body[m.end():body.rfind(")", 0, end)]
     a = _numbers(s, np.float64)
     return a.reshape(n, -1) if m.group(1) != "scalar" else a
+
+
+# ------------------------------------------------------------- lagrangian
+# The cloud's restart files (uniGasParcelIO.C:89-181, uniGasCloud.C:577-703):
+# <time>/lagrangian/uniGas/ holds one IOField per parcel property (U,
+# cellWeight, radialWeight, ERot, ELevel, typeId, newParcel, vibLevel) and
+# the parcel positions, either as OpenFOAM's "coordinates" (barycentric in a
+# tet of the cell: "(a b c d) celli tetFacei tetPti") or in the legacy
+# "positions" form ("(x y z) celli"), which OpenFOAM reads when
+# <time>/uniform/lagrangian/uniGas/cloudProperties says "geometry positions".
+# The cell fields of a restart are uniGasSigmaTcRMax, uniGasCollisionModelId,
+# uniGasCellWeightFactor and uniGasSubCellLevels, and <time>/uniform/time
+# holds deltaT.
+CLOUD = "uniGas"
+LAGRANGIAN_FIELDS = {"U": "vectorField", "cellWeight": "scalarField", "radialWeight": "scalarField",
+                     "ERot": "scalarField", "ELevel": "labelField", "typeId": "labelField",
+                     "newParcel": "labelField"}
+
+
+def _write_list_file(path, cls, obj, location, lines, n):
+    with open(path, "w") as f:
+        f.write(_header(cls, obj, location))
+        f.write(f"{n}\n(\n")
+        f.writelines(lines)
+        f.write(")\n")
+
+
+def _vol_file(path, cls, obj, location, dims, a, mesh: PolyMesh):
+    kind = {"volScalarField": "scalar", "volVectorField": "vector"}[cls]
+    with open(path, "w") as f:
+        f.write(_header(cls, obj, location))
+        f.write(f"dimensions      {dims};\n\n")
+        f.write(f"internalField   nonuniform List<{kind}>\n{mesh.n_cells}\n(\n{_values(a, cls)})\n;\n\n")
+        f.write("boundaryField\n{\n")
+        for p in mesh.patches:
+            t = {abi.EMPTY: "empty", abi.CYCLIC: "cyclic"}.get(p.type, "zeroGradient")
+            f.write(f"    {p.name}\n    {{\n        type            {t};\n    }}\n")
+        f.write("}\n")
+
+
+def write_lagrangian(case_dir, time_name, mesh: PolyMesh, particles, cell_weights=None, sigma_max=None,
+                     cell_model=None, subcell_levels=None, delta_t=None, cloud=CLOUD):
+    """Write a cloud in uniGasFoam's restart layout.  `particles` holds
+    position (global, [n, 3]), cell, velocity, erot and species (type ids);
+    the per-cell arrays, when given, go to the restart cell fields and
+    delta_t to uniform/time.  Positions are written in the legacy form with
+    17 significant digits (cloudProperties: geometry positions)."""
+    t = str(time_name)
+    d = os.path.join(case_dir, t, "lagrangian", cloud)
+    os.makedirs(d, exist_ok=True)
+    loc = f"{t}/lagrangian/{cloud}"
+    pos = np.asarray(particles["position"], dtype=np.float64).reshape(-1, 3)
+    cell = np.asarray(particles["cell"], dtype=np.int64)
+    n = len(cell)
+    cw = np.ones(n) if cell_weights is None else np.asarray(cell_weights, dtype=np.float64)[cell]
+    _write_list_file(os.path.join(d, "positions"), f"Cloud<{cloud}Parcel>", "positions", loc,
+                     [f"({x:.17g} {y:.17g} {z:.17g}) {c}\n" for (x, y, z), c in zip(pos, cell)], n)
+    cols = {"U": np.asarray(particles["velocity"], dtype=np.float64).reshape(-1, 3), "cellWeight": cw,
+            "radialWeight": np.ones(n), "ERot": np.asarray(particles["erot"], dtype=np.float64),
+            "ELevel": np.zeros(n, dtype=np.int64), "typeId": np.asarray(particles["species"], dtype=np.int64),
+            "newParcel": np.zeros(n, dtype=np.int64)}
+    for name, cls in LAGRANGIAN_FIELDS.items():
+        a = cols[name]
+        if cls == "vectorField":
+            lines = ["(" + " ".join(f"{v:.17g}" for v in row) + ")\n" for row in a]
+        elif cls == "scalarField":
+            lines = [f"{v:.17g}\n" for v in a]
+        else:
+            lines = [f"{int(v)}\n" for v in a]
+        _write_list_file(os.path.join(d, name), cls, name, loc, lines, n)
+    _write_list_file(os.path.join(d, "vibLevel"), "labelFieldField", "vibLevel", loc, ["0()\n"] * n, n)
+    u = os.path.join(case_dir, t, "uniform", "lagrangian", cloud)
+    os.makedirs(u, exist_ok=True)
+    with open(os.path.join(u, "cloudProperties"), "w") as f:
+        f.write(_header("dictionary", "cloudProperties", f"{t}/uniform/lagrangian/{cloud}"))
+        f.write(f"geometry        positions;\n\nprocessor0\n{{\n    particleCount   {n};\n}}\n")
+    tl = os.path.join(case_dir, t)
+    if sigma_max is not None:
+        _vol_file(os.path.join(tl, f"{cloud}SigmaTcRMax"), "volScalarField", f"{cloud}SigmaTcRMax", t,
+                  "[0 3 -1 0 0 0 0]", np.asarray(sigma_max, dtype=np.float64), mesh)
+    if cell_model is not None:
+        _vol_file(os.path.join(tl, f"{cloud}CollisionModelId"), "volScalarField", f"{cloud}CollisionModelId", t,
+                  "[0 0 0 0 0 0 0]", np.asarray(cell_model, dtype=np.float64), mesh)
+    if cell_weights is not None:
+        _vol_file(os.path.join(tl, f"{cloud}CellWeightFactor"), "volScalarField", f"{cloud}CellWeightFactor", t,
+                  "[0 0 0 0 0 0 0]", np.asarray(cell_weights, dtype=np.float64), mesh)
+    if subcell_levels is not None:
+        _vol_file(os.path.join(tl, f"{cloud}SubCellLevels"), "volVectorField", f"{cloud}SubCellLevels", t,
+                  "[0 0 0 0 0 0 0]", np.asarray(subcell_levels, dtype=np.float64).reshape(-1, 3), mesh)
+    if delta_t is not None:
+        ud = os.path.join(case_dir, t, "uniform")
+        with open(os.path.join(ud, "time"), "w") as f:
+            f.write(_header("dictionary", "time", f"{t}/uniform"))
+            f.write(f"value           {float(time_name):.17g};\nname            \"{t}\";\n"
+                    f"index           0;\ndeltaT          {float(delta_t):.17g};\ndeltaT0         {float(delta_t):.17g};\n")
+    return d
+
+
+def foam_centroids(mesh: PolyMesh):
+    """OpenFOAM's face centroids and cell centroids of a mesh
+    (primitiveMeshFaceCentresAndAreas, primitiveMeshCellCentresAndVols)."""
+    P = mesh.points
+    nf = len(mesh.face_offsets) - 1
+    fc = np.zeros((nf, 3))
+    fa = np.zeros((nf, 3))
+    for f in range(nf):
+        ids = mesh.face_points[mesh.face_offsets[f]:mesh.face_offsets[f + 1]]
+        pts = P[ids]
+        if len(ids) == 3:
+            fc[f] = (1.0 / 3.0) * pts.sum(axis=0)
+            fa[f] = 0.5 * np.cross(pts[1] - pts[0], pts[2] - pts[0])
+            continue
+        mean = pts.mean(axis=0)
+        nxt = np.roll(pts, -1, axis=0)
+        c = pts + nxt + mean
+        nn = np.cross(nxt - pts, mean - pts)
+        a = np.linalg.norm(nn, axis=1)
+        fc[f] = (1.0 / 3.0) * (a[:, None] * c).sum(axis=0) / a.sum()
+        fa[f] = 0.5 * nn.sum(axis=0)
+    nc = mesh.n_cells
+    ni = len(mesh.neighbour)
+    own, nei = np.asarray(mesh.owner), np.asarray(mesh.neighbour)
+    cest = np.zeros((nc, 3))
+    cnt = np.zeros(nc)
+    np.add.at(cest, own, fc)
+    np.add.at(cnt, own, 1.0)
+    np.add.at(cest, nei, fc[:ni])
+    np.add.at(cnt, nei, 1.0)
+    cest /= cnt[:, None]
+    cctr = np.zeros((nc, 3))
+    vol = np.zeros(nc)
+    pyr = np.maximum((fa * (fc - cest[own])).sum(axis=1), 1e-300)
+    np.add.at(cctr, own, pyr[:, None] * (0.75 * fc + 0.25 * cest[own]))
+    np.add.at(vol, own, pyr)
+    pyr = np.maximum((fa[:ni] * (cest[nei] - fc[:ni])).sum(axis=1), 1e-300)
+    np.add.at(cctr, nei, pyr[:, None] * (0.75 * fc[:ni] + 0.25 * cest[nei]))
+    np.add.at(vol, nei, pyr)
+    return fc, cctr / vol[:, None]
+
+
+def _read_list_lines(path):
+    n, inner = _list(_read(path), path)
+    return n, inner
+
+
+def read_lagrangian(case_dir, time_name, mesh: PolyMesh, cloud=CLOUD):
+    """Read a cloud written by uniGasFoam (or write_lagrangian): positions
+    from "positions" (legacy form) or from "coordinates" (barycentric in the
+    tet (cell centroid, p0, p_i, p_i+1) of face tetFacei, the last two swapped
+    in the face's neighbour cell), and the parcel fields.  Returns position,
+    cell, velocity, erot, species and cell_weight arrays; a parcel with
+    ELevel or vibLevel set is refused (no electronic or vibrational states)."""
+    d = os.path.join(case_dir, str(time_name), "lagrangian", cloud)
+    if os.path.exists(os.path.join(d, "positions")):
+        n, inner = _read_list_lines(os.path.join(d, "positions"))
+        rows = re.findall(r"\(\s*([^()]+?)\s*\)\s*(-?\d+)", inner)
+        if len(rows) != n:
+            raise ValueError(f"{d}/positions: {len(rows)} entries, header says {n}")
+        pos = np.array([r[0].split() for r in rows], dtype=np.float64).reshape(n, 3)
+        cell = np.array([r[1] for r in rows], dtype=np.int32)
+    elif os.path.exists(os.path.join(d, "coordinates")):
+        n, inner = _read_list_lines(os.path.join(d, "coordinates"))
+        rows = re.findall(r"\(\s*([^()]+?)\s*\)\s*(-?\d+)\s+(-?\d+)\s+(-?\d+)", inner)
+        if len(rows) != n:
+            raise ValueError(f"{d}/coordinates: {len(rows)} entries, header says {n}")
+        bary = np.array([r[0].split() for r in rows], dtype=np.float64).reshape(n, 4)
+        cell = np.array([r[1] for r in rows], dtype=np.int32)
+        tf = np.array([r[2] for r in rows], dtype=np.int64)
+        tp = np.array([r[3] for r in rows], dtype=np.int64)
+        _, cctr = foam_centroids(mesh)
+        pos = np.zeros((n, 3))
+        P = mesh.points
+        for i in range(n):
+            ids = mesh.face_points[mesh.face_offsets[tf[i]]:mesh.face_offsets[tf[i] + 1]]
+            a, b = ids[tp[i] % len(ids)], ids[(tp[i] + 1) % len(ids)]
+            if mesh.owner[tf[i]] != cell[i]:
+                a, b = b, a
+            pos[i] = bary[i, 0] * cctr[cell[i]] + bary[i, 1] * P[ids[0]] + bary[i, 2] * P[a] + bary[i, 3] * P[b]
+    else:
+        raise ValueError(f"{d}: neither positions nor coordinates")
+
+    def field(name, dtype, width=1):
+        path = os.path.join(d, name)
+        body = _read(path)
+        u = re.match(r"\s*(\d+)\s*\{([^}]*)\}", body)
+        if u:
+            # OpenFOAM's compact form of a uniform list: N{value}
+            m = int(u.group(1))
+            v = _numbers(u.group(2), dtype)
+            a = np.tile(v, m)
+        else:
+            m, inner = _list(body, path)
+            a = _numbers(inner, dtype)
+        if len(a) != m * width:
+            raise ValueError(f"{d}/{name}: {len(a)} values for {m} parcels")
+        return a.reshape(m, width) if width > 1 else a
+
+    out = {"position": pos, "cell": cell, "velocity": field("U", np.float64, 3), "erot": field("ERot", np.float64),
+           "species": field("typeId", np.int32)}
+    if os.path.exists(os.path.join(d, "cellWeight")):
+        out["cell_weight"] = field("cellWeight", np.float64)
+    if os.path.exists(os.path.join(d, "ELevel")) and field("ELevel", np.int64).any():
+        raise ValueError(f"{d}: parcels in excited electronic levels")
+    if os.path.exists(os.path.join(d, "vibLevel")):
+        body = _read(os.path.join(d, "vibLevel"))
+        if not re.match(r"\s*\d+\s*\{\s*0\s*\(\s*\)\s*\}", body):
+            _, inner = _list(body, os.path.join(d, "vibLevel"))
+            if re.search(r"\b[1-9]\d*\s*\(", inner):
+                raise ValueError(f"{d}: parcels with vibrational levels")
+    for k in ("velocity", "erot", "species"):
+        if len(out[k]) != n:
+            raise ValueError(f"{d}: {k} holds {len(out[k])} parcels, positions {n}")
+    return out
+
+
+def read_restart_fields(case_dir, time_name, cloud=CLOUD):
+    """The restart cell fields and deltaT that exist under <time>."""
+    t = os.path.join(case_dir, str(time_name))
+    out = {}
+    for key, stem in (("sigma_max", "SigmaTcRMax"), ("cell_model", "CollisionModelId"),
+                      ("cell_weights", "CellWeightFactor"), ("subcell_levels", "SubCellLevels")):
+        p = os.path.join(t, cloud + stem)
+        if os.path.exists(p):
+            out[key] = read_internal_field(p)
+    p = os.path.join(t, "uniform", "time")
+    if os.path.exists(p):
+        m = re.search(r"\bdeltaT\s+([^;]+);", _read(p))
+        if m:
+            out["delta_t"] = float(m.group(1))
+    return out
+
+
+def write_cloud(case_dir, time_name, case, cloud=CLOUD):
+    """A running Case's simulators and restart cell fields in uniGasFoam's
+    restart layout (write_lagrangian)."""
+    case.sync()
+    p = case.particles()
+    return write_lagrangian(case_dir, time_name, case.spec.mesh, p, cell_weights=case.cell_weights(),
+                            sigma_max=case.sigma_max(), cell_model=case.cell_model(),
+                            subcell_levels=case.subcell_levels(), delta_t=case.delta_t(), cloud=cloud)
+
+
+def case_from_cloud(spec, case_dir, time_name, cloud=CLOUD, **case_kw):
+    """A new Case of spec holding the cloud of <case_dir>/<time_name> (as
+    uniGasCloud.C:577-703 starts from existing lagrangian data): the
+    simulators at their positions and cells, then (σ_T c_r)max, the cell
+    weights and subcell levels where the files hold them; a deltaT in
+    uniform/time replaces the spec's.  The samplers start empty."""
+    import dataclasses
+
+    from .core import Case
+    rf = read_restart_fields(case_dir, time_name, cloud)
+    if "delta_t" in rf:
+        spec = dataclasses.replace(spec, delta_t=rf["delta_t"])
+    parts = read_lagrangian(case_dir, time_name, spec.mesh, cloud)
+    c = Case(spec, **case_kw)
+    if "cell_weights" in rf and spec.cell_weighted:
+        c.set_cell_weights(np.asarray(rf["cell_weights"], dtype=np.float64).reshape(spec.mesh.n_cells))
+    if "subcell_levels" in rf:
+        c.set_subcell_levels(np.rint(np.asarray(rf["subcell_levels"]).reshape(-1, 3)).astype(np.int32))
+    if len(parts["cell"]):
+        c.upload(velocity=parts["velocity"], cell=parts["cell"], species=parts["species"], erot=parts["erot"],
+                 position=parts["position"])
+    if "sigma_max" in rf:
+        c.set_sigma_max(np.asarray(rf["sigma_max"], dtype=np.float64).reshape(spec.mesh.n_cells))
+    if "cell_model" in rf and spec.collision_model == "hybrid":
+        st = c.state()
+        st["cell_model"] = np.rint(np.asarray(rf["cell_model"])).astype(np.int32).reshape(spec.mesh.n_cells, 1)
+        c.restore_state(st)
+    return c
