@@ -7,6 +7,7 @@ The cases under tests/cases/ are written for these tests: heatBath is the
 platePlume a three-block N2/O2 flow over a plate with inflow, deletion and a
 diffuse wall (config C3 in small).
 """
+import math
 import os
 import shutil
 
@@ -197,8 +198,7 @@ def test_reference_tutorials_report_exactly_their_unsupported_switches():
         "expansionInVacuum": {"uniGasProperties": common["uniGasProperties"] + ["axisymmetricSimulation"],
                               "fieldPropertiesDict": ["fieldModel uniGasMassFluxSurface"],
                               "uniGasInitialisationDict": ["configuration type uniGasMeshFieldFill"]},
-        "hypersonicCylinder": {"uniGasProperties": common["uniGasProperties"],
-                               "blockMeshDict": ["curved edges ('arc'"]},
+        "hypersonicCylinder": {"uniGasProperties": common["uniGasProperties"]},
         "plumeImpingement": {"uniGasProperties": common["uniGasProperties"] + ["axisymmetricSimulation"],
                              "blockMeshDict": ["curved edges ('polyLine'"],
                              "boundariesDict": ["uniGasLiouFangPressureInletPatch on inlet"],
@@ -216,6 +216,14 @@ def test_reference_tutorials_report_exactly_their_unsupported_switches():
             assert len(got[f]) == len(items), (tut, f, got[f])
             for prefix, g in zip(items, got[f]):
                 assert g.startswith(prefix), (tut, f, g)
+    # the cylinder tutorial's arc-edged half annulus builds: its volume is
+    # the half annulus less the chords' segments (100 per quarter arc)
+    inp = case_dir.scan_case(os.path.join(REF_TUT, "hypersonicCylinder"))
+    assert inp.spec.mesh.n_cells == 20000
+    ft = 0.3048
+    seg = 1.0 - math.sin(0.5 * math.pi / 100) / (0.5 * math.pi / 100)
+    want = 0.5 * math.pi * (4.0 - 0.25) * ft ** 2 * (0.1 * ft) * (1.0 - seg)
+    assert core.geometry(inp.spec)["vol"].sum() == pytest.approx(want, rel=1e-9)
     # the plate tutorial's three-block mesh builds and nothing in it is
     # unsupported: its Mach-3 argon case runs as the reference configures it
     inp = case_dir.scan_case(os.path.join(REF_TUT, "supersonicPlate"))
@@ -229,13 +237,65 @@ def test_reference_tutorials_report_exactly_their_unsupported_switches():
     assert np.all(core.geometry(inp.spec)["vol"] > 0)
 
 
+QUARTER_ANNULUS = """scale 0.5;
+vertices ((1 0 0) (3 0 0) (0 3 0) (0 1 0) (1 0 0.2) (3 0 0.2) (0 3 0.2) (0 1 0.2));
+blocks ( hex (0 1 2 3 4 5 6 7) (4 12 1) simpleGrading (2 1 1) );
+edges ( {edges} );
+boundary ( w {{ type wall; faces ((3 7 4 0) (1 2 6 5)); }} f {{ type empty; faces ((0 3 2 1) (4 5 6 7)); }} );"""
+
+
+@pytest.mark.parametrize("form", ["point", "angle"])
+def test_blockmesh_arc_edges(form):
+    """A quarter annulus r in [1, 3] (scale 0.5) from one block with arcs on
+    both curved sides: every point lies on a ray of the angle its column
+    has and at the radius of its row; the volume is the exact annulus area
+    less the chords' segments, (N/2) r^2 (theta - sin theta) per arc of N
+    segments of angle theta; the 'angle (axis)' form gives the same mesh as
+    the form through a point."""
+    h = math.sqrt(0.5)
+    if form == "point":
+        e = f"arc 1 2 ({3 * h} {3 * h} 0) arc 0 3 ({h} {h} 0) arc 5 6 ({3 * h} {3 * h} 0.2) arc 4 7 ({h} {h} 0.2)"
+    else:
+        e = "arc 1 2 90 (0 0 1) arc 0 3 90 (0 0 1) arc 5 6 90 (0 0 1) arc 4 7 90 (0 0 1)"
+    d = foam_dict.parse(QUARTER_ANNULUS.format(edges=e), counted=False)
+    m = blockmesh.build(d)
+    assert m.n_cells == 48
+    P = m.points / 0.5
+    r = np.hypot(P[:, 0], P[:, 1])
+    # radii of the graded rows (x direction 1 -> 3 with expansion 2 over 4 cells)
+    rows = 1.0 + 2.0 * blockmesh._graded(4, 2.0)
+    assert np.allclose(np.sort(np.unique(np.round(r, 12))), rows, atol=1e-12)
+    ang = np.arctan2(P[:, 1], P[:, 0])
+    assert np.allclose(np.sort(np.unique(np.round(ang, 12))), np.linspace(0.0, 0.5 * math.pi, 13), atol=1e-12)
+    vol = core.geometry(case_dir_spec(m))["vol"].sum() / 0.5 ** 3
+    th = 0.5 * math.pi / 12
+    area = sum(0.5 * 12 * (rows[k + 1] ** 2 - rows[k] ** 2) * math.sin(th) for k in range(4))
+    assert vol == pytest.approx(area * 0.2, rel=1e-12)
+    assert area < 0.25 * math.pi * (9 - 1)
+
+
+def case_dir_spec(mesh):
+    from unigasfoam_amd.spec import Boundary, CaseSpec, Species
+    return CaseSpec(mesh=mesh, species=[Species("Ar", mass=6.63e-26, diameter=4.17e-10, omega=0.81)], delta_t=1e-6,
+                    n_equivalent_particles=1.0, collision_model="dsmc", boundaries=[Boundary("w", abi.BND_SPECULAR)])
+
+
+def test_arc_vertex_labels_are_not_list_counts():
+    """'arc 2 3 (x y z)': with counted lists off the second label stays a
+    label although a three-entry list follows it."""
+    d = foam_dict.parse("edges ( arc 2 3 (1 2 3) arc 0 1 (4 5 6) );", counted=False)
+    assert d["edges"] == ["arc", 2, 3, [1, 2, 3], "arc", 0, 1, [4, 5, 6]]
+    assert foam_dict.parse("v 3 (1 2 3);")["v"] == [1, 2, 3]
+
+
 def test_blockmesh_refuses_what_it_does_not_build():
     base = """vertices ((0 0 0) (1 0 0) (1 1 0) (0 1 0) (0 0 1) (1 0 1) (1 1 1) (0 1 1));
     blocks ( hex (0 1 2 3 4 5 6 7) (2 2 2) simpleGrading (1 1 1) );
     boundary ( w { type wall; faces ((0 4 7 3)); } );"""
     m = blockmesh.build(foam_dict.parse(base))
     assert [p.name for p in m.patches] == ["w", "defaultFaces"]
-    for extra in ["edges ( arc 0 1 (0.5 -0.1 0) );", "mergePatchPairs ( (a b) );"]:
+    for extra in ["edges ( polyLine 0 1 ((0.5 -0.1 0)) );", "edges ( spline 0 1 ((0.5 -0.1 0)) );",
+                  "mergePatchPairs ( (a b) );"]:
         with pytest.raises(blockmesh.Unsupported):
             blockmesh.build(foam_dict.parse(base + extra))
     with pytest.raises(blockmesh.Unsupported):

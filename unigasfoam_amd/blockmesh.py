@@ -2,18 +2,24 @@
 
 Covered: ``scale`` / ``convertToMeters``, ``vertices``, ``blocks`` of
 ``hex (8 vertices) [zone] (nx ny nz) simpleGrading (gx gy gz)`` with scalar
-expansion ratios, straight edges (an empty ``edges`` list), and ``boundary``
-(or the older ``patches``) entries of type patch, wall, symmetry,
-symmetryPlane, empty and cyclic (``neighbourPatch``).  Blocks are merged
-where they share faces, as blockMesh does; block faces that no patch names
-go to ``defaultFaces`` (type empty, blockMesh's default).  Anything else
-(curved edges, multi-grading, edgeGrading, mergePatchPairs, projection)
-raises Unsupported with the entry's name.
+expansion ratios, straight edges and circular ``arc`` edges (through a
+point, ``arc v1 v2 (x y z)``, or by angle and axis, ``arc v1 v2 angle
+(ax ay az)``), and ``boundary`` (or the older ``patches``) entries of type
+patch, wall, symmetry, symmetryPlane, empty and cyclic (``neighbourPatch``).
+Blocks are merged where they share faces, as blockMesh does; block faces
+that no patch names go to ``defaultFaces`` (type empty, blockMesh's
+default).  Anything else (other curved edges, multi-grading, edgeGrading,
+mergePatchPairs, projection) raises Unsupported with the entry's name.
 
 Points inside a block: the graded parameters of each direction (cell sizes
 in geometric progression, last/first = the expansion ratio) mapped through
 the block's vertices: exactly v0 + s (v1 - v0) + t (v3 - v0) + u (v4 - v0)
-for a parallelepiped block, the trilinear map otherwise.  Cells are numbered
+for a parallelepiped block, the trilinear map otherwise.  A block with arc
+edges takes the edge-based transfinite interpolation: the sum over its
+twelve edges, each evaluated at its graded parameter (an arc at that
+fraction of its angle) and weighted bilinearly by the other two
+parameters, less twice the trilinear map, which reproduces every edge and
+is the trilinear map when the edges are straight.  Cells are numbered
 block by block, x fastest; faces follow OpenFOAM's order (internal faces by
 owner then neighbour, then the boundary patch by patch), each right-handed
 about the normal out of its owner.
@@ -86,23 +92,121 @@ def _block_points(V, div, grad):
     return X  # [nx+1, ny+1, nz+1, 3]
 
 
+# the 12 edges of a hex block: (corner a, corner b, direction, the two
+# other-direction parameters' sides), in blockMesh's local corner numbers
+_HEX_EDGES = (
+    (0, 1, 0, (0, 0)), (3, 2, 0, (1, 0)), (4, 5, 0, (0, 1)), (7, 6, 0, (1, 1)),
+    (0, 3, 1, (0, 0)), (1, 2, 1, (1, 0)), (4, 7, 1, (0, 1)), (5, 6, 1, (1, 1)),
+    (0, 4, 2, (0, 0)), (1, 5, 2, (1, 0)), (3, 7, 2, (0, 1)), (2, 6, 2, (1, 1)),
+)
+
+
+class _Arc:
+    """A circular arc from p0 to p1 (arcEdge): by a third point on it or by
+    the swept angle (degrees) about an axis; position(f) is the point at
+    fraction f of the angle."""
+
+    def __init__(self, p0, p1, mid=None, angle=None, axis=None):
+        p0, p1 = np.asarray(p0, float), np.asarray(p1, float)
+        if mid is not None:
+            pm = np.asarray(mid, float)
+            a, b = pm - p0, p1 - p0
+            n = np.cross(a, b)
+            nn = np.dot(n, n)
+            if nn <= 0.0:
+                raise DictError("arc through three collinear points")
+            # circumcentre of p0, pm, p1
+            c = p0 + (np.dot(a, a) * np.cross(b, n) + np.dot(b, b) * np.cross(n, a)) / (2.0 * nn)
+            n = n / np.sqrt(nn)
+            r0, r1, rm = p0 - c, p1 - c, pm - c
+            ang = np.arctan2(np.dot(np.cross(r0, r1), n), np.dot(r0, r1))
+            if ang <= 0.0:
+                ang += 2.0 * np.pi
+            self.c, self.n, self.r0, self.ang = c, n, r0, ang
+        else:
+            ax = np.asarray(axis, float)
+            ax = ax / np.linalg.norm(ax)
+            ang = np.radians(float(angle))
+            # the centre on the bisector plane of the chord at the distance the angle fixes
+            chord = p1 - p0
+            half = 0.5 * np.linalg.norm(chord)
+            mid = 0.5 * (p0 + p1)
+            toward = np.cross(ax, chord)
+            toward = toward / np.linalg.norm(toward)
+            c = mid + toward * (half / np.tan(0.5 * ang))
+            self.c, self.n, self.r0, self.ang = c, ax, p0 - c, ang
+
+    def position(self, f):
+        f = np.asarray(f, float)[..., None]
+        th = f * self.ang
+        r0, n = self.r0, self.n
+        return self.c + r0 * np.cos(th) + np.cross(n, r0) * np.sin(th) + n * np.dot(n, r0) * (1.0 - np.cos(th))
+
+
+def _parse_edges(edges, verts, scale):
+    """{(v1, v2): curve} of the arc edges (verts already scaled)."""
+    out = {}
+    i = 0
+    while i < len(edges):
+        kind = edges[i]
+        if kind != "arc":
+            raise Unsupported(f"curved edges ({kind!r} ...)")
+        v1, v2 = int(edges[i + 1]), int(edges[i + 2])
+        if isinstance(edges[i + 3], list):
+            out[(v1, v2)] = _Arc(verts[v1], verts[v2], mid=np.asarray(edges[i + 3], float) * scale)
+            i += 4
+        else:
+            out[(v1, v2)] = _Arc(verts[v1], verts[v2], angle=edges[i + 3], axis=edges[i + 4])
+            i += 5
+    return out
+
+
+def _curved_block_points(V, vid, div, grad, curves):
+    """Edge-based transfinite interpolation of a hex block with curved edges."""
+    s, t, u = (_graded(div[a], grad[a]) for a in range(3))
+    par = (s, t, u)
+    S, T, Uu = np.meshgrid(s, t, u, indexing="ij")
+    S, T, Uu = S[..., None], T[..., None], Uu[..., None]
+    P = (S, T, Uu)
+    X = -2.0 * ((1 - S) * (1 - T) * (1 - Uu) * V[0] + S * (1 - T) * (1 - Uu) * V[1] + S * T * (1 - Uu) * V[2]
+                + (1 - S) * T * (1 - Uu) * V[3] + (1 - S) * (1 - T) * Uu * V[4] + S * (1 - T) * Uu * V[5]
+                + S * T * Uu * V[6] + (1 - S) * T * Uu * V[7])
+    for a, b, dirn, (sa, sb) in _HEX_EDGES:
+        p = par[dirn]
+        ga, gb = vid[a], vid[b]
+        if (ga, gb) in curves:
+            e = curves[(ga, gb)].position(p)
+        elif (gb, ga) in curves:
+            e = curves[(gb, ga)].position(1.0 - p)
+        else:
+            e = V[a][None, :] + p[:, None] * (V[b] - V[a])[None, :]
+        o1, o2 = [P[k] for k in range(3) if k != dirn]
+        w = (o1 if sa else 1 - o1) * (o2 if sb else 1 - o2)
+        shape = [1, 1, 1, 3]
+        shape[dirn] = len(p)
+        X = X + w * e.reshape(shape)
+    return X
+
+
 def build(d: FoamDict) -> PolyMesh:
     scale = float(d.get("scale", d.get("convertToMeters", 1.0)))
     edges = d.get("edges", [])
-    if edges:
-        raise Unsupported(f"curved edges ({edges[0]!r} ...)")
+    verts = np.array(d["vertices"], dtype=np.float64) * scale
+    curves = _parse_edges(edges, verts, scale) if edges else {}
     if d.get("mergePatchPairs", []):
         raise Unsupported("mergePatchPairs")
     for key in ("geometry", "faces", "projectVertices"):
         if key in d:
             raise Unsupported(key)
-    verts = np.array(d["vertices"], dtype=np.float64) * scale
     blocks = _parse_blocks(d["blocks"])
     # points of every block; coincident points of neighbouring blocks merged
     pts, cells, bface = [], [], []
     base = 0
     for b, (vid, div, grad) in enumerate(blocks):
-        X = _block_points(verts[vid], div, grad)
+        if any((vid[a], vid[c]) in curves or (vid[c], vid[a]) in curves for a, c, _, _ in _HEX_EDGES):
+            X = _curved_block_points(verts[vid], vid, div, grad, curves)
+        else:
+            X = _block_points(verts[vid], div, grad)
         nx, ny, nz = div
         P = X.transpose(2, 1, 0, 3).reshape(-1, 3)    # k slowest, i fastest
         pts.append(P)

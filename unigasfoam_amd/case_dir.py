@@ -322,7 +322,7 @@ def scan_case(case, seed=12345) -> CaseInputs:
         from .foam_io import read_polymesh
         mesh = read_polymesh(case)
     elif os.path.exists(_path(case, "system", "blockMeshDict")):
-        bm = read_dict(_path(case, "system", "blockMeshDict"))
+        bm = read_dict(_path(case, "system", "blockMeshDict"), counted=False)
         try:
             mesh = blockmesh.build(bm)
         except blockmesh.Unsupported as e:

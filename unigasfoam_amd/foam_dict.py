@@ -143,11 +143,12 @@ def _atom(tok):
 
 
 class _Parser:
-    def __init__(self, text, path):
+    def __init__(self, text, path, counted=True):
         self.toks = _TOKEN.findall(_strip_comments(text))
         self.i = 0
         self.path = path
         self.top = None
+        self.counted = counted
 
     def err(self, msg):
         return DictError(f"{self.path}: {msg} (token {self.i})")
@@ -196,9 +197,16 @@ class _Parser:
             raise self.err(f"unexpected {tok!r}")
         if tok.startswith("#"):
             raise self.err(f"directive {tok} is not supported")
-        # a count-prefixed list: 3 ( ... )
-        if self.peek() == "(" and re.fullmatch(r"\d+", tok) is not None:
-            return self.lst()
+        # a count-prefixed list: 3 ( ... ); a number before a list of another
+        # length is a value of its own.  Files without counted lists (a
+        # blockMeshDict, whose "arc 2 3 (x y z)" would read as one) turn
+        # the detection off.
+        if self.counted and self.peek() == "(" and re.fullmatch(r"\d+", tok) is not None:
+            at = self.i
+            items = self.lst()
+            if len(items) == int(tok):
+                return items
+            self.i = at
         return self.resolve(tok)
 
     def lst(self):
@@ -256,14 +264,16 @@ class _Parser:
         return d
 
 
-def parse(text, path="<string>") -> FoamDict:
-    p = _Parser(text, path)
+def parse(text, path="<string>", counted=True) -> FoamDict:
+    """A dictionary from its text; counted=False reads "N ( ... )" as a
+    number followed by a list (files that hold no counted lists)."""
+    p = _Parser(text, path, counted)
     d = p.dict_body(None)
     d.pop("FoamFile", None)
     return d
 
 
-def read(path) -> FoamDict:
+def read(path, counted=True) -> FoamDict:
     with open(path) as f:
         text = f.read()
-    return parse(text, path)
+    return parse(text, path, counted)
