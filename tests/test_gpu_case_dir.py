@@ -1,5 +1,6 @@
 """Cases read from OpenFOAM dictionaries, run through the C ABI on the GPU.
 
+cylinderWake gathers the reference tutorials' switches on a small mesh.
 platePlume is a three-block N2/O2 plate flow with freestream inflow,
 deletion, a diffuse wall at 500 K, symmetry and empty patches and
 Larsen-Borgnakke collisions (config C3's physics on 2400 cells); couetteN2
@@ -21,6 +22,35 @@ from test_gpu_parity import assert_identical
 
 pytestmark = pytest.mark.gpu
 CASES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cases")
+
+
+def test_cylinder_wake_case_identical_to_oracle(require_gpu):
+    """cylinderWake: the reference tutorials' feature set on 160 cells: an
+    arc-edged half annulus around a diffuse cylinder, freestream inflow,
+    symmetry and empty patches, hybrid USP/DSMC with macroInterpolation,
+    cell weighting, and time-step, subcell and cell-weight adaptation every
+    five steps.  Bit-identical to the oracle, including the adapted time
+    step, the simulated time, the subcell levels and the weights."""
+    spec, fills, run = case_dir.read_case(os.path.join(CASES, "cylinderWake"))
+    assert spec.macro_interpolation and spec.adaptive and spec.cell_weight_adaptation and spec.collision_model == "hybrid"
+    g = Case(spec, precision="fp64", capacity=1 << 18)
+    o = OracleCase(spec, capacity=1 << 18)
+    for f in fills:
+        g.fill(**f)
+        o.fill(**f)
+    for k in range(3):
+        g.step(5)
+        o.step(5)
+        what = f"cylinderWake after {5 * (k + 1)} steps"
+        assert_identical(g, o, what)
+        assert g.delta_t() == o.delta_t() and g.time() == o.time(), what
+        assert np.array_equal(g.subcell_levels(), o.subcell_levels()), what
+        assert np.array_equal(g.cell_weights(), o.cell_weights()), what
+    c = g.counters()
+    assert c["collisions"] > 100 and c["bgk_collisions"] > 1000 and c["inserted"] > 100 and c["cloned"] > 100
+    assert 0 < g.cell_model().sum() < spec.mesh.n_cells and g.subcell_levels().max() > 1
+    g.close()
+    o.close()
 
 
 def test_plate_case_identical_to_oracle(require_gpu):
