@@ -15,7 +15,7 @@ from unigasfoam_amd.batch import Batch
 from unigasfoam_amd.core import Case
 
 from test_gpu_parity import assert_identical
-from test_oracle_adapt import adaptive_box
+from test_oracle_adapt import adaptive_box, mfp, symmetric_box, AR
 from test_oracle_physics import SIDES, box_case
 from unigasfoam_amd.spec import Boundary
 
@@ -59,7 +59,15 @@ def closed_box():
     return adaptive_box(L=1.5e-4, interval=5, fn=2e8), {"number_density": {"Ar": 1.2e22}, "T": 300.0}
 
 
-CASES = {"closed_box": closed_box, "inflow": inflow_case, "inflow_weighted": lambda: inflow_case(True),
+def symmetry_box():
+    """A symmetryPlane at y = 0: the smoothed size ratios reflect on it."""
+    n, T = 1.2e22, 300.0
+    return (symmetric_box(2.25 * mfp(AR, n, T), passes=3, interval=5, fn=4e8),
+            {"number_density": {"Ar": n}, "T": T})
+
+
+CASES = {"closed_box": closed_box, "symmetry_box": symmetry_box, "inflow": inflow_case,
+         "inflow_weighted": lambda: inflow_case(True),
          "hybrid_nozzle": nozzle_case}
 
 

@@ -258,6 +258,35 @@ def face_graph(o, nc):
     return nbrs
 
 
+def symmetric_box(L, passes=1, **kw):
+    """adaptive_box with a symmetryPlane at y = 0 in place of the wall."""
+    spec = adaptive_box(L=L, passes=passes, **kw)
+    spec.mesh.patches[spec.mesh.patch_index("ymin")].type = abi.SYMMETRY
+    spec.boundaries = [b for b in spec.boundaries if b.patch != "ymin"]
+    return spec
+
+
+def test_symmetry_plane_lowers_only_the_normal_levels():
+    """cellSizeMFPRatio is smoothed as a volVectorField, so on a
+    symmetryPlane face it takes its reflection r - n (n.r): the face's normal
+    component is zero.  A uniform gas with L / (0.5 lambda) = 4.5 in a
+    4x4x4 box, one smoothing pass: a cell on the plane has five faces of
+    area a carrying r and one carrying 0 in y, so its y ratio is 5/6 * 4.5 =
+    3.75 (4 subcells) while x and z stay at 4.5 (5 subcells), as does every
+    component away from the plane (walls take the cell value)."""
+    n, T = 1.2e22, 300.0
+    L = 2.25 * mfp(AR, n, T)
+    spec = symmetric_box(L, fn=1e9)
+    o = orc.OracleCase(spec, capacity=1 << 16)
+    o.fill({"Ar": n}, T)
+    lv = o.subcell_levels()
+    jy = (np.arange(spec.mesh.n_cells) // 4) % 4           # box cells are numbered x fastest
+    on = jy == 0
+    assert (lv[on, 1] == 4).all() and (lv[on][:, [0, 2]] == 5).all()
+    assert (lv[~on] == 5).all()
+    o.close()
+
+
 def test_max_ratio_smoothing_is_the_decayed_maximum():
     """fvc::smooth(W, 1.3): every cell ends at the largest W_j / 2.3^d(j, c)
     over all cells j, d the number of faces crossed (breadth-first search

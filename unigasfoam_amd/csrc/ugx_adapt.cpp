@@ -98,9 +98,14 @@ static void smooth_field(const MeshGeom& g, std::vector<double>& f, int ncomp)
                 const double dN = std::fabs((pl[0] * dl[0] + pl[1] * dl[1] + pl[2] * dl[2]) + pl[3]);
                 wt = dN / (dP + dN);
             }
+            // a symmetryPlane patch holds a vector field's reflection: U - n (n.U)
+            const bool mirror = kind == kFaceSymmetry && ncomp == 3;
+            const double* u = &f[(size_t)c * ncomp];
+            const double un = mirror ? pl[0] * u[0] + pl[1] * u[1] + pl[2] * u[2] : 0.0;
             for (int d = 0; d < ncomp; ++d) {
-                const double fv = inner ? wt * f[(size_t)c * ncomp + d] + (1.0 - wt) * f[(size_t)o * ncomp + d]
-                                        : f[(size_t)c * ncomp + d];
+                const double fv = inner    ? wt * f[(size_t)c * ncomp + d] + (1.0 - wt) * f[(size_t)o * ncomp + d]
+                                  : mirror ? u[d] - un * pl[d]
+                                           : f[(size_t)c * ncomp + d];
                 num[d] = num[d] + a * fv;
             }
             den = den + a;
