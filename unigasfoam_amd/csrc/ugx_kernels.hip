@@ -383,6 +383,55 @@ __device__ inline uint32_t chunk_valid(uint64_t base, uint32_t n0)
     return (base < n0) ? (uint32_t)min<uint64_t>(kMoveChunk, n0 - base) : 0u;
 }
 
+// Retired move results of a wave's most recent chunks, staged in LDS so that
+// each chunk is stored to HBM as whole lines once all of its simulators have
+// finished.  Entry e of ring slot q % kMoveRing holds slot base(q) + e; a
+// cell of kMoveUnset marks an entry whose simulator has not finished.
+constexpr uint32_t kMoveRing = 4;
+constexpr uint32_t kMoveUnset = 0xFFFFFFFEu;
+constexpr int kMoveWaves = kBlock / 64;  // waves of a move block
+
+template <class R>
+struct MoveStage {
+    R x[kMoveWaves][kMoveRing][kMoveChunk];
+    R y[kMoveWaves][kMoveRing][kMoveChunk];
+    R z[kMoveWaves][kMoveRing][kMoveChunk];
+    uint32_t cell[kMoveWaves][kMoveRing][kMoveChunk];
+};
+
+// Orders a wave's LDS stores before its later LDS loads from other lanes.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stores the finished entries of ring chunk q (slots base.., valid of them)
+// to HBM, lane k storing entry k.
+template <class R>
+__device__ inline void move_flush(const WorldView<R>& w, MoveStage<R>& st, int wv, uint32_t q, uint64_t base,
+                                  uint32_t valid, uint32_t lane)
+{
+    const uint32_t r = q % kMoveRing;
+    const uint32_t c = st.cell[wv][r][lane];
+    if (lane < valid && c != kMoveUnset) {
+        const uint64_t s = base + lane;
+        w.a.x[s] = st.x[wv][r][lane];
+        w.a.y[s] = st.y[wv][r][lane];
+        w.a.z[s] = st.z[wv][r][lane];
+        w.a.cell[s] = c;
+    }
+}
+
+// True when every entry of ring chunk q has finished.
+template <class R>
+__device__ inline bool move_complete(const MoveStage<R>& st, int wv, uint32_t q, uint32_t valid, uint32_t lane)
+{
+    const uint32_t c = st.cell[wv][q % kMoveRing][lane];
+    return __ballot(lane < valid && c == kMoveUnset) == 0ull;
+}
+
 // Ballistic move of the existing simulators (the face-crossing loop of
 // track_t<true> restated lane by lane).  Specular walls, cyclic and deletion
 // faces are handled in line; a simulator reaching a diffuse wall stops there
@@ -396,10 +445,14 @@ __device__ inline uint32_t chunk_valid(uint64_t base, uint32_t n0)
 // fast flow are spread over all of them.  The records of the current chunk
 // and of the next are held in registers (lane k holds slot base + k), so a
 // refill is a cross-lane exchange rather than a memory round trip; the next
-// chunk is loaded when the current one is used up.  The arithmetic of each
-// simulator is exactly that of track_t<true>.
+// chunk is loaded when the current one is used up.  A finished simulator's
+// position and cell go to the wave's LDS ring (MoveStage) and a chunk is
+// stored as whole lines when its last simulator finishes; a chunk still
+// unfinished when the ring needs its place is stored without the stragglers,
+// which then store their own results.  The arithmetic of each simulator is
+// exactly that of track_t<true>.
 template <class R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_move(WorldView<R> w)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_move(WorldView<R> w)
 {
     TCount<uint32_t> tc{-1, {0u, 0u, 0u, 0u}};
     const uint32_t n0 = w.st->n;
@@ -414,13 +467,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     MoveRec<R> C{}, N{};
     load_chunk(w, cbase, n0, lane, C);
     load_chunk(w, nbase, n0, lane, N);
+    // chunk indices: C is chunk cq, N chunk cq + 1; chunks oldest..cq + 1 are in the ring
+    __shared__ MoveStage<R> stage;
+    const int wv = (int)(threadIdx.x >> 6);
+    const uint64_t base0 = cbase;
+    uint32_t cq = 0u, oldest = 0u;
+    // slots are 32-bit: the sum wraps to the slot exactly
+    auto slot_of = [&](uint32_t v) { return (uint32_t)base0 + (v >> 6) * (uint32_t)cstride + (v & 63u); };
+    stage.cell[wv][0][lane] = kMoveUnset;
+    stage.cell[wv][1][lane] = kMoveUnset;
     // case cache: slots [wb, we) belong to case wc
     int wc = 0;
     uint64_t wb = 1u, we = 0u;
     MoveCase<R> wcs{};
     // lane state
     bool has = false;
-    uint32_t t = 0u, c = 0u, mb = 0u;
+    // tqo: the chunk index of the lane's slot times 64 plus its entry in the chunk
+    uint32_t tqo = 0u, c = 0u, mb = 0u;
     int ccase = 0, it = 0, hits = 0;
     R x[3] = {(R)0, (R)0, (R)0}, U[3] = {(R)0, (R)0, (R)0};
     R tau = (R)0;
@@ -456,11 +519,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                     cc = case_of_slot(w, sl);
                     cs = move_case(w, cc);
                 }
+                const uint32_t q = fromn ? cq + 1u : cq;
+                const uint32_t e = fromn ? k - availc : pos + k;
                 // simulators of a paused case (a retired batch case) stay where they
-                // are; their lane takes another slot
-                if (cs.active) {
+                // are (their entry keeps the record); their lane takes another slot
+                if (!cs.active) {
+                    const uint32_t r = q % kMoveRing;
+                    stage.x[wv][r][e] = g.x; stage.y[wv][r][e] = g.y; stage.z[wv][r][e] = g.z;
+                    stage.cell[wv][r][e] = g.cell;
+                } else {
+                    tqo = (q << 6) | e;
                     has = true;
-                    t = sl;
                     ccase = cc;
                     c = g.cell;
                     if (w.has_weighting) w.cstart[sl] = g.cell;
@@ -481,6 +550,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                 nbase += cstride;
                 nvalid = chunk_valid(nbase, n0);
                 load_chunk(w, nbase, n0, lane, N);
+                cq += 1u;
+                // the new N takes ring place cq + 1: store the oldest chunk first if it
+                // holds that place (stragglers of a chunk stored early store themselves)
+                wave_sync();
+                while (cq + 1u - oldest >= kMoveRing) {
+                    const uint64_t ob = base0 + (uint64_t)oldest * cstride;
+                    move_flush(w, stage, wv, oldest, ob, chunk_valid(ob, n0), lane);
+                    oldest += 1u;
+                }
+                wave_sync();
+                stage.cell[wv][(cq + 1u) % kMoveRing][lane] = kMoveUnset;
+                wave_sync();
             }
             need = __ballot(!has);
         }
@@ -550,6 +631,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                         const WallSink sk = wall_sink(w);
                         if (sk.measured(c, fmin)) {
                             const double post[3] = {(double)U[0], (double)U[1], (double)U[2]};
+                            const uint32_t t = slot_of(tqo);
                             const double er = (double)w.a.erot[t];
                             sk.hit(c, fmin, t, it, pre, post, er, er, w.a.sp[t]);
                         }
@@ -563,8 +645,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         }
         if (fin) {
             has = false;
-            w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
-            w.a.cell[t] = c;
+            const uint32_t t = slot_of(tqo);
+            if ((tqo >> 6) >= oldest) {
+                const uint32_t r = (tqo >> 6) % kMoveRing, e = tqo & 63u;
+                stage.x[wv][r][e] = x[0]; stage.y[wv][r][e] = x[1]; stage.z[wv][r][e] = x[2];
+                stage.cell[wv][r][e] = c;
+            } else {
+                w.a.x[t] = x[0]; w.a.y[t] = x[1]; w.a.z[t] = x[2];
+                w.a.cell[t] = c;
+            }
             if (hits) {
                 w.a.vx[t] = U[0]; w.a.vy[t] = U[1]; w.a.vz[t] = U[2];
             }
@@ -582,6 +671,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
             tcount_add(tc, w.counters, ccase, 1, (uint32_t)hits);
             tcount_add(tc, w.counters, ccase, 2, err ? 1u : 0u);
         }
+        // store the taken chunks whose simulators have all finished
+        wave_sync();
+        while (oldest < cq) {
+            const uint64_t ob = base0 + (uint64_t)oldest * cstride;
+            const uint32_t ov = chunk_valid(ob, n0);
+            if (!move_complete(stage, wv, oldest, ov, lane)) break;
+            move_flush(w, stage, wv, oldest, ob, ov, lane);
+            oldest += 1u;
+        }
+        wave_sync();
+    }
+    // every simulator has finished: store what the ring still holds
+    wave_sync();
+    for (; oldest <= cq; ++oldest) {
+        const uint64_t ob = base0 + (uint64_t)oldest * cstride;
+        move_flush(w, stage, wv, oldest, ob, chunk_valid(ob, n0), lane);
     }
     tcount_block_flush(tc, w.counters);
 }
@@ -2339,11 +2444,13 @@ __device__ inline void macro_params(const WorldView<R>& w, const CaseDev& cs, ui
     }
 }
 
-template <class R>
+// Macro: the world has macroInterpolation cases (a separate instance keeps
+// the interpolation's registers out of the plain draws).
+template <class R, bool Macro>
 __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t i, uint32_t k,
                                 const BgkP& P, unsigned long long* raise, const GlobAcc<R>& A)
 {
-    if (w.has_macro && cs.macro) {
+    if (Macro && cs.macro) {
         BgkP Q = P;
         macro_params<R>(w, cs, c, i, Q);
         bgk_draw_core<R>(w, cs, c, i, k, Q, raise, A);
@@ -2375,7 +2482,7 @@ __global__ void k_point_fields(WorldView<R> w)
 // conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
 // canonical sums, and the USP maxProb update; eight lanes per cell.
 constexpr int kConsReg = 8;   // simulators per lane kept in registers between the sums and the rescale
-template <class R>
+template <class R, bool Macro>
 __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
 {
     const uint32_t tid = threadIdx.x;
@@ -2433,7 +2540,7 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
             for (int q = 0; q < 8; ++q)
                 if (t >= pf[q] && t < pf[q + 1]) { k = t - pf[q]; slot_at = sbg[q] + k; ct = cg[q]; gg = (uint32_t)q; }
             const uint32_t cc = cw + gg;
-            bgk_draw<R>(w, w.cases[ct], cc, w.sel_slot[slot_at], k, w.bgkp[cc], &w.raise[cc], A);
+            bgk_draw<R, Macro>(w, w.cases[ct], cc, w.sel_slot[slot_at], k, w.bgkp[cc], &w.raise[cc], A);
         }
     }
     __threadfence_block();
@@ -3226,7 +3333,10 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
             hipLaunchKernelGGL(k_point_fields<R>, dim3(grid_for(w.npts)), dim3(kBlock), 0, s, w);
         if (w.has_bgk) {
             const unsigned gcons = (unsigned)(((uint64_t)w.nc * 8u + kBlock - 1) / kBlock);
-            hipLaunchKernelGGL(k_bgk_conserve<R>, dim3(gcons), dim3(kBlock), 0, s, w);
+            if (w.has_macro)
+                hipLaunchKernelGGL((k_bgk_conserve<R, true>), dim3(gcons), dim3(kBlock), 0, s, w);
+            else
+                hipLaunchKernelGGL((k_bgk_conserve<R, false>), dim3(gcons), dim3(kBlock), 0, s, w);
         }
     }
     if (w.has_hybrid) {
