@@ -31,7 +31,15 @@ COLLISION_DESC = {"hybrid": "hybrid USP/DSMC (NTC+VHS)", "dsmc": "DSMC (NTC+VHS)
 
 
 def build_workload(name, scale, collision="hybrid"):
+    """(spec, init, desc): init(case) gives a new case its starting state."""
     from unigasfoam_amd import workloads
+    if name == "expansion":
+        nx, ny, nz = int(200 * scale), int(100 * scale), int(100 * scale)
+        spec, state = workloads.nozzle_expansion(nx=nx, ny=ny, nz=nz, per_cell=50)
+        desc = (f"3-D argon nozzle expansion from a 1 bar, 300 K reservoir through a 2 mm throat towards a 1 Pa "
+                f"background, {nx}x{ny}x{nz} hex cells, {nx * ny * nz * 50:.3g} simulators (50/cell, cell-weighted), "
+                f"hybrid USP/DSMC (NTC+VHS), started from the isentropic expansion")
+        return spec, (lambda c: workloads.apply_state(c, state)), desc
     if name == "nozzle":
         nx, ny, nz = int(200 * scale), int(100 * scale), int(100 * scale)
         spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50, collision_model=collision)
@@ -48,7 +56,7 @@ def build_workload(name, scale, collision="hybrid"):
         desc = "0-D argon heat bath, 10^3 cells, 2e4 simulators, VHS, specular box"
     else:
         raise SystemExit(f"unknown workload {name}")
-    return spec, fill, desc
+    return spec, (lambda c: c.fill(**fill)), desc
 
 
 def cpu_baseline(name, seconds, scale):
@@ -61,12 +69,19 @@ def cpu_baseline(name, seconds, scale):
     if name == "nozzle":
         nx, ny, nz = 40, 20, 20
         spec, fill = workloads.nozzle(nx=nx, ny=ny, nz=nz, per_cell=50)
+        init = (lambda c: c.fill(**fill))
         sample = f"nozzle {nx}x{ny}x{nz} cells, {nx * ny * nz * 50} simulators, same per-cell load and models"
+    elif name == "expansion":
+        nx, ny, nz = 40, 20, 20
+        spec, state = workloads.nozzle_expansion(nx=nx, ny=ny, nz=nz, per_cell=50)
+        init = (lambda c: workloads.apply_state(c, state))
+        sample = (f"1 bar expansion {nx}x{ny}x{nz} cells, {nx * ny * nz * 50} simulators, same per-cell load, "
+                  f"weighting and models")
     else:
-        spec, fill, _ = build_workload(name, scale)
+        spec, init, _ = build_workload(name, scale)
         sample = "same workload"
     o = orc.OracleCase(spec, capacity=4 * (spec.mesh.n_cells * 60 + 4096))
-    o.fill(**fill)
+    init(o)
     o.step(1)
     c0 = o.counters()["particle_steps"]
     t0 = time.perf_counter()
@@ -164,13 +179,13 @@ def precision_leg():
             "fields": out}
 
 
-def fp64_line(args, spec, fill, steps=20):
+def fp64_line(args, spec, init, steps=20):
     """The same workload and developed state with all-FP64 particle state
     (the precision whose results equal the oracle bit for bit)."""
     from unigasfoam_amd.core import Case
     print("[bench] FP64 line ...", file=sys.stderr, flush=True)
     case = Case(spec, precision="fp64")
-    case.fill(**fill)
+    init(case)
     case.step(args.develop + args.warmup)
     case.sync()
     c0 = case.counters()
@@ -217,11 +232,11 @@ def main():
     from unigasfoam_amd import roofline
     from unigasfoam_amd.core import Case, probe_copy
 
-    spec, fill, desc = build_workload(args.workload, args.scale, args.collision)
+    spec, init, desc = build_workload(args.workload, args.scale, args.collision)
     print(f"[bench] workload: {desc}", file=sys.stderr, flush=True)
     t = time.perf_counter()
     case = Case(spec, precision=args.precision)
-    case.fill(**fill)
+    init(case)
     case.sync()
     print(f"[bench] setup {time.perf_counter() - t:.1f} s, {case.count()} simulators", file=sys.stderr, flush=True)
     if args.develop > 0:
@@ -304,7 +319,7 @@ def main():
         out["hbm_probe"] = {"copy_bytes": nbytes, "GBs": round(2 * nbytes / s / 1e9, 1)}
     case.close()
     if args.precision == "fp32" and not args.no_fp64_line:
-        out["fp64_line"] = fp64_line(args, spec, fill)
+        out["fp64_line"] = fp64_line(args, spec, init)
     if args.sweep_cases > 0:
         out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision, args.sweep_saturated_rate)
     if not args.no_precision:

@@ -1,0 +1,51 @@
+"""The 1 bar nozzle expansion on the GPU: a case started from the isentropic
+state (cell weights, (sigma_T c_r)max and simulators uploaded) runs
+bit-identically to the oracle started from the same state, through the
+weighting's clones and removals, the reservoir and background inflows and
+the hybrid decomposition; the FP32 state keeps the per-cell load."""
+import numpy as np
+import pytest
+
+from oracle.orc import OracleCase
+from unigasfoam_amd import workloads
+from unigasfoam_amd.core import Case
+
+from test_gpu_parity import assert_identical
+
+pytestmark = pytest.mark.gpu
+
+
+def small_expansion(per_cell=20):
+    spec, st = workloads.nozzle_expansion(nx=16, ny=6, nz=6, per_cell=per_cell)
+    spec.decomposition_interval = 4
+    return spec, st
+
+
+def test_expansion_identical_to_oracle(require_gpu):
+    spec, st = small_expansion()
+    g = Case(spec, precision="fp64", capacity=1 << 17)
+    o = OracleCase(spec, capacity=1 << 17)
+    for c in (g, o):
+        workloads.apply_state(c, st)
+    for k in range(4):
+        g.step(3)
+        o.step(3)
+        assert_identical(g, o, f"expansion after {3 * (k + 1)} steps")
+    cg = g.counters()
+    assert cg["bgk_collisions"] > 1000
+    assert np.array_equal(g.cell_model(), o.cell_model())
+    g.close()
+    o.close()
+
+
+def test_expansion_fp32_keeps_the_load(require_gpu):
+    spec, st = small_expansion(per_cell=50)
+    g = Case(spec, precision="fp32", capacity=1 << 18)
+    workloads.apply_state(g, st)
+    n0 = g.count()
+    g.step(40)
+    g.sync()
+    n1 = g.count()
+    assert 0.7 * n0 < n1 < 1.4 * n0, (n0, n1)
+    assert np.isfinite(g.particles()["velocity"]).all()
+    g.close()

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <utility>
 
 #include "ugx_kernels.hpp"
@@ -2151,13 +2152,16 @@ __device__ inline void bgk_params(const WorldView<R>& w, const CaseDev& cs, uint
 // displaced entries: after step i only positions above i are read again,
 // and each step displaces at most one entry (to position j > i), so the
 // list holds the (position, entry) pairs that differ from the identity and
-// a lookup falls back to the position itself.  Larger selections swap in
-// the cell's index array.
+// a lookup falls back to the position itself.  Larger selections in cells
+// of up to kSelLds simulators swap in a lane-private index array in LDS
+// (perm, entry e of the lane at perm[e * kBlock]); larger cells swap in the
+// cell's index array in HBM.
 constexpr int kSelReg = 16;
+constexpr uint32_t kSelLds = 96;
 
 template <class R>
 __device__ inline void bgk_select(const WorldView<R>& w, const CaseDev& cs, uint32_t c, uint32_t b, uint32_t nC,
-                                  uint32_t nrel, uint32_t base, const GlobAcc<R>& A)
+                                  uint32_t nrel, uint32_t base, const GlobAcc<R>& A, uint16_t* perm)
 {
     Rng r;
     rng_init(r, cs.seed_lo, cs.seed_hi, kRngBgk, 0u, c - (uint32_t)cs.cell_begin, cs.step);
@@ -2185,6 +2189,17 @@ __device__ inline void bgk_select(const WorldView<R>& w, const CaseDev& cs, uint
                 }
                 if (!have_j) ++nd;
             }
+            w.sel_slot[base + i] = b + tj;
+        }
+        return;
+    }
+    if (nC <= kSelLds) {
+        for (uint32_t i = 0; i < nC; ++i) perm[i * kBlock] = (uint16_t)i;
+        for (uint32_t i = 0; i < nrel; ++i) {
+            const uint32_t j = i + uindex(r, nC - i);
+            const uint32_t ti = perm[i * kBlock], tj = perm[j * kBlock];
+            perm[i * kBlock] = (uint16_t)tj;
+            perm[j * kBlock] = (uint16_t)ti;
             w.sel_slot[base + i] = b + tj;
         }
         return;
@@ -2258,7 +2273,8 @@ __global__ void __launch_bounds__(kBlock) k_collide(WorldView<R> w)
         w.bgkp[c].sel_base = base;
         w.bgkp[c].nrel = nrel;
     }
-    if (nrel) bgk_select<R>(w, w.cases[ccase], c, b, nC, nrel, base, A);
+    __shared__ uint16_t s_perm[kSelLds * kBlock];
+    if (nrel) bgk_select<R>(w, w.cases[ccase], c, b, nC, nrel, base, A, &s_perm[threadIdx.x]);
     if (dsmc && nC >= 2u) ntc_cell<R>(w, w.cases[ccase], c, b, nC, A, cand, coll);
     case_add(w.counters, ccase, kCntCandidates, cand);
     case_add(w.counters, ccase, kCntCollisions, coll);
@@ -2459,6 +2475,115 @@ __device__ inline void bgk_draw(const WorldView<R>& w, const CaseDev& cs, uint32
     }
 }
 
+// One selected simulator's draw held by a lane across its rejection trials
+// (the same stream and arithmetic as bgk_draw_core, one trial at a time).
+// T is the draw's precision: single for the FP32 state.  e0..e8 hold the USP
+// stress (xx xy xz yy yz zz) or the ES-BGK matrix (row major).  Scalar
+// members, not arrays, so that the state stays in registers.
+template <class R>
+struct DrawLane {
+    using T = typename std::conditional<sizeof(R) == sizeof(float), float, double>::type;
+    Rng rv;
+    T m0, inv_m0, a, bq, q0, q1, q2, e0, e1, e2, e3, e4, e5, e6, e7, e8;
+    double u0, U0, U1, U2;
+    uint32_t i, cc;
+    int model, it;
+};
+
+// The draw's parameter f of the cell (BgkP's doubles in order: U 0-2, s2kT 3,
+// p 4, cQ 5, cS 6, m0 7, T 8, q 9-11, sm 12-20) comes from get(f).
+constexpr int kDrawPars = 21;
+
+__device__ __attribute__((always_inline)) inline double bgkp_field(const BgkP& P, int f)
+{
+    switch (f) {
+    case 0: return P.U[0];
+    case 1: return P.U[1];
+    case 2: return P.U[2];
+    case 3: return P.s2kT;
+    case 4: return P.p;
+    case 5: return P.cQ;
+    case 6: return P.cS;
+    case 7: return P.m0;
+    case 8: return P.T;
+    case 9: return P.q[0];
+    case 10: return P.q[1];
+    case 11: return P.q[2];
+    default: return P.sm[f - 12];
+    }
+}
+
+template <class R, class Get>
+__device__ __attribute__((always_inline)) inline void draw_setup(const WorldView<R>& w, const CaseDev& cs, uint32_t c,
+                                                                 uint32_t i, uint32_t k, int model, Get get,
+                                                                 const GlobAcc<R>& A, DrawLane<R>& D)
+{
+    using T = typename DrawLane<R>::T;
+    rng_init(D.rv, cs.seed_lo, cs.seed_hi, kRngBgkV, k, c - (uint32_t)cs.cell_begin, cs.step);
+    const int ls = (cs.ns == 1) ? 0 : A.local_species(i);
+    D.u0 = get(3) / sqrt(w.sp.mass[cs.sp_base + ls]);
+    D.model = model;
+    D.i = i;
+    D.cc = c;
+    D.it = 0;
+    D.U0 = get(0); D.U1 = get(1); D.U2 = get(2);
+    if (model == UGX_BGK_USP) {
+        D.m0 = (T)get(7);
+        D.a = (T)(get(5) / (get(4) * D.u0));
+        D.bq = (T)(get(6) / get(4));
+        D.inv_m0 = (T)1 / D.m0;
+        D.q0 = (T)get(9); D.q1 = (T)get(10); D.q2 = (T)get(11);
+    }
+    if (model == UGX_BGK_USP || model == UGX_BGK_ESBGK) {
+        D.e0 = (T)get(12); D.e1 = (T)get(13); D.e2 = (T)get(14); D.e3 = (T)get(15); D.e4 = (T)get(16);
+        D.e5 = (T)get(17);
+    }
+    if (model == UGX_BGK_ESBGK) {
+        D.e6 = (T)get(18); D.e7 = (T)get(19); D.e8 = (T)get(20);
+    }
+}
+
+// One trial: true when (x0, x1, x2) holds the drawn thermal velocity
+// (accepted, the only draw of the BGK and ES-BGK models, or the last
+// allowed USP trial).
+template <class R>
+__device__ __attribute__((always_inline)) inline bool draw_trial(DrawLane<R>& D, unsigned long long* raise,
+                                                                 typename DrawLane<R>::T& x0,
+                                                                 typename DrawLane<R>::T& x1,
+                                                                 typename DrawLane<R>::T& x2)
+{
+    using T = typename DrawLane<R>::T;
+    T g0, g1, g2;
+    if constexpr (sizeof(T) == sizeof(float)) gauss3f(D.rv, g0, g1, g2);
+    else gauss3(D.rv, g0, g1, g2);
+    x0 = g0 * (T)kSqrtHalf; x1 = g1 * (T)kSqrtHalf; x2 = g2 * (T)kSqrtHalf;
+    if (D.model == UGX_BGK_USP) {
+        const T vSq = x0 * x0 + x1 * x1 + x2 * x2;
+        const T vTr = vSq * (T)kThird;
+        const T prob = (T)1 + D.a * (D.q0 * x0 + D.q1 * x1 + D.q2 * x2) * (vSq * (T)0.4 - (T)1)
+                       + D.bq * (D.e0 * (x0 * x0 - vTr) + D.e3 * (x1 * x1 - vTr)
+                                 + D.e5 * (x2 * x2 - vTr) + (T)2 * D.e1 * x0 * x1
+                                 + (T)2 * D.e2 * x0 * x2 + (T)2 * D.e4 * x1 * x2);
+        if (prob > D.m0 && prob < (T)10) {
+            atomicMax(raise, (unsigned long long)double_to_bits((double)prob));
+            return true;
+        }
+        T u;
+        if constexpr (sizeof(T) == sizeof(float)) u = uniformf(D.rv);
+        else u = uniform(D.rv);
+        if (u < prob * D.inv_m0) return true;
+        D.it += 1;
+        return D.it >= 100000;
+    }
+    if (D.model == UGX_BGK_ESBGK) {
+        const T y0 = D.e0 * x0 + D.e1 * x1 + D.e2 * x2;
+        const T y1 = D.e3 * x0 + D.e4 * x1 + D.e5 * x2;
+        const T y2 = D.e6 * x0 + D.e7 * x1 + D.e8 * x2;
+        x0 = y0; x1 = y1; x2 = y2;
+    }
+    return true;
+}
+
 // The point values of the step's fields (volPointInterpolation): per point
 // the weighted sum of its cells' rows, in the order of its entries.
 template <class R>
@@ -2478,6 +2603,9 @@ __global__ void k_point_fields(WorldView<R> w)
         for (int q = 0; q < kMiRows; ++q) w.pfield[(size_t)kMiRows * p + q] = v[q];
     }
 }
+
+// Selections of a conservation wave staged in LDS at a time.
+constexpr uint32_t kDrawWindow = 512;
 
 // conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
 // canonical sums, and the USP maxProb update; eight lanes per cell.
@@ -2533,14 +2661,82 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
         }
         const uint32_t cw = (blockIdx.x * blockDim.x + (tid & ~63u)) >> 3;
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
-        for (uint32_t t = lane; t < pf[8]; t += 64u) {
-            uint32_t k = 0u, slot_at = 0u, gg = 0u;
-            int ct = 0;
+        // the eight cells' draw parameters and a window of the wave's
+        // selections are staged in LDS, so that a draw's set-up makes no
+        // memory round trip
+        __shared__ double s_par[kBlock / 8][kDrawPars + 1];
+        __shared__ uint32_t s_sel[kBlock / 64][kDrawWindow];
+        const uint32_t wv = tid >> 6, cl0 = wv * 8u;
+        if (c < (uint32_t)w.nc && nrel > 0u) {
+            const double* src = reinterpret_cast<const double*>(&w.bgkp[c]);
+            for (uint32_t f = j; f < (uint32_t)kDrawPars; f += 8u) s_par[tid >> 3][f] = src[f];
+            if (j == 0u) s_par[tid >> 3][kDrawPars] = (double)w.bgkp[c].model;
+        }
+        uint32_t w0 = 0u;
+        auto stage_sel = [&](uint32_t from) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (t >= pf[q] && t < pf[q + 1]) { k = t - pf[q]; slot_at = sbg[q] + k; ct = cg[q]; gg = (uint32_t)q; }
-            const uint32_t cc = cw + gg;
-            bgk_draw<R, Macro>(w, w.cases[ct], cc, w.sel_slot[slot_at], k, w.bgkp[cc], &w.raise[cc], A);
+            for (uint32_t m = 0; m < kDrawWindow / 64u; ++m) {
+                const uint32_t t = from + lane + 64u * m;
+                if (t < pf[8]) {
+                    uint32_t slot_at = 0u;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (t >= pf[q] && t < pf[q + 1]) slot_at = sbg[q] + (t - pf[q]);
+                    s_sel[wv][t - from] = w.sel_slot[slot_at];
+                }
+            }
+        };
+        stage_sel(0u);
+        wave_sync();
+        // a lane whose draw is accepted takes the next draw of the wave at
+        // once, so the USP rejection trials of a wave do not wait for its
+        // slowest lane; each draw keeps its own stream, whichever lane runs it
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const uint32_t total = pf[8];
+        uint32_t tnext = 0u;
+        bool has = false;
+        DrawLane<R> D;
+        for (;;) {
+            const unsigned long long need = __ballot(!has);
+            if (need != 0ull && tnext < total) {
+                if (tnext + 64u > w0 + kDrawWindow) {
+                    // the next draws leave the staged window: stage the one from tnext
+                    wave_sync();
+                    w0 = tnext;
+                    stage_sel(w0);
+                    wave_sync();
+                }
+                const uint32_t t = tnext + (uint32_t)__popcll(need & below);
+                if (!has && t < total) {
+                    uint32_t k = 0u, gg = 0u;
+                    int ct = 0;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (t >= pf[q] && t < pf[q + 1]) { k = t - pf[q]; ct = cg[q]; gg = (uint32_t)q; }
+                    const uint32_t cc = cw + gg;
+                    const CaseDev& cs = w.cases[ct];
+                    const uint32_t i = s_sel[wv][t - w0];
+                    const double* sp = s_par[cl0 + gg];
+                    const int model = (int)sp[kDrawPars];
+                    if (Macro && cs.macro) {
+                        BgkP Q = w.bgkp[cc];
+                        macro_params<R>(w, cs, cc, i, Q);
+                        draw_setup<R>(w, cs, cc, i, k, Q.model, [&](int f) { return bgkp_field(Q, f); }, A, D);
+                    } else {
+                        draw_setup<R>(w, cs, cc, i, k, model, [&](int f) { return sp[f]; }, A, D);
+                    }
+                    has = true;
+                }
+                tnext += (uint32_t)__popcll(need);
+            }
+            if (__ballot(has) == 0ull) break;
+            if (has) {
+                typename DrawLane<R>::T x0, x1, x2;
+                if (draw_trial<R>(D, &w.raise[D.cc], x0, x1, x2)) {
+                    A.store(D.i, D.U0 + D.u0 * (double)x0, D.U1 + D.u0 * (double)x1, D.U2 + D.u0 * (double)x2);
+                    has = false;
+                }
+            }
         }
     }
     __threadfence_block();

@@ -4,6 +4,8 @@ heat_bath   0-D argon heat bath, 10^3 cells, specular box, 2e4 simulators
 couette     1-D N2 Couette flow (LB rotational exchange), 500 cells, 1e5 simulators
 cylinder    Mach-10 N2/O2 flow over a half cylinder (symmetry plane), 400x200x1
 nozzle      3-D argon nozzle-type expansion, 2e6 cells, 1e8 simulators, hybrid USP/DSMC
+nozzle_expansion  the same nozzle from a 1 bar reservoir towards a 1 Pa background:
+            cell-weighted, started from the isentropic expansion (spec, state)
 sweep       parametric Couette/Fourier cases for the batched and served forms
 
 Each function returns (CaseSpec, fill kwargs).  Meshes are structured
@@ -108,14 +110,19 @@ def cylinder(n_theta=400, n_r=200, r0=0.1, R=1.0, depth=0.01, n_inf=1.0e21, T_in
     return spec, fill
 
 
+def nozzle_half_width(x, L=0.02, h_in=4.0e-3, h_throat=1.0e-3, h_out=6.0e-3):
+    """Half-width h(x) of the nozzle's square cross-section: h_in at the inlet,
+    h_throat at the throat x = L/4, h_out at the exit."""
+    xt = 0.25 * L
+    return np.where(x < xt, h_throat + (h_in - h_throat) * ((xt - x) / xt) ** 2,
+                    h_throat + (h_out - h_throat) * np.clip((x - xt) / (L - xt), 0.0, None) ** 1.5)
+
+
 def nozzle_mesh(nx, ny, nz, L=0.02, h_in=4.0e-3, h_throat=1.0e-3, h_out=6.0e-3):
     """Converging-diverging channel: square cross-section of half-width h(x),
     throat at x = L/4; cross-sections scaled along x keep every face planar."""
-    xt = 0.25 * L
-
     def h(x):
-        return np.where(x < xt, h_throat + (h_in - h_throat) * ((xt - x) / xt) ** 2,
-                        h_throat + (h_out - h_throat) * np.clip((x - xt) / (L - xt), 0.0, None) ** 1.5)
+        return nozzle_half_width(x, L, h_in, h_throat, h_out)
 
     def fn(S):
         s, t, u = S[..., 0], S[..., 1], S[..., 2]
@@ -156,6 +163,113 @@ def nozzle(nx=200, ny=100, nz=100, per_cell=50, n_stag=2.4e25, T0=300.0, n_back=
                     breakdown_max=0.05, kn_theta=0.5)
     fill = dict(number_density={"Ar": n_fill}, T=T0, exact_per_cell=per_cell)
     return spec, fill
+
+
+def isentropic_mach(area_ratio, supersonic):
+    """Mach number of isentropic quasi-one-dimensional flow of a monatomic
+    gas (gamma = 5/3) at area ratio A/A* >= 1.  For gamma = 5/3 the
+    area-Mach relation (1/M) [(2/(g+1))(1 + (g-1)/2 M^2)]^((g+1)/(2(g-1)))
+    is ((3 + M^2)/4)^2 / M; it is bisected on the subsonic or the supersonic
+    branch."""
+    r = np.asarray(area_ratio, dtype=np.float64)
+    sup = np.broadcast_to(np.asarray(supersonic, dtype=bool), r.shape)
+    lo = np.where(sup, 1.0, 1.0e-9)
+    hi = np.where(sup, 100.0, 1.0)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        f = ((3.0 + mid * mid) / 4.0) ** 2 / mid - r
+        up = np.where(sup, f < 0.0, f > 0.0)      # the root lies above mid
+        lo = np.where(up, mid, lo)
+        hi = np.where(up, hi, mid)
+    return 0.5 * (lo + hi)
+
+
+def nozzle_expansion(nx=200, ny=100, nz=100, per_cell=50, p0=1.0e5, T0=300.0, p_back=1.0, T_back=300.0,
+                     dt=1.0e-8, bgk_model="unifiedStochasticParticleSBGK", seed=4, L=0.02, h_in=4.0e-3,
+                     h_throat=1.0e-3, h_out=6.0e-3):
+    """Argon expanding from a 1 bar, 300 K reservoir through the nozzle's
+    2 mm throat towards a 1 Pa background, with cell weighting so that every
+    cell holds per_cell simulators although the density falls by two orders
+    of magnitude along the nozzle (and by five to the background).
+
+    The initial state is the isentropic quasi-one-dimensional expansion:
+    per cell column, the Mach number of the column's area ratio (subsonic
+    before the throat, supersonic after it), T = T0 / (1 + M^2 / 3),
+    n = n0 (T / T0)^(3/2) and U = M sqrt(5 k T / (3 m)) along x.  The inlet
+    feeds the reservoir gas at the inlet state, the outlet the 1 Pa
+    background; both also delete what leaves.  Returns (spec, state): the
+    state's cell weights, (sigma_T c_r)max and simulators are applied to a
+    case by apply_state."""
+    mesh = nozzle_mesh(nx, ny, nz, L, h_in, h_throat, h_out)
+    m = ARGON.mass
+    n0 = p0 / (KB * T0)
+    xs = L * np.arange(nx + 1) / nx                       # column faces
+    hf = nozzle_half_width(xs, L, h_in, h_throat, h_out)
+    xc = 0.5 * (xs[:-1] + xs[1:])
+    hc = 0.5 * (hf[:-1] + hf[1:])
+    Mc = isentropic_mach((hc / h_throat) ** 2, xc > 0.25 * L)
+    Tc = T0 / (1.0 + Mc * Mc / 3.0)
+    nc_col = n0 * (Tc / T0) ** 1.5
+    Uc = Mc * np.sqrt(5.0 * KB * Tc / (3.0 * m))
+    # column volumes: the cross-section's half-width is linear in x between
+    # the column's faces, so V = (4 / (ny nz)) dx (h0^2 + h0 h1 + h1^2) / 3
+    dx = L / nx
+    vol_col = 4.0 / (ny * nz) * dx * (hf[:-1] ** 2 + hf[:-1] * hf[1:] + hf[1:] ** 2) / 3.0
+    # F_N so that an inlet cell has weight 1
+    fn = nc_col[0] * vol_col[0] / per_cell
+    wcol = nc_col * vol_col / (per_cell * fn)
+    # inlet state: the first column's (the reservoir gas flowing in)
+    M_in = float(isentropic_mach((h_in / h_throat) ** 2, False))
+    T_in = T0 / (1.0 + M_in * M_in / 3.0)
+    n_in = n0 * (T_in / T0) ** 1.5
+    U_in = M_in * math.sqrt(5.0 * KB * T_in / (3.0 * m))
+    n_back = p_back / (KB * T_back)
+    bnd = [Boundary("inlet", abi.BND_DELETION),
+           Boundary("inlet", abi.BND_FREESTREAM, temperature=T_in, velocity=(U_in, 0.0, 0.0),
+                    number_density={"Ar": n_in}),
+           Boundary("outlet", abi.BND_DELETION),
+           Boundary("outlet", abi.BND_FREESTREAM, temperature=T_back, velocity=(0.0, 0.0, 0.0),
+                    number_density={"Ar": n_back}),
+           Boundary("ymin", abi.BND_DIFFUSE, temperature=T0), Boundary("ymax", abi.BND_DIFFUSE, temperature=T0),
+           Boundary("zmin", abi.BND_DIFFUSE, temperature=T0), Boundary("zmax", abi.BND_DIFFUSE, temperature=T0)]
+    spec = CaseSpec(mesh=mesh, species=[ARGON], delta_t=dt, n_equivalent_particles=fn,
+                    collision_model="hybrid", dsmc_model="variableHardSphere", bgk_model=bgk_model,
+                    Tref=273.0, boundaries=bnd, seed=seed, decomposition_interval=20, smoothing_passes=5,
+                    breakdown_max=0.05, kn_theta=0.5, cell_weighted=True, particles_per_subcell=per_cell)
+    ncell = nx * ny * nz
+    col = np.arange(ncell) % nx
+    # simulators: per_cell in every cell, uniform in the cell's parameters
+    # (the trilinear map of its corners), Maxwellian at the column's state
+    rng = np.random.default_rng(seed)
+    cell = np.repeat(np.arange(ncell, dtype=np.int32), per_cell)
+    i = cell % nx
+    j = (cell // nx) % ny
+    k = cell // (nx * ny)
+    a = rng.random(len(cell))
+    x = xs[i] + a * dx
+    hl = hf[i] + a * (hf[i + 1] - hf[i])
+    y = hl * (2.0 * (j + rng.random(len(cell))) / ny - 1.0)
+    z = hl * (2.0 * (k + rng.random(len(cell))) / nz - 1.0)
+    del a
+    pos = np.stack([x, y, z], axis=1)
+    del x, y, z, hl, j, k
+    sd = np.sqrt(KB * Tc[i] / m)
+    vel = rng.standard_normal((len(cell), 3)) * sd[:, None]
+    vel[:, 0] += Uc[i]
+    del sd, i
+    d = ARGON.diameter
+    state = {"cell_weights": wcol[col], "sigma_max": math.pi * d * d * np.sqrt(2.0 * KB * Tc[col] / m),
+             "position": pos, "velocity": vel, "cell": cell,
+             "columns": {"x": xc, "M": Mc, "T": Tc, "n": nc_col, "U": Uc, "volume": vol_col, "weight": wcol}}
+    return spec, state
+
+
+def apply_state(case, state):
+    """Give a new case (GPU or oracle) the cell weights, (sigma_T c_r)max and
+    simulators of a nozzle_expansion state."""
+    case.set_cell_weights(state["cell_weights"])
+    case.set_sigma_max(state["sigma_max"])
+    case.upload(velocity=state["velocity"], cell=state["cell"], position=state["position"])
 
 
 def sweep_params(n_cases=256, seed=5):
