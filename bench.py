@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the particle time step (move + collide + sample) on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload nozzle]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload expansion]
 
 A step is one uniGasCloud::evolve() equivalent over the whole case.  The
-default workload is the 3-D nozzle expansion: 200x100x100 planar-faced hexes
-(2e6 cells), 1e8 argon simulators (50 per cell), FP32 particle state with
-FP64 sampler / (sigma_T c_r)max / NTC accumulators.  value = particle-steps/s
-= sum over timed steps of the simulator count / wall time, with the inputs
-resident in HBM, timed after the flow has developed from the fill (DSMC
-cells switched on, collisions running).  Prints one JSON line with the
-per-phase HIP-event timings,
-the roofline of the dominant kernel and a CPU baseline (the FP64 OpenMP
-oracle on a bounded sample of the same workload).
+default workload is the 3-D nozzle expansion of argon from a 1 bar, 300 K
+reservoir through a 2 mm throat towards a 1 Pa background: 200x100x100
+planar-faced hexes (2e6 cells), 1e8 simulators (50 per cell, cell-weighted),
+hybrid USP/DSMC switching on the local Knudsen number, dt = 1e-8 s, started
+from the isentropic expansion, FP32 particle state with FP64 sampler /
+(sigma_T c_r)max / NTC accumulators.  value = particle-steps/s = sum over
+timed steps of the simulator count / wall time, with the inputs resident in
+HBM, timed after 300 steps (three hybrid decompositions and more).  Prints
+one JSON line with the per-phase HIP-event timings, the roofline of the
+dominant phase, an all-FP64 line, the mixed-regime nozzle (uniform fill at
+1e-3 of the reservoir density), the batched/served sweep, the FP32-vs-FP64
+field comparison and a CPU baseline (the FP64 OpenMP oracle on a bounded
+sample of the same workload).
 """
 import argparse
 import json
@@ -179,6 +183,35 @@ def precision_leg():
             "fields": out}
 
 
+def nozzle_line(args, steps=20):
+    """The mixed-regime nozzle (a uniform fill at 1e-3 of the reservoir
+    density, no cell weighting: DSMC and USP cells side by side), timed the
+    same way as the headline."""
+    from unigasfoam_amd.core import Case
+    print("[bench] mixed-regime nozzle line ...", file=sys.stderr, flush=True)
+    spec, init, desc = build_workload("nozzle", args.scale, args.collision)
+    case = Case(spec, precision=args.precision)
+    init(case)
+    case.step(args.develop + args.warmup)
+    case.sync()
+    c0 = case.counters()
+    case.timing(True)
+    case.sync()
+    t0 = time.perf_counter()
+    case.step(steps)
+    case.sync()
+    el = time.perf_counter() - t0
+    c1 = case.counters()
+    phases, tsteps = case.phase_times()
+    case.close()
+    ps = c1["particle_steps"] - c0["particle_steps"]
+    return {"workload": desc, "value": ps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3,
+            "steps": steps, "untimed_steps": args.develop + args.warmup,
+            "phases_ms": {p: round(ms / max(tsteps, 1), 4) for p, ms in phases.items()},
+            "counters_per_step": {k: (c1[k] - c0[k]) / steps for k in ("candidates", "collisions", "bgk_collisions")},
+            "dsmc_cells": c1["n_dsmc_cells"]}
+
+
 def fp64_line(args, spec, init, steps=20):
     """The same workload and developed state with all-FP64 particle state
     (the precision whose results equal the oracle bit for bit)."""
@@ -209,7 +242,10 @@ def main():
                     help="steps run from the fill before the warm-up, so the timed steps see a developed flow "
                          "(DSMC cells, collisions) rather than the initial state")
     ap.add_argument("--no-fp64-line", action="store_true", help="skip the all-FP64 timing of the same workload")
-    ap.add_argument("--workload", default="nozzle")
+    ap.add_argument("--workload", default="expansion",
+                    help="expansion (the 1 bar nozzle expansion, default), nozzle (mixed-regime nozzle), cylinder, "
+                         "couette, heat_bath")
+    ap.add_argument("--no-nozzle-line", action="store_true", help="skip the mixed-regime nozzle line")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the nozzle mesh")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -301,7 +337,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "form": "plain (one case)", "precision":
                    "FP32 particle state, FP64 samplers/sigma_max/NTC" if R == 4 else "all FP64",
-                   "state": f"developed: timed after {args.develop + args.warmup} steps from the fill",
+                   "state": (f"timed after {args.develop + args.warmup} steps from the "
+                             + ("isentropic start" if args.workload == "expansion" else "fill")),
                    "decomposition_interval": spec.decomposition_interval},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
@@ -320,6 +357,8 @@ def main():
     case.close()
     if args.precision == "fp32" and not args.no_fp64_line:
         out["fp64_line"] = fp64_line(args, spec, init)
+    if args.workload == "expansion" and not args.no_nozzle_line:
+        out["mixed_nozzle_line"] = nozzle_line(args)
     if args.sweep_cases > 0:
         out["sweep"] = sweep_leg(args.sweep_cases, args.sweep_rate, args.precision, args.sweep_saturated_rate)
     if not args.no_precision:
