@@ -1237,63 +1237,75 @@ __global__ void k_gather(WorldView<R> w)
 
 // --------------------------------------------------------------- scan
 
-__global__ void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* bsum)
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t v)
 {
-    __shared__ uint32_t sh[kBlock];
-    uint32_t b = blockIdx.x * kScanBlock;
-    uint32_t s = 0;
-    for (int k = 0; k < 4; ++k) {
-        uint32_t i = b + threadIdx.x * 4 + k;
-        if (i < n) s += in[i];
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        if (lane >= o) v += u;
     }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
+    return v;
 }
 
+// Block sums of kScanBlock-entry blocks (256 threads, four entries each).
+__global__ void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* bsum)
+{
+    __shared__ uint32_t sh[kBlock / 64];
+    const uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (b + k < n) s += in[b + k];
+    s = wave_incl_scan_u32(s);
+    if ((threadIdx.x & 63u) == 63u) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int q = 0; q < kBlock / 64; ++q) t += sh[q];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of the nb block sums in place, one block of 1024 threads:
+// each thread sums a contiguous segment, the segment totals are scanned
+// across the block, and each thread writes its segment's prefixes.
 __global__ void k_scan_bsum(uint32_t* bsum, uint32_t nb)
 {
-    __shared__ uint32_t sh[1024];
-    const int tid = threadIdx.x;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += 1024) {
-        uint32_t k = base + tid;
-        uint32_t v0 = (k < nb) ? bsum[k] : 0u;
-        sh[tid] = v0;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint32_t v = (tid >= o) ? sh[tid - o] : 0u;
-            __syncthreads();
-            sh[tid] += v;
-            __syncthreads();
-        }
-        if (k < nb) bsum[k] = carry + sh[tid] - v0;
-        uint32_t tot = sh[1023];
-        __syncthreads();
-        carry += tot;
+    __shared__ uint32_t sh[16];
+    const uint32_t tid = threadIdx.x, wv = tid >> 6;
+    const uint32_t seg = (nb + 1023u) / 1024u;
+    const uint32_t lo = min(nb, tid * seg), hi = min(nb, lo + seg);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += bsum[i];
+    const uint32_t inc = wave_incl_scan_u32(s);
+    if ((tid & 63u) == 63u) sh[wv] = inc;
+    __syncthreads();
+    uint32_t run = inc - s;
+    for (uint32_t q = 0; q < wv; ++q) run += sh[q];
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
     }
 }
 
 // Exclusive scan of in[0..n) into out; out[n-1] is the total when in[n-1]==0.
 __global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* bsum, uint32_t* out, uint32_t* total_out)
 {
-    __shared__ uint32_t sh[kBlock];
-    uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    __shared__ uint32_t sh[kBlock / 64];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t b = blockIdx.x * kScanBlock + threadIdx.x * 4;
     uint32_t v[4], s = 0;
+#pragma unroll
     for (int k = 0; k < 4; ++k) { v[k] = (b + k < n) ? in[b + k] : 0u; s += v[k]; }
-    sh[threadIdx.x] = s;
+    const uint32_t inc = wave_incl_scan_u32(s);
+    if ((threadIdx.x & 63u) == 63u) sh[wv] = inc;
     __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-        uint32_t t = ((int)threadIdx.x >= o) ? sh[threadIdx.x - o] : 0u;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = bsum[blockIdx.x] + sh[threadIdx.x] - s;
+    uint32_t run = bsum[blockIdx.x] + inc - s;
+    for (uint32_t q = 0; q < wv; ++q) run += sh[q];
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (b + k < n) {
             out[b + k] = run;
@@ -1303,11 +1315,25 @@ __global__ void k_scan_apply(const uint32_t* in, uint32_t n, const uint32_t* bsu
     }
 }
 
+// Exclusive scan of in[0..n) into out.  bsum holds scan_bsum_words(n) words:
+// the nb block sums, and past them, when nb exceeds kScanDirect, the sums of
+// the block sums (scanned in blocks themselves, as a single block scanning
+// many sums is slow).
+constexpr uint32_t kScanDirect = 4096;
+
 static void scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* bsum, uint32_t* total, hipStream_t s)
 {
-    uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+    const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, bsum);
-    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, s, bsum, nb);
+    if (nb > kScanDirect) {
+        uint32_t* b2 = bsum + nb + 1;
+        const uint32_t nb2 = (nb + kScanBlock - 1) / kScanBlock;
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb2), dim3(kBlock), 0, s, bsum, nb, b2);
+        hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, s, b2, nb2);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb2), dim3(kBlock), 0, s, bsum, nb, b2, bsum, nullptr);
+    } else {
+        hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, s, bsum, nb);
+    }
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s, in, n, bsum, out, total);
 }
 

@@ -46,6 +46,14 @@ struct BgkP {
     uint32_t nrel;      // number of selections
 };
 
+// Words of the block-sum buffer of an exclusive scan of n entries (1024
+// entries per block): the block sums, then the sums of the block sums.
+inline size_t scan_bsum_words(uint64_t n)
+{
+    const size_t nb = (size_t)((n + 1023) / 1024);
+    return nb + 1 + (nb + 1023) / 1024 + 2;
+}
+
 // Per-case cumulative counters (uint64).
 enum CounterIdx {
     kCntCandidates = 0, kCntCollisions, kCntBgk, kCntInserted, kCntDeleted,

@@ -447,7 +447,7 @@ int World::alloc_particles(uint32_t newcap)
         if (int rc = dalloc(d_cstart, newcap)) return rc;
         if (int rc = dalloc(d_wclone, (size_t)newcap + 1)) return rc;
         if (int rc = dalloc(d_wcoff, (size_t)newcap + 1)) return rc;
-        if (int rc = dalloc(d_wbsum, ((size_t)newcap + 1 + 1023) / 1024 + 2)) return rc;
+        if (int rc = dalloc(d_wbsum, scan_bsum_words((uint64_t)newcap + 1))) return rc;
     }
     if (has_subcells) {
         dfree(d_scid);
@@ -930,7 +930,7 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         std::vector<double> zw((size_t)nwf * kNS * kWRows, 0.0);
         if ((rc = up(d_wsamp, zw))) return rc;
     }
-    size_t nbs = (std::max<size_t>((size_t)nc * kNS + 2, (size_t)nwf + 1) + 1023) / 1024 + 2;
+    size_t nbs = scan_bsum_words(std::max<size_t>((size_t)nc * kNS + 2, (size_t)nwf + 1));
     if ((rc = dalloc(d_blocksum, nbs))) return rc;
     std::vector<unsigned long long> zcnt(specs.size() * kCntNum, 0ull);
     if ((rc = up(d_counters, zcnt))) return rc;
