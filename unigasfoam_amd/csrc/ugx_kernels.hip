@@ -751,6 +751,7 @@ __global__ void k_weight(WorldView<R> w)
     const uint32_t n_all = n0 + w.st->n_ins;
     // every entry up to cap is written (zero past the simulators): the scan
     // runs over a host-known length
+    const bool one_case = w.ncases == 1;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t <= w.cap; t += gridDim.x * blockDim.x) {
         uint32_t k = 0u;
         bool del = false;
@@ -758,7 +759,7 @@ __global__ void k_weight(WorldView<R> w)
         if (t < n_all) {
             const uint32_t c = w.a.cell[t];
             if (c != kDead) {
-                ccase = w.cell_case[c];
+                ccase = one_case ? 0 : w.cell_case[c];
                 const CaseDev& cs = w.cases[ccase];
                 if (cs.active && cs.weighted) {
                     const double wo = w.cw[w.cstart[t]], wn = w.cw[c];
