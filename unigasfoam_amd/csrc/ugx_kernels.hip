@@ -2688,6 +2688,33 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
         }
         const uint32_t cw = (blockIdx.x * blockDim.x + (tid & ~63u)) >> 3;
         GlobAcc<R> A{w.b, w.sp.local, w.perm};
+        const uint32_t total = pf[8];
+        if (total <= 64u) {
+            // one draw per lane: set up from memory and run the trials in place
+            if (lane < total) {
+                const uint32_t t = lane;
+                uint32_t k = 0u, slot_at = 0u, gg = 0u;
+                int ct = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (t >= pf[q] && t < pf[q + 1]) { k = t - pf[q]; slot_at = sbg[q] + k; ct = cg[q]; gg = (uint32_t)q; }
+                const uint32_t cc = cw + gg;
+                const CaseDev& cs = w.cases[ct];
+                const uint32_t i = w.sel_slot[slot_at];
+                DrawLane<R> D;
+                if (Macro && cs.macro) {
+                    BgkP Q = w.bgkp[cc];
+                    macro_params<R>(w, cs, cc, i, Q);
+                    draw_setup<R>(w, cs, cc, i, k, Q.model, [&](int f) { return bgkp_field(Q, f); }, A, D);
+                } else {
+                    const BgkP& P = w.bgkp[cc];
+                    draw_setup<R>(w, cs, cc, i, k, P.model, [&](int f) { return bgkp_field(P, f); }, A, D);
+                }
+                typename DrawLane<R>::T x0, x1, x2;
+                while (!draw_trial<R>(D, &w.raise[cc], x0, x1, x2)) {}
+                A.store(i, D.U0 + D.u0 * (double)x0, D.U1 + D.u0 * (double)x1, D.U2 + D.u0 * (double)x2);
+            }
+        } else {
         // the eight cells' draw parameters and a window of the wave's
         // selections are staged in LDS, so that a draw's set-up makes no
         // memory round trip
@@ -2719,7 +2746,6 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
         // once, so the USP rejection trials of a wave do not wait for its
         // slowest lane; each draw keeps its own stream, whichever lane runs it
         const unsigned long long below = (1ull << lane) - 1ull;
-        const uint32_t total = pf[8];
         uint32_t tnext = 0u;
         bool has = false;
         DrawLane<R> D;
@@ -2764,6 +2790,7 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
                     has = false;
                 }
             }
+        }
         }
     }
     __threadfence_block();
