@@ -34,6 +34,20 @@ COLLISION_DESC = {"hybrid": "hybrid USP/DSMC (NTC+VHS)", "dsmc": "DSMC (NTC+VHS)
                   "none": "collisionless"}
 
 
+# HBM bytes per launch of each phase's kernels at scale 1, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs of this bench line (FETCH_SIZE scaled by
+# 2 and WRITE_SIZE by 1, the factors a 2 GiB copy kernel reads; averages of the
+# last 20 launches; the profile summary named beside each holds the tables).
+PMC_TRAFFIC = {
+    "expansion": {"profile": "profiles/nozzle_expansion_1e8_fp32.md",
+                  "collide": (0.71 + 0.95 + 2.38 + 2.38) * 1e9,      # k_collide + k_bgk_conserve
+                  "cells": (9.59 + 4.59) * 1e9, "move": (4.24 + 2.07) * 1e9},
+    "nozzle": {"profile": "profiles/nozzle_1e8_fp32.md",
+               "collide": (0.78 + 0.63 + 1.94 + 1.44) * 1e9,
+               "cells": (11.20 + 4.60) * 1e9, "move": (4.31 + 1.61) * 1e9},
+}
+
+
 def build_workload(name, scale, collision="hybrid"):
     """(spec, init, desc): init(case) gives a new case its starting state."""
     from unigasfoam_amd import workloads
@@ -321,6 +335,9 @@ def main():
     dom_s = per_phase[dom]["ms"] * 1e-3
     achieved = dom_bytes / dom_s / 1e9
     step_b = roofline.step_bytes(R, n_avg, nc, cand, coll, n_bgk, **kw)
+    pm = PMC_TRAFFIC.get(args.workload, {}) if (args.scale == 1.0 and args.precision == "fp32") else {}
+    traffic = int(pm[dom]) if dom in pm else None
+    traffic_src = (pm["profile"] + " (rocprofv3 --pmc FETCH_SIZE x2 + WRITE_SIZE)") if traffic else None
     step_gbs = step_b / (elapsed / args.steps) / 1e9
     out = {
         "metric": METRIC,
@@ -341,8 +358,8 @@ def main():
                              + ("isentropic start" if args.workload == "expansion" else "fill")),
                    "decomposition_interval": spec.decomposition_interval},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(dom_bytes)},
+                     "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": int(dom_bytes)},
         "step_bandwidth": {"algorithmic_bytes_per_step": int(step_b), "GBs": round(step_gbs, 1),
                            "frac": round(step_gbs / roofline.HBM_PEAK_GBS, 4)},
         "phases": per_phase,
