@@ -2638,7 +2638,7 @@ constexpr uint32_t kDrawWindow = 512;
 // canonical sums, and the USP maxProb update; eight lanes per cell.
 constexpr int kConsReg = 8;   // simulators per lane kept in registers between the sums and the rescale
 template <class R, bool Macro>
-__global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
+__device__ __attribute__((always_inline)) inline void bgk_conserve_body(const WorldView<R>& w)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, g = lane >> 3, j = lane & 7u;
@@ -2871,6 +2871,20 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
         const double raise = bits_to_double(w.raise[c]);
         w.uspmax[c] = (raise > 0.0) ? raise : w.bgkp[c].m0 * 0.999;
     }
+}
+
+template <class R, bool Macro>
+__global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
+{
+    bgk_conserve_body<R, Macro>(w);
+}
+
+// The FP32-state instance without macroInterpolation at six waves per SIMD
+// (its conservation registers spill there): faster where the USP draws
+// dominate, slower where the conservation does; the autotuner picks.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) k_bgk_conserve6(WorldView<float> w)
+{
+    bgk_conserve_body<float, false>(w);
 }
 
 // --------------------------------------------------- hybrid decomposition
@@ -3585,8 +3599,14 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
             const unsigned gcons = (unsigned)(((uint64_t)w.nc * 8u + kBlock - 1) / kBlock);
             if (w.has_macro)
                 hipLaunchKernelGGL((k_bgk_conserve<R, true>), dim3(gcons), dim3(kBlock), 0, s, w);
-            else
-                hipLaunchKernelGGL((k_bgk_conserve<R, false>), dim3(gcons), dim3(kBlock), 0, s, w);
+            else {
+                bool six = false;
+                if constexpr (sizeof(R) == sizeof(float)) {
+                    six = w.cons_variant == 1;
+                    if (six) hipLaunchKernelGGL(k_bgk_conserve6, dim3(gcons), dim3(kBlock), 0, s, w);
+                }
+                if (!six) hipLaunchKernelGGL((k_bgk_conserve<R, false>), dim3(gcons), dim3(kBlock), 0, s, w);
+            }
         }
     }
     if (w.has_hybrid) {

@@ -108,6 +108,7 @@ struct WorldView {
     int32_t has_collide;      // some case collides (DSMC or BGK family)
     int32_t has_hybrid;       // some case switches models per cell
     int32_t tile_variant;     // cell-tile configuration (0..kTileVariants-1)
+    int32_t cons_variant;     // BGK conservation kernel: 0 four waves per SIMD, 1 six (FP32 state)
     uint32_t move_grid;       // blocks of the move: as many as are resident at once
     int32_t kn_due;           // a hybrid case decomposes at this step
     int32_t kn_passes;        // largest smoothing pass count of the due cases
@@ -251,6 +252,7 @@ constexpr int kNMom = UGX_S_SPECIES_MCC;
 constexpr uint32_t kCellW = 8;
 constexpr uint32_t kMinTileN = 128;
 constexpr int kTileVariants = 3;
+constexpr int kConsVariants = 2;
 constexpr uint32_t kLdsSpecies = 64;   // species tables up to this size are staged in LDS per tile
 inline uint64_t max_tiles(uint64_t cap, uint64_t nc) { return (cap + kCellW * nc) / kMinTileN + 2; }
 

@@ -525,6 +525,7 @@ WorldView<R> World::view() const
     w.sp = Species{d_mass, d_diam, d_rotdof, d_omega, d_muref, d_prterm, d_local};
     w.has_hybrid = has_hybrid;
     w.tile_variant = tile_variant;
+    w.cons_variant = cons_variant;
     w.move_grid = move_grid;
     w.kn_due = 0;
     w.kn_passes = 0;
@@ -1067,8 +1068,15 @@ int World::step(int n)
             }
         // the tuner times each cell-tile configuration on one of the first
         // kTileVariants steps, with or without step timing
-        const bool tuning = autotune && tune_step < kTileVariants;
-        int variant = tile_variant;
+        // and, as the flow develops, it times the conservation kernel's two
+        // occupancies on the collide phase of the last kConsVariants steps of
+        // every kConsRetune (not on decomposition steps)
+        const bool tile_tuning = autotune && tune_step < kTileVariants;
+        const uint64_t phase = steps_done % kConsRetune;
+        const bool cons_tuning = autotune && !tile_tuning && !kn_due && precision == UGX_FP32_STATE && has_bgk
+                                 && !has_macro && phase >= kConsRetune - kConsVariants;
+        const bool tuning = tile_tuning || cons_tuning;
+        int variant = tile_variant, cvariant = cons_variant;
         if (tuning) {
             if (!marks) {
                 if (tune_ev.empty()) {
@@ -1077,7 +1085,8 @@ int World::step(int n)
                 }
                 marks = tune_ev.data();
             }
-            variant = tune_step;
+            if (tile_tuning) variant = tune_step;
+            if (cons_tuning) cvariant = (int)(phase - (kConsRetune - kConsVariants));
         }
         const bool graph = use_graphs && !marks && !kn_due;
         if (graph && !gexec[cur]) {
@@ -1097,26 +1106,41 @@ int World::step(int n)
             w.kn_due = kn_due;
             w.kn_passes = kn_passes;
             w.tile_variant = variant;
+            w.cons_variant = cvariant;
             launch_step<float>(w, stream, marks);
         } else {
             WorldView<double> w = view<double>();
             w.kn_due = kn_due;
             w.kn_passes = kn_passes;
             w.tile_variant = variant;
+            w.cons_variant = cvariant;
             launch_step<double>(w, stream, marks);
         }
         if (tuning) {
             HIP_TRY(hipEventSynchronize(marks[kPhases]));
             float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, marks[6], marks[7]));
-            tune_ms[variant] += ms;
-            if (++tune_step == kTileVariants) {
-                int best = 0;
-                for (int v = 1; v < kTileVariants; ++v) if (tune_ms[v] < tune_ms[best]) best = v;
-                tile_variant = best;
-                drop_graphs();
+            if (tile_tuning) {
+                HIP_TRY(hipEventElapsedTime(&ms, marks[6], marks[7]));
+                tune_ms[variant] += ms;
+                if (++tune_step == kTileVariants) {
+                    int best = 0;
+                    for (int v = 1; v < kTileVariants; ++v) if (tune_ms[v] < tune_ms[best]) best = v;
+                    tile_variant = best;
+                    drop_graphs();
+                }
+            } else {
+                HIP_TRY(hipEventElapsedTime(&ms, marks[7], marks[8]));
+                cons_ms[cvariant] = ms;
+                if (cvariant == kConsVariants - 1) {
+                    const int best = (cons_ms[1] < cons_ms[0]) ? 1 : 0;
+                    if (best != cons_variant) {
+                        cons_variant = best;
+                        drop_graphs();
+                    }
+                }
             }
         }
+        steps_done += 1;
         cur = 1 - cur;
         for (size_t ci = 0; ci < cases.size(); ++ci) {
             CaseDev& c = cases[ci];

@@ -227,6 +227,12 @@ struct World {
     uint32_t move_grid = 0;       // resident blocks of the move kernel
     int tune_step = 0;
     double tune_ms[kTileVariants] = {};
+    // FP32 worlds with BGK-family cells and no macroInterpolation: the
+    // conservation kernel's occupancy, re-timed every kConsRetune steps
+    static constexpr uint64_t kConsRetune = 256;
+    int cons_variant = 0;
+    double cons_ms[kConsVariants] = {};
+    uint64_t steps_done = 0;
     std::vector<hipEvent_t> tune_ev;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     void drop_graphs();
