@@ -2636,8 +2636,8 @@ constexpr uint32_t kDrawWindow = 512;
 
 // conserveMomentumAndEnergy (stochasticParticleBGK.C:731-780) with the
 // canonical sums, and the USP maxProb update; eight lanes per cell.
-constexpr int kConsReg = 8;   // simulators per lane kept in registers between the sums and the rescale
-template <class R, bool Macro>
+constexpr int kConsRegs = 8;  // simulators per lane kept in registers between the sums and the rescale
+template <class R, bool Macro, int kConsReg>
 __device__ __attribute__((always_inline)) inline void bgk_conserve_body(const WorldView<R>& w)
 {
     const uint32_t tid = threadIdx.x;
@@ -2876,15 +2876,16 @@ __device__ __attribute__((always_inline)) inline void bgk_conserve_body(const Wo
 template <class R, bool Macro>
 __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
 {
-    bgk_conserve_body<R, Macro>(w);
+    bgk_conserve_body<R, Macro, kConsRegs>(w);
 }
 
-// The FP32-state instance without macroInterpolation at six waves per SIMD
-// (its conservation registers spill there): faster where the USP draws
-// dominate, slower where the conservation does; the autotuner picks.
+// The FP32-state instance without macroInterpolation at six waves per SIMD,
+// keeping four simulators per lane in registers (the same sums in the same
+// order): faster where the USP draws dominate, slower where the
+// conservation does; the autotuner picks.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) k_bgk_conserve6(WorldView<float> w)
 {
-    bgk_conserve_body<float, false>(w);
+    bgk_conserve_body<float, false, 4>(w);
 }
 
 // --------------------------------------------------- hybrid decomposition
