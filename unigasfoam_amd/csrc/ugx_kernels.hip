@@ -174,9 +174,9 @@ __global__ void k_begin_step(WorldView<R> w)
 // One block: insert counts per face-species (uniGasGeneralBoundary.C:602-609)
 // and their exclusive scan.
 template <class R>
-__global__ void k_inflow_count(WorldView<R> w)
+__global__ void __launch_bounds__(1024) k_inflow_count(WorldView<R> w)
 {
-    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t sh[16];
     const int tid = threadIdx.x;
     uint32_t carry = 0;
     for (int base = 0; base < w.n_inflow; base += 1024) {
@@ -197,16 +197,24 @@ __global__ void k_inflow_count(WorldView<R> w)
                 cnt = (uint32_t)c;
             }
         }
-        sh[tid] = cnt;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint32_t v = (tid >= o) ? sh[tid - o] : 0u;
-            __syncthreads();
-            sh[tid] += v;
-            __syncthreads();
+        // block scan: shuffles within each wave, then the 16 wave totals
+        uint32_t inc = cnt;
+        const int lane = tid & 63;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
         }
-        if (k < w.n_inflow) w.ins_off[k] = carry + sh[tid] - cnt;
-        uint32_t tot = sh[1023];
+        if (lane == 63) sh[tid >> 6] = inc;
+        __syncthreads();
+        uint32_t below = 0u, tot = 0u;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t v = sh[q];
+            if (q < (tid >> 6)) below += v;
+            tot += v;
+        }
+        if (k < w.n_inflow) w.ins_off[k] = carry + below + inc - cnt;
         __syncthreads();
         carry += tot;
     }
