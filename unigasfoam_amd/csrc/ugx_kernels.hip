@@ -2548,15 +2548,22 @@ __device__ __attribute__((always_inline)) inline double bgkp_field(const BgkP& P
     }
 }
 
+// pre, when not null, holds the cell's u0 and the USP draw's a, bq and
+// inv_m0 as computed below once per cell (single-species cells).
 template <class R, class Get>
 __device__ __attribute__((always_inline)) inline void draw_setup(const WorldView<R>& w, const CaseDev& cs, uint32_t c,
                                                                  uint32_t i, uint32_t k, int model, Get get,
-                                                                 const GlobAcc<R>& A, DrawLane<R>& D)
+                                                                 const GlobAcc<R>& A, DrawLane<R>& D,
+                                                                 const double* pre = nullptr)
 {
     using T = typename DrawLane<R>::T;
     rng_init(D.rv, cs.seed_lo, cs.seed_hi, kRngBgkV, k, c - (uint32_t)cs.cell_begin, cs.step);
-    const int ls = (cs.ns == 1) ? 0 : A.local_species(i);
-    D.u0 = get(3) / sqrt(w.sp.mass[cs.sp_base + ls]);
+    if (pre) {
+        D.u0 = pre[0];
+    } else {
+        const int ls = (cs.ns == 1) ? 0 : A.local_species(i);
+        D.u0 = get(3) / sqrt(w.sp.mass[cs.sp_base + ls]);
+    }
     D.model = model;
     D.i = i;
     D.cc = c;
@@ -2564,9 +2571,15 @@ __device__ __attribute__((always_inline)) inline void draw_setup(const WorldView
     D.U0 = get(0); D.U1 = get(1); D.U2 = get(2);
     if (model == UGX_BGK_USP) {
         D.m0 = (T)get(7);
-        D.a = (T)(get(5) / (get(4) * D.u0));
-        D.bq = (T)(get(6) / get(4));
-        D.inv_m0 = (T)1 / D.m0;
+        if (pre) {
+            D.a = (T)pre[1];
+            D.bq = (T)pre[2];
+            D.inv_m0 = (T)pre[3];
+        } else {
+            D.a = (T)(get(5) / (get(4) * D.u0));
+            D.bq = (T)(get(6) / get(4));
+            D.inv_m0 = (T)1 / D.m0;
+        }
         D.q0 = (T)get(9); D.q1 = (T)get(10); D.q2 = (T)get(11);
     }
     if (model == UGX_BGK_USP || model == UGX_BGK_ESBGK) {
@@ -2726,13 +2739,32 @@ __device__ __attribute__((always_inline)) inline void bgk_conserve_body(const Wo
         // the eight cells' draw parameters and a window of the wave's
         // selections are staged in LDS, so that a draw's set-up makes no
         // memory round trip
-        __shared__ double s_par[kBlock / 8][kDrawPars + 1];
+        // (and, for single-species cells, the draw constants that depend
+        // only on the cell: entries kDrawPars + 1.. as draw_setup's pre,
+        // entry kDrawPars + 1 negative when the draws compute them)
+        __shared__ double s_par[kBlock / 8][kDrawPars + 5];
         __shared__ uint32_t s_sel[kBlock / 64][kDrawWindow];
         const uint32_t wv = tid >> 6, cl0 = wv * 8u;
         if (c < (uint32_t)w.nc && nrel > 0u) {
             const double* src = reinterpret_cast<const double*>(&w.bgkp[c]);
             for (uint32_t f = j; f < (uint32_t)kDrawPars; f += 8u) s_par[tid >> 3][f] = src[f];
-            if (j == 0u) s_par[tid >> 3][kDrawPars] = (double)w.bgkp[c].model;
+            if (j == 0u) {
+                using T = typename DrawLane<R>::T;
+                const BgkP& P = w.bgkp[c];
+                const CaseDev& cs = w.cases[ccase];
+                double* pre = &s_par[tid >> 3][kDrawPars + 1];
+                s_par[tid >> 3][kDrawPars] = (double)P.model;
+                pre[0] = -1.0;
+                if (cs.ns == 1 && !(Macro && cs.macro)) {
+                    const double u0 = P.s2kT / sqrt(w.sp.mass[cs.sp_base]);
+                    pre[0] = u0;
+                    if (P.model == UGX_BGK_USP) {
+                        pre[1] = (double)(T)(P.cQ / (P.p * u0));
+                        pre[2] = (double)(T)(P.cS / P.p);
+                        pre[3] = (double)((T)1 / (T)P.m0);
+                    }
+                }
+            }
         }
         uint32_t w0 = 0u;
         auto stage_sel = [&](uint32_t from) {
@@ -2784,7 +2816,8 @@ __device__ __attribute__((always_inline)) inline void bgk_conserve_body(const Wo
                         macro_params<R>(w, cs, cc, i, Q);
                         draw_setup<R>(w, cs, cc, i, k, Q.model, [&](int f) { return bgkp_field(Q, f); }, A, D);
                     } else {
-                        draw_setup<R>(w, cs, cc, i, k, model, [&](int f) { return sp[f]; }, A, D);
+                        const double* pre = (sp[kDrawPars + 1] >= 0.0) ? &sp[kDrawPars + 1] : nullptr;
+                        draw_setup<R>(w, cs, cc, i, k, model, [&](int f) { return sp[f]; }, A, D, pre);
                     }
                     has = true;
                 }
