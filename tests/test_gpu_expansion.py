@@ -49,3 +49,28 @@ def test_expansion_fp32_keeps_the_load(require_gpu):
     assert 0.7 * n0 < n1 < 1.4 * n0, (n0, n1)
     assert np.isfinite(g.particles()["velocity"]).all()
     g.close()
+
+
+def test_conservation_occupancy_variants_agree(require_gpu):
+    """The autotuner runs the FP32 conservation kernel's second occupancy on
+    step 255 (and keeps whichever is faster after it); a run continued from
+    a checkpoint at step 250 retunes from scratch and keeps the first one
+    through step 260, so both instances must give the same simulators."""
+    spec, st = small_expansion(per_cell=30)
+    a = Case(spec, precision="fp32", capacity=1 << 18)
+    workloads.apply_state(a, st)
+    a.step(250)
+    a.sync()
+    ck = a.checkpoint()
+    a.step(10)
+    a.sync()
+    b = Case.restart(spec, ck, capacity=1 << 18)
+    b.step(10)
+    b.sync()
+    pa, pb = a.particles(), b.particles()
+    assert pa["cell"].size == pb["cell"].size
+    for k in ("cell", "local", "velocity"):
+        assert np.array_equal(pa[k], pb[k]), k
+    assert a.counters()["bgk_collisions"] > 0
+    a.close()
+    b.close()

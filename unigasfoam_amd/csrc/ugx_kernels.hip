@@ -2920,11 +2920,11 @@ __global__ void __launch_bounds__(kBlock) k_bgk_conserve(WorldView<R> w)
     bgk_conserve_body<R, Macro, kConsRegs>(w);
 }
 
-// The FP32-state instance without macroInterpolation at six waves per SIMD,
+// The FP32-state instance without macroInterpolation at five waves per SIMD,
 // keeping four simulators per lane in registers (the same sums in the same
 // order): faster where the USP draws dominate, slower where the
 // conservation does; the autotuner picks.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) k_bgk_conserve6(WorldView<float> w)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_bgk_conserve5(WorldView<float> w)
 {
     bgk_conserve_body<float, false, 4>(w);
 }
@@ -3645,7 +3645,7 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
                 bool six = false;
                 if constexpr (sizeof(R) == sizeof(float)) {
                     six = w.cons_variant == 1;
-                    if (six) hipLaunchKernelGGL(k_bgk_conserve6, dim3(gcons), dim3(kBlock), 0, s, w);
+                    if (six) hipLaunchKernelGGL(k_bgk_conserve5, dim3(gcons), dim3(kBlock), 0, s, w);
                 }
                 if (!six) hipLaunchKernelGGL((k_bgk_conserve<R, false>), dim3(gcons), dim3(kBlock), 0, s, w);
             }

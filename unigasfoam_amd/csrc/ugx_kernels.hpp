@@ -108,7 +108,7 @@ struct WorldView {
     int32_t has_collide;      // some case collides (DSMC or BGK family)
     int32_t has_hybrid;       // some case switches models per cell
     int32_t tile_variant;     // cell-tile configuration (0..kTileVariants-1)
-    int32_t cons_variant;     // BGK conservation kernel: 0 four waves per SIMD, 1 six (FP32 state)
+    int32_t cons_variant;     // BGK conservation kernel: 0 four waves per SIMD, 1 five (FP32 state)
     uint32_t move_grid;       // blocks of the move: as many as are resident at once
     int32_t kn_due;           // a hybrid case decomposes at this step
     int32_t kn_passes;        // largest smoothing pass count of the due cases
