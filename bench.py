@@ -40,7 +40,7 @@ COLLISION_DESC = {"hybrid": "hybrid USP/DSMC (NTC+VHS)", "dsmc": "DSMC (NTC+VHS)
 # last 20 launches; the profile summary named beside each holds the tables).
 PMC_TRAFFIC = {
     "expansion": {"profile": "profiles/nozzle_expansion_1e8_fp32.md",
-                  "collide": (0.71 + 0.95 + 3.31 + 4.91) * 1e9,      # k_collide + k_bgk_conserve6
+                  "collide": (0.71 + 0.95 + 2.41 + 2.69) * 1e9,      # k_collide + k_bgk_conserve5
                   "cells": (9.59 + 4.56) * 1e9, "move": (4.24 + 2.07) * 1e9},
     "nozzle": {"profile": "profiles/nozzle_1e8_fp32.md",
                "collide": (0.78 + 0.63 + 1.94 + 1.44) * 1e9,
