@@ -43,8 +43,8 @@ PMC_TRAFFIC = {
                   "collide": (0.71 + 0.95 + 2.41 + 2.69) * 1e9,      # k_collide + k_bgk_conserve5
                   "cells": (9.59 + 4.56) * 1e9, "move": (4.24 + 2.07) * 1e9},
     "nozzle": {"profile": "profiles/nozzle_1e8_fp32.md",
-               "collide": (0.78 + 0.63 + 1.94 + 1.44) * 1e9,
-               "cells": (11.20 + 4.60) * 1e9, "move": (4.31 + 1.61) * 1e9},
+               "collide": (0.78 + 0.63 + 2.00 + 1.56) * 1e9,      # k_collide + k_bgk_conserve5
+               "cells": (12.01 + 4.71) * 1e9, "move": (4.31 + 1.61) * 1e9},
 }
 
 
