@@ -74,3 +74,21 @@ def test_conservation_occupancy_variants_agree(require_gpu):
     assert a.counters()["bgk_collisions"] > 0
     a.close()
     b.close()
+
+
+def test_expansion_many_inflow_faces_identical(require_gpu):
+    """More than 4096 inflow face-species: the counts are drawn per face in
+    parallel and scanned in place, the same as the oracle's."""
+    spec, st = workloads.nozzle_expansion(nx=8, ny=48, nz=48, per_cell=2)
+    spec.decomposition_interval = 4
+    g = Case(spec, precision="fp64", capacity=1 << 19)
+    o = OracleCase(spec, capacity=1 << 19)
+    for c in (g, o):
+        workloads.apply_state(c, st)
+    for k in range(3):
+        g.step(2)
+        o.step(2)
+        assert_identical(g, o, f"many inflow faces, after {2 * (k + 1)} steps")
+    assert g.counters()["inserted"] > 0
+    g.close()
+    o.close()

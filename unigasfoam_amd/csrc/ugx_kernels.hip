@@ -228,6 +228,44 @@ __global__ void __launch_bounds__(1024) k_inflow_count(WorldView<R> w)
     }
 }
 
+// The same counts with a thread per face-species, written into ins_off for
+// an exclusive scan in place (ins_off[n_inflow] = 0 ends it).
+template <class R>
+__global__ void k_inflow_face_count(WorldView<R> w)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= w.n_inflow; k += gridDim.x * blockDim.x) {
+    uint32_t cnt = 0;
+    if (k < w.n_inflow) {
+        const InflowFS& e = w.inflow[k];
+        const CaseDev& cs = w.cases[e.ccase];
+        if (cs.active) {
+            Rng r;
+            rng_init(r, cs.seed_lo, cs.seed_hi, kRngInflowN, 0u, (uint32_t)e.local_id, cs.step);
+            const double nb = e.nbar_num / (e.nbar_den * w.cw[e.cell]);
+            const double fl = floor(nb);
+            long long c = (long long)fl;
+            if (c < 0) c = 0;
+            if (uniform(r) < nb - fl) c += 1;
+            cnt = (uint32_t)c;
+        }
+    }
+    w.ins_off[k] = cnt;
+    }
+}
+
+// After the scan: the step's inflow total, or none when it would exceed the
+// capacity (the overflow is reported).
+template <class R>
+__global__ void k_inflow_finish(WorldView<R> w)
+{
+    uint32_t carry = w.ins_off[w.n_inflow];
+    if ((unsigned long long)w.st->n + carry > (unsigned long long)w.cap) {
+        w.st->overflow = 1u;
+        carry = 0u;
+    }
+    w.st->n_ins = carry;
+}
+
 // Bird eq. 12.5 insertion + first move over (1 - R) deltaT.
 template <class R>
 __device__ inline int make_inflow(const WorldView<R>& w, const InflowFS& e, const CaseDev& cs, Rng& r,
@@ -3594,7 +3632,14 @@ void launch_step(const WorldView<R>& w0, hipStream_t s, hipEvent_t* marks)
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncp)), dim3(kBlock), 0, s, w.ocnt, ncp);
     mark(1);
     if (w.n_inflow > 0) {
-        hipLaunchKernelGGL(k_inflow_count<R>, dim3(1), dim3(1024), 0, s, w);
+        if (w.n_inflow > 4096) {
+            // many inflow faces: counted in parallel and scanned in place
+            hipLaunchKernelGGL(k_inflow_face_count<R>, dim3(grid_for((uint64_t)w.n_inflow + 1)), dim3(kBlock), 0, s, w);
+            scan_u32(w.ins_off, (uint32_t)w.n_inflow + 1u, w.ins_off, w.blocksum, nullptr, s);
+            hipLaunchKernelGGL(k_inflow_finish<R>, dim3(1), dim3(1), 0, s, w);
+        } else {
+            hipLaunchKernelGGL(k_inflow_count<R>, dim3(1), dim3(1024), 0, s, w);
+        }
         hipLaunchKernelGGL(k_inflow_gen<R>, dim3(grid_for(w.max_ins)), dim3(kBlock), 0, s, w);
     }
     mark(2);

@@ -931,7 +931,8 @@ int World::build(std::vector<CaseSpec>& specs, int prec, int device, uint64_t ca
         std::vector<double> zw((size_t)nwf * kNS * kWRows, 0.0);
         if ((rc = up(d_wsamp, zw))) return rc;
     }
-    size_t nbs = scan_bsum_words(std::max<size_t>((size_t)nc * kNS + 2, (size_t)nwf + 1));
+    size_t nbs = scan_bsum_words(std::max<size_t>(std::max<size_t>((size_t)nc * kNS + 2, (size_t)nwf + 1),
+                                                  (size_t)n_inflow + 1));
     if ((rc = dalloc(d_blocksum, nbs))) return rc;
     std::vector<unsigned long long> zcnt(specs.size() * kCntNum, 0ull);
     if ((rc = up(d_counters, zcnt))) return rc;
